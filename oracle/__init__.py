@@ -1,0 +1,286 @@
+"""numpy front end for the CPU oracle and for the reference build.
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py`` as the checker; the product
+(``spyker_amd``) never imports it.
+
+``Oracle`` wraps ``liboracle.so`` (the C restatement, spyker_oracle.c).
+``Reference`` wraps ``_ref/libspyker_ref.so`` (the reference's own CPU
+backend compiled from its sources, ref_harness.cpp) when that was built.
+Both expose the same methods with the same arguments, so a test can run
+one input through each and compare.
+"""
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+I64, F32, F64, I32 = C.c_int64, C.c_float, C.c_double, C.c_int32
+P = C.c_void_p
+
+WINNER = np.dtype([("c", "<i8"), ("t", "<i8"), ("z", "<i8"), ("y", "<i8"), ("x", "<i8")])
+CONFIG = np.dtype(
+    [("positive", "<f8"), ("negative", "<f8"), ("stabilize", "<i4"), ("pad", "<i4"), ("lower", "<f8"),
+     ("upper", "<f8")],
+    align=True)
+
+
+def build():
+    """Compile liboracle.so (and the reference library where its sources exist)."""
+    subprocess.check_call(["make", "-s", "-C", HERE, "-j8"])
+
+
+def _ptr(a):
+    return a.ctypes.data_as(P)
+
+
+def configs_array(configs):
+    arr = np.zeros(len(configs), dtype=CONFIG)
+    for i, c in enumerate(configs):
+        pos, neg = c[0], c[1]
+        stab = c[2] if len(c) > 2 else True
+        lo = c[3] if len(c) > 3 else 0.0
+        hi = c[4] if len(c) > 4 else 1.0
+        arr[i] = (pos, neg, int(bool(stab)), 0, lo, hi)
+    return arr
+
+
+class _Lib:
+    prefix = None
+    path = None
+
+    def __init__(self):
+        if not os.path.exists(self.path):
+            raise FileNotFoundError(self.path + " is not built (run make -C oracle)")
+        self.lib = C.CDLL(self.path)
+
+    def _f(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = ""
+            if hasattr(self.lib, "ref_last_error"):
+                self.lib.ref_last_error.restype = C.c_char_p
+                msg = self.lib.ref_last_error().decode()
+            raise RuntimeError("%s failed: %s" % (what, msg))
+
+    # rank-order code ------------------------------------------------------
+    def rank_code(self, x, time, sort=True):
+        """x: [N, L] float32 -> [N, time, L] uint8 (cumulative spikes)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        n, l = x.shape
+        out = np.zeros((n, time, l), np.uint8)
+        if self.prefix == "ref_":
+            self._check(self._f("rank_code_f32")(_ptr(x), I64(n), I64(l), I64(time), _ptr(out), I32(int(sort))),
+                        "rank_code")
+        elif sort:
+            self._f("rank_code_sort_f32")(_ptr(x), I64(n), I64(l), I64(time), _ptr(out), I32(os.cpu_count() or 1))
+        else:
+            self._f("rank_code_latency_f32")(_ptr(x), I64(n), I64(l), I64(time), _ptr(out))
+        return out
+
+    def threshold(self, x, th, value=0.0):
+        x = np.array(x, dtype=np.float32, copy=True)
+        rc = self._f("threshold_f32")(_ptr(x), I64(x.size), F32(th), F32(value))
+        if self.prefix == "ref_":
+            self._check(rc, "threshold")
+        return x
+
+    def fire(self, x, th=0.0):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        out = np.zeros(x.shape, np.uint8)
+        rc = self._f("rank_fire_f32")(_ptr(x), _ptr(out), I64(x.size), F32(th))
+        if self.prefix == "ref_":
+            self._check(rc, "fire")
+        return out
+
+    def inhibit(self, x, th=0.0):
+        """x: [N, T, C, H, W] float32 -> inhibited copy."""
+        x = np.array(x, dtype=np.float32, copy=True)
+        n, t, c = x.shape[:3]
+        hw = int(np.prod(x.shape[3:]))
+        if self.prefix == "ref_":
+            self._check(self._f("rank_inhibit_f32")(_ptr(x), I64(n), I64(t), I64(c), I64(hw), F32(th)), "inhibit")
+        else:
+            self._f("rank_inhibit_f32")(_ptr(x), I64(n), I64(t), I64(c), I64(hw), F32(th), I32(os.cpu_count() or 1))
+        return x
+
+    def convwta(self, x, radius, count, th=0.0):
+        """x: [N, T, C, H, W] -> list (per sample) of (z, y, x) tuples."""
+        ry, rx = (radius, radius) if np.isscalar(radius) else radius
+        n, t, c, h, w = x.shape
+        win = np.zeros(n * count, WINNER)
+        cnt = np.zeros(n, np.int64)
+        if x.dtype == np.uint8:
+            x = np.ascontiguousarray(x)
+            rc = self._f("rank_convwta_u8")(_ptr(x), I64(n), I64(t), I64(c), I64(h), I64(w), I64(ry), I64(rx),
+                                             I64(count), _ptr(win), _ptr(cnt))
+        else:
+            x = np.ascontiguousarray(x, dtype=np.float32)
+            args = [_ptr(x), I64(n), I64(t), I64(c), I64(h), I64(w), I64(ry), I64(rx), I64(count), F32(th),
+                    _ptr(win), _ptr(cnt)]
+            if self.prefix == "orc_":
+                args.append(I32(os.cpu_count() or 1))
+            rc = self._f("rank_convwta_f32")(*args)
+        if self.prefix == "ref_":
+            self._check(rc, "convwta")
+        return win.reshape(n, count), cnt
+
+    def fcwta(self, x, radius, count, th=0.0):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        n, t, f = x.shape
+        win = np.zeros(n * count, WINNER)
+        cnt = np.zeros(n, np.int64)
+        rc = self._f("rank_fcwta_f32")(_ptr(x), I64(n), I64(t), I64(f), I64(radius), I64(count), F32(th),
+                                        _ptr(win), _ptr(cnt))
+        if self.prefix == "ref_":
+            self._check(rc, "fcwta")
+        return win.reshape(n, count), cnt
+
+    def pool(self, x, kernel, stride=None, pad=(0, 0, 0, 0)):
+        """x: [..., H, W] uint8 or float32; pad (top, left, bottom, right)."""
+        kh, kw = (kernel, kernel) if np.isscalar(kernel) else kernel
+        stride = kernel if stride is None else stride
+        sh, sw = (stride, stride) if np.isscalar(stride) else stride
+        pt, pl, pb, pr = pad
+        h, w = x.shape[-2:]
+        planes = int(np.prod(x.shape[:-2]))
+        oh, ow = (h + pt + pb - kh) // sh + 1, (w + pl + pr - kw) // sw + 1
+        kind = "u8" if x.dtype == np.uint8 else "f32"
+        x = np.ascontiguousarray(x, dtype=np.uint8 if kind == "u8" else np.float32)
+        out = np.zeros(x.shape[:-2] + (oh, ow), x.dtype)
+        rc = self._f("rank_pool_" + kind)(_ptr(x), I64(planes), I64(h), I64(w), _ptr(out), I64(kh), I64(kw),
+                                          I64(sh), I64(sw), I64(pt), I64(pl), I64(pb), I64(pr))
+        if self.prefix == "ref_":
+            self._check(rc, "pool")
+        return out
+
+    def convstdp(self, spk_in, kernel, spk_out, configs, winners, counts, stride=1, pad=(0, 0, 0, 0)):
+        """Returns the updated kernel (a copy)."""
+        spk_in = np.ascontiguousarray(spk_in, np.uint8)
+        spk_out = np.ascontiguousarray(spk_out, np.uint8)
+        k = np.array(kernel, dtype=np.float32, copy=True)
+        cfg = configs_array(configs)
+        win = np.ascontiguousarray(winners, WINNER)
+        cnt = np.ascontiguousarray(counts, np.int64)
+        n, t, cin, h, w = spk_in.shape
+        cout, _, kh, kw = k.shape
+        ho, wo = spk_out.shape[-2:]
+        sy, sx = (stride, stride) if np.isscalar(stride) else stride
+        rc = self._f("rank_convstdp_f32")(
+            _ptr(spk_in), I64(n), I64(t), I64(cin), I64(h), I64(w), _ptr(k), I64(cout), I64(kh), I64(kw),
+            _ptr(spk_out), I64(ho), I64(wo), _ptr(cfg), I64(len(cfg)), _ptr(win), _ptr(cnt), I64(win.shape[1]),
+            I64(sy), I64(sx), *[I64(p) for p in pad])
+        self._check(rc, "convstdp")
+        return k
+
+    def fcstdp(self, spk_in, kernel, spk_out, configs, winners, counts):
+        spk_in = np.ascontiguousarray(spk_in, np.uint8)
+        spk_out = np.ascontiguousarray(spk_out, np.uint8)
+        k = np.array(kernel, dtype=np.float32, copy=True)
+        cfg = configs_array(configs)
+        win = np.ascontiguousarray(winners, WINNER)
+        cnt = np.ascontiguousarray(counts, np.int64)
+        n, t, fin = spk_in.shape
+        fout = k.shape[0]
+        rc = self._f("rank_fcstdp_f32")(_ptr(spk_in), I64(n), I64(t), I64(fin), _ptr(k), I64(fout), _ptr(spk_out),
+                                        _ptr(cfg), I64(len(cfg)), _ptr(win), _ptr(cnt), I64(win.shape[1]))
+        self._check(rc, "fcstdp")
+        return k
+
+    def gaussian_kernel(self, k, std):
+        out = np.zeros((k, k), np.float64)
+        rc = self._f("gaussian_kernel")(_ptr(out), I64(k), F64(std))
+        if self.prefix == "ref_":
+            self._check(rc, "gaussian_kernel")
+        return out
+
+    def gabor_kernel(self, k, sigma, theta, gamma, lam, psi):
+        out = np.zeros((k, k), np.float64)
+        rc = self._f("gabor_kernel")(_ptr(out), I64(k), F64(sigma), F64(theta), F64(gamma), F64(lam), F64(psi))
+        if self.prefix == "ref_":
+            self._check(rc, "gabor_kernel")
+        return out
+
+
+class Oracle(_Lib):
+    prefix = "orc_"
+    path = os.path.join(HERE, "liboracle.so")
+
+    def __init__(self):
+        super().__init__()
+        self.lib.orc_stdp_update.restype = C.c_float
+
+    def stdp_update(self, w, potentiate, config):
+        cfg = configs_array([config])
+        return self.lib.orc_stdp_update(F32(w), I32(int(potentiate)), _ptr(cfg))
+
+    def conv(self, x, kernel, stride=1, pad=(0, 0, 0, 0), f64=False, threads=None):
+        """x: [N, Cin, H, W] uint8/float32; kernel [Cout, Cin, Kh, Kw] float32."""
+        is_u8 = x.dtype == np.uint8
+        x = np.ascontiguousarray(x, np.uint8 if is_u8 else np.float32)
+        kernel = np.ascontiguousarray(kernel, np.float32)
+        n, cin, h, w = x.shape
+        cout, _, kh, kw = kernel.shape
+        sy, sx = (stride, stride) if np.isscalar(stride) else stride
+        pt, pl, pb, pr = pad
+        ho, wo = (h + pt + pb - kh) // sy + 1, (w + pl + pr - kw) // sx + 1
+        out = np.zeros((n, cout, ho, wo), np.float64 if f64 else np.float32)
+        fn = self.lib.orc_conv_f64 if f64 else self.lib.orc_conv_f32
+        fn(_ptr(x), I32(int(is_u8)), I64(n), I64(cin), I64(h), I64(w), _ptr(kernel), I64(cout), I64(kh), I64(kw),
+           I64(sy), I64(sx), I64(pt), I64(pl), I64(pb), I64(pr), _ptr(out), I32(threads or os.cpu_count() or 1))
+        return out
+
+    def fc(self, x, kernel):
+        is_u8 = x.dtype == np.uint8
+        x = np.ascontiguousarray(x, np.uint8 if is_u8 else np.float32)
+        kernel = np.ascontiguousarray(kernel, np.float32)
+        fin = x.shape[-1]
+        rows = x.size // fin
+        fout = kernel.shape[0]
+        out = np.zeros(x.shape[:-1] + (fout,), np.float32)
+        self.lib.orc_fc_f32(_ptr(x), I32(int(is_u8)), I64(rows), I64(fin), _ptr(kernel), I64(fout), _ptr(out))
+        return out
+
+    def dog(self, x, kernel, pad):
+        """x: [N, C, H, W] float32, kernel [2, count, k, k] float32 -> [N, C*count, Ho, Wo]."""
+        x = np.ascontiguousarray(x, np.float32)
+        kernel = np.ascontiguousarray(kernel, np.float32)
+        n, c, h, w = x.shape
+        count, k = kernel.shape[1], kernel.shape[2]
+        pt, pl, pb, pr = pad
+        ho, wo = h + pt + pb - k + 1, w + pl + pr - k + 1
+        out = np.zeros((n, c * count, ho, wo), np.float32)
+        self.lib.orc_dog_f32(_ptr(x), I64(n * c), I64(h), I64(w), _ptr(kernel), I64(count), I64(k), I64(pt),
+                             I64(pl), I64(pb), I64(pr), _ptr(out), I32(os.cpu_count() or 1))
+        return out
+
+    def gabor(self, x, kernel, pad):
+        x = np.ascontiguousarray(x, np.float32)
+        kernel = np.ascontiguousarray(kernel, np.float32)
+        n, c, h, w = x.shape
+        count, k = kernel.shape[0], kernel.shape[1]
+        pt, pl, pb, pr = pad
+        ho, wo = h + pt + pb - k + 1, w + pl + pr - k + 1
+        out = np.zeros((n, c * count, ho, wo), np.float32)
+        self.lib.orc_gabor_f32(_ptr(x), I64(n * c), I64(h), I64(w), _ptr(kernel), I64(count), I64(k), I64(pt),
+                               I64(pl), I64(pb), I64(pr), _ptr(out), I32(os.cpu_count() or 1))
+        return out
+
+
+class Reference(_Lib):
+    prefix = "ref_"
+    path = os.path.join(HERE, "_ref", "libspyker_ref.so")
+
+    def normal_kernel(self, n, mean, std, seed):
+        out = np.zeros(n, np.float64)
+        self._check(self.lib.ref_normal_kernel(_ptr(out), I64(n), F64(mean), F64(std), I64(seed)), "normal_kernel")
+        return out
+
+
+def reference_available():
+    return os.path.exists(Reference.path)

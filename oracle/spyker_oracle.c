@@ -1,0 +1,595 @@
+/*
+ * CPU oracle for the Spyker dense STDP layer path (C11 + OpenMP).
+ *
+ * TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py as the checker.  The product never links it.
+ *
+ * Each function restates the reference CPU function cited beside it.  Where
+ * the reference is compiled with FMA contraction (-march=native on an FMA
+ * host, GCC's default -ffp-contract=fast), the contracted form is written
+ * out with an explicit fma() so the result does not depend on how this file
+ * is compiled.  The file is built with -ffp-contract=off for the same reason.
+ *
+ * Parity status of each function (what pins it) is listed in the header of
+ * tests/test_oracle.py.
+ */
+#include "spyker_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define EPS 1e-5f /* Epsilon, src/spyker/base.h:776 */
+
+static inline int64_t imax(int64_t a, int64_t b) { return a > b ? a : b; }
+static inline int64_t imin(int64_t a, int64_t b) { return a < b ? a : b; }
+
+/* float -> U16 as GCC on x86-64 emits it: truncate to a 32-bit integer, keep
+ * the low 16 bits (src/spyker/cpu/code.cpp:268,316 store F32 into U16). */
+static inline uint16_t f32_to_u16(float f) { return (uint16_t)(uint32_t)(int32_t)f; }
+
+/* ------------------------------------------------------------------------ */
+/* Rank-order coding                                                        */
+/* ------------------------------------------------------------------------ */
+
+typedef struct
+{
+    float v;
+    uint32_t i;
+} keyed;
+
+/* Order: value descending, index ascending.  The reference uses std::sort
+ * (unstable) with comp = input(i) > input(j) (src/spyker/cpu/code.cpp:314);
+ * ties therefore occupy one contiguous block of ranks whose internal order is
+ * unspecified there and index-ascending here. */
+static int keyed_cmp(const void *a, const void *b)
+{
+    const keyed *x = (const keyed *)a, *y = (const keyed *)b;
+    if (x->v > y->v) return -1;
+    if (x->v < y->v) return 1;
+    return (x->i > y->i) - (x->i < y->i);
+}
+
+/* src/spyker/cpu/code.cpp:301-317 (sorted) */
+void orc_rank_times_sort_f32(const float *in, int64_t len, int64_t time, uint16_t *times)
+{
+    keyed *k = (keyed *)malloc(sizeof(keyed) * (size_t)len);
+    for (int64_t i = 0; i < len; ++i) k[i].v = in[i], k[i].i = (uint32_t)i;
+    qsort(k, (size_t)len, sizeof(keyed), keyed_cmp);
+    /* lower_bound of the last (smallest) element under comp: the number of
+     * elements strictly greater than the minimum (code.cpp:302-307). */
+    float last = k[len - 1].v;
+    int64_t lo = 0, hi = len;
+    while (lo < hi)
+    {
+        int64_t mid = lo + (hi - lo) / 2;
+        if (k[mid].v > last)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    float scale = ((float)time + EPS) / (float)lo;
+    for (int64_t i = 0; i < len; ++i) times[k[i].i] = f32_to_u16((float)i * scale);
+    free(k);
+}
+
+/* src/spyker/cpu/code.cpp:271-282: spike[t] = (time <= t), cumulative. */
+static void times_to_spikes(const uint16_t *times, int64_t len, int64_t time, uint8_t *out)
+{
+    for (int64_t t = 0; t < time; ++t)
+    {
+        uint16_t max = (uint16_t)t;
+        uint8_t *od = out + t * len;
+        for (int64_t j = 0; j < len; ++j) od[j] = times[j] <= max;
+    }
+}
+
+/* src/spyker/cpu/code.cpp:319-335 */
+void orc_rank_code_sort_f32(const float *in, int64_t n, int64_t len, int64_t time, uint8_t *out, int32_t threads)
+{
+#pragma omp parallel for num_threads(threads > 0 ? threads : 1) schedule(dynamic)
+    for (int64_t i = 0; i < n; ++i)
+    {
+        uint16_t *times = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)len);
+        orc_rank_times_sort_f32(in + i * len, len, time, times);
+        times_to_spikes(times, len, time, out + i * time * len);
+        free(times);
+    }
+}
+
+/* src/spyker/cpu/code.cpp:258-269 (latency); the subtraction from `end` is
+ * contracted into one fused multiply-add in the reference build. */
+void orc_rank_code_latency_f32(const float *in, int64_t n, int64_t len, int64_t time, uint8_t *out)
+{
+    uint16_t *times = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)len);
+    for (int64_t s = 0; s < n; ++s)
+    {
+        const float *x = in + s * len;
+        float mn = x[0], mx = x[0];
+        for (int64_t i = 0; i < len; ++i)
+        {
+            mn = x[i] < mn ? x[i] : mn; /* std::min(a,b) = b < a ? b : a */
+            mx = mx < x[i] ? x[i] : mx; /* std::max(a,b) = a < b ? b : a */
+        }
+        float end = (float)time + EPS, scale = end / (mx - mn);
+        for (int64_t i = 0; i < len; ++i) times[i] = f32_to_u16(fmaf(-scale, x[i] - mn, end));
+        times_to_spikes(times, len, time, out + s * time * len);
+    }
+    free(times);
+}
+
+/* ------------------------------------------------------------------------ */
+/* threshold / fire                                                         */
+/* ------------------------------------------------------------------------ */
+
+/* src/spyker/cpu/fire.cpp:41-46 */
+void orc_threshold_f32(float *x, int64_t n, float threshold, float value)
+{
+    for (int64_t i = 0; i < n; ++i)
+        if (x[i] <= threshold) x[i] = value;
+}
+
+/* src/spyker/cpu/fire.cpp:67-71 */
+void orc_rank_fire_f32(const float *x, uint8_t *out, int64_t n, float threshold)
+{
+    for (int64_t i = 0; i < n; ++i) out[i] = x[i] > threshold;
+}
+
+/* ------------------------------------------------------------------------ */
+/* inhibit                                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* src/spyker/cpu/inhibit.cpp:41-121.  Per sample and pixel: m_t, ch_t = the
+ * first channel maximum at step t (strict >, starting from channel 0); cnt =
+ * #{t : m_t > threshold}; the kept channel is ch at step T - max(cnt, 1);
+ * every value is multiplied by (channel == kept). */
+void orc_rank_inhibit_f32(float *x, int64_t n, int64_t t, int64_t c, int64_t hw, float threshold, int32_t threads)
+{
+#pragma omp parallel for num_threads(threads > 0 ? threads : 1)
+    for (int64_t s = 0; s < n; ++s)
+    {
+        float *xs = x + s * t * c * hw;
+        uint16_t *ch = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)(t * hw));
+        uint16_t *sum = (uint16_t *)calloc((size_t)hw, sizeof(uint16_t));
+        for (int64_t i = 0; i < t; ++i)
+            for (int64_t p = 0; p < hw; ++p)
+            {
+                float m = xs[(i * c) * hw + p];
+                uint16_t a = 0;
+                for (int64_t j = 1; j < c; ++j)
+                {
+                    float v = xs[(i * c + j) * hw + p];
+                    if (v > m) m = v, a = (uint16_t)j;
+                }
+                ch[i * hw + p] = a;
+                sum[p] = (uint16_t)(sum[p] + (m > threshold));
+            }
+        for (int64_t p = 0; p < hw; ++p)
+        {
+            uint16_t cnt = sum[p] > 1 ? sum[p] : 1;
+            uint16_t keep = ch[(int64_t)(uint16_t)(t - cnt) * hw + p];
+            for (int64_t i = 0; i < t; ++i)
+                for (int64_t j = 0; j < c; ++j) xs[(i * c + j) * hw + p] *= (float)((uint16_t)j == keep);
+        }
+        free(ch);
+        free(sum);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* winner-take-all                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* src/spyker/cpu/wta.cpp:49-64: value of the earliest step above threshold. */
+static void wta_values_f32(const float *x, int64_t t, int64_t len, float threshold, float *temp)
+{
+    float mn = x[0], mx = x[0];
+    for (int64_t i = 0; i < t * len; ++i)
+    {
+        mn = x[i] < mn ? x[i] : mn;
+        mx = mx < x[i] ? x[i] : mx;
+    }
+    float diff = (mx - mn) * 2.0f;
+    for (int64_t j = 0; j < len; ++j) temp[j] = -FLT_MAX;
+    for (int64_t i = t - 1; i >= 0; --i)
+    {
+        const float *xd = x + i * len;
+        float level = (float)(t - i - 1);
+        for (int64_t j = 0; j < len; ++j)
+            if (xd[j] > threshold) temp[j] = level + (xd[j] - mn) / diff;
+    }
+}
+
+/* std::max_element: first maximum under operator<. */
+static int64_t first_max_f32(const float *v, int64_t len)
+{
+    int64_t best = 0;
+    for (int64_t i = 1; i < len; ++i)
+        if (v[best] < v[i]) best = i;
+    return best;
+}
+
+/* src/spyker/cpu/wta.cpp:128-150 */
+static int conv_select_f32(float *temp, int64_t c, int64_t h, int64_t w, int64_t ry, int64_t rx, orc_winner *win)
+{
+    int64_t len = h * w;
+    int64_t idx = first_max_f32(temp, c * len);
+    if (fabsf(temp[idx] - (-FLT_MAX)) < EPS) return 0;
+    win->c = 0, win->t = 0;
+    win->z = idx / len, win->y = (idx % len) / w, win->x = (idx % len) % w;
+    int64_t sj = imax(0, win->y - ry), ej = imin(h, win->y + ry + 1);
+    int64_t sk = imax(0, win->x - rx), ek = imin(w, win->x + rx + 1);
+    for (int64_t i = 0; i < c; ++i)
+        for (int64_t j = sj; j < ej; ++j)
+            for (int64_t k = sk; k < ek; ++k) temp[(i * h + j) * w + k] = -FLT_MAX;
+    for (int64_t j = 0; j < len; ++j) temp[win->z * len + j] = -FLT_MAX;
+    return 1;
+}
+
+/* src/spyker/cpu/wta.cpp:152-182 */
+void orc_rank_convwta_f32(const float *x, int64_t n, int64_t t, int64_t c, int64_t h, int64_t w, int64_t ry,
+                          int64_t rx, int64_t count, float threshold, orc_winner *winners, int64_t *counts,
+                          int32_t threads)
+{
+    int64_t len = c * h * w;
+#pragma omp parallel for num_threads(threads > 0 ? threads : 1)
+    for (int64_t s = 0; s < n; ++s)
+    {
+        float *temp = (float *)malloc(sizeof(float) * (size_t)len);
+        wta_values_f32(x + s * t * len, t, len, threshold, temp);
+        int64_t k = 0;
+        for (; k < count; ++k)
+            if (!conv_select_f32(temp, c, h, w, ry, rx, &winners[s * count + k])) break;
+        counts[s] = k;
+        free(temp);
+    }
+}
+
+/* Integer branch, src/spyker/cpu/wta.cpp:66-85 (threshold unused there). */
+void orc_rank_convwta_u8(const uint8_t *x, int64_t n, int64_t t, int64_t c, int64_t h, int64_t w, int64_t ry,
+                         int64_t rx, int64_t count, orc_winner *winners, int64_t *counts)
+{
+    int64_t len = c * h * w, hw = h * w;
+    uint16_t *temp = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)len);
+    for (int64_t s = 0; s < n; ++s)
+    {
+        const uint8_t *xs = x + s * t * len;
+        memset(temp, 0, sizeof(uint16_t) * (size_t)len);
+        for (int64_t i = t - 1; i >= 0; --i)
+            for (int64_t j = 0; j < len; ++j)
+                if (xs[i * len + j]) temp[j] = (uint16_t)(t - i);
+        int64_t k = 0;
+        for (; k < count; ++k)
+        {
+            int64_t idx = 0;
+            for (int64_t j = 1; j < len; ++j)
+                if (temp[idx] < temp[j]) idx = j;
+            if (temp[idx] == 0) break;
+            orc_winner *win = &winners[s * count + k];
+            win->c = 0, win->t = 0;
+            win->z = idx / hw, win->y = (idx % hw) / w, win->x = (idx % hw) % w;
+            int64_t sj = imax(0, win->y - ry), ej = imin(h, win->y + ry + 1);
+            int64_t sk = imax(0, win->x - rx), ek = imin(w, win->x + rx + 1);
+            for (int64_t i = 0; i < c; ++i)
+                for (int64_t j = sj; j < ej; ++j)
+                    for (int64_t q = sk; q < ek; ++q) temp[(i * h + j) * w + q] = 0;
+            for (int64_t j = 0; j < hw; ++j) temp[win->z * hw + j] = 0;
+        }
+        counts[s] = k;
+    }
+    free(temp);
+}
+
+/* src/spyker/cpu/wta.cpp:87-126 */
+void orc_rank_fcwta_f32(const float *x, int64_t n, int64_t t, int64_t f, int64_t radius, int64_t count,
+                        float threshold, orc_winner *winners, int64_t *counts)
+{
+    float *temp = (float *)malloc(sizeof(float) * (size_t)f);
+    for (int64_t s = 0; s < n; ++s)
+    {
+        wta_values_f32(x + s * t * f, t, f, threshold, temp);
+        int64_t k = 0;
+        for (; k < count; ++k)
+        {
+            int64_t idx = first_max_f32(temp, f);
+            if (fabsf(temp[idx] - (-FLT_MAX)) < EPS) break;
+            orc_winner *win = &winners[s * count + k];
+            win->c = win->t = win->z = win->y = 0;
+            win->x = idx;
+            int64_t a = imax(0, idx - radius), b = imin(f, idx + radius + 1);
+            for (int64_t j = a; j < b; ++j) temp[j] = -FLT_MAX;
+        }
+        counts[s] = k;
+    }
+    free(temp);
+}
+
+/* ------------------------------------------------------------------------ */
+/* pad / pool                                                               */
+/* ------------------------------------------------------------------------ */
+
+/* src/spyker/cpu/pool.cpp:41-51 */
+void orc_pad_u8(const uint8_t *in, int64_t planes, int64_t h, int64_t w, uint8_t *out, int64_t pt, int64_t pl,
+                int64_t pb, int64_t pr, uint8_t value)
+{
+    int64_t oh = h + pt + pb, ow = w + pl + pr;
+    for (int64_t i = 0; i < planes; ++i)
+    {
+        memset(out + i * oh * ow, value, (size_t)(oh * ow));
+        for (int64_t j = 0; j < h; ++j) memcpy(out + (i * oh + j + pt) * ow + pl, in + (i * h + j) * w, (size_t)w);
+    }
+}
+
+/* src/spyker/cpu/pool.cpp:53-86: zero pad, window max seeded with the type's
+ * lowest value. */
+#define POOL_BODY(T, LOWEST)                                                                           \
+    int64_t ph = h + pt + pb, pw = w + pl + pr;                                                        \
+    int64_t oh = (ph - kh) / sh + 1, ow = (pw - kw) / sw + 1;                                          \
+    for (int64_t i = 0; i < planes; ++i)                                                               \
+        for (int64_t y = 0; y < oh; ++y)                                                               \
+            for (int64_t x = 0; x < ow; ++x)                                                           \
+            {                                                                                          \
+                T m = LOWEST;                                                                          \
+                for (int64_t a = 0; a < kh; ++a)                                                       \
+                    for (int64_t b = 0; b < kw; ++b)                                                   \
+                    {                                                                                  \
+                        int64_t iy = y * sh + a - pt, ix = x * sw + b - pl;                            \
+                        T v = (iy >= 0 && iy < h && ix >= 0 && ix < w) ? in[(i * h + iy) * w + ix] : (T)0; \
+                        m = m < v ? v : m;                                                             \
+                    }                                                                                  \
+                out[(i * oh + y) * ow + x] = m;                                                        \
+            }
+
+void orc_rank_pool_u8(const uint8_t *in, int64_t planes, int64_t h, int64_t w, uint8_t *out, int64_t kh, int64_t kw,
+                      int64_t sh, int64_t sw, int64_t pt, int64_t pl, int64_t pb, int64_t pr)
+{
+    POOL_BODY(uint8_t, 0)
+}
+
+void orc_rank_pool_f32(const float *in, int64_t planes, int64_t h, int64_t w, float *out, int64_t kh, int64_t kw,
+                       int64_t sh, int64_t sw, int64_t pt, int64_t pl, int64_t pb, int64_t pr)
+{
+    POOL_BODY(float, -FLT_MAX)
+}
+
+/* ------------------------------------------------------------------------ */
+/* STDP                                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* src/spyker/cpu/stdp.cpp:121-137: computed in F64 and rounded to the kernel
+ * type; `old + rate * value` is contracted into one fused multiply-add in the
+ * reference build. */
+float orc_stdp_update(float w, int32_t potentiate, const orc_stdp_config *cfg)
+{
+    double old = (double)w;
+    double value = cfg->stabilize ? (old - cfg->lower) * (cfg->upper - old) : 1.0;
+    value = fma(potentiate ? cfg->positive : cfg->negative, value, old);
+    double hi = value < cfg->upper ? value : cfg->upper;         /* std::min(value, upper) */
+    double r = cfg->lower < hi ? hi : cfg->lower;                 /* std::max(lower, .) */
+    return (float)r;
+}
+
+/* src/spyker/cpu/stdp.cpp:139-169 */
+int32_t orc_rank_convstdp_f32(const uint8_t *in, int64_t n, int64_t t, int64_t cin, int64_t h, int64_t w,
+                              float *kernel, int64_t cout, int64_t kh, int64_t kw, const uint8_t *out, int64_t ho,
+                              int64_t wo, const orc_stdp_config *configs, int64_t nconfigs,
+                              const orc_winner *winners, const int64_t *counts, int64_t maxcount, int64_t sy,
+                              int64_t sx, int64_t pt, int64_t pl, int64_t pb, int64_t pr)
+{
+    (void)pb, (void)pr;
+    int64_t ksz = cin * kh * kw;
+    uint16_t *isum = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)ksz);
+    for (int64_t s = 0; s < n; ++s)
+        for (int64_t k = 0; k < counts[s]; ++k)
+        {
+            orc_winner win = winners[s * maxcount + k];
+            if (win.c < 0) continue;
+            if (win.c >= nconfigs)
+            {
+                free(isum);
+                return -1;
+            }
+            /* first step with a nonzero input at each tap of the padded input */
+            for (int64_t c = 0; c < cin; ++c)
+                for (int64_t a = 0; a < kh; ++a)
+                    for (int64_t b = 0; b < kw; ++b)
+                    {
+                        int64_t iy = win.y * sy + a - pt, ix = win.x * sx + b - pl;
+                        uint16_t first = (uint16_t)t;
+                        if (iy >= 0 && iy < h && ix >= 0 && ix < w)
+                            for (int64_t i = t - 1; i >= 0; --i)
+                                if (in[(((s * t + i) * cin + c) * h + iy) * w + ix]) first = (uint16_t)i;
+                        isum[(c * kh + a) * kw + b] = first;
+                    }
+            uint16_t osum = (uint16_t)t;
+            for (int64_t i = t - 1; i >= 0; --i)
+                if (out[(((s * t + i) * cout + win.z) * ho + win.y) * wo + win.x]) osum = (uint16_t)i;
+            float *kd = kernel + win.z * ksz;
+            for (int64_t i = 0; i < ksz; ++i) kd[i] = orc_stdp_update(kd[i], isum[i] <= osum, &configs[win.c]);
+        }
+    free(isum);
+    return 0;
+}
+
+/* src/spyker/cpu/stdp.cpp:42-97 */
+int32_t orc_rank_fcstdp_f32(const uint8_t *in, int64_t n, int64_t t, int64_t fin, float *kernel, int64_t fout,
+                            const uint8_t *out, const orc_stdp_config *configs, int64_t nconfigs,
+                            const orc_winner *winners, const int64_t *counts, int64_t maxcount)
+{
+    uint16_t *isum = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)fin);
+    for (int64_t s = 0; s < n; ++s)
+    {
+        for (int64_t f = 0; f < fin; ++f) isum[f] = (uint16_t)t;
+        for (int64_t i = t - 1; i >= 0; --i)
+            for (int64_t f = 0; f < fin; ++f)
+                if (in[(s * t + i) * fin + f]) isum[f] = (uint16_t)i;
+        for (int64_t k = 0; k < counts[s]; ++k)
+        {
+            orc_winner win = winners[s * maxcount + k];
+            if (win.c < 0) continue;
+            if (win.c >= nconfigs)
+            {
+                free(isum);
+                return -1;
+            }
+            uint16_t osum = (uint16_t)t;
+            for (int64_t i = t - 1; i >= 0; --i)
+                if (out[(s * t + i) * fout + win.x]) osum = (uint16_t)i;
+            float *kd = kernel + win.x * fin;
+            for (int64_t f = 0; f < fin; ++f) kd[f] = orc_stdp_update(kd[f], isum[f] <= osum, &configs[win.c]);
+        }
+    }
+    free(isum);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* contractions                                                             */
+/* ------------------------------------------------------------------------ */
+
+static inline float load_in(const void *in, int32_t is_u8, int64_t i)
+{
+    return is_u8 ? (float)((const uint8_t *)in)[i] : ((const float *)in)[i];
+}
+
+/* Cross-correlation with zero pad (src/spyker/cpu/conv.cpp semantics; the
+ * reference's arithmetic lives in oneDNN/cuDNN, so the summation order here
+ * is this project's canonical order). */
+void orc_conv_f32(const void *in, int32_t is_u8, int64_t n, int64_t cin, int64_t h, int64_t w, const float *kernel,
+                  int64_t cout, int64_t kh, int64_t kw, int64_t sy, int64_t sx, int64_t pt, int64_t pl, int64_t pb,
+                  int64_t pr, float *out, int32_t threads)
+{
+    int64_t ho = (h + pt + pb - kh) / sy + 1, wo = (w + pl + pr - kw) / sx + 1;
+#pragma omp parallel for collapse(2) num_threads(threads > 0 ? threads : 1)
+    for (int64_t s = 0; s < n; ++s)
+        for (int64_t o = 0; o < cout; ++o)
+            for (int64_t y = 0; y < ho; ++y)
+                for (int64_t x = 0; x < wo; ++x)
+                {
+                    float acc = 0.0f;
+                    for (int64_t c = 0; c < cin; ++c)
+                        for (int64_t a = 0; a < kh; ++a)
+                        {
+                            int64_t iy = y * sy + a - pt;
+                            if (iy < 0 || iy >= h) continue;
+                            for (int64_t b = 0; b < kw; ++b)
+                            {
+                                int64_t ix = x * sx + b - pl;
+                                if (ix < 0 || ix >= w) continue;
+                                acc = fmaf(load_in(in, is_u8, ((s * cin + c) * h + iy) * w + ix),
+                                           kernel[((o * cin + c) * kh + a) * kw + b], acc);
+                            }
+                        }
+                    out[((s * cout + o) * ho + y) * wo + x] = acc;
+                }
+}
+
+void orc_conv_f64(const void *in, int32_t is_u8, int64_t n, int64_t cin, int64_t h, int64_t w, const float *kernel,
+                  int64_t cout, int64_t kh, int64_t kw, int64_t sy, int64_t sx, int64_t pt, int64_t pl, int64_t pb,
+                  int64_t pr, double *out, int32_t threads)
+{
+    int64_t ho = (h + pt + pb - kh) / sy + 1, wo = (w + pl + pr - kw) / sx + 1;
+#pragma omp parallel for collapse(2) num_threads(threads > 0 ? threads : 1)
+    for (int64_t s = 0; s < n; ++s)
+        for (int64_t o = 0; o < cout; ++o)
+            for (int64_t y = 0; y < ho; ++y)
+                for (int64_t x = 0; x < wo; ++x)
+                {
+                    double acc = 0.0;
+                    for (int64_t c = 0; c < cin; ++c)
+                        for (int64_t a = 0; a < kh; ++a)
+                        {
+                            int64_t iy = y * sy + a - pt;
+                            if (iy < 0 || iy >= h) continue;
+                            for (int64_t b = 0; b < kw; ++b)
+                            {
+                                int64_t ix = x * sx + b - pl;
+                                if (ix < 0 || ix >= w) continue;
+                                acc += (double)load_in(in, is_u8, ((s * cin + c) * h + iy) * w + ix) *
+                                       (double)kernel[((o * cin + c) * kh + a) * kw + b];
+                            }
+                        }
+                    out[((s * cout + o) * ho + y) * wo + x] = acc;
+                }
+}
+
+/* out[r][o] = sum_f in[r][f] * k[o][f] (src/spyker/cpu/fc.cpp semantics). */
+void orc_fc_f32(const void *in, int32_t is_u8, int64_t rows, int64_t fin, const float *kernel, int64_t fout,
+                float *out)
+{
+    for (int64_t r = 0; r < rows; ++r)
+        for (int64_t o = 0; o < fout; ++o)
+        {
+            float acc = 0.0f;
+            for (int64_t f = 0; f < fin; ++f) acc = fmaf(load_in(in, is_u8, r * fin + f), kernel[o * fin + f], acc);
+            out[r * fout + o] = acc;
+        }
+}
+
+/* src/spyker/cpu/dog.cpp:39-60 */
+void orc_dog_f32(const float *in, int64_t planes, int64_t h, int64_t w, const float *kernel, int64_t count, int64_t k,
+                 int64_t pt, int64_t pl, int64_t pb, int64_t pr, float *out, int32_t threads)
+{
+    int64_t ho = h + pt + pb - k + 1, wo = w + pl + pr - k + 1, len = count * ho * wo;
+    float *mid = (float *)malloc(sizeof(float) * (size_t)(planes * 2 * len));
+    orc_conv_f32(in, 0, planes, 1, h, w, kernel, 2 * count, k, k, 1, 1, pt, pl, pb, pr, mid, threads);
+    for (int64_t i = 0; i < planes; ++i)
+        for (int64_t j = 0; j < len; ++j)
+        {
+            float d = mid[(i * 2) * len + j] - mid[(i * 2 + 1) * len + j];
+            out[i * len + j] = d < 0.0f ? 0.0f : d; /* std::max<T>(d, 0) */
+        }
+    free(mid);
+}
+
+/* src/spyker/cpu/dog.cpp:94 */
+void orc_gabor_f32(const float *in, int64_t planes, int64_t h, int64_t w, const float *kernel, int64_t count,
+                   int64_t k, int64_t pt, int64_t pl, int64_t pb, int64_t pr, float *out, int32_t threads)
+{
+    orc_conv_f32(in, 0, planes, 1, h, w, kernel, count, k, k, 1, 1, pt, pl, pb, pr, out, threads);
+}
+
+/* ------------------------------------------------------------------------ */
+/* host kernel generators                                                   */
+/* ------------------------------------------------------------------------ */
+
+static void normalize(double *v, int64_t n)
+{
+    double sum = 0;
+    for (int64_t i = 0; i < n; ++i) sum += v[i];
+    for (int64_t i = 0; i < n; ++i) v[i] /= sum;
+}
+
+/* src/spyker/cpu/init.cpp:51-62 */
+void orc_gaussian_kernel(double *kernel, int64_t k, double std)
+{
+    int64_t size = k / 2;
+    for (int64_t i = -size; i <= size; ++i)
+        for (int64_t j = -size; j <= size; ++j)
+        {
+            double dist = sqrt((double)(i * i + j * j));
+            double r = dist / std;
+            kernel[(i + size) * k + (j + size)] = exp(-.5 * (r * r));
+        }
+    normalize(kernel, k * k);
+}
+
+/* src/spyker/cpu/init.cpp:64-77; the sums of products are contracted into
+ * fused multiply-adds in the reference build. */
+void orc_gabor_kernel(double *kernel, int64_t k, double sigma, double theta, double gamma, double lambda, double psi)
+{
+    const double pi = 3.14159265358979323846;
+    int64_t size = k / 2;
+    double ct = cos(theta), st = sin(theta);
+    for (int64_t i = -size; i <= size; ++i)
+        for (int64_t j = -size; j <= size; ++j)
+        {
+            double x = fma((double)j, st, (double)i * ct);
+            double y = fma((double)j, ct, (double)-i * st);
+            double A = exp(-fma(x, x, gamma * gamma * y * y) / (2 * sigma * sigma));
+            double B = cos(2 * pi * x / lambda + psi);
+            kernel[(i + size) * k + (j + size)] = A * B;
+        }
+    normalize(kernel, k * k);
+}
