@@ -1,0 +1,219 @@
+/*
+ * spyker_hip.h — C ABI of the MI355X (gfx950) device backend for Spyker's
+ * dense STDP layer path.
+ *
+ * Every entry point replaces one device-side function of the reference's
+ * backend seam, Core::cuda_<op> (declared through the Declare macro,
+ * src/spyker/impl.h:40-43), and is cited beside its declaration.  The
+ * arguments are those of the seam flattened to plain pointers and sizes:
+ *
+ *   - tensors are device pointers to dense row-major buffers; a DynN view
+ *     of the reference (src/spyker/base.h:363-491) becomes (ptr, type, dims);
+ *   - Scalar arguments (src/spyker/utils.h:424-474) become double, converted
+ *     to the tensor's element type inside, as Scalar::to does;
+ *   - Len2/Len4 become explicit integers; pads are in the reference's order
+ *     (top, left, bottom, right) = Len4{t, z, y, x};
+ *   - Winners (a host std::vector, src/spyker/utils.h:502) become a device
+ *     array of spk_winner records [n][count] plus a device count per sample,
+ *     so a layer step never leaves the device;
+ *   - every call takes the hipStream_t it is ordered on (NULL = the null
+ *     stream) and is asynchronous unless stated.
+ *
+ * Status: 0 on success; otherwise a negative code and a message readable
+ * with spk_last_error() (thread-local).  The C++ layer above turns a nonzero
+ * status into std::runtime_error in the reference's message format
+ * (src/spyker/utils.h:50-66).
+ */
+#ifndef SPYKER_HIP_H
+#define SPYKER_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Element types, in the order of Spyker::Type (src/spyker/utils.h:96-117). */
+enum spk_type
+{
+    SPK_I8 = 0,
+    SPK_I16 = 1,
+    SPK_I32 = 2,
+    SPK_I64 = 3,
+    SPK_U8 = 4,
+    SPK_U16 = 5,
+    SPK_U32 = 6,
+    SPK_U64 = 7,
+    SPK_F16 = 8,
+    SPK_F32 = 9,
+    SPK_F64 = 10
+};
+
+enum spk_status
+{
+    SPK_OK = 0,
+    SPK_ERR_ARGUMENT = -1, /* shape, type or value check failed */
+    SPK_ERR_UNSUPPORTED = -2, /* element type not implemented on this device */
+    SPK_ERR_RUNTIME = -3, /* HIP runtime error */
+    SPK_ERR_CONFIG = -4 /* STDP configuration index out of range */
+};
+
+/* Winner: src/spyker/utils.h:483-499 (c = STDP config index, t = time,
+ * z = channel, y/x = position). */
+typedef struct spk_winner
+{
+    int64_t c, t, z, y, x;
+} spk_winner;
+
+/* STDPConfig: src/spyker/utils.h:505-524. */
+typedef struct spk_stdp_config
+{
+    double positive, negative;
+    int32_t stabilize;
+    double lower, upper;
+} spk_stdp_config;
+
+/* Maximum number of STDP configurations one call accepts. */
+#define SPK_MAX_CONFIGS 32
+
+/* -- runtime ------------------------------------------------------------- */
+
+/* Message of the last failing call on this thread. */
+const char *spk_last_error(void);
+/* Library version as MAJOR*10000 + MINOR*100 + PATCH. */
+int32_t spk_version(void);
+/* Core::cuda_device_count / cuda_set_device / cuda_sync (src/spyker/impl.h:75-86). */
+int32_t spk_device_count(int32_t *count);
+int32_t spk_set_device(int32_t device);
+int32_t spk_synchronize(void *stream);
+
+/* Caching allocator, Core::cuda_alloc/dealloc (src/spyker/cuda/memory.cu:139-155)
+ * and its queries cuda_memory_taken/used, cuda_cache_clear
+ * (src/spyker/cuda/memory.cu:157-198). */
+int32_t spk_alloc(void **ptr, int64_t bytes);
+int32_t spk_dealloc(void *ptr);
+int32_t spk_memory_taken(int64_t *bytes);
+int32_t spk_memory_used(int64_t *bytes);
+int32_t spk_cache_clear(void);
+
+/* -- elementwise --------------------------------------------------------- */
+
+/* Core::cuda_threshold(Dyn1 input, Scalar threshold, Scalar value), impl.h:235:
+ * in place, x <= threshold ? value : x.  Types: F32, F16, F64. */
+int32_t spk_threshold(void *x, int32_t type, int64_t n, double threshold, double value, void *stream);
+
+/* Core::cuda_rank_fire(Dyn1 input, Dyn1 output, Scalar threshold), impl.h:247:
+ * out = (in > threshold).  in: F32/F16/F64; out: U8 or F32. */
+int32_t spk_rank_fire(const void *in, int32_t in_type, void *out, int32_t out_type, int64_t n, double threshold,
+                      void *stream);
+
+/* Core::cuda_cast for one dense buffer (src/spyker/impl.h:157-161), used by
+ * the host layer for Tensor::to(type).  Pairs: U8->F32, U8->F16, F32->F16,
+ * F16->F32, F32->F64, F64->F32, F32->U8 (truncating). */
+int32_t spk_cast(const void *in, int32_t in_type, void *out, int32_t out_type, int64_t n, void *stream);
+
+/* Core::cuda_pad(Dyn3 input, Dyn3 output, Len4 pad, Scalar value), impl.h:231:
+ * planes of [h][w] into [h+pt+pb][w+pl+pr] filled with value.  Types: U8, F32. */
+int32_t spk_pad(const void *in, void *out, int32_t type, int64_t planes, int64_t h, int64_t w, int64_t pt, int64_t pl,
+                int64_t pb, int64_t pr, double value, void *stream);
+
+/* -- pooling, inhibition, winner-take-all --------------------------------- */
+
+/* Core::cuda_rank_pool(Dyn4 input, Dyn4 output, Len2 kernel, Len2 stride,
+ * Len4 pad), impl.h:259: per plane, zero pad then window max seeded with the
+ * type's lowest value.  Types: U8, F32. */
+int32_t spk_rank_pool(const void *in, void *out, int32_t type, int64_t planes, int64_t h, int64_t w, int64_t kh,
+                      int64_t kw, int64_t sh, int64_t sw, int64_t pt, int64_t pl, int64_t pb, int64_t pr, void *stream);
+
+/* Core::cuda_rank_inhibit(Dyn4 input, Scalar threshold), impl.h:263:
+ * x is [n][t][c][hw], in place.  Types: F32. */
+int32_t spk_rank_inhibit(void *x, int32_t type, int64_t n, int64_t t, int64_t c, int64_t hw, double threshold,
+                         void *stream);
+
+/* Core::cuda_rank_convwta(Dyn5 input, Len2 radius, Size count, Scalar
+ * threshold), impl.h:271: x is [n][t][c][h][w]; winners (device) receives
+ * n*count records, counts (device) n entries.  Types: F32, U8. */
+int32_t spk_rank_convwta(const void *x, int32_t type, int64_t n, int64_t t, int64_t c, int64_t h, int64_t w,
+                         int64_t ry, int64_t rx, int64_t count, double threshold, spk_winner *winners,
+                         int64_t *counts, void *stream);
+
+/* Core::cuda_rank_fcwta(Dyn3 input, Size radius, Size count, Scalar
+ * threshold), impl.h:267: x is [n][t][f].  Types: F32. */
+int32_t spk_rank_fcwta(const void *x, int32_t type, int64_t n, int64_t t, int64_t f, int64_t radius, int64_t count,
+                       double threshold, spk_winner *winners, int64_t *counts, void *stream);
+
+/* -- contractions and filters --------------------------------------------- */
+
+/* Core::cuda_conv(Dyn4 input, Dyn4 kernel, Dyn4 output, Len2 stride, Len4
+ * pad), impl.h:221: cross-correlation, zero pad.  in: U8 or F32
+ * [n][cin][h][w]; kernel F32 [cout][cin][kh][kw]; out F32 (or F64 with
+ * kernel F32, the high-precision check path).  FP32 accumulation is one fused
+ * multiply-add per tap in (cin, ky, kx) order starting from +0. */
+int32_t spk_conv(const void *in, int32_t in_type, const void *kernel, int32_t kernel_type, void *out,
+                 int32_t out_type, int64_t n, int64_t cin, int64_t h, int64_t w, int64_t cout, int64_t kh, int64_t kw,
+                 int64_t sy, int64_t sx, int64_t pt, int64_t pl, int64_t pb, int64_t pr, void *stream);
+
+/* Core::cuda_fc(Dyn3 input, Dyn2 kernel, Dyn3 output), impl.h:225:
+ * out[r][o] = sum_f in[r][f]*k[o][f], one fused multiply-add per f in
+ * order.  in: U8 or F32; kernel/out F32. */
+int32_t spk_fc(const void *in, int32_t in_type, const void *kernel, void *out, int64_t rows, int64_t fin,
+               int64_t fout, void *stream);
+
+/* Core::cuda_dog(Dyn4 input, Dyn4 kernel, Dyn4 output, Len4 pad), impl.h:201:
+ * in F32 [planes][h][w], kernel F32 [2][count][k][k], out F32
+ * [planes][count][ho][wo] = max(g1 - g2, 0). */
+int32_t spk_dog(const void *in, const void *kernel, void *out, int64_t planes, int64_t h, int64_t w, int64_t count,
+                int64_t k, int64_t pt, int64_t pl, int64_t pb, int64_t pr, void *stream);
+
+/* Core::cuda_gabor(Dyn4 input, Dyn4 kernel, Dyn4 output, Len4 pad), impl.h:205:
+ * in F32 [planes][h][w], kernel F32 [count][k][k], out F32 [planes][count][ho][wo]. */
+int32_t spk_gabor(const void *in, const void *kernel, void *out, int64_t planes, int64_t h, int64_t w, int64_t count,
+                  int64_t k, int64_t pt, int64_t pl, int64_t pb, int64_t pr, void *stream);
+
+/* -- coding --------------------------------------------------------------- */
+
+/* Core::cuda_rank_code(Dyn2 input, Dyn3 output, bool sort), impl.h:243:
+ * in F32 [n][len] -> out U8 [n][time][len], cumulative spikes.  sort=1 ranks
+ * by value (descending, ties by ascending index); sort=0 is latency coding. */
+int32_t spk_rank_code(const void *in, int32_t in_type, void *out, int32_t out_type, int64_t n, int64_t len,
+                      int64_t time, int32_t sort, void *stream);
+
+/* -- learning ------------------------------------------------------------- */
+
+/* Core::cuda_rank_convstdp(Dyn5 input, Dyn4 kernel, Dyn5 output, configs,
+ * Winners, Len2 stride, Len4 pad), impl.h:275-281.  in/out U8 spikes
+ * [n][t][cin][h][w] / [n][t][cout][ho][wo]; kernel F32 [cout][cin][kh][kw]
+ * updated in place in (sample, winner) order; configs on the host; winners
+ * and counts on the device ([n][maxcount], [n]).  Winners with c < 0 are
+ * skipped; any c >= nconfigs fails the call before any weight changes
+ * (SPK_ERR_CONFIG). */
+int32_t spk_rank_convstdp(const void *in, void *kernel, int32_t kernel_type, const void *out, int64_t n, int64_t t,
+                          int64_t cin, int64_t h, int64_t w, int64_t cout, int64_t kh, int64_t kw, int64_t ho,
+                          int64_t wo, const spk_stdp_config *configs, int64_t nconfigs, const spk_winner *winners,
+                          const int64_t *counts, int64_t maxcount, int64_t sy, int64_t sx, int64_t pt, int64_t pl,
+                          int64_t pb, int64_t pr, void *stream);
+
+/* Core::cuda_rank_fcstdp(Dyn3 input, Dyn2 kernel, Dyn3 output, configs,
+ * Winners), impl.h:282-286.  in [n][t][fin] U8, out [n][t][fout] U8,
+ * kernel F32 [fout][fin]. */
+int32_t spk_rank_fcstdp(const void *in, void *kernel, int32_t kernel_type, const void *out, int64_t n, int64_t t,
+                        int64_t fin, int64_t fout, const spk_stdp_config *configs, int64_t nconfigs,
+                        const spk_winner *winners, const int64_t *counts, int64_t maxcount, void *stream);
+
+/* -- host kernel generators (F64, host memory) ---------------------------- */
+
+/* Core::random_seed (src/spyker/impl.cpp:43): reseeds the std::mt19937 that
+ * spk_normal_kernel draws from. */
+int32_t spk_random_seed(int64_t seed);
+/* Core::normal_kernel (src/spyker/cpu/init.cpp:45-49): std::normal_distribution<double>. */
+int32_t spk_normal_kernel(double *kernel, int64_t n, double mean, double std);
+/* Core::gaussian_kernel (src/spyker/cpu/init.cpp:51-62): k x k, sum-normalised. */
+int32_t spk_gaussian_kernel(double *kernel, int64_t k, double std);
+/* Core::gabor_kernel (src/spyker/cpu/init.cpp:64-77): k x k, sum-normalised, theta in radians. */
+int32_t spk_gabor_kernel(double *kernel, int64_t k, double sigma, double theta, double gamma, double lambda,
+                         double psi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -25,9 +25,14 @@
 static inline int64_t imax(int64_t a, int64_t b) { return a > b ? a : b; }
 static inline int64_t imin(int64_t a, int64_t b) { return a < b ? a : b; }
 
-/* float -> U16 as GCC on x86-64 emits it: truncate to a 32-bit integer, keep
- * the low 16 bits (src/spyker/cpu/code.cpp:268,316 store F32 into U16). */
-static inline uint16_t f32_to_u16(float f) { return (uint16_t)(uint32_t)(int32_t)f; }
+/* float -> U16 as GCC on x86-64 emits it (src/spyker/cpu/code.cpp:268,316
+ * store F32 into U16): cvttss2si to a 32-bit integer, where NaN and values
+ * out of range give 0x80000000, then keep the low 16 bits. */
+static inline uint16_t f32_to_u16(float f)
+{
+    if (!(f > -2147483904.0f && f < 2147483648.0f)) return 0;
+    return (uint16_t)(uint32_t)(int32_t)f;
+}
 
 /* ------------------------------------------------------------------------ */
 /* Rank-order coding                                                        */

@@ -1,0 +1,520 @@
+"""Spyker's dense STDP layer path on MI355X (gfx950).
+
+The functions and modules here mirror the reference's Python package
+(src/python/spyker/module.py) for the hot path -- same names, argument
+meaning, defaults and shape rules -- over torch tensors on a ROCm device.
+Every computation runs in libspyker_hip.so through its C ABI
+(include/spyker_hip.h); nothing falls back to the CPU.
+
+    DoG, Gabor, Conv, FC            module.py:174,253,537,665
+    threshold, fire, code           module.py:923,1072,993
+    pool, inhibit, convwta, fcwta   module.py:1177,1230,1293,1262
+    conv, fc, pad                   module.py:835,801,882
+    STDPConfig, DoGFilter, GaborFilter, Winner
+
+Winner lists stay on the device: convwta/fcwta return a ``Winners`` object
+holding device tensors, which ``Conv.stdp``/``FC.stdp`` consume directly and
+which reads like the reference's list of lists when indexed.
+"""
+
+import ctypes as C
+import math
+from typing import List, Sequence, Union
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call
+
+__all__ = [
+    "DoGFilter", "GaborFilter", "STDPConfig", "Winner", "Winners", "DoG", "Gabor", "Conv", "FC", "threshold",
+    "fire", "code", "pool", "inhibit", "convwta", "fcwta", "conv", "fc", "pad", "random_seed", "expand2", "expand4",
+]
+
+_TYPES = {torch.uint8: _lib.U8, torch.float16: _lib.F16, torch.float32: _lib.F32, torch.float64: _lib.F64,
+          torch.int8: _lib.I8, torch.int16: _lib.I16, torch.int32: _lib.I32, torch.int64: _lib.I64}
+_DTYPES = {"u8": torch.uint8, "f16": torch.float16, "f32": torch.float32, "f64": torch.float64}
+
+
+def _dtype(d):
+    if isinstance(d, torch.dtype):
+        return d
+    return _DTYPES[d]
+
+
+def _type(t):
+    try:
+        return _TYPES[t.dtype]
+    except KeyError:
+        raise RuntimeError("Data type %s is not supported." % t.dtype)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream(t):
+    return C.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+
+
+def _dev(t):
+    if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
+        raise RuntimeError("spyker_amd operates on ROCm device tensors (device 'cuda').")
+    if not t.is_contiguous():
+        raise RuntimeError("Input tensor must be contiguous.")
+    return t
+
+
+# -- shape helpers (src/python/spyker/private.py:90-109, src/spyker/opers.cpp:156-227) --
+
+def expand2(v):
+    if isinstance(v, int):
+        return (v, v)
+    v = tuple(int(x) for x in v)
+    if len(v) == 1:
+        return (v[0], v[0])
+    if len(v) != 2:
+        raise RuntimeError("Incorrect input shape.")
+    return v
+
+
+def expand4(v):
+    """Pad as (top, left, bottom, right): one value for all sides, (vertical, horizontal), or four."""
+    if isinstance(v, int):
+        return (v, v, v, v)
+    v = tuple(int(x) for x in v)
+    if len(v) == 1:
+        return (v[0],) * 4
+    if len(v) == 2:
+        return (v[0], v[1], v[0], v[1])
+    if len(v) != 4:
+        raise RuntimeError("Incorrect input shape.")
+    return v
+
+
+def _least(t, n):
+    s = list(t.shape)
+    while len(s) < n:
+        s.insert(0, 1)
+    return t.reshape(s)
+
+
+def _exact(t, n):
+    s = list(t.shape)
+    if len(s) == n - 1:
+        s.insert(0, 1)
+    if len(s) != n:
+        raise RuntimeError("Input dimensions couldn't be viewed as %dD." % n)
+    return t.reshape(s)
+
+
+# -- parameter containers ------------------------------------------------------
+
+class DoGFilter:
+    """Difference-of-Gaussians parameters (src/spyker/opers.h:162-173)."""
+
+    def __init__(self, std1: float, std2: float):
+        self.std1, self.std2 = float(std1), float(std2)
+
+
+class GaborFilter:
+    """Gabor parameters (src/spyker/opers.h:175-211); theta in degrees."""
+
+    def __init__(self, sigma: float, theta: float, gamma: float, lambda_: float, psi: float):
+        self.sigma, self.gamma, self.lambda_, self.psi = float(sigma), float(gamma), float(lambda_), float(psi)
+        self.theta = float(theta) * 0.017453292519943295
+
+
+class STDPConfig:
+    """STDP parameters (src/spyker/utils.h:505-524)."""
+
+    def __init__(self, positive: float, negative: float, stabilize: bool = True, lower: float = 0.0,
+                 upper: float = 1.0):
+        self.positive, self.negative = float(positive), float(negative)
+        self.stabilize, self.lower, self.upper = bool(stabilize), float(lower), float(upper)
+
+    def _c(self):
+        return _lib.spk_stdp_config(self.positive, self.negative, int(self.stabilize), self.lower, self.upper)
+
+    def __repr__(self):
+        return "STDPConfig(%g, %g, %s, %g, %g)" % (self.positive, self.negative, self.stabilize, self.lower,
+                                                   self.upper)
+
+
+class Winner:
+    """Winner record (src/spyker/utils.h:483-499)."""
+
+    __slots__ = ("c", "t", "z", "y", "x")
+
+    def __init__(self, c=0, t=0, z=0, y=0, x=0):
+        self.c, self.t, self.z, self.y, self.x = int(c), int(t), int(z), int(y), int(x)
+
+    def __eq__(self, o):
+        return (self.c, self.t, self.z, self.y, self.x) == (o.c, o.t, o.z, o.y, o.x)
+
+    def __repr__(self):
+        return "Winner(c=%d, t=%d, z=%d, y=%d, x=%d)" % (self.c, self.t, self.z, self.y, self.x)
+
+
+class Winners:
+    """Per-sample winner lists kept on the device.
+
+    ``data`` is an int64 device tensor [n, count, 5] of (c, t, z, y, x) and
+    ``counts`` an int64 device tensor [n].  Indexing or iterating copies them
+    to the host once and yields lists of ``Winner``.
+    """
+
+    def __init__(self, data: torch.Tensor, counts: torch.Tensor):
+        self.data, self.counts = data, counts
+        self._host = None
+
+    @staticmethod
+    def from_lists(lists, device):
+        n = len(lists)
+        cap = max([len(w) for w in lists] + [1])
+        arr = np.zeros((n, cap, 5), np.int64)
+        cnt = np.zeros(n, np.int64)
+        for i, ws in enumerate(lists):
+            cnt[i] = len(ws)
+            for k, w in enumerate(ws):
+                arr[i, k] = (w.c, w.t, w.z, w.y, w.x)
+        return Winners(torch.from_numpy(arr).to(device), torch.from_numpy(cnt).to(device))
+
+    def tolist(self) -> List[List[Winner]]:
+        if self._host is None:
+            d, c = self.data.cpu().numpy(), self.counts.cpu().numpy()
+            self._host = [[Winner(*d[i, k]) for k in range(int(c[i]))] for i in range(d.shape[0])]
+        return self._host
+
+    def set_config(self, index):
+        """Sets every winner's STDP configuration index (R-STDP); index: int or per-sample sequence."""
+        idx = torch.as_tensor(index, dtype=torch.int64, device=self.data.device)
+        if idx.dim() == 0:
+            self.data[:, :, 0] = idx
+        else:
+            self.data[:, :, 0] = idx.reshape(-1, 1)
+        self._host = None
+
+    def __len__(self):
+        return self.data.shape[0]
+
+    def __getitem__(self, i):
+        return self.tolist()[i]
+
+    def __iter__(self):
+        return iter(self.tolist())
+
+
+def _winners(w, device):
+    if isinstance(w, Winners):
+        return w
+    return Winners.from_lists(w, device)
+
+
+def random_seed(seed: int):
+    """Reseeds the host generator used by Conv/FC initialisation (src/spyker/impl.cpp:43)."""
+    call("spk_random_seed", C.c_int64(int(seed)))
+
+
+# -- free functions ------------------------------------------------------------
+
+def threshold(array: torch.Tensor, threshold: float, value: float = 0.0, inplace: bool = True) -> torch.Tensor:
+    """x <= threshold -> value (src/spyker/opers.cpp:472-478)."""
+    if not inplace:
+        array = array.clone()
+    _dev(array)
+    call("spk_threshold", _ptr(array), _type(array), array.numel(), float(threshold), float(value), _stream(array))
+    return array
+
+
+def fire(array: torch.Tensor, threshold: float = 0.0, dtype="u8") -> torch.Tensor:
+    """Spikes where x > threshold (src/spyker/opers.cpp:522-552)."""
+    _dev(array)
+    out = torch.empty(array.shape, dtype=_dtype(dtype), device=array.device)
+    call("spk_rank_fire", _ptr(array), _type(array), _ptr(out), _type(out), array.numel(), float(threshold),
+         _stream(array))
+    return out
+
+
+def code(array: torch.Tensor, time: int, sort: bool = True, dtype="u8") -> torch.Tensor:
+    """Rank-order code into cumulative spikes [N, time, ...] (src/spyker/opers.cpp:488-511)."""
+    _dev(array)
+    a = _least(array, 2)
+    if time <= 0:
+        raise RuntimeError("Input time can't be less than 1.")
+    shape = [a.shape[0], int(time)] + list(a.shape[1:])
+    out = torch.empty(shape, dtype=_dtype(dtype), device=array.device)
+    n = a.shape[0]
+    call("spk_rank_code", _ptr(a), _type(a), _ptr(out), _type(out), n, a.numel() // max(n, 1), int(time),
+         int(bool(sort)), _stream(array))
+    return out
+
+
+def pool(array: torch.Tensor, kernel, stride=None, pad=0) -> torch.Tensor:
+    """Max pooling per plane with zero pad (src/spyker/opers.cpp:600-637)."""
+    _dev(array)
+    if stride is None:
+        stride = kernel
+    kh, kw = expand2(kernel)
+    sh, sw = expand2(stride)
+    if (sh, sw) == (0, 0):
+        sh, sw = kh, kw
+    pt, pl, pb, pr = expand4(pad)
+    a = _exact(array, 5)
+    h, w = a.shape[3], a.shape[4]
+    oh, ow = (h + pt + pb - kh) // sh + 1, (w + pl + pr - kw) // sw + 1
+    if kh < 1 or kw < 1 or sh < 1 or sw < 1 or min(pt, pl, pb, pr) < 0 or oh <= 0 or ow <= 0:
+        raise RuntimeError("Input and kernel shapes are not compatible.")
+    out = torch.empty(list(a.shape[:3]) + [oh, ow], dtype=a.dtype, device=a.device)
+    planes = a.shape[0] * a.shape[1] * a.shape[2]
+    call("spk_rank_pool", _ptr(a), _ptr(out), _type(a), planes, h, w, kh, kw, sh, sw, pt, pl, pb, pr, _stream(a))
+    return out
+
+
+def inhibit(array: torch.Tensor, threshold: float = 0.0, inplace: bool = True) -> torch.Tensor:
+    """Lateral inhibition across channels (src/spyker/opers.cpp:639-646)."""
+    if not inplace:
+        array = array.clone()
+    _dev(array)
+    a = _exact(array, 5)
+    n, t, c = a.shape[:3]
+    call("spk_rank_inhibit", _ptr(a), _type(a), n, t, c, a.shape[3] * a.shape[4], float(threshold), _stream(a))
+    return array
+
+
+def convwta(array: torch.Tensor, radius, count: int, threshold: float = 0.0) -> Winners:
+    """Convolutional winner-take-all (src/spyker/opers.cpp:656-662)."""
+    _dev(array)
+    ry, rx = expand2(radius)
+    if ry < 0 or rx < 0:
+        raise RuntimeError("Radius items are incorrect.")
+    if count <= 0:
+        raise RuntimeError("Winners count can't be less than 1.")
+    a = _exact(array, 5)
+    n, t, c, h, w = a.shape
+    data = torch.zeros((n, count, 5), dtype=torch.int64, device=a.device)
+    cnt = torch.zeros((n,), dtype=torch.int64, device=a.device)
+    call("spk_rank_convwta", _ptr(a), _type(a), n, t, c, h, w, ry, rx, int(count), float(threshold), _ptr(data),
+         _ptr(cnt), _stream(a))
+    return Winners(data, cnt)
+
+
+def fcwta(array: torch.Tensor, radius: int, count: int, threshold: float = 0.0) -> Winners:
+    """Fully-connected winner-take-all (src/spyker/opers.cpp:648-654)."""
+    _dev(array)
+    if radius < 0:
+        raise RuntimeError("Radius can't be less than 0.")
+    if count <= 0:
+        raise RuntimeError("Winners count can't be less than 1.")
+    a = _exact(array, 3)
+    n, t, f = a.shape
+    data = torch.zeros((n, count, 5), dtype=torch.int64, device=a.device)
+    cnt = torch.zeros((n,), dtype=torch.int64, device=a.device)
+    call("spk_rank_fcwta", _ptr(a), _type(a), n, t, f, int(radius), int(count), float(threshold), _ptr(data),
+         _ptr(cnt), _stream(a))
+    return Winners(data, cnt)
+
+
+def conv(array: torch.Tensor, kernel: torch.Tensor, stride=1, pad=0) -> torch.Tensor:
+    """Time-unrolled cross-correlation, T folded into the batch (src/spyker/opers.cpp:400-423)."""
+    _dev(array)
+    _dev(kernel)
+    sy, sx = expand2(stride)
+    pt, pl, pb, pr = expand4(pad)
+    a = _exact(array, 5)
+    n, t, cin, h, w = a.shape
+    cout, kc, kh, kw = kernel.shape
+    if kc != cin:
+        raise RuntimeError("Input and kernel channels are not compatible.")
+    ho, wo = (h + pt + pb - kh) // sy + 1, (w + pl + pr - kw) // sx + 1
+    out = torch.empty((n, t, cout, ho, wo), dtype=kernel.dtype, device=a.device)
+    call("spk_conv", _ptr(a), _type(a), _ptr(kernel), _type(kernel), _ptr(out), _type(out), n * t, cin, h, w, cout,
+         kh, kw, sy, sx, pt, pl, pb, pr, _stream(a))
+    return out
+
+
+def fc(array: torch.Tensor, kernel: torch.Tensor) -> torch.Tensor:
+    """out[n,t,o] = sum_f in[n,t,f] k[o,f] (src/spyker/opers.cpp:425-445)."""
+    _dev(array)
+    _dev(kernel)
+    a = _exact(array, 3)
+    n, t, fin = a.shape
+    fout, kf = kernel.shape
+    if kf != fin:
+        raise RuntimeError("Input and kernel shapes don't match.")
+    out = torch.empty((n, t, fout), dtype=kernel.dtype, device=a.device)
+    call("spk_fc", _ptr(a), _type(a), _ptr(kernel), _ptr(out), n * t, fin, fout, _stream(a))
+    return out
+
+
+def pad(array: torch.Tensor, pad, value: float = 0.0) -> torch.Tensor:
+    """Spatial padding of the last two dimensions (src/spyker/opers.cpp:447-470)."""
+    _dev(array)
+    pt, pl, pb, pr = expand4(pad)
+    a = _exact(array, 5)
+    h, w = a.shape[3], a.shape[4]
+    out = torch.empty(list(a.shape[:3]) + [h + pt + pb, w + pl + pr], dtype=a.dtype, device=a.device)
+    call("spk_pad", _ptr(a), _ptr(out), _type(a), a.shape[0] * a.shape[1] * a.shape[2], h, w, pt, pl, pb, pr,
+         float(value), _stream(a))
+    return out
+
+
+def _to_f32(t):
+    if t.dtype == torch.float32:
+        return t
+    out = torch.empty(t.shape, dtype=torch.float32, device=t.device)
+    call("spk_cast", _ptr(t), _type(t), _ptr(out), _lib.F32, t.numel(), _stream(t))
+    return out
+
+
+def _host_kernel(fn, shape, *args):
+    k = np.zeros(shape, np.float64)
+    call(fn, k.ctypes.data_as(C.c_void_p), *args)
+    return k
+
+
+def _device(device):
+    if device is None:
+        return torch.device("cuda", torch.cuda.current_device())
+    return torch.device(device)
+
+
+# -- modules -------------------------------------------------------------------
+
+class DoG:
+    """Difference-of-Gaussians filter bank (src/spyker/opers.cpp:702-735)."""
+
+    def __init__(self, size: int, filters: Union[DoGFilter, Sequence[DoGFilter]], pad=0, device=None, dtype="f32"):
+        if size <= 0:
+            raise RuntimeError("Input size can't be less than 1.")
+        flist = [filters] if isinstance(filters, DoGFilter) else list(filters)
+        if not flist:
+            raise RuntimeError("Input filters count can't be less then 1.")
+        k = 2 * size + 1
+        ker = np.zeros((2, len(flist), k, k), np.float64)
+        for i, f in enumerate(flist):
+            ker[0, i] = _host_kernel("spk_gaussian_kernel", (k, k), k, f.std1)
+            ker[1, i] = _host_kernel("spk_gaussian_kernel", (k, k), k, f.std2)
+        self._pad = expand4(pad)
+        self.kernel = torch.from_numpy(ker).to(_dtype(dtype)).to(_device(device))
+
+    def __call__(self, array: torch.Tensor) -> torch.Tensor:
+        _dev(array)
+        a = _to_f32(_exact(array, 4)).contiguous()
+        n, c, h, w = a.shape
+        count, k = self.kernel.shape[1], self.kernel.shape[2]
+        pt, pl, pb, pr = self._pad
+        ho, wo = h + pt + pb - k + 1, w + pl + pr - k + 1
+        if ho <= 0 or wo <= 0:
+            raise RuntimeError("Input and kernel shapes are not compatible.")
+        out = torch.empty((n, c * count, ho, wo), dtype=self.kernel.dtype, device=a.device)
+        call("spk_dog", _ptr(a), _ptr(self.kernel), _ptr(out), n * c, h, w, count, k, pt, pl, pb, pr, _stream(a))
+        return out
+
+
+class Gabor:
+    """Gabor filter bank (src/spyker/opers.cpp:739-772)."""
+
+    def __init__(self, size: int, filters: Union[GaborFilter, Sequence[GaborFilter]], pad=0, device=None,
+                 dtype="f32"):
+        if size <= 0:
+            raise RuntimeError("Input size can't be less than 1.")
+        flist = [filters] if isinstance(filters, GaborFilter) else list(filters)
+        if not flist:
+            raise RuntimeError("Input filters count can't be less then 1.")
+        k = 2 * size + 1
+        ker = np.zeros((len(flist), k, k), np.float64)
+        for i, f in enumerate(flist):
+            ker[i] = _host_kernel("spk_gabor_kernel", (k, k), k, f.sigma, f.theta, f.gamma, f.lambda_, f.psi)
+        self._pad = expand4(pad)
+        self.kernel = torch.from_numpy(ker).to(_dtype(dtype)).to(_device(device))
+
+    def __call__(self, array: torch.Tensor) -> torch.Tensor:
+        _dev(array)
+        a = _to_f32(_exact(array, 4)).contiguous()
+        n, c, h, w = a.shape
+        count, k = self.kernel.shape[0], self.kernel.shape[1]
+        pt, pl, pb, pr = self._pad
+        ho, wo = h + pt + pb - k + 1, w + pl + pr - k + 1
+        if ho <= 0 or wo <= 0:
+            raise RuntimeError("Input and kernel shapes are not compatible.")
+        out = torch.empty((n, c * count, ho, wo), dtype=self.kernel.dtype, device=a.device)
+        call("spk_gabor", _ptr(a), _ptr(self.kernel), _ptr(out), n * c, h, w, count, k, pt, pl, pb, pr, _stream(a))
+        return out
+
+
+def _configs(lst):
+    if not lst:
+        raise RuntimeError("At least one STDP Config must be given.")
+    arr = (_lib.spk_stdp_config * len(lst))(*[c._c() for c in lst])
+    return arr, len(lst)
+
+
+class Conv:
+    """2D convolution layer with rank STDP (src/spyker/opers.cpp:838-889)."""
+
+    def __init__(self, insize: int, outsize: int, kernel, stride=1, pad=0, mean: float = 0.5, std: float = 0.02,
+                 device=None, dtype="f32"):
+        if insize <= 0:
+            raise RuntimeError("Input size can't be less than 1.")
+        if outsize <= 0:
+            raise RuntimeError("Output size can't be less than 1.")
+        kh, kw = expand2(kernel)
+        self._stride, self._pad = expand2(stride), expand4(pad)
+        ker = _host_kernel("spk_normal_kernel", (outsize, insize, kh, kw), outsize * insize * kh * kw, float(mean),
+                           float(std))
+        self.kernel = torch.from_numpy(ker).to(_dtype(dtype)).to(_device(device))
+        self.stdpconfig: List[STDPConfig] = []
+
+    def __call__(self, array: torch.Tensor) -> torch.Tensor:
+        return conv(array, self.kernel, self._stride, self._pad)
+
+    def stdp(self, array: torch.Tensor, winners, output: torch.Tensor) -> None:
+        _dev(array)
+        _dev(output)
+        if len(winners) == 0:
+            raise RuntimeError("Winners can't be empty.")
+        cfg, ncfg = _configs(self.stdpconfig)
+        a, o = _exact(array, 5), _exact(output, 5)
+        if a.dtype != o.dtype:
+            raise RuntimeError("Input and output must have the same data type.")
+        if a.dtype != torch.uint8:
+            raise RuntimeError("STDP: spikes must be u8.")
+        wn = _winners(winners, a.device)
+        n, t, cin, h, w = a.shape
+        cout, _, kh, kw = self.kernel.shape
+        sy, sx = self._stride
+        pt, pl, pb, pr = self._pad
+        ho, wo = o.shape[3], o.shape[4]
+        call("spk_rank_convstdp", _ptr(a), _ptr(self.kernel), _type(self.kernel), _ptr(o), n, t, cin, h, w, cout, kh,
+             kw, ho, wo, cfg, ncfg, _ptr(wn.data), _ptr(wn.counts), wn.data.shape[1], sy, sx, pt, pl, pb, pr,
+             _stream(a))
+
+
+class FC:
+    """Fully-connected layer with rank STDP (src/spyker/opers.cpp:891-942)."""
+
+    def __init__(self, insize: int, outsize: int, mean: float = 0.5, std: float = 0.02, device=None, dtype="f32"):
+        if insize <= 0:
+            raise RuntimeError("Input size can't be less than 1.")
+        if outsize <= 0:
+            raise RuntimeError("Output size can't be less than 1.")
+        ker = _host_kernel("spk_normal_kernel", (outsize, insize), outsize * insize, float(mean), float(std))
+        self.kernel = torch.from_numpy(ker).to(_dtype(dtype)).to(_device(device))
+        self.stdpconfig: List[STDPConfig] = []
+
+    def __call__(self, array: torch.Tensor) -> torch.Tensor:
+        return fc(array, self.kernel)
+
+    def stdp(self, array: torch.Tensor, winners, output: torch.Tensor) -> None:
+        _dev(array)
+        _dev(output)
+        if len(winners) == 0:
+            raise RuntimeError("Winners can't be empty.")
+        cfg, ncfg = _configs(self.stdpconfig)
+        a, o = _exact(array, 3), _exact(output, 3)
+        wn = _winners(winners, a.device)
+        n, t, fin = a.shape
+        call("spk_rank_fcstdp", _ptr(a), _ptr(self.kernel), _type(self.kernel), _ptr(o), n, t, fin,
+             self.kernel.shape[0], cfg, ncfg, _ptr(wn.data), _ptr(wn.counts), wn.data.shape[1], _stream(a))

@@ -1,0 +1,87 @@
+"""ctypes binding of libspyker_hip.so (the C ABI declared in include/spyker_hip.h).
+
+The library is loaded from this package directory.  There is no fallback:
+if it is missing or cannot be loaded, every call raises.
+"""
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PATH = os.path.join(HERE, "libspyker_hip.so")
+
+I8, I16, I32, I64, U8, U16, U32, U64, F16, F32, F64 = range(11)
+
+_i32, _i64, _f64, _p = C.c_int32, C.c_int64, C.c_double, C.c_void_p
+
+
+class spk_winner(C.Structure):
+    _fields_ = [("c", C.c_int64), ("t", C.c_int64), ("z", C.c_int64), ("y", C.c_int64), ("x", C.c_int64)]
+
+
+class spk_stdp_config(C.Structure):
+    _fields_ = [("positive", C.c_double), ("negative", C.c_double), ("stabilize", C.c_int32),
+                ("lower", C.c_double), ("upper", C.c_double)]
+
+
+# name -> argument types (all return int32 status unless listed in _RESTYPE)
+SIGNATURES = {
+    "spk_last_error": [],
+    "spk_version": [],
+    "spk_device_count": [_p],
+    "spk_set_device": [_i32],
+    "spk_synchronize": [_p],
+    "spk_alloc": [_p, _i64],
+    "spk_dealloc": [_p],
+    "spk_memory_taken": [_p],
+    "spk_memory_used": [_p],
+    "spk_cache_clear": [],
+    "spk_threshold": [_p, _i32, _i64, _f64, _f64, _p],
+    "spk_rank_fire": [_p, _i32, _p, _i32, _i64, _f64, _p],
+    "spk_cast": [_p, _i32, _p, _i32, _i64, _p],
+    "spk_pad": [_p, _p, _i32, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _f64, _p],
+    "spk_rank_pool": [_p, _p, _i32, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _p],
+    "spk_rank_inhibit": [_p, _i32, _i64, _i64, _i64, _i64, _f64, _p],
+    "spk_rank_convwta": [_p, _i32, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _f64, _p, _p, _p],
+    "spk_rank_fcwta": [_p, _i32, _i64, _i64, _i64, _i64, _i64, _f64, _p, _p, _p],
+    "spk_conv": [_p, _i32, _p, _i32, _p, _i32] + [_i64] * 13 + [_p],
+    "spk_fc": [_p, _i32, _p, _p, _i64, _i64, _i64, _p],
+    "spk_dog": [_p, _p, _p] + [_i64] * 9 + [_p],
+    "spk_gabor": [_p, _p, _p] + [_i64] * 9 + [_p],
+    "spk_rank_code": [_p, _i32, _p, _i32, _i64, _i64, _i64, _i32, _p],
+    "spk_rank_convstdp": [_p, _p, _i32, _p] + [_i64] * 10 + [_p, _i64, _p, _p, _i64] + [_i64] * 6 + [_p],
+    "spk_rank_fcstdp": [_p, _p, _i32, _p, _i64, _i64, _i64, _i64, _p, _i64, _p, _p, _i64, _p],
+    "spk_random_seed": [_i64],
+    "spk_normal_kernel": [_p, _i64, _f64, _f64],
+    "spk_gaussian_kernel": [_p, _i64, _f64],
+    "spk_gabor_kernel": [_p, _i64, _f64, _f64, _f64, _f64, _f64],
+}
+_RESTYPE = {"spk_last_error": C.c_char_p}
+
+_lib = None
+
+
+def load():
+    """Returns the loaded library, raising if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(PATH):
+        raise RuntimeError("spyker_amd: %s is not built (python -m spyker_amd.build)" % PATH)
+    lib = C.CDLL(PATH)
+    for name, args in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = _RESTYPE.get(name, C.c_int32)
+    _lib = lib
+    return lib
+
+
+def call(name, *args):
+    """Calls one entry point; a nonzero status raises RuntimeError with the library's message."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        msg = lib.spk_last_error().decode(errors="replace")
+        raise RuntimeError("%s failed (%d): %s" % (name, rc, msg))
+    return rc

@@ -1,0 +1,157 @@
+// Runtime part of the C ABI: errors, devices and the caching allocator.
+//
+// The allocator follows the reference's CUDA allocator
+// (src/spyker/cuda/memory.cu:40-155): blocks are cached by exact size and
+// handed out again on the next request of that size; cache_clear returns
+// the cached blocks to the driver.  Per-call scratch does not use it: it
+// comes from the stream-ordered pool (hipMallocAsync), so a kernel still
+// reading a block never sees it reused by another stream.
+#include <map>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+#include "common.hpp"
+
+namespace spk
+{
+namespace
+{
+thread_local std::string g_last_error;
+
+struct Cache
+{
+    std::mutex mu;
+    std::multimap<int64_t, void *> free;     // size -> block
+    std::unordered_map<void *, int64_t> taken;  // block -> size
+    int64_t cached_bytes = 0, used_bytes = 0;
+};
+Cache &cache()
+{
+    static Cache *c = new Cache;  // never destroyed: blocks may outlive static teardown
+    return *c;
+}
+}  // namespace
+
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+const char *type_name(int32_t type)
+{
+    static const char *names[] = {"I8", "I16", "I32", "I64", "U8", "U16", "U32", "U64", "F16", "F32", "F64"};
+    return (type >= 0 && type <= 10) ? names[type] : "unknown";
+}
+
+Scratch::Scratch(int64_t bytes_, hipStream_t stream_) : bytes(bytes_), stream(stream_)
+{
+    if (bytes > 0) SPK_HIP(hipMallocAsync(&ptr, size_t(bytes), stream));
+}
+
+Scratch::~Scratch()
+{
+    if (ptr) (void)hipFreeAsync(ptr, stream);
+}
+}  // namespace spk
+
+using namespace spk;
+
+extern "C" {
+
+const char *spk_last_error(void) { return g_last_error.c_str(); }
+
+int32_t spk_version(void) { return 1 * 10000 + 0 * 100 + 0; }
+
+int32_t spk_device_count(int32_t *count)
+{
+    return guard([&] {
+        int c = 0;
+        hipError_t e = hipGetDeviceCount(&c);
+        *count = (e == hipSuccess) ? c : 0;
+    });
+}
+
+int32_t spk_set_device(int32_t device) { return guard([&] { SPK_HIP(hipSetDevice(device)); }); }
+
+int32_t spk_synchronize(void *stream) { return guard([&] { SPK_HIP(hipStreamSynchronize(S(stream))); }); }
+
+int32_t spk_alloc(void **ptr, int64_t bytes)
+{
+    return guard([&] {
+        SPK_ARG(bytes >= 0, "Allocation size can't be negative.");
+        Cache &c = cache();
+        std::lock_guard<std::mutex> lock(c.mu);
+        auto it = c.free.find(bytes);
+        void *p = nullptr;
+        if (it != c.free.end())
+        {
+            p = it->second;
+            c.free.erase(it);
+        }
+        else
+        {
+            hipError_t e = hipMalloc(&p, size_t(bytes > 0 ? bytes : 1));
+            if (e != hipSuccess)
+            {
+                // Out of memory: release the cache and try once more
+                // (src/spyker/cuda/memory.cu:67-83 frees cached blocks under pressure).
+                (void)hipGetLastError();
+                SPK_HIP(hipDeviceSynchronize());
+                for (auto &kv : c.free) (void)hipFree(kv.second);
+                c.free.clear();
+                c.cached_bytes = 0;
+                SPK_HIP(hipMalloc(&p, size_t(bytes > 0 ? bytes : 1)));
+            }
+            c.cached_bytes += bytes;
+        }
+        c.taken[p] = bytes;
+        c.used_bytes += bytes;
+        *ptr = p;
+    });
+}
+
+int32_t spk_dealloc(void *ptr)
+{
+    return guard([&] {
+        if (!ptr) return;
+        Cache &c = cache();
+        std::lock_guard<std::mutex> lock(c.mu);
+        auto it = c.taken.find(ptr);
+        SPK_ARG(it != c.taken.end(), "Pointer was not allocated by this library.");
+        c.used_bytes -= it->second;
+        c.free.emplace(it->second, ptr);
+        c.taken.erase(it);
+    });
+}
+
+int32_t spk_memory_taken(int64_t *bytes)
+{
+    return guard([&] {
+        Cache &c = cache();
+        std::lock_guard<std::mutex> lock(c.mu);
+        *bytes = c.cached_bytes;
+    });
+}
+
+int32_t spk_memory_used(int64_t *bytes)
+{
+    return guard([&] {
+        Cache &c = cache();
+        std::lock_guard<std::mutex> lock(c.mu);
+        *bytes = c.used_bytes;
+    });
+}
+
+int32_t spk_cache_clear(void)
+{
+    return guard([&] {
+        Cache &c = cache();
+        std::lock_guard<std::mutex> lock(c.mu);
+        SPK_HIP(hipDeviceSynchronize());
+        for (auto &kv : c.free)
+        {
+            SPK_HIP(hipFree(kv.second));
+            c.cached_bytes -= kv.first;
+        }
+        c.free.clear();
+    });
+}
+}

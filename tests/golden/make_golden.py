@@ -1,0 +1,139 @@
+"""Writes tests/golden/reference_vectors.npz from the reference's own CPU code.
+
+Every expected output here comes from oracle/_ref/libspyker_ref.so, which
+oracle/Makefile compiles from the reference's sources (cpu/*.cpp,
+utils.cpp, ...) in this container.  The reference's conv/FC arithmetic
+lives in oneDNN (not available), so the one pipeline vector takes its
+potentials from the C oracle's canonical FP32 conv and every later stage
+from the reference library.
+
+Run:  make -C oracle && python tests/golden/make_golden.py
+Inputs are seeded (PCG64); the file is small enough to commit.
+"""
+
+import math
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from spyker_amd import synthetic  # noqa: E402
+
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_vectors.npz")
+
+
+def main():
+    R, O = oracle.Reference(), oracle.Oracle()
+    rng = np.random.Generator(np.random.PCG64(2024))
+    v = {}
+
+    # rank code: values with zeros, ties and a constant sample
+    x = rng.random((5, 300)).astype(np.float32)
+    x[x < 0.3] = 0
+    x[1, :40] = 0.5  # a tie block
+    x[4] = 0.25      # every value equal (scale = inf)
+    v["code_in"] = x
+    for t in (1, 15, 30):
+        v["code_sort_T%d" % t] = R.rank_code(x, t, True)
+        v["code_latency_T%d" % t] = R.rank_code(x[:4], t, False)
+
+    # threshold / fire / inhibit / convwta on random potentials with ties
+    p = np.floor(rng.random((3, 6, 7, 9, 9)) * 40).astype(np.float32) / 2
+    v["pot"] = p
+    v["threshold_15"] = R.threshold(p, 15.0)
+    v["threshold_15_v3"] = R.threshold(p, 15.0, 3.0)
+    v["fire_15"] = R.fire(p, 15.0)
+    v["fire_0"] = R.fire(p, 0.0)
+    th = R.threshold(p, 15.0)
+    v["inhibit"] = R.inhibit(th)
+    v["inhibit_th5"] = R.inhibit(th, 5.0)
+    inh = v["inhibit"]
+    for name, args in {"r1k4": (1, 4), "r3k5": (3, 5), "r12k3": ((1, 2), 3), "r0k20": (0, 20)}.items():
+        w, c = R.convwta(inh, *args)
+        v["convwta_%s_w" % name], v["convwta_%s_c" % name] = w, c
+    w, c = R.convwta(th, 1, 6, 16.0)
+    v["convwta_th16_w"], v["convwta_th16_c"] = w, c
+    s8 = (rng.random((3, 6, 7, 9, 9)) < 0.05).astype(np.uint8)
+    v["spikes_u8"] = s8
+    w, c = R.convwta(s8, 2, 5)
+    v["convwta_u8_w"], v["convwta_u8_c"] = w, c
+
+    f = np.floor(rng.random((4, 5, 60)) * 40).astype(np.float32) / 2
+    f = R.threshold(f, 12.0)
+    v["fc_pot"] = f
+    w, c = R.fcwta(f, 2, 6)
+    v["fcwta_w"], v["fcwta_c"] = w, c
+
+    # pool
+    for pad in [(0, 0, 0, 0), (1, 1, 1, 1), (1, 0, 2, 1)]:
+        key = "".join(str(q) for q in pad)
+        v["pool_u8_k2s2_p" + key] = R.pool(s8, 2, 2, pad)
+        v["pool_u8_k3s3_p" + key] = R.pool(s8, 3, 3, pad)
+        v["pool_f32_k3s2_p" + key] = R.pool(p - 5.0, 3, 2, pad)
+
+    # conv STDP (plain and R-STDP configurations)
+    n, t, cin, h, wd, cout, k = 6, 5, 3, 9, 9, 4, 3
+    sin = np.maximum.accumulate((rng.random((n, t, cin, h, wd)) < 0.25).astype(np.uint8), axis=1)
+    ker = np.clip(rng.normal(0.8, 0.05, (cout, cin, k, k)), 0, 1).astype(np.float32)
+    pot = O.conv(sin.reshape(n * t, cin, h, wd), ker, 1, (1, 1, 1, 1)).reshape(n, t, cout, h, wd)
+    pot = R.inhibit(R.threshold(pot, 3.0))
+    win, cnt = R.convwta(pot, 1, 3)
+    sout = R.fire(pot)
+    v["stdp_in"], v["stdp_kernel"], v["stdp_out"], v["stdp_win"], v["stdp_cnt"] = sin, ker, sout, win, cnt
+    v["stdp_kernel_after"] = R.convstdp(sin, ker, sout, [(0.004, -0.003)], win, cnt, 1, (1, 1, 1, 1))
+    win2 = win.copy()
+    win2["c"] = rng.integers(-1, 2, win2.shape)
+    v["stdp_win_rstdp"] = win2
+    v["stdp_kernel_after_rstdp"] = R.convstdp(sin, ker, sout, [(0.004, -0.003), (-0.002, 0.005, False, 0.1, 0.9)],
+                                              win2, cnt, 1, (1, 1, 1, 1))
+    v["stdp_kernel_after_s2"] = R.convstdp(sin, ker, sout[:, :, :, :4, :4], [(0.01, -0.02)], _clip(win, 4), cnt, 2,
+                                           (1, 1, 0, 0))
+
+    fin = (rng.random((n, t, 40)) < 0.3).astype(np.uint8)
+    fk = rng.random((10, 40)).astype(np.float32)
+    fp = R.threshold(O.fc(fin, fk), 5.0)
+    fw, fc_ = R.fcwta(fp, 1, 3)
+    fo = R.fire(fp)
+    v["fcstdp_in"], v["fcstdp_kernel"], v["fcstdp_out"], v["fcstdp_win"], v["fcstdp_cnt"] = fin, fk, fo, fw, fc_
+    v["fcstdp_kernel_after"] = R.fcstdp(fin, fk, fo, [(0.004, -0.003)], fw, fc_)
+
+    # host kernel generators
+    for kk, sd in [(3, 1.0), (3, 2.0), (7, 1.0), (7, 2.0)]:
+        v["gauss_%d_%g" % (kk, sd)] = R.gaussian_kernel(kk, sd)
+    for th_deg in (0, 45, 90, 135):
+        for kk, lam in [(7, 4.0), (11, 8.0)]:
+            v["gabor_%d_%d" % (kk, th_deg)] = R.gabor_kernel(kk, 0.56 * lam, math.radians(th_deg), 0.5, lam, 0.0)
+    v["normal_seed7"] = R.normal_kernel(64, 0.8, 0.05, 7)
+
+    # a small end-to-end layer (stroke images -> spikes -> potentials -> STDP)
+    img = synthetic.strokes(4, 28, seed=11).astype(np.float32) / 255
+    spk = (img.reshape(4, 1, 28, 28) > 0.2).astype(np.uint8)
+    spk = np.repeat(spk[:, None], 6, axis=1)  # [4, 6, 1, 28, 28], cumulative
+    spk[:, :2] = 0
+    kern = np.clip(np.random.Generator(np.random.PCG64(5)).normal(0.8, 0.05, (8, 1, 5, 5)), 0, 1).astype(np.float32)
+    pot = O.conv(spk.reshape(24, 1, 28, 28), kern, 1, (2, 2, 2, 2)).reshape(4, 6, 8, 28, 28)
+    pot = R.inhibit(R.threshold(pot, 8.0))
+    w, c = R.convwta(pot, 2, 5)
+    so = R.fire(pot)
+    v["layer_in"], v["layer_kernel"], v["layer_pot"] = spk, kern, pot
+    v["layer_win"], v["layer_cnt"], v["layer_out"] = w, c, so
+    v["layer_kernel_after"] = R.convstdp(spk, kern, so, [(0.004, -0.003)], w, c, 1, (2, 2, 2, 2))
+    v["layer_pool"] = R.pool(so, 2, 2, (0, 0, 0, 0))
+
+    np.savez_compressed(OUT, **v)
+    print("wrote", OUT, os.path.getsize(OUT), "bytes,", len(v), "arrays")
+
+
+def _clip(win, lim):
+    w = win.copy()
+    w["y"] = np.minimum(w["y"], lim - 1)
+    w["x"] = np.minimum(w["x"], lim - 1)
+    return w
+
+
+if __name__ == "__main__":
+    main()
