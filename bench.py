@@ -12,13 +12,14 @@ layer 2 on [1024, 15, 30, 14, 14] spikes:
 A step is one pass of that over the whole batch; value = images*timesteps/s
 = 1024*15 / step time.  Two implementations run on the same inputs:
 
-    plain  the FP32 per-stage kernels (bit-exact to the oracle)
-    fused  the FP16-MFMA fused layer kernel (when built)
+    fused  the FP16-MFMA fused layer step (spk_stdp_layer_fused) -> `value`
+    plain  the FP32 per-stage kernels, bit-exact to the oracle -> `plain`
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--path fused|plain]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -30,7 +31,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 N, T_STEPS = 1024, 15
-CIN, COUT, K, TH, RADIUS, COUNT = 30, 250, 3, 10.0, 1, 8
+CIN, COUT, K, H, TH, RADIUS, COUNT = 30, 250, 3, 14, 10.0, 1, 8
+FLOP_PER_IT = 2.0 * COUT * H * H * CIN * K * K          # dense time-unrolled conv, per image*timestep
+BYTES_PER_IT = CIN * H * H + COUT * H * H + COUT * 4 * 4  # U8 spikes in, spikes out, pooled out
 PEAK_FP32 = 157.3e12
 PEAK_FP16 = 2.5e15
 PEAK_HBM = 8.0e12
@@ -63,7 +66,58 @@ def plain_step(spk, x, conv):
     win = spk.convwta(pot, RADIUS, COUNT)
     out = spk.fire(pot)
     conv.stdp(x, win, out)
-    return spk.pool(out, 3, 3), win
+    return spk.pool(out, 3, 3), win, out
+
+
+def fused_step(spk, x, conv):
+    out, win, pooled = spk.layer_step(conv, x, TH, RADIUS, COUNT, 3, 3)
+    return pooled, win, out
+
+
+def timed(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def precision(spk, x, k2, dev, samples=64):
+    """FP16 path vs the FP64 check conv and vs the FP32 plain step."""
+    import torch
+    from spyker_amd import _lib
+    xs = x[:samples].contiguous()
+    n = xs.shape[0]
+    kd = torch.from_numpy(k2).to(dev)
+    p64 = torch.empty((n, T_STEPS, COUT, H, H), dtype=torch.float64, device=dev)
+    spk.call("spk_conv", spk._ptr(xs), _lib.U8, spk._ptr(kd), _lib.F32, spk._ptr(p64), _lib.F64, n * T_STEPS, CIN,
+             H, H, COUT, K, K, 1, 1, 1, 1, 1, 1, spk._stream(xs))
+    conv = spk.Conv(CIN, COUT, K, pad=1, device=dev)
+    conv.kernel = kd.clone()
+    conv.stdpconfig = [spk.STDPConfig(0.004, -0.003)]
+    p16 = torch.empty((n, T_STEPS, COUT, H, H), dtype=torch.float32, device=dev)
+    out_f, win_f, _ = spk.layer_step(conv, xs, TH, RADIUS, COUNT, 3, 3, potentials=p16)
+    conv_p = spk.Conv(CIN, COUT, K, pad=1, device=dev)
+    conv_p.kernel = kd.clone()
+    conv_p.stdpconfig = [spk.STDPConfig(0.004, -0.003)]
+    _, win_p, out_p = plain_step(spk, xs, conv_p)
+    torch.cuda.synchronize()
+    flips = ((p16.double() > TH) != (p64 > TH)).double().mean().item()
+    wf, wp = win_f.data[:, :, 2:].cpu(), win_p.data[:, :, 2:].cpu()
+    cf, cp = win_f.counts.cpu(), win_p.counts.cpu()
+    same = sum(int(cf[i] == cp[i] and bool((wf[i, :cf[i]] == wp[i, :cp[i]]).all())) for i in range(n))
+    return {
+        "samples": n,
+        "threshold_flip_rate_vs_fp64": flips,
+        "max_abs_potential_error_vs_fp64": (p16.double() - p64).abs().max().item(),
+        "spike_mismatch_rate_vs_fp32_plain": (out_f != out_p).double().mean().item(),
+        "samples_with_identical_winners_vs_fp32_plain": same,
+        "max_abs_weight_diff_vs_fp32_plain_after_step": (conv.kernel - conv_p.kernel).abs().max().item(),
+    }
 
 
 def cpu_baseline(x_host, k2, samples):
@@ -74,8 +128,8 @@ def cpu_baseline(x_host, k2, samples):
     x = np.ascontiguousarray(x_host[:samples])
     n = x.shape[0]
     t0 = time.perf_counter()
-    pot = orc.conv(x.reshape(n * T_STEPS, CIN, 14, 14), k2, 1, (1, 1, 1, 1), threads=threads)
-    pot = pot.reshape(n, T_STEPS, COUT, 14, 14)
+    pot = orc.conv(x.reshape(n * T_STEPS, CIN, H, H), k2, 1, (1, 1, 1, 1), threads=threads)
+    pot = pot.reshape(n, T_STEPS, COUT, H, H)
     pot = orc.threshold(pot, TH)
     pot = orc.inhibit(pot)
     w, c = orc.convwta(pot, RADIUS, COUNT)
@@ -84,17 +138,19 @@ def cpu_baseline(x_host, k2, samples):
     orc.pool(out, 3, 3)
     dt = time.perf_counter() - t0
     return {"value": n * T_STEPS / dt, "unit": "images*timesteps/s", "cores": threads, "kind": "port",
-            "sample": "%d images x %d steps of the same layer step (oracle C port, OpenMP)" % (n, T_STEPS)}
+            "sample": "%d images x %d steps of the same layer step (oracle C port, OpenMP, %d threads)"
+                      % (n, T_STEPS, threads)}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--path", choices=["fused", "plain"], default="plain")
-    ap.add_argument("--cpu-samples", type=int, default=32)
+    ap.add_argument("--plain-steps", type=int, default=5)
+    ap.add_argument("--cpu-samples", type=int, default=192)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the layer path has no multi-GPU form in this build "
@@ -106,30 +162,20 @@ def main():
 
     dev = torch.device("cuda", 0)
     x, k2 = make_inputs(dev)
-    conv = spk.Conv(CIN, COUT, K, pad=1, device=dev)
-    conv.kernel = torch.from_numpy(k2).to(dev)
-    conv.stdpconfig = [spk.STDPConfig(0.004, -0.003)]
 
+    conv_f = spk.Conv(CIN, COUT, K, pad=1, device=dev)
+    conv_f.kernel = torch.from_numpy(k2).to(dev)
+    conv_f.stdpconfig = [spk.STDPConfig(0.004, -0.003)]
     for _ in range(a.warmup):
-        plain_step(spk, x, conv)
+        fused_step(spk, x, conv_f)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        plain_step(spk, x, conv)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / a.steps
-
-    # dominant kernel (the FP32 direct conv), HIP events on the launch stream
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    reps = 5
-    ev[0].record()
-    for _ in range(reps):
-        conv(x)
-    ev[1].record()
-    torch.cuda.synchronize()
-    conv_s = ev[0].elapsed_time(ev[1]) / 1e3 / reps
-    flop = 2.0 * N * T_STEPS * COUT * 14 * 14 * CIN * K * K
-    achieved = flop / conv_s
+    spk.call("spk_kernel_timer", 1)
+    dt = timed(lambda: fused_step(spk, x, conv_f), a.steps, 0)
+    total_ms, launches = C.c_double(0), C.c_int64(0)
+    spk.call("spk_kernel_timer_read", C.byref(total_ms), C.byref(launches))
+    spk.call("spk_kernel_timer", 0)
+    kern_s = total_ms.value / 1e3 / max(launches.value, 1)
+    achieved = FLOP_PER_IT * N * T_STEPS / kern_s
 
     line = {
         "metric": "images*timesteps/s, conv+fire+wta+STDP layer, 1 GPU",
@@ -142,15 +188,24 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f16-mfma/f32-acc",
         "data": "synthetic",
-        "config": {"workload": "layer 2 of the two-layer stroke-image STDP network: batch 1024, 28x28 input, "
-                               "T=15, conv 30->250 3x3 pad 1 on 14x14, th 10, convwta r1 k8, pool 3x3",
-                   "path": "plain-fp32"},
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32, "traffic": None, "kernel": "conv_direct",
-                     "kernel_ms": conv_s * 1e3},
+        "config": {"workload": "layer 2 of the two-layer stroke-image STDP network: batch 1024 of 28x28 images, "
+                               "T=15, conv 30->250 3x3 pad 1 on 14x14 spikes, threshold 10, inhibit, convwta r1 k8, "
+                               "STDP, pool 3x3 s3",
+                   "path": "fused FP16-MFMA layer step, FP32 master weights"},
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP16 / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP16, "traffic": None, "kernel": "fused_layer",
+                     "kernel_ms": kern_s * 1e3, "launches_timed": launches.value},
     }
+    if not a.no_plain:
+        conv_p = spk.Conv(CIN, COUT, K, pad=1, device=dev)
+        conv_p.kernel = torch.from_numpy(k2).to(dev)
+        conv_p.stdpconfig = [spk.STDPConfig(0.004, -0.003)]
+        dtp = timed(lambda: plain_step(spk, x, conv_p), a.plain_steps, 1)
+        line["plain"] = {"value": N * T_STEPS / dtp, "ms_per_step": dtp * 1e3, "dtype": "f32",
+                         "speedup_fused_over_plain": dtp / dt}
+    line["precision"] = precision(spk, x, k2, dev)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(x.cpu().numpy(), k2, a.cpu_samples)
     print(json.dumps(line))

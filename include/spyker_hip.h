@@ -200,6 +200,45 @@ int32_t spk_rank_fcstdp(const void *in, void *kernel, int32_t kernel_type, const
                         int64_t fin, int64_t fout, const spk_stdp_config *configs, int64_t nconfigs,
                         const spk_winner *winners, const int64_t *counts, int64_t maxcount, void *stream);
 
+/* -- fused layer step (FP16 MFMA) ------------------------------------------ */
+
+/* One training step of a conv layer as the reference runs it in separate
+ * calls (examples/kheradpisheh_mnist_dense.py:62-64):
+ *
+ *     pot = inhibit(threshold(conv(in), threshold))    opers.cpp:408,472,639
+ *     winners = convwta(pot, (ry, rx), count)          opers.cpp:656
+ *     out = fire(pot)                                  opers.cpp:522
+ *     Conv::stdp(in, winners, out)                     opers.cpp:873
+ *     pooled = pool(out, (pkh, pkw), (psh, psw))       opers.cpp:600
+ *
+ * with the inhibit/convwta/fire thresholds at their default 0, stride 1 and
+ * threshold >= 0.  The conv runs on FP16 MFMA (weights = the FP32 masters
+ * rounded to FP16, spikes exact, FP32 accumulation); the STDP update is
+ * applied to the FP32 masters in `kernel` with the reference's order and
+ * arithmetic.  in: U8 [n][t][cin][h][w]; kernel: F32 [cout][cin][kh][kw]
+ * (updated in place when nconfigs > 0); out: U8 [n][t][cout][ho][wo] or NULL
+ * (required for learning); pooled: U8 [n][t][cout][pho][pwo] or NULL;
+ * winners/counts: device [n][count], [n]; potentials: NULL, or F32
+ * [n][t][cout][ho][wo] receiving the raw FP16-path potentials (a check
+ * output).  Limits: t <= 32, 2 <= cout <= 256, cin*kh*kw <= 512 after
+ * padding cin to 8, one sample's input staged in 160 KiB of LDS, ho*wo <=
+ * 8192.  No counterpart in the reference as one call. */
+int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, int64_t h, int64_t w, void *kernel,
+                             int64_t cout, int64_t kh, int64_t kw, int64_t pt, int64_t pl, int64_t pb, int64_t pr,
+                             double threshold, int64_t ry, int64_t rx, int64_t count,
+                             const spk_stdp_config *configs, int64_t nconfigs, void *out, void *pooled, int64_t pkh,
+                             int64_t pkw, int64_t psh, int64_t psw, spk_winner *winners, int64_t *counts,
+                             void *potentials, void *stream);
+
+/* HIP-event timing of the main kernel of spk_stdp_layer_fused, recorded on
+ * the launch stream around each launch while enabled (no reference
+ * counterpart; the reference's unused Timer is src/spyker/base.h:80-95).
+ * spk_kernel_timer(1) clears and starts, spk_kernel_timer(0) clears and
+ * stops; spk_kernel_timer_read waits for the recorded launches and returns
+ * their summed duration and count. */
+int32_t spk_kernel_timer(int32_t enable);
+int32_t spk_kernel_timer_read(double *total_ms, int64_t *launches);
+
 /* -- host kernel generators (F64, host memory) ---------------------------- */
 
 /* Core::random_seed (src/spyker/impl.cpp:43): reseeds the std::mt19937 that

@@ -30,6 +30,7 @@ from ._lib import call
 __all__ = [
     "DoGFilter", "GaborFilter", "STDPConfig", "Winner", "Winners", "DoG", "Gabor", "Conv", "FC", "threshold",
     "fire", "code", "pool", "inhibit", "convwta", "fcwta", "conv", "fc", "pad", "random_seed", "expand2", "expand4",
+    "layer_step",
 ]
 
 _TYPES = {torch.uint8: _lib.U8, torch.float16: _lib.F16, torch.float32: _lib.F32, torch.float64: _lib.F64,
@@ -490,6 +491,55 @@ class Conv:
         call("spk_rank_convstdp", _ptr(a), _ptr(self.kernel), _type(self.kernel), _ptr(o), n, t, cin, h, w, cout, kh,
              kw, ho, wo, cfg, ncfg, _ptr(wn.data), _ptr(wn.counts), wn.data.shape[1], sy, sx, pt, pl, pb, pr,
              _stream(a))
+
+
+def layer_step(conv: "Conv", array: torch.Tensor, threshold: float, radius, count: int, pool_kernel=None,
+               pool_stride=None, learn: bool = True, potentials: torch.Tensor = None):
+    """One fused training step of a conv layer on FP16 MFMA:
+
+        pot = inhibit(threshold(conv(array), threshold)); winners = convwta(pot, radius, count)
+        out = fire(pot); conv.stdp(array, winners, out); pooled = pool(out, pool_kernel, pool_stride)
+
+    in a handful of launches that never materialise the FP32 potentials.
+    Returns (out, winners, pooled-or-None).  ``potentials``, if given, is an
+    F32 tensor shaped like ``out`` that receives the raw FP16-path potentials.
+    """
+    _dev(array)
+    a = _exact(array, 5)
+    if a.dtype != torch.uint8:
+        raise RuntimeError("Fused layer: spikes must be u8.")
+    if conv.kernel.dtype != torch.float32:
+        raise RuntimeError("Fused layer: the master weights must be f32.")
+    if conv._stride != (1, 1):
+        raise RuntimeError("Fused layer: stride must be 1.")
+    n, t, cin, h, w = a.shape
+    cout, _, kh, kw = conv.kernel.shape
+    pt, pl, pb, pr = conv._pad
+    ho, wo = h + pt + pb - kh + 1, w + pl + pr - kw + 1
+    ry, rx = expand2(radius)
+    out = torch.empty((n, t, cout, ho, wo), dtype=torch.uint8, device=a.device)
+    pooled = None
+    pkh = pkw = psh = psw = 0
+    if pool_kernel is not None:
+        pkh, pkw = expand2(pool_kernel)
+        psh, psw = expand2(pool_stride if pool_stride is not None else pool_kernel)
+        pooled = torch.empty((n, t, cout, (ho - pkh) // psh + 1, (wo - pkw) // psw + 1), dtype=torch.uint8,
+                             device=a.device)
+    data = torch.zeros((n, count, 5), dtype=torch.int64, device=a.device)
+    cnt = torch.zeros((n,), dtype=torch.int64, device=a.device)
+    if learn:
+        cfg, ncfg = _configs(conv.stdpconfig)
+    else:
+        cfg, ncfg = None, 0
+    if potentials is not None:
+        _dev(potentials)
+        if potentials.dtype != torch.float32 or tuple(potentials.shape) != tuple(out.shape):
+            raise RuntimeError("potentials must be f32 shaped like the output")
+    call("spk_stdp_layer_fused", _ptr(a), n, t, cin, h, w, _ptr(conv.kernel), cout, kh, kw, pt, pl, pb, pr,
+         float(threshold), ry, rx, int(count), cfg, ncfg, _ptr(out),
+         _ptr(pooled) if pooled is not None else None, pkh, pkw, psh, psw, _ptr(data), _ptr(cnt),
+         _ptr(potentials) if potentials is not None else None, _stream(a))
+    return out, Winners(data, cnt), pooled
 
 
 class FC:

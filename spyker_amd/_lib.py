@@ -51,6 +51,10 @@ SIGNATURES = {
     "spk_rank_code": [_p, _i32, _p, _i32, _i64, _i64, _i64, _i32, _p],
     "spk_rank_convstdp": [_p, _p, _i32, _p] + [_i64] * 10 + [_p, _i64, _p, _p, _i64] + [_i64] * 6 + [_p],
     "spk_rank_fcstdp": [_p, _p, _i32, _p, _i64, _i64, _i64, _i64, _p, _i64, _p, _p, _i64, _p],
+    "spk_stdp_layer_fused": [_p] + [_i64] * 5 + [_p] + [_i64] * 7 + [_f64] + [_i64] * 3 + [_p, _i64, _p, _p]
+                            + [_i64] * 4 + [_p, _p, _p, _p],
+    "spk_kernel_timer": [_i32],
+    "spk_kernel_timer_read": [_p, _p],
     "spk_random_seed": [_i64],
     "spk_normal_kernel": [_p, _i64, _f64, _f64],
     "spk_gaussian_kernel": [_p, _i64, _f64],

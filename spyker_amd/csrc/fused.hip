@@ -1,0 +1,529 @@
+// Fused FP16-MFMA layer step: conv -> threshold -> inhibit -> fire summary,
+// then convwta, spike/pool writer and STDP on the FP32 master weights.
+//
+// Semantics are the composition the reference runs as separate calls
+// (src/spyker/opers.cpp:408-423 conv, 472-478 threshold, 639-646 inhibit,
+// 656-662 convwta, 522-552 fire, 600-637 pool, 679-693 STDP) with the
+// inhibit/convwta/fire thresholds at their default 0, a conv threshold
+// th >= 0 and replacement value 0.  Under those conditions every value after
+// the threshold is 0 or > th >= 0, so for each sample and output pixel the
+// whole post-inhibit tensor is described by
+//
+//     c*     the kept channel (inhibit.cpp:64-85: first channel maximum at
+//            step T - max(cnt, 1), cnt = #steps whose maximum is > 0)
+//     mask   bit t set iff the kept channel's value at t is > 0 (= fire)
+//     first  the value at the first set bit (convwta's t0 value)
+//     max    the largest value over t (for convwta's per-sample max)
+//
+// and every other channel is 0 there.  The conv is the dense time-unrolled
+// contraction: per sample and step a [Cout x K] x [K x pixels] product with
+// K = taps * Cin, on v_mfma_f32_16x16x32_f16 (spikes are exact in FP16; the
+// weights are the FP32 masters rounded to FP16; accumulation is FP32).
+//
+// Layout of the main kernel (one workgroup of NW waves per CU, persistent
+// over samples):
+//   - wave w owns output channels [32w, 32w+32) as two 16-row MFMA tiles;
+//     its FP16 weight fragments for all K stay in VGPRs for the whole kernel;
+//   - the sample's input for all T steps is staged once in LDS as
+//     [t][padded y][padded x][cin_pad] bytes (channel innermost), so one
+//     MFMA B fragment -- 8 consecutive input channels at one tap -- is one
+//     ds_read_b64, widened to FP16 exactly (0x6400|b minus 1024);
+//   - columns are 16 output pixels; for each column tile the kernel runs
+//     t = 0..T-1, keeping the per-(channel, pixel) summaries in registers, and
+//     reduces the per-pixel channel argmax across lanes (shuffles) and waves
+//     (LDS, double-buffered by step parity: one barrier per step).
+#include <vector>
+
+#include "stdp.hpp"
+
+namespace spk
+{
+namespace
+{
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+typedef float float4v __attribute__((ext_vector_type(4)));
+
+constexpr int MAX_STEPS = 16;  // K-steps of 32 -> K <= 512
+constexpr int MAX_WAVES = 8;   // Cout <= 256
+constexpr int COLS = 16;
+
+struct Summary
+{
+    int32_t c;
+    uint32_t mask;
+    float first, max;
+};
+
+struct FusedGeom
+{
+    int32_t n, t, cin, h, w, cout, kh, kw, pt, pl, ho, wo;
+    int32_t ph, pw, cin_pad, groups, steps, waves, tiles;
+    int32_t plane;  // bytes of one staged step
+    float th;
+};
+
+// [row tile][step][lane][8] FP16 fragments of the FP32 masters.
+__global__ void prep_fragments(const float *__restrict__ k, half8 *__restrict__ frag, FusedGeom g)
+{
+    const int32_t rows = g.waves * 2, total = rows * g.steps * 64;
+    for (int32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x)
+    {
+        const int32_t lane = e % 64, s = (e / 64) % g.steps, rt = e / (64 * g.steps);
+        const int32_t o = rt * 16 + lane % 16, grp = 4 * s + lane / 16;
+        const int32_t cg = g.cin_pad / 8, tap = grp / cg, c8 = grp % cg;
+        half8 v;
+        for (int j = 0; j < 8; ++j)
+        {
+            const int32_t c = c8 * 8 + j;
+            float x = 0.0f;
+            if (o < g.cout && grp < g.groups && c < g.cin)
+                x = k[((int64_t(o) * g.cin + c) * g.kh + tap / g.kw) * g.kw + tap % g.kw];
+            v[j] = _Float16(x);
+        }
+        frag[e] = v;
+    }
+}
+
+__device__ __forceinline__ half8 widen(uint2 raw)
+{
+    // bytes -> FP16 exactly: 0x6400 | b is 1024 + b, minus 1024
+    const half2v bias = {_Float16(1024.0f), _Float16(1024.0f)};
+    uint32_t w[4];
+    w[0] = (raw.x & 0xFFu) | ((raw.x & 0xFF00u) << 8);
+    w[1] = ((raw.x >> 16) & 0xFFu) | ((raw.x >> 8) & 0xFF0000u);
+    w[2] = (raw.y & 0xFFu) | ((raw.y & 0xFF00u) << 8);
+    w[3] = ((raw.y >> 16) & 0xFFu) | ((raw.y >> 8) & 0xFF0000u);
+    half8 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+    {
+        half2v h = __builtin_bit_cast(half2v, w[i] | 0x64006400u) - bias;
+        r[2 * i] = h[0], r[2 * i + 1] = h[1];
+    }
+    return r;
+}
+
+__device__ __forceinline__ void combine(float &v, int32_t &i, float ov, int32_t oi)
+{
+    if (ov > v || (ov == v && oi < i)) v = ov, i = oi;
+}
+
+template <int S>
+__global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ in, const half8 *__restrict__ frag,
+                                                   Summary *__restrict__ summary, float *__restrict__ pot_out,
+                                                   FusedGeom g)
+{
+    extern __shared__ __align__(16) uint8_t lds[];
+    uint8_t *stage = lds;
+    float *red_v = reinterpret_cast<float *>(lds + g.t * g.plane);  // [2][MAX_WAVES][COLS]
+    int32_t *red_i = reinterpret_cast<int32_t *>(red_v + 2 * MAX_WAVES * COLS);
+    int32_t *cstar = red_i + 2 * MAX_WAVES * COLS;                     // [COLS]
+    uint16_t *ch_tab = reinterpret_cast<uint16_t *>(cstar + COLS);     // [t][COLS]
+
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, q = lane >> 4, j = lane & 15;
+    const int32_t hw = g.ho * g.wo;
+
+    // weight fragments and tap offsets, resident for the whole kernel
+    half8 a[2][S];
+    int32_t offs[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+    {
+        a[0][s] = frag[((w * 2 + 0) * S + s) * 64 + lane];
+        a[1][s] = frag[((w * 2 + 1) * S + s) * 64 + lane];
+        const int32_t grp = 4 * s + q, cg = g.cin_pad / 8;
+        const int32_t tap = grp < g.groups ? grp / cg : 0, c8 = grp < g.groups ? grp % cg : 0;
+        offs[s] = ((tap / g.kw) * g.pw + tap % g.kw) * g.cin_pad + c8 * 8;
+    }
+
+    for (int32_t n = blockIdx.x; n < g.n; n += gridDim.x)
+    {
+        __syncthreads();
+        // stage [t][ph][pw][cin_pad]: zero everything, then copy the image
+        {
+            uint4 *z = reinterpret_cast<uint4 *>(stage);
+            const int32_t words = g.t * g.plane / 16;
+            for (int32_t i = threadIdx.x; i < words; i += blockDim.x) z[i] = make_uint4(0, 0, 0, 0);
+        }
+        __syncthreads();
+        {
+            const uint8_t *src = in + int64_t(n) * g.t * g.cin * g.h * g.w;
+            const int32_t c4 = (g.cin + 3) / 4, per_t = c4 * g.h * g.w;
+            for (int32_t i = threadIdx.x; i < g.t * per_t; i += blockDim.x)
+            {
+                const int32_t tt = i / per_t, r = i % per_t, cq = r / (g.h * g.w), pix = r % (g.h * g.w);
+                const int32_t y = pix / g.w, x = pix % g.w;
+                const uint8_t *s0 = src + (int64_t(tt) * g.cin + cq * 4) * g.h * g.w + pix;
+                uint32_t v = 0;
+                for (int b = 0; b < 4; ++b)
+                    if (cq * 4 + b < g.cin) v |= uint32_t(s0[int64_t(b) * g.h * g.w]) << (8 * b);
+                *reinterpret_cast<uint32_t *>(stage + tt * g.plane + ((y + g.pt) * g.pw + x + g.pl) * g.cin_pad +
+                                              cq * 4) = v;
+            }
+        }
+        __syncthreads();
+
+        for (int32_t tile = 0; tile < g.tiles; ++tile)
+        {
+            const int32_t p = tile * COLS + j;
+            const bool pvalid = p < hw;
+            const int32_t pc = pvalid ? p : hw - 1, y = pc / g.wo, x = pc % g.wo;
+            const int32_t base = (y * g.pw + x) * g.cin_pad;
+            uint32_t mask[2][4];
+            float first[2][4], mx[2][4];
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) mask[r][i] = 0, first[r][i] = 0.0f, mx[r][i] = 0.0f;
+            int32_t cnt = 0;
+
+            for (int32_t tt = 0; tt < g.t; ++tt)
+            {
+                float4v acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+                const uint8_t *sb = stage + tt * g.plane + base;
+#pragma unroll
+                for (int s = 0; s < S; ++s)
+                {
+                    const half8 b = widen(*reinterpret_cast<const uint2 *>(sb + offs[s]));
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0][s], b, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1][s], b, acc1, 0, 0, 0);
+                }
+                // epilogue: threshold, summaries, first channel maximum
+                float bv = -INFINITY;
+                int32_t bi = INT32_MAX;
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                    {
+                        const int32_t o = (w * 2 + r) * 16 + 4 * q + i;
+                        const float v = r == 0 ? acc0[i] : acc1[i];
+                        const float xp = v > g.th ? v : 0.0f;
+                        if (xp > 0.0f)
+                        {
+                            if (mask[r][i] == 0) first[r][i] = xp;
+                            mask[r][i] |= 1u << tt;
+                            mx[r][i] = mx[r][i] < xp ? xp : mx[r][i];
+                        }
+                        if (o < g.cout) combine(bv, bi, xp, o);
+                        if (pot_out && pvalid && o < g.cout)
+                            pot_out[((int64_t(n) * g.t + tt) * g.cout + o) * hw + p] = v;
+                    }
+                combine(bv, bi, __shfl_xor(bv, 16, 64), __shfl_xor(bi, 16, 64));
+                combine(bv, bi, __shfl_xor(bv, 32, 64), __shfl_xor(bi, 32, 64));
+                const int par = tt & 1;
+                if (lane < COLS) red_v[(par * MAX_WAVES + w) * COLS + lane] = bv, red_i[(par * MAX_WAVES + w) * COLS + lane] = bi;
+                __syncthreads();
+                if (w == 0 && lane < COLS)
+                {
+                    float m = red_v[(par * MAX_WAVES) * COLS + lane];
+                    int32_t c = red_i[(par * MAX_WAVES) * COLS + lane];
+                    for (int ow = 1; ow < g.waves; ++ow)
+                        combine(m, c, red_v[(par * MAX_WAVES + ow) * COLS + lane],
+                                red_i[(par * MAX_WAVES + ow) * COLS + lane]);
+                    ch_tab[tt * COLS + lane] = uint16_t(c);
+                    cnt += m > 0.0f;
+                }
+            }
+            if (w == 0 && lane < COLS) cstar[lane] = ch_tab[(g.t - (cnt > 1 ? cnt : 1)) * COLS + lane];
+            __syncthreads();
+            const int32_t cs = cstar[j];
+            if (pvalid)
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        if ((w * 2 + r) * 16 + 4 * q + i == cs)
+                            summary[int64_t(n) * hw + p] = Summary{cs, mask[r][i], first[r][i], mx[r][i]};
+        }
+    }
+}
+
+// convwta over the summaries: candidates are the kept (channel, pixel) pairs;
+// value (T-1-t0) + first/diff with min 0 and diff = 2*max (wta.cpp:49-64);
+// k rounds of first maximum in flat (c, y, x) order with box and channel
+// masking (wta.cpp:128-150).  One block per sample; candidates in LDS.
+__global__ void __launch_bounds__(256) fused_select(const Summary *__restrict__ summary, int32_t t, int32_t ho,
+                                                    int32_t wo, int32_t ry, int32_t rx, int32_t count,
+                                                    spk_winner *__restrict__ win, int64_t *__restrict__ counts)
+{
+    extern __shared__ __align__(16) uint8_t lds[];
+    const int32_t hw = ho * wo;
+    float *val = reinterpret_cast<float *>(lds);
+    int32_t *chan = reinterpret_cast<int32_t *>(val + hw);
+    __shared__ float mm[32];
+    __shared__ ArgMax am[16];
+    const int64_t s = blockIdx.x;
+    const Summary *sm = summary + s * hw;
+    // per-sample maximum of the inhibited potentials
+    float m = 0.0f;
+    for (int32_t p = threadIdx.x; p < hw; p += blockDim.x) m = m < sm[p].max ? sm[p].max : m;
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) mm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    m = mm[0];
+    for (int i = 1; i < int(blockDim.x >> 6); ++i) m = m < mm[i] ? mm[i] : m;
+    const float diff = (m - 0.0f) * 2.0f;
+    for (int32_t p = threadIdx.x; p < hw; p += blockDim.x)
+    {
+        const Summary e = sm[p];
+        chan[p] = e.c;
+        val[p] = e.mask ? float(t - int32_t(__builtin_ctz(e.mask)) - 1) + (e.first - 0.0f) / diff : -FLT_MAX;
+    }
+    __syncthreads();
+    int32_t k = 0;
+    for (; k < count; ++k)
+    {
+        ArgMax a{-INFINITY, INT32_MAX};
+        for (int32_t p = threadIdx.x; p < hw; p += blockDim.x)
+            if (better(val[p], chan[p] * hw + p, a.v, a.i)) a.v = val[p], a.i = chan[p] * hw + p;
+        a = block_argmax(a, am);
+        if (fabsf(a.v - (-FLT_MAX)) < 1e-5f) break;
+        const int32_t z = a.i / hw, y = (a.i % hw) / wo, x = (a.i % hw) % wo;
+        if (threadIdx.x == 0) win[s * count + k] = spk_winner{0, 0, z, y, x};
+        for (int32_t p = threadIdx.x; p < hw; p += blockDim.x)
+        {
+            const int32_t py = p / wo, px = p % wo;
+            if (chan[p] == z || (abs(py - y) <= ry && abs(px - x) <= rx)) val[p] = -FLT_MAX;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) counts[s] = k;
+}
+
+// Spikes [n][t][cout][ho][wo] and (optionally) their max-pool, from the
+// summaries.  One block per (sample, step); each thread writes 16 bytes.
+__global__ void __launch_bounds__(256) fused_write(const Summary *__restrict__ summary, uint8_t *__restrict__ out,
+                                                   uint8_t *__restrict__ pooled, int32_t t, int32_t cout, int32_t ho,
+                                                   int32_t wo, int32_t pkh, int32_t pkw, int32_t psh, int32_t psw,
+                                                   int32_t pho, int32_t pwo)
+{
+    extern __shared__ __align__(16) uint8_t lds[];
+    int16_t *act = reinterpret_cast<int16_t *>(lds);  // active channel per pixel at this step, or -1
+    const int32_t hw = ho * wo;
+    const int64_t ns = blockIdx.x / t;
+    const int32_t step = blockIdx.x % t;
+    for (int32_t p = threadIdx.x; p < hw; p += blockDim.x)
+    {
+        const Summary e = summary[ns * hw + p];
+        act[p] = (e.mask >> step) & 1u ? int16_t(e.c) : int16_t(-1);
+    }
+    __syncthreads();
+    if (out)
+    {
+        // bytes [g0, g0 + total) of out, walked in 16-byte aligned chunks
+        const int64_t total = int64_t(cout) * hw;
+        const int64_t g0 = int64_t(blockIdx.x) * total;
+        const int64_t head = int64_t(reinterpret_cast<uintptr_t>(out + g0) & 15);
+        for (int64_t c0 = int64_t(threadIdx.x) * 16 - head; c0 < total; c0 += int64_t(blockDim.x) * 16)
+        {
+            uint8_t v[16];
+            const int64_t start = c0 < 0 ? 0 : c0;
+            int32_t c = int32_t(start / hw), p = int32_t(start % hw);
+#pragma unroll
+            for (int b = 0; b < 16; ++b)
+            {
+                const int64_t i = c0 + b;
+                v[b] = (i >= 0 && i < total && act[p] == c) ? 1 : 0;
+                if (i >= 0 && ++p == hw) p = 0, ++c;
+            }
+            uint8_t *d = out + g0 + c0;
+            if (c0 >= 0 && c0 + 16 <= total)
+                *reinterpret_cast<uint4 *>(d) = *reinterpret_cast<uint4 *>(v);
+            else
+                for (int b = 0; b < 16; ++b)
+                    if (c0 + b >= 0 && c0 + b < total) d[b] = v[b];
+        }
+    }
+    if (pooled)
+    {
+        const int32_t phw = pho * pwo, total = cout * phw;
+        uint8_t *dst = pooled + int64_t(blockIdx.x) * total;
+        for (int32_t i = threadIdx.x; i < total; i += blockDim.x)
+        {
+            const int32_t c = i / phw, r = i % phw, py = r / pwo, px = r % pwo;
+            uint8_t m = 0;
+            for (int32_t a = 0; a < pkh; ++a)
+                for (int32_t b = 0; b < pkw; ++b)
+                {
+                    const int32_t y = py * psh + a, x = px * psw + b;
+                    if (y < ho && x < wo && act[y * wo + x] == c) m = 1;
+                }
+            dst[i] = m;
+        }
+    }
+}
+
+__global__ void clear_flag(int32_t *flag) { *flag = 0; }
+
+// Optional HIP-event timing of the main fused kernel (spk_kernel_timer).
+struct KernelTimer
+{
+    bool on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+};
+KernelTimer &timer()
+{
+    static KernelTimer *t = new KernelTimer;
+    return *t;
+}
+
+template <int S>
+void launch_layer(const uint8_t *in, const half8 *frag, Summary *summary, float *pot, const FusedGeom &g, size_t lds,
+                  hipStream_t s)
+{
+    static bool configured = false;  // per instantiation; the attribute is per function
+    if (!configured)
+    {
+        SPK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&fused_layer<S>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(lds > 65536 ? 163840 : 65536)));
+        configured = true;
+    }
+    int dev = 0, cus = 0;
+    SPK_HIP(hipGetDevice(&dev));
+    SPK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int grid = g.n < cus ? g.n : cus;
+    KernelTimer &tm = timer();
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (tm.on)
+    {
+        SPK_HIP(hipEventCreate(&e0));
+        SPK_HIP(hipEventCreate(&e1));
+        SPK_HIP(hipEventRecord(e0, s));
+    }
+    fused_layer<S><<<dim3(grid), dim3(64 * g.waves), lds, s>>>(in, frag, summary, pot, g);
+    SPK_LAUNCHED();
+    if (tm.on)
+    {
+        SPK_HIP(hipEventRecord(e1, s));
+        tm.events.emplace_back(e0, e1);
+    }
+}
+
+void dispatch_layer(int steps, const uint8_t *in, const half8 *frag, Summary *summary, float *pot,
+                    const FusedGeom &g, size_t lds, hipStream_t s)
+{
+    switch (steps)
+    {
+#define CASE(k) \
+    case k: return launch_layer<k>(in, frag, summary, pot, g, lds, s);
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+        CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
+#undef CASE
+    default: SPK_ARG(false, "Fused layer: K-steps out of range.");
+    }
+}
+}  // namespace
+}  // namespace spk
+
+using namespace spk;
+
+extern "C" {
+
+int32_t spk_kernel_timer(int32_t enable)
+{
+    return guard([&] {
+        KernelTimer &tm = timer();
+        for (auto &e : tm.events) (void)hipEventDestroy(e.first), (void)hipEventDestroy(e.second);
+        tm.events.clear();
+        tm.on = enable != 0;
+    });
+}
+
+int32_t spk_kernel_timer_read(double *total_ms, int64_t *launches)
+{
+    return guard([&] {
+        KernelTimer &tm = timer();
+        double sum = 0;
+        for (auto &e : tm.events)
+        {
+            SPK_HIP(hipEventSynchronize(e.second));
+            float ms = 0;
+            SPK_HIP(hipEventElapsedTime(&ms, e.first, e.second));
+            sum += ms;
+        }
+        *total_ms = sum;
+        *launches = int64_t(tm.events.size());
+    });
+}
+
+int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, int64_t h, int64_t w, void *kernel,
+                             int64_t cout, int64_t kh, int64_t kw, int64_t pt, int64_t pl, int64_t pb, int64_t pr,
+                             double threshold, int64_t ry, int64_t rx, int64_t count,
+                             const spk_stdp_config *configs, int64_t nconfigs, void *out, void *pooled, int64_t pkh,
+                             int64_t pkw, int64_t psh, int64_t psw, spk_winner *winners, int64_t *counts,
+                             void *potentials, void *stream)
+{
+    return guard([&] {
+        SPK_ARG(n >= 1 && t >= 1 && cin >= 1 && h >= 1 && w >= 1 && cout >= 2 && kh >= 1 && kw >= 1,
+                "Fused layer: invalid shapes.");
+        SPK_ARG(pt >= 0 && pl >= 0 && pb >= 0 && pr >= 0, "Pad items are incorrect.");
+        SPK_ARG(t <= 32, "Fused layer: at most 32 time steps.");
+        SPK_ARG(cout <= 32 * MAX_WAVES, "Fused layer: at most " << 32 * MAX_WAVES << " output channels.");
+        SPK_ARG(threshold >= 0.0, "Fused layer: the conv threshold must be >= 0.");
+        SPK_ARG(ry >= 0 && rx >= 0 && count >= 1, "Fused layer: invalid convwta parameters.");
+        SPK_ARG(winners && counts, "Fused layer: winner buffers are required.");
+        const int64_t ho = h + pt + pb - kh + 1, wo = w + pl + pr - kw + 1;
+        SPK_ARG(ho >= 1 && wo >= 1, "Input and kernel shapes are not compatible.");
+        SPK_ARG(n < (1LL << 31), "Fused layer: too many samples.");
+        const bool learn = nconfigs > 0;
+        SPK_ARG(!learn || out, "Fused layer: STDP needs the spike output.");
+        int64_t pho = 0, pwo = 0;
+        if (pooled)
+        {
+            SPK_ARG(pkh >= 1 && pkw >= 1 && psh >= 1 && psw >= 1, "Pool: kernel and stride must be positive.");
+            pho = (ho - pkh) / psh + 1, pwo = (wo - pkw) / psw + 1;
+            SPK_ARG(ho >= pkh && wo >= pkw, "Pool: input and kernel shapes are not compatible.");
+        }
+
+        FusedGeom g{};
+        g.n = int32_t(n), g.t = int32_t(t), g.cin = int32_t(cin), g.h = int32_t(h), g.w = int32_t(w);
+        g.cout = int32_t(cout), g.kh = int32_t(kh), g.kw = int32_t(kw), g.pt = int32_t(pt), g.pl = int32_t(pl);
+        g.ho = int32_t(ho), g.wo = int32_t(wo);
+        g.ph = int32_t(h + pt + pb), g.pw = int32_t(w + pl + pr);
+        g.cin_pad = int32_t(cdiv(cin, 8) * 8);
+        g.groups = int32_t(kh * kw * (g.cin_pad / 8));
+        g.steps = int32_t(cdiv(g.groups, 4));
+        g.waves = int32_t(cdiv(cout, 32));
+        g.tiles = int32_t(cdiv(ho * wo, COLS));
+        g.plane = int32_t(cdiv(int64_t(g.ph) * g.pw * g.cin_pad + 8, 16) * 16);  // + slack for the last tap read
+        g.th = float(threshold);
+        SPK_ARG(g.steps <= MAX_STEPS, "Fused layer: Cin*Kh*Kw too large (K <= 512 after padding).");
+        const size_t lds = size_t(g.t) * g.plane + 2 * MAX_WAVES * COLS * 8 + COLS * 4 + size_t(g.t) * COLS * 2;
+        SPK_ARG(lds <= 163840, "Fused layer: the staged sample (" << lds << " bytes) exceeds the 160 KiB LDS.");
+        SPK_ARG(int64_t(cout) * ho * wo < (1LL << 31), "Fused layer: output plane too large.");
+        SPK_ARG(ho * wo <= 8192, "Fused layer: convwta supports at most 8192 output pixels per sample.");
+
+        hipStream_t s = S(stream);
+        const int64_t hw = ho * wo;
+        Scratch frag(int64_t(g.waves) * 2 * g.steps * 64 * 16, s), summary(n * hw * int64_t(sizeof(Summary)), s);
+        prep_fragments<<<dim3(unsigned(cdiv(int64_t(g.waves) * 2 * g.steps * 64, 256))), 256, 0, s>>>(
+            (const float *)kernel, frag.as<half8>(), g);
+        SPK_LAUNCHED();
+        dispatch_layer(g.steps, (const uint8_t *)in, frag.as<half8>(), summary.as<Summary>(), (float *)potentials,
+                       g, lds, s);
+        fused_select<<<dim3(unsigned(n)), 256, size_t(hw) * 8, s>>>(summary.as<Summary>(), g.t, g.ho, g.wo,
+                                                                    int32_t(ry), int32_t(rx), int32_t(count), winners,
+                                                                    counts);
+        SPK_LAUNCHED();
+        if (out || pooled)
+        {
+            fused_write<<<dim3(unsigned(n * t)), 256, size_t(hw) * 2, s>>>(
+                summary.as<Summary>(), (uint8_t *)out, (uint8_t *)pooled, g.t, g.cout, g.ho, g.wo, int32_t(pkh),
+                int32_t(pkw), int32_t(psh), int32_t(psw), int32_t(pho), int32_t(pwo));
+            SPK_LAUNCHED();
+        }
+        if (learn)
+        {
+            StdpConfigs c = pack_configs(configs, nconfigs);
+            StdpGeom sg{g.t, g.cin, g.h, g.w, g.cout, g.kh, g.kw, g.ho, g.wo, 1, 1, g.pt, g.pl, n, count,
+                        int32_t(nconfigs)};
+            Scratch flag(sizeof(int32_t), s), bits(stdp_bits_bytes(sg, false), s);
+            clear_flag<<<1, 1, 0, s>>>(flag.as<int32_t>());
+            stdp_launch((const uint8_t *)in, (float *)kernel, (const uint8_t *)out, sg, c, winners, counts,
+                        bits.as<uint32_t>(), flag.as<int32_t>(), false, s);
+        }
+    });
+}
+}

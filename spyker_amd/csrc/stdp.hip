@@ -1,29 +1,36 @@
 // Rank STDP for conv and FC kernels (src/spyker/cpu/stdp.cpp:42-169).
 //
 // The reference walks samples in order and, inside a sample, winners in
-// list order, updating kernel[winner.z] in place (stdp.cpp:139-154).  Updates
-// to different output channels touch disjoint weights, so only the order
-// along one channel matters: one block owns one output channel (conv) or one
-// output neuron (FC), compacts the winners that name it in (sample, rank)
-// order, and applies them one after another.  Inside a block each thread owns
-// a fixed set of weights, so consecutive updates need no barrier.
+// list order, updating kernel[winner.z] in place (stdp.cpp:139-154).  Each
+// update of one weight depends only on that weight and on one bit: whether
+// the input at its tap fired no later than the winner's output neuron
+// (isum <= osum).  So the work splits in two launches:
 //
-// The update itself is the reference's F64 formula (stdp.cpp:121-137),
-// including the fused multiply-add its build contracts `old + rate*value`
-// into, rounded once to the F32 master weight.
-#include "common.hpp"
+//   records  one block per winner slot computes osum and, for every tap, the
+//            bit isum <= osum, packed 32 to a word -- fully parallel;
+//   apply    one block per output channel (conv) or output neuron (FC)
+//            gathers the winners that name it, in (sample, rank) order, and
+//            runs the sequential chain of updates with the weights held in
+//            registers and the bits staged in LDS.
+//
+// Updates to different channels touch disjoint weights, so running the
+// channels in parallel keeps the reference's order along every weight.
+//
+// The update is the reference's F64 formula (stdp.cpp:121-137), including
+// the fused multiply-add its build contracts `old + rate*value` into, rounded
+// once to the F32 weight.  Winners naming a configuration that does not
+// exist fail the call before any weight changes (the reference throws at the
+// first such winner, after applying the earlier ones).
+#include "stdp.hpp"
 
 namespace spk
 {
 namespace
 {
-constexpr int MAXCFG = SPK_MAX_CONFIGS;
-
-struct Configs
-{
-    spk_stdp_config c[MAXCFG];
-    int32_t n;
-};
+constexpr int APPLY_THREADS = 256;
+constexpr int MAX_WEIGHTS_PER_THREAD = 4;
+constexpr int SLICE = APPLY_THREADS * MAX_WEIGHTS_PER_THREAD;  // weights per block
+constexpr int SLICE_WORDS = SLICE / 32;
 
 __device__ __forceinline__ float stdp_update(float w, bool potentiate, const spk_stdp_config &cfg)
 {
@@ -35,171 +42,205 @@ __device__ __forceinline__ float stdp_update(float w, bool potentiate, const spk
     return float(r);
 }
 
-// Fails the whole call (no weight changes) if any active winner names a
-// configuration that does not exist (flag 1) or a neuron outside the output
-// (flag 2; the reference does not check this and would index out of range).
-__device__ bool winners_valid(const spk_winner *__restrict__ win, const int64_t *__restrict__ counts, int64_t n,
-                              int64_t maxcount, int32_t ncfg, int64_t zlim, int64_t ylim, int64_t xlim,
-                              int32_t *flag)
+// Packs one bit per lane of a 64-wide pass into the record words.
+__device__ __forceinline__ void store_bits(uint32_t *rec, int32_t words, int32_t base, bool bit)
 {
-    __shared__ int32_t bad;
-    if (threadIdx.x == 0) bad = 0;
-    __syncthreads();
-    for (int64_t e = threadIdx.x; e < n * maxcount; e += blockDim.x)
+    const uint64_t ballot = __ballot(bit);
+    if ((threadIdx.x & 63) == 0)
     {
-        const int64_t s = e / maxcount, k = e % maxcount;
-        if (k >= counts[s]) continue;
-        const spk_winner w = win[e];
-        if (w.c >= ncfg) atomicMax(&bad, 1);
-        if (w.c >= 0 && (w.z < 0 || w.z >= zlim || w.y < 0 || w.y >= ylim || w.x < 0 || w.x >= xlim))
-            atomicMax(&bad, 2);
+        if (base / 32 < words) rec[base / 32] = uint32_t(ballot);
+        if (base / 32 + 1 < words) rec[base / 32 + 1] = uint32_t(ballot >> 32);
     }
-    __syncthreads();
-    const int32_t b = bad;
-    if (b != 0 && blockIdx.x == 0 && threadIdx.x == 0) *flag = b;
-    return b == 0;
 }
 
-// Order-preserving compaction of one chunk of winner slots whose key equals
-// `key`; returns the number of entries written to list.
-__device__ int32_t compact(const spk_winner *__restrict__ win, const int64_t *__restrict__ counts, int64_t maxcount,
-                           int64_t e0, int64_t total, int64_t key, bool by_x, int32_t *list, int32_t *wave_count)
+// One block (256 threads) per winner slot.
+__global__ void __launch_bounds__(256)
+    conv_records(const uint8_t *__restrict__ in, const uint8_t *__restrict__ out, StdpGeom g,
+                 const spk_winner *__restrict__ win, const int64_t *__restrict__ counts,
+                 uint32_t *__restrict__ bits, int32_t *__restrict__ flag)
 {
-    const int64_t e = e0 + threadIdx.x;
-    bool match = false;
-    if (e < total)
+    const int64_t e = blockIdx.x, s = e / g.maxcount, k = e % g.maxcount;
+    if (k >= counts[s]) return;
+    const spk_winner w = win[e];
+    if (w.c < 0) return;
+    if (w.c >= g.ncfg)
     {
-        const int64_t s = e / maxcount, k = e % maxcount;
-        if (k < counts[s])
-        {
-            const spk_winner w = win[e];
-            match = w.c >= 0 && (by_x ? w.x : w.z) == key;
-        }
+        if (threadIdx.x == 0) atomicMax(flag, 1);
+        return;
     }
-    const uint64_t ballot = __ballot(match);
+    if (w.z < 0 || w.z >= g.cout || w.y < 0 || w.y >= g.ho || w.x < 0 || w.x >= g.wo)
+    {
+        if (threadIdx.x == 0) atomicMax(flag, 2);
+        return;
+    }
+    const int64_t out_plane = int64_t(g.ho) * g.wo, in_plane = int64_t(g.h) * g.w, in_step = g.cin * in_plane;
+    const uint8_t *os = out + (s * g.t * g.cout + w.z) * out_plane + w.y * g.wo + w.x;
+    int32_t osum = g.t;
+    for (int32_t i = 0; i < g.t; ++i)
+        if (os[int64_t(i) * g.cout * out_plane])
+        {
+            osum = i;
+            break;
+        }
+    const int32_t taps = g.kh * g.kw, ksz = g.cin * taps, words = (ksz + 31) / 32;
+    const uint8_t *is = in + s * g.t * in_step;
+    uint32_t *rec = bits + e * words;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) wave_count[wave] = __popcll(ballot);
-    __syncthreads();
-    int32_t base = 0, total_n = 0;
-    for (int i = 0; i < int(blockDim.x >> 6); ++i)
+    for (int32_t base = wave * 64; base < words * 32; base += 256)
     {
-        if (i < wave) base += wave_count[i];
-        total_n += wave_count[i];
-    }
-    if (match) list[base + __popcll(ballot & ((1ull << lane) - 1))] = int32_t(threadIdx.x);
-    __syncthreads();
-    return total_n;
-}
-
-__global__ void __launch_bounds__(256)
-    convstdp_kernel(const uint8_t *__restrict__ in, float *__restrict__ kernel, const uint8_t *__restrict__ out,
-                    int32_t t, int32_t cin, int32_t h, int32_t w, int32_t cout, int32_t kh, int32_t kw, int32_t ho,
-                    int32_t wo, Configs cfg, const spk_winner *__restrict__ win, const int64_t *__restrict__ counts,
-                    int64_t n, int64_t maxcount, int32_t sy, int32_t sx, int32_t pt, int32_t pl,
-                    int32_t *__restrict__ flag)
-{
-    __shared__ int32_t list[256];
-    __shared__ int32_t wave_count[4];
-    if (!winners_valid(win, counts, n, maxcount, cfg.n, cout, ho, wo, flag)) return;
-    const int32_t z = blockIdx.x;
-    const int32_t ksz = cin * kh * kw;
-    float *kz = kernel + int64_t(z) * ksz;
-    const int64_t total = n * maxcount;
-    const int64_t in_plane = int64_t(h) * w, in_step = cin * in_plane, out_plane = int64_t(ho) * wo;
-    for (int64_t e0 = 0; e0 < total; e0 += 256)
-    {
-        const int32_t m = compact(win, counts, maxcount, e0, total, z, false, list, wave_count);
-        for (int32_t q = 0; q < m; ++q)
+        const int32_t i = base + lane;
+        bool bit = false;
+        if (i < ksz)
         {
-            const int64_t e = e0 + list[q], s = e / maxcount;
-            const spk_winner wv = win[e];
-            const uint8_t *os = out + ((s * t) * cout + z) * out_plane + wv.y * wo + wv.x;
-            int32_t osum = t;
-            for (int32_t i = 0; i < t; ++i)
-                if (os[int64_t(i) * cout * out_plane])
-                {
-                    osum = i;
-                    break;
-                }
-            const spk_stdp_config &c = cfg.c[wv.c];
-            const uint8_t *is = in + s * t * in_step;
-            for (int32_t i = threadIdx.x; i < ksz; i += blockDim.x)
+            const int32_t ch = i / taps, a = (i / g.kw) % g.kh, b = i % g.kw;
+            const int32_t iy = int32_t(w.y) * g.sy + a - g.pt, ix = int32_t(w.x) * g.sx + b - g.pl;
+            int32_t isum = g.t;
+            if (iy >= 0 && iy < g.h && ix >= 0 && ix < g.w)
             {
-                const int32_t ch = i / (kh * kw), a = (i / kw) % kh, b = i % kw;
-                const int32_t iy = int32_t(wv.y) * sy + a - pt, ix = int32_t(wv.x) * sx + b - pl;
-                int32_t isum = t;
-                if (iy >= 0 && iy < h && ix >= 0 && ix < w)
-                {
-                    const uint8_t *p = is + ch * in_plane + int64_t(iy) * w + ix;
-                    for (int32_t j = 0; j < t; ++j)
-                        if (p[int64_t(j) * in_step])
-                        {
-                            isum = j;
-                            break;
-                        }
-                }
-                kz[i] = stdp_update(kz[i], isum <= osum, c);
-            }
-        }
-        __syncthreads();
-    }
-}
-
-__global__ void __launch_bounds__(256)
-    fcstdp_kernel(const uint8_t *__restrict__ in, float *__restrict__ kernel, const uint8_t *__restrict__ out,
-                  int32_t t, int32_t fin, int32_t fout, Configs cfg, const spk_winner *__restrict__ win,
-                  const int64_t *__restrict__ counts, int64_t n, int64_t maxcount, int32_t *__restrict__ flag)
-{
-    __shared__ int32_t list[256];
-    __shared__ int32_t wave_count[4];
-    if (!winners_valid(win, counts, n, maxcount, cfg.n, INT64_MAX, INT64_MAX, fout, flag)) return;
-    const int32_t o = blockIdx.x;
-    float *ko = kernel + int64_t(o) * fin;
-    const int64_t total = n * maxcount;
-    for (int64_t e0 = 0; e0 < total; e0 += 256)
-    {
-        const int32_t m = compact(win, counts, maxcount, e0, total, o, true, list, wave_count);
-        for (int32_t q = 0; q < m; ++q)
-        {
-            const int64_t e = e0 + list[q], s = e / maxcount;
-            const spk_winner wv = win[e];
-            const uint8_t *os = out + s * t * int64_t(fout) + o;
-            int32_t osum = t;
-            for (int32_t i = 0; i < t; ++i)
-                if (os[int64_t(i) * fout])
-                {
-                    osum = i;
-                    break;
-                }
-            const spk_stdp_config &c = cfg.c[wv.c];
-            const uint8_t *is = in + s * t * int64_t(fin);
-            for (int32_t f = threadIdx.x; f < fin; f += blockDim.x)
-            {
-                int32_t isum = t;
-                for (int32_t j = 0; j < t; ++j)
-                    if (is[int64_t(j) * fin + f])
+                const uint8_t *p = is + ch * in_plane + int64_t(iy) * g.w + ix;
+                for (int32_t j = 0; j < g.t; ++j)
+                    if (p[int64_t(j) * in_step])
                     {
                         isum = j;
                         break;
                     }
-                ko[f] = stdp_update(ko[f], isum <= osum, c);
+            }
+            bit = isum <= osum;
+        }
+        store_bits(rec, words, base, bit);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+    fc_records(const uint8_t *__restrict__ in, const uint8_t *__restrict__ out, StdpGeom g,
+               const spk_winner *__restrict__ win, const int64_t *__restrict__ counts, uint32_t *__restrict__ bits,
+               int32_t *__restrict__ flag)
+{
+    const int64_t e = blockIdx.x, s = e / g.maxcount, k = e % g.maxcount;
+    if (k >= counts[s]) return;
+    const spk_winner w = win[e];
+    if (w.c < 0) return;
+    if (w.c >= g.ncfg)
+    {
+        if (threadIdx.x == 0) atomicMax(flag, 1);
+        return;
+    }
+    if (w.x < 0 || w.x >= g.cout)
+    {
+        if (threadIdx.x == 0) atomicMax(flag, 2);
+        return;
+    }
+    const uint8_t *os = out + s * g.t * int64_t(g.cout) + w.x;
+    int32_t osum = g.t;
+    for (int32_t i = 0; i < g.t; ++i)
+        if (os[int64_t(i) * g.cout])
+        {
+            osum = i;
+            break;
+        }
+    const int32_t fin = g.cin, words = (fin + 31) / 32;
+    const uint8_t *is = in + s * g.t * int64_t(fin);
+    uint32_t *rec = bits + e * words;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int32_t base = wave * 64; base < words * 32; base += 256)
+    {
+        const int32_t f = base + lane;
+        bool bit = false;
+        if (f < fin)
+        {
+            int32_t isum = g.t;
+            for (int32_t j = 0; j < g.t; ++j)
+                if (is[int64_t(j) * fin + f])
+                {
+                    isum = j;
+                    break;
+                }
+            bit = isum <= osum;
+        }
+        store_bits(rec, words, base, bit);
+    }
+}
+
+// One block per (output neuron, slice of SLICE weights); key = winner.z
+// (conv) or winner.x (FC).
+__global__ void __launch_bounds__(APPLY_THREADS)
+    apply_records(float *__restrict__ kernel, StdpGeom g, StdpConfigs cfg, const spk_winner *__restrict__ win,
+                  const int64_t *__restrict__ counts, const uint32_t *__restrict__ bits,
+                  const int32_t *__restrict__ flag, int32_t by_x)
+{
+    __shared__ uint32_t lds_bits[APPLY_THREADS * SLICE_WORDS];
+    __shared__ int32_t list[APPLY_THREADS];
+    __shared__ int32_t cfgidx[APPLY_THREADS];
+    __shared__ int32_t wave_count[APPLY_THREADS / 64];
+    if (*flag != 0) return;
+    const int32_t key = blockIdx.x, w0 = blockIdx.y * SLICE;
+    const int32_t ksz = by_x ? g.cin : g.cin * g.kh * g.kw, words = (ksz + 31) / 32;
+    const int32_t nw = min(SLICE, ksz - w0), sw = (nw + 31) / 32, word0 = w0 / 32;
+    float *kz = kernel + int64_t(key) * ksz + w0;
+
+    float wr[MAX_WEIGHTS_PER_THREAD];
+#pragma unroll
+    for (int j = 0; j < MAX_WEIGHTS_PER_THREAD; ++j)
+    {
+        const int32_t i = threadIdx.x + j * APPLY_THREADS;
+        wr[j] = i < nw ? kz[i] : 0.0f;
+    }
+
+    const int64_t total = g.n * g.maxcount;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int64_t e0 = 0; e0 < total; e0 += APPLY_THREADS)
+    {
+        // order-preserving compaction of this chunk's winners that name `key`
+        const int64_t e = e0 + threadIdx.x;
+        bool match = false;
+        int32_t c = 0;
+        if (e < total && e % g.maxcount < counts[e / g.maxcount])
+        {
+            const spk_winner w = win[e];
+            match = w.c >= 0 && (by_x ? w.x : w.z) == key;
+            c = int32_t(w.c);
+        }
+        const uint64_t ballot = __ballot(match);
+        if (lane == 0) wave_count[wave] = __popcll(ballot);
+        __syncthreads();
+        int32_t base = 0, m = 0;
+        for (int i = 0; i < APPLY_THREADS / 64; ++i)
+        {
+            if (i < wave) base += wave_count[i];
+            m += wave_count[i];
+        }
+        if (match)
+        {
+            const int32_t slot = base + __popcll(ballot & ((1ull << lane) - 1));
+            list[slot] = int32_t(threadIdx.x);
+            cfgidx[slot] = c;
+        }
+        __syncthreads();
+        for (int32_t q = threadIdx.x; q < m * sw; q += APPLY_THREADS)
+            lds_bits[q] = bits[(e0 + list[q / sw]) * words + word0 + q % sw];
+        __syncthreads();
+        for (int32_t q = 0; q < m; ++q)
+        {
+            const spk_stdp_config &cf = cfg.c[cfgidx[q]];
+            const uint32_t *b = lds_bits + q * sw;
+#pragma unroll
+            for (int j = 0; j < MAX_WEIGHTS_PER_THREAD; ++j)
+            {
+                const int32_t i = threadIdx.x + j * APPLY_THREADS;
+                if (i < nw) wr[j] = stdp_update(wr[j], (b[i >> 5] >> (i & 31)) & 1u, cf);
             }
         }
         __syncthreads();
     }
+#pragma unroll
+    for (int j = 0; j < MAX_WEIGHTS_PER_THREAD; ++j)
+    {
+        const int32_t i = threadIdx.x + j * APPLY_THREADS;
+        if (i < nw) kz[i] = wr[j];
+    }
 }
 
-Configs pack(const spk_stdp_config *configs, int64_t nconfigs)
-{
-    SPK_ARG(nconfigs > 0, "At least one STDP Config must be given.");
-    SPK_ARG(nconfigs <= MAXCFG, "At most " << MAXCFG << " STDP configurations are supported.");
-    Configs c{};
-    for (int64_t i = 0; i < nconfigs; ++i) c.c[i] = configs[i];
-    c.n = int32_t(nconfigs);
-    return c;
-}
-
-// Reports a configuration error found on the device (synchronises the stream).
+// Reports a rejection found on the device (synchronises the stream).
 void check_flag(int32_t *dflag, hipStream_t s)
 {
     int32_t flag = 0;
@@ -208,7 +249,43 @@ void check_flag(int32_t *dflag, hipStream_t s)
     SPK_REQUIRE(flag != 1, SPK_ERR_CONFIG, "STDP configuration index is out of range.");
     SPK_REQUIRE(flag == 0, SPK_ERR_ARGUMENT, "Winner index is out of range.");
 }
+
+__global__ void clear_flag(int32_t *flag) { *flag = 0; }
 }  // namespace
+
+StdpConfigs pack_configs(const spk_stdp_config *configs, int64_t nconfigs)
+{
+    SPK_ARG(nconfigs > 0, "At least one STDP Config must be given.");
+    SPK_ARG(nconfigs <= SPK_MAX_CONFIGS, "At most " << SPK_MAX_CONFIGS << " STDP configurations are supported.");
+    StdpConfigs c{};
+    for (int64_t i = 0; i < nconfigs; ++i) c.c[i] = configs[i];
+    c.n = int32_t(nconfigs);
+    return c;
+}
+
+int64_t stdp_bits_bytes(const StdpGeom &g, bool fc)
+{
+    const int64_t ksz = fc ? g.cin : int64_t(g.cin) * g.kh * g.kw;
+    return g.n * g.maxcount * ((ksz + 31) / 32) * int64_t(sizeof(uint32_t));
+}
+
+void stdp_launch(const uint8_t *in, float *kernel, const uint8_t *out, const StdpGeom &g, const StdpConfigs &cfg,
+                 const spk_winner *winners, const int64_t *counts, uint32_t *bits, int32_t *flag, bool fc,
+                 hipStream_t s)
+{
+    const int64_t ksz = fc ? g.cin : int64_t(g.cin) * g.kh * g.kw;
+    SPK_ARG(ksz < (1LL << 30), "STDP: kernel too large.");
+    const int64_t slots = g.n * g.maxcount;
+    SPK_ARG(slots < (1LL << 31), "STDP: too many winner slots.");
+    if (fc)
+        fc_records<<<dim3(unsigned(slots)), 256, 0, s>>>(in, out, g, winners, counts, bits, flag);
+    else
+        conv_records<<<dim3(unsigned(slots)), 256, 0, s>>>(in, out, g, winners, counts, bits, flag);
+    SPK_LAUNCHED();
+    apply_records<<<dim3(unsigned(g.cout), unsigned(cdiv(ksz, SLICE))), APPLY_THREADS, 0, s>>>(
+        kernel, g, cfg, winners, counts, bits, flag, fc ? 1 : 0);
+    SPK_LAUNCHED();
+}
 }  // namespace spk
 
 using namespace spk;
@@ -230,15 +307,15 @@ int32_t spk_rank_convstdp(const void *in, void *kernel, int32_t kernel_type, con
         SPK_ARG(ho == (h + pt + pb - kh) / sy + 1 && wo == (w + pl + pr - kw) / sx + 1,
                 "Tensor shapes are not compatible.");
         SPK_ARG(maxcount >= 1, "STDP: winner capacity must be positive.");
-        Configs c = pack(configs, nconfigs);
+        StdpConfigs c = pack_configs(configs, nconfigs);
+        StdpGeom g{int32_t(t), int32_t(cin), int32_t(h), int32_t(w), int32_t(cout), int32_t(kh), int32_t(kw),
+                   int32_t(ho), int32_t(wo), int32_t(sy), int32_t(sx), int32_t(pt), int32_t(pl), n, maxcount,
+                   int32_t(nconfigs)};
         hipStream_t s = S(stream);
-        Scratch flag(sizeof(int32_t), s);
-        SPK_HIP(hipMemsetAsync(flag.ptr, 0, sizeof(int32_t), s));
-        convstdp_kernel<<<dim3(unsigned(cout)), 256, 0, s>>>(
-            (const uint8_t *)in, (float *)kernel, (const uint8_t *)out, int32_t(t), int32_t(cin), int32_t(h),
-            int32_t(w), int32_t(cout), int32_t(kh), int32_t(kw), int32_t(ho), int32_t(wo), c, winners, counts, n,
-            maxcount, int32_t(sy), int32_t(sx), int32_t(pt), int32_t(pl), flag.as<int32_t>());
-        SPK_LAUNCHED();
+        Scratch flag(sizeof(int32_t), s), bits(stdp_bits_bytes(g, false), s);
+        clear_flag<<<1, 1, 0, s>>>(flag.as<int32_t>());
+        stdp_launch((const uint8_t *)in, (float *)kernel, (const uint8_t *)out, g, c, winners, counts,
+                    bits.as<uint32_t>(), flag.as<int32_t>(), false, s);
         check_flag(flag.as<int32_t>(), s);
     });
 }
@@ -253,15 +330,14 @@ int32_t spk_rank_fcstdp(const void *in, void *kernel, int32_t kernel_type, const
         SPK_ARG(n > 0, "Winners can't be empty.");
         SPK_ARG(t >= 1 && fin >= 1 && fout >= 1, "STDP: invalid shapes.");
         SPK_ARG(maxcount >= 1, "STDP: winner capacity must be positive.");
-        Configs c = pack(configs, nconfigs);
+        StdpConfigs c = pack_configs(configs, nconfigs);
+        StdpGeom g{int32_t(t), int32_t(fin), 1, 1, int32_t(fout), 1, 1, 1, 1, 1, 1, 0, 0, n, maxcount,
+                   int32_t(nconfigs)};
         hipStream_t s = S(stream);
-        Scratch flag(sizeof(int32_t), s);
-        SPK_HIP(hipMemsetAsync(flag.ptr, 0, sizeof(int32_t), s));
-        fcstdp_kernel<<<dim3(unsigned(fout)), 256, 0, s>>>((const uint8_t *)in, (float *)kernel,
-                                                           (const uint8_t *)out, int32_t(t), int32_t(fin),
-                                                           int32_t(fout), c, winners, counts, n, maxcount,
-                                                           flag.as<int32_t>());
-        SPK_LAUNCHED();
+        Scratch flag(sizeof(int32_t), s), bits(stdp_bits_bytes(g, true), s);
+        clear_flag<<<1, 1, 0, s>>>(flag.as<int32_t>());
+        stdp_launch((const uint8_t *)in, (float *)kernel, (const uint8_t *)out, g, c, winners, counts,
+                    bits.as<uint32_t>(), flag.as<int32_t>(), true, s);
         check_flag(flag.as<int32_t>(), s);
     });
 }

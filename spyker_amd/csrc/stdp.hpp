@@ -1,0 +1,35 @@
+// Rank STDP launch interface shared by the plain entry points and the fused
+// layer step.
+#pragma once
+
+#include "common.hpp"
+
+namespace spk
+{
+struct StdpGeom
+{
+    // FC uses cin = input features, cout = output neurons, the rest unused.
+    int32_t t, cin, h, w, cout, kh, kw, ho, wo, sy, sx, pt, pl;
+    int64_t n, maxcount;
+    int32_t ncfg;
+};
+
+struct StdpConfigs
+{
+    spk_stdp_config c[SPK_MAX_CONFIGS];
+    int32_t n;
+};
+
+StdpConfigs pack_configs(const spk_stdp_config *configs, int64_t nconfigs);
+
+// Bytes of the per-winner bit records the launch needs as workspace.
+int64_t stdp_bits_bytes(const StdpGeom &g, bool fc);
+
+// Enqueues the record and apply kernels.  flag must be a device int32 that
+// is zero on entry; a nonzero value afterwards means the call was rejected
+// (1: configuration index out of range, 2: winner index out of range) and no
+// weight changed.
+void stdp_launch(const uint8_t *in, float *kernel, const uint8_t *out, const StdpGeom &g, const StdpConfigs &cfg,
+                 const spk_winner *winners, const int64_t *counts, uint32_t *bits, int32_t *flag, bool fc,
+                 hipStream_t s);
+}  // namespace spk

@@ -1,0 +1,110 @@
+"""The fused FP16-MFMA layer step.
+
+Two bars:
+  * structure: with weights exactly representable in FP16 and potentials
+    that sum exactly in FP32 (multiples of 1/64), the FP16 path computes the
+    same potentials as the FP32 plain path, so spikes, winners, pooled spikes
+    and STDP-updated weights must equal the oracle's bit for bit;
+  * precision: with N(0.8, 0.05) weights, raw potentials stay within
+    K * 2^-11 * max|w| of FP64 and the threshold-flip rate stays <= 0.1 %.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import spyker_amd as spk
+
+pytestmark = pytest.mark.gpu
+
+
+def T(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def H(t):
+    return t.cpu().numpy()
+
+
+def oracle_step(orc, x, k, th, radius, count, pad, pool):
+    n, t, cin, h, w = x.shape
+    cout, _, kh, kw = k.shape
+    pt, pl, pb, pr = pad
+    ho, wo = h + pt + pb - kh + 1, w + pl + pr - kw + 1
+    pot = orc.conv(x.reshape(n * t, cin, h, w), k, 1, pad).reshape(n, t, cout, ho, wo)
+    pot = orc.inhibit(orc.threshold(pot, th))
+    win, cnt = orc.convwta(pot, radius, count)
+    out = orc.fire(pot)
+    k2 = orc.convstdp(x, k, out, [(0.004, -0.003)], win, cnt, 1, pad)
+    pooled = orc.pool(out, pool[0], pool[1]) if pool else None
+    return out, win, cnt, k2, pooled
+
+
+def spikes(rng, n, t, c, h, w, p=0.15):
+    return np.maximum.accumulate((rng.random((n, t, c, h, w)) < p / t * 3).astype(np.uint8), axis=1)
+
+
+def check_winners(wn, win, cnt):
+    d, c = H(wn.data), H(wn.counts)
+    assert np.array_equal(c, cnt)
+    for i in range(len(c)):
+        for k in range(c[i]):
+            assert tuple(d[i, k, 2:]) == (win[i, k]["z"], win[i, k]["y"], win[i, k]["x"])
+
+
+@pytest.mark.parametrize("case", [
+    # n, t, cin, h, w, cout, k, pad, th, radius, count, pool
+    (6, 15, 30, 14, 14, 250, 3, (1, 1, 1, 1), 10.0, 1, 8, (3, 3)),
+    (5, 15, 4, 28, 28, 30, 5, (2, 2, 2, 2), 15.0, 3, 5, (2, 2)),
+    (3, 8, 8, 20, 17, 64, 7, (3, 3, 3, 3), 20.0, 5, 10, (2, 2)),
+    (4, 12, 3, 11, 9, 40, 3, (0, 1, 2, 0), 4.0, (1, 2), 4, None),
+])
+def test_fused_exact_on_dyadic_weights(dev, orc, case):
+    n, t, cin, h, w, cout, k, pad, th, radius, count, pool = case
+    rng = np.random.default_rng(11)
+    x = spikes(rng, n, t, cin, h, w, 0.5)
+    kern = (rng.integers(40, 64, (cout, cin, k, k)) / 64).astype(np.float32)
+    # scale the threshold to the potentials of this input
+    pot = orc.conv(x.reshape(n * t, cin, h, w), kern, 1, pad)
+    th = float(np.quantile(pot, 0.9)) + 0.5 / 64 if th else th
+    conv = spk.Conv(cin, cout, k, pad=pad, device=dev)
+    conv.kernel = T(kern, dev)
+    conv.stdpconfig = [spk.STDPConfig(0.004, -0.003)]
+    out, wn, pooled = spk.layer_step(conv, T(x, dev), th, radius, count, *(pool if pool else (None, None)))
+    o_out, o_win, o_cnt, o_k, o_pool = oracle_step(orc, x, kern, th, radius, count, pad, pool)
+    assert o_cnt.sum() > 0
+    assert np.array_equal(H(out), o_out)
+    check_winners(wn, o_win, o_cnt)
+    assert np.array_equal(H(conv.kernel), o_k)
+    if pool:
+        assert np.array_equal(H(pooled), o_pool)
+
+
+def test_fused_precision_vs_fp64(dev, orc):
+    n, t, cin, h, w, cout = 8, 15, 30, 14, 14, 250
+    rng = np.random.default_rng(12)
+    x = spikes(rng, n, t, cin, h, w, 0.5)
+    kern = np.clip(rng.normal(0.8, 0.05, (cout, cin, 3, 3)), 0, 1).astype(np.float32)
+    pot64 = orc.conv(x.reshape(n * t, cin, h, w), kern, 1, (1, 1, 1, 1), f64=True).reshape(n, t, cout, h, w)
+    th = float(np.quantile(pot64, 0.7))
+    conv = spk.Conv(cin, cout, 3, pad=1, device=dev)
+    conv.kernel = T(kern, dev)
+    conv.stdpconfig = [spk.STDPConfig(0.004, -0.003)]
+    pots = torch.empty((n, t, cout, h, w), dtype=torch.float32, device=dev)
+    spk.layer_step(conv, T(x, dev), th, 1, 8, learn=False, potentials=pots)
+    p16 = H(pots).astype(np.float64)
+    K = cin * 9
+    assert np.abs(p16 - pot64).max() <= K * 2.0 ** -11 * kern.max()
+    flips = np.mean((p16 > th) != (pot64 > th))
+    assert flips <= 1e-3, flips
+
+
+def test_fused_inference_only_leaves_weights(dev, orc):
+    rng = np.random.default_rng(13)
+    x = spikes(rng, 3, 10, 4, 16, 16, 0.5)
+    kern = np.clip(rng.normal(0.8, 0.05, (20, 4, 5, 5)), 0, 1).astype(np.float32)
+    conv = spk.Conv(4, 20, 5, pad=2, device=dev)
+    conv.kernel = T(kern, dev)
+    out, wn, _ = spk.layer_step(conv, T(x, dev), 5.0, 2, 3, learn=False)
+    assert np.array_equal(H(conv.kernel), kern)
+    assert H(out).sum() > 0
