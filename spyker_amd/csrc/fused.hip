@@ -59,7 +59,8 @@ struct FusedGeom
 {
     int32_t n, t, cin, h, w, cout, kh, kw, pt, pl, ho, wo;
     int32_t ph, pw, cin_pad, groups, steps, waves, tiles;
-    int32_t plane;  // bytes of one staged step
+    int32_t plane;      // bytes of one staged step
+    int32_t win_units;  // 16-byte units of one FP16 window buffer
     float th;
 };
 
@@ -106,19 +107,45 @@ __device__ __forceinline__ half8 widen(uint2 raw)
 
 __device__ __forceinline__ void combine(float &v, int32_t &i, float ov, int32_t oi)
 {
-    if (ov > v || (ov == v && oi < i)) v = ov, i = oi;
+    const bool take = (ov > v) | ((ov == v) & (oi < i));
+    v = take ? ov : v;
+    i = take ? oi : i;
 }
 
-template <int S>
+// Writes `count` consecutive input bytes, starting at flat index i of the
+// sample's [t][cin][h][w] tensor, to their [t][y+pt][x+pl][c] LDS slots.
+__device__ __forceinline__ void stage_bytes(uint8_t *stage, const FusedGeom &g, int32_t i, int count, uint32_t v,
+                                            int32_t chw, int32_t hwi)
+{
+    int32_t tt = i / chw, r = i - tt * chw, c = r / hwi, pix = r - c * hwi;
+    int32_t y = pix / g.w, x = pix - y * g.w;
+    for (int b = 0; b < count; ++b)
+    {
+        stage[tt * g.plane + ((y + g.pt) * g.pw + x + g.pl) * g.cin_pad + c] = uint8_t(v >> (8 * b));
+        if (++x == g.w)
+        {
+            x = 0;
+            if (++y == g.h)
+            {
+                y = 0;
+                if (++c == g.cin) c = 0, ++tt;
+            }
+        }
+    }
+}
+
+template <int S, bool DIAG>
 __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ in, const half8 *__restrict__ frag,
                                                    Summary *__restrict__ summary, float *__restrict__ pot_out,
                                                    FusedGeom g)
 {
     extern __shared__ __align__(16) uint8_t lds[];
-    uint8_t *stage = lds;
-    float *red_v = reinterpret_cast<float *>(lds + g.t * g.plane);  // [2][MAX_WAVES][COLS]
-    int32_t *red_i = reinterpret_cast<int32_t *>(red_v + 2 * MAX_WAVES * COLS);
-    int32_t *cstar = red_i + 2 * MAX_WAVES * COLS;                     // [COLS]
+    uint8_t *stage = lds;                                             // [t][ph][pw][cin_pad] U8
+    uint4 *win16 = reinterpret_cast<uint4 *>(lds + g.t * g.plane);   // [2][rows][pw][cin_pad] FP16
+    // per (step, wave, column) partial channel maximum; combined once per tile
+    float *red_v = reinterpret_cast<float *>(win16 + 2 * g.win_units);  // [t][MAX_WAVES][COLS]
+    int32_t *red_i = reinterpret_cast<int32_t *>(red_v + g.t * MAX_WAVES * COLS);
+    int32_t *cstar = red_i + g.t * MAX_WAVES * COLS;                   // [COLS]
     uint16_t *ch_tab = reinterpret_cast<uint16_t *>(cstar + COLS);     // [t][COLS]
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, q = lane >> 4, j = lane & 15;
@@ -134,7 +161,7 @@ __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ i
         a[1][s] = frag[((w * 2 + 1) * S + s) * 64 + lane];
         const int32_t grp = 4 * s + q, cg = g.cin_pad / 8;
         const int32_t tap = grp < g.groups ? grp / cg : 0, c8 = grp < g.groups ? grp % cg : 0;
-        offs[s] = ((tap / g.kw) * g.pw + tap % g.kw) * g.cin_pad + c8 * 8;
+        offs[s] = (((tap / g.kw) * g.pw + tap % g.kw) * g.cin_pad + c8 * 8) * 2;  // bytes in the FP16 window
     }
 
     for (int32_t n = blockIdx.x; n < g.n; n += gridDim.x)
@@ -148,19 +175,30 @@ __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ i
         }
         __syncthreads();
         {
+            // the sample is t*cin*h*w contiguous bytes: read it as dwords, U per
+            // thread in flight, then scatter each byte to its channel-innermost
+            // slot
+            constexpr int U = 8;
             const uint8_t *src = in + int64_t(n) * g.t * g.cin * g.h * g.w;
-            const int32_t c4 = (g.cin + 3) / 4, per_t = c4 * g.h * g.w;
-            for (int32_t i = threadIdx.x; i < g.t * per_t; i += blockDim.x)
-            {
-                const int32_t tt = i / per_t, r = i % per_t, cq = r / (g.h * g.w), pix = r % (g.h * g.w);
-                const int32_t y = pix / g.w, x = pix % g.w;
-                const uint8_t *s0 = src + (int64_t(tt) * g.cin + cq * 4) * g.h * g.w + pix;
-                uint32_t v = 0;
-                for (int b = 0; b < 4; ++b)
-                    if (cq * 4 + b < g.cin) v |= uint32_t(s0[int64_t(b) * g.h * g.w]) << (8 * b);
-                *reinterpret_cast<uint32_t *>(stage + tt * g.plane + ((y + g.pt) * g.pw + x + g.pl) * g.cin_pad +
-                                              cq * 4) = v;
-            }
+            const int32_t hwi = g.h * g.w, chw = g.cin * hwi, total = g.t * chw;
+            const int32_t stride = int32_t(blockDim.x) * 4;
+            if ((reinterpret_cast<uintptr_t>(src) & 3) == 0 && (total & 3) == 0)
+                for (int32_t i0 = int32_t(threadIdx.x) * 4; i0 < total; i0 += stride * U)
+                {
+                    uint32_t v[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        v[u] = *reinterpret_cast<const uint32_t *>(src + min(i0 + u * stride, total - 4));
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                    {
+                        const int32_t i = i0 + u * stride;
+                        if (i < total) stage_bytes(stage, g, i, 4, v[u], chw, hwi);
+                    }
+                }
+            else
+                for (int32_t i = int32_t(threadIdx.x); i < total; i += int32_t(blockDim.x))
+                    stage_bytes(stage, g, i, 1, src[i], chw, hwi);
         }
         __syncthreads();
 
@@ -169,7 +207,12 @@ __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ i
             const int32_t p = tile * COLS + j;
             const bool pvalid = p < hw;
             const int32_t pc = pvalid ? p : hw - 1, y = pc / g.wo, x = pc % g.wo;
-            const int32_t base = (y * g.pw + x) * g.cin_pad;
+            // the tile's input window: whole padded rows [y_first, y_first + rows)
+            const int32_t y_first = (tile * COLS) / g.wo;
+            const int32_t rows = min(tile * COLS + COLS - 1, hw - 1) / g.wo - y_first + g.kh;
+            const int32_t wgroups = rows * g.pw * (g.cin_pad / 8);
+            const uint8_t *wsrc = stage + y_first * g.pw * g.cin_pad;
+            const int32_t base = ((y - y_first) * g.pw + x) * g.cin_pad * 2;  // bytes in the FP16 window
             uint32_t mask[2][4];
             float first[2][4], mx[2][4];
 #pragma unroll
@@ -178,19 +221,38 @@ __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ i
                 for (int i = 0; i < 4; ++i) mask[r][i] = 0, first[r][i] = 0.0f, mx[r][i] = 0.0f;
             int32_t cnt = 0;
 
+            // widen step 0's window once for the whole block
+            for (int32_t e = threadIdx.x; e < wgroups; e += blockDim.x)
+                win16[e] = __builtin_bit_cast(uint4, widen(*reinterpret_cast<const uint2 *>(wsrc + e * 8)));
+            __syncthreads();
             for (int32_t tt = 0; tt < g.t; ++tt)
             {
+                const uint8_t *wb = reinterpret_cast<const uint8_t *>(win16 + (tt & 1) * g.win_units) + base;
+                half8 b[S];
+#pragma unroll
+                for (int s = 0; s < S; ++s) b[s] = *reinterpret_cast<const half8 *>(wb + offs[s]);
+                // widen step tt+1's window into the other buffer meanwhile
+                if (tt + 1 < g.t)
+                {
+                    uint4 *dst = win16 + ((tt + 1) & 1) * g.win_units;
+                    const uint8_t *src = wsrc + (tt + 1) * g.plane;
+                    for (int32_t e = threadIdx.x; e < wgroups; e += blockDim.x)
+                        dst[e] = __builtin_bit_cast(uint4, widen(*reinterpret_cast<const uint2 *>(src + e * 8)));
+                }
                 float4v acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
-                const uint8_t *sb = stage + tt * g.plane + base;
 #pragma unroll
                 for (int s = 0; s < S; ++s)
                 {
-                    const half8 b = widen(*reinterpret_cast<const uint2 *>(sb + offs[s]));
-                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0][s], b, acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1][s], b, acc1, 0, 0, 0);
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0][s], b[s], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1][s], b[s], acc1, 0, 0, 0);
                 }
-                // epilogue: threshold, summaries, first channel maximum
-                float bv = -INFINITY;
+                // epilogue: threshold, summaries, first channel maximum.  The
+                // lane's channels ascend with (r, i), so strict '>' keeps the
+                // first maximum; padded channels have zero weights, so their
+                // value 0 never beats a real channel.  th >= 0, so a value
+                // fires exactly when it passes the threshold.
+                const uint32_t bit = 1u << tt;
+                float bv = -1.0f;
                 int32_t bi = INT32_MAX;
 #pragma unroll
                 for (int r = 0; r < 2; ++r)
@@ -199,34 +261,49 @@ __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ i
                     {
                         const int32_t o = (w * 2 + r) * 16 + 4 * q + i;
                         const float v = r == 0 ? acc0[i] : acc1[i];
-                        const float xp = v > g.th ? v : 0.0f;
-                        if (xp > 0.0f)
-                        {
-                            if (mask[r][i] == 0) first[r][i] = xp;
-                            mask[r][i] |= 1u << tt;
-                            mx[r][i] = mx[r][i] < xp ? xp : mx[r][i];
-                        }
-                        if (o < g.cout) combine(bv, bi, xp, o);
-                        if (pot_out && pvalid && o < g.cout)
-                            pot_out[((int64_t(n) * g.t + tt) * g.cout + o) * hw + p] = v;
+                        const bool f = v > g.th;
+                        const float xp = f ? v : 0.0f;
+                        first[r][i] = mask[r][i] == 0 ? xp : first[r][i];
+                        mask[r][i] = f ? (mask[r][i] | bit) : mask[r][i];
+                        mx[r][i] = fmaxf(mx[r][i], xp);
+                        const bool take = xp > bv;
+                        bv = take ? xp : bv;
+                        bi = take ? o : bi;
                     }
+                if (DIAG && pvalid)
+#pragma unroll
+                    for (int r = 0; r < 2; ++r)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                        {
+                            const int32_t o = (w * 2 + r) * 16 + 4 * q + i;
+                            if (o < g.cout)
+                                pot_out[((int64_t(n) * g.t + tt) * g.cout + o) * hw + p] = r == 0 ? acc0[i] : acc1[i];
+                        }
                 combine(bv, bi, __shfl_xor(bv, 16, 64), __shfl_xor(bi, 16, 64));
                 combine(bv, bi, __shfl_xor(bv, 32, 64), __shfl_xor(bi, 32, 64));
-                const int par = tt & 1;
-                if (lane < COLS) red_v[(par * MAX_WAVES + w) * COLS + lane] = bv, red_i[(par * MAX_WAVES + w) * COLS + lane] = bi;
-                __syncthreads();
-                if (w == 0 && lane < COLS)
-                {
-                    float m = red_v[(par * MAX_WAVES) * COLS + lane];
-                    int32_t c = red_i[(par * MAX_WAVES) * COLS + lane];
-                    for (int ow = 1; ow < g.waves; ++ow)
-                        combine(m, c, red_v[(par * MAX_WAVES + ow) * COLS + lane],
-                                red_i[(par * MAX_WAVES + ow) * COLS + lane]);
-                    ch_tab[tt * COLS + lane] = uint16_t(c);
-                    cnt += m > 0.0f;
-                }
+                if (lane < COLS)
+                    red_v[(tt * MAX_WAVES + w) * COLS + lane] = bv, red_i[(tt * MAX_WAVES + w) * COLS + lane] = bi;
+                __syncthreads();  // next window ready; this one free for step tt+2
             }
-            if (w == 0 && lane < COLS) cstar[lane] = ch_tab[(g.t - (cnt > 1 ? cnt : 1)) * COLS + lane];
+            // first channel maximum per (step, pixel) across the waves
+            for (int32_t e = threadIdx.x; e < g.t * COLS; e += blockDim.x)
+            {
+                const int32_t tt = e / COLS, col = e % COLS;
+                float m = red_v[(tt * MAX_WAVES) * COLS + col];
+                int32_t c = red_i[(tt * MAX_WAVES) * COLS + col];
+                for (int ow = 1; ow < g.waves; ++ow)
+                    combine(m, c, red_v[(tt * MAX_WAVES + ow) * COLS + col], red_i[(tt * MAX_WAVES + ow) * COLS + col]);
+                ch_tab[tt * COLS + col] = uint16_t(c);
+                red_v[(tt * MAX_WAVES) * COLS + col] = m;
+            }
+            __syncthreads();
+            // kept channel: first maximum at step T - max(cnt, 1)
+            if (int32_t(threadIdx.x) < COLS)
+            {
+                for (int32_t tt = 0; tt < g.t; ++tt) cnt += red_v[(tt * MAX_WAVES) * COLS + threadIdx.x] > 0.0f;
+                cstar[threadIdx.x] = ch_tab[(g.t - (cnt > 1 ? cnt : 1)) * COLS + threadIdx.x];
+            }
             __syncthreads();
             const int32_t cs = cstar[j];
             if (pvalid)
@@ -310,10 +387,28 @@ __global__ void __launch_bounds__(256) fused_write(const Summary *__restrict__ s
         act[p] = (e.mask >> step) & 1u ? int16_t(e.c) : int16_t(-1);
     }
     __syncthreads();
-    if (out)
+    const int64_t total_out = int64_t(cout) * hw;
+    if (out && hw % 4 == 0 && (reinterpret_cast<uintptr_t>(out + int64_t(blockIdx.x) * total_out) & 3) == 0)
+    {
+        // four pixels of one channel plane per 32-bit store, consecutive
+        // threads on consecutive pixels of the same plane
+        uint8_t *dst = out + int64_t(blockIdx.x) * total_out;
+        const int32_t quads = hw / 4;
+        for (int32_t e = threadIdx.x; e < quads * cout; e += blockDim.x)
+        {
+            const int32_t c = e / quads, qd = e % quads;
+            const uint2 a2 = *reinterpret_cast<const uint2 *>(act + 4 * qd);
+            const int32_t a0 = int16_t(a2.x & 0xFFFF), a1 = int16_t(a2.x >> 16);
+            const int32_t a3v = int16_t(a2.y >> 16), a2v = int16_t(a2.y & 0xFFFF);
+            const uint32_t word = uint32_t(a0 == c) | uint32_t(a1 == c) << 8 | uint32_t(a2v == c) << 16 |
+                                  uint32_t(a3v == c) << 24;
+            *reinterpret_cast<uint32_t *>(dst + int64_t(c) * hw + 4 * qd) = word;
+        }
+    }
+    else if (out)
     {
         // bytes [g0, g0 + total) of out, walked in 16-byte aligned chunks
-        const int64_t total = int64_t(cout) * hw;
+        const int64_t total = total_out;
         const int64_t g0 = int64_t(blockIdx.x) * total;
         const int64_t head = int64_t(reinterpret_cast<uintptr_t>(out + g0) & 15);
         for (int64_t c0 = int64_t(threadIdx.x) * 16 - head; c0 < total; c0 += int64_t(blockDim.x) * 16)
@@ -369,14 +464,14 @@ KernelTimer &timer()
     return *t;
 }
 
-template <int S>
+template <int S, bool DIAG>
 void launch_layer(const uint8_t *in, const half8 *frag, Summary *summary, float *pot, const FusedGeom &g, size_t lds,
                   hipStream_t s)
 {
     static bool configured = false;  // per instantiation; the attribute is per function
     if (!configured)
     {
-        SPK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&fused_layer<S>),
+        SPK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&fused_layer<S, DIAG>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds > 65536 ? 163840 : 65536)));
         configured = true;
     }
@@ -392,7 +487,7 @@ void launch_layer(const uint8_t *in, const half8 *frag, Summary *summary, float 
         SPK_HIP(hipEventCreate(&e1));
         SPK_HIP(hipEventRecord(e0, s));
     }
-    fused_layer<S><<<dim3(grid), dim3(64 * g.waves), lds, s>>>(in, frag, summary, pot, g);
+    fused_layer<S, DIAG><<<dim3(grid), dim3(64 * g.waves), lds, s>>>(in, frag, summary, pot, g);
     SPK_LAUNCHED();
     if (tm.on)
     {
@@ -407,7 +502,9 @@ void dispatch_layer(int steps, const uint8_t *in, const half8 *frag, Summary *su
     switch (steps)
     {
 #define CASE(k) \
-    case k: return launch_layer<k>(in, frag, summary, pot, g, lds, s);
+    case k:                                                                                 \
+        return pot ? launch_layer<k, true>(in, frag, summary, pot, g, lds, s)                 \
+                   : launch_layer<k, false>(in, frag, summary, pot, g, lds, s);
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
         CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
 #undef CASE
@@ -487,10 +584,14 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
         g.steps = int32_t(cdiv(g.groups, 4));
         g.waves = int32_t(cdiv(cout, 32));
         g.tiles = int32_t(cdiv(ho * wo, COLS));
-        g.plane = int32_t(cdiv(int64_t(g.ph) * g.pw * g.cin_pad + 8, 16) * 16);  // + slack for the last tap read
+        g.plane = int32_t(cdiv(int64_t(g.ph) * g.pw * g.cin_pad, 16) * 16);
+        // a tile of COLS row-major pixels spans at most ceil((COLS-1)/wo)+1 rows
+        const int64_t win_rows = cdiv(COLS - 1, wo) + kh;
+        g.win_units = int32_t(win_rows * g.pw * (g.cin_pad / 8));
         g.th = float(threshold);
         SPK_ARG(g.steps <= MAX_STEPS, "Fused layer: Cin*Kh*Kw too large (K <= 512 after padding).");
-        const size_t lds = size_t(g.t) * g.plane + 2 * MAX_WAVES * COLS * 8 + COLS * 4 + size_t(g.t) * COLS * 2;
+        const size_t lds = size_t(g.t) * g.plane + size_t(2) * g.win_units * 16 +
+                           size_t(g.t) * MAX_WAVES * COLS * 8 + COLS * 4 + size_t(g.t) * COLS * 2;
         SPK_ARG(lds <= 163840, "Fused layer: the staged sample (" << lds << " bytes) exceeds the 160 KiB LDS.");
         SPK_ARG(int64_t(cout) * ho * wo < (1LL << 31), "Fused layer: output plane too large.");
         SPK_ARG(ho * wo <= 8192, "Fused layer: convwta supports at most 8192 output pixels per sample.");

@@ -27,10 +27,8 @@ namespace spk
 {
 namespace
 {
-constexpr int APPLY_THREADS = 256;
-constexpr int MAX_WEIGHTS_PER_THREAD = 4;
-constexpr int SLICE = APPLY_THREADS * MAX_WEIGHTS_PER_THREAD;  // weights per block
-constexpr int SLICE_WORDS = SLICE / 32;
+constexpr int APPLY_THREADS = 1024;
+constexpr int SLICE = APPLY_THREADS;  // weights per block, one per thread
 
 __device__ __forceinline__ float stdp_update(float w, bool potentiate, const spk_stdp_config &cfg)
 {
@@ -57,22 +55,19 @@ __device__ __forceinline__ void store_bits(uint32_t *rec, int32_t words, int32_t
 __global__ void __launch_bounds__(256)
     conv_records(const uint8_t *__restrict__ in, const uint8_t *__restrict__ out, StdpGeom g,
                  const spk_winner *__restrict__ win, const int64_t *__restrict__ counts,
-                 uint32_t *__restrict__ bits, int32_t *__restrict__ flag)
+                 uint32_t *__restrict__ bits, int2 *__restrict__ slot, int32_t *__restrict__ flag)
 {
     const int64_t e = blockIdx.x, s = e / g.maxcount, k = e % g.maxcount;
-    if (k >= counts[s]) return;
-    const spk_winner w = win[e];
-    if (w.c < 0) return;
-    if (w.c >= g.ncfg)
+    const spk_winner w = k < counts[s] ? win[e] : spk_winner{-1, 0, 0, 0, 0};
+    int32_t bad = 0;
+    if (w.c >= g.ncfg) bad = 1;
+    else if (w.c >= 0 && (w.z < 0 || w.z >= g.cout || w.y < 0 || w.y >= g.ho || w.x < 0 || w.x >= g.wo)) bad = 2;
+    if (threadIdx.x == 0)
     {
-        if (threadIdx.x == 0) atomicMax(flag, 1);
-        return;
+        slot[e] = make_int2(w.c >= 0 && !bad ? int32_t(w.z) : -1, int32_t(w.c));
+        if (bad) atomicMax(flag, bad);
     }
-    if (w.z < 0 || w.z >= g.cout || w.y < 0 || w.y >= g.ho || w.x < 0 || w.x >= g.wo)
-    {
-        if (threadIdx.x == 0) atomicMax(flag, 2);
-        return;
-    }
+    if (w.c < 0 || bad) return;
     const int64_t out_plane = int64_t(g.ho) * g.wo, in_plane = int64_t(g.h) * g.w, in_step = g.cin * in_plane;
     const uint8_t *os = out + (s * g.t * g.cout + w.z) * out_plane + w.y * g.wo + w.x;
     int32_t osum = g.t;
@@ -114,22 +109,19 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(256)
     fc_records(const uint8_t *__restrict__ in, const uint8_t *__restrict__ out, StdpGeom g,
                const spk_winner *__restrict__ win, const int64_t *__restrict__ counts, uint32_t *__restrict__ bits,
-               int32_t *__restrict__ flag)
+               int2 *__restrict__ slot, int32_t *__restrict__ flag)
 {
     const int64_t e = blockIdx.x, s = e / g.maxcount, k = e % g.maxcount;
-    if (k >= counts[s]) return;
-    const spk_winner w = win[e];
-    if (w.c < 0) return;
-    if (w.c >= g.ncfg)
+    const spk_winner w = k < counts[s] ? win[e] : spk_winner{-1, 0, 0, 0, 0};
+    int32_t bad = 0;
+    if (w.c >= g.ncfg) bad = 1;
+    else if (w.c >= 0 && (w.x < 0 || w.x >= g.cout)) bad = 2;
+    if (threadIdx.x == 0)
     {
-        if (threadIdx.x == 0) atomicMax(flag, 1);
-        return;
+        slot[e] = make_int2(w.c >= 0 && !bad ? int32_t(w.x) : -1, int32_t(w.c));
+        if (bad) atomicMax(flag, bad);
     }
-    if (w.x < 0 || w.x >= g.cout)
-    {
-        if (threadIdx.x == 0) atomicMax(flag, 2);
-        return;
-    }
+    if (w.c < 0 || bad) return;
     const uint8_t *os = out + s * g.t * int64_t(g.cout) + w.x;
     int32_t osum = g.t;
     for (int32_t i = 0; i < g.t; ++i)
@@ -161,14 +153,15 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// One block per (output neuron, slice of SLICE weights); key = winner.z
-// (conv) or winner.x (FC).
+// One block per (output neuron, slice of SLICE weights): scans the slot
+// keys written by the record kernel (winner.z for conv, winner.x for FC),
+// stages the bits of the matching winners of each chunk in LDS, and applies
+// them in slot order -- (sample, rank) order -- with each weight in a register.
 __global__ void __launch_bounds__(APPLY_THREADS)
-    apply_records(float *__restrict__ kernel, StdpGeom g, StdpConfigs cfg, const spk_winner *__restrict__ win,
-                  const int64_t *__restrict__ counts, const uint32_t *__restrict__ bits,
-                  const int32_t *__restrict__ flag, int32_t by_x)
+    apply_records(float *__restrict__ kernel, StdpGeom g, StdpConfigs cfg, const int2 *__restrict__ slot,
+                  const uint32_t *__restrict__ bits, const int32_t *__restrict__ flag, int32_t by_x)
 {
-    __shared__ uint32_t lds_bits[APPLY_THREADS * SLICE_WORDS];
+    extern __shared__ uint32_t lds_bits[];  // [APPLY_THREADS][sw]
     __shared__ int32_t list[APPLY_THREADS];
     __shared__ int32_t cfgidx[APPLY_THREADS];
     __shared__ int32_t wave_count[APPLY_THREADS / 64];
@@ -177,67 +170,42 @@ __global__ void __launch_bounds__(APPLY_THREADS)
     const int32_t ksz = by_x ? g.cin : g.cin * g.kh * g.kw, words = (ksz + 31) / 32;
     const int32_t nw = min(SLICE, ksz - w0), sw = (nw + 31) / 32, word0 = w0 / 32;
     float *kz = kernel + int64_t(key) * ksz + w0;
-
-    float wr[MAX_WEIGHTS_PER_THREAD];
-#pragma unroll
-    for (int j = 0; j < MAX_WEIGHTS_PER_THREAD; ++j)
-    {
-        const int32_t i = threadIdx.x + j * APPLY_THREADS;
-        wr[j] = i < nw ? kz[i] : 0.0f;
-    }
+    const int32_t i = threadIdx.x;
+    float wr = i < nw ? kz[i] : 0.0f;
 
     const int64_t total = g.n * g.maxcount;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int64_t e0 = 0; e0 < total; e0 += APPLY_THREADS)
     {
-        // order-preserving compaction of this chunk's winners that name `key`
+        // order-preserving compaction of this chunk's slots that name `key`
         const int64_t e = e0 + threadIdx.x;
-        bool match = false;
-        int32_t c = 0;
-        if (e < total && e % g.maxcount < counts[e / g.maxcount])
-        {
-            const spk_winner w = win[e];
-            match = w.c >= 0 && (by_x ? w.x : w.z) == key;
-            c = int32_t(w.c);
-        }
+        const int2 sl = e < total ? slot[e] : make_int2(-1, 0);
+        const bool match = sl.x == key;
         const uint64_t ballot = __ballot(match);
         if (lane == 0) wave_count[wave] = __popcll(ballot);
         __syncthreads();
         int32_t base = 0, m = 0;
-        for (int i = 0; i < APPLY_THREADS / 64; ++i)
+        for (int v = 0; v < APPLY_THREADS / 64; ++v)
         {
-            if (i < wave) base += wave_count[i];
-            m += wave_count[i];
+            if (v < wave) base += wave_count[v];
+            m += wave_count[v];
         }
         if (match)
         {
-            const int32_t slot = base + __popcll(ballot & ((1ull << lane) - 1));
-            list[slot] = int32_t(threadIdx.x);
-            cfgidx[slot] = c;
+            const int32_t at = base + __popcll(ballot & ((1ull << lane) - 1));
+            list[at] = int32_t(threadIdx.x);
+            cfgidx[at] = sl.y;
         }
         __syncthreads();
         for (int32_t q = threadIdx.x; q < m * sw; q += APPLY_THREADS)
             lds_bits[q] = bits[(e0 + list[q / sw]) * words + word0 + q % sw];
         __syncthreads();
-        for (int32_t q = 0; q < m; ++q)
-        {
-            const spk_stdp_config &cf = cfg.c[cfgidx[q]];
-            const uint32_t *b = lds_bits + q * sw;
-#pragma unroll
-            for (int j = 0; j < MAX_WEIGHTS_PER_THREAD; ++j)
-            {
-                const int32_t i = threadIdx.x + j * APPLY_THREADS;
-                if (i < nw) wr[j] = stdp_update(wr[j], (b[i >> 5] >> (i & 31)) & 1u, cf);
-            }
-        }
+        if (i < nw)
+            for (int32_t q = 0; q < m; ++q)
+                wr = stdp_update(wr, (lds_bits[q * sw + (i >> 5)] >> (i & 31)) & 1u, cfg.c[cfgidx[q]]);
         __syncthreads();
     }
-#pragma unroll
-    for (int j = 0; j < MAX_WEIGHTS_PER_THREAD; ++j)
-    {
-        const int32_t i = threadIdx.x + j * APPLY_THREADS;
-        if (i < nw) kz[i] = wr[j];
-    }
+    if (i < nw) kz[i] = wr;
 }
 
 // Reports a rejection found on the device (synchronises the stream).
@@ -263,10 +231,16 @@ StdpConfigs pack_configs(const spk_stdp_config *configs, int64_t nconfigs)
     return c;
 }
 
-int64_t stdp_bits_bytes(const StdpGeom &g, bool fc)
+// bits [slots][words] (u32, rounded to an even count) then slot keys [slots] (int2)
+static int64_t bits_words(const StdpGeom &g, bool fc)
 {
     const int64_t ksz = fc ? g.cin : int64_t(g.cin) * g.kh * g.kw;
-    return g.n * g.maxcount * ((ksz + 31) / 32) * int64_t(sizeof(uint32_t));
+    return (g.n * g.maxcount * ((ksz + 31) / 32) + 1) / 2 * 2;
+}
+
+int64_t stdp_bits_bytes(const StdpGeom &g, bool fc)
+{
+    return bits_words(g, fc) * int64_t(sizeof(uint32_t)) + g.n * g.maxcount * int64_t(sizeof(int2));
 }
 
 void stdp_launch(const uint8_t *in, float *kernel, const uint8_t *out, const StdpGeom &g, const StdpConfigs &cfg,
@@ -277,13 +251,19 @@ void stdp_launch(const uint8_t *in, float *kernel, const uint8_t *out, const Std
     SPK_ARG(ksz < (1LL << 30), "STDP: kernel too large.");
     const int64_t slots = g.n * g.maxcount;
     SPK_ARG(slots < (1LL << 31), "STDP: too many winner slots.");
+    int2 *slot = reinterpret_cast<int2 *>(bits + bits_words(g, fc));
     if (fc)
-        fc_records<<<dim3(unsigned(slots)), 256, 0, s>>>(in, out, g, winners, counts, bits, flag);
+        fc_records<<<dim3(unsigned(slots)), 256, 0, s>>>(in, out, g, winners, counts, bits, slot, flag);
     else
-        conv_records<<<dim3(unsigned(slots)), 256, 0, s>>>(in, out, g, winners, counts, bits, flag);
+        conv_records<<<dim3(unsigned(slots)), 256, 0, s>>>(in, out, g, winners, counts, bits, slot, flag);
     SPK_LAUNCHED();
-    apply_records<<<dim3(unsigned(g.cout), unsigned(cdiv(ksz, SLICE))), APPLY_THREADS, 0, s>>>(
-        kernel, g, cfg, winners, counts, bits, flag, fc ? 1 : 0);
+    const int64_t sw = (ksz < SLICE ? (ksz + 31) / 32 : SLICE / 32);
+    const size_t lds = size_t(APPLY_THREADS) * sw * sizeof(uint32_t);
+    if (lds > 65536)
+        SPK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&apply_records),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    apply_records<<<dim3(unsigned(g.cout), unsigned(cdiv(ksz, SLICE))), APPLY_THREADS, lds, s>>>(
+        kernel, g, cfg, slot, bits, flag, fc ? 1 : 0);
     SPK_LAUNCHED();
 }
 }  // namespace spk
