@@ -59,8 +59,9 @@ struct FusedGeom
 {
     int32_t n, t, cin, h, w, cout, kh, kw, pt, pl, ho, wo;
     int32_t ph, pw, cin_pad, groups, steps, waves, tiles;
-    int32_t plane;      // bytes of one staged step
-    int32_t win_units;  // 16-byte units of one FP16 window buffer
+    int32_t stage_bytes;  // bytes of the staged sample ([t][h][w][cin_pad] U8, 16-aligned)
+    int32_t win_units;    // 16-byte units of one step's FP16 window
+    int32_t tc;           // steps whose windows are widened together
     float th;
 };
 
@@ -113,25 +114,49 @@ __device__ __forceinline__ void combine(float &v, int32_t &i, float ov, int32_t 
 }
 
 // Writes `count` consecutive input bytes, starting at flat index i of the
-// sample's [t][cin][h][w] tensor, to their [t][y+pt][x+pl][c] LDS slots.
+// sample's [t][cin][h][w] tensor, to their [t][y][x][c] slots of the
+// (unpadded, channel-innermost) U8 stage.
 __device__ __forceinline__ void stage_bytes(uint8_t *stage, const FusedGeom &g, int32_t i, int count, uint32_t v,
                                             int32_t chw, int32_t hwi)
 {
     int32_t tt = i / chw, r = i - tt * chw, c = r / hwi, pix = r - c * hwi;
-    int32_t y = pix / g.w, x = pix - y * g.w;
     for (int b = 0; b < count; ++b)
     {
-        stage[tt * g.plane + ((y + g.pt) * g.pw + x + g.pl) * g.cin_pad + c] = uint8_t(v >> (8 * b));
-        if (++x == g.w)
+        stage[(tt * hwi + pix) * g.cin_pad + c] = uint8_t(v >> (8 * b));
+        if (++pix == hwi)
         {
-            x = 0;
-            if (++y == g.h)
-            {
-                y = 0;
-                if (++c == g.cin) c = 0, ++tt;
-            }
+            pix = 0;
+            if (++c == g.cin) c = 0, ++tt;
         }
     }
+}
+
+// Epilogue of one step for one lane: threshold, the per-(channel, pixel)
+// summaries, and the lane's first channel maximum.  The lane's channels
+// ascend with (r, i), so strict '>' keeps the first maximum; padded channels
+// have zero weights, so their value 0 never beats a real channel.  th >= 0,
+// so a value fires exactly when it passes the threshold.
+__device__ __forceinline__ void epilogue(const float4v &acc0, const float4v &acc1, int32_t tt, float th,
+                                         int32_t o0, uint32_t (&mask)[2][4], float (&first)[2][4],
+                                         float (&mx)[2][4], float &bv, int32_t &bi)
+{
+    const uint32_t bit = 1u << tt;
+    bv = -1.0f, bi = INT32_MAX;
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+        {
+            const float v = r == 0 ? acc0[i] : acc1[i];
+            const bool f = v > th;
+            const float xp = f ? v : 0.0f;
+            first[r][i] = mask[r][i] == 0 ? xp : first[r][i];
+            mask[r][i] = f ? (mask[r][i] | bit) : mask[r][i];
+            mx[r][i] = mx[r][i] < xp ? xp : mx[r][i];
+            const bool take = xp > bv;
+            bv = take ? xp : bv;
+            bi = take ? o0 + r * 16 + i : bi;
+        }
 }
 
 template <int S, bool DIAG>
@@ -140,16 +165,16 @@ __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ i
                                                    FusedGeom g)
 {
     extern __shared__ __align__(16) uint8_t lds[];
-    uint8_t *stage = lds;                                             // [t][ph][pw][cin_pad] U8
-    uint4 *win16 = reinterpret_cast<uint4 *>(lds + g.t * g.plane);   // [2][rows][pw][cin_pad] FP16
-    // per (step, wave, column) partial channel maximum; combined once per tile
-    float *red_v = reinterpret_cast<float *>(win16 + 2 * g.win_units);  // [t][MAX_WAVES][COLS]
+    uint8_t *stage = lds;                                            // [t][h][w][cin_pad] U8
+    uint4 *win16 = reinterpret_cast<uint4 *>(lds + g.stage_bytes);   // [tc][rows][pw][cin_pad] FP16
+    float *red_v = reinterpret_cast<float *>(win16 + g.tc * g.win_units);  // [t][MAX_WAVES][COLS]
     int32_t *red_i = reinterpret_cast<int32_t *>(red_v + g.t * MAX_WAVES * COLS);
-    int32_t *cstar = red_i + g.t * MAX_WAVES * COLS;                   // [COLS]
-    uint16_t *ch_tab = reinterpret_cast<uint16_t *>(cstar + COLS);     // [t][COLS]
+    int32_t *cstar = red_i + g.t * MAX_WAVES * COLS;                  // [COLS]
+    uint16_t *ch_tab = reinterpret_cast<uint16_t *>(cstar + COLS);    // [t][COLS]
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, q = lane >> 4, j = lane & 15;
-    const int32_t hw = g.ho * g.wo;
+    const int32_t hw = g.ho * g.wo, hwi = g.h * g.w, cg = g.cin_pad / 8;
+    const int32_t o0 = w * 32 + 4 * q;  // the lane's first channel; rows r*16 + i follow
 
     // weight fragments and tap offsets, resident for the whole kernel
     half8 a[2][S];
@@ -159,29 +184,26 @@ __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ i
     {
         a[0][s] = frag[((w * 2 + 0) * S + s) * 64 + lane];
         a[1][s] = frag[((w * 2 + 1) * S + s) * 64 + lane];
-        const int32_t grp = 4 * s + q, cg = g.cin_pad / 8;
+        const int32_t grp = 4 * s + q;
         const int32_t tap = grp < g.groups ? grp / cg : 0, c8 = grp < g.groups ? grp % cg : 0;
-        offs[s] = (((tap / g.kw) * g.pw + tap % g.kw) * g.cin_pad + c8 * 8) * 2;  // bytes in the FP16 window
+        offs[s] = (((tap / g.kw) * g.pw + tap % g.kw) * g.cin_pad + c8 * 8) * 2;  // bytes in a window
     }
+    const int32_t win_step = g.win_units * 16;  // bytes of one step's window
 
     for (int32_t n = blockIdx.x; n < g.n; n += gridDim.x)
     {
         __syncthreads();
-        // stage [t][ph][pw][cin_pad]: zero everything, then copy the image
+        // stage the sample as [t][y][x][cin_pad] bytes: zero (for the padded
+        // channels), then scatter the [t][cin][h][w] input with dword loads
         {
             uint4 *z = reinterpret_cast<uint4 *>(stage);
-            const int32_t words = g.t * g.plane / 16;
-            for (int32_t i = threadIdx.x; i < words; i += blockDim.x) z[i] = make_uint4(0, 0, 0, 0);
+            for (int32_t i = threadIdx.x; i < g.stage_bytes / 16; i += blockDim.x) z[i] = make_uint4(0, 0, 0, 0);
         }
         __syncthreads();
         {
-            // the sample is t*cin*h*w contiguous bytes: read it as dwords, U per
-            // thread in flight, then scatter each byte to its channel-innermost
-            // slot
             constexpr int U = 8;
-            const uint8_t *src = in + int64_t(n) * g.t * g.cin * g.h * g.w;
-            const int32_t hwi = g.h * g.w, chw = g.cin * hwi, total = g.t * chw;
-            const int32_t stride = int32_t(blockDim.x) * 4;
+            const uint8_t *src = in + int64_t(n) * g.t * g.cin * hwi;
+            const int32_t chw = g.cin * hwi, total = g.t * chw, stride = int32_t(blockDim.x) * 4;
             if ((reinterpret_cast<uintptr_t>(src) & 3) == 0 && (total & 3) == 0)
                 for (int32_t i0 = int32_t(threadIdx.x) * 4; i0 < total; i0 += stride * U)
                 {
@@ -191,10 +213,7 @@ __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ i
                         v[u] = *reinterpret_cast<const uint32_t *>(src + min(i0 + u * stride, total - 4));
 #pragma unroll
                     for (int u = 0; u < U; ++u)
-                    {
-                        const int32_t i = i0 + u * stride;
-                        if (i < total) stage_bytes(stage, g, i, 4, v[u], chw, hwi);
-                    }
+                        if (i0 + u * stride < total) stage_bytes(stage, g, i0 + u * stride, 4, v[u], chw, hwi);
                 }
             else
                 for (int32_t i = int32_t(threadIdx.x); i < total; i += int32_t(blockDim.x))
@@ -207,84 +226,100 @@ __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ i
             const int32_t p = tile * COLS + j;
             const bool pvalid = p < hw;
             const int32_t pc = pvalid ? p : hw - 1, y = pc / g.wo, x = pc % g.wo;
-            // the tile's input window: whole padded rows [y_first, y_first + rows)
+            // the tile's window: padded rows [y_first, y_first + rows), all padded columns
             const int32_t y_first = (tile * COLS) / g.wo;
             const int32_t rows = min(tile * COLS + COLS - 1, hw - 1) / g.wo - y_first + g.kh;
-            const int32_t wgroups = rows * g.pw * (g.cin_pad / 8);
-            const uint8_t *wsrc = stage + y_first * g.pw * g.cin_pad;
-            const int32_t base = ((y - y_first) * g.pw + x) * g.cin_pad * 2;  // bytes in the FP16 window
+            const int32_t base = ((y - y_first) * g.pw + x) * g.cin_pad * 2;
             uint32_t mask[2][4];
             float first[2][4], mx[2][4];
 #pragma unroll
             for (int r = 0; r < 2; ++r)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) mask[r][i] = 0, first[r][i] = 0.0f, mx[r][i] = 0.0f;
-            int32_t cnt = 0;
 
-            // widen step 0's window once for the whole block
-            for (int32_t e = threadIdx.x; e < wgroups; e += blockDim.x)
-                win16[e] = __builtin_bit_cast(uint4, widen(*reinterpret_cast<const uint2 *>(wsrc + e * 8)));
-            __syncthreads();
-            for (int32_t tt = 0; tt < g.t; ++tt)
+            for (int32_t t0 = 0; t0 < g.t; t0 += g.tc)
             {
-                const uint8_t *wb = reinterpret_cast<const uint8_t *>(win16 + (tt & 1) * g.win_units) + base;
-                half8 b[S];
-#pragma unroll
-                for (int s = 0; s < S; ++s) b[s] = *reinterpret_cast<const half8 *>(wb + offs[s]);
-                // widen step tt+1's window into the other buffer meanwhile
-                if (tt + 1 < g.t)
+                const int32_t tc = min(g.tc, g.t - t0);
+                // widen the windows of steps [t0, t0 + tc) once for the block;
+                // a thread owns one (column, channel group) and walks the
+                // (step, row) pairs; positions in the zero pad are written 0
                 {
-                    uint4 *dst = win16 + ((tt + 1) & 1) * g.win_units;
-                    const uint8_t *src = wsrc + (tt + 1) * g.plane;
-                    for (int32_t e = threadIdx.x; e < wgroups; e += blockDim.x)
-                        dst[e] = __builtin_bit_cast(uint4, widen(*reinterpret_cast<const uint2 *>(src + e * 8)));
-                }
-                float4v acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
-#pragma unroll
-                for (int s = 0; s < S; ++s)
-                {
-                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0][s], b[s], acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1][s], b[s], acc1, 0, 0, 0);
-                }
-                // epilogue: threshold, summaries, first channel maximum.  The
-                // lane's channels ascend with (r, i), so strict '>' keeps the
-                // first maximum; padded channels have zero weights, so their
-                // value 0 never beats a real channel.  th >= 0, so a value
-                // fires exactly when it passes the threshold.
-                const uint32_t bit = 1u << tt;
-                float bv = -1.0f;
-                int32_t bi = INT32_MAX;
-#pragma unroll
-                for (int r = 0; r < 2; ++r)
-#pragma unroll
-                    for (int i = 0; i < 4; ++i)
+                    const int32_t units = g.pw * cg, lanes_per = int32_t(blockDim.x) / units;
+                    const int32_t cu = int32_t(threadIdx.x) % units, slot = int32_t(threadIdx.x) / units;
+                    const int32_t col = cu / cg, g8 = cu % cg, ix = col - g.pl;
+                    if (slot < lanes_per)
                     {
-                        const int32_t o = (w * 2 + r) * 16 + 4 * q + i;
-                        const float v = r == 0 ? acc0[i] : acc1[i];
-                        const bool f = v > g.th;
-                        const float xp = f ? v : 0.0f;
-                        first[r][i] = mask[r][i] == 0 ? xp : first[r][i];
-                        mask[r][i] = f ? (mask[r][i] | bit) : mask[r][i];
-                        mx[r][i] = fmaxf(mx[r][i], xp);
-                        const bool take = xp > bv;
-                        bv = take ? xp : bv;
-                        bi = take ? o : bi;
-                    }
-                if (DIAG && pvalid)
-#pragma unroll
-                    for (int r = 0; r < 2; ++r)
-#pragma unroll
-                        for (int i = 0; i < 4; ++i)
+                        int32_t ts = slot / rows, r = slot % rows;
+                        for (int32_t e = slot; e < tc * rows; e += lanes_per)
                         {
-                            const int32_t o = (w * 2 + r) * 16 + 4 * q + i;
-                            if (o < g.cout)
-                                pot_out[((int64_t(n) * g.t + tt) * g.cout + o) * hw + p] = r == 0 ? acc0[i] : acc1[i];
+                            const int32_t iy = y_first + r - g.pt;
+                            uint2 raw = make_uint2(0, 0);
+                            if (iy >= 0 && iy < g.h && ix >= 0 && ix < g.w)
+                                raw = *reinterpret_cast<const uint2 *>(
+                                    stage + (((t0 + ts) * g.h + iy) * g.w + ix) * g.cin_pad + g8 * 8);
+                            win16[ts * g.win_units + (r * g.pw + col) * cg + g8] =
+                                __builtin_bit_cast(uint4, widen(raw));
+                            r += lanes_per;
+                            while (r >= rows) r -= rows, ++ts;
                         }
-                combine(bv, bi, __shfl_xor(bv, 16, 64), __shfl_xor(bi, 16, 64));
-                combine(bv, bi, __shfl_xor(bv, 32, 64), __shfl_xor(bi, 32, 64));
-                if (lane < COLS)
-                    red_v[(tt * MAX_WAVES + w) * COLS + lane] = bv, red_i[(tt * MAX_WAVES + w) * COLS + lane] = bi;
-                __syncthreads();  // next window ready; this one free for step tt+2
+                    }
+                }
+                __syncthreads();
+                // steps t0 .. t0+tc-1 without barriers; the MFMAs of step
+                // tt+1 are issued before the epilogue of step tt
+                const uint8_t *wb = reinterpret_cast<const uint8_t *>(win16) + base;
+                float4v acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+                {
+                    half8 b[S];
+#pragma unroll
+                    for (int s = 0; s < S; ++s) b[s] = *reinterpret_cast<const half8 *>(wb + offs[s]);
+#pragma unroll
+                    for (int s = 0; s < S; ++s)
+                    {
+                        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0][s], b[s], acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1][s], b[s], acc1, 0, 0, 0);
+                    }
+                }
+                for (int32_t k = 0; k < tc; ++k)
+                {
+                    const int32_t tt = t0 + k;
+                    // one basic block per step: the next step's MFMAs (the last
+                    // step of a chunk recomputes itself, discarded) interleave
+                    // with this step's epilogue
+                    float4v nx0 = {0, 0, 0, 0}, nx1 = {0, 0, 0, 0};
+                    {
+                        const uint8_t *nb = wb + min(k + 1, tc - 1) * win_step;
+                        half8 b[S];
+#pragma unroll
+                        for (int s = 0; s < S; ++s) b[s] = *reinterpret_cast<const half8 *>(nb + offs[s]);
+#pragma unroll
+                        for (int s = 0; s < S; ++s)
+                        {
+                            nx0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0][s], b[s], nx0, 0, 0, 0);
+                            nx1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1][s], b[s], nx1, 0, 0, 0);
+                        }
+                    }
+                    float bv;
+                    int32_t bi;
+                    epilogue(acc0, acc1, tt, g.th, o0, mask, first, mx, bv, bi);
+                    if (DIAG && pvalid)
+#pragma unroll
+                        for (int r = 0; r < 2; ++r)
+#pragma unroll
+                            for (int i = 0; i < 4; ++i)
+                            {
+                                const int32_t o = o0 + r * 16 + i;
+                                if (o < g.cout)
+                                    pot_out[((int64_t(n) * g.t + tt) * g.cout + o) * hw + p] =
+                                        r == 0 ? acc0[i] : acc1[i];
+                            }
+                    combine(bv, bi, __shfl_xor(bv, 16, 64), __shfl_xor(bi, 16, 64));
+                    combine(bv, bi, __shfl_xor(bv, 32, 64), __shfl_xor(bi, 32, 64));
+                    if (lane < COLS)
+                        red_v[(tt * MAX_WAVES + w) * COLS + lane] = bv, red_i[(tt * MAX_WAVES + w) * COLS + lane] = bi;
+                    acc0 = nx0, acc1 = nx1;
+                }
+                __syncthreads();  // windows free for the next chunk; partials visible
             }
             // first channel maximum per (step, pixel) across the waves
             for (int32_t e = threadIdx.x; e < g.t * COLS; e += blockDim.x)
@@ -301,6 +336,7 @@ __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ i
             // kept channel: first maximum at step T - max(cnt, 1)
             if (int32_t(threadIdx.x) < COLS)
             {
+                int32_t cnt = 0;
                 for (int32_t tt = 0; tt < g.t; ++tt) cnt += red_v[(tt * MAX_WAVES) * COLS + threadIdx.x] > 0.0f;
                 cstar[threadIdx.x] = ch_tab[(g.t - (cnt > 1 ? cnt : 1)) * COLS + threadIdx.x];
             }
@@ -311,7 +347,7 @@ __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ i
                 for (int r = 0; r < 2; ++r)
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
-                        if ((w * 2 + r) * 16 + 4 * q + i == cs)
+                        if (o0 + r * 16 + i == cs)
                             summary[int64_t(n) * hw + p] = Summary{cs, mask[r][i], first[r][i], mx[r][i]};
         }
     }
@@ -393,16 +429,21 @@ __global__ void __launch_bounds__(256) fused_write(const Summary *__restrict__ s
         // four pixels of one channel plane per 32-bit store, consecutive
         // threads on consecutive pixels of the same plane
         uint8_t *dst = out + int64_t(blockIdx.x) * total_out;
-        const int32_t quads = hw / 4;
-        for (int32_t e = threadIdx.x; e < quads * cout; e += blockDim.x)
+        const int32_t quads = hw / 4, lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+        for (int32_t qd = lane; qd < quads; qd += 64)
         {
-            const int32_t c = e / quads, qd = e % quads;
+            // the lane's four pixels keep their active channels in registers
+            // while the wave walks the channel planes
             const uint2 a2 = *reinterpret_cast<const uint2 *>(act + 4 * qd);
             const int32_t a0 = int16_t(a2.x & 0xFFFF), a1 = int16_t(a2.x >> 16);
-            const int32_t a3v = int16_t(a2.y >> 16), a2v = int16_t(a2.y & 0xFFFF);
-            const uint32_t word = uint32_t(a0 == c) | uint32_t(a1 == c) << 8 | uint32_t(a2v == c) << 16 |
-                                  uint32_t(a3v == c) << 24;
-            *reinterpret_cast<uint32_t *>(dst + int64_t(c) * hw + 4 * qd) = word;
+            const int32_t a2v = int16_t(a2.y & 0xFFFF), a3v = int16_t(a2.y >> 16);
+            uint8_t *d = dst + 4 * qd;
+            for (int32_t c = wave; c < cout; c += waves)
+            {
+                const uint32_t word = uint32_t(a0 == c) | uint32_t(a1 == c) << 8 | uint32_t(a2v == c) << 16 |
+                                      uint32_t(a3v == c) << 24;
+                *reinterpret_cast<uint32_t *>(d + int64_t(c) * hw) = word;
+            }
         }
     }
     else if (out)
@@ -433,20 +474,46 @@ __global__ void __launch_bounds__(256) fused_write(const Summary *__restrict__ s
     }
     if (pooled)
     {
-        const int32_t phw = pho * pwo, total = cout * phw;
+        const int32_t phw = pho * pwo, total = cout * phw, win = pkh * pkw;
         uint8_t *dst = pooled + int64_t(blockIdx.x) * total;
-        for (int32_t i = threadIdx.x; i < total; i += blockDim.x)
+        if (win <= 16)
         {
-            const int32_t c = i / phw, r = i % phw, py = r / pwo, px = r % pwo;
-            uint8_t m = 0;
-            for (int32_t a = 0; a < pkh; ++a)
-                for (int32_t b = 0; b < pkw; ++b)
+            // each thread owns one pooled position (its window's active
+            // channels in registers) and a strided set of channels
+            const int32_t P = min(phw, int32_t(blockDim.x)), C = int32_t(blockDim.x) / P;
+            const int32_t clane = int32_t(threadIdx.x) / P;
+            for (int32_t pos = int32_t(threadIdx.x) % P; clane < C && pos < phw; pos += P)
+            {
+                const int32_t py = pos / pwo, px = pos % pwo;
+                int32_t wa[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k)
                 {
-                    const int32_t y = py * psh + a, x = px * psw + b;
-                    if (y < ho && x < wo && act[y * wo + x] == c) m = 1;
+                    const int32_t y = py * psh + k / pkw, x = px * psw + k % pkw;
+                    wa[k] = (k < win && y < ho && x < wo) ? act[y * wo + x] : -1;
                 }
-            dst[i] = m;
+                for (int32_t c = clane; c < cout; c += C)
+                {
+                    bool m = false;
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) m |= wa[k] == c;
+                    dst[int64_t(c) * phw + pos] = m;
+                }
+            }
         }
+        else
+            for (int32_t i = threadIdx.x; i < total; i += blockDim.x)
+            {
+                const int32_t c = i / phw, r = i % phw, py = r / pwo, px = r % pwo;
+                uint8_t m = 0;
+                for (int32_t a = 0; a < pkh; ++a)
+                    for (int32_t b = 0; b < pkw; ++b)
+                    {
+                        const int32_t y = py * psh + a, x = px * psw + b;
+                        if (y < ho && x < wo && act[y * wo + x] == c) m = 1;
+                    }
+                dst[i] = m;
+            }
     }
 }
 
@@ -584,14 +651,19 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
         g.steps = int32_t(cdiv(g.groups, 4));
         g.waves = int32_t(cdiv(cout, 32));
         g.tiles = int32_t(cdiv(ho * wo, COLS));
-        g.plane = int32_t(cdiv(int64_t(g.ph) * g.pw * g.cin_pad, 16) * 16);
+        g.stage_bytes = int32_t(cdiv(int64_t(t) * h * w * g.cin_pad, 16) * 16);
         // a tile of COLS row-major pixels spans at most ceil((COLS-1)/wo)+1 rows
         const int64_t win_rows = cdiv(COLS - 1, wo) + kh;
         g.win_units = int32_t(win_rows * g.pw * (g.cin_pad / 8));
         g.th = float(threshold);
         SPK_ARG(g.steps <= MAX_STEPS, "Fused layer: Cin*Kh*Kw too large (K <= 512 after padding).");
-        const size_t lds = size_t(g.t) * g.plane + size_t(2) * g.win_units * 16 +
-                           size_t(g.t) * MAX_WAVES * COLS * 8 + COLS * 4 + size_t(g.t) * COLS * 2;
+        SPK_ARG(int64_t(g.pw) * (g.cin_pad / 8) <= 64 * g.waves, "Fused layer: padded width * Cin too large.");
+        const size_t fixed = size_t(g.stage_bytes) + size_t(g.t) * MAX_WAVES * COLS * 8 + COLS * 4 +
+                             size_t(g.t) * COLS * 2;
+        const int64_t room = (163840 - int64_t(fixed)) / (int64_t(g.win_units) * 16);
+        SPK_ARG(room >= 1, "Fused layer: the staged sample (" << fixed << " bytes) leaves no room in the 160 KiB LDS.");
+        g.tc = int32_t(room < t ? room : t);
+        const size_t lds = fixed + size_t(g.tc) * g.win_units * 16;
         SPK_ARG(lds <= 163840, "Fused layer: the staged sample (" << lds << " bytes) exceeds the 160 KiB LDS.");
         SPK_ARG(int64_t(cout) * ho * wo < (1LL << 31), "Fused layer: output plane too large.");
         SPK_ARG(ho * wo <= 8192, "Fused layer: convwta supports at most 8192 output pixels per sample.");

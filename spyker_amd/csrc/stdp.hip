@@ -70,13 +70,10 @@ __global__ void __launch_bounds__(256)
     if (w.c < 0 || bad) return;
     const int64_t out_plane = int64_t(g.ho) * g.wo, in_plane = int64_t(g.h) * g.w, in_step = g.cin * in_plane;
     const uint8_t *os = out + (s * g.t * g.cout + w.z) * out_plane + w.y * g.wo + w.x;
+    // first spiking step, scanned downwards as the reference does; the loads
+    // are independent, so they are all in flight at once
     int32_t osum = g.t;
-    for (int32_t i = 0; i < g.t; ++i)
-        if (os[int64_t(i) * g.cout * out_plane])
-        {
-            osum = i;
-            break;
-        }
+    for (int32_t i = g.t - 1; i >= 0; --i) osum = os[int64_t(i) * g.cout * out_plane] ? i : osum;
     const int32_t taps = g.kh * g.kw, ksz = g.cin * taps, words = (ksz + 31) / 32;
     const uint8_t *is = in + s * g.t * in_step;
     uint32_t *rec = bits + e * words;
@@ -93,12 +90,7 @@ __global__ void __launch_bounds__(256)
             if (iy >= 0 && iy < g.h && ix >= 0 && ix < g.w)
             {
                 const uint8_t *p = is + ch * in_plane + int64_t(iy) * g.w + ix;
-                for (int32_t j = 0; j < g.t; ++j)
-                    if (p[int64_t(j) * in_step])
-                    {
-                        isum = j;
-                        break;
-                    }
+                for (int32_t j = g.t - 1; j >= 0; --j) isum = p[int64_t(j) * in_step] ? j : isum;
             }
             bit = isum <= osum;
         }
@@ -124,12 +116,7 @@ __global__ void __launch_bounds__(256)
     if (w.c < 0 || bad) return;
     const uint8_t *os = out + s * g.t * int64_t(g.cout) + w.x;
     int32_t osum = g.t;
-    for (int32_t i = 0; i < g.t; ++i)
-        if (os[int64_t(i) * g.cout])
-        {
-            osum = i;
-            break;
-        }
+    for (int32_t i = g.t - 1; i >= 0; --i) osum = os[int64_t(i) * g.cout] ? i : osum;
     const int32_t fin = g.cin, words = (fin + 31) / 32;
     const uint8_t *is = in + s * g.t * int64_t(fin);
     uint32_t *rec = bits + e * words;
@@ -141,12 +128,7 @@ __global__ void __launch_bounds__(256)
         if (f < fin)
         {
             int32_t isum = g.t;
-            for (int32_t j = 0; j < g.t; ++j)
-                if (is[int64_t(j) * fin + f])
-                {
-                    isum = j;
-                    break;
-                }
+            for (int32_t j = g.t - 1; j >= 0; --j) isum = is[int64_t(j) * fin + f] ? j : isum;
             bit = isum <= osum;
         }
         store_bits(rec, words, base, bit);
@@ -165,7 +147,9 @@ __global__ void __launch_bounds__(APPLY_THREADS)
     __shared__ int32_t list[APPLY_THREADS];
     __shared__ int32_t cfgidx[APPLY_THREADS];
     __shared__ int32_t wave_count[APPLY_THREADS / 64];
+    __shared__ spk_stdp_config lcfg[SPK_MAX_CONFIGS];
     if (*flag != 0) return;
+    for (int32_t k = threadIdx.x; k < cfg.n; k += blockDim.x) lcfg[k] = cfg.c[k];  // read after the chunk barriers
     const int32_t key = blockIdx.x, w0 = blockIdx.y * SLICE;
     const int32_t ksz = by_x ? g.cin : g.cin * g.kh * g.kw, words = (ksz + 31) / 32;
     const int32_t nw = min(SLICE, ksz - w0), sw = (nw + 31) / 32, word0 = w0 / 32;
@@ -200,9 +184,20 @@ __global__ void __launch_bounds__(APPLY_THREADS)
         for (int32_t q = threadIdx.x; q < m * sw; q += APPLY_THREADS)
             lds_bits[q] = bits[(e0 + list[q / sw]) * words + word0 + q % sw];
         __syncthreads();
-        if (i < nw)
+        if (i < nw && m > 0)
+        {
+            // the chain of updates on this weight, with the next winner's bit
+            // and configuration fetched one update ahead
+            uint32_t word = lds_bits[i >> 5];
+            int32_t ci = cfgidx[0];
             for (int32_t q = 0; q < m; ++q)
-                wr = stdp_update(wr, (lds_bits[q * sw + (i >> 5)] >> (i & 31)) & 1u, cfg.c[cfgidx[q]]);
+            {
+                const uint32_t cur = word;
+                const spk_stdp_config cf = lcfg[ci];
+                if (q + 1 < m) word = lds_bits[(q + 1) * sw + (i >> 5)], ci = cfgidx[q + 1];
+                wr = stdp_update(wr, (cur >> (i & 31)) & 1u, cf);
+            }
+        }
         __syncthreads();
     }
     if (i < nw) kz[i] = wr;
