@@ -24,14 +24,22 @@
 // over samples):
 //   - wave w owns output channels [32w, 32w+32) as two 16-row MFMA tiles;
 //     its FP16 weight fragments for all K stay in VGPRs for the whole kernel;
-//   - the sample's input for all T steps is staged once in LDS as
-//     [t][padded y][padded x][cin_pad] bytes (channel innermost), so one
-//     MFMA B fragment -- 8 consecutive input channels at one tap -- is one
-//     ds_read_b64, widened to FP16 exactly (0x6400|b minus 1024);
-//   - columns are 16 output pixels; for each column tile the kernel runs
-//     t = 0..T-1, keeping the per-(channel, pixel) summaries in registers, and
-//     reduces the per-pixel channel argmax across lanes (shuffles) and waves
-//     (LDS, double-buffered by step parity: one barrier per step).
+//   - the sample's input for all T steps is staged once in LDS as U8
+//     [t][y][x][cin_pad] (channel innermost);
+//   - columns are 16 row-major output pixels.  For a column tile, the FP16
+//     windows of a chunk of steps are widened once for the block into
+//     [step][row][cin_pad/8 planes of padded width], row pitch = wo mod 16 so
+//     the 16 pixels of one B fragment fall in distinct LDS banks; every
+//     wave then runs the chunk's steps without barriers, loading the next
+//     step's B fragments while it works on the current one;
+//   - per step and (channel, pixel) a lane only shifts "v > th" into a mask
+//     and keeps the running maximum;
+//   - at the end of a tile, cnt = popcount(OR of all masks) gives inhibit's
+//     step ts = T - max(cnt, 1); one MFMA pass over B fragments gathered
+//     from the stage at each pixel's own ts yields the kept channel c*, and
+//     a second at c*'s first firing step yields convwta's t0 value.  The
+//     gathered passes issue the same MFMA sequence on the same operands as
+//     the step loop, so they reproduce its potentials bit for bit.
 #include <vector>
 
 #include "stdp.hpp"
@@ -60,10 +68,29 @@ struct FusedGeom
     int32_t n, t, cin, h, w, cout, kh, kw, pt, pl, ho, wo;
     int32_t ph, pw, cin_pad, groups, steps, waves, tiles;
     int32_t stage_bytes;  // bytes of the staged sample ([t][h][w][cin_pad] U8, 16-aligned)
-    int32_t win_units;    // 16-byte units of one step's FP16 window
+    int32_t rows;         // window rows of a column tile (at most)
+    int32_t pitch;        // 16-byte units per window row: cin_pad/8 planes of pw, padded to = wo (mod 16)
+    int32_t win_units;    // 16-byte units of one step's FP16 window (rows * pitch)
     int32_t tc;           // steps whose windows are widened together
     float th;
 };
+
+// Where K-group grp (8 input channels at one tap) of a pixel lives: byte
+// offset in the stage relative to the pixel's own position, the tap's
+// row/column offset (for the bounds test) and its unit offset in a window.
+// Groups past the real K (zero weights) reuse group 0's operands.
+struct TapEntry
+{
+    int32_t soff, dy, dx, woff;
+};
+
+__device__ __forceinline__ TapEntry tap_entry(const FusedGeom &g, int32_t grp)
+{
+    const int32_t cg = g.cin_pad / 8;
+    const int32_t gg = grp < g.groups ? grp : 0, tap = gg / cg, c8 = gg % cg, ky = tap / g.kw, kx = tap % g.kw;
+    const int32_t dy = ky - g.pt, dx = kx - g.pl;
+    return TapEntry{(dy * g.w + dx) * g.cin_pad + c8 * 8, dy, dx, ky * g.pitch + c8 * g.pw + kx};
+}
 
 // [row tile][step][lane][8] FP16 fragments of the FP32 masters.
 __global__ void prep_fragments(const float *__restrict__ k, half8 *__restrict__ frag, FusedGeom g)
@@ -87,23 +114,16 @@ __global__ void prep_fragments(const float *__restrict__ k, half8 *__restrict__ 
     }
 }
 
-__device__ __forceinline__ half8 widen(uint2 raw)
+__device__ __forceinline__ uint4 widen(uint2 raw)
 {
-    // bytes -> FP16 exactly: 0x6400 | b is 1024 + b, minus 1024
+    // bytes -> FP16 exactly: 0x64 above byte b is the half 1024 + b, minus 1024
     const half2v bias = {_Float16(1024.0f), _Float16(1024.0f)};
-    uint32_t w[4];
-    w[0] = (raw.x & 0xFFu) | ((raw.x & 0xFF00u) << 8);
-    w[1] = ((raw.x >> 16) & 0xFFu) | ((raw.x >> 8) & 0xFF0000u);
-    w[2] = (raw.y & 0xFFu) | ((raw.y & 0xFF00u) << 8);
-    w[3] = ((raw.y >> 16) & 0xFFu) | ((raw.y >> 8) & 0xFF0000u);
-    half8 r;
+    const uint32_t hi = 0x64646464u;
+    uint32_t w[4] = {__builtin_amdgcn_perm(hi, raw.x, 0x04010400u), __builtin_amdgcn_perm(hi, raw.x, 0x04030402u),
+                     __builtin_amdgcn_perm(hi, raw.y, 0x04010400u), __builtin_amdgcn_perm(hi, raw.y, 0x04030402u)};
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-    {
-        half2v h = __builtin_bit_cast(half2v, w[i] | 0x64006400u) - bias;
-        r[2 * i] = h[0], r[2 * i + 1] = h[1];
-    }
-    return r;
+    for (int i = 0; i < 4; ++i) w[i] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(half2v, w[i]) - bias);
+    return make_uint4(w[0], w[1], w[2], w[3]);
 }
 
 __device__ __forceinline__ void combine(float &v, int32_t &i, float ov, int32_t oi)
@@ -131,32 +151,44 @@ __device__ __forceinline__ void stage_bytes(uint8_t *stage, const FusedGeom &g, 
     }
 }
 
-// Epilogue of one step for one lane: threshold, the per-(channel, pixel)
-// summaries, and the lane's first channel maximum.  The lane's channels
-// ascend with (r, i), so strict '>' keeps the first maximum; padded channels
-// have zero weights, so their value 0 never beats a real channel.  th >= 0,
-// so a value fires exactly when it passes the threshold.
-__device__ __forceinline__ void epilogue(const float4v &acc0, const float4v &acc1, int32_t tt, float th,
-                                         int32_t o0, uint32_t (&mask)[2][4], float (&first)[2][4],
-                                         float (&mx)[2][4], float &bv, int32_t &bi)
+// One step of one lane's 8 (channel, pixel) values: shift "v > th" into the
+// masks (newest step in bit 0, via the carry input of v_addc) and keep the
+// running maximum as a signed compare of the float bits, which orders every
+// positive value correctly; the maximum is only read when a value fired,
+// and then it is the largest fired value (> th >= 0).
+__device__ __forceinline__ void step_update(const float4v &acc0, const float4v &acc1, float th,
+                                            uint32_t (&mask)[2][4], int32_t (&mxb)[2][4])
 {
-    const uint32_t bit = 1u << tt;
-    bv = -1.0f, bi = INT32_MAX;
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
         for (int i = 0; i < 4; ++i)
         {
             const float v = r == 0 ? acc0[i] : acc1[i];
-            const bool f = v > th;
-            const float xp = f ? v : 0.0f;
-            first[r][i] = mask[r][i] == 0 ? xp : first[r][i];
-            mask[r][i] = f ? (mask[r][i] | bit) : mask[r][i];
-            mx[r][i] = mx[r][i] < xp ? xp : mx[r][i];
-            const bool take = xp > bv;
-            bv = take ? xp : bv;
-            bi = take ? o0 + r * 16 + i : bi;
+            const uint64_t fired = __builtin_amdgcn_ballot_w64(v > th);
+            uint64_t carry;
+            asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(mask[r][i]), "=s"(carry) : "v"(mask[r][i]), "s"(fired));
+            mxb[r][i] = max(mxb[r][i], __builtin_bit_cast(int32_t, v));
         }
+}
+
+template <int S>
+__device__ __forceinline__ void load_b(half8 (&b)[S], const uint8_t *base, const int32_t (&offs)[S])
+{
+#pragma unroll
+    for (int s = 0; s < S; ++s) b[s] = *reinterpret_cast<const half8 *>(base + offs[s]);
+}
+
+template <int S>
+__device__ __forceinline__ void mfma_chain(const half8 (&a)[2][S], const half8 (&b)[S], float4v &c0, float4v &c1)
+{
+    c0 = float4v{0, 0, 0, 0}, c1 = float4v{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+    {
+        c0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0][s], b[s], c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1][s], b[s], c1, 0, 0, 0);
+    }
 }
 
 template <int S, bool DIAG>
@@ -165,18 +197,20 @@ __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ i
                                                    FusedGeom g)
 {
     extern __shared__ __align__(16) uint8_t lds[];
-    uint8_t *stage = lds;                                            // [t][h][w][cin_pad] U8
-    uint4 *win16 = reinterpret_cast<uint4 *>(lds + g.stage_bytes);   // [tc][rows][pw][cin_pad] FP16
-    float *red_v = reinterpret_cast<float *>(win16 + g.tc * g.win_units);  // [t][MAX_WAVES][COLS]
-    int32_t *red_i = reinterpret_cast<int32_t *>(red_v + g.t * MAX_WAVES * COLS);
-    int32_t *cstar = red_i + g.t * MAX_WAVES * COLS;                  // [COLS]
-    uint16_t *ch_tab = reinterpret_cast<uint16_t *>(cstar + COLS);    // [t][COLS]
+    uint8_t *stage = lds;                                                    // [t][h][w][cin_pad] U8
+    uint4 *win16 = reinterpret_cast<uint4 *>(lds + g.stage_bytes);           // [tc][rows][pitch] FP16 x 8
+    uint4 *patch = win16 + g.tc * g.win_units;                               // [4S][COLS] gathered B fragments
+    uint32_t *any_s = reinterpret_cast<uint32_t *>(patch + 4 * S * COLS);    // [MAX_WAVES][COLS]
+    float *red_v = reinterpret_cast<float *>(any_s + MAX_WAVES * COLS);      // [MAX_WAVES][COLS]
+    int32_t *red_i = reinterpret_cast<int32_t *>(red_v + MAX_WAVES * COLS);  // [MAX_WAVES][COLS]
+    int32_t *tsel = red_i + MAX_WAVES * COLS;                                // [COLS]
+    int4 *gtab = reinterpret_cast<int4 *>(tsel + COLS);                      // [4S] TapEntry
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, q = lane >> 4, j = lane & 15;
     const int32_t hw = g.ho * g.wo, hwi = g.h * g.w, cg = g.cin_pad / 8;
     const int32_t o0 = w * 32 + 4 * q;  // the lane's first channel; rows r*16 + i follow
 
-    // weight fragments and tap offsets, resident for the whole kernel
+    // weight fragments and window offsets, resident for the whole kernel
     half8 a[2][S];
     int32_t offs[S];
 #pragma unroll
@@ -184,22 +218,33 @@ __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ i
     {
         a[0][s] = frag[((w * 2 + 0) * S + s) * 64 + lane];
         a[1][s] = frag[((w * 2 + 1) * S + s) * 64 + lane];
-        const int32_t grp = 4 * s + q;
-        const int32_t tap = grp < g.groups ? grp / cg : 0, c8 = grp < g.groups ? grp % cg : 0;
-        offs[s] = (((tap / g.kw) * g.pw + tap % g.kw) * g.cin_pad + c8 * 8) * 2;  // bytes in a window
+        offs[s] = tap_entry(g, 4 * s + q).woff * 16;
+    }
+    for (int32_t e = threadIdx.x; e < 4 * S; e += blockDim.x)
+    {
+        const TapEntry te = tap_entry(g, e);
+        gtab[e] = make_int4(te.soff, te.dy, te.dx, te.woff);
     }
     const int32_t win_step = g.win_units * 16;  // bytes of one step's window
+
+    // window build: a thread owns one unit of a window row (channel plane
+    // c8, padded column col) and walks the (step, row) pairs
+    const int32_t row_units = cg * g.pw, lanes_per = int32_t(blockDim.x) / row_units;
+    const int32_t cu = int32_t(threadIdx.x) % row_units, slot = int32_t(threadIdx.x) / row_units;
+    const int32_t bc8 = cu / g.pw, bix = cu % g.pw - g.pl;
+    const bool bcol = bix >= 0 && bix < g.w;
 
     for (int32_t n = blockIdx.x; n < g.n; n += gridDim.x)
     {
         __syncthreads();
         // stage the sample as [t][y][x][cin_pad] bytes: zero (for the padded
         // channels), then scatter the [t][cin][h][w] input with dword loads
+        if (g.cin != g.cin_pad)
         {
             uint4 *z = reinterpret_cast<uint4 *>(stage);
             for (int32_t i = threadIdx.x; i < g.stage_bytes / 16; i += blockDim.x) z[i] = make_uint4(0, 0, 0, 0);
+            __syncthreads();
         }
-        __syncthreads();
         {
             constexpr int U = 8;
             const uint8_t *src = in + int64_t(n) * g.t * g.cin * hwi;
@@ -229,79 +274,45 @@ __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ i
             // the tile's window: padded rows [y_first, y_first + rows), all padded columns
             const int32_t y_first = (tile * COLS) / g.wo;
             const int32_t rows = min(tile * COLS + COLS - 1, hw - 1) / g.wo - y_first + g.kh;
-            const int32_t base = ((y - y_first) * g.pw + x) * g.cin_pad * 2;
+            const uint8_t *wb = reinterpret_cast<const uint8_t *>(win16) + ((y - y_first) * g.pitch + x) * 16;
             uint32_t mask[2][4];
-            float first[2][4], mx[2][4];
+            int32_t mxb[2][4];
 #pragma unroll
             for (int r = 0; r < 2; ++r)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) mask[r][i] = 0, first[r][i] = 0.0f, mx[r][i] = 0.0f;
+                for (int i = 0; i < 4; ++i) mask[r][i] = 0, mxb[r][i] = 0;
 
             for (int32_t t0 = 0; t0 < g.t; t0 += g.tc)
             {
                 const int32_t tc = min(g.tc, g.t - t0);
                 // widen the windows of steps [t0, t0 + tc) once for the block;
-                // a thread owns one (column, channel group) and walks the
-                // (step, row) pairs; positions in the zero pad are written 0
+                // positions in the zero pad are written 0
+                if (slot < lanes_per)
                 {
-                    const int32_t units = g.pw * cg, lanes_per = int32_t(blockDim.x) / units;
-                    const int32_t cu = int32_t(threadIdx.x) % units, slot = int32_t(threadIdx.x) / units;
-                    const int32_t col = cu / cg, g8 = cu % cg, ix = col - g.pl;
-                    if (slot < lanes_per)
+                    int32_t ts = slot / rows, r = slot % rows;
+                    for (int32_t e = slot; e < tc * rows; e += lanes_per)
                     {
-                        int32_t ts = slot / rows, r = slot % rows;
-                        for (int32_t e = slot; e < tc * rows; e += lanes_per)
-                        {
-                            const int32_t iy = y_first + r - g.pt;
-                            uint2 raw = make_uint2(0, 0);
-                            if (iy >= 0 && iy < g.h && ix >= 0 && ix < g.w)
-                                raw = *reinterpret_cast<const uint2 *>(
-                                    stage + (((t0 + ts) * g.h + iy) * g.w + ix) * g.cin_pad + g8 * 8);
-                            win16[ts * g.win_units + (r * g.pw + col) * cg + g8] =
-                                __builtin_bit_cast(uint4, widen(raw));
-                            r += lanes_per;
-                            while (r >= rows) r -= rows, ++ts;
-                        }
+                        const int32_t iy = y_first + r - g.pt;
+                        uint2 raw = make_uint2(0, 0);
+                        if (bcol && iy >= 0 && iy < g.h)
+                            raw = *reinterpret_cast<const uint2 *>(
+                                stage + (((t0 + ts) * g.h + iy) * g.w + bix) * g.cin_pad + bc8 * 8);
+                        win16[ts * g.win_units + r * g.pitch + cu] = widen(raw);
+                        r += lanes_per;
+                        while (r >= rows) r -= rows, ++ts;
                     }
                 }
                 __syncthreads();
-                // steps t0 .. t0+tc-1 without barriers; the MFMAs of step
-                // tt+1 are issued before the epilogue of step tt
-                const uint8_t *wb = reinterpret_cast<const uint8_t *>(win16) + base;
-                float4v acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
-                {
-                    half8 b[S];
-#pragma unroll
-                    for (int s = 0; s < S; ++s) b[s] = *reinterpret_cast<const half8 *>(wb + offs[s]);
-#pragma unroll
-                    for (int s = 0; s < S; ++s)
-                    {
-                        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0][s], b[s], acc0, 0, 0, 0);
-                        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1][s], b[s], acc1, 0, 0, 0);
-                    }
-                }
-                for (int32_t k = 0; k < tc; ++k)
-                {
-                    const int32_t tt = t0 + k;
-                    // one basic block per step: the next step's MFMAs (the last
-                    // step of a chunk recomputes itself, discarded) interleave
-                    // with this step's epilogue
-                    float4v nx0 = {0, 0, 0, 0}, nx1 = {0, 0, 0, 0};
-                    {
-                        const uint8_t *nb = wb + min(k + 1, tc - 1) * win_step;
-                        half8 b[S];
-#pragma unroll
-                        for (int s = 0; s < S; ++s) b[s] = *reinterpret_cast<const half8 *>(nb + offs[s]);
-#pragma unroll
-                        for (int s = 0; s < S; ++s)
-                        {
-                            nx0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0][s], b[s], nx0, 0, 0, 0);
-                            nx1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1][s], b[s], nx1, 0, 0, 0);
-                        }
-                    }
-                    float bv;
-                    int32_t bi;
-                    epilogue(acc0, acc1, tt, g.th, o0, mask, first, mx, bv, bi);
+                // the chunk's steps without barriers: the loads of step k+2 and
+                // the MFMAs of step k+1 are issued before the update of step k
+                // (loads past the chunk re-read its last step, unused)
+                half8 b0[S], b1[S];
+                float4v c0, c1, n0, n1;
+                load_b(b0, wb, offs);
+                mfma_chain(a, b0, c0, c1);
+                load_b(b0, wb + min(1, tc - 1) * win_step, offs);
+                auto update = [&](int32_t k, const float4v &u0, const float4v &u1) {
+                    step_update(u0, u1, g.th, mask, mxb);
                     if (DIAG && pvalid)
 #pragma unroll
                         for (int r = 0; r < 2; ++r)
@@ -310,45 +321,123 @@ __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ i
                             {
                                 const int32_t o = o0 + r * 16 + i;
                                 if (o < g.cout)
-                                    pot_out[((int64_t(n) * g.t + tt) * g.cout + o) * hw + p] =
-                                        r == 0 ? acc0[i] : acc1[i];
+                                    pot_out[((int64_t(n) * g.t + t0 + k) * g.cout + o) * hw + p] =
+                                        r == 0 ? u0[i] : u1[i];
                             }
-                    combine(bv, bi, __shfl_xor(bv, 16, 64), __shfl_xor(bi, 16, 64));
-                    combine(bv, bi, __shfl_xor(bv, 32, 64), __shfl_xor(bi, 32, 64));
-                    if (lane < COLS)
-                        red_v[(tt * MAX_WAVES + w) * COLS + lane] = bv, red_i[(tt * MAX_WAVES + w) * COLS + lane] = bi;
-                    acc0 = nx0, acc1 = nx1;
+                };
+                for (int32_t k = 0; k < tc; k += 2)
+                {
+                    load_b(b1, wb + min(k + 2, tc - 1) * win_step, offs);
+                    if (k + 1 < tc) mfma_chain(a, b0, n0, n1);
+                    update(k, c0, c1);
+                    load_b(b0, wb + min(k + 3, tc - 1) * win_step, offs);
+                    if (k + 2 < tc) mfma_chain(a, b1, c0, c1);
+                    if (k + 1 < tc) update(k + 1, n0, n1);
                 }
-                __syncthreads();  // windows free for the next chunk; partials visible
+                if (t0 + tc < g.t) __syncthreads();  // windows free for the next chunk
             }
-            // first channel maximum per (step, pixel) across the waves
-            for (int32_t e = threadIdx.x; e < g.t * COLS; e += blockDim.x)
+
+            // steps at which some channel fired, per pixel (bit T-1-t for step t)
             {
-                const int32_t tt = e / COLS, col = e % COLS;
-                float m = red_v[(tt * MAX_WAVES) * COLS + col];
-                int32_t c = red_i[(tt * MAX_WAVES) * COLS + col];
-                for (int ow = 1; ow < g.waves; ++ow)
-                    combine(m, c, red_v[(tt * MAX_WAVES + ow) * COLS + col], red_i[(tt * MAX_WAVES + ow) * COLS + col]);
-                ch_tab[tt * COLS + col] = uint16_t(c);
-                red_v[(tt * MAX_WAVES) * COLS + col] = m;
+                uint32_t any = 0;
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) any |= mask[r][i];
+                any |= __shfl_xor(any, 16, 64);
+                any |= __shfl_xor(any, 32, 64);
+                if (lane < COLS) any_s[w * COLS + lane] = any;
             }
             __syncthreads();
-            // kept channel: first maximum at step T - max(cnt, 1)
-            if (int32_t(threadIdx.x) < COLS)
+            // the B fragments of the tile's pixels, each at its own step st
+            // (the same bytes the step loop's window held for that step)
+            auto gather = [&](int32_t st) {
+                for (int32_t grp = int32_t(threadIdx.x) >> 4; grp < 4 * S; grp += int32_t(blockDim.x) >> 4)
+                {
+                    const int4 te = gtab[grp];
+                    const int32_t iy = y + te.y, ix = x + te.z;
+                    uint2 raw = make_uint2(0, 0);
+                    if (st >= 0 && iy >= 0 && iy < g.h && ix >= 0 && ix < g.w)
+                        raw = *reinterpret_cast<const uint2 *>(stage + (st * hwi + y * g.w + x) * g.cin_pad + te.x);
+                    patch[grp * COLS + j] = widen(raw);
+                }
+            };
             {
-                int32_t cnt = 0;
-                for (int32_t tt = 0; tt < g.t; ++tt) cnt += red_v[(tt * MAX_WAVES) * COLS + threadIdx.x] > 0.0f;
-                cstar[threadIdx.x] = ch_tab[(g.t - (cnt > 1 ? cnt : 1)) * COLS + threadIdx.x];
+                uint32_t any = 0;
+                for (int32_t ow = 0; ow < g.waves; ++ow) any |= any_s[ow * COLS + j];
+                const int32_t cnt = __builtin_popcount(any);
+                gather(g.t - (cnt > 1 ? cnt : 1));
             }
             __syncthreads();
-            const int32_t cs = cstar[j];
-            if (pvalid)
+            int32_t poffs[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) poffs[s] = (s * 64 + lane) * 16;
+            // pass 1: the potentials at inhibit's step; the kept channel is the
+            // first maximum of the thresholded values (inhibit.cpp:64-85).  The
+            // lane's channels ascend with (r, i), so strict '>' keeps the first
+            // maximum; padded channels have zero weights and value 0, which never
+            // beats a lower channel.
+            float4v g0, g1;
+            {
+                half8 b[S];
+                load_b(b, reinterpret_cast<const uint8_t *>(patch), poffs);
+                mfma_chain(a, b, g0, g1);
+                float bv = -1.0f;
+                int32_t bi = INT32_MAX;
 #pragma unroll
                 for (int r = 0; r < 2; ++r)
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
-                        if (o0 + r * 16 + i == cs)
-                            summary[int64_t(n) * hw + p] = Summary{cs, mask[r][i], first[r][i], mx[r][i]};
+                    {
+                        const float v = r == 0 ? g0[i] : g1[i];
+                        const float xp = v > g.th ? v : 0.0f;
+                        const bool take = xp > bv;
+                        bv = take ? xp : bv;
+                        bi = take ? o0 + r * 16 + i : bi;
+                    }
+                combine(bv, bi, __shfl_xor(bv, 16, 64), __shfl_xor(bi, 16, 64));
+                combine(bv, bi, __shfl_xor(bv, 32, 64), __shfl_xor(bi, 32, 64));
+                if (lane < COLS) red_v[w * COLS + lane] = bv, red_i[w * COLS + lane] = bi;
+            }
+            __syncthreads();
+            int32_t cs;
+            {
+                float m = red_v[j];
+                cs = red_i[j];
+                for (int32_t ow = 1; ow < g.waves; ++ow) combine(m, cs, red_v[ow * COLS + j], red_i[ow * COLS + j]);
+            }
+            // the lane holding (c*, pixel) publishes c*'s first firing step
+            const bool own = (cs >> 5) == w && ((cs >> 2) & 3) == q;
+            uint32_t om = 0;
+            int32_t ox = 0;
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (((cs >> 4) & 1) == r && (cs & 3) == i) om = mask[r][i], ox = mxb[r][i];
+            if (own) tsel[j] = om ? g.t - 32 + int32_t(__builtin_clz(om)) : -1;
+            __syncthreads();
+            gather(tsel[j]);
+            __syncthreads();
+            // pass 2: the value of c* at its first firing step (convwta's t0 value)
+            float first = 0.0f;
+            if (__builtin_amdgcn_ballot_w64(own && om != 0))
+            {
+                half8 b[S];
+                load_b(b, reinterpret_cast<const uint8_t *>(patch), poffs);
+                mfma_chain(a, b, g0, g1);
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        if (((cs >> 4) & 1) == r && (cs & 3) == i) first = r == 0 ? g0[i] : g1[i];
+            }
+            if (own && pvalid)
+            {
+                const uint32_t fwd = om ? __builtin_bitreverse32(om) >> (32 - g.t) : 0u;
+                summary[int64_t(n) * hw + p] =
+                    Summary{cs, fwd, om ? first : 0.0f, om ? __builtin_bit_cast(float, ox) : 0.0f};
+            }
         }
     }
 }
@@ -653,16 +742,22 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
         g.tiles = int32_t(cdiv(ho * wo, COLS));
         g.stage_bytes = int32_t(cdiv(int64_t(t) * h * w * g.cin_pad, 16) * 16);
         // a tile of COLS row-major pixels spans at most ceil((COLS-1)/wo)+1 rows
-        const int64_t win_rows = cdiv(COLS - 1, wo) + kh;
-        g.win_units = int32_t(win_rows * g.pw * (g.cin_pad / 8));
+        g.rows = int32_t(cdiv(COLS - 1, wo) + kh);
+        // row pitch = wo (mod 16) puts 16 consecutive output pixels in 16
+        // distinct 16-byte bank groups
+        const int64_t row_units = int64_t(g.pw) * (g.cin_pad / 8);
+        g.pitch = int32_t(row_units + ((wo - row_units) % 16 + 16) % 16);
+        g.win_units = g.rows * g.pitch;
         g.th = float(threshold);
         SPK_ARG(g.steps <= MAX_STEPS, "Fused layer: Cin*Kh*Kw too large (K <= 512 after padding).");
-        SPK_ARG(int64_t(g.pw) * (g.cin_pad / 8) <= 64 * g.waves, "Fused layer: padded width * Cin too large.");
-        const size_t fixed = size_t(g.stage_bytes) + size_t(g.t) * MAX_WAVES * COLS * 8 + COLS * 4 +
-                             size_t(g.t) * COLS * 2;
+        SPK_ARG(row_units <= 64 * g.waves, "Fused layer: padded width * Cin too large.");
+        // stage, gathered fragments, any/argmax partials, step table, tap table
+        const size_t fixed = size_t(g.stage_bytes) + size_t(g.steps) * 4 * COLS * 16 + MAX_WAVES * COLS * 12 +
+                             COLS * 4 + size_t(g.steps) * 4 * 16;
         const int64_t room = (163840 - int64_t(fixed)) / (int64_t(g.win_units) * 16);
         SPK_ARG(room >= 1, "Fused layer: the staged sample (" << fixed << " bytes) leaves no room in the 160 KiB LDS.");
-        g.tc = int32_t(room < t ? room : t);
+        const int64_t chunks = cdiv(t, room < t ? room : t);
+        g.tc = int32_t(cdiv(t, chunks));
         const size_t lds = fixed + size_t(g.tc) * g.win_units * 16;
         SPK_ARG(lds <= 163840, "Fused layer: the staged sample (" << lds << " bytes) exceeds the 160 KiB LDS.");
         SPK_ARG(int64_t(cout) * ho * wo < (1LL << 31), "Fused layer: output plane too large.");
