@@ -72,6 +72,7 @@ struct FusedGeom
     int32_t pitch;        // 16-byte units per window row: cin_pad/8 planes of pw, padded to = wo (mod 16)
     int32_t win_units;    // 16-byte units of one step's FP16 window (rows * pitch)
     int32_t tc;           // steps whose windows are widened together
+    int32_t stage_fast;   // staging by quad transposes (hw % 4 == 0, aligned input, cin_pad/4 | block quads)
     float th;
 };
 
@@ -133,22 +134,13 @@ __device__ __forceinline__ void combine(float &v, int32_t &i, float ov, int32_t 
     i = take ? oi : i;
 }
 
-// Writes `count` consecutive input bytes, starting at flat index i of the
-// sample's [t][cin][h][w] tensor, to their [t][y][x][c] slots of the
-// (unpadded, channel-innermost) U8 stage.
-__device__ __forceinline__ void stage_bytes(uint8_t *stage, const FusedGeom &g, int32_t i, int count, uint32_t v,
-                                            int32_t chw, int32_t hwi)
+// Writes input byte i of the sample's [t][cin][h][w] tensor to its
+// [t][y][x][c] slot of the (unpadded, channel-innermost) U8 stage.
+__device__ __forceinline__ void stage_bytes(uint8_t *stage, const FusedGeom &g, int32_t i, uint8_t v, int32_t chw,
+                                            int32_t hwi)
 {
-    int32_t tt = i / chw, r = i - tt * chw, c = r / hwi, pix = r - c * hwi;
-    for (int b = 0; b < count; ++b)
-    {
-        stage[(tt * hwi + pix) * g.cin_pad + c] = uint8_t(v >> (8 * b));
-        if (++pix == hwi)
-        {
-            pix = 0;
-            if (++c == g.cin) c = 0, ++tt;
-        }
-    }
+    const int32_t tt = i / chw, r = i - tt * chw, c = r / hwi, pix = r - c * hwi;
+    stage[(tt * hwi + pix) * g.cin_pad + c] = v;
 }
 
 // One step of one lane's 8 (channel, pixel) values: shift "v > th" into the
@@ -237,32 +229,69 @@ __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ i
     for (int32_t n = blockIdx.x; n < g.n; n += gridDim.x)
     {
         __syncthreads();
-        // stage the sample as [t][y][x][cin_pad] bytes: zero (for the padded
-        // channels), then scatter the [t][cin][h][w] input with dword loads
-        if (g.cin != g.cin_pad)
+        // stage the sample as [t][y][x][cin_pad] bytes
+        if (g.stage_fast)
         {
-            uint4 *z = reinterpret_cast<uint4 *>(stage);
-            for (int32_t i = threadIdx.x; i < g.stage_bytes / 16; i += blockDim.x) z[i] = make_uint4(0, 0, 0, 0);
-            __syncthreads();
-        }
-        {
+            // a lane loads one dword (4 pixels of channel c) of the
+            // [t][cin][h][w] input; the 4 lanes of a quad hold channels
+            // 4*c4 .. 4*c4+3 of the same 4 pixels and transpose the 4x4 bytes
+            // (two DPP exchanges + v_perm), after which lane l4 holds the 4
+            // channels of pixel l4: one dword store per lane.  With
+            // cin_pad / 4 dividing 16, the 64 stores of a wave hit 64 banks.
+            const int32_t D = g.cin_pad / 4, npq = hwi / 4, step = int32_t(blockDim.x) / 4 / D;
+            const int32_t Q = int32_t(threadIdx.x) >> 2, l4 = int32_t(threadIdx.x) & 3, c4 = Q % D;
+            const int32_t c = 4 * c4 + l4;
+            const uint32_t sel1 = (l4 & 1) ? 0x03070105u : 0x06020400u;
+            const uint32_t sel2 = (l4 & 2) ? 0x03020706u : 0x05040100u;
+            // raw buffer loads of the sample: 32-bit offsets, and offsets past
+            // its end (the walk's tail) read 0
+            const __amdgpu_buffer_rsrc_t src = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<uint8_t *>(in + int64_t(n) * g.t * g.cin * hwi), 0, g.t * g.cin * hwi, 0x00020000);
+            uint32_t *dst = reinterpret_cast<uint32_t *>(stage);
+            // lin = tt * npq + pq walks the (step, pixel quad) pairs by `step`
+            // (<= npq, so pq wraps at most once per move); off is the byte
+            // offset of the lane's dword, channel c of step tt
+            int32_t lin = Q / D, pq = lin, off = (c * npq + pq) * 4;
+            const int32_t total = g.t * npq, wrap_add = (g.cin - 1) * npq * 4;
             constexpr int U = 8;
-            const uint8_t *src = in + int64_t(n) * g.t * g.cin * hwi;
-            const int32_t chw = g.cin * hwi, total = g.t * chw, stride = int32_t(blockDim.x) * 4;
-            if ((reinterpret_cast<uintptr_t>(src) & 3) == 0 && (total & 3) == 0)
-                for (int32_t i0 = int32_t(threadIdx.x) * 4; i0 < total; i0 += stride * U)
+            for (; lin < total; lin += U * step)
+            {
+                uint32_t v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
                 {
-                    uint32_t v[U];
-#pragma unroll
-                    for (int u = 0; u < U; ++u)
-                        v[u] = *reinterpret_cast<const uint32_t *>(src + min(i0 + u * stride, total - 4));
-#pragma unroll
-                    for (int u = 0; u < U; ++u)
-                        if (i0 + u * stride < total) stage_bytes(stage, g, i0 + u * stride, 4, v[u], chw, hwi);
+                    v[u] = uint32_t(__builtin_amdgcn_raw_buffer_load_b32(src, off, 0, 0));
+                    pq += step, off += step * 4;
+                    const bool wrap = pq >= npq;
+                    pq = wrap ? pq - npq : pq;
+                    off = wrap ? off + wrap_add : off;
                 }
-            else
-                for (int32_t i = int32_t(threadIdx.x); i < total; i += int32_t(blockDim.x))
-                    stage_bytes(stage, g, i, 1, src[i], chw, hwi);
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                {
+                    const uint32_t x = c < g.cin ? v[u] : 0u;
+                    const uint32_t y = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(x), 0xB1, 0xF, 0xF, true));  // lane^1
+                    const uint32_t r1 = __builtin_amdgcn_perm(y, x, sel1);
+                    const uint32_t y2 = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(r1), 0x4E, 0xF, 0xF, true));  // lane^2
+                    const int32_t li = lin + u * step;
+                    if (li < total) dst[(4 * li + l4) * D + c4] = __builtin_amdgcn_perm(y2, r1, sel2);
+                }
+            }
+        }
+        else
+        {
+            // any other shape: zero (for the padded channels), then scatter
+            // the [t][cin][h][w] input byte by byte
+            if (g.cin != g.cin_pad)
+            {
+                uint4 *z = reinterpret_cast<uint4 *>(stage);
+                for (int32_t i = threadIdx.x; i < g.stage_bytes / 16; i += blockDim.x) z[i] = make_uint4(0, 0, 0, 0);
+                __syncthreads();
+            }
+            const uint8_t *src = in + int64_t(n) * g.t * g.cin * hwi;
+            const int32_t chw = g.cin * hwi, total = g.t * chw;
+            for (int32_t i = int32_t(threadIdx.x); i < total; i += int32_t(blockDim.x))
+                stage_bytes(stage, g, i, src[i], chw, hwi);
         }
         __syncthreads();
 
@@ -285,21 +314,26 @@ __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ i
             for (int32_t t0 = 0; t0 < g.t; t0 += g.tc)
             {
                 const int32_t tc = min(g.tc, g.t - t0);
-                // widen the windows of steps [t0, t0 + tc) once for the block;
-                // positions in the zero pad are written 0
+                // widen the windows of steps [t0, t0 + tc) once for the block:
+                // a thread slot takes steps slot, slot + lanes_per, ... and walks
+                // their rows; rows in the zero pad ([0, r_lo) and [r_hi, rows))
+                // and columns outside the input are written 0
                 if (slot < lanes_per)
                 {
-                    int32_t ts = slot / rows, r = slot % rows;
-                    for (int32_t e = slot; e < tc * rows; e += lanes_per)
+                    const int32_t r_lo = max(0, g.pt - y_first), r_hi = min(rows, g.h + g.pt - y_first);
+                    const int32_t row_bytes = g.w * g.cin_pad;
+                    for (int32_t ts = slot; ts < tc; ts += lanes_per)
                     {
-                        const int32_t iy = y_first + r - g.pt;
-                        uint2 raw = make_uint2(0, 0);
-                        if (bcol && iy >= 0 && iy < g.h)
-                            raw = *reinterpret_cast<const uint2 *>(
-                                stage + (((t0 + ts) * g.h + iy) * g.w + bix) * g.cin_pad + bc8 * 8);
-                        win16[ts * g.win_units + r * g.pitch + cu] = widen(raw);
-                        r += lanes_per;
-                        while (r >= rows) r -= rows, ++ts;
+                        const uint8_t *sp =
+                            stage + (((t0 + ts) * g.h + y_first - g.pt) * g.w + bix) * g.cin_pad + bc8 * 8;
+                        uint4 *dp = win16 + ts * g.win_units + cu;
+                        for (int32_t r = 0; r < rows; ++r)
+                        {
+                            uint2 raw = make_uint2(0, 0);
+                            if (bcol && r >= r_lo && r < r_hi)
+                                raw = *reinterpret_cast<const uint2 *>(sp + r * row_bytes);
+                            dp[r * g.pitch] = widen(raw);
+                        }
                     }
                 }
                 __syncthreads();
@@ -758,6 +792,8 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
         SPK_ARG(room >= 1, "Fused layer: the staged sample (" << fixed << " bytes) leaves no room in the 160 KiB LDS.");
         const int64_t chunks = cdiv(t, room < t ? room : t);
         g.tc = int32_t(cdiv(t, chunks));
+        g.stage_fast = (h * w) % 4 == 0 && (reinterpret_cast<uintptr_t>(in) & 3) == 0 &&
+                       (16 * g.waves) % (g.cin_pad / 4) == 0 && 16 * g.waves / (g.cin_pad / 4) <= h * w / 4;
         const size_t lds = fixed + size_t(g.tc) * g.win_units * 16;
         SPK_ARG(lds <= 163840, "Fused layer: the staged sample (" << lds << " bytes) exceeds the 160 KiB LDS.");
         SPK_ARG(int64_t(cout) * ho * wo < (1LL << 31), "Fused layer: output plane too large.");
