@@ -143,6 +143,16 @@ __device__ __forceinline__ void stage_bytes(uint8_t *stage, const FusedGeom &g, 
     stage[(tt * hwi + pix) * g.cin_pad + c] = v;
 }
 
+// For the 4 bytes of v seen at step tt: bytes of res whose input is nonzero
+// for the first time (not yet in `seen`) become tt.
+__device__ __forceinline__ void first_nonzero(uint32_t &res, uint32_t &seen, uint32_t v, int32_t tt)
+{
+    const uint32_t nz = (((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v) & 0x80808080u;  // 0x80 in nonzero bytes
+    const uint32_t fresh = ((nz & ~seen) >> 7) * 0xFFu;
+    seen |= nz;
+    res = (res & ~fresh) | (uint32_t(tt) * 0x01010101u & fresh);
+}
+
 // One step of one lane's 8 (channel, pixel) values: shift "v > th" into the
 // masks (newest step in bit 0, via the carry input of v_addc) and keep the
 // running maximum as a signed compare of the float bits, which orders every
@@ -186,7 +196,7 @@ __device__ __forceinline__ void mfma_chain(const half8 (&a)[2][S], const half8 (
 template <int S, bool DIAG>
 __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ in, const half8 *__restrict__ frag,
                                                    Summary *__restrict__ summary, float *__restrict__ pot_out,
-                                                   FusedGeom g)
+                                                   uint8_t *__restrict__ fs_out, FusedGeom g)
 {
     extern __shared__ __align__(16) uint8_t lds[];
     uint8_t *stage = lds;                                                    // [t][h][w][cin_pad] U8
@@ -294,6 +304,24 @@ __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ i
                 stage_bytes(stage, g, i, src[i], chw, hwi);
         }
         __syncthreads();
+        if (fs_out)
+        {
+            // first spiking step of every staged input byte (t when it never
+            // spikes), 8 channels of one pixel per thread: [n][h*w][cin_pad]
+            const uint32_t tb = uint32_t(g.t) * 0x01010101u;
+            for (int32_t u = threadIdx.x; u < hwi * cg; u += blockDim.x)
+            {
+                const uint8_t *sp = stage + u * 8;
+                uint32_t lo = tb, hi = tb, seen_lo = 0, seen_hi = 0;
+                for (int32_t tt = 0; tt < g.t; ++tt)
+                {
+                    const uint2 v = *reinterpret_cast<const uint2 *>(sp + tt * hwi * g.cin_pad);
+                    first_nonzero(lo, seen_lo, v.x, tt);
+                    first_nonzero(hi, seen_hi, v.y, tt);
+                }
+                *reinterpret_cast<uint2 *>(fs_out + (int64_t(n) * hwi * cg + u) * 8) = make_uint2(lo, hi);
+            }
+        }
 
         for (int32_t tile = 0; tile < g.tiles; ++tile)
         {
@@ -480,9 +508,12 @@ __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ i
 // value (T-1-t0) + first/diff with min 0 and diff = 2*max (wta.cpp:49-64);
 // k rounds of first maximum in flat (c, y, x) order with box and channel
 // masking (wta.cpp:128-150).  One block per sample; candidates in LDS.
+// wkey[slot] = channel << 16 | pixel of winner `slot` (sample * count +
+// rank), -1 past the sample's winners: the compact list the STDP reads.
 __global__ void __launch_bounds__(256) fused_select(const Summary *__restrict__ summary, int32_t t, int32_t ho,
                                                     int32_t wo, int32_t ry, int32_t rx, int32_t count,
-                                                    spk_winner *__restrict__ win, int64_t *__restrict__ counts)
+                                                    spk_winner *__restrict__ win, int64_t *__restrict__ counts,
+                                                    int32_t *__restrict__ wkey)
 {
     extern __shared__ __align__(16) uint8_t lds[];
     const int32_t hw = ho * wo;
@@ -517,7 +548,8 @@ __global__ void __launch_bounds__(256) fused_select(const Summary *__restrict__ 
         a = block_argmax(a, am);
         if (fabsf(a.v - (-FLT_MAX)) < 1e-5f) break;
         const int32_t z = a.i / hw, y = (a.i % hw) / wo, x = (a.i % hw) % wo;
-        if (threadIdx.x == 0) win[s * count + k] = spk_winner{0, 0, z, y, x};
+        if (threadIdx.x == 0)
+            win[s * count + k] = spk_winner{0, 0, z, y, x}, wkey[s * count + k] = z << 16 | (a.i % hw);
         for (int32_t p = threadIdx.x; p < hw; p += blockDim.x)
         {
             const int32_t py = p / wo, px = p % wo;
@@ -526,6 +558,7 @@ __global__ void __launch_bounds__(256) fused_select(const Summary *__restrict__ 
         __syncthreads();
     }
     if (threadIdx.x == 0) counts[s] = k;
+    for (int32_t r = k + int32_t(threadIdx.x); r < count; r += blockDim.x) wkey[s * count + r] = -1;
 }
 
 // Spikes [n][t][cout][ho][wo] and (optionally) their max-pool, from the
@@ -640,7 +673,150 @@ __global__ void __launch_bounds__(256) fused_write(const Summary *__restrict__ s
     }
 }
 
-__global__ void clear_flag(int32_t *flag) { *flag = 0; }
+struct FusedStdp
+{
+    int64_t slots;  // n * count
+    int32_t count, cin, cin_pad, h, w, kh, kw, pt, pl, wo, hw, t, words;
+};
+
+// Potentiate/depress bits of every winner slot, one wave per slot: bit i of
+// slot e is isum <= osum for weight i = (ch, ky, kx) of the winner's kernel
+// (stdp.cpp:88-118).  A winner is always its pixel's kept channel, so osum
+// is the first set bit of that pixel's fire mask; isum comes from the
+// first-spike map ([n][h*w][cin_pad], t outside the input).
+__global__ void __launch_bounds__(256)
+    fused_records(const uint8_t *__restrict__ fs, const Summary *__restrict__ summary,
+                  const int32_t *__restrict__ wkey, FusedStdp p, uint32_t *__restrict__ bits)
+{
+    const int64_t e = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (e >= p.slots) return;
+    const int32_t k = wkey[e];
+    if (k < 0) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t s = e / p.count;
+    const int32_t pos = k & 0xFFFF, y = pos / p.wo - p.pt, x = pos % p.wo - p.pl;
+    const uint32_t mask = summary[s * p.hw + pos].mask;
+    const int32_t osum = mask ? int32_t(__builtin_ctz(mask)) : p.t;
+    const int32_t taps = p.kh * p.kw, ksz = p.cin * taps;
+    const uint8_t *fsn = fs + s * p.h * p.w * p.cin_pad;
+    uint32_t *rec = bits + e * p.words;
+    for (int32_t base = 0; base < ksz; base += 64)
+    {
+        const int32_t i = base + lane;
+        bool bit = false;
+        if (i < ksz)
+        {
+            const int32_t ch = i / taps, iy = y + (i / p.kw) % p.kh, ix = x + i % p.kw;
+            const int32_t isum =
+                (iy >= 0 && iy < p.h && ix >= 0 && ix < p.w) ? fsn[(iy * p.w + ix) * p.cin_pad + ch] : p.t;
+            bit = isum <= osum;
+        }
+        const uint64_t b = __ballot(bit);
+        if (lane == 0 && base / 32 < p.words) rec[base / 32] = uint32_t(b);
+        if (lane == 32 && base / 32 + 1 < p.words) rec[base / 32 + 1] = uint32_t(b >> 32);
+    }
+}
+
+constexpr int STDP_THREADS = 256;
+constexpr int STDP_LIST = 1024;  // winners staged in LDS per pass
+
+// The update chains of the fused step (stdp.cpp:139-154): the winners of
+// every sample in rank order, samples in order, each updating the kernel of
+// its channel; all use configuration 0.  One block per (output channel,
+// slice of STDP_THREADS weights): each thread scans a contiguous range of
+// winner slots, one block-wide scan lists the slots naming the channel in
+// slot order, their bit words are staged in LDS, and every thread runs the
+// chain of its weight 32 updates at a time from a register of bits.
+__global__ void __launch_bounds__(STDP_THREADS)
+    fused_stdp(float *__restrict__ kernel, const uint32_t *__restrict__ bits, const int32_t *__restrict__ wkey,
+               FusedStdp p, spk_stdp_config cfg)
+{
+    __shared__ int32_t list[STDP_LIST];
+    __shared__ uint32_t lbits[STDP_LIST * (STDP_THREADS / 32)];  // the slice's words of each listed winner
+    __shared__ int32_t wave_sum[STDP_THREADS / 64];
+    const int32_t key = blockIdx.x, ksz = p.cin * p.kh * p.kw;
+    const int32_t i = blockIdx.y * STDP_THREADS + threadIdx.x;
+    const bool live = i < ksz;
+    float *kz = kernel + int64_t(key) * ksz;
+    float wr = live ? kz[i] : 0.0f;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // this slice's words of a winner's record
+    const int32_t word0 = blockIdx.y * (STDP_THREADS / 32);
+    const int32_t sw = min(STDP_THREADS / 32, p.words - word0), my_word = (i / 32) - word0;
+
+    // the thread's slots [e0, e1), a multiple of 4 wide so keys load as int4
+    const int64_t per = (p.slots + 4 * STDP_THREADS - 1) / (4 * STDP_THREADS) * 4;
+    const int64_t e0 = min(int64_t(threadIdx.x) * per, p.slots), e1 = min(e0 + per, p.slots);
+    auto keys4 = [&](int64_t e) {
+        if (e + 4 <= p.slots) return *reinterpret_cast<const int4 *>(wkey + e);
+        int4 k = make_int4(-1, -1, -1, -1);
+        if (e < p.slots) k.x = wkey[e];
+        if (e + 1 < p.slots) k.y = wkey[e + 1];
+        if (e + 2 < p.slots) k.z = wkey[e + 2];
+        return k;
+    };
+    auto hit = [&](int32_t k) { return (k >> 16) == key; };
+    int32_t cnt = 0;
+    for (int64_t e = e0; e < e1; e += 4)
+    {
+        const int4 k = keys4(e);
+        cnt += hit(k.x) + hit(k.y) + hit(k.z) + hit(k.w);
+    }
+    int32_t incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1)
+    {
+        const int32_t o = __shfl_up(incl, d, 64);
+        incl += lane >= d ? o : 0;
+    }
+    if (lane == 63) wave_sum[wave] = incl;
+    __syncthreads();
+    int32_t first = incl - cnt, m = 0;
+    for (int v = 0; v < STDP_THREADS / 64; ++v)
+    {
+        first += v < wave ? wave_sum[v] : 0;
+        m += wave_sum[v];
+    }
+
+    for (int32_t pass = 0; pass < m; pass += STDP_LIST)
+    {
+        // the thread's matches with list index in [pass, pass + STDP_LIST)
+        if (first < pass + STDP_LIST && first + cnt > pass)
+        {
+            int32_t at = first;
+            for (int64_t e = e0; e < e1; e += 4)
+            {
+                const int4 k4 = keys4(e);
+                const int32_t kk[4] = {k4.x, k4.y, k4.z, k4.w};
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (hit(kk[u]))
+                    {
+                        if (at >= pass && at < pass + STDP_LIST) list[at - pass] = int32_t(e + u);
+                        ++at;
+                    }
+            }
+        }
+        __syncthreads();
+        const int32_t mm = min(STDP_LIST, m - pass);
+        for (int32_t q = threadIdx.x; q < mm * sw; q += STDP_THREADS)
+            lbits[q] = bits[int64_t(list[q / sw]) * p.words + word0 + q % sw];
+        __syncthreads();
+        if (live)
+            for (int32_t q0 = 0; q0 < mm; q0 += 32)
+            {
+                // bits of the next 32 updates of this weight, then the chain
+                uint32_t b = 0;
+#pragma unroll
+                for (int u = 0; u < 32; ++u)
+                    b |= ((lbits[min(q0 + u, mm - 1) * sw + my_word] >> (i & 31)) & 1u) << u;
+                const int32_t n_up = min(32, mm - q0);
+                for (int u = 0; u < n_up; ++u) wr = stdp_update(wr, (b >> u) & 1u, cfg);
+            }
+        __syncthreads();
+    }
+    if (live) kz[i] = wr;
+}
 
 // Optional HIP-event timing of the main fused kernel (spk_kernel_timer).
 struct KernelTimer
@@ -655,7 +831,8 @@ KernelTimer &timer()
 }
 
 template <int S, bool DIAG>
-void launch_layer(const uint8_t *in, const half8 *frag, Summary *summary, float *pot, const FusedGeom &g, size_t lds,
+void launch_layer(const uint8_t *in, const half8 *frag, Summary *summary, float *pot, uint8_t *fs, const FusedGeom &g,
+                  size_t lds,
                   hipStream_t s)
 {
     static bool configured = false;  // per instantiation; the attribute is per function
@@ -677,7 +854,7 @@ void launch_layer(const uint8_t *in, const half8 *frag, Summary *summary, float 
         SPK_HIP(hipEventCreate(&e1));
         SPK_HIP(hipEventRecord(e0, s));
     }
-    fused_layer<S, DIAG><<<dim3(grid), dim3(64 * g.waves), lds, s>>>(in, frag, summary, pot, g);
+    fused_layer<S, DIAG><<<dim3(grid), dim3(64 * g.waves), lds, s>>>(in, frag, summary, pot, fs, g);
     SPK_LAUNCHED();
     if (tm.on)
     {
@@ -686,15 +863,15 @@ void launch_layer(const uint8_t *in, const half8 *frag, Summary *summary, float 
     }
 }
 
-void dispatch_layer(int steps, const uint8_t *in, const half8 *frag, Summary *summary, float *pot,
+void dispatch_layer(int steps, const uint8_t *in, const half8 *frag, Summary *summary, float *pot, uint8_t *fs,
                     const FusedGeom &g, size_t lds, hipStream_t s)
 {
     switch (steps)
     {
 #define CASE(k) \
     case k:                                                                                 \
-        return pot ? launch_layer<k, true>(in, frag, summary, pot, g, lds, s)                 \
-                   : launch_layer<k, false>(in, frag, summary, pot, g, lds, s);
+        return pot ? launch_layer<k, true>(in, frag, summary, pot, fs, g, lds, s)             \
+                   : launch_layer<k, false>(in, frag, summary, pot, fs, g, lds, s);
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
         CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
 #undef CASE
@@ -755,7 +932,6 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
         SPK_ARG(ho >= 1 && wo >= 1, "Input and kernel shapes are not compatible.");
         SPK_ARG(n < (1LL << 31), "Fused layer: too many samples.");
         const bool learn = nconfigs > 0;
-        SPK_ARG(!learn || out, "Fused layer: STDP needs the spike output.");
         int64_t pho = 0, pwo = 0;
         if (pooled)
         {
@@ -799,17 +975,21 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
         SPK_ARG(int64_t(cout) * ho * wo < (1LL << 31), "Fused layer: output plane too large.");
         SPK_ARG(ho * wo <= 8192, "Fused layer: convwta supports at most 8192 output pixels per sample.");
 
+        const StdpConfigs cfg = learn ? pack_configs(configs, nconfigs) : StdpConfigs{};
+        SPK_ARG(!learn || n * count < (1LL << 31), "Fused layer: too many winner slots.");
+
         hipStream_t s = S(stream);
-        const int64_t hw = ho * wo;
+        const int64_t hw = ho * wo, hwi = h * w;
         Scratch frag(int64_t(g.waves) * 2 * g.steps * 64 * 16, s), summary(n * hw * int64_t(sizeof(Summary)), s);
+        Scratch wkey(n * count * int64_t(sizeof(int32_t)), s), fsm(learn ? n * hwi * g.cin_pad : 0, s);
         prep_fragments<<<dim3(unsigned(cdiv(int64_t(g.waves) * 2 * g.steps * 64, 256))), 256, 0, s>>>(
             (const float *)kernel, frag.as<half8>(), g);
         SPK_LAUNCHED();
         dispatch_layer(g.steps, (const uint8_t *)in, frag.as<half8>(), summary.as<Summary>(), (float *)potentials,
-                       g, lds, s);
+                       learn ? fsm.as<uint8_t>() : nullptr, g, lds, s);
         fused_select<<<dim3(unsigned(n)), 256, size_t(hw) * 8, s>>>(summary.as<Summary>(), g.t, g.ho, g.wo,
                                                                     int32_t(ry), int32_t(rx), int32_t(count), winners,
-                                                                    counts);
+                                                                    counts, wkey.as<int32_t>());
         SPK_LAUNCHED();
         if (out || pooled)
         {
@@ -820,13 +1000,16 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
         }
         if (learn)
         {
-            StdpConfigs c = pack_configs(configs, nconfigs);
-            StdpGeom sg{g.t, g.cin, g.h, g.w, g.cout, g.kh, g.kw, g.ho, g.wo, 1, 1, g.pt, g.pl, n, count,
-                        int32_t(nconfigs)};
-            Scratch flag(sizeof(int32_t), s), bits(stdp_bits_bytes(sg, false), s);
-            clear_flag<<<1, 1, 0, s>>>(flag.as<int32_t>());
-            stdp_launch((const uint8_t *)in, (float *)kernel, (const uint8_t *)out, sg, c, winners, counts,
-                        bits.as<uint32_t>(), flag.as<int32_t>(), false, s);
+            const int64_t ksz = cin * kh * kw;
+            const FusedStdp fp{n * count, int32_t(count), g.cin, g.cin_pad, g.h, g.w, g.kh, g.kw, g.pt, g.pl,
+                               g.wo, int32_t(hw), g.t, int32_t(cdiv(ksz, 32))};
+            Scratch bits(n * count * fp.words * int64_t(sizeof(uint32_t)), s);
+            fused_records<<<dim3(unsigned(cdiv(n * count, 4))), 256, 0, s>>>(
+                fsm.as<uint8_t>(), summary.as<Summary>(), wkey.as<int32_t>(), fp, bits.as<uint32_t>());
+            SPK_LAUNCHED();
+            fused_stdp<<<dim3(unsigned(cout), unsigned(cdiv(ksz, STDP_THREADS))), STDP_THREADS, 0, s>>>(
+                (float *)kernel, bits.as<uint32_t>(), wkey.as<int32_t>(), fp, cfg.c[0]);
+            SPK_LAUNCHED();
         }
     });
 }

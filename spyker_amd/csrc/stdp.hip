@@ -30,16 +30,6 @@ namespace
 constexpr int APPLY_THREADS = 1024;
 constexpr int SLICE = APPLY_THREADS;  // weights per block, one per thread
 
-__device__ __forceinline__ float stdp_update(float w, bool potentiate, const spk_stdp_config &cfg)
-{
-    const double old = double(w);
-    double value = cfg.stabilize ? (old - cfg.lower) * (cfg.upper - old) : 1.0;
-    value = __builtin_fma(potentiate ? cfg.positive : cfg.negative, value, old);
-    const double hi = value < cfg.upper ? value : cfg.upper;
-    const double r = cfg.lower < hi ? hi : cfg.lower;
-    return float(r);
-}
-
 // Packs one bit per lane of a 64-wide pass into the record words.
 __device__ __forceinline__ void store_bits(uint32_t *rec, int32_t words, int32_t base, bool bit)
 {

@@ -20,6 +20,19 @@ struct StdpConfigs
     int32_t n;
 };
 
+// One update of one weight (stdp.cpp:121-137): the F64 formula with the
+// multiply-add the reference's build contracts `old + rate * value` into,
+// then the clamp, rounded once to the F32 weight.
+__device__ __forceinline__ float stdp_update(float w, bool potentiate, const spk_stdp_config &cfg)
+{
+    const double old = double(w);
+    double value = cfg.stabilize ? (old - cfg.lower) * (cfg.upper - old) : 1.0;
+    value = __builtin_fma(potentiate ? cfg.positive : cfg.negative, value, old);
+    const double hi = value < cfg.upper ? value : cfg.upper;
+    const double r = cfg.lower < hi ? hi : cfg.lower;
+    return float(r);
+}
+
 StdpConfigs pack_configs(const spk_stdp_config *configs, int64_t nconfigs);
 
 // Bytes of the per-winner bit records the launch needs as workspace.

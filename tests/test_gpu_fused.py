@@ -40,8 +40,9 @@ def oracle_step(orc, x, k, th, radius, count, pad, pool):
     return out, win, cnt, k2, pooled
 
 
-def spikes(rng, n, t, c, h, w, p=0.15):
-    return np.maximum.accumulate((rng.random((n, t, c, h, w)) < p / t * 3).astype(np.uint8), axis=1)
+def spikes(rng, n, t, c, h, w, p=0.15, cumulative=True):
+    s = (rng.random((n, t, c, h, w)) < p / t * 3).astype(np.uint8)
+    return np.maximum.accumulate(s, axis=1) if cumulative else s
 
 
 def check_winners(wn, win, cnt):
@@ -53,16 +54,18 @@ def check_winners(wn, win, cnt):
 
 
 @pytest.mark.parametrize("case", [
-    # n, t, cin, h, w, cout, k, pad, th, radius, count, pool
-    (6, 15, 30, 14, 14, 250, 3, (1, 1, 1, 1), 10.0, 1, 8, (3, 3)),
-    (5, 15, 4, 28, 28, 30, 5, (2, 2, 2, 2), 15.0, 3, 5, (2, 2)),
-    (3, 8, 8, 20, 17, 64, 7, (3, 3, 3, 3), 20.0, 5, 10, (2, 2)),
-    (4, 12, 3, 11, 9, 40, 3, (0, 1, 2, 0), 4.0, (1, 2), 4, None),
+    # n, t, cin, h, w, cout, k, pad, th, radius, count, pool, cumulative input
+    (6, 15, 30, 14, 14, 250, 3, (1, 1, 1, 1), 10.0, 1, 8, (3, 3), True),
+    (5, 15, 4, 28, 28, 30, 5, (2, 2, 2, 2), 15.0, 3, 5, (2, 2), True),
+    (3, 8, 8, 20, 17, 64, 7, (3, 3, 3, 3), 20.0, 5, 10, (2, 2), True),
+    (4, 12, 3, 11, 9, 40, 3, (0, 1, 2, 0), 4.0, (1, 2), 4, None, True),
+    # isolated spikes: potentials rise and fall, fire masks have gaps
+    (4, 10, 16, 12, 12, 48, 3, (1, 1, 1, 1), 6.0, 1, 6, (2, 2), False),
 ])
 def test_fused_exact_on_dyadic_weights(dev, orc, case):
-    n, t, cin, h, w, cout, k, pad, th, radius, count, pool = case
+    n, t, cin, h, w, cout, k, pad, th, radius, count, pool, cumulative = case
     rng = np.random.default_rng(11)
-    x = spikes(rng, n, t, cin, h, w, 0.5)
+    x = spikes(rng, n, t, cin, h, w, 0.5 if cumulative else 1.0, cumulative)
     kern = (rng.integers(40, 64, (cout, cin, k, k)) / 64).astype(np.float32)
     # scale the threshold to the potentials of this input
     pot = orc.conv(x.reshape(n * t, cin, h, w), kern, 1, pad)
