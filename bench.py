@@ -37,6 +37,10 @@ BYTES_PER_IT = CIN * H * H + COUT * H * H + COUT * 4 * 4  # U8 spikes in, spikes
 PEAK_FP32 = 157.3e12
 PEAK_FP16 = 2.5e15
 PEAK_HBM = 8.0e12
+# HBM bytes one full-batch fused_layer launch moves on this workload, from rocprofv3
+# FETCH_SIZE (x2.0, the factor a 512 MiB device copy reads as) + WRITE_SIZE (x1.0):
+# profiles/fused_layer_c2_traffic.txt.  Algorithmic: 90.3 MB input + 9.6 MB summaries/first-spike map.
+TRAFFIC_FUSED_LAYER = 104.8e6
 
 
 def make_inputs(dev, n=N, seed=2):
@@ -151,9 +155,20 @@ def main():
     ap.add_argument("--cpu-samples", type=int, default=192)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device (HBM counter calibration under rocprofv3)")
     a = ap.parse_args()
+    if a.copy_probe:
+        import torch
+        src = torch.ones(a.copy_probe << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.copy_probe << 20}))
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the layer path has no multi-GPU form in this build "
+        print("bench.py measures one GPU: the layer path runs on a single GPU "
               "(--gpus must be 1).", file=sys.stderr)
         sys.exit(2)
 
@@ -195,8 +210,9 @@ def main():
                                "STDP, pool 3x3 s3",
                    "path": "fused FP16-MFMA layer step, FP32 master weights"},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP16 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP16, "traffic": None, "kernel": "fused_layer",
-                     "kernel_ms": kern_s * 1e3, "launches_timed": launches.value},
+                     "frac": achieved / PEAK_FP16, "traffic": TRAFFIC_FUSED_LAYER, "traffic_unit": "bytes/launch",
+                     "kernel": "fused_layer", "kernel_ms": kern_s * 1e3, "launches_timed": launches.value,
+                     "flop_per_launch": FLOP_PER_IT * N * T_STEPS},
     }
     if not a.no_plain:
         conv_p = spk.Conv(CIN, COUT, K, pad=1, device=dev)

@@ -98,21 +98,21 @@ int32_t spk_cache_clear(void);
 
 /* -- elementwise --------------------------------------------------------- */
 
-/* Core::cuda_threshold(Dyn1 input, Scalar threshold, Scalar value), impl.h:235:
+/* Core::cuda_threshold(Dyn1 input, Scalar threshold, Scalar value), impl.h:240:
  * in place, x <= threshold ? value : x.  Types: F32, F16, F64. */
 int32_t spk_threshold(void *x, int32_t type, int64_t n, double threshold, double value, void *stream);
 
-/* Core::cuda_rank_fire(Dyn1 input, Dyn1 output, Scalar threshold), impl.h:247:
+/* Core::cuda_rank_fire(Dyn1 input, Dyn1 output, Scalar threshold), impl.h:256:
  * out = (in > threshold).  in: F32/F16/F64; out: U8 or F32. */
 int32_t spk_rank_fire(const void *in, int32_t in_type, void *out, int32_t out_type, int64_t n, double threshold,
                       void *stream);
 
-/* Core::cuda_cast for one dense buffer (src/spyker/impl.h:157-161), used by
+/* Core::cuda_cast for one dense buffer (src/spyker/impl.h:160), used by
  * the host layer for Tensor::to(type).  Pairs: U8->F32, U8->F16, F32->F16,
  * F16->F32, F32->F64, F64->F32, F32->U8 (truncating). */
 int32_t spk_cast(const void *in, int32_t in_type, void *out, int32_t out_type, int64_t n, void *stream);
 
-/* Core::cuda_pad(Dyn3 input, Dyn3 output, Len4 pad, Scalar value), impl.h:231:
+/* Core::cuda_pad(Dyn3 input, Dyn3 output, Len4 pad, Scalar value), impl.h:236:
  * planes of [h][w] into [h+pt+pb][w+pl+pr] filled with value.  Types: U8, F32. */
 int32_t spk_pad(const void *in, void *out, int32_t type, int64_t planes, int64_t h, int64_t w, int64_t pt, int64_t pl,
                 int64_t pb, int64_t pr, double value, void *stream);
@@ -120,32 +120,32 @@ int32_t spk_pad(const void *in, void *out, int32_t type, int64_t planes, int64_t
 /* -- pooling, inhibition, winner-take-all --------------------------------- */
 
 /* Core::cuda_rank_pool(Dyn4 input, Dyn4 output, Len2 kernel, Len2 stride,
- * Len4 pad), impl.h:259: per plane, zero pad then window max seeded with the
+ * Len4 pad), impl.h:268: per plane, zero pad then window max seeded with the
  * type's lowest value.  Types: U8, F32. */
 int32_t spk_rank_pool(const void *in, void *out, int32_t type, int64_t planes, int64_t h, int64_t w, int64_t kh,
                       int64_t kw, int64_t sh, int64_t sw, int64_t pt, int64_t pl, int64_t pb, int64_t pr, void *stream);
 
-/* Core::cuda_rank_inhibit(Dyn4 input, Scalar threshold), impl.h:263:
+/* Core::cuda_rank_inhibit(Dyn4 input, Scalar threshold), impl.h:272:
  * x is [n][t][c][hw], in place.  Types: F32. */
 int32_t spk_rank_inhibit(void *x, int32_t type, int64_t n, int64_t t, int64_t c, int64_t hw, double threshold,
                          void *stream);
 
 /* Core::cuda_rank_convwta(Dyn5 input, Len2 radius, Size count, Scalar
- * threshold), impl.h:271: x is [n][t][c][h][w]; winners (device) receives
+ * threshold), impl.h:280: x is [n][t][c][h][w]; winners (device) receives
  * n*count records, counts (device) n entries.  Types: F32, U8. */
 int32_t spk_rank_convwta(const void *x, int32_t type, int64_t n, int64_t t, int64_t c, int64_t h, int64_t w,
                          int64_t ry, int64_t rx, int64_t count, double threshold, spk_winner *winners,
                          int64_t *counts, void *stream);
 
 /* Core::cuda_rank_fcwta(Dyn3 input, Size radius, Size count, Scalar
- * threshold), impl.h:267: x is [n][t][f].  Types: F32. */
+ * threshold), impl.h:276: x is [n][t][f].  Types: F32. */
 int32_t spk_rank_fcwta(const void *x, int32_t type, int64_t n, int64_t t, int64_t f, int64_t radius, int64_t count,
                        double threshold, spk_winner *winners, int64_t *counts, void *stream);
 
 /* -- contractions and filters --------------------------------------------- */
 
 /* Core::cuda_conv(Dyn4 input, Dyn4 kernel, Dyn4 output, Len2 stride, Len4
- * pad), impl.h:221: cross-correlation, zero pad.  in: U8 or F32
+ * pad), impl.h:224: cross-correlation, zero pad.  in: U8 or F32
  * [n][cin][h][w]; kernel F32 [cout][cin][kh][kw]; out F32 (or F64 with
  * kernel F32, the high-precision check path).  FP32 accumulation is one fused
  * multiply-add per tap in (cin, ky, kx) order starting from +0. */
@@ -153,26 +153,26 @@ int32_t spk_conv(const void *in, int32_t in_type, const void *kernel, int32_t ke
                  int32_t out_type, int64_t n, int64_t cin, int64_t h, int64_t w, int64_t cout, int64_t kh, int64_t kw,
                  int64_t sy, int64_t sx, int64_t pt, int64_t pl, int64_t pb, int64_t pr, void *stream);
 
-/* Core::cuda_fc(Dyn3 input, Dyn2 kernel, Dyn3 output), impl.h:225:
+/* Core::cuda_fc(Dyn3 input, Dyn2 kernel, Dyn3 output), impl.h:228:
  * out[r][o] = sum_f in[r][f]*k[o][f], one fused multiply-add per f in
  * order.  in: U8 or F32; kernel/out F32. */
 int32_t spk_fc(const void *in, int32_t in_type, const void *kernel, void *out, int64_t rows, int64_t fin,
                int64_t fout, void *stream);
 
-/* Core::cuda_dog(Dyn4 input, Dyn4 kernel, Dyn4 output, Len4 pad), impl.h:201:
+/* Core::cuda_dog(Dyn4 input, Dyn4 kernel, Dyn4 output, Len4 pad), impl.h:200:
  * in F32 [planes][h][w], kernel F32 [2][count][k][k], out F32
  * [planes][count][ho][wo] = max(g1 - g2, 0). */
 int32_t spk_dog(const void *in, const void *kernel, void *out, int64_t planes, int64_t h, int64_t w, int64_t count,
                 int64_t k, int64_t pt, int64_t pl, int64_t pb, int64_t pr, void *stream);
 
-/* Core::cuda_gabor(Dyn4 input, Dyn4 kernel, Dyn4 output, Len4 pad), impl.h:205:
+/* Core::cuda_gabor(Dyn4 input, Dyn4 kernel, Dyn4 output, Len4 pad), impl.h:204:
  * in F32 [planes][h][w], kernel F32 [count][k][k], out F32 [planes][count][ho][wo]. */
 int32_t spk_gabor(const void *in, const void *kernel, void *out, int64_t planes, int64_t h, int64_t w, int64_t count,
                   int64_t k, int64_t pt, int64_t pl, int64_t pb, int64_t pr, void *stream);
 
 /* -- coding --------------------------------------------------------------- */
 
-/* Core::cuda_rank_code(Dyn2 input, Dyn3 output, bool sort), impl.h:243:
+/* Core::cuda_rank_code(Dyn2 input, Dyn3 output, bool sort), impl.h:248:
  * in F32 [n][len] -> out U8 [n][time][len], cumulative spikes.  sort=1 ranks
  * by value (descending, ties by ascending index); sort=0 is latency coding. */
 int32_t spk_rank_code(const void *in, int32_t in_type, void *out, int32_t out_type, int64_t n, int64_t len,
@@ -181,7 +181,7 @@ int32_t spk_rank_code(const void *in, int32_t in_type, void *out, int32_t out_ty
 /* -- learning ------------------------------------------------------------- */
 
 /* Core::cuda_rank_convstdp(Dyn5 input, Dyn4 kernel, Dyn5 output, configs,
- * Winners, Len2 stride, Len4 pad), impl.h:275-281.  in/out U8 spikes
+ * Winners, Len2 stride, Len4 pad), impl.h:284-289.  in/out U8 spikes
  * [n][t][cin][h][w] / [n][t][cout][ho][wo]; kernel F32 [cout][cin][kh][kw]
  * updated in place in (sample, winner) order; configs on the host; winners
  * and counts on the device ([n][maxcount], [n]).  Winners with c < 0 are
@@ -194,7 +194,7 @@ int32_t spk_rank_convstdp(const void *in, void *kernel, int32_t kernel_type, con
                           int64_t pb, int64_t pr, void *stream);
 
 /* Core::cuda_rank_fcstdp(Dyn3 input, Dyn2 kernel, Dyn3 output, configs,
- * Winners), impl.h:282-286.  in [n][t][fin] U8, out [n][t][fout] U8,
+ * Winners), impl.h:290-294.  in [n][t][fin] U8, out [n][t][fout] U8,
  * kernel F32 [fout][fin]. */
 int32_t spk_rank_fcstdp(const void *in, void *kernel, int32_t kernel_type, const void *out, int64_t n, int64_t t,
                         int64_t fin, int64_t fout, const spk_stdp_config *configs, int64_t nconfigs,
