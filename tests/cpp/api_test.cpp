@@ -1,0 +1,170 @@
+// Drives the C++ interface (include/spyker_amd/spyker.h) the way a user of
+// the reference's C++ API would.
+//
+//   api_test host          host-side checks (no GPU): tensors, views,
+//                          conversions, argument errors; prints "host ok"
+//   api_test layer DIR     reads DIR/{x.u8,k.f32,img.f32,shape.txt}, runs
+//                          code, DoG, conv, threshold, inhibit, convwta,
+//                          fire, Conv::stdp, pool and layerStep on the GPU,
+//                          writes the results to DIR for the Python test
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <string>
+
+#include "spyker_amd/spyker.h"
+
+using namespace Spyker;
+
+namespace
+{
+int failures = 0;
+
+void expect(bool cond, const char *what)
+{
+    if (!cond) std::cerr << "FAILED: " << what << "\n", ++failures;
+}
+
+template <typename F>
+void expectThrow(F f, const std::string &needle, const char *what)
+{
+    try
+    {
+        f();
+        std::cerr << "FAILED (no throw): " << what << "\n", ++failures;
+    }
+    catch (const std::runtime_error &e)
+    {
+        if (std::string(e.what()).find(needle) == std::string::npos)
+            std::cerr << "FAILED (message '" << e.what() << "'): " << what << "\n", ++failures;
+    }
+}
+
+Tensor readFile(const std::string &path, Type type, const Shape &shape)
+{
+    Tensor t(type, shape);
+    std::ifstream f(path, std::ios::binary);
+    f.read(static_cast<char *>(t.data()), t.bytes());
+    if (!f) throw std::runtime_error("cannot read " + path);
+    return t;
+}
+
+void writeFile(const std::string &path, const Tensor &t)
+{
+    Tensor h = t.to(Device(Kind::CPU));
+    std::ofstream f(path, std::ios::binary);
+    f.write(static_cast<const char *>(h.data()), h.bytes());
+}
+
+void writeWinners(const std::string &path, const Winners &w)
+{
+    std::ofstream f(path);
+    for (size_t i = 0; i < w.size(); ++i)
+    {
+        f << w[i].size();
+        for (auto &e : w[i]) f << " " << e.c << " " << e.t << " " << e.z << " " << e.y << " " << e.x;
+        f << "\n";
+    }
+}
+
+int host()
+{
+    Tensor a(Type::F32, {2, 3, 4});
+    expect(a.numel() == 24 && a.bytes() == 96 && a.dims() == 3, "shape queries");
+    for (Size i = 0; i < 24; ++i) a.data<float>()[i] = float(i) + 0.75f;
+    Tensor b = a.reshape({-1, 4});
+    expect(b.shape() == (Shape{6, 4}) && b.data() == a.data(), "reshape is a view");
+    Tensor c = a[1];
+    expect(c.shape() == (Shape{3, 4}) && c.data<float>()[0] == 12.75f, "index view");
+    Tensor d = a.to(Type::U8);
+    expect(d.type() == Type::U8 && d.data<uint8_t>()[5] == 5, "host conversion truncates");
+    Tensor e = a.copy();
+    e.fill(2.5);
+    expect(e.data<float>()[23] == 2.5f && a.data<float>()[23] == 23.75f, "copy and fill");
+    expect(Expand4(Shape{1, 2}).get() == (Shape{1, 2, 1, 2}), "Expand4 of two");
+    expect(Expand2(3).get() == (Shape{3, 3}), "Expand2 of one");
+    expectThrow([] { Conv(Device(Kind::CUDA), 0, 4, 3); }, "Input size can't be less than 1.", "conv input size");
+    expectThrow([] { DoG(Device(Kind::CUDA), 1, {}); }, "Input filters count can't be less then 1.", "dog filters");
+    expectThrow([] { Conv(Device(Kind::CUDA), 2, 4, 3, 1, Expand4(-1)); }, "Pad items are incorrect.", "conv pad");
+    expectThrow([&] { threshold(a, 0.5); }, "computes on the GPU", "host tensor is refused");
+    expectThrow([&] { a.reshape({5, 5}); }, "can't be reshaped", "bad reshape");
+    expect(GaborFilter(1, 90, 0.5, 4, 0).theta == 90 * 0.017453292519943295, "gabor degrees");
+    if (failures == 0) std::cout << "host ok\n";
+    return failures ? 1 : 0;
+}
+
+int layer(const std::string &dir)
+{
+    Size n, t, cin, h, w, cout, k, pad, radius, count, poolk;
+    double th, thc;
+    {
+        std::ifstream f(dir + "/shape.txt");
+        f >> n >> t >> cin >> h >> w >> cout >> k >> pad >> th >> radius >> count >> poolk >> thc;
+        if (!f) throw std::runtime_error("cannot read shape.txt");
+    }
+    const Device gpu(Kind::CUDA);
+    randomSeed(7);
+
+    // front end: DoG on the image, threshold, rank code
+    Tensor img = readFile(dir + "/img.f32", Type::F32, {n, 1, 28, 28}).to(gpu);
+    DoG filt(gpu, 1, {DoGFilter(1, 2), DoGFilter(2, 1)}, 1);
+    Tensor f = filt(img);
+    threshold(f, 0.01);
+    Tensor spikes = code(f, t);
+    writeFile(dir + "/dog.out", f);
+    writeFile(dir + "/code.out", spikes);
+
+    // the plain layer step on the given spikes and weights
+    Tensor x = readFile(dir + "/x.u8", Type::U8, {n, t, cin, h, w}).to(gpu);
+    Conv layerc(gpu, cin, cout, k, 1, pad);
+    layerc.kernel = readFile(dir + "/k.f32", Type::F32, {cout, cin, k, k}).to(gpu);
+    layerc.stdpconfig = {STDPConfig(0.004, -0.003)};
+    Tensor pot = layerc(x);
+    threshold(pot, th);
+    inhibit(pot);
+    Winners win = convwta(pot, radius, count);
+    Tensor out = fire(pot);
+    layerc.stdp(x, win, out);
+    Tensor pooled = pool(out, poolk);
+    writeFile(dir + "/pot.out", pot);
+    writeFile(dir + "/out.out", out);
+    writeFile(dir + "/kernel.out", layerc.kernel);
+    writeFile(dir + "/pooled.out", pooled);
+    writeWinners(dir + "/winners.out", win);
+
+    // the fused step on the same input from the original weights
+    Conv fused(gpu, cin, cout, k, 1, pad);
+    fused.kernel = readFile(dir + "/k.f32", Type::F32, {cout, cin, k, k}).to(gpu);
+    fused.stdpconfig = {STDPConfig(0.004, -0.003)};
+    LayerStep step = layerStep(fused, x, th, radius, count, poolk);
+    writeFile(dir + "/fused_out.out", step.output);
+    writeFile(dir + "/fused_pooled.out", step.pooled);
+    writeFile(dir + "/fused_kernel.out", fused.kernel);
+    writeWinners(dir + "/fused_winners.out", step.winners);
+
+    // errors surface as the reference's exceptions
+    expectThrow([&] { layerc.stdpconfig = {STDPConfig(0.004, -0.003)}; layerc.stdp(x, {}, out); },
+                "Winners can't be empty.", "stdp without winners");
+    expectThrow([&] { convwta(pot, radius, 0); }, "Winners count can't be less than 1.", "convwta count");
+    expectThrow([&] { pool(out, 0); }, "Kernel items are incorrect.", "pool kernel");
+    if (failures == 0) std::cout << "layer ok\n";
+    return failures ? 1 : 0;
+}
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    try
+    {
+        if (argc >= 2 && std::string(argv[1]) == "host") return host();
+        if (argc >= 3 && std::string(argv[1]) == "layer") return layer(argv[2]);
+        std::cerr << "usage: api_test host | api_test layer DIR\n";
+        return 2;
+    }
+    catch (const std::exception &e)
+    {
+        std::cerr << "error: " << e.what() << "\n";
+        return 1;
+    }
+}

@@ -1,0 +1,99 @@
+"""The C++ interface (include/spyker_amd/spyker.h) driven by tests/cpp/api_test.
+
+The driver uses the interface the way a C++ user of the reference would
+(Spyker::DoG, code, Conv, threshold, inhibit, convwta, fire, Conv::stdp,
+pool, plus the fused layerStep) and writes what it computed; the results are
+checked against the oracle here.
+"""
+
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "tests", "cpp", "api_test")
+LIB = os.path.join(ROOT, "spyker_amd", "libspyker_cpp.so")
+
+
+def _driver():
+    if not os.path.exists(DRIVER):
+        pytest.fail("tests/cpp/api_test is not built (python -m spyker_amd.build)")
+    return DRIVER
+
+
+def test_cpp_host_side():
+    r = subprocess.run([_driver(), "host"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "host ok" in r.stdout, r.stdout + r.stderr
+
+
+def test_cpp_library_exports_the_interface():
+    out = subprocess.run(["nm", "-DC", "--defined-only", LIB], capture_output=True, text=True, check=True).stdout
+    for name in ["Spyker::Conv::stdp(", "Spyker::Conv::Conv(", "Spyker::DoG::DoG(", "Spyker::Gabor::Gabor(",
+                 "Spyker::FC::stdp(", "Spyker::code(", "Spyker::fire(", "Spyker::threshold(", "Spyker::inhibit(",
+                 "Spyker::convwta(", "Spyker::fcwta(", "Spyker::pool(", "Spyker::conv(", "Spyker::fc(",
+                 "Spyker::pad(", "Spyker::layerStep(", "Spyker::Tensor::to("]:
+        assert name in out, name
+
+
+def _winners(path):
+    rows = []
+    for line in open(path):
+        v = [int(x) for x in line.split()]
+        rows.append([tuple(v[1 + 5 * k:6 + 5 * k]) for k in range(v[0])])
+    return rows
+
+
+@pytest.mark.gpu
+def test_cpp_layer_matches_oracle(tmp_path, orc):
+    from spyker_amd import synthetic
+    n, t, cin, h, w, cout, k, pad, radius, count, poolk = 4, 15, 8, 14, 14, 40, 3, 1, 1, 5, 2
+    rng = np.random.default_rng(21)
+    img = (synthetic.strokes(n, 28, seed=21).astype(np.float32) / 255)
+    x = np.maximum.accumulate((rng.random((n, t, cin, h, w)) < 0.04).astype(np.uint8), axis=1)
+    kern = (rng.integers(40, 64, (cout, cin, k, k)) / 64).astype(np.float32)  # exact in FP16, sums exact in FP32
+    pot = orc.conv(x.reshape(n * t, cin, h, w), kern, 1, (pad,) * 4)
+    th = float(np.quantile(pot, 0.9)) + 0.5 / 64
+    d = str(tmp_path)
+    img.tofile(os.path.join(d, "img.f32"))
+    x.tofile(os.path.join(d, "x.u8"))
+    kern.tofile(os.path.join(d, "k.f32"))
+    with open(os.path.join(d, "shape.txt"), "w") as f:
+        f.write("%d %d %d %d %d %d %d %d %.17g %d %d %d %.17g\n" % (n, t, cin, h, w, cout, k, pad, th, radius, count,
+                                                                   poolk, 0.0))
+    r = subprocess.run([_driver(), "layer", d], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "layer ok" in r.stdout, r.stdout + r.stderr
+
+    def rd(name, dtype, shape):
+        return np.fromfile(os.path.join(d, name), dtype=dtype).reshape(shape)
+
+    # front end: DoG (same pad 1, sigma pairs (1,2), (2,1)), threshold 0.01, rank code
+    g1, g2 = orc.gaussian_kernel(3, 1.0), orc.gaussian_kernel(3, 2.0)
+    dk = np.stack([np.stack([g1, g2]), np.stack([g2, g1])]).astype(np.float32)
+    dog = orc.threshold(orc.dog(img.reshape(n, 1, 28, 28), dk, (1, 1, 1, 1)), 0.01)
+    assert np.array_equal(rd("dog.out", np.float32, dog.shape), dog)
+    assert np.array_equal(rd("code.out", np.uint8, (n, t, 2, 28, 28)),
+                          orc.rank_code(dog.reshape(n, -1), t).reshape(n, t, 2, 28, 28))
+
+    # plain layer step
+    ho = h + 2 * pad - k + 1
+    opot = orc.inhibit(orc.threshold(pot.reshape(n, t, cout, ho, ho), th))
+    ow, oc = orc.convwta(opot, radius, count)
+    oout = orc.fire(opot)
+    ok = orc.convstdp(x, kern, oout, [(0.004, -0.003)], ow, oc, 1, (pad,) * 4)
+    assert oc.sum() > 0
+    assert np.array_equal(rd("pot.out", np.float32, opot.shape), opot)
+    assert np.array_equal(rd("out.out", np.uint8, oout.shape), oout)
+    assert np.array_equal(rd("kernel.out", np.float32, kern.shape), ok)
+    assert np.array_equal(rd("pooled.out", np.uint8, (n, t, cout, ho // poolk, ho // poolk)), orc.pool(oout, poolk))
+    want = [[(0, 0, int(ow[i, j]["z"]), int(ow[i, j]["y"]), int(ow[i, j]["x"])) for j in range(oc[i])]
+            for i in range(n)]
+    assert _winners(os.path.join(d, "winners.out")) == want
+
+    # the fused step from the same weights: exact on these dyadic weights
+    assert np.array_equal(rd("fused_out.out", np.uint8, oout.shape), oout)
+    assert np.array_equal(rd("fused_kernel.out", np.float32, kern.shape), ok)
+    assert np.array_equal(rd("fused_pooled.out", np.uint8, (n, t, cout, ho // poolk, ho // poolk)),
+                          orc.pool(oout, poolk))
+    assert _winners(os.path.join(d, "fused_winners.out")) == want
