@@ -48,6 +48,24 @@ def configs_array(configs):
     return arr
 
 
+class SparseSpikes:
+    """Event lists of a [N, T, C, H, W] spike tensor: list (i, j) holds the
+    spikes new at step j, as events ev[off[i*T+j]:off[i*T+j+1]] packed
+    z << 16 | y << 8 | x (src/spyker/base.h:493-528)."""
+
+    def __init__(self, off, ev, shape):
+        self.off, self.ev, self.shape = np.asarray(off, np.int64), np.asarray(ev, np.int32), tuple(shape)
+
+    def lists(self):
+        n, t = self.shape[:2]
+        return [[[(int(e) >> 16 & 255, int(e) >> 8 & 255, int(e) & 255) for e in self.ev[self.off[i * t + j]:
+                                                                                    self.off[i * t + j + 1]]]
+                 for j in range(t)] for i in range(n)]
+
+    def __eq__(self, o):
+        return self.shape == o.shape and np.array_equal(self.off, o.off) and np.array_equal(self.ev, o.ev)
+
+
 class _Lib:
     prefix = None
     path = None
@@ -131,12 +149,17 @@ class _Lib:
         return win.reshape(n, count), cnt
 
     def fcwta(self, x, radius, count, th=0.0):
-        x = np.ascontiguousarray(x, dtype=np.float32)
         n, t, f = x.shape
         win = np.zeros(n * count, WINNER)
         cnt = np.zeros(n, np.int64)
-        rc = self._f("rank_fcwta_f32")(_ptr(x), I64(n), I64(t), I64(f), I64(radius), I64(count), F32(th),
-                                        _ptr(win), _ptr(cnt))
+        if x.dtype == np.uint8:
+            x = np.ascontiguousarray(x)
+            rc = self._f("rank_fcwta_u8")(_ptr(x), I64(n), I64(t), I64(f), I64(radius), I64(count), _ptr(win),
+                                           _ptr(cnt))
+        else:
+            x = np.ascontiguousarray(x, dtype=np.float32)
+            rc = self._f("rank_fcwta_f32")(_ptr(x), I64(n), I64(t), I64(f), I64(radius), I64(count), F32(th),
+                                            _ptr(win), _ptr(cnt))
         if self.prefix == "ref_":
             self._check(rc, "fcwta")
         return win.reshape(n, count), cnt
@@ -190,6 +213,102 @@ class _Lib:
         rc = self._f("rank_fcstdp_f32")(_ptr(spk_in), I64(n), I64(t), I64(fin), _ptr(k), I64(fout), _ptr(spk_out),
                                         _ptr(cfg), I64(len(cfg)), _ptr(win), _ptr(cnt), I64(win.shape[1]))
         self._check(rc, "fcstdp")
+        return k
+
+    # -- sparse event path -------------------------------------------------
+
+    def _sparse_out(self, name, n, t, cap, *args):
+        off = np.zeros(n * t + 1, np.int64)
+        ev = np.zeros(max(cap, 1), np.int32)
+        f = self._f(name)
+        f.restype = C.c_int64
+        total = f(*args, _ptr(off), _ptr(ev), I64(cap))
+        if total < 0:
+            raise RuntimeError("%s: %s" % (name, "capacity exceeded" if total == -1 else "reference error"))
+        return off, ev[:total].copy()
+
+    def sparse_code(self, x, time, sort=True):
+        """x: [N, C, H, W] float32 -> SparseSpikes of shape [N, time, C, H, W]."""
+        x = np.ascontiguousarray(x, np.float32)
+        n, c, h, w = x.shape
+        off, ev = self._sparse_out("sparse_code_f32", n, time, n * c * h * w, _ptr(x), I64(n), I64(c), I64(h),
+                                   I64(w), I64(time), I32(int(bool(sort))))
+        return SparseSpikes(off, ev, (n, time, c, h, w))
+
+    def sparse_from_dense(self, x, threshold=0):
+        """x: [N, T, C, H, W] uint8 -> SparseSpikes (event at the first step > threshold)."""
+        x = np.ascontiguousarray(x, np.uint8)
+        n, t, c, h, w = x.shape
+        off, ev = self._sparse_out("sparse_from_dense_u8", n, t, n * c * h * w, _ptr(x), I64(n), I64(t), I64(c),
+                                   I64(h), I64(w), C.c_uint8(int(threshold)))
+        return SparseSpikes(off, ev, x.shape)
+
+    def sparse_to_dense(self, sp):
+        n, t, c, h, w = sp.shape
+        out = np.zeros(sp.shape, np.uint8)
+        self._f("sparse_to_dense_u8")(_ptr(sp.off), _ptr(sp.ev), I64(n), I64(t), I64(c), I64(h), I64(w), _ptr(out))
+        return out
+
+    def sparse_gather(self, sp):
+        n, t, c, h, w = sp.shape
+        out = np.zeros((n, c, h, w), np.uint8)
+        self._f("sparse_gather_u8")(_ptr(sp.off), _ptr(sp.ev), I64(n), I64(t), I64(c), I64(h), I64(w), _ptr(out))
+        return out
+
+    def sparse_pad(self, sp, pad):
+        pt, pl, pb, pr = pad
+        n, t, c, h, w = sp.shape
+        out = np.zeros(max(len(sp.ev), 1), np.int32)
+        self._f("sparse_pad")(_ptr(sp.off), _ptr(sp.ev), I64(n), I64(t), I64(pt), I64(pl), _ptr(out))
+        return SparseSpikes(sp.off.copy(), out[:len(sp.ev)].copy(), (n, t, c, h + pt + pb, w + pl + pr))
+
+    def sparse_pool(self, sp, kernel, pad=(0, 0, 0, 0)):
+        kh, kw = (kernel, kernel) if np.isscalar(kernel) else kernel
+        pt, pl, pb, pr = pad
+        n, t, c, h, w = sp.shape
+        ho, wo = (h + pt + pb - kh) // kh + 1, (w + pl + pr - kw) // kw + 1
+        off, ev = self._sparse_out("sparse_pool", n, t, max(len(sp.ev), 1), _ptr(sp.off), _ptr(sp.ev), I64(n),
+                                   I64(t), I64(c), I64(ho), I64(wo), I64(kh), I64(kw), I64(pt), I64(pl))
+        return SparseSpikes(off, ev, (n, t, c, ho, wo))
+
+    def sparse_inhibit(self, sp):
+        n, t, c, h, w = sp.shape
+        off, ev = self._sparse_out("sparse_inhibit", n, t, max(len(sp.ev), 1), _ptr(sp.off), _ptr(sp.ev), I64(n),
+                                   I64(t), I64(h), I64(w))
+        return SparseSpikes(off, ev, sp.shape)
+
+    def sparse_convwta(self, sp, radius, count):
+        ry, rx = (radius, radius) if np.isscalar(radius) else radius
+        n, t, c, h, w = sp.shape
+        win = np.zeros(n * count, WINNER)
+        cnt = np.zeros(n, np.int64)
+        self._f("sparse_convwta")(_ptr(sp.off), _ptr(sp.ev), I64(n), I64(t), I64(h), I64(w), I64(ry), I64(rx),
+                                  I64(count), _ptr(win), _ptr(cnt))
+        return win.reshape(n, count), cnt
+
+    def sparse_conv(self, sp, kernel, threshold, pad=(0, 0, 0, 0)):
+        kernel = np.ascontiguousarray(kernel, np.float32)
+        pt, pl, pb, pr = pad
+        n, t, cin, h, w = sp.shape
+        cout, _, kh, kw = kernel.shape
+        ho, wo = h + pt + pb - kh + 1, w + pl + pr - kw + 1
+        off, ev = self._sparse_out("sparse_conv", n, t, n * cout * ho * wo, _ptr(sp.off), _ptr(sp.ev), I64(n),
+                                   I64(t), I64(cin), I64(h), I64(w), _ptr(kernel), I64(cout), I64(kh), I64(kw),
+                                   F32(threshold), I64(pt), I64(pl), I64(pb), I64(pr))
+        return SparseSpikes(off, ev, (n, t, cout, ho, wo))
+
+    def sparse_stdp(self, sp, kernel, configs, winners, counts, pad=(0, 0, 0, 0)):
+        """Returns the updated kernel (a copy); pad (top, left, ...) as given to the conv."""
+        k = np.array(kernel, dtype=np.float32, copy=True)
+        cfg = configs_array(configs)
+        win = np.ascontiguousarray(winners, WINNER)
+        cnt = np.ascontiguousarray(counts, np.int64)
+        n, t, cin, h, w = sp.shape
+        cout, _, kh, kw = k.shape
+        rc = self._f("sparse_stdp")(_ptr(sp.off), _ptr(sp.ev), I64(n), I64(t), I64(cin), _ptr(k), I64(cout), I64(kh),
+                                    I64(kw), I64(pad[0]), I64(pad[1]), _ptr(cfg), I64(len(cfg)), _ptr(win),
+                                    _ptr(cnt), I64(win.shape[1]))
+        self._check(rc, "sparse_stdp")
         return k
 
     def gaussian_kernel(self, k, std):

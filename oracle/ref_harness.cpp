@@ -67,6 +67,36 @@ Winners get(const RefWinner *win, const int64_t *counts, int64_t n, int64_t maxc
     return w;
 }
 
+std::vector<STDPConfig> configs(const RefConfig *c, int64_t n);
+
+// list set (off, packed events) <-> the reference's Sparse5
+Sparse5 to_sparse(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t c, int64_t h, int64_t w)
+{
+    Sparse5 s = sparse_alloc(Len5(n, t, c, h, w));
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < t; ++j)
+            for (int64_t e = off[i * t + j]; e < off[i * t + j + 1]; ++e)
+                s(i, j).add(Spridx(U8((ev[e] >> 16) & 0xFF), U8((ev[e] >> 8) & 0xFF), U8(ev[e] & 0xFF)));
+    return s;
+}
+
+int64_t from_sparse(Sparse5 s, int64_t *off, int32_t *ev, int64_t cap)
+{
+    int64_t total = 0;
+    off[0] = 0;
+    for (int64_t i = 0; i < s.u; ++i)
+        for (int64_t j = 0; j < s.t; ++j)
+        {
+            for (const Spridx &x : s(i, j))
+            {
+                if (total >= cap) return -1;
+                ev[total++] = int32_t(uint32_t(x.z) << 16 | uint32_t(x.y) << 8 | uint32_t(x.x));
+            }
+            off[i * s.t + j + 1] = total;
+        }
+    return total;
+}
+
 std::vector<STDPConfig> configs(const RefConfig *c, int64_t n)
 {
     std::vector<STDPConfig> out;
@@ -187,6 +217,149 @@ int ref_normal_kernel(double *kernel, int64_t n, double mean, double std, int64_
     return guard([&] {
         random_seed(seed);
         normal_kernel(Vec1<F64>(kernel, n), mean, std);
+    });
+}
+
+// -- sparse event path (Core::sparse_*, src/spyker/impl.h:328-356) ----------
+
+int64_t ref_sparse_code_f32(const float *in, int64_t n, int64_t c, int64_t h, int64_t w, int64_t t, int32_t sort,
+                            int64_t *off, int32_t *ev, int64_t cap)
+{
+    int64_t r = -2;
+    guard([&] {
+        Sparse5 s = sparse_alloc(Len5(n, t, c, h, w));
+        sparse_code(Dyn4((void *)in, Type::F32, n, c, h, w), s, sort != 0);
+        r = from_sparse(s, off, ev, cap);
+        sparse_dealloc(s);
+    });
+    return r;
+}
+
+int64_t ref_sparse_from_dense_u8(const uint8_t *in, int64_t n, int64_t t, int64_t c, int64_t h, int64_t w,
+                                 uint8_t threshold, int64_t *off, int32_t *ev, int64_t cap)
+{
+    int64_t r = -2;
+    guard([&] {
+        Sparse5 s = sparse_alloc(Len5(n, t, c, h, w));
+        sparse_convert(Dyn5((void *)in, Type::U8, n, t, c, h, w), s, Scalar(threshold));
+        r = from_sparse(s, off, ev, cap);
+        sparse_dealloc(s);
+    });
+    return r;
+}
+
+void ref_sparse_to_dense_u8(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t c, int64_t h,
+                            int64_t w, uint8_t *out)
+{
+    guard([&] {
+        Sparse5 s = to_sparse(off, ev, n, t, c, h, w);
+        sparse_convert(s, Dyn5(out, Type::U8, n, t, c, h, w));
+        sparse_dealloc(s);
+    });
+}
+
+void ref_sparse_gather_u8(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t c, int64_t h,
+                          int64_t w, uint8_t *out)
+{
+    guard([&] {
+        Sparse5 s = to_sparse(off, ev, n, t, c, h, w);
+        sparse_gather(s, Dyn4(out, Type::U8, n, c, h, w));
+        sparse_dealloc(s);
+    });
+}
+
+void ref_sparse_pad(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t pt, int64_t pl,
+                    int32_t *out)
+{
+    guard([&] {
+        // shape only bounds the coordinates; 255 is the Spridx limit
+        Sparse5 s = to_sparse(off, ev, n, t, 255, 255, 255), o = sparse_alloc(Len5(n, t, 255, 255, 255));
+        sparse_pad(s, o, Len4(pt, pl, 0, 0));
+        std::vector<int64_t> oo(size_t(n * t + 1));
+        from_sparse(o, oo.data(), out, off[n * t]);
+        sparse_dealloc(s), sparse_dealloc(o);
+    });
+}
+
+int64_t ref_sparse_pool(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t c, int64_t ho,
+                        int64_t wo, int64_t kh, int64_t kw, int64_t pt, int64_t pl, int64_t *oof, int32_t *oev,
+                        int64_t cap)
+{
+    int64_t r = -2;
+    guard([&] {
+        Sparse5 s = to_sparse(off, ev, n, t, c, 255, 255), o = sparse_alloc(Len5(n, t, c, ho, wo));
+        sparse_pool(s, o, Len2(kh, kw), Len2(kh, kw), Len4(pt, pl, 0, 0));
+        r = from_sparse(o, oof, oev, cap);
+        sparse_dealloc(s), sparse_dealloc(o);
+    });
+    return r;
+}
+
+int64_t ref_sparse_inhibit(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t h, int64_t w,
+                           int64_t *oof, int32_t *oev, int64_t cap)
+{
+    int64_t r = -2;
+    guard([&] {
+        Sparse5 s = to_sparse(off, ev, n, t, 255, h, w), o = sparse_alloc(Len5(n, t, 255, h, w));
+        sparse_inhibit(s, o);
+        r = from_sparse(o, oof, oev, cap);
+        sparse_dealloc(s), sparse_dealloc(o);
+    });
+    return r;
+}
+
+void ref_sparse_convwta(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t h, int64_t w,
+                        int64_t ry, int64_t rx, int64_t count, RefWinner *winners, int64_t *counts)
+{
+    guard([&] {
+        Sparse5 s = to_sparse(off, ev, n, t, 255, h, w);
+        put(sparse_convwta(s, Len2(ry, rx), count), count, winners, counts);
+        sparse_dealloc(s);
+    });
+}
+
+int64_t ref_sparse_conv(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t cin, int64_t h,
+                        int64_t w, const float *kernel, int64_t cout, int64_t kh, int64_t kw, float threshold,
+                        int64_t pt, int64_t pl, int64_t pb, int64_t pr, int64_t *oof, int32_t *oev, int64_t cap)
+{
+    int64_t r = -2;
+    guard([&] {
+        const int64_t ho = h + pt + pb - kh + 1, wo = w + pl + pr - kw + 1;
+        Sparse5 s = to_sparse(off, ev, n, t, cin, h, w), o = sparse_alloc(Len5(n, t, cout, ho, wo));
+        sparse_conv(s, Dyn4((void *)kernel, Type::F32, cout, cin, kh, kw), Scalar(threshold), o, Len2(1, 1),
+                    Len4(pt, pl, pb, pr));
+        r = from_sparse(o, oof, oev, cap);
+        sparse_dealloc(s), sparse_dealloc(o);
+    });
+    return r;
+}
+
+int32_t ref_sparse_stdp(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t cin, float *kernel,
+                        int64_t cout, int64_t kh, int64_t kw, int64_t pt, int64_t pl, const RefConfig *cfg,
+                        int64_t ncfg, const RefWinner *winners, const int64_t *counts, int64_t maxcount)
+{
+    return guard([&] {
+        Sparse5 s = to_sparse(off, ev, n, t, cin, 255, 255);
+        try
+        {
+            sparse_stdp(s, Dyn4(kernel, Type::F32, cout, cin, kh, kw), configs(cfg, ncfg),
+                        get(winners, counts, n, maxcount), Len4(pt, pl, 0, 0));
+        }
+        catch (...)
+        {
+            sparse_dealloc(s);
+            throw;
+        }
+        sparse_dealloc(s);
+    });
+}
+
+int ref_rank_fcwta_u8(const uint8_t *x, int64_t n, int64_t t, int64_t f, int64_t radius, int64_t count,
+                      RefWinner *winners, int64_t *counts)
+{
+    return guard([&] {
+        auto r = cpu_rank_fcwta(Dyn3((void *)x, Type::U8, n, t, f), radius, count, Scalar(U8(0)));
+        put(r, count, winners, counts);
     });
 }
 }

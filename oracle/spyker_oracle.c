@@ -598,3 +598,358 @@ void orc_gabor_kernel(double *kernel, int64_t k, double sigma, double theta, dou
         }
     normalize(kernel, k * k);
 }
+
+/* ------------------------------------------------------------------------ */
+/* sparse event path (src/spyker/sparse/*.cpp)                              */
+/* ------------------------------------------------------------------------ */
+/*
+ * A sparse tensor of shape [n][t][c][h][w] holds, for every (sample i, step
+ * j), the list of spikes that are new at step j (base.h:493-528).  Here a
+ * list set is off[n*t + 1] (list i*t+j is events [off[i*t+j], off[i*t+j+1]))
+ * and events packed as z << 16 | y << 8 | x (the reference's U8 Spridx).
+ * Functions that make a list set write at most `cap` events and return the
+ * total, or -1 if it would not fit.
+ */
+
+#define EV(z, y, x) ((int32_t)(((uint32_t)(z) << 16) | ((uint32_t)(y) << 8) | (uint32_t)(x)))
+#define EZ(e) ((int64_t)(((uint32_t)(e) >> 16) & 0xFFu))
+#define EY(e) ((int64_t)(((uint32_t)(e) >> 8) & 0xFFu))
+#define EX(e) ((int64_t)((uint32_t)(e)&0xFFu))
+
+/* Lists from a first-spike-time map [n][c][h][w] (times >= t: no spike):
+ * each spike goes to list (i, time) in (z, y, x) scan order
+ * (sparse/code.cpp:56-66, sparse/opers.cpp:57-67). */
+static int64_t lists_from_times(const uint16_t *times, int64_t n, int64_t t, int64_t len, int64_t h, int64_t w,
+                                int64_t *off, int32_t *ev, int64_t cap)
+{
+    int64_t total = 0;
+    off[0] = 0;
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < t; ++j)
+        {
+            for (int64_t k = 0; k < len; ++k)
+                if (times[i * len + k] == j)
+                {
+                    if (total >= cap) return -1;
+                    const int64_t z = k / (h * w), y = (k / w) % h, x = k % w;
+                    ev[total++] = EV(z, y, x);
+                }
+            off[i * t + j + 1] = total;
+        }
+    return total;
+}
+
+/* sparse/code.cpp:98-132: the dense rank code's spike times, as events. */
+int64_t orc_sparse_code_f32(const float *in, int64_t n, int64_t c, int64_t h, int64_t w, int64_t t, int32_t sort,
+                            int64_t *off, int32_t *ev, int64_t cap)
+{
+    const int64_t len = c * h * w;
+    uint8_t *sp = (uint8_t *)malloc((size_t)(n * t * len));
+    uint16_t *times = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)(n * len));
+    if (sort)
+        orc_rank_code_sort_f32(in, n, len, t, sp, 1);
+    else
+        orc_rank_code_latency_f32(in, n, len, t, sp);
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t k = 0; k < len; ++k)
+        {
+            uint16_t f = (uint16_t)t;
+            for (int64_t j = t - 1; j >= 0; --j)
+                if (sp[(i * t + j) * len + k]) f = (uint16_t)j;
+            times[i * len + k] = f;
+        }
+    const int64_t r = lists_from_times(times, n, t, len, h, w, off, ev, cap);
+    free(sp);
+    free(times);
+    return r;
+}
+
+/* SparseTensor(tensor, threshold), sparse/opers.cpp:38-80: an element's event
+ * is at the first step whose value is > threshold. */
+int64_t orc_sparse_from_dense_u8(const uint8_t *in, int64_t n, int64_t t, int64_t c, int64_t h, int64_t w,
+                                 uint8_t threshold, int64_t *off, int32_t *ev, int64_t cap)
+{
+    const int64_t len = c * h * w;
+    uint16_t *times = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)(n * len));
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t k = 0; k < len; ++k)
+        {
+            uint16_t f = (uint16_t)t;
+            for (int64_t j = t - 1; j >= 0; --j)
+                if (in[(i * t + j) * len + k] > threshold) f = (uint16_t)j;
+            times[i * len + k] = f;
+        }
+    const int64_t r = lists_from_times(times, n, t, len, h, w, off, ev, cap);
+    free(times);
+    return r;
+}
+
+/* SparseTensor::dense(), sparse/opers.cpp:82-100: cumulative U8 spikes. */
+void orc_sparse_to_dense_u8(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t c, int64_t h,
+                            int64_t w, uint8_t *out)
+{
+    const int64_t len = c * h * w;
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < t; ++j)
+        {
+            uint8_t *o = out + (i * t + j) * len;
+            if (j == 0)
+                memset(o, 0, (size_t)len);
+            else
+                memcpy(o, o - len, (size_t)len);
+            for (int64_t e = off[i * t + j]; e < off[i * t + j + 1]; ++e)
+                o[(EZ(ev[e]) * h + EY(ev[e])) * w + EX(ev[e])] = 1;
+        }
+}
+
+/* Sparse::gather, sparse/opers.cpp:102-113: first spike step per element
+ * (t when none), [n][c][h][w]. */
+void orc_sparse_gather_u8(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t c, int64_t h,
+                          int64_t w, uint8_t *out)
+{
+    const int64_t len = c * h * w;
+    for (int64_t i = 0; i < n; ++i)
+    {
+        uint8_t *o = out + i * len;
+        memset(o, (int)t, (size_t)len);
+        for (int64_t j = t - 1; j >= 0; --j)
+            for (int64_t e = off[i * t + j]; e < off[i * t + j + 1]; ++e)
+                o[(EZ(ev[e]) * h + EY(ev[e])) * w + EX(ev[e])] = (uint8_t)j;
+    }
+}
+
+/* sparse/pad.cpp:36-56: every event moves by (top, left). */
+void orc_sparse_pad(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t pt, int64_t pl,
+                    int32_t *out)
+{
+    for (int64_t e = 0; e < off[n * t]; ++e) out[e] = EV(EZ(ev[e]), EY(ev[e]) + pt, EX(ev[e]) + pl);
+}
+
+/* sparse/pool.cpp:36-65 (stride = kernel): an event maps to pooled cell
+ * ((pt + y) / kh, (pl + x) / kw); the first event (step, then list order)
+ * reaching a cell is kept, in the input's list order; cells past the
+ * output are dropped. */
+int64_t orc_sparse_pool(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t c, int64_t ho,
+                        int64_t wo, int64_t kh, int64_t kw, int64_t pt, int64_t pl, int64_t *oof, int32_t *oev,
+                        int64_t cap)
+{
+    uint8_t *seen = (uint8_t *)malloc((size_t)(c * ho * wo));
+    int64_t total = 0;
+    oof[0] = 0;
+    for (int64_t i = 0; i < n; ++i)
+    {
+        memset(seen, 0, (size_t)(c * ho * wo));
+        for (int64_t j = 0; j < t; ++j)
+        {
+            for (int64_t e = off[i * t + j]; e < off[i * t + j + 1]; ++e)
+            {
+                const int64_t z = EZ(ev[e]), y = (uint16_t)(pt + EY(ev[e])) / kh, x = (uint16_t)(pl + EX(ev[e])) / kw;
+                if (y >= ho || x >= wo) continue;
+                if (seen[(z * ho + y) * wo + x]) continue;
+                seen[(z * ho + y) * wo + x] = 1;
+                if (total >= cap)
+                {
+                    free(seen);
+                    return -1;
+                }
+                oev[total++] = EV(z, y, x);
+            }
+            oof[i * t + j + 1] = total;
+        }
+    }
+    free(seen);
+    return total;
+}
+
+/* sparse/inhibit.cpp:36-66: a pixel keeps the channel of its first event
+ * (step, then list order); every event of that channel there is kept. */
+int64_t orc_sparse_inhibit(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t h, int64_t w,
+                           int64_t *oof, int32_t *oev, int64_t cap)
+{
+    int32_t *ch = (int32_t *)malloc(sizeof(int32_t) * (size_t)(h * w));
+    int64_t total = 0;
+    oof[0] = 0;
+    for (int64_t i = 0; i < n; ++i)
+    {
+        for (int64_t p = 0; p < h * w; ++p) ch[p] = -1;
+        for (int64_t e = off[i * t]; e < off[i * t + t]; ++e)
+            if (ch[EY(ev[e]) * w + EX(ev[e])] < 0) ch[EY(ev[e]) * w + EX(ev[e])] = (int32_t)EZ(ev[e]);
+        for (int64_t j = 0; j < t; ++j)
+        {
+            for (int64_t e = off[i * t + j]; e < off[i * t + j + 1]; ++e)
+                if (ch[EY(ev[e]) * w + EX(ev[e])] == EZ(ev[e]))
+                {
+                    if (total >= cap)
+                    {
+                        free(ch);
+                        return -1;
+                    }
+                    oev[total++] = ev[e];
+                }
+            oof[i * t + j + 1] = total;
+        }
+    }
+    free(ch);
+    return total;
+}
+
+/* sparse/wta.cpp:40-96: events in step then list order; an event is a
+ * winner unless its channel already won or it lies in the box (clipped to
+ * the tensor) of an earlier winner; at most `count`. */
+void orc_sparse_convwta(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t h, int64_t w,
+                        int64_t ry, int64_t rx, int64_t count, orc_winner *winners, int64_t *counts)
+{
+    for (int64_t i = 0; i < n; ++i)
+    {
+        int64_t found = 0;
+        for (int64_t j = 0; j < t && found < count; ++j)
+            for (int64_t e = off[i * t + j]; e < off[i * t + j + 1] && found < count; ++e)
+            {
+                const int64_t z = EZ(ev[e]), y = EY(ev[e]), x = EX(ev[e]);
+                int ok = 1;
+                for (int64_t k = 0; k < found && ok; ++k)
+                {
+                    const orc_winner *b = &winners[i * count + k];
+                    if (z == b->z) ok = 0;
+                    const int64_t ys = b->y - ry > 0 ? b->y - ry : 0, ye = b->y + ry < h - 1 ? b->y + ry : h - 1;
+                    const int64_t xs = b->x - rx > 0 ? b->x - rx : 0, xe = b->x + rx < w - 1 ? b->x + rx : w - 1;
+                    if (ys <= y && y <= ye && xs <= x && x <= xe) ok = 0;
+                }
+                if (ok) winners[i * count + found++] = (orc_winner){0, j, z, y, x};
+            }
+        counts[i] = found;
+    }
+}
+
+/* sparse/conv.cpp:46-100 (stride 1): per sample and output channel the
+ * potentials start at 0; the events of each step, in list order, add
+ * kernel[c][z][ky][kx] (one F32 add each) to every output position they
+ * reach; after a step with events, every potential > threshold spikes
+ * (output list order: channel, row, column) and becomes -FLT_MAX.  The
+ * output is returned as its first-spike-time map [n][cout][ho][wo]. */
+void orc_sparse_conv_times(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t cin, int64_t h,
+                           int64_t w, const float *kernel, int64_t cout, int64_t kh, int64_t kw, float threshold,
+                           int64_t pt, int64_t pl, int64_t pb, int64_t pr, uint16_t *times)
+{
+    const int64_t ho = h + pt + pb - kh + 1, wo = w + pl + pr - kw + 1;
+    float *pot = (float *)malloc(sizeof(float) * (size_t)(ho * wo));
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t c = 0; c < cout; ++c)
+        {
+            uint16_t *tm = times + (i * cout + c) * ho * wo;
+            for (int64_t p = 0; p < ho * wo; ++p) pot[p] = 0.0f, tm[p] = (uint16_t)t;
+            for (int64_t j = 0; j < t; ++j)
+            {
+                const int64_t e0 = off[i * t + j], e1 = off[i * t + j + 1];
+                for (int64_t e = e0; e < e1; ++e)
+                {
+                    const int64_t z = EZ(ev[e]), y = EY(ev[e]) + pt, x = EX(ev[e]) + pl;
+                    for (int64_t oy = 0; oy < ho; ++oy)
+                        for (int64_t ky = 0; ky < kh; ++ky)
+                            for (int64_t kx = 0; kx < kw; ++kx)
+                            {
+                                const int64_t ox = x - kx;
+                                if (oy + ky != y || ox < 0 || ox >= wo) continue;
+                                pot[oy * wo + ox] += kernel[((c * cin + z) * kh + ky) * kw + kx];
+                            }
+                }
+                if (e1 > e0)
+                    for (int64_t p = 0; p < ho * wo; ++p)
+                        if (pot[p] > threshold) tm[p] = (uint16_t)j, pot[p] = -FLT_MAX;
+            }
+        }
+    free(pot);
+}
+
+int64_t orc_sparse_conv(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t cin, int64_t h,
+                        int64_t w, const float *kernel, int64_t cout, int64_t kh, int64_t kw, float threshold,
+                        int64_t pt, int64_t pl, int64_t pb, int64_t pr, int64_t *oof, int32_t *oev, int64_t cap)
+{
+    const int64_t ho = h + pt + pb - kh + 1, wo = w + pl + pr - kw + 1;
+    uint16_t *times = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)(n * cout * ho * wo));
+    orc_sparse_conv_times(off, ev, n, t, cin, h, w, kernel, cout, kh, kw, threshold, pt, pl, pb, pr, times);
+    const int64_t r = lists_from_times(times, n, t, cout * ho * wo, ho, wo, oof, oev, cap);
+    free(times);
+    return r;
+}
+
+/* sparse/stdp.cpp:57-136: for each winner (samples, then list order) and
+ * each weight (z, ky, kx) of kernel[winner.z], the bit is "the last step
+ * with an event at input (z, y + ky, x + kx) of the padded sample is <=
+ * winner.t" (no event: t, so false); the update is the reference's F32
+ * formula, its F64 terms included:
+ *     mult = (F32)((k - lower) * (upper - k))         (1 without stabilize)
+ *     k = (F32)fma(rate, mult, k)                     (old + rate * mult, contracted)
+ *     k = (F32)max(lower, min(k, upper))
+ * Returns -1 when a winner's config index is out of range (updates before
+ * it stay applied, as in the reference). */
+int32_t orc_sparse_stdp(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t cin, float *kernel,
+                        int64_t cout, int64_t kh, int64_t kw, int64_t pt, int64_t pl,
+                        const orc_stdp_config *configs, int64_t nconfigs, const orc_winner *winners,
+                        const int64_t *counts, int64_t maxcount)
+{
+    const int64_t ksz = cin * kh * kw;
+    uint16_t *direct = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)ksz);
+    (void)cout;
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t k = 0; k < counts[i]; ++k)
+        {
+            const orc_winner *wn = &winners[i * maxcount + k];
+            if (wn->c < 0) continue;
+            if (wn->c >= nconfigs)
+            {
+                free(direct);
+                return -1;
+            }
+            for (int64_t q = 0; q < ksz; ++q) direct[q] = (uint16_t)t;
+            for (int64_t j = 0; j < t; ++j)
+                for (int64_t e = off[i * t + j]; e < off[i * t + j + 1]; ++e)
+                {
+                    const int64_t z = EZ(ev[e]), y = EY(ev[e]) + pt, x = EX(ev[e]) + pl;
+                    if (wn->y <= y && y <= wn->y + kh - 1 && wn->x <= x && x <= wn->x + kw - 1)
+                        direct[(z * kh + (y - wn->y)) * kw + (x - wn->x)] = (uint16_t)j;
+                }
+            const orc_stdp_config *cf = &configs[wn->c];
+            float *kz = kernel + wn->z * ksz;
+            for (int64_t q = 0; q < ksz; ++q)
+            {
+                const float mult = cf->stabilize ? (float)(((double)kz[q] - cf->lower) * (cf->upper - (double)kz[q]))
+                                                 : 1.0f;
+                kz[q] = (float)fma(direct[q] <= wn->t ? cf->positive : cf->negative, (double)mult, (double)kz[q]);
+                const double hi = (double)kz[q] < cf->upper ? (double)kz[q] : cf->upper;
+                kz[q] = (float)(cf->lower < hi ? hi : cf->lower);
+            }
+        }
+    free(direct);
+    return 0;
+}
+
+/* Integer branch of fcwta, src/spyker/cpu/wta.cpp:66-85,87-126: value =
+ * t - first nonzero step (0 if none); first maximum; stop at 0; the
+ * [x - radius, x + radius] window is cleared. */
+void orc_rank_fcwta_u8(const uint8_t *x, int64_t n, int64_t t, int64_t f, int64_t radius, int64_t count,
+                       orc_winner *winners, int64_t *counts)
+{
+    uint16_t *temp = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)f);
+    for (int64_t s = 0; s < n; ++s)
+    {
+        const uint8_t *xs = x + s * t * f;
+        memset(temp, 0, sizeof(uint16_t) * (size_t)f);
+        for (int64_t i = t - 1; i >= 0; --i)
+            for (int64_t j = 0; j < f; ++j)
+                if (xs[i * f + j]) temp[j] = (uint16_t)(t - i);
+        int64_t k = 0;
+        for (; k < count; ++k)
+        {
+            int64_t best = 0;
+            for (int64_t j = 1; j < f; ++j)
+                if (temp[best] < temp[j]) best = j;
+            if (temp[best] == 0) break;
+            winners[s * count + k] = (orc_winner){0, 0, 0, 0, best};
+            const int64_t a = best - radius > 0 ? best - radius : 0, b = best + radius + 1 < f ? best + radius + 1 : f;
+            for (int64_t j = a; j < b; ++j) temp[j] = 0;
+        }
+        counts[s] = k;
+    }
+    free(temp);
+}

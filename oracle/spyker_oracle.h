@@ -102,6 +102,40 @@ void orc_gabor_f32(const float *in, int64_t planes, int64_t h, int64_t w, const 
 void orc_gaussian_kernel(double *kernel, int64_t k, double std);
 void orc_gabor_kernel(double *kernel, int64_t k, double sigma, double theta, double gamma, double lambda, double psi);
 
+/* -- sparse event path (src/spyker/sparse/*.cpp) ------------------------ */
+/* A list set: off[n*t + 1] and events packed z << 16 | y << 8 | x; list
+ * (i, j) is [off[i*t+j], off[i*t+j+1]).  Producers return the event total,
+ * or -1 when more than `cap` events would be written. */
+int64_t orc_sparse_code_f32(const float *in, int64_t n, int64_t c, int64_t h, int64_t w, int64_t t, int32_t sort,
+                            int64_t *off, int32_t *ev, int64_t cap);
+int64_t orc_sparse_from_dense_u8(const uint8_t *in, int64_t n, int64_t t, int64_t c, int64_t h, int64_t w,
+                                 uint8_t threshold, int64_t *off, int32_t *ev, int64_t cap);
+void orc_sparse_to_dense_u8(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t c, int64_t h,
+                            int64_t w, uint8_t *out);
+void orc_sparse_gather_u8(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t c, int64_t h,
+                          int64_t w, uint8_t *out);
+void orc_sparse_pad(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t pt, int64_t pl,
+                    int32_t *out);
+int64_t orc_sparse_pool(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t c, int64_t ho,
+                        int64_t wo, int64_t kh, int64_t kw, int64_t pt, int64_t pl, int64_t *oof, int32_t *oev,
+                        int64_t cap);
+int64_t orc_sparse_inhibit(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t h, int64_t w,
+                           int64_t *oof, int32_t *oev, int64_t cap);
+void orc_sparse_convwta(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t h, int64_t w,
+                        int64_t ry, int64_t rx, int64_t count, orc_winner *winners, int64_t *counts);
+void orc_sparse_conv_times(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t cin, int64_t h,
+                           int64_t w, const float *kernel, int64_t cout, int64_t kh, int64_t kw, float threshold,
+                           int64_t pt, int64_t pl, int64_t pb, int64_t pr, uint16_t *times);
+int64_t orc_sparse_conv(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t cin, int64_t h,
+                        int64_t w, const float *kernel, int64_t cout, int64_t kh, int64_t kw, float threshold,
+                        int64_t pt, int64_t pl, int64_t pb, int64_t pr, int64_t *oof, int32_t *oev, int64_t cap);
+int32_t orc_sparse_stdp(const int64_t *off, const int32_t *ev, int64_t n, int64_t t, int64_t cin, float *kernel,
+                        int64_t cout, int64_t kh, int64_t kw, int64_t pt, int64_t pl,
+                        const orc_stdp_config *configs, int64_t nconfigs, const orc_winner *winners,
+                        const int64_t *counts, int64_t maxcount);
+void orc_rank_fcwta_u8(const uint8_t *x, int64_t n, int64_t t, int64_t f, int64_t radius, int64_t count,
+                       orc_winner *winners, int64_t *counts);
+
 #ifdef __cplusplus
 }
 #endif
