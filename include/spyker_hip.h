@@ -95,6 +95,10 @@ int32_t spk_dealloc(void *ptr);
 int32_t spk_memory_taken(int64_t *bytes);
 int32_t spk_memory_used(int64_t *bytes);
 int32_t spk_cache_clear(void);
+/* Core::copy on the device (src/spyker/impl.h:154-157, cuda_copy): copies
+ * `bytes` between any two host or device buffers, ordered on `stream`; a copy
+ * to or from pageable host memory returns when it is done. */
+int32_t spk_copy(const void *src, void *dst, int64_t bytes, void *stream);
 
 /* -- elementwise --------------------------------------------------------- */
 
@@ -138,7 +142,8 @@ int32_t spk_rank_convwta(const void *x, int32_t type, int64_t n, int64_t t, int6
                          int64_t *counts, void *stream);
 
 /* Core::cuda_rank_fcwta(Dyn3 input, Size radius, Size count, Scalar
- * threshold), impl.h:276: x is [n][t][f].  Types: F32. */
+ * threshold), impl.h:276: x is [n][t][f].  Types: F32, and U8 (the
+ * reference's integer branch, value = t - first spike step, cpu/wta.cpp:66-85). */
 int32_t spk_rank_fcwta(const void *x, int32_t type, int64_t n, int64_t t, int64_t f, int64_t radius, int64_t count,
                        double threshold, spk_winner *winners, int64_t *counts, void *stream);
 
@@ -199,6 +204,88 @@ int32_t spk_rank_convstdp(const void *in, void *kernel, int32_t kernel_type, con
 int32_t spk_rank_fcstdp(const void *in, void *kernel, int32_t kernel_type, const void *out, int64_t n, int64_t t,
                         int64_t fin, int64_t fout, const spk_stdp_config *configs, int64_t nconfigs,
                         const spk_winner *winners, const int64_t *counts, int64_t maxcount, void *stream);
+
+/* -- sparse event path (Core::sparse_*, impl.h:328-356; sparse/*.cpp) ------- */
+
+/* A sparse spike tensor of shape [n][t][c][h][w] holds, for every sample i
+ * and step j, the list of spikes that are new at step j (base.h:493-528).
+ * Device layout: offsets[n*t + 1] (int64) and events[total] (int32 packed
+ * z << 16 | y << 8 | x, the reference's U8 Spridx); list (i, j) is
+ * events[offsets[i*t + j] .. offsets[i*t + j + 1]).  c, h and w are at most
+ * 255, t at most 4096 and n at most 65535.  A function that makes a sparse tensor allocates its offsets and
+ * events with spk_alloc and fills every field; release it with
+ * spk_sparse_release.  Making one reads its size back to the host, so those
+ * calls synchronize their stream. */
+typedef struct spk_sparse
+{
+    int64_t n, t, c, h, w;
+    int64_t total;
+    int64_t *offsets;
+    int32_t *events;
+} spk_sparse;
+
+/* Frees what a producer allocated and clears the pointers. */
+int32_t spk_sparse_release(spk_sparse *s);
+
+/* SparseTensor(tensor, threshold) (sparse/opers.cpp:38-80): an element's
+ * event is at the first step whose value is > threshold.  in: [n][t][c][h][w],
+ * types U8, F32. */
+int32_t spk_sparse_from_dense(const void *in, int32_t type, int64_t n, int64_t t, int64_t c, int64_t h, int64_t w,
+                              double threshold, spk_sparse *out, void *stream);
+
+/* Sparse::code (Core::sparse_code, impl.h:350; sparse/code.cpp:98-132): rank
+ * coding of F32 [n][c][h][w] into `time` steps; sort = 0 is latency coding. */
+int32_t spk_sparse_code(const void *in, int32_t type, int64_t n, int64_t c, int64_t h, int64_t w, int64_t time,
+                        int32_t sort, spk_sparse *out, void *stream);
+
+/* SparseTensor::dense (Core::sparse_convert, impl.h:340): cumulative U8
+ * spikes [n][t][c][h][w]. */
+int32_t spk_sparse_to_dense(const spk_sparse *in, void *out, void *stream);
+
+/* Sparse::gather (Core::sparse_gather, impl.h:348): first spike step of
+ * every element, t where none, [n][c][h][w].  Types: U8, U16, I32, F32. */
+int32_t spk_sparse_gather(const spk_sparse *in, void *out, int32_t type, void *stream);
+
+/* Sparse::pad (Core::sparse_pad, impl.h:346): every event moves by (pt, pl). */
+int32_t spk_sparse_pad(const spk_sparse *in, int64_t pt, int64_t pl, int64_t pb, int64_t pr, spk_sparse *out,
+                       void *stream);
+
+/* Sparse::pool (Core::sparse_pool, impl.h:344; sparse/pool.cpp): stride must
+ * equal the kernel; the first event reaching a pooled cell is kept, in the
+ * input's list order. */
+int32_t spk_sparse_pool(const spk_sparse *in, int64_t kh, int64_t kw, int64_t sh, int64_t sw, int64_t pt, int64_t pl,
+                        int64_t pb, int64_t pr, spk_sparse *out, void *stream);
+
+/* Sparse::inhibit (Core::sparse_inhibit, impl.h:352): a pixel keeps the
+ * channel of its first event. */
+int32_t spk_sparse_inhibit(const spk_sparse *in, spk_sparse *out, void *stream);
+
+/* Sparse::convwta (Core::sparse_convwta, impl.h:354; sparse/wta.cpp): first
+ * come in step and list order, channel and box exclusion; winners/counts on
+ * the device as for spk_rank_convwta; count at most 3072. */
+int32_t spk_sparse_convwta(const spk_sparse *in, int64_t ry, int64_t rx, int64_t count, spk_winner *winners,
+                           int64_t *counts, void *stream);
+
+/* Conv::operator()(SparseTensor, threshold) (Core::sparse_conv, impl.h:342;
+ * sparse/conv.cpp): stride 1; per sample and output neuron the F32
+ * potential adds kernel[c][z][ky][kx] once per input event in list order,
+ * spikes the first time a step with events leaves it > threshold, and then
+ * never again.  kernel: F32 [cout][c][kh][kw] (the reference reads any
+ * kernel as F32), c*kh*kw at most 480. */
+int32_t spk_sparse_conv(const spk_sparse *in, const void *kernel, int32_t kernel_type, int64_t cout, int64_t kh,
+                        int64_t kw, double threshold, int64_t sy, int64_t sx, int64_t pt, int64_t pl, int64_t pb,
+                        int64_t pr, spk_sparse *out, void *stream);
+
+/* Conv::stdp(SparseTensor, Winners) (Core::sparse_stdp, impl.h:356;
+ * sparse/stdp.cpp): for each winner in (sample, rank) order and each weight
+ * (z, ky, kx) of kernel[winner.z]: potentiate iff the last event at input
+ * (z, winner.y + ky - pt, winner.x + kx - pl) is at a step <= winner.t; the
+ * update is the sparse path's F32 rule.  Winners with c < 0 are skipped; a
+ * c >= nconfigs fails the call before any weight changes. */
+int32_t spk_sparse_convstdp(const spk_sparse *in, void *kernel, int32_t kernel_type, int64_t cout, int64_t kh,
+                            int64_t kw, const spk_stdp_config *configs, int64_t nconfigs, const spk_winner *winners,
+                            const int64_t *counts, int64_t maxcount, int64_t pt, int64_t pl, int64_t pb, int64_t pr,
+                            void *stream);
 
 /* -- fused layer step (FP16 MFMA) ------------------------------------------ */
 

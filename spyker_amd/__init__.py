@@ -30,7 +30,7 @@ from ._lib import call
 __all__ = [
     "DoGFilter", "GaborFilter", "STDPConfig", "Winner", "Winners", "DoG", "Gabor", "Conv", "FC", "threshold",
     "fire", "code", "pool", "inhibit", "convwta", "fcwta", "conv", "fc", "pad", "random_seed", "expand2", "expand4",
-    "layer_step",
+    "layer_step", "sparse", "SparseTensor", "to_sparse",
 ]
 
 _TYPES = {torch.uint8: _lib.U8, torch.float16: _lib.F16, torch.float32: _lib.F32, torch.float64: _lib.F64,
@@ -468,10 +468,17 @@ class Conv:
         self.kernel = torch.from_numpy(ker).to(_dtype(dtype)).to(_device(device))
         self.stdpconfig: List[STDPConfig] = []
 
-    def __call__(self, array: torch.Tensor) -> torch.Tensor:
+    def __call__(self, array, threshold: float = None):
+        if isinstance(array, sparse.SparseTensor):  # opers.cpp:866-871
+            return sparse.conv(array, self.kernel, 0.0 if threshold is None else threshold, self._stride, self._pad)
         return conv(array, self.kernel, self._stride, self._pad)
 
-    def stdp(self, array: torch.Tensor, winners, output: torch.Tensor) -> None:
+    def stdp(self, array, winners, output: torch.Tensor = None) -> None:
+        if isinstance(array, sparse.SparseTensor):  # opers.cpp:882-889
+            sparse.conv_stdp(array, self.kernel, self.stdpconfig, winners, self._pad)
+            return
+        if output is None:
+            raise RuntimeError("STDP: dense input needs the output spikes.")
         _dev(array)
         _dev(output)
         if len(winners) == 0:
@@ -568,3 +575,13 @@ class FC:
         n, t, fin = a.shape
         call("spk_rank_fcstdp", _ptr(a), _ptr(self.kernel), _type(self.kernel), _ptr(o), n, t, fin,
              self.kernel.shape[0], cfg, ncfg, _ptr(wn.data), _ptr(wn.counts), wn.data.shape[1], _stream(a))
+
+
+from . import sparse  # noqa: E402
+from .sparse import SparseTensor  # noqa: E402
+
+
+def to_sparse(*arrays: torch.Tensor, threshold: float = 0.0):
+    """Dense [n, t, c, h, w] spikes to sparse lists (src/python/spyker/utils.py:506-529)."""
+    out = tuple(SparseTensor(a, threshold) for a in arrays)
+    return out[0] if len(out) == 1 else out

@@ -24,6 +24,11 @@ class spk_stdp_config(C.Structure):
                 ("lower", C.c_double), ("upper", C.c_double)]
 
 
+class spk_sparse(C.Structure):
+    _fields_ = [("n", C.c_int64), ("t", C.c_int64), ("c", C.c_int64), ("h", C.c_int64), ("w", C.c_int64),
+                ("total", C.c_int64), ("offsets", C.c_void_p), ("events", C.c_void_p)]
+
+
 # name -> argument types (all return int32 status unless listed in _RESTYPE)
 SIGNATURES = {
     "spk_last_error": [],
@@ -36,6 +41,7 @@ SIGNATURES = {
     "spk_memory_taken": [_p],
     "spk_memory_used": [_p],
     "spk_cache_clear": [],
+    "spk_copy": [_p, _p, _i64, _p],
     "spk_threshold": [_p, _i32, _i64, _f64, _f64, _p],
     "spk_rank_fire": [_p, _i32, _p, _i32, _i64, _f64, _p],
     "spk_cast": [_p, _i32, _p, _i32, _i64, _p],
@@ -51,6 +57,17 @@ SIGNATURES = {
     "spk_rank_code": [_p, _i32, _p, _i32, _i64, _i64, _i64, _i32, _p],
     "spk_rank_convstdp": [_p, _p, _i32, _p] + [_i64] * 10 + [_p, _i64, _p, _p, _i64] + [_i64] * 6 + [_p],
     "spk_rank_fcstdp": [_p, _p, _i32, _p, _i64, _i64, _i64, _i64, _p, _i64, _p, _p, _i64, _p],
+    "spk_sparse_release": [_p],
+    "spk_sparse_from_dense": [_p, _i32, _i64, _i64, _i64, _i64, _i64, _f64, _p, _p],
+    "spk_sparse_code": [_p, _i32, _i64, _i64, _i64, _i64, _i64, _i32, _p, _p],
+    "spk_sparse_to_dense": [_p, _p, _p],
+    "spk_sparse_gather": [_p, _p, _i32, _p],
+    "spk_sparse_pad": [_p, _i64, _i64, _i64, _i64, _p, _p],
+    "spk_sparse_pool": [_p] + [_i64] * 8 + [_p, _p],
+    "spk_sparse_inhibit": [_p, _p, _p],
+    "spk_sparse_convwta": [_p, _i64, _i64, _i64, _p, _p, _p],
+    "spk_sparse_conv": [_p, _p, _i32, _i64, _i64, _i64, _f64] + [_i64] * 6 + [_p, _p],
+    "spk_sparse_convstdp": [_p, _p, _i32, _i64, _i64, _i64, _p, _i64, _p, _p, _i64] + [_i64] * 4 + [_p],
     "spk_stdp_layer_fused": [_p] + [_i64] * 5 + [_p] + [_i64] * 7 + [_f64] + [_i64] * 3 + [_p, _i64, _p, _p]
                             + [_i64] * 4 + [_p, _p, _p, _p],
     "spk_kernel_timer": [_i32],

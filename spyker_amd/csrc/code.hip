@@ -14,7 +14,7 @@
 //      (time <= t), written coalesced per step.
 // sort = false: latency coding, time = U16(end - scale * (x - min)) with the
 // subtraction fused as in the reference build (one fma).
-#include "common.hpp"
+#include "code.hpp"
 
 namespace spk
 {
@@ -151,6 +151,36 @@ __global__ void spikes_from_times(const uint16_t *__restrict__ times, O *__restr
 
 using namespace spk;
 
+namespace spk
+{
+void rank_times_launch(const float *in, int64_t n, int64_t len, int64_t time, bool sort, uint16_t *times,
+                       hipStream_t s)
+{
+    if (sort)
+    {
+        Scratch k0(n * len * int64_t(sizeof(uint64_t)), s), k1(n * len * int64_t(sizeof(uint64_t)), s);
+        const int64_t chunks = cdiv(len, CHUNK);
+        chunk_sort<<<dim3(unsigned(chunks), unsigned(n)), SORT_THREADS, 0, s>>>(in, k0.as<uint64_t>(), len);
+        SPK_LAUNCHED();
+        uint64_t *src = k0.as<uint64_t>(), *dst = k1.as<uint64_t>();
+        const unsigned blocks = unsigned(cdiv(len, 256) < 1024 ? cdiv(len, 256) : 1024);
+        for (int64_t run = CHUNK; run < len; run <<= 1)
+        {
+            merge_runs<<<dim3(blocks, unsigned(n)), 256, 0, s>>>(src, dst, len, run);
+            SPK_LAUNCHED();
+            std::swap(src, dst);
+        }
+        rank_times<<<dim3(blocks, unsigned(n)), 256, 0, s>>>(src, times, len, int32_t(time));
+        SPK_LAUNCHED();
+    }
+    else
+    {
+        latency_times<<<dim3(unsigned(n)), 1024, 0, s>>>(in, times, len, int32_t(time));
+        SPK_LAUNCHED();
+    }
+}
+}  // namespace spk
+
 extern "C" {
 
 int32_t spk_rank_code(const void *in, int32_t in_type, void *out, int32_t out_type, int64_t n, int64_t len,
@@ -167,30 +197,7 @@ int32_t spk_rank_code(const void *in, int32_t in_type, void *out, int32_t out_ty
         if (n == 0) return;
         hipStream_t s = S(stream);
         Scratch times(n * len * int64_t(sizeof(uint16_t)), s);
-        if (sort)
-        {
-            Scratch k0(n * len * int64_t(sizeof(uint64_t)), s), k1(n * len * int64_t(sizeof(uint64_t)), s);
-            const int64_t chunks = cdiv(len, CHUNK);
-            chunk_sort<<<dim3(unsigned(chunks), unsigned(n)), SORT_THREADS, 0, s>>>((const float *)in,
-                                                                                   k0.as<uint64_t>(), len);
-            SPK_LAUNCHED();
-            uint64_t *src = k0.as<uint64_t>(), *dst = k1.as<uint64_t>();
-            const unsigned blocks = unsigned(cdiv(len, 256) < 1024 ? cdiv(len, 256) : 1024);
-            for (int64_t run = CHUNK; run < len; run <<= 1)
-            {
-                merge_runs<<<dim3(blocks, unsigned(n)), 256, 0, s>>>(src, dst, len, run);
-                SPK_LAUNCHED();
-                std::swap(src, dst);
-            }
-            rank_times<<<dim3(blocks, unsigned(n)), 256, 0, s>>>(src, times.as<uint16_t>(), len, int32_t(time));
-            SPK_LAUNCHED();
-        }
-        else
-        {
-            latency_times<<<dim3(unsigned(n)), 1024, 0, s>>>((const float *)in, times.as<uint16_t>(), len,
-                                                             int32_t(time));
-            SPK_LAUNCHED();
-        }
+        rank_times_launch((const float *)in, n, len, time, sort != 0, times.as<uint16_t>(), s);
         if (out_type == SPK_U8)
             spikes_from_times<uint8_t><<<flat_grid(n * len, 256), 256, 0, s>>>(times.as<uint16_t>(), (uint8_t *)out,
                                                                                n, len, int32_t(time));

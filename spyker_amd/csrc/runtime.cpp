@@ -122,6 +122,16 @@ int32_t spk_dealloc(void *ptr)
     });
 }
 
+int32_t spk_copy(const void *src, void *dst, int64_t bytes, void *stream)
+{
+    return guard([&] {
+        SPK_ARG(bytes >= 0, "Copy size can't be negative.");
+        if (bytes == 0) return;
+        SPK_ARG(src && dst, "Copy buffers can't be null.");
+        SPK_HIP(hipMemcpyAsync(dst, src, size_t(bytes), hipMemcpyDefault, S(stream)));
+    });
+}
+
 int32_t spk_memory_taken(int64_t *bytes)
 {
     return guard([&] {

@@ -227,6 +227,33 @@ __global__ void __launch_bounds__(1024) fcwta_kernel(const float *__restrict__ x
     if (threadIdx.x == 0) cnt[s] = k;
 }
 
+// Integer input (wta.cpp:87-126 with the U8 values): first maximum of
+// T - first spiking step, stop at 0, the window is set to 0.
+__global__ void __launch_bounds__(1024) fcwta_u8_kernel(const uint8_t *__restrict__ x, int32_t t, int32_t f,
+                                                        int32_t radius, int32_t count, float *__restrict__ temp,
+                                                        spk_winner *__restrict__ win, int64_t *__restrict__ cnt)
+{
+    __shared__ ArgMax am[16];
+    const int64_t s = blockIdx.x;
+    float *ts = temp + s * int64_t(f);
+    wta_values_u8(x + s * t * int64_t(f), t, f, ts);
+    __syncthreads();
+    int32_t k = 0;
+    for (; k < count; ++k)
+    {
+        ArgMax a{-INFINITY, INT32_MAX};
+        for (int32_t j = threadIdx.x; j < f; j += blockDim.x)
+            if (better(ts[j], j, a.v, a.i)) a.v = ts[j], a.i = j;
+        a = block_argmax(a, am);
+        if (a.v == 0.0f) break;
+        if (threadIdx.x == 0) win[s * count + k] = spk_winner{0, 0, 0, 0, a.i};
+        const int32_t lo = max(0, a.i - radius), hi = min(f, a.i + radius + 1);
+        for (int32_t j = lo + int32_t(threadIdx.x); j < hi; j += blockDim.x) ts[j] = 0.0f;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) cnt[s] = k;
+}
+
 int wta_threads(int64_t len) { return len >= 64 * 1024 ? 1024 : len >= 8192 ? 512 : 256; }
 }  // namespace
 }  // namespace spk
@@ -307,16 +334,22 @@ int32_t spk_rank_fcwta(const void *x, int32_t type, int64_t n, int64_t t, int64_
                        double threshold, spk_winner *winners, int64_t *counts, void *stream)
 {
     return guard([&] {
-        SPK_REQUIRE(type == SPK_F32, SPK_ERR_UNSUPPORTED, "FCWTA: data type " << type_name(type) << " is not supported.");
+        SPK_REQUIRE(type == SPK_F32 || type == SPK_U8, SPK_ERR_UNSUPPORTED,
+                    "FCWTA: data type " << type_name(type) << " is not supported.");
         SPK_ARG(radius >= 0, "Radius can't be less than 0.");
         SPK_ARG(count > 0, "Winners count can't be less than 1.");
         SPK_ARG(n >= 0 && t >= 1 && f >= 1 && f < INT32_MAX, "FCWTA: input shape is invalid.");
         if (n == 0) return;
         hipStream_t s = S(stream);
         Scratch temp(n * f * int64_t(sizeof(float)), s);
-        fcwta_kernel<<<dim3(unsigned(n)), wta_threads(t * f), 0, s>>>(
-            (const float *)x, int32_t(t), int32_t(f), int32_t(radius), int32_t(count), float(threshold),
-            temp.as<float>(), winners, counts);
+        if (type == SPK_U8)
+            fcwta_u8_kernel<<<dim3(unsigned(n)), wta_threads(t * f), 0, s>>>(
+                (const uint8_t *)x, int32_t(t), int32_t(f), int32_t(radius), int32_t(count), temp.as<float>(),
+                winners, counts);
+        else
+            fcwta_kernel<<<dim3(unsigned(n)), wta_threads(t * f), 0, s>>>(
+                (const float *)x, int32_t(t), int32_t(f), int32_t(radius), int32_t(count), float(threshold),
+                temp.as<float>(), winners, counts);
         SPK_LAUNCHED();
     });
 }

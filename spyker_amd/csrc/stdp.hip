@@ -129,6 +129,7 @@ __global__ void __launch_bounds__(256)
 // keys written by the record kernel (winner.z for conv, winner.x for FC),
 // stages the bits of the matching winners of each chunk in LDS, and applies
 // them in slot order -- (sample, rank) order -- with each weight in a register.
+template <bool SPARSE_RULE>
 __global__ void __launch_bounds__(APPLY_THREADS)
     apply_records(float *__restrict__ kernel, StdpGeom g, StdpConfigs cfg, const int2 *__restrict__ slot,
                   const uint32_t *__restrict__ bits, const int32_t *__restrict__ flag, int32_t by_x)
@@ -185,7 +186,8 @@ __global__ void __launch_bounds__(APPLY_THREADS)
                 const uint32_t cur = word;
                 const spk_stdp_config cf = lcfg[ci];
                 if (q + 1 < m) word = lds_bits[(q + 1) * sw + (i >> 5)], ci = cfgidx[q + 1];
-                wr = stdp_update(wr, (cur >> (i & 31)) & 1u, cf);
+                wr = SPARSE_RULE ? sparse_stdp_update(wr, (cur >> (i & 31)) & 1u, cf)
+                                 : stdp_update(wr, (cur >> (i & 31)) & 1u, cf);
             }
         }
         __syncthreads();
@@ -242,13 +244,23 @@ void stdp_launch(const uint8_t *in, float *kernel, const uint8_t *out, const Std
     else
         conv_records<<<dim3(unsigned(slots)), 256, 0, s>>>(in, out, g, winners, counts, bits, slot, flag);
     SPK_LAUNCHED();
+    stdp_apply(kernel, g, cfg, slot, bits, flag, fc, false, s);
+}
+
+void stdp_apply(float *kernel, const StdpGeom &g, const StdpConfigs &cfg, const int2 *slot, const uint32_t *bits,
+                const int32_t *flag, bool fc, bool sparse_rule, hipStream_t s)
+{
+    const int64_t ksz = fc ? g.cin : int64_t(g.cin) * g.kh * g.kw;
     const int64_t sw = (ksz < SLICE ? (ksz + 31) / 32 : SLICE / 32);
     const size_t lds = size_t(APPLY_THREADS) * sw * sizeof(uint32_t);
-    if (lds > 65536)
-        SPK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&apply_records),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-    apply_records<<<dim3(unsigned(g.cout), unsigned(cdiv(ksz, SLICE))), APPLY_THREADS, lds, s>>>(
-        kernel, g, cfg, slot, bits, flag, fc ? 1 : 0);
+    const void *fn = sparse_rule ? reinterpret_cast<const void *>(&apply_records<true>)
+                                 : reinterpret_cast<const void *>(&apply_records<false>);
+    if (lds > 65536) SPK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    const dim3 grid(unsigned(g.cout), unsigned(cdiv(ksz, SLICE)));
+    if (sparse_rule)
+        apply_records<true><<<grid, APPLY_THREADS, lds, s>>>(kernel, g, cfg, slot, bits, flag, fc ? 1 : 0);
+    else
+        apply_records<false><<<grid, APPLY_THREADS, lds, s>>>(kernel, g, cfg, slot, bits, flag, fc ? 1 : 0);
     SPK_LAUNCHED();
 }
 }  // namespace spk

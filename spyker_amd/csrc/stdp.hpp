@@ -33,6 +33,19 @@ __device__ __forceinline__ float stdp_update(float w, bool potentiate, const spk
     return float(r);
 }
 
+// The sparse path's update of one weight (src/spyker/sparse/stdp.cpp:111-118):
+// F32 weight arithmetic with the F64 terms of the configuration,
+//     mult = (F32)((k - lower) * (upper - k))     (1 without stabilize)
+//     k = (F32)(k + rate * mult)                  (contracted to one F64 fma)
+//     k = (F32)max(lower, min(k, upper))
+__device__ __forceinline__ float sparse_stdp_update(float k, bool potentiate, const spk_stdp_config &cfg)
+{
+    const float mult = cfg.stabilize ? float((double(k) - cfg.lower) * (cfg.upper - double(k))) : 1.0f;
+    const float r = float(__builtin_fma(potentiate ? cfg.positive : cfg.negative, double(mult), double(k)));
+    const double hi = double(r) < cfg.upper ? double(r) : cfg.upper;
+    return float(cfg.lower < hi ? hi : cfg.lower);
+}
+
 StdpConfigs pack_configs(const spk_stdp_config *configs, int64_t nconfigs);
 
 // Bytes of the per-winner bit records the launch needs as workspace.
@@ -45,4 +58,8 @@ int64_t stdp_bits_bytes(const StdpGeom &g, bool fc);
 void stdp_launch(const uint8_t *in, float *kernel, const uint8_t *out, const StdpGeom &g, const StdpConfigs &cfg,
                  const spk_winner *winners, const int64_t *counts, uint32_t *bits, int32_t *flag, bool fc,
                  hipStream_t s);
+// The apply launch alone, for records made elsewhere (the sparse path):
+// slot[e] = {output channel or -1, config}, bits [slots][ceil(K/32)].
+void stdp_apply(float *kernel, const StdpGeom &g, const StdpConfigs &cfg, const int2 *slot, const uint32_t *bits,
+                const int32_t *flag, bool fc, bool sparse_rule, hipStream_t s);
 }  // namespace spk
