@@ -15,6 +15,12 @@ A step is one pass of that over the whole batch; value = images*timesteps/s
     fused  the FP16-MFMA fused layer step (spk_stdp_layer_fused) -> `value`
     plain  the FP32 per-stage kernels, bit-exact to the oracle -> `plain`
 
+A second, smaller line item (`sparse`) times the sparse event-stream
+workload: 256 DVS-like samples of 2x128x128 events over 20 bins (2% density
+modulated by moving blobs) -> sparse lists -> sparse conv 2->64 5x5 pad 2,
+threshold 8 -> fcwta k16 on the densified output -> sparse STDP of the
+winners, in samples*bins/s, next to the reference's own sparse CPU code.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 
@@ -146,6 +152,84 @@ def cpu_baseline(x_host, k2, samples):
                       % (n, T_STEPS, threads)}
 
 
+SP_N, SP_BINS, SP_SIZE, SP_COUT, SP_K, SP_TH, SP_COUNT = 256, 20, 128, 64, 5, 8.0, 16
+
+
+def sparse_inputs(dev, n=SP_N, seed=5):
+    import torch
+    from spyker_amd import synthetic
+    ev = synthetic.events(n, SP_BINS, SP_SIZE, seed=seed, density=0.02)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    k = np.clip(rng.normal(0.8, 0.05, (SP_COUT, 2, SP_K, SP_K)), 0, 1).astype(np.float32)
+    return ev, torch.from_numpy(ev).to(dev), k
+
+
+def sparse_step(spk, ev, conv):
+    """events -> sparse conv -> fcwta -> sparse STDP.  The fcwta runs on the
+    lists and returns the winners fcwta(out.dense()) picks, each with its
+    (z, y, x) and spike step, which the sparse STDP takes directly."""
+    s = spk.SparseTensor(ev)
+    out = conv(s, SP_TH)
+    win = spk.sparse.fcwta(out, 0, SP_COUNT)
+    conv.stdp(s, win)
+    return out, win
+
+
+def sparse_dense_step(spk, ev, conv):
+    """The same step with fcwta on the densified output (the reference's only route)."""
+    s = spk.SparseTensor(ev)
+    out = conv(s, SP_TH)
+    n, hw = ev.shape[0], SP_SIZE * SP_SIZE
+    win = spk.fcwta(out.dense().reshape(n, SP_BINS, -1), 0, SP_COUNT)
+    first = spk.sparse.gather(out, "i32").reshape(n, -1)
+    idx = win.data[:, :, 4]
+    d = win.data.clone()
+    d[:, :, 1], d[:, :, 2], d[:, :, 3], d[:, :, 4] = first.gather(1, idx).long(), idx // hw, \
+        (idx % hw) // SP_SIZE, idx % SP_SIZE
+    conv.stdp(s, spk.Winners(d, win.counts))
+    return out, win
+
+
+def sparse_bench(spk, dev, steps, warmup, cpu):
+    import torch
+    ev_host, ev, k = sparse_inputs(dev)
+    conv = spk.Conv(2, SP_COUT, SP_K, pad=2, device=dev)
+    conv.kernel = torch.from_numpy(k).to(dev)
+    conv.stdpconfig = [spk.STDPConfig(0.004, -0.003)]
+    dt = timed(lambda: sparse_step(spk, ev, conv), steps, warmup)
+    dtd = timed(lambda: sparse_dense_step(spk, ev, conv), 2, 1)
+    out, _ = sparse_step(spk, ev, conv)
+    s = spk.SparseTensor(ev)
+    res = {"workload": "256 DVS-like samples, 2x128x128, 20 bins, 2% density; sparse conv 2->64 5x5 pad 2, "
+                       "threshold 8, fcwta k16 (the winners of fcwta on the densified output, picked on the lists), "
+                       "sparse STDP",
+           "value": SP_N * SP_BINS / dt, "unit": "samples*bins/s", "ms_per_step": dt * 1e3, "steps": steps,
+           "input_events_per_sample": s.numel / SP_N, "output_events_per_sample": out.numel / SP_N,
+           "dtype": "f32", "ms_per_step_with_densified_fcwta": dtd * 1e3}
+    if cpu:
+        res["cpu_baseline"] = sparse_cpu_baseline(ev_host, k, 32)
+    return res
+
+
+def sparse_cpu_baseline(ev, k, samples):
+    """The reference's own sparse CPU code (oracle/_ref) on a sample of the same workload."""
+    import oracle
+    if oracle.reference_available():
+        X, kind, what = oracle.Reference(), "reference", "reference sparse code (oracle/_ref, OpenMP)"
+    else:
+        X, kind, what = oracle.Oracle(), "port", "oracle C port"
+    ev = np.ascontiguousarray(ev[:samples])
+    n = ev.shape[0]
+    t0 = time.perf_counter()
+    s = X.sparse_from_dense(ev)
+    out = X.sparse_conv(s, k, SP_TH, (2, 2, 2, 2))
+    w, c = X.fcwta(X.sparse_to_dense(out).reshape(n, SP_BINS, -1), 0, SP_COUNT)
+    dt = time.perf_counter() - t0  # STDP on the CPU is left out: it is a small share there
+    return {"value": n * SP_BINS / dt, "unit": "samples*bins/s", "cores": os.cpu_count() or 1, "kind": kind,
+            "sample": "%d samples x %d bins: to-sparse, sparse conv, densify, fcwta (%s; STDP not timed)"
+                      % (n, SP_BINS, what)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +239,8 @@ def main():
     ap.add_argument("--cpu-samples", type=int, default=192)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-sparse", action="store_true")
+    ap.add_argument("--sparse-only", action="store_true", help="time only the sparse event-stream workload")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (HBM counter calibration under rocprofv3)")
     a = ap.parse_args()
@@ -176,6 +262,9 @@ def main():
     import spyker_amd as spk
 
     dev = torch.device("cuda", 0)
+    if a.sparse_only:
+        print(json.dumps({"sparse": sparse_bench(spk, dev, a.steps, a.warmup, not a.no_cpu)}))
+        return
     x, k2 = make_inputs(dev)
 
     conv_f = spk.Conv(CIN, COUT, K, pad=1, device=dev)
@@ -224,6 +313,10 @@ def main():
     line["precision"] = precision(spk, x, k2, dev)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(x.cpu().numpy(), k2, a.cpu_samples)
+    if not a.no_sparse:
+        del x
+        torch.cuda.empty_cache()
+        line["sparse"] = sparse_bench(spk, dev, 5, 1, not a.no_cpu)
     print(json.dumps(line))
 
 

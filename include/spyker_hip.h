@@ -266,12 +266,23 @@ int32_t spk_sparse_inhibit(const spk_sparse *in, spk_sparse *out, void *stream);
 int32_t spk_sparse_convwta(const spk_sparse *in, int64_t ry, int64_t rx, int64_t count, spk_winner *winners,
                            int64_t *counts, void *stream);
 
+/* fcwta on spike lists.  No single reference call: it returns the winners
+ * that Core::rank_fcwta's integer branch (cpu/wta.cpp:66-126) picks on
+ * SparseTensor::dense() (utils.cpp:251-257) viewed as [n][t][c*h*w], without
+ * materialising the dense tensor: candidates in (step, flat index) order,
+ * flat = (z*h + y)*w + x, each a winner unless within `radius` of an earlier
+ * one in flat index.  Each winner is returned as {c=0, t=step, z, y, x}
+ * (the dense call returns {0, 0, 0, 0, flat}), so the list feeds
+ * spk_sparse_convstdp directly.  count at most 8192. */
+int32_t spk_sparse_fcwta(const spk_sparse *in, int64_t radius, int64_t count, spk_winner *winners, int64_t *counts,
+                         void *stream);
+
 /* Conv::operator()(SparseTensor, threshold) (Core::sparse_conv, impl.h:342;
  * sparse/conv.cpp): stride 1; per sample and output neuron the F32
  * potential adds kernel[c][z][ky][kx] once per input event in list order,
  * spikes the first time a step with events leaves it > threshold, and then
  * never again.  kernel: F32 [cout][c][kh][kw] (the reference reads any
- * kernel as F32), c*kh*kw at most 480. */
+ * kernel as F32), c*kh*kw at most 416, t at most 1024. */
 int32_t spk_sparse_conv(const spk_sparse *in, const void *kernel, int32_t kernel_type, int64_t cout, int64_t kh,
                         int64_t kw, double threshold, int64_t sy, int64_t sx, int64_t pt, int64_t pl, int64_t pb,
                         int64_t pr, spk_sparse *out, void *stream);

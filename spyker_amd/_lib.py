@@ -66,6 +66,7 @@ SIGNATURES = {
     "spk_sparse_pool": [_p] + [_i64] * 8 + [_p, _p],
     "spk_sparse_inhibit": [_p, _p, _p],
     "spk_sparse_convwta": [_p, _i64, _i64, _i64, _p, _p, _p],
+    "spk_sparse_fcwta": [_p, _i64, _i64, _p, _p, _p],
     "spk_sparse_conv": [_p, _p, _i32, _i64, _i64, _i64, _f64] + [_i64] * 6 + [_p, _p],
     "spk_sparse_convstdp": [_p, _p, _i32, _i64, _i64, _i64, _p, _i64, _p, _p, _i64] + [_i64] * 4 + [_p],
     "spk_stdp_layer_fused": [_p] + [_i64] * 5 + [_p] + [_i64] * 7 + [_f64] + [_i64] * 3 + [_p, _i64, _p, _p]

@@ -31,6 +31,7 @@ namespace
 constexpr int CHUNK = 1024;  // elements per builder chunk (one wave)
 constexpr int CONV_TILE = 16;
 constexpr int CONV_CC = 32;  // output channels per thread in the conv
+constexpr int CONV_MAX_T = 1024;  // steps a conv block keeps the list offsets of
 
 __device__ __forceinline__ int32_t pack(int32_t z, int32_t y, int32_t x) { return (z << 16) | (y << 8) | x; }
 __device__ __forceinline__ int32_t ez(int32_t e) { return (e >> 16) & 0xFF; }
@@ -74,8 +75,8 @@ __global__ void __launch_bounds__(SCAN_THREADS) scan_segments(const I *__restric
 
 __global__ void scan_add(int64_t *__restrict__ out, int64_t m, const int64_t *__restrict__ seg)
 {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < m) out[i] += seg[i / SCAN_SEG];
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < m; i += int64_t(gridDim.x) * blockDim.x)
+        out[i] += seg[i / SCAN_SEG];
 }
 
 template <typename I>
@@ -129,58 +130,50 @@ int64_t read_total(const int64_t *p, hipStream_t s)
 }
 
 // -- list builder -------------------------------------------------------------
+//
+// A first-spike-time map [n][len] (U16, >= t: no spike) becomes lists in
+// scan order.  The elements of a sample are cut into chunks of CHUNK;
+// cnt[(i*t + j)*chunks + c] counts chunk c's spikes at step j.  One exclusive
+// scan of cnt in that order gives every (sample, step, chunk) its first
+// position in the event array, and offsets[i*t + j] = pos[(i*t + j)*chunks].
 
-// Per-chunk step histograms of a time map [n][len]: hist[(i*chunks + c)*t + j].
-__global__ void __launch_bounds__(64) chunk_hist(const uint16_t *__restrict__ times, int64_t len, int32_t t,
-                                                 int64_t chunks, int32_t *__restrict__ hist)
+__global__ void __launch_bounds__(256) chunk_hist(const uint16_t *__restrict__ times, int64_t len, int32_t t,
+                                                  int64_t chunks, int32_t *__restrict__ cnt)
 {
-    extern __shared__ int32_t h[];
+    extern __shared__ int32_t h[];  // [t]
     const int64_t i = blockIdx.y, c = blockIdx.x;
-    for (int32_t k = threadIdx.x; k < t; k += 64) h[k] = 0;
+    for (int32_t k = threadIdx.x; k < t; k += 256) h[k] = 0;
     __syncthreads();
     const uint16_t *ts = times + i * len;
     const int64_t e1 = min(c * CHUNK + CHUNK, len);
-    for (int64_t e = c * CHUNK + threadIdx.x; e < e1; e += 64)
+    for (int64_t e = c * CHUNK + threadIdx.x; e < e1; e += 256)
     {
         const int32_t v = ts[e];
         if (v < t) atomicAdd(&h[v], 1);
     }
     __syncthreads();
-    int32_t *out = hist + (i * chunks + c) * t;
-    for (int32_t k = threadIdx.x; k < t; k += 64) out[k] = h[k];
+    for (int32_t k = threadIdx.x; k < t; k += 256) cnt[(i * t + k) * chunks + c] = h[k];
 }
 
-// counts[i*t + j] = events of list (i, j); hist becomes each chunk's start
-// inside its list.
-__global__ void chunk_prefix(int32_t *__restrict__ hist, int64_t n, int32_t t, int64_t chunks,
-                             int64_t *__restrict__ counts)
+__global__ void list_offsets(const int64_t *__restrict__ pos, int64_t chunks, int64_t lists, int64_t m,
+                             int64_t *__restrict__ offsets)
 {
-    for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < n * t;
-         idx += int64_t(gridDim.x) * blockDim.x)
-    {
-        const int64_t i = idx / t, j = idx % t;
-        int64_t run = 0;
-        for (int64_t c = 0; c < chunks; ++c)
-        {
-            int32_t &v = hist[(i * chunks + c) * t + j];
-            const int32_t here = v;
-            v = int32_t(run);
-            run += here;
-        }
-        counts[idx] = run;
-    }
+    for (int64_t l = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; l <= lists; l += int64_t(gridDim.x) * blockDim.x)
+        offsets[l] = l < lists ? pos[l * chunks] : pos[m];
 }
 
-// Stable scatter of a chunk's spikes to their lists, one wave per chunk.
+// Stable scatter of a chunk's spikes to their lists, one wave per chunk, 64
+// elements a round: the lanes holding the same step find each other from
+// one ballot per bit of the step (`bits` = bit width of t), so a round costs
+// the same however many steps it holds.
 __global__ void __launch_bounds__(64) chunk_scatter(const uint16_t *__restrict__ times, int64_t len, int32_t t,
-                                                    int64_t chunks, const int32_t *__restrict__ hist,
-                                                    const int64_t *__restrict__ offsets, int32_t h, int32_t w,
-                                                    int32_t *__restrict__ events)
+                                                    int32_t bits, int64_t chunks, const int64_t *__restrict__ pos,
+                                                    int32_t h, int32_t w, int32_t *__restrict__ events)
 {
     extern __shared__ int64_t next[];  // [t] next position in each list
     const int64_t i = blockIdx.y, c = blockIdx.x;
     const int lane = threadIdx.x;
-    for (int32_t k = lane; k < t; k += 64) next[k] = offsets[i * t + k] + hist[(i * chunks + c) * t + k];
+    for (int32_t k = lane; k < t; k += 64) next[k] = pos[(i * t + k) * chunks + c];
     __syncthreads();
     const uint16_t *ts = times + i * len;
     const int32_t hw = h * w;
@@ -188,24 +181,23 @@ __global__ void __launch_bounds__(64) chunk_scatter(const uint16_t *__restrict__
     for (int64_t e0 = c * CHUNK; e0 < e1; e0 += 64)
     {
         const int64_t e = e0 + lane;
-        const int32_t v = e < e1 ? int32_t(ts[e]) : t;
-        uint64_t act = __ballot(v < t);
-        while (act)
+        const int32_t v = e < e1 ? min(int32_t(ts[e]), t) : t;
+        uint64_t same = ~0ull;
+        for (int32_t b = 0; b < bits; ++b)
         {
-            const int leader = __builtin_ctzll(act);
-            const int32_t vl = __shfl(v, leader, 64);
-            const uint64_t same = __ballot(v == vl) & act;
-            const int64_t b = next[vl];
-            if ((same >> lane) & 1ull)
-            {
-                const int32_t k = int32_t(e);
-                events[b + __popcll(same & lanes_below(lane))] = pack(k / hw, (k / w) % h, k % w);
-            }
-            __syncthreads();
-            if (lane == leader) next[vl] = b + __popcll(same);
-            __syncthreads();
-            act &= ~same;
+            const uint64_t m = __ballot((v >> b) & 1);
+            same &= ((v >> b) & 1) ? m : ~m;
         }
+        const int32_t rank = __popcll(same & lanes_below(lane));
+        const int64_t base = v < t ? next[v] : 0;
+        __syncthreads();
+        if (v < t)
+        {
+            const int32_t k = int32_t(e);
+            events[base + rank] = pack(k / hw, (k / w) % h, k % w);
+            if (rank == 0) next[v] = base + __popcll(same);
+        }
+        __syncthreads();
     }
 }
 
@@ -214,31 +206,29 @@ __global__ void __launch_bounds__(64) chunk_scatter(const uint16_t *__restrict__
 void build_lists(const uint16_t *times, int64_t n, int64_t t, int64_t c, int64_t h, int64_t w, spk_sparse *out,
                  hipStream_t s)
 {
-    const int64_t len = c * h * w, chunks = cdiv(len, CHUNK);
+    const int64_t len = c * h * w, chunks = cdiv(len, CHUNK), m = n * t * chunks;
     *out = spk_sparse{n, t, c, h, w, 0, nullptr, nullptr};
     void *p = nullptr;
     SPK_REQUIRE(spk_alloc(&p, (n * t + 1) * 8) == SPK_OK, SPK_ERR_RUNTIME, spk_last_error());
     out->offsets = static_cast<int64_t *>(p);
-    Scratch hist(std::max<int64_t>(n * chunks * t, 1) * 4, s), counts(std::max<int64_t>(n * t, 1) * 8, s);
-    if (n > 0 && len > 0)
+    Scratch cnt(std::max<int64_t>(m, 1) * 4, s), pos((m + 1) * 8, s);
+    if (m > 0)
     {
-        chunk_hist<<<dim3(unsigned(chunks), unsigned(n)), 64, size_t(t) * 4, s>>>(times, len, int32_t(t), chunks,
-                                                                                hist.as<int32_t>());
-        SPK_LAUNCHED();
-        chunk_prefix<<<flat_grid(n * t, 256), 256, 0, s>>>(hist.as<int32_t>(), n, int32_t(t), chunks,
-                                                            counts.as<int64_t>());
+        chunk_hist<<<dim3(unsigned(chunks), unsigned(n)), 256, size_t(t) * 4, s>>>(times, len, int32_t(t), chunks,
+                                                                                 cnt.as<int32_t>());
         SPK_LAUNCHED();
     }
-    else
-        fill_launch<int64_t>(counts.as<int64_t>(), n * t, 0, s);
-    exclusive_scan<int64_t>(counts.as<int64_t>(), out->offsets, n * t, s);
-    out->total = read_total(out->offsets + n * t, s);
+    exclusive_scan<int32_t>(cnt.as<int32_t>(), pos.as<int64_t>(), m, s);
+    list_offsets<<<flat_grid(n * t + 1, 256), 256, 0, s>>>(pos.as<int64_t>(), chunks, n * t, m, out->offsets);
+    SPK_LAUNCHED();
+    out->total = read_total(pos.as<int64_t>() + m, s);
     SPK_REQUIRE(spk_alloc(&p, std::max<int64_t>(out->total, 1) * 4) == SPK_OK, SPK_ERR_RUNTIME, spk_last_error());
     out->events = static_cast<int32_t *>(p);
     if (out->total > 0)
     {
+        const int32_t bits = 32 - __builtin_clz(uint32_t(t));
         chunk_scatter<<<dim3(unsigned(chunks), unsigned(n)), 64, size_t(t) * 8, s>>>(
-            times, len, int32_t(t), chunks, hist.as<int32_t>(), out->offsets, int32_t(h), int32_t(w), out->events);
+            times, len, int32_t(t), bits, chunks, pos.as<int64_t>(), int32_t(h), int32_t(w), out->events);
         SPK_LAUNCHED();
     }
 }
@@ -259,21 +249,18 @@ __global__ void first_over(const T *__restrict__ in, int64_t n, int32_t t, int64
     }
 }
 
-// Per event, with its sample and step: block per list (i, j).
-// mode 0: first step per element (atomicMin), 1: last step (atomicMax).
+// Step of every event written at its element: block per list (i, j).
+// Every producer emits an element at most once per sample (its first
+// spike), so a plain store suffices and first and last step coincide.
 __global__ void step_map(const int64_t *__restrict__ off, const int32_t *__restrict__ ev, int32_t t, int32_t h,
-                         int32_t w, int64_t len, int32_t *__restrict__ map, int32_t mode)
+                         int32_t w, int64_t len, int32_t *__restrict__ map)
 {
     const int64_t list = blockIdx.x, i = list / t;
     const int32_t j = int32_t(list % t);
     for (int64_t e = off[list] + threadIdx.x; e < off[list + 1]; e += blockDim.x)
     {
         const int32_t v = ev[e];
-        int32_t *m = map + i * len + (int64_t(ez(v)) * h + ey(v)) * w + ex(v);
-        if (mode == 0)
-            atomicMin(m, j);
-        else
-            atomicMax(m, j);
+        map[i * len + (int64_t(ez(v)) * h + ey(v)) * w + ex(v)] = j;
     }
 }
 
@@ -284,15 +271,116 @@ __global__ void map_out(const int32_t *__restrict__ map, O *__restrict__ out, in
         out[i] = O(map[i]);
 }
 
+// Cumulative spikes from the first-step map: a thread reads the steps of 4
+// consecutive elements once and writes their 4 bytes at every step (one
+// dword store when `vec`: len % 4 == 0 and out 4-byte aligned).
 __global__ void dense_from_map(const int32_t *__restrict__ map, uint8_t *__restrict__ out, int64_t n, int32_t t,
-                               int64_t len)
+                               int64_t len, int32_t vec)
 {
-    for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < n * t * len;
+    const int64_t groups = (len + 3) / 4;
+    for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < n * groups;
          idx += int64_t(gridDim.x) * blockDim.x)
     {
-        const int64_t k = idx % len, j = (idx / len) % t, i = idx / (len * t);
-        out[idx] = map[i * len + k] <= j;
+        const int64_t i = idx / groups, k0 = (idx % groups) * 4;
+        const int32_t *m = map + i * len + k0;
+        int32_t f[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) f[q] = k0 + q < len ? m[q] : t;
+        uint8_t *o = out + i * t * len + k0;
+        for (int32_t j = 0; j < t; ++j, o += len)
+        {
+            if (vec)
+                *reinterpret_cast<uint32_t *>(o) = uint32_t(f[0] <= j) | uint32_t(f[1] <= j) << 8 |
+                                                   uint32_t(f[2] <= j) << 16 | uint32_t(f[3] <= j) << 24;
+            else
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (k0 + q < len) o[q] = f[q] <= j;
+        }
     }
+}
+
+// fcwta on the spike lists: the winners of fcwta(dense(), radius, count)
+// over the flat index (z*h + y)*w + x.  Candidates come in (step, flat
+// index) order -- the dense branch's order of T - first step, descending,
+// then index -- and one is a winner unless an earlier winner lies within
+// `radius` of it.  One wave per sample; a list already in ascending flat
+// order (what every scan-order producer emits) is walked 64 events at a
+// time, any other list by repeated minimum selection.
+__global__ void __launch_bounds__(64) sparse_fcwta(const int64_t *__restrict__ off, const int32_t *__restrict__ ev,
+                                                   int32_t t, int32_t h, int32_t w, int32_t radius, int32_t count,
+                                                   spk_winner *__restrict__ win, int64_t *__restrict__ counts)
+{
+    extern __shared__ int32_t wflat[];  // [count]
+    const int64_t i = blockIdx.x;
+    const int lane = threadIdx.x;
+    auto flat = [&](int32_t v) { return (ez(v) * h + ey(v)) * w + ex(v); };
+    auto excluded = [&](int32_t f, int32_t found) {
+        bool x = false;
+        for (int32_t k = 0; k < found; ++k) x |= abs(f - wflat[k]) <= radius;
+        return x;
+    };
+    auto add = [&](int32_t v, int32_t j, int32_t found) {
+        if (lane == 0)
+        {
+            win[i * count + found] = spk_winner{0, j, ez(v), ey(v), ex(v)};
+            wflat[found] = flat(v);
+        }
+        __syncthreads();
+    };
+    int32_t found = 0;
+    for (int32_t j = 0; j < t && found < count; ++j)
+    {
+        const int64_t e0 = off[i * t + j], e1 = off[i * t + j + 1];
+        bool sorted = true;
+        for (int64_t b = e0 + 1; b < e1 && sorted; b += 64)
+        {
+            const int64_t e = b + lane;
+            sorted = !__ballot(e < e1 && flat(ev[e - 1]) >= flat(ev[e]));
+        }
+        if (sorted)
+        {
+            for (int64_t b = e0; b < e1 && found < count; b += 64)
+            {
+                const int64_t e = b + lane;
+                const int32_t v = e < e1 ? ev[e] : 0, f = flat(v);
+                bool ok = e < e1 && !excluded(f, found);
+                uint64_t pass = __ballot(ok);
+                while (pass && found < count)
+                {
+                    const int l = __builtin_ctzll(pass);
+                    const int32_t vl = __shfl(v, l, 64);
+                    add(vl, j, found);
+                    ++found;
+                    if (abs(f - flat(vl)) <= radius) ok = false;
+                    pass = __ballot(ok) & (l == 63 ? 0ull : ~0ull << (l + 1));
+                }
+            }
+        }
+        else
+            while (found < count)
+            {
+                int32_t best = INT32_MAX, bv = 0;
+                for (int64_t b = e0; b < e1; b += 64)
+                {
+                    const int64_t e = b + lane;
+                    if (e < e1)
+                    {
+                        const int32_t v = ev[e], f = flat(v);
+                        if (f < best && !excluded(f, found)) best = f, bv = v;
+                    }
+                }
+                for (int d = 32; d >= 1; d >>= 1)
+                {
+                    const int32_t ob = __shfl_xor(best, d, 64), ov = __shfl_xor(bv, d, 64);
+                    if (ob < best) best = ob, bv = ov;
+                }
+                if (best == INT32_MAX) break;
+                add(bv, j, found);
+                ++found;
+            }
+    }
+    if (lane == 0) counts[i] = found;
 }
 
 __global__ void shift_events(const int32_t *__restrict__ in, int32_t *__restrict__ out, int64_t m, int32_t dy,
@@ -444,16 +532,28 @@ struct SparseConvGeom
     float threshold;
 };
 
-// One 16x16 output tile of one sample and CONV_CC output channels per block;
-// writes the first spike step of every neuron (t where none).
+// One 16x16 output tile of one sample and CONV_CC output channels per block,
+// a thread per pixel with its CONV_CC potentials in registers; the weights
+// sit in LDS as [z][ky][kx][CONV_CC] so a tap's channels are 8 float4 reads.
+// The sample's events stream through in rounds of 256 (the next round's
+// loads in flight while this one is applied); the events of a round that
+// reach the tile are compacted in list order with their step, and every
+// potential gets its adds in the reference's order.  The threshold check
+// runs where the tile's hits change step, after the last hit, and after the
+// sample's first step with events: a potential that no event reached since
+// the last check cannot newly exceed the threshold, except the initial 0
+// when the threshold is negative, which that first check covers.  Writes the
+// first spike step of every neuron (t where none).
 __global__ void __launch_bounds__(256) sparse_conv_tile(const int64_t *__restrict__ off,
                                                         const int32_t *__restrict__ ev,
                                                         const float *__restrict__ kernel, SparseConvGeom g,
                                                         uint16_t *__restrict__ times)
 {
-    extern __shared__ float wl[];  // [CONV_CC][cin][kh][kw]
-    __shared__ int32_t elist[256];
+    extern __shared__ float4 wl4[];  // [cin][kh][kw][CONV_CC / 4]
+    __shared__ int32_t elist[256], estep[256];
     __shared__ int32_t wave_n[4];
+    __shared__ int64_t loff[CONV_MAX_T + 1];
+    float *wl = reinterpret_cast<float *>(wl4);
     const int64_t i = blockIdx.y;
     const int32_t c0 = blockIdx.z * CONV_CC;
     const int32_t ty0 = (blockIdx.x / g.tiles_x) * CONV_TILE, tx0 = (blockIdx.x % g.tiles_x) * CONV_TILE;
@@ -463,49 +563,77 @@ __global__ void __launch_bounds__(256) sparse_conv_tile(const int64_t *__restric
     const int32_t taps = g.cin * g.kh * g.kw;
     for (int32_t q = tid; q < CONV_CC * taps; q += 256)
     {
-        const int32_t c = c0 + q / taps;
-        wl[q] = c < g.cout ? kernel[int64_t(c) * taps + q % taps] : 0.0f;
+        const int32_t c = c0 + q % CONV_CC;
+        wl[q] = c < g.cout ? kernel[int64_t(c) * taps + q / CONV_CC] : 0.0f;
     }
+    for (int32_t k = tid; k <= g.t; k += 256) loff[k] = off[i * g.t + k];
     float pot[CONV_CC];
     int32_t fire[CONV_CC];
 #pragma unroll
     for (int c = 0; c < CONV_CC; ++c) pot[c] = 0.0f, fire[c] = g.t;
     __syncthreads();
-    for (int32_t j = 0; j < g.t; ++j)
-    {
-        const int64_t e0 = off[i * g.t + j], e1 = off[i * g.t + j + 1];
-        if (e0 == e1) continue;
-        for (int64_t r0 = e0; r0 < e1; r0 += 256)
-        {
-            // the round's events that reach the tile, in list order
-            const int64_t e = r0 + tid;
-            const int32_t v = e < e1 ? ev[e] : -1;
-            const int32_t dy = v >= 0 ? ey(v) + g.pt - ty0 : -1, dx = v >= 0 ? ex(v) + g.pl - tx0 : -1;
-            const bool hit = v >= 0 && dy >= 0 && dy < CONV_TILE + g.kh - 1 && dx >= 0 && dx < CONV_TILE + g.kw - 1;
-            const uint64_t b = __ballot(hit);
-            if (lane == 0) wave_n[wave] = __popcll(b);
-            __syncthreads();
-            int32_t at = __popcll(b & lanes_below(lane)), total = 0;
-            for (int k = 0; k < 4; ++k) at += k < wave ? wave_n[k] : 0, total += wave_n[k];
-            if (hit) elist[at] = v;
-            __syncthreads();
-            for (int32_t q = 0; q < total; ++q)
-            {
-                const int32_t u = elist[q];
-                const int32_t ky = ey(u) + g.pt - oy, kx = ex(u) + g.pl - ox;
-                if (pix && ky >= 0 && ky < g.kh && kx >= 0 && kx < g.kw)
-                {
-                    const float *wk = wl + (ez(u) * g.kh + ky) * g.kw + kx;
-#pragma unroll
-                    for (int c = 0; c < CONV_CC; ++c) pot[c] += wk[c * taps];
-                }
-            }
-            __syncthreads();
-        }
+    auto check = [&](int32_t j) {
 #pragma unroll
         for (int c = 0; c < CONV_CC; ++c)
             if (pot[c] > g.threshold) fire[c] = j, pot[c] = -FLT_MAX;
+    };
+    const int64_t s0 = loff[0], s1 = loff[g.t];
+    // step of the last applied hits, starting at the sample's first step with
+    // events so that step is checked after its hits even if none reach here
+    int32_t cur = -1;
+    if (s0 < s1)
+    {
+        cur = 0;
+        while (loff[cur + 1] == s0) ++cur;
     }
+    int32_t vn = s0 + tid < s1 ? ev[s0 + tid] : -1;
+    for (int64_t r0 = s0; r0 < s1; r0 += 256)
+    {
+        const int64_t e = r0 + tid;
+        const int32_t v = vn;
+        vn = e + 256 < s1 ? ev[e + 256] : -1;  // next round, in flight meanwhile
+        const int32_t dy = v >= 0 ? ey(v) + g.pt - ty0 : -1, dx = v >= 0 ? ex(v) + g.pl - tx0 : -1;
+        const bool hit = v >= 0 && dy >= 0 && dy < CONV_TILE + g.kh - 1 && dx >= 0 && dx < CONV_TILE + g.kw - 1;
+        const uint64_t b = __ballot(hit);
+        if (lane == 0) wave_n[wave] = __popcll(b);
+        __syncthreads();
+        int32_t at = __popcll(b & lanes_below(lane)), total = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) at += k < wave ? wave_n[k] : 0, total += wave_n[k];
+        if (hit)
+        {
+            // step of event e: the last k with loff[k] <= e
+            int32_t lo = 0, hi = g.t - 1;
+            while (lo < hi)
+            {
+                const int32_t mid = (lo + hi + 1) >> 1;
+                if (loff[mid] <= e)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            elist[at] = v, estep[at] = lo;
+        }
+        __syncthreads();
+        for (int32_t q = 0; q < total; ++q)
+        {
+            const int32_t u = elist[q], j = estep[q];
+            if (j != cur) check(cur), cur = j;
+            const int32_t ky = ey(u) + g.pt - oy, kx = ex(u) + g.pl - ox;
+            if (pix && ky >= 0 && ky < g.kh && kx >= 0 && kx < g.kw)
+            {
+                const float4 *wk = wl4 + ((ez(u) * g.kh + ky) * g.kw + kx) * (CONV_CC / 4);
+#pragma unroll
+                for (int c = 0; c < CONV_CC / 4; ++c)
+                {
+                    const float4 w4 = wk[c];
+                    pot[4 * c] += w4.x, pot[4 * c + 1] += w4.y, pot[4 * c + 2] += w4.z, pot[4 * c + 3] += w4.w;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (cur >= 0) check(cur);
     if (pix)
 #pragma unroll
         for (int c = 0; c < CONV_CC; ++c)
@@ -581,7 +709,7 @@ void step_map_launch(const spk_sparse *in, int32_t *map, int32_t mode, hipStream
     if (lists > 0 && in->total > 0)
     {
         step_map<<<dim3(unsigned(lists)), 256, 0, s>>>(in->offsets, in->events, int32_t(in->t), int32_t(in->h),
-                                                       int32_t(in->w), len, map, mode);
+                                                       int32_t(in->w), len, map);
         SPK_LAUNCHED();
     }
 }
@@ -658,8 +786,9 @@ int32_t spk_sparse_to_dense(const spk_sparse *in, void *out, void *stream)
         step_map_launch(in, map.as<int32_t>(), 0, s);
         if (in->n * in->t * len > 0)
         {
-            dense_from_map<<<flat_grid(in->n * in->t * len, 256), 256, 0, s>>>(map.as<int32_t>(), (uint8_t *)out,
-                                                                              in->n, int32_t(in->t), len);
+            const int32_t vec = len % 4 == 0 && reinterpret_cast<uintptr_t>(out) % 4 == 0;
+            dense_from_map<<<flat_grid(in->n * cdiv(len, 4), 256), 256, 0, s>>>(map.as<int32_t>(), (uint8_t *)out,
+                                                                                in->n, int32_t(in->t), len, vec);
             SPK_LAUNCHED();
         }
     });
@@ -791,6 +920,24 @@ int32_t spk_sparse_convwta(const spk_sparse *in, int64_t ry, int64_t rx, int64_t
     });
 }
 
+int32_t spk_sparse_fcwta(const spk_sparse *in, int64_t radius, int64_t count, spk_winner *winners, int64_t *counts,
+                         void *stream)
+{
+    return guard([&] {
+        check_sparse(in);
+        SPK_ARG(radius >= 0, "Radius can't be less than 0.");
+        SPK_ARG(count > 0, "Winners count can't be less than 1.");
+        SPK_ARG(count <= 8192, "Sparse fcwta: at most 8192 winners per sample.");
+        SPK_ARG(winners && counts, "Sparse fcwta: winner buffers are required.");
+        if (in->n == 0) return;
+        hipStream_t s = S(stream);
+        sparse_fcwta<<<dim3(unsigned(in->n)), 64, size_t(count) * 4, s>>>(
+            in->offsets, in->events, int32_t(in->t), int32_t(in->h), int32_t(in->w),
+            int32_t(std::min<int64_t>(radius, INT32_MAX)), int32_t(count), winners, counts);
+        SPK_LAUNCHED();
+    });
+}
+
 int32_t spk_sparse_conv(const spk_sparse *in, const void *kernel, int32_t kernel_type, int64_t cout, int64_t kh,
                         int64_t kw, double threshold, int64_t sy, int64_t sx, int64_t pt, int64_t pl, int64_t pb,
                         int64_t pr, spk_sparse *out, void *stream)
@@ -807,8 +954,9 @@ int32_t spk_sparse_conv(const spk_sparse *in, const void *kernel, int32_t kernel
         SPK_ARG(ho >= 1 && wo >= 1, "Input and kernel shapes are not compatible.");
         SPK_ARG(cout <= 255 && ho <= 255 && wo <= 255, "Sparse tensors hold at most 255 channels, rows and columns.");
         const int64_t lds = int64_t(CONV_CC) * in->c * kh * kw * 4;
-        SPK_ARG(lds <= 61440, "Sparse conv: Cin*Kh*Kw can't exceed 480.");
+        SPK_ARG(lds <= 53248, "Sparse conv: Cin*Kh*Kw can't exceed 416.");
         hipStream_t s = S(stream);
+        SPK_ARG(in->t <= CONV_MAX_T, "Sparse conv: at most " << CONV_MAX_T << " steps.");
         const int64_t tiles_x = cdiv(wo, CONV_TILE), tiles = cdiv(ho, CONV_TILE) * tiles_x;
         Scratch times(std::max<int64_t>(in->n * cout * ho * wo, 1) * 2, s);
         if (in->n > 0)

@@ -10,6 +10,7 @@ synchronizes its stream.
     SparseTensor(tensor, threshold), .dense()   utils.cpp:219-259
     code, conv, gather, pool, inhibit, convwta  sparse.py:37-210
     pad                                         opers.cpp:992-998
+    fcwta                                       cpu/wta.cpp:66-126 on .dense()
     Conv.__call__(SparseTensor, threshold)      opers.cpp:866-871
     Conv.stdp(SparseTensor, winners)            opers.cpp:882-889
 """
@@ -218,6 +219,22 @@ def convwta(array: SparseTensor, radius, count: int):
     data = torch.zeros((s.n, count, 5), dtype=torch.int64, device=array.device)
     cnt = torch.zeros((s.n,), dtype=torch.int64, device=array.device)
     call("spk_sparse_convwta", C.byref(s), ry, rx, int(count), C.c_void_p(data.data_ptr()),
+         C.c_void_p(cnt.data_ptr()), _stream(array.device))
+    return Winners(data, cnt)
+
+
+def fcwta(array: SparseTensor, radius: int, count: int):
+    """The winners of fcwta(array.dense(), radius, count) (src/spyker/cpu/wta.cpp:66-126) without
+    densifying, as conv coordinates: each Winner is (c=0, t=spike step, z, y, x)."""
+    from . import Winners
+    s = _sp(array)
+    if radius < 0:
+        raise RuntimeError("Radius can't be less than 0.")
+    if count <= 0:
+        raise RuntimeError("Winners count can't be less than 1.")
+    data = torch.zeros((s.n, count, 5), dtype=torch.int64, device=array.device)
+    cnt = torch.zeros((s.n,), dtype=torch.int64, device=array.device)
+    call("spk_sparse_fcwta", C.byref(s), int(radius), int(count), C.c_void_p(data.data_ptr()),
          C.c_void_p(cnt.data_ptr()), _stream(array.device))
     return Winners(data, cnt)
 

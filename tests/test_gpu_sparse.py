@@ -228,3 +228,76 @@ def test_sparse_layer_pipeline(dev, orc):
     fw, fcnt = winners_np(spk.fcwta(out.dense().reshape(4, 15, -1), 0, 16))
     gw, gc = orc.fcwta(orc.sparse_to_dense(oo).reshape(4, 15, -1), 0, 16)
     same_winners(fw, fcnt, gw.reshape(4, 16), gc)
+
+
+def check_fcwta(orc, o, w, c, radius, count):
+    """Sparse fcwta = the dense integer fcwta on the densified tensor (its
+    flat index, c = 0), with t = the winner's spike step."""
+    n, t, ch, h, wd = o.shape
+    gw, gc = orc.fcwta(orc.sparse_to_dense(o).reshape(n, t, -1), radius, count)
+    gw = gw.reshape(n, count)
+    first = orc.sparse_gather(o).reshape(n, -1)
+    assert np.array_equal(c, gc)
+    for i in range(n):
+        got = w[i, :c[i]]
+        assert np.array_equal((got["z"] * h + got["y"]) * wd + got["x"], gw[i, :gc[i]]["x"])
+        assert (got["c"] == 0).all()
+        assert np.array_equal(got["t"], first[i, gw[i, :gc[i]]["x"]])
+
+
+def test_sparse_fcwta(dev, g, orc):
+    """Sorted lists (conv output) and unsorted ones (pooled lists keep the
+    input's order), radius 0 and wider, counts past the events."""
+    s = spk.SparseTensor(T(g["dense_in"], dev))
+    conv_out = sparse.conv(s, T(g["conv_kernel"], dev), 6.0, pad=(2, 2, 2, 2))
+    pooled = sparse.pool(s, 2, pad=(1, 0, 0, 0))
+    assert host(conv_out) == sp(g, "conv")
+    for x in (conv_out, pooled):
+        o = host(x)
+        for r, k in ((0, 16), (3, 5), (40, 7), (0, 5000)):
+            w, c = winners_np(sparse.fcwta(x, r, k))
+            check_fcwta(orc, o, w, c, r, k)
+    # golden dense fcwta on the densified conv output
+    for r, cnt in ((0, 16), (3, 5)):
+        w, c = winners_np(sparse.fcwta(conv_out, r, cnt))
+        assert np.array_equal(c, g["fcwta_u8_r%d_k%d_cnt" % (r, cnt)])
+        flat = (w["z"] * 24 + w["y"]) * 24 + w["x"]
+        gw = g["fcwta_u8_r%d_k%d_win" % (r, cnt)].reshape(3, cnt)
+        for i in range(3):
+            assert np.array_equal(flat[i, :c[i]], gw[i, :c[i]]["x"])
+
+
+def test_sparse_fcwta_stdp_pipeline(dev, orc):
+    """events -> sparse conv -> sparse fcwta -> sparse STDP, the event-stream
+    layer step, against the oracle's dense fcwta on the densified output."""
+    ev = synthetic.events(3, 20, 64, seed=23, density=0.02)
+    o = orc.sparse_from_dense(ev)
+    rng = np.random.default_rng(24)
+    k = np.clip(rng.normal(0.8, 0.05, (64, 2, 5, 5)), 0, 1).astype(np.float32)
+    s = spk.SparseTensor(T(ev, dev))
+    kd = T(k, dev)
+    out = sparse.conv(s, kd, 8.0, pad=2)
+    oo = orc.sparse_conv(o, k, 8.0, (2, 2, 2, 2))
+    assert host(out) == oo
+    win = sparse.fcwta(out, 0, 16)
+    w, c = winners_np(win)
+    check_fcwta(orc, oo, w, c, 0, 16)
+    sparse.conv_stdp(s, kd, [spk.STDPConfig(0.004, -0.003)], win, 2)
+    assert np.array_equal(H(kd), orc.sparse_stdp(o, k, [(0.004, -0.003)], w, c, (2, 2, 2, 2)))
+
+
+def test_sparse_scans_past_one_grid(dev, orc):
+    """Prefix scans longer than one capped launch grid (4,194,304 entries):
+    2048 x 2100 single-element lists in the builder, and a 4.6M-event filter
+    (pool 1x1 keeps every event)."""
+    rng = np.random.default_rng(25)
+    d = (rng.random((2048, 2100, 1, 1, 1)) < 0.002).astype(np.uint8)
+    d[-1, -1] = 1
+    s = spk.SparseTensor(T(d, dev))
+    o = orc.sparse_from_dense(d)
+    assert host(s) == o and s.numel == len(o.ev)
+    full = spk.SparseTensor(torch.ones((1, 1, 80, 240, 240), dtype=torch.uint8, device=dev))
+    assert full.numel == 80 * 240 * 240
+    same = sparse.pool(full, 1)
+    assert same.numel == full.numel
+    assert torch.equal(same.offsets(), full.offsets()) and torch.equal(same.events(), full.events())
