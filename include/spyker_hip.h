@@ -328,6 +328,38 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
                              int64_t pkw, int64_t psh, int64_t psw, spk_winner *winners, int64_t *counts,
                              void *potentials, void *stream);
 
+/* -- layer plans: batches of uneven instances ------------------------------- */
+
+/* No single reference counterpart: the fused step above for a batch of n
+ * instances of their own size and step count, the batch the reference would
+ * run as n calls in order with one STDP pass over all winners afterwards
+ * (the reference's batch order: every instance's forward pass on the weights
+ * as they were, then the winners of instance 0, 1, ... in rank order).
+ * shapes: host array [n][3] of (t, h, w); t <= 32, Ho*Wo <= 65536.  The plan
+ * allocates its tables and work buffers once; spk_layer_plan_run then
+ * allocates nothing and never waits on the host, so a stream capture of it
+ * (hipGraph) can be replayed.  tune != 0: the first run times the region
+ * heights that fit and keeps the fastest (run it once before capturing).
+ * pkh = 0: no pooling. */
+typedef struct spk_layer_plan spk_layer_plan;
+int32_t spk_layer_plan_create(spk_layer_plan **plan, int64_t n, const int32_t *shapes, int64_t cin, int64_t cout,
+                              int64_t kh, int64_t kw, int64_t pt, int64_t pl, int64_t pb, int64_t pr,
+                              double threshold, int64_t ry, int64_t rx, int64_t count, int64_t pkh, int64_t pkw,
+                              int64_t psh, int64_t psw, int32_t tune);
+/* Byte offsets of each instance in the packed buffers (host arrays [n], any
+ * may be null): input [t][cin][h][w], spikes [t][cout][ho][wo], pooled
+ * spikes [t][cout][pho][pwo]; totals[3] = their total sizes. */
+int32_t spk_layer_plan_layout(const spk_layer_plan *plan, int64_t *in_off, int64_t *out_off, int64_t *pool_off,
+                              int64_t *totals);
+/* One step: packed U8 input, the FP32 master kernel [cout][cin][kh][kw]
+ * (updated in place when nconfigs > 0; every winner uses configuration 0),
+ * packed spikes and pooled spikes (either may be null), winners [n][count]
+ * and counts [n] as for spk_rank_convwta. */
+int32_t spk_layer_plan_run(spk_layer_plan *plan, const void *in, void *kernel, int32_t kernel_type,
+                           const spk_stdp_config *configs, int64_t nconfigs, void *out, void *pooled,
+                           spk_winner *winners, int64_t *counts, void *stream);
+int32_t spk_layer_plan_destroy(spk_layer_plan *plan);
+
 /* HIP-event timing of the main kernel of spk_stdp_layer_fused, recorded on
  * the launch stream around each launch while enabled (no reference
  * counterpart; the reference's unused Timer is src/spyker/base.h:80-95).

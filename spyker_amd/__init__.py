@@ -30,7 +30,7 @@ from ._lib import call
 __all__ = [
     "DoGFilter", "GaborFilter", "STDPConfig", "Winner", "Winners", "DoG", "Gabor", "Conv", "FC", "threshold",
     "fire", "code", "pool", "inhibit", "convwta", "fcwta", "conv", "fc", "pad", "random_seed", "expand2", "expand4",
-    "layer_step", "sparse", "SparseTensor", "to_sparse",
+    "layer_step", "LayerPlan", "sparse", "SparseTensor", "to_sparse",
 ]
 
 _TYPES = {torch.uint8: _lib.U8, torch.float16: _lib.F16, torch.float32: _lib.F32, torch.float64: _lib.F64,
@@ -547,6 +547,99 @@ def layer_step(conv: "Conv", array: torch.Tensor, threshold: float, radius, coun
          _ptr(pooled) if pooled is not None else None, pkh, pkw, psh, psw, _ptr(data), _ptr(cnt),
          _ptr(potentials) if potentials is not None else None, _stream(a))
     return out, Winners(data, cnt), pooled
+
+
+class LayerPlan:
+    """The fused layer step over a fixed batch of instances, each with its own
+    size and step count (spk_layer_plan_*): one persistent region kernel per
+    step, then convwta, the spike/pool writer and STDP, in the reference's
+    batch order (every instance's forward pass, then the winners of instance
+    0, 1, ... in rank order).
+
+    ``shapes`` lists (t, h, w) per instance.  Inputs, spikes and pooled spikes
+    live in packed U8 buffers (``pack``, ``spikes``, ``pooled``).  The plan
+    holds every work buffer, so ``run`` allocates nothing when given its
+    output buffers, and a captured run (torch.cuda.graph) replays.
+    ``tune=True`` times the region heights that fit on the first run.
+    """
+
+    def __init__(self, conv: "Conv", shapes, threshold: float, radius, count: int, pool_kernel=None,
+                 pool_stride=None, tune: bool = False):
+        sh = np.ascontiguousarray(np.asarray(shapes, dtype=np.int32).reshape(-1, 3))
+        if conv.kernel.dtype != torch.float32:
+            raise RuntimeError("Fused layer: the master weights must be f32.")
+        if conv._stride != (1, 1):
+            raise RuntimeError("Fused layer: stride must be 1.")
+        self.conv, self.shapes, self.count = conv, sh, int(count)
+        self.device = conv.kernel.device
+        cout, cin, kh, kw = conv.kernel.shape
+        pt, pl, pb, pr = conv._pad
+        ry, rx = expand2(radius)
+        pkh = pkw = psh = psw = 0
+        if pool_kernel is not None:
+            pkh, pkw = expand2(pool_kernel)
+            psh, psw = expand2(pool_stride if pool_stride is not None else pool_kernel)
+        self._pool = (pkh, pkw, psh, psw)
+        n = sh.shape[0]
+        self._h = C.c_void_p()
+        with torch.cuda.device(self.device):
+            call("spk_layer_plan_create", C.byref(self._h), n, C.c_void_p(sh.ctypes.data), cin, cout, kh, kw, pt, pl,
+                 pb, pr, float(threshold), ry, rx, int(count), pkh, pkw, psh, psw, 1 if tune else 0)
+        self.in_off, self.out_off, self.pool_off = (np.zeros(n, np.int64) for _ in range(3))
+        tot = np.zeros(3, np.int64)
+        call("spk_layer_plan_layout", self._h, C.c_void_p(self.in_off.ctypes.data),
+             C.c_void_p(self.out_off.ctypes.data), C.c_void_p(self.pool_off.ctypes.data), C.c_void_p(tot.ctypes.data))
+        self.in_bytes, self.out_bytes, self.pool_bytes = (int(v) for v in tot)
+        self.out_shapes = [(t, cout, h + pt + pb - kh + 1, w + pl + pr - kw + 1) for t, h, w in sh]
+        self.pool_shapes = [(t, cout, (ho - pkh) // psh + 1, (wo - pkw) // psw + 1) if pkh else None
+                            for t, _, ho, wo in self.out_shapes]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().spk_layer_plan_destroy(h)
+            except Exception:
+                pass
+
+    def __len__(self):
+        return self.shapes.shape[0]
+
+    def pack(self, arrays) -> torch.Tensor:
+        """Packs per-instance U8 inputs [t, cin, h, w] into one device buffer."""
+        buf = torch.zeros(self.in_bytes, dtype=torch.uint8, device=self.device)
+        for i, a in enumerate(arrays):
+            flat = a.reshape(-1)
+            buf[int(self.in_off[i]):int(self.in_off[i]) + flat.numel()] = flat.to(self.device)
+        return buf
+
+    def spikes(self, packed: torch.Tensor, i: int) -> torch.Tensor:
+        """Instance i's spikes [t, cout, ho, wo] in a packed spike buffer."""
+        shp = self.out_shapes[i]
+        return packed[int(self.out_off[i]):int(self.out_off[i]) + int(np.prod(shp))].view(shp)
+
+    def pooled(self, packed: torch.Tensor, i: int) -> torch.Tensor:
+        shp = self.pool_shapes[i]
+        return packed[int(self.pool_off[i]):int(self.pool_off[i]) + int(np.prod(shp))].view(shp)
+
+    def buffers(self):
+        """Fresh (spikes, pooled-or-None, winners data, counts) buffers for run()."""
+        out = torch.empty(self.out_bytes, dtype=torch.uint8, device=self.device)
+        pooled = torch.empty(self.pool_bytes, dtype=torch.uint8, device=self.device) if self._pool[0] else None
+        data = torch.zeros((len(self), self.count, 5), dtype=torch.int64, device=self.device)
+        cnt = torch.zeros((len(self),), dtype=torch.int64, device=self.device)
+        return out, pooled, data, cnt
+
+    def run(self, packed: torch.Tensor, learn: bool = True, buffers=None):
+        """One step on a packed input; returns (spikes, pooled-or-None, Winners)."""
+        _dev(packed)
+        if packed.dtype != torch.uint8 or packed.numel() < self.in_bytes:
+            raise RuntimeError("Fused layer: packed U8 input of the plan's size expected.")
+        out, pooled, data, cnt = buffers if buffers is not None else self.buffers()
+        cfg, ncfg = _configs(self.conv.stdpconfig) if learn else (None, 0)
+        call("spk_layer_plan_run", self._h, _ptr(packed), _ptr(self.conv.kernel), _type(self.conv.kernel), cfg, ncfg,
+             _ptr(out), _ptr(pooled) if pooled is not None else None, _ptr(data), _ptr(cnt), _stream(packed))
+        return out, pooled, Winners(data, cnt)
 
 
 class FC:

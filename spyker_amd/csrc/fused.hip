@@ -40,29 +40,12 @@
 //     a second at c*'s first firing step yields convwta's t0 value.  The
 //     gathered passes issue the same MFMA sequence on the same operands as
 //     the step loop, so they reproduce its potentials bit for bit.
-#include <vector>
-
-#include "stdp.hpp"
+#include "fused.hpp"
 
 namespace spk
 {
 namespace
 {
-typedef _Float16 half8 __attribute__((ext_vector_type(8)));
-typedef _Float16 half2v __attribute__((ext_vector_type(2)));
-typedef float float4v __attribute__((ext_vector_type(4)));
-
-constexpr int MAX_STEPS = 16;  // K-steps of 32 -> K <= 512
-constexpr int MAX_WAVES = 8;   // Cout <= 256
-constexpr int COLS = 16;
-
-struct Summary
-{
-    int32_t c;
-    uint32_t mask;
-    float first, max;
-};
-
 struct FusedGeom
 {
     int32_t n, t, cin, h, w, cout, kh, kw, pt, pl, ho, wo;
@@ -93,46 +76,6 @@ __device__ __forceinline__ TapEntry tap_entry(const FusedGeom &g, int32_t grp)
     return TapEntry{(dy * g.w + dx) * g.cin_pad + c8 * 8, dy, dx, ky * g.pitch + c8 * g.pw + kx};
 }
 
-// [row tile][step][lane][8] FP16 fragments of the FP32 masters.
-__global__ void prep_fragments(const float *__restrict__ k, half8 *__restrict__ frag, FusedGeom g)
-{
-    const int32_t rows = g.waves * 2, total = rows * g.steps * 64;
-    for (int32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x)
-    {
-        const int32_t lane = e % 64, s = (e / 64) % g.steps, rt = e / (64 * g.steps);
-        const int32_t o = rt * 16 + lane % 16, grp = 4 * s + lane / 16;
-        const int32_t cg = g.cin_pad / 8, tap = grp / cg, c8 = grp % cg;
-        half8 v;
-        for (int j = 0; j < 8; ++j)
-        {
-            const int32_t c = c8 * 8 + j;
-            float x = 0.0f;
-            if (o < g.cout && grp < g.groups && c < g.cin)
-                x = k[((int64_t(o) * g.cin + c) * g.kh + tap / g.kw) * g.kw + tap % g.kw];
-            v[j] = _Float16(x);
-        }
-        frag[e] = v;
-    }
-}
-
-__device__ __forceinline__ uint4 widen(uint2 raw)
-{
-    // bytes -> FP16 exactly: 0x64 above byte b is the half 1024 + b, minus 1024
-    const half2v bias = {_Float16(1024.0f), _Float16(1024.0f)};
-    const uint32_t hi = 0x64646464u;
-    uint32_t w[4] = {__builtin_amdgcn_perm(hi, raw.x, 0x04010400u), __builtin_amdgcn_perm(hi, raw.x, 0x04030402u),
-                     __builtin_amdgcn_perm(hi, raw.y, 0x04010400u), __builtin_amdgcn_perm(hi, raw.y, 0x04030402u)};
-#pragma unroll
-    for (int i = 0; i < 4; ++i) w[i] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(half2v, w[i]) - bias);
-    return make_uint4(w[0], w[1], w[2], w[3]);
-}
-
-__device__ __forceinline__ void combine(float &v, int32_t &i, float ov, int32_t oi)
-{
-    const bool take = (ov > v) | ((ov == v) & (oi < i));
-    v = take ? ov : v;
-    i = take ? oi : i;
-}
 
 // Writes input byte i of the sample's [t][cin][h][w] tensor to its
 // [t][y][x][c] slot of the (unpadded, channel-innermost) U8 stage.
@@ -141,56 +84,6 @@ __device__ __forceinline__ void stage_bytes(uint8_t *stage, const FusedGeom &g, 
 {
     const int32_t tt = i / chw, r = i - tt * chw, c = r / hwi, pix = r - c * hwi;
     stage[(tt * hwi + pix) * g.cin_pad + c] = v;
-}
-
-// For the 4 bytes of v seen at step tt: bytes of res whose input is nonzero
-// for the first time (not yet in `seen`) become tt.
-__device__ __forceinline__ void first_nonzero(uint32_t &res, uint32_t &seen, uint32_t v, int32_t tt)
-{
-    const uint32_t nz = (((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v) & 0x80808080u;  // 0x80 in nonzero bytes
-    const uint32_t fresh = ((nz & ~seen) >> 7) * 0xFFu;
-    seen |= nz;
-    res = (res & ~fresh) | (uint32_t(tt) * 0x01010101u & fresh);
-}
-
-// One step of one lane's 8 (channel, pixel) values: shift "v > th" into the
-// masks (newest step in bit 0, via the carry input of v_addc) and keep the
-// running maximum as a signed compare of the float bits, which orders every
-// positive value correctly; the maximum is only read when a value fired,
-// and then it is the largest fired value (> th >= 0).
-__device__ __forceinline__ void step_update(const float4v &acc0, const float4v &acc1, float th,
-                                            uint32_t (&mask)[2][4], int32_t (&mxb)[2][4])
-{
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-        {
-            const float v = r == 0 ? acc0[i] : acc1[i];
-            const uint64_t fired = __builtin_amdgcn_ballot_w64(v > th);
-            uint64_t carry;
-            asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(mask[r][i]), "=s"(carry) : "v"(mask[r][i]), "s"(fired));
-            mxb[r][i] = max(mxb[r][i], __builtin_bit_cast(int32_t, v));
-        }
-}
-
-template <int S>
-__device__ __forceinline__ void load_b(half8 (&b)[S], const uint8_t *base, const int32_t (&offs)[S])
-{
-#pragma unroll
-    for (int s = 0; s < S; ++s) b[s] = *reinterpret_cast<const half8 *>(base + offs[s]);
-}
-
-template <int S>
-__device__ __forceinline__ void mfma_chain(const half8 (&a)[2][S], const half8 (&b)[S], float4v &c0, float4v &c1)
-{
-    c0 = float4v{0, 0, 0, 0}, c1 = float4v{0, 0, 0, 0};
-#pragma unroll
-    for (int s = 0; s < S; ++s)
-    {
-        c0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0][s], b[s], c0, 0, 0, 0);
-        c1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1][s], b[s], c1, 0, 0, 0);
-    }
 }
 
 template <int S, bool DIAG>
@@ -673,163 +566,16 @@ __global__ void __launch_bounds__(256) fused_write(const Summary *__restrict__ s
     }
 }
 
-struct FusedStdp
-{
-    int64_t slots;  // n * count
-    int32_t count, cin, cin_pad, h, w, kh, kw, pt, pl, wo, hw, t, words;
-};
+}  // namespace
 
-// Potentiate/depress bits of every winner slot, one wave per slot: bit i of
-// slot e is isum <= osum for weight i = (ch, ky, kx) of the winner's kernel
-// (stdp.cpp:88-118).  A winner is always its pixel's kept channel, so osum
-// is the first set bit of that pixel's fire mask; isum comes from the
-// first-spike map ([n][h*w][cin_pad], t outside the input).
-__global__ void __launch_bounds__(256)
-    fused_records(const uint8_t *__restrict__ fs, const Summary *__restrict__ summary,
-                  const int32_t *__restrict__ wkey, FusedStdp p, uint32_t *__restrict__ bits)
-{
-    const int64_t e = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
-    if (e >= p.slots) return;
-    const int32_t k = wkey[e];
-    if (k < 0) return;
-    const int lane = threadIdx.x & 63;
-    const int64_t s = e / p.count;
-    const int32_t pos = k & 0xFFFF, y = pos / p.wo - p.pt, x = pos % p.wo - p.pl;
-    const uint32_t mask = summary[s * p.hw + pos].mask;
-    const int32_t osum = mask ? int32_t(__builtin_ctz(mask)) : p.t;
-    const int32_t taps = p.kh * p.kw, ksz = p.cin * taps;
-    const uint8_t *fsn = fs + s * p.h * p.w * p.cin_pad;
-    uint32_t *rec = bits + e * p.words;
-    for (int32_t base = 0; base < ksz; base += 64)
-    {
-        const int32_t i = base + lane;
-        bool bit = false;
-        if (i < ksz)
-        {
-            const int32_t ch = i / taps, iy = y + (i / p.kw) % p.kh, ix = x + i % p.kw;
-            const int32_t isum =
-                (iy >= 0 && iy < p.h && ix >= 0 && ix < p.w) ? fsn[(iy * p.w + ix) * p.cin_pad + ch] : p.t;
-            bit = isum <= osum;
-        }
-        const uint64_t b = __ballot(bit);
-        if (lane == 0 && base / 32 < p.words) rec[base / 32] = uint32_t(b);
-        if (lane == 32 && base / 32 + 1 < p.words) rec[base / 32 + 1] = uint32_t(b >> 32);
-    }
-}
-
-constexpr int STDP_THREADS = 256;
-constexpr int STDP_LIST = 1024;  // winners staged in LDS per pass
-
-// The update chains of the fused step (stdp.cpp:139-154): the winners of
-// every sample in rank order, samples in order, each updating the kernel of
-// its channel; all use configuration 0.  One block per (output channel,
-// slice of STDP_THREADS weights): each thread scans a contiguous range of
-// winner slots, one block-wide scan lists the slots naming the channel in
-// slot order, their bit words are staged in LDS, and every thread runs the
-// chain of its weight 32 updates at a time from a register of bits.
-__global__ void __launch_bounds__(STDP_THREADS)
-    fused_stdp(float *__restrict__ kernel, const uint32_t *__restrict__ bits, const int32_t *__restrict__ wkey,
-               FusedStdp p, spk_stdp_config cfg)
-{
-    __shared__ int32_t list[STDP_LIST];
-    __shared__ uint32_t lbits[STDP_LIST * (STDP_THREADS / 32)];  // the slice's words of each listed winner
-    __shared__ int32_t wave_sum[STDP_THREADS / 64];
-    const int32_t key = blockIdx.x, ksz = p.cin * p.kh * p.kw;
-    const int32_t i = blockIdx.y * STDP_THREADS + threadIdx.x;
-    const bool live = i < ksz;
-    float *kz = kernel + int64_t(key) * ksz;
-    float wr = live ? kz[i] : 0.0f;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    // this slice's words of a winner's record
-    const int32_t word0 = blockIdx.y * (STDP_THREADS / 32);
-    const int32_t sw = min(STDP_THREADS / 32, p.words - word0), my_word = (i / 32) - word0;
-
-    // the thread's slots [e0, e1), a multiple of 4 wide so keys load as int4
-    const int64_t per = (p.slots + 4 * STDP_THREADS - 1) / (4 * STDP_THREADS) * 4;
-    const int64_t e0 = min(int64_t(threadIdx.x) * per, p.slots), e1 = min(e0 + per, p.slots);
-    auto keys4 = [&](int64_t e) {
-        if (e + 4 <= p.slots) return *reinterpret_cast<const int4 *>(wkey + e);
-        int4 k = make_int4(-1, -1, -1, -1);
-        if (e < p.slots) k.x = wkey[e];
-        if (e + 1 < p.slots) k.y = wkey[e + 1];
-        if (e + 2 < p.slots) k.z = wkey[e + 2];
-        return k;
-    };
-    auto hit = [&](int32_t k) { return (k >> 16) == key; };
-    int32_t cnt = 0;
-    for (int64_t e = e0; e < e1; e += 4)
-    {
-        const int4 k = keys4(e);
-        cnt += hit(k.x) + hit(k.y) + hit(k.z) + hit(k.w);
-    }
-    int32_t incl = cnt;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1)
-    {
-        const int32_t o = __shfl_up(incl, d, 64);
-        incl += lane >= d ? o : 0;
-    }
-    if (lane == 63) wave_sum[wave] = incl;
-    __syncthreads();
-    int32_t first = incl - cnt, m = 0;
-    for (int v = 0; v < STDP_THREADS / 64; ++v)
-    {
-        first += v < wave ? wave_sum[v] : 0;
-        m += wave_sum[v];
-    }
-
-    for (int32_t pass = 0; pass < m; pass += STDP_LIST)
-    {
-        // the thread's matches with list index in [pass, pass + STDP_LIST)
-        if (first < pass + STDP_LIST && first + cnt > pass)
-        {
-            int32_t at = first;
-            for (int64_t e = e0; e < e1; e += 4)
-            {
-                const int4 k4 = keys4(e);
-                const int32_t kk[4] = {k4.x, k4.y, k4.z, k4.w};
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    if (hit(kk[u]))
-                    {
-                        if (at >= pass && at < pass + STDP_LIST) list[at - pass] = int32_t(e + u);
-                        ++at;
-                    }
-            }
-        }
-        __syncthreads();
-        const int32_t mm = min(STDP_LIST, m - pass);
-        for (int32_t q = threadIdx.x; q < mm * sw; q += STDP_THREADS)
-            lbits[q] = bits[int64_t(list[q / sw]) * p.words + word0 + q % sw];
-        __syncthreads();
-        if (live)
-            for (int32_t q0 = 0; q0 < mm; q0 += 32)
-            {
-                // bits of the next 32 updates of this weight, then the chain
-                uint32_t b = 0;
-#pragma unroll
-                for (int u = 0; u < 32; ++u)
-                    b |= ((lbits[min(q0 + u, mm - 1) * sw + my_word] >> (i & 31)) & 1u) << u;
-                const int32_t n_up = min(32, mm - q0);
-                for (int u = 0; u < n_up; ++u) wr = stdp_update(wr, (b >> u) & 1u, cfg);
-            }
-        __syncthreads();
-    }
-    if (live) kz[i] = wr;
-}
-
-// Optional HIP-event timing of the main fused kernel (spk_kernel_timer).
-struct KernelTimer
-{
-    bool on = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
-};
 KernelTimer &timer()
 {
     static KernelTimer *t = new KernelTimer;
     return *t;
 }
 
+namespace
+{
 template <int S, bool DIAG>
 void launch_layer(const uint8_t *in, const half8 *frag, Summary *summary, float *pot, uint8_t *fs, const FusedGeom &g,
                   size_t lds,
@@ -960,12 +706,19 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
         g.win_units = g.rows * g.pitch;
         g.th = float(threshold);
         SPK_ARG(g.steps <= MAX_STEPS, "Fused layer: Cin*Kh*Kw too large (K <= 512 after padding).");
-        SPK_ARG(row_units <= 64 * g.waves, "Fused layer: padded width * Cin too large.");
         // stage, gathered fragments, any/argmax partials, step table, tap table
         const size_t fixed = size_t(g.stage_bytes) + size_t(g.steps) * 4 * COLS * 16 + MAX_WAVES * COLS * 12 +
                              COLS * 4 + size_t(g.steps) * 4 * 16;
         const int64_t room = (163840 - int64_t(fixed)) / (int64_t(g.win_units) * 16);
-        SPK_ARG(room >= 1, "Fused layer: the staged sample (" << fixed << " bytes) leaves no room in the 160 KiB LDS.");
+        if (room < 1 || row_units > 64 * g.waves || ho * wo > 8192 || fixed > 163840)
+        {
+            // the sample does not fit the LDS whole: the region path
+            const StdpConfigs rcfg = learn ? pack_configs(configs, nconfigs) : StdpConfigs{};
+            region_layer_step((const uint8_t *)in, n, t, cin, h, w, (float *)kernel, cout, kh, kw, pt, pl, pb, pr,
+                              threshold, ry, rx, count, learn ? &rcfg.c[0] : nullptr, (uint8_t *)out,
+                              (uint8_t *)pooled, pkh, pkw, psh, psw, winners, counts, (float *)potentials, S(stream));
+            return;
+        }
         const int64_t chunks = cdiv(t, room < t ? room : t);
         g.tc = int32_t(cdiv(t, chunks));
         g.stage_fast = (h * w) % 4 == 0 && (reinterpret_cast<uintptr_t>(in) & 3) == 0 &&
@@ -973,7 +726,6 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
         const size_t lds = fixed + size_t(g.tc) * g.win_units * 16;
         SPK_ARG(lds <= 163840, "Fused layer: the staged sample (" << lds << " bytes) exceeds the 160 KiB LDS.");
         SPK_ARG(int64_t(cout) * ho * wo < (1LL << 31), "Fused layer: output plane too large.");
-        SPK_ARG(ho * wo <= 8192, "Fused layer: convwta supports at most 8192 output pixels per sample.");
 
         const StdpConfigs cfg = learn ? pack_configs(configs, nconfigs) : StdpConfigs{};
         SPK_ARG(!learn || n * count < (1LL << 31), "Fused layer: too many winner slots.");
@@ -983,7 +735,8 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
         Scratch frag(int64_t(g.waves) * 2 * g.steps * 64 * 16, s), summary(n * hw * int64_t(sizeof(Summary)), s);
         Scratch wkey(n * count * int64_t(sizeof(int32_t)), s), fsm(learn ? n * hwi * g.cin_pad : 0, s);
         prep_fragments<<<dim3(unsigned(cdiv(int64_t(g.waves) * 2 * g.steps * 64, 256))), 256, 0, s>>>(
-            (const float *)kernel, frag.as<half8>(), g);
+            (const float *)kernel, frag.as<half8>(), g.cout, g.cin, g.kh, g.kw, g.cin_pad, g.groups, g.steps,
+            g.waves * 2);
         SPK_LAUNCHED();
         dispatch_layer(g.steps, (const uint8_t *)in, frag.as<half8>(), summary.as<Summary>(), (float *)potentials,
                        learn ? fsm.as<uint8_t>() : nullptr, g, lds, s);

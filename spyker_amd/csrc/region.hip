@@ -1,0 +1,935 @@
+// Fused FP16-MFMA layer step over output regions: for samples whose input
+// does not fit the LDS whole, and for batches of instances of uneven size and
+// step count (one persistent launch per layer).
+//
+// The semantics are those of the whole-sample kernel (fused.hip): conv ->
+// threshold -> inhibit -> fire summary per output pixel, then convwta, the
+// spike/pool writer and STDP on the FP32 masters, in the reference's batch
+// order (src/spyker/opers.cpp:408-693).  What changes is the unit of work:
+//
+//   - a work item is (instance, region), a region being RH output rows x 16
+//     output columns; the LDS holds the region's input rectangle with its halo
+//     for all of the instance's steps, [t][SH][SW][cin_pad] U8, read from the
+//     instance's [t][cin][h][w] input (dword loads and quad transposes when
+//     w % 4 == 0);
+//   - a block of CG x NCT waves works on NCT region rows at once: wave
+//     (channel group, column tile) owns 32 output channels of one row of 16
+//     pixels, its FP16 weight fragments in VGPRs for the whole kernel;
+//   - the FP16 windows of a chunk of steps cover the NCT rows and their halo;
+//     every wave runs the chunk's steps without barriers, as in fused.hip;
+//   - at a row's end the gathered MFMA passes (inhibit's kept channel, then
+//     convwta's t0 value) load each lane's B operands straight from the
+//     stage at the pixel's own step, the same bytes and the same MFMA
+//     sequence as the step loop, so they reproduce its potentials exactly;
+//   - the region also writes the first-spike map of the input pixels it owns
+//     (a partition of the input), for the STDP records.
+//
+// A plan (spk_layer_plan_*) fixes the shapes once: instance and item tables
+// and every work buffer are allocated when it is made, so running it
+// allocates nothing and never waits on the host, and a stream capture
+// (hipGraph) of a run replays it.  Region height and chunking are chosen by
+// timing the candidates on the first run when the plan was made with tuning.
+#include <algorithm>
+#include <memory>
+#include <vector>
+
+#include "fused.hpp"
+
+namespace spk
+{
+namespace
+{
+constexpr int RW = 16;            // region width: one column tile per region row
+constexpr int REGION_WAVES = 8;   // waves per block
+constexpr int LDS_BYTES = 163840;
+
+struct RegionInst
+{
+    int64_t in_off;    // bytes: the instance's input [t][cin][h][w]
+    int64_t out_off;   // bytes: its spikes [t][cout][ho][wo]
+    int64_t pool_off;  // bytes: its pooled spikes [t][cout][pho][pwo]
+    int64_t sum_off;   // its first Summary (and select slot)
+    int64_t fs_off;    // bytes: its first-spike map [h*w][cin_pad]
+    int32_t t, h, w, ho, wo, pho, pwo, cols;  // cols = regions per region row
+};
+
+struct RegionGeom
+{
+    int32_t cin, cout, kh, kw, pt, pl;
+    int32_t cin_pad, groups, steps, cg, nct, waves;
+    int32_t rh;                  // region rows, a multiple of nct
+    int32_t sh, sw;              // stage rows and columns (sw a multiple of 4)
+    int32_t wrows, wcols, pitch, win_units, tc;
+    int32_t stage_bytes;         // tmax * sh * sw * cin_pad, 16-aligned
+    int32_t fast;                // dword staging (every w % 4 == 0, 4-aligned inputs)
+    int64_t items;
+    float th;
+};
+
+// LDS bytes past the stage and windows: any/argmax partials, step table, tap table.
+inline int64_t region_fixed(int32_t waves, int32_t nct, int32_t steps)
+{
+    return int64_t(waves) * COLS * 12 + int64_t(nct) * COLS * 4 + int64_t(steps) * 4 * 16;
+}
+
+template <int S, bool DIAG>
+__global__ void __launch_bounds__(64 * REGION_WAVES)
+    fused_region(const uint8_t *__restrict__ in, const half8 *__restrict__ frag, const RegionInst *__restrict__ inst,
+                 const int2 *__restrict__ items, Summary *__restrict__ summary, float *__restrict__ pot_out,
+                 uint8_t *__restrict__ fs_out, RegionGeom g)
+{
+    extern __shared__ __align__(16) uint8_t lds[];
+    uint8_t *stage = lds;                                                          // [t][sh][sw][cin_pad]
+    uint4 *win16 = reinterpret_cast<uint4 *>(lds + g.stage_bytes);                 // [tc][wrows][pitch]
+    uint32_t *any_s = reinterpret_cast<uint32_t *>(win16 + g.tc * g.win_units);    // [waves][COLS]
+    float *red_v = reinterpret_cast<float *>(any_s + g.waves * COLS);             // [waves][COLS]
+    int32_t *red_i = reinterpret_cast<int32_t *>(red_v + g.waves * COLS);         // [waves][COLS]
+    int32_t *tsel = red_i + g.waves * COLS;                                       // [nct][COLS]
+    int4 *gtab = reinterpret_cast<int4 *>(tsel + g.nct * COLS);                   // [4S] (dy, dx, byte, 0)
+
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, q = lane >> 4, j = lane & 15;
+    const int32_t cgw = w % g.cg, ct = w / g.cg;
+    const int32_t o0 = cgw * 32 + 4 * q;  // the lane's first channel; rows r*16 + i follow
+    const int32_t planes = g.cin_pad / 8;
+
+    // weight fragments and window offsets, resident for the whole kernel;
+    // groups past the real K (zero weights) reuse group 0's operands
+    half8 a[2][S];
+    int32_t offs[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+    {
+        a[0][s] = frag[((cgw * 2 + 0) * S + s) * 64 + lane];
+        a[1][s] = frag[((cgw * 2 + 1) * S + s) * 64 + lane];
+        const int32_t grp = 4 * s + q, gg = grp < g.groups ? grp : 0, tap = gg / planes, c8 = gg % planes;
+        offs[s] = ((tap / g.kw) * g.pitch + c8 * g.wcols + tap % g.kw) * 16;
+    }
+    for (int32_t e = threadIdx.x; e < 4 * S; e += blockDim.x)
+    {
+        const int32_t gg = e < g.groups ? e : 0, tap = gg / planes, c8 = gg % planes;
+        gtab[e] = make_int4(tap / g.kw - g.pt, tap % g.kw - g.pl, c8 * 8, 0);
+    }
+    const int32_t win_step = g.win_units * 16;
+
+    // window build roles: a thread owns one unit (plane, column) of a window
+    // row and walks the (step, row) pairs of a chunk by `wslots`
+    const int32_t row_units = planes * g.wcols, wslots = int32_t(blockDim.x) / row_units;
+    const int32_t wcu = int32_t(threadIdx.x) % row_units, wslot = int32_t(threadIdx.x) / row_units;
+    const int32_t wc8 = wcu / g.wcols, wcc = wcu % g.wcols;
+    const int32_t wdq = wslots / g.wrows, wdr = wslots % g.wrows;
+
+    // staging roles (dword path): quad Q of a slot holds channel quad c4 of
+    // pixel quad lq; the slot walks the (step, stage row) pairs
+    const int32_t D = g.cin_pad / 4, nq = g.sw / 4, sroles = nq * D * 4;
+    const int32_t sslot = int32_t(threadIdx.x) / sroles, srole = int32_t(threadIdx.x) % sroles;
+    const int32_t sslots = int32_t(blockDim.x) / sroles;
+    const int32_t sQ = srole >> 2, l4 = srole & 3, c4 = sQ % D, lq = sQ / D, sc = 4 * c4 + l4;
+    const uint32_t sel1 = (l4 & 1) ? 0x03070105u : 0x06020400u;
+    const uint32_t sel2 = (l4 & 2) ? 0x03020706u : 0x05040100u;
+    // staging roles (byte path): (channel, column) pairs, columns fastest so
+    // a wave reads consecutive input bytes; a thread takes every
+    // blockDim-th pair and walks the (step, stage row) pairs
+    const int32_t broles = g.sw * g.cin;
+
+    // blocks of one XCD (dispatched round-robin over 8) take consecutive
+    // items, so neighbouring regions share their halo in that XCD's L2
+    const int64_t nb = gridDim.x;
+    const int64_t first = nb % 8 == 0 ? (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : blockIdx.x;
+
+    for (int64_t it = first; it < g.items; it += nb)
+    {
+        const int2 itm = items[it];
+        const RegionInst I = inst[itm.x];
+        const int32_t oy0 = (itm.y / I.cols) * g.rh, ox0 = (itm.y % I.cols) * RW;
+        const int32_t sy0 = oy0 - g.pt, sx0 = (ox0 - g.pl) & ~3;  // stage origin (input coordinates)
+        const int32_t xoff = ox0 - g.pl - sx0;
+        const uint8_t *src = in + I.in_off;
+        __syncthreads();  // the previous item's stage and windows are free
+
+        // stage the input rectangle [sy0, sy0 + sh) x [sx0, sx0 + sw), all steps
+        const int32_t srows = I.t * g.sh;
+        if (g.fast)
+        {
+            if (sslot < sslots)
+            {
+                uint32_t *dst = reinterpret_cast<uint32_t *>(stage);
+                const int32_t ix0 = sx0 + 4 * lq;
+                const bool colv = ix0 >= 0 && ix0 < I.w && sc < g.cin;
+                int32_t tt = sslot / g.sh, lr = sslot % g.sh;
+                const int32_t dq = sslots / g.sh, dr = sslots % g.sh;
+                for (int32_t r = sslot; r < srows; r += sslots)
+                {
+                    const int32_t iy = sy0 + lr;
+                    uint32_t x = 0;
+                    if (colv && iy >= 0 && iy < I.h)
+                        x = *reinterpret_cast<const uint32_t *>(src + ((int64_t(tt) * g.cin + sc) * I.h + iy) * I.w +
+                                                               ix0);
+                    // 4x4 byte transpose within the quad: lane l4 then holds
+                    // channels 4*c4 .. 4*c4+3 of pixel 4*lq + l4
+                    const uint32_t y = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(x), 0xB1, 0xF, 0xF, true));
+                    const uint32_t r1 = __builtin_amdgcn_perm(y, x, sel1);
+                    const uint32_t y2 = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(r1), 0x4E, 0xF, 0xF, true));
+                    dst[((tt * g.sh + lr) * g.sw + 4 * lq + l4) * D + c4] = __builtin_amdgcn_perm(y2, r1, sel2);
+                    tt += dq, lr += dr;
+                    if (lr >= g.sh) lr -= g.sh, ++tt;
+                }
+            }
+        }
+        else
+        {
+            // the padded channels cin .. cin_pad of every staged pixel are 0
+            if (g.cin != g.cin_pad)
+                for (int32_t u = threadIdx.x; u < srows * g.sw; u += blockDim.x)
+                    for (int32_t c = g.cin; c < g.cin_pad; ++c) stage[u * g.cin_pad + c] = 0;
+            for (int32_t role = threadIdx.x; role < broles; role += blockDim.x)
+            {
+                const int32_t blc = role % g.sw, bc = role / g.sw, ix = sx0 + blc;
+                const bool colv = ix >= 0 && ix < I.w;
+                int32_t tt = 0, lr = 0;
+                for (int32_t r = 0; r < srows; ++r)
+                {
+                    const int32_t iy = sy0 + lr;
+                    uint8_t v = 0;
+                    if (colv && iy >= 0 && iy < I.h) v = src[((int64_t(tt) * g.cin + bc) * I.h + iy) * I.w + ix];
+                    stage[((tt * g.sh + lr) * g.sw + blc) * g.cin_pad + bc] = v;
+                    if (++lr == g.sh) lr = 0, ++tt;
+                }
+            }
+        }
+        __syncthreads();
+
+        if (fs_out)
+        {
+            // first spiking step of the input pixels this region owns (t when
+            // never): rows [oy0 - pt, oy0 + rh - pt), columns [ox0 - pl,
+            // ox0 + 16 - pl), clipped, the last band of either extending to
+            // the input's edge; 8 channels per thread, [h*w][cin_pad]
+            const int32_t iy_lo = max(0, sy0), ix_lo = max(0, ox0 - g.pl);
+            const int32_t iy_hi = oy0 + g.rh >= I.ho ? I.h : min(I.h, oy0 + g.rh - g.pt);
+            const int32_t ix_hi = ox0 + RW >= I.wo ? I.w : min(I.w, ox0 + RW - g.pl);
+            const int32_t ow = ix_hi - ix_lo, units = (iy_hi - iy_lo) * ow * planes;
+            const uint32_t tb = uint32_t(I.t) * 0x01010101u;
+            for (int32_t u = threadIdx.x; u < units; u += blockDim.x)
+            {
+                const int32_t c8 = u % planes, px = u / planes, iy = iy_lo + px / ow, ix = ix_lo + px % ow;
+                const uint8_t *sp = stage + ((iy - sy0) * g.sw + ix - sx0) * g.cin_pad + c8 * 8;
+                uint32_t lo = tb, hi = tb, seen_lo = 0, seen_hi = 0;
+                for (int32_t tt = 0; tt < I.t; ++tt)
+                {
+                    const uint2 v = *reinterpret_cast<const uint2 *>(sp + tt * g.sh * g.sw * g.cin_pad);
+                    first_nonzero(lo, seen_lo, v.x, tt);
+                    first_nonzero(hi, seen_hi, v.y, tt);
+                }
+                *reinterpret_cast<uint2 *>(fs_out + I.fs_off + (int64_t(iy) * I.w + ix) * g.cin_pad + c8 * 8) =
+                    make_uint2(lo, hi);
+            }
+        }
+
+        // groups of nct region rows; wave (cgw, ct) takes row grp0 + ct
+        for (int32_t grp0 = 0; grp0 < g.rh && oy0 + grp0 < I.ho; grp0 += g.nct)
+        {
+            const int32_t oy = oy0 + grp0 + ct, ox = ox0 + j;
+            const bool pvalid = oy < I.ho && ox < I.wo;
+            const uint8_t *wb = reinterpret_cast<const uint8_t *>(win16) + (ct * g.pitch + j) * 16;
+            uint32_t mask[2][4];
+            int32_t mxb[2][4];
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) mask[r][i] = 0, mxb[r][i] = 0;
+
+            for (int32_t t0 = 0; t0 < I.t; t0 += g.tc)
+            {
+                const int32_t tc = min(g.tc, I.t - t0);
+                // widen the windows of steps [t0, t0 + tc): stage rows
+                // grp0 .. grp0 + wrows, columns xoff .. xoff + wcols; units
+                // outside the input are 0
+                if (wslot < wslots)
+                {
+                    const int32_t ix = ox0 - g.pl + wcc;
+                    const bool colv = ix >= 0 && ix < I.w;
+                    const uint8_t *sp0 = stage + ((grp0 * g.sw) + xoff + wcc) * g.cin_pad + wc8 * 8;
+                    int32_t ts = wslot / g.wrows, r = wslot % g.wrows;
+                    for (int32_t pr = wslot; pr < tc * g.wrows; pr += wslots)
+                    {
+                        const int32_t iy = sy0 + grp0 + r;
+                        uint2 raw = make_uint2(0, 0);
+                        if (colv && iy >= 0 && iy < I.h)
+                            raw = *reinterpret_cast<const uint2 *>(sp0 + ((t0 + ts) * g.sh + r) * g.sw * g.cin_pad);
+                        win16[ts * g.win_units + r * g.pitch + wcu] = widen(raw);
+                        ts += wdq, r += wdr;
+                        if (r >= g.wrows) r -= g.wrows, ++ts;
+                    }
+                }
+                __syncthreads();
+                // the chunk's steps without barriers: the loads of step k+2 and
+                // the MFMAs of step k+1 are issued before the update of step k
+                // (loads past the chunk re-read its last step, unused)
+                half8 b0[S], b1[S];
+                float4v c0, c1, n0, n1;
+                load_b(b0, wb, offs);
+                mfma_chain(a, b0, c0, c1);
+                load_b(b0, wb + min(1, tc - 1) * win_step, offs);
+                auto update = [&](int32_t k, const float4v &u0, const float4v &u1) {
+                    step_update(u0, u1, g.th, mask, mxb);
+                    if (DIAG && pvalid)
+#pragma unroll
+                        for (int r = 0; r < 2; ++r)
+#pragma unroll
+                            for (int i = 0; i < 4; ++i)
+                            {
+                                const int32_t o = o0 + r * 16 + i;
+                                if (o < g.cout)
+                                    pot_out[((int64_t(itm.x) * I.t + t0 + k) * g.cout + o) * (I.ho * I.wo) +
+                                            oy * I.wo + ox] = r == 0 ? u0[i] : u1[i];
+                            }
+                };
+                for (int32_t k = 0; k < tc; k += 2)
+                {
+                    load_b(b1, wb + min(k + 2, tc - 1) * win_step, offs);
+                    if (k + 1 < tc) mfma_chain(a, b0, n0, n1);
+                    update(k, c0, c1);
+                    load_b(b0, wb + min(k + 3, tc - 1) * win_step, offs);
+                    if (k + 2 < tc) mfma_chain(a, b1, c0, c1);
+                    if (k + 1 < tc) update(k + 1, n0, n1);
+                }
+                if (t0 + tc < I.t) __syncthreads();  // windows free for the next chunk
+            }
+
+            // steps at which some channel fired, per pixel (bit T-1-t for step t)
+            {
+                uint32_t any = 0;
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) any |= mask[r][i];
+                any |= __shfl_xor(any, 16, 64);
+                any |= __shfl_xor(any, 32, 64);
+                if (lane < COLS) any_s[w * COLS + lane] = any;
+            }
+            __syncthreads();
+            // the lane's B operands for pixel (oy, ox) at step st, from the
+            // stage: the bytes the step loop's window held for that step
+            auto gather = [&](half8 (&b)[S], int32_t st) {
+#pragma unroll
+                for (int s = 0; s < S; ++s)
+                {
+                    const int4 te = gtab[4 * s + q];
+                    const int32_t iy = oy + te.x, ix = ox + te.y;
+                    uint2 raw = make_uint2(0, 0);
+                    if (st >= 0 && iy >= 0 && iy < I.h && ix >= 0 && ix < I.w)
+                        raw = *reinterpret_cast<const uint2 *>(
+                            stage + ((st * g.sh + iy - sy0) * g.sw + ix - sx0) * g.cin_pad + te.z);
+                    b[s] = __builtin_bit_cast(half8, widen(raw));
+                }
+            };
+            int32_t ts;
+            {
+                uint32_t any = 0;
+                for (int32_t k = 0; k < g.cg; ++k) any |= any_s[(ct * g.cg + k) * COLS + j];
+                const int32_t cnt = __builtin_popcount(any);
+                ts = I.t - (cnt > 1 ? cnt : 1);
+            }
+            // pass 1: the potentials at inhibit's step; the kept channel is the
+            // first maximum of the thresholded values (inhibit.cpp:64-85)
+            float4v g0, g1;
+            {
+                half8 b[S];
+                gather(b, ts);
+                mfma_chain(a, b, g0, g1);
+                float bv = -1.0f;
+                int32_t bi = INT32_MAX;
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                    {
+                        const float v = r == 0 ? g0[i] : g1[i];
+                        const float xp = v > g.th ? v : 0.0f;
+                        const bool take = xp > bv;
+                        bv = take ? xp : bv;
+                        bi = take ? o0 + r * 16 + i : bi;
+                    }
+                combine(bv, bi, __shfl_xor(bv, 16, 64), __shfl_xor(bi, 16, 64));
+                combine(bv, bi, __shfl_xor(bv, 32, 64), __shfl_xor(bi, 32, 64));
+                if (lane < COLS) red_v[w * COLS + lane] = bv, red_i[w * COLS + lane] = bi;
+            }
+            __syncthreads();
+            int32_t cs;
+            {
+                float m = red_v[ct * g.cg * COLS + j];
+                cs = red_i[ct * g.cg * COLS + j];
+                for (int32_t k = 1; k < g.cg; ++k)
+                    combine(m, cs, red_v[(ct * g.cg + k) * COLS + j], red_i[(ct * g.cg + k) * COLS + j]);
+            }
+            // the lane holding (c*, pixel) publishes c*'s first firing step
+            const bool own = (cs >> 5) == cgw && ((cs >> 2) & 3) == q;
+            uint32_t om = 0;
+            int32_t oxb = 0;
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (((cs >> 4) & 1) == r && (cs & 3) == i) om = mask[r][i], oxb = mxb[r][i];
+            if (own) tsel[ct * COLS + j] = om ? I.t - 32 + int32_t(__builtin_clz(om)) : -1;
+            __syncthreads();
+            // pass 2: the value of c* at its first firing step (convwta's t0 value)
+            float first = 0.0f;
+            if (__builtin_amdgcn_ballot_w64(own && om != 0))
+            {
+                half8 b[S];
+                gather(b, tsel[ct * COLS + j]);
+                mfma_chain(a, b, g0, g1);
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        if (((cs >> 4) & 1) == r && (cs & 3) == i) first = r == 0 ? g0[i] : g1[i];
+            }
+            if (own && pvalid)
+            {
+                const uint32_t fwd = om ? __builtin_bitreverse32(om) >> (32 - I.t) : 0u;
+                summary[I.sum_off + int64_t(oy) * I.wo + ox] =
+                    Summary{cs, fwd, om ? first : 0.0f, om ? __builtin_bit_cast(float, oxb) : 0.0f};
+            }
+        }
+    }
+}
+
+// convwta over one instance's summaries (fused_select's rule, wta.cpp:49-64,
+// 128-150) with the candidates in global memory, for planes of any size.
+// One block per instance; wkey[slot] = channel << 16 | pixel, -1 past the
+// instance's winners.
+__global__ void __launch_bounds__(1024)
+    region_select(const Summary *__restrict__ summary, const RegionInst *__restrict__ inst, int32_t ry, int32_t rx,
+                  int32_t count, float *__restrict__ val, int32_t *__restrict__ chan, spk_winner *__restrict__ win,
+                  int64_t *__restrict__ counts, int32_t *__restrict__ wkey)
+{
+    __shared__ float mm[16];
+    __shared__ ArgMax am[16];
+    const int64_t s = blockIdx.x;
+    const RegionInst I = inst[s];
+    const int32_t hw = I.ho * I.wo;
+    const Summary *sm = summary + I.sum_off;
+    float *v = val + I.sum_off;
+    int32_t *ch = chan + I.sum_off;
+    float m = 0.0f;
+    for (int32_t p = threadIdx.x; p < hw; p += blockDim.x) m = m < sm[p].max ? sm[p].max : m;
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) mm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    m = mm[0];
+    for (int i = 1; i < int(blockDim.x >> 6); ++i) m = m < mm[i] ? mm[i] : m;
+    const float diff = (m - 0.0f) * 2.0f;
+    for (int32_t p = threadIdx.x; p < hw; p += blockDim.x)
+    {
+        const Summary e = sm[p];
+        ch[p] = e.c;
+        v[p] = e.mask ? float(I.t - int32_t(__builtin_ctz(e.mask)) - 1) + (e.first - 0.0f) / diff : -FLT_MAX;
+    }
+    __syncthreads();
+    int32_t k = 0;
+    for (; k < count; ++k)
+    {
+        ArgMax a{-INFINITY, INT32_MAX};
+        for (int32_t p = threadIdx.x; p < hw; p += blockDim.x)
+            if (better(v[p], ch[p] * hw + p, a.v, a.i)) a.v = v[p], a.i = ch[p] * hw + p;
+        a = block_argmax(a, am);
+        if (fabsf(a.v - (-FLT_MAX)) < 1e-5f) break;
+        const int32_t z = a.i / hw, y = (a.i % hw) / I.wo, x = (a.i % hw) % I.wo;
+        if (threadIdx.x == 0)
+            win[s * count + k] = spk_winner{0, 0, z, y, x}, wkey[s * count + k] = z << 16 | (a.i % hw);
+        for (int32_t p = threadIdx.x; p < hw; p += blockDim.x)
+        {
+            const int32_t py = p / I.wo, px = p % I.wo;
+            if (ch[p] == z || (abs(py - y) <= ry && abs(px - x) <= rx)) v[p] = -FLT_MAX;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) counts[s] = k;
+    for (int32_t r = k + int32_t(threadIdx.x); r < count; r += blockDim.x) wkey[s * count + r] = -1;
+}
+
+// Spikes [t][cout][ho][wo] and pooled spikes of every instance from the
+// summaries; one block per (instance, step), steps past an instance's t idle.
+__global__ void __launch_bounds__(256)
+    region_write(const Summary *__restrict__ summary, const RegionInst *__restrict__ inst, int32_t tmax, int32_t cout,
+                 uint8_t *__restrict__ out, uint8_t *__restrict__ pooled, int32_t pkh, int32_t pkw, int32_t psh,
+                 int32_t psw)
+{
+    const int64_t s = blockIdx.x / tmax;
+    const int32_t step = blockIdx.x % tmax;
+    const RegionInst I = inst[s];
+    if (step >= I.t) return;
+    const int32_t hw = I.ho * I.wo;
+    const Summary *sm = summary + I.sum_off;
+    auto act = [&](int32_t p) {
+        const Summary e = sm[p];
+        return (e.mask >> step) & 1u ? e.c : -1;
+    };
+    if (out)
+    {
+        uint8_t *dst = out + I.out_off + int64_t(step) * cout * hw;
+        if (hw % 4 == 0 && (reinterpret_cast<uintptr_t>(dst) & 3) == 0)
+        {
+            // four pixels of one channel plane per 32-bit store
+            const int32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+            for (int32_t qd = lane; qd < hw / 4; qd += 64)
+            {
+                const int32_t a0 = act(4 * qd), a1 = act(4 * qd + 1), a2 = act(4 * qd + 2), a3 = act(4 * qd + 3);
+                for (int32_t c = wave; c < cout; c += waves)
+                    *reinterpret_cast<uint32_t *>(dst + int64_t(c) * hw + 4 * qd) =
+                        uint32_t(a0 == c) | uint32_t(a1 == c) << 8 | uint32_t(a2 == c) << 16 | uint32_t(a3 == c) << 24;
+            }
+        }
+        else
+            for (int32_t p = threadIdx.x; p < hw; p += blockDim.x)
+            {
+                const int32_t a0 = act(p);
+                for (int32_t c = 0; c < cout; ++c) dst[int64_t(c) * hw + p] = a0 == c;
+            }
+    }
+    if (pooled)
+    {
+        const int32_t phw = I.pho * I.pwo;
+        uint8_t *dst = pooled + I.pool_off + int64_t(step) * cout * phw;
+        for (int32_t pos = threadIdx.x; pos < phw; pos += blockDim.x)
+        {
+            const int32_t py = pos / I.pwo, px = pos % I.pwo;
+            int32_t wa[16];
+            const int32_t win = pkh * pkw;
+#pragma unroll
+            for (int k = 0; k < 16; ++k)
+            {
+                const int32_t y = py * psh + k / pkw, x = px * psw + k % pkw;
+                wa[k] = (k < win && y < I.ho && x < I.wo) ? act(y * I.wo + x) : -1;
+            }
+            for (int32_t c = 0; c < cout; ++c)
+            {
+                bool m = false;
+#pragma unroll
+                for (int k = 0; k < 16; ++k) m |= wa[k] == c;
+                dst[int64_t(c) * phw + pos] = m;
+            }
+        }
+    }
+}
+
+// Potentiate/depress bits of every winner slot (fused_records with the
+// instance tables): bit i of slot e is isum <= osum for weight i = (ch, ky,
+// kx) of the winner's kernel (stdp.cpp:88-118).
+__global__ void __launch_bounds__(256)
+    region_records(const uint8_t *__restrict__ fs, const Summary *__restrict__ summary,
+                   const RegionInst *__restrict__ inst, const int32_t *__restrict__ wkey, FusedStdp p, int32_t pt,
+                   int32_t pl, uint32_t *__restrict__ bits)
+{
+    const int64_t e = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (e >= p.slots) return;
+    const int32_t k = wkey[e];
+    if (k < 0) return;
+    const int lane = threadIdx.x & 63;
+    const RegionInst I = inst[e / p.count];
+    const int32_t pos = k & 0xFFFF, y = pos / I.wo - pt, x = pos % I.wo - pl;
+    const uint32_t mask = summary[I.sum_off + pos].mask;
+    const int32_t osum = mask ? int32_t(__builtin_ctz(mask)) : I.t;
+    const int32_t taps = p.kh * p.kw, ksz = p.cin * taps;
+    const uint8_t *fsn = fs + I.fs_off;
+    uint32_t *rec = bits + e * p.words;
+    for (int32_t base = 0; base < ksz; base += 64)
+    {
+        const int32_t i = base + lane;
+        bool bit = false;
+        if (i < ksz)
+        {
+            const int32_t ch = i / taps, iy = y + (i / p.kw) % p.kh, ix = x + i % p.kw;
+            const int32_t isum =
+                (iy >= 0 && iy < I.h && ix >= 0 && ix < I.w) ? fsn[(iy * I.w + ix) * p.cin_pad + ch] : I.t;
+            bit = isum <= osum;
+        }
+        const uint64_t b = __ballot(bit);
+        if (lane == 0 && base / 32 < p.words) rec[base / 32] = uint32_t(b);
+        if (lane == 32 && base / 32 + 1 < p.words) rec[base / 32 + 1] = uint32_t(b >> 32);
+    }
+}
+
+// A candidate launch configuration of the region kernel.
+struct RegionConfig
+{
+    RegionGeom g;
+    size_t lds = 0;
+    int2 *items = nullptr;  // device table [g.items] of (instance, region)
+};
+
+template <int S, bool DIAG>
+void launch_region(const uint8_t *in, const half8 *frag, const RegionInst *inst, const RegionConfig &rc,
+                   Summary *summary, float *pot, uint8_t *fs, int grid, hipStream_t s)
+{
+    static bool configured = false;  // per instantiation; the attribute is per function
+    if (!configured)
+    {
+        SPK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&fused_region<S, DIAG>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, LDS_BYTES));
+        configured = true;
+    }
+    fused_region<S, DIAG><<<dim3(grid), dim3(64 * rc.g.waves), rc.lds, s>>>(in, frag, inst, rc.items, summary, pot,
+                                                                          fs, rc.g);
+    SPK_LAUNCHED();
+}
+
+void dispatch_region(const uint8_t *in, const half8 *frag, const RegionInst *inst, const RegionConfig &rc,
+                     Summary *summary, float *pot, uint8_t *fs, int grid, hipStream_t s)
+{
+    switch (rc.g.steps)
+    {
+#define CASE(k)                                                                        \
+    case k:                                                                            \
+        return pot ? launch_region<k, true>(in, frag, inst, rc, summary, pot, fs, grid, s) \
+                   : launch_region<k, false>(in, frag, inst, rc, summary, pot, fs, grid, s);
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+        CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
+#undef CASE
+    default: SPK_ARG(false, "Fused layer: K-steps out of range.");
+    }
+}
+
+template <typename T>
+struct DeviceBuffer
+{
+    T *p = nullptr;
+    explicit DeviceBuffer(int64_t count = 0) { reset(count); }
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    ~DeviceBuffer()
+    {
+        if (p) (void)hipFree(p);
+    }
+    void reset(int64_t count)
+    {
+        if (p) (void)hipFree(p), p = nullptr;
+        if (count > 0) SPK_HIP(hipMalloc(reinterpret_cast<void **>(&p), size_t(count) * sizeof(T)));
+    }
+};
+}  // namespace
+
+// A fixed-shape layer step: instance table, candidate configurations and
+// every work buffer, made once.
+struct LayerPlan
+{
+    int device = 0;
+    int64_t n = 0;
+    int32_t cin = 0, cout = 0, kh = 0, kw = 0, pt = 0, pl = 0, tmax = 0, count = 0, ry = 0, rx = 0;
+    int32_t pkh = 0, pkw = 0, psh = 0, psw = 0;
+    bool pool = false, tune = false, tuned = false, fast = false;
+    float th = 0;
+    std::vector<RegionInst> host;
+    int64_t in_bytes = 0, out_bytes = 0, pool_bytes = 0, pixels = 0, fs_bytes = 0;
+    std::vector<std::unique_ptr<RegionConfig>> configs;
+    std::vector<std::unique_ptr<DeviceBuffer<int2>>> item_tables;
+    int chosen = 0, grid = 0;
+    DeviceBuffer<RegionInst> inst;
+    DeviceBuffer<half8> frag;
+    DeviceBuffer<Summary> summary;
+    DeviceBuffer<uint8_t> fs;
+    DeviceBuffer<int32_t> wkey, chan;
+    DeviceBuffer<uint32_t> bits;
+    DeviceBuffer<float> val;
+    FusedStdp stdp{};
+
+    void run(const uint8_t *in, float *kernel, const spk_stdp_config *cfg, uint8_t *out, uint8_t *pooled,
+             spk_winner *winners, int64_t *counts, float *pot, hipStream_t s);
+    void choose(const uint8_t *in, hipStream_t s);
+};
+
+namespace
+{
+// Geometry of one candidate: region rows = nct * m.
+bool region_config(const LayerPlan &P, int32_t m, bool fast, RegionGeom &g, size_t &lds)
+{
+    g = RegionGeom{};
+    g.fast = fast;
+    g.cin = P.cin, g.cout = P.cout, g.kh = P.kh, g.kw = P.kw, g.pt = P.pt, g.pl = P.pl;
+    g.cin_pad = int32_t(cdiv(P.cin, 8) * 8);
+    const int32_t planes = g.cin_pad / 8;
+    g.groups = P.kh * P.kw * planes;
+    g.steps = int32_t(cdiv(g.groups, 4));
+    g.cg = int32_t(cdiv(P.cout, 32));
+    g.nct = std::max(1, REGION_WAVES / g.cg);
+    g.waves = g.cg * g.nct;
+    g.rh = g.nct * m;
+    g.sh = g.rh + P.kh - 1;
+    g.sw = int32_t(cdiv(RW + P.kw - 1 + 3, 4) * 4);
+    g.wrows = g.nct + P.kh - 1;
+    g.wcols = RW + P.kw - 1;
+    g.pitch = planes * g.wcols;
+    g.win_units = g.wrows * g.pitch;
+    g.th = P.th;
+    const int64_t stage = cdiv(int64_t(P.tmax) * g.sh * g.sw * g.cin_pad, 16) * 16;
+    const int64_t fixed = stage + region_fixed(g.waves, g.nct, g.steps);
+    const int64_t room = (LDS_BYTES - fixed) / (int64_t(g.win_units) * 16);
+    if (room < 1) return false;
+    // thread roles: a window row's units, and one stage row's dwords (or bytes)
+    if (g.pitch > 64 * g.waves) return false;
+    if (fast && g.sw * g.cin_pad / 4 > 64 * g.waves) return false;
+    const int64_t chunks = cdiv(P.tmax, std::min<int64_t>(room, P.tmax));
+    g.tc = int32_t(cdiv(P.tmax, chunks));
+    g.stage_bytes = int32_t(stage);
+    lds = size_t(fixed + int64_t(g.tc) * g.win_units * 16);
+    return true;
+}
+}  // namespace
+
+void LayerPlan::choose(const uint8_t *in, hipStream_t s)
+{
+    // time each candidate's region kernel on this input and keep the fastest
+    // (the kernel only writes summaries and the first-spike map, which the
+    // run that follows rewrites)
+    std::vector<float> ms(configs.size(), 0.0f);
+    hipEvent_t e0, e1;
+    SPK_HIP(hipEventCreate(&e0));
+    SPK_HIP(hipEventCreate(&e1));
+    for (size_t c = 0; c < configs.size(); ++c)
+        for (int rep = 0; rep < 2; ++rep)  // the first launch configures and warms
+        {
+            SPK_HIP(hipEventRecord(e0, s));
+            dispatch_region(in, frag.p, inst.p, *configs[c], summary.p, nullptr, fs.p, grid, s);
+            SPK_HIP(hipEventRecord(e1, s));
+            SPK_HIP(hipEventSynchronize(e1));
+            SPK_HIP(hipEventElapsedTime(&ms[c], e0, e1));
+        }
+    (void)hipEventDestroy(e0), (void)hipEventDestroy(e1);
+    chosen = int(std::min_element(ms.begin(), ms.end()) - ms.begin());
+    tuned = true;
+}
+
+void LayerPlan::run(const uint8_t *in, float *kernel, const spk_stdp_config *cfg, uint8_t *out, uint8_t *pooled,
+                    spk_winner *winners, int64_t *counts, float *pot, hipStream_t s)
+{
+    const RegionGeom &g0 = configs[0]->g;
+    prep_fragments<<<dim3(unsigned(cdiv(int64_t(g0.cg) * 2 * g0.steps * 64, 256))), 256, 0, s>>>(
+        kernel, frag.p, g0.cout, g0.cin, g0.kh, g0.kw, g0.cin_pad, g0.groups, g0.steps, g0.cg * 2);
+    SPK_LAUNCHED();
+    if (tune && !tuned) choose(in, s);
+    const bool learn = cfg != nullptr;
+    KernelTimer &tm = timer();
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (tm.on)
+    {
+        SPK_HIP(hipEventCreate(&e0));
+        SPK_HIP(hipEventCreate(&e1));
+        SPK_HIP(hipEventRecord(e0, s));
+    }
+    dispatch_region(in, frag.p, inst.p, *configs[chosen], summary.p, pot, learn ? fs.p : nullptr, grid, s);
+    if (tm.on)
+    {
+        SPK_HIP(hipEventRecord(e1, s));
+        tm.events.emplace_back(e0, e1);
+    }
+    region_select<<<dim3(unsigned(n)), 1024, 0, s>>>(summary.p, inst.p, ry, rx, count, val.p, chan.p, winners, counts,
+                                                     wkey.p);
+    SPK_LAUNCHED();
+    if (out || pooled)
+    {
+        region_write<<<dim3(unsigned(n * tmax)), 256, 0, s>>>(summary.p, inst.p, tmax, cout, out,
+                                                              pool ? pooled : nullptr, pkh, pkw, psh, psw);
+        SPK_LAUNCHED();
+    }
+    if (learn)
+    {
+        region_records<<<dim3(unsigned(cdiv(stdp.slots, 4))), 256, 0, s>>>(fs.p, summary.p, inst.p, wkey.p, stdp,
+                                                                          pt, pl, bits.p);
+        SPK_LAUNCHED();
+        fused_stdp<<<dim3(unsigned(cout), unsigned(cdiv(int64_t(cin) * kh * kw, STDP_THREADS))), STDP_THREADS, 0,
+                     s>>>(kernel, bits.p, wkey.p, stdp, *cfg);
+        SPK_LAUNCHED();
+    }
+}
+
+// Makes a plan; shapes: host [n][3] of (t, h, w).
+LayerPlan *make_plan(int64_t n, const int32_t *shapes, int64_t cin, int64_t cout, int64_t kh, int64_t kw, int64_t pt,
+                     int64_t pl, int64_t pb, int64_t pr, double threshold, int64_t ry, int64_t rx, int64_t count,
+                     int64_t pkh, int64_t pkw, int64_t psh, int64_t psw, bool tune, const uint8_t *in_ptr_hint)
+{
+    SPK_ARG(n >= 1 && n < (1LL << 31) && shapes, "Fused layer: no instances.");
+    SPK_ARG(cin >= 1 && cout >= 2 && kh >= 1 && kw >= 1, "Fused layer: invalid shapes.");
+    SPK_ARG(pt >= 0 && pl >= 0 && pb >= 0 && pr >= 0, "Pad items are incorrect.");
+    SPK_ARG(cout <= 256, "Fused layer: at most 256 output channels.");
+    SPK_ARG(threshold >= 0.0, "Fused layer: the conv threshold must be >= 0.");
+    SPK_ARG(ry >= 0 && rx >= 0 && count >= 1, "Fused layer: invalid convwta parameters.");
+    SPK_ARG(kh + RW + kw <= 64, "Fused layer: kernel too large for the region path.");
+    auto P = std::make_unique<LayerPlan>();
+    SPK_HIP(hipGetDevice(&P->device));
+    P->n = n, P->cin = int32_t(cin), P->cout = int32_t(cout), P->kh = int32_t(kh), P->kw = int32_t(kw);
+    P->pt = int32_t(pt), P->pl = int32_t(pl), P->count = int32_t(count), P->ry = int32_t(ry), P->rx = int32_t(rx);
+    P->th = float(threshold), P->tune = tune;
+    P->pool = pkh > 0;
+    if (P->pool)
+    {
+        SPK_ARG(pkh >= 1 && pkw >= 1 && psh >= 1 && psw >= 1, "Pool: kernel and stride must be positive.");
+        SPK_ARG(pkh * pkw <= 16, "Fused layer: pool windows of at most 16 pixels.");
+        P->pkh = int32_t(pkh), P->pkw = int32_t(pkw), P->psh = int32_t(psh), P->psw = int32_t(psw);
+    }
+    const int64_t cin_pad = cdiv(cin, 8) * 8;
+    P->host.resize(size_t(n));
+    int64_t in_b = 0, out_b = 0, pool_b = 0, pix = 0, fs_b = 0;
+    for (int64_t i = 0; i < n; ++i)
+    {
+        const int64_t t = shapes[3 * i], h = shapes[3 * i + 1], w = shapes[3 * i + 2];
+        SPK_ARG(t >= 1 && t <= 32, "Fused layer: every instance needs 1 to 32 time steps.");
+        SPK_ARG(h >= 1 && w >= 1, "Fused layer: invalid instance shape.");
+        const int64_t ho = h + pt + pb - kh + 1, wo = w + pl + pr - kw + 1;
+        SPK_ARG(ho >= 1 && wo >= 1, "Input and kernel shapes are not compatible.");
+        SPK_ARG(ho * wo <= 65536, "Fused layer: at most 65536 output pixels per instance.");
+        int64_t pho = 0, pwo = 0;
+        if (P->pool)
+        {
+            SPK_ARG(ho >= pkh && wo >= pkw, "Pool: input and kernel shapes are not compatible.");
+            pho = (ho - pkh) / psh + 1, pwo = (wo - pkw) / psw + 1;
+        }
+        RegionInst &I = P->host[size_t(i)];
+        I.in_off = in_b, I.out_off = out_b, I.pool_off = pool_b, I.sum_off = pix, I.fs_off = fs_b;
+        I.t = int32_t(t), I.h = int32_t(h), I.w = int32_t(w), I.ho = int32_t(ho), I.wo = int32_t(wo);
+        I.pho = int32_t(pho), I.pwo = int32_t(pwo), I.cols = int32_t(cdiv(wo, RW));
+        in_b += t * cin * h * w, out_b += t * cout * ho * wo, pool_b += t * cout * pho * pwo;
+        pix += ho * wo, fs_b += h * w * cin_pad;
+        P->tmax = std::max(P->tmax, int32_t(t));
+    }
+    P->in_bytes = in_b, P->out_bytes = out_b, P->pool_bytes = pool_b, P->pixels = pix, P->fs_bytes = fs_b;
+    // dword staging: every row a whole number of aligned dwords (then every
+    // instance's packed input starts 4-aligned too)
+    bool fast = true;
+    for (auto &I : P->host) fast = fast && I.w % 4 == 0;
+    fast = fast && (reinterpret_cast<uintptr_t>(in_ptr_hint) & 3) == 0;
+    P->fast = fast;
+
+    // candidates: region rows nct * m, m = 1, 2, 4, 8 where the LDS holds them
+    for (int32_t m : {1, 2, 4, 8})
+    {
+        RegionGeom g;
+        size_t lds = 0;
+        if (!region_config(*P, m, fast, g, lds)) continue;
+        std::vector<int2> items;
+        for (int64_t i = 0; i < n; ++i)
+        {
+            const RegionInst &I = P->host[size_t(i)];
+            const int64_t regions = cdiv(I.ho, g.rh) * I.cols;
+            for (int64_t r = 0; r < regions; ++r) items.push_back(make_int2(int32_t(i), int32_t(r)));
+        }
+        SPK_ARG(items.size() < (1ull << 31), "Fused layer: too many regions.");
+        g.items = int64_t(items.size());
+        auto tab = std::make_unique<DeviceBuffer<int2>>(g.items);
+        SPK_HIP(hipMemcpy(tab->p, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
+        auto rc = std::make_unique<RegionConfig>();
+        rc->g = g, rc->lds = lds, rc->items = tab->p;
+        P->item_tables.push_back(std::move(tab));
+        P->configs.push_back(std::move(rc));
+        if (!tune) break;  // without tuning: the smallest region that fits
+    }
+    SPK_ARG(!P->configs.empty(), "Fused layer: the region stage does not fit the 160 KiB LDS (T * Cin too large).");
+    const RegionGeom &g = P->configs[0]->g;
+    SPK_ARG(g.steps <= MAX_STEPS, "Fused layer: Cin*Kh*Kw too large (K <= 512 after padding).");
+    int cus = 0;
+    SPK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, P->device));
+    int64_t min_items = g.items;
+    for (auto &c : P->configs) min_items = std::min(min_items, c->g.items);
+    P->grid = int(std::min<int64_t>(min_items, cus));
+
+    P->inst.reset(n);
+    SPK_HIP(hipMemcpy(P->inst.p, P->host.data(), size_t(n) * sizeof(RegionInst), hipMemcpyHostToDevice));
+    P->frag.reset(int64_t(g.cg) * 2 * g.steps * 64);
+    P->summary.reset(pix);
+    P->fs.reset(fs_b);
+    P->wkey.reset(n * count);
+    P->chan.reset(pix);
+    P->val.reset(pix);
+    const int64_t ksz = cin * kh * kw;
+    P->stdp = FusedStdp{n * count, int32_t(count), int32_t(cin), int32_t(cin_pad), 0, 0, int32_t(kh), int32_t(kw),
+                        0, 0, 0, 0, 0, int32_t(cdiv(ksz, 32))};
+    SPK_ARG(n * count < (1LL << 31), "Fused layer: too many winner slots.");
+    P->bits.reset(n * count * P->stdp.words);
+    return P.release();
+}
+
+// Uniform batches whose sample does not fit the LDS whole (spk_stdp_layer_fused).
+void region_layer_step(const uint8_t *in, int64_t n, int64_t t, int64_t cin, int64_t h, int64_t w, float *kernel,
+                       int64_t cout, int64_t kh, int64_t kw, int64_t pt, int64_t pl, int64_t pb, int64_t pr,
+                       double threshold, int64_t ry, int64_t rx, int64_t count, const spk_stdp_config *cfg,
+                       uint8_t *out, uint8_t *pooled, int64_t pkh, int64_t pkw, int64_t psh, int64_t psw,
+                       spk_winner *winners, int64_t *counts, float *pot, hipStream_t s)
+{
+    std::vector<int32_t> shapes(size_t(3 * n));
+    for (int64_t i = 0; i < n; ++i)
+        shapes[size_t(3 * i)] = int32_t(t), shapes[size_t(3 * i + 1)] = int32_t(h), shapes[size_t(3 * i + 2)] = int32_t(w);
+    std::unique_ptr<LayerPlan> P(make_plan(n, shapes.data(), cin, cout, kh, kw, pt, pl, pb, pr, threshold, ry, rx,
+                                           count, pooled ? pkh : 0, pkw, psh, psw, false, in));
+    SPK_ARG(P->in_bytes == n * t * cin * h * w, "Fused layer: input is not packed contiguously.");
+    P->run(in, kernel, cfg, out, pooled, winners, counts, pot, s);
+    SPK_HIP(hipStreamSynchronize(s));  // before the plan's buffers are freed
+}
+}  // namespace spk
+
+using namespace spk;
+
+extern "C" {
+
+int32_t spk_layer_plan_create(spk_layer_plan **plan, int64_t n, const int32_t *shapes, int64_t cin, int64_t cout,
+                              int64_t kh, int64_t kw, int64_t pt, int64_t pl, int64_t pb, int64_t pr,
+                              double threshold, int64_t ry, int64_t rx, int64_t count, int64_t pkh, int64_t pkw,
+                              int64_t psh, int64_t psw, int32_t tune)
+{
+    return guard([&] {
+        SPK_ARG(plan, "Plan pointer is null.");
+        *plan = nullptr;
+        LayerPlan *P = make_plan(n, shapes, cin, cout, kh, kw, pt, pl, pb, pr, threshold, ry, rx, count, pkh, pkw,
+                                 psh, psw, tune != 0, nullptr);
+        *plan = reinterpret_cast<spk_layer_plan *>(P);
+    });
+}
+
+int32_t spk_layer_plan_layout(const spk_layer_plan *handle, int64_t *in_off, int64_t *out_off, int64_t *pool_off,
+                              int64_t *totals)
+{
+    return guard([&] {
+        SPK_ARG(handle, "Plan is null.");
+        const LayerPlan *plan = reinterpret_cast<const LayerPlan *>(handle);
+        for (int64_t i = 0; i < plan->n; ++i)
+        {
+            const RegionInst &I = plan->host[size_t(i)];
+            if (in_off) in_off[i] = I.in_off;
+            if (out_off) out_off[i] = I.out_off;
+            if (pool_off) pool_off[i] = I.pool_off;
+        }
+        if (totals) totals[0] = plan->in_bytes, totals[1] = plan->out_bytes, totals[2] = plan->pool_bytes;
+    });
+}
+
+int32_t spk_layer_plan_run(spk_layer_plan *handle, const void *in, void *kernel, int32_t kernel_type,
+                           const spk_stdp_config *configs, int64_t nconfigs, void *out, void *pooled,
+                           spk_winner *winners, int64_t *counts, void *stream)
+{
+    return guard([&] {
+        SPK_ARG(handle, "Plan is null.");
+        LayerPlan *plan = reinterpret_cast<LayerPlan *>(handle);
+        SPK_REQUIRE(kernel_type == SPK_F32, SPK_ERR_UNSUPPORTED,
+                    "Fused layer: kernel data type " << type_name(kernel_type) << " is not supported.");
+        SPK_ARG(in && kernel && winners && counts, "Fused layer: input, kernel and winner buffers are required.");
+        SPK_ARG(!plan->fast || (reinterpret_cast<uintptr_t>(in) & 3) == 0,
+                "Fused layer: the input must be 4-byte aligned.");
+        SPK_ARG(!pooled || plan->pool, "Fused layer: the plan was made without pooling.");
+        int dev = 0;
+        SPK_HIP(hipGetDevice(&dev));
+        SPK_ARG(dev == plan->device, "Fused layer: the plan belongs to another device.");
+        const StdpConfigs cfg = nconfigs > 0 ? pack_configs(configs, nconfigs) : StdpConfigs{};
+        plan->run((const uint8_t *)in, (float *)kernel, nconfigs > 0 ? &cfg.c[0] : nullptr, (uint8_t *)out,
+                  (uint8_t *)pooled, winners, counts, nullptr, S(stream));
+    });
+}
+
+int32_t spk_layer_plan_destroy(spk_layer_plan *handle)
+{
+    return guard([&] {
+        if (!handle) return;
+        SPK_HIP(hipDeviceSynchronize());
+        delete reinterpret_cast<LayerPlan *>(handle);
+    });
+}
+}
