@@ -230,6 +230,157 @@ def sparse_cpu_baseline(ev, k, samples):
                       % (n, SP_BINS, what)}
 
 
+def guarded(fn):
+    """Runs one extra workload; a failure is reported in its place, never lost."""
+    import torch
+    try:
+        return fn()
+    except Exception as e:  # the headline line must still print
+        return {"error": "%s: %s" % (type(e).__name__, e)}
+    finally:
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+
+
+# Precision variant: 512 bar images 128x128, Gabor 4 orientations x {7x7 lambda 4, 11x11 lambda 8},
+# gamma 0.5 -> 8 channels, rank code T=30, conv 8->64 7x7 pad 3, threshold 20, inhibit,
+# convwta r5 k10, pool 2x2, STDP to the FP32 masters.
+C3_N, C3_T, C3_S, C3_COUT, C3_K, C3_TH = 512, 30, 128, 64, 7, 20.0
+
+
+def c3_inputs(spk, dev, n=C3_N, seed=3):
+    import torch
+    from spyker_amd import synthetic
+    img = torch.from_numpy(synthetic.bars(n, C3_S, seed=seed)).to(dev).reshape(n, 1, C3_S, C3_S)
+    banks = []
+    for size, lam in ((3, 4.0), (5, 8.0)):
+        filt = [spk.GaborFilter(0.56 * lam, th, 0.5, lam, 0.0) for th in (0.0, 45.0, 90.0, 135.0)]
+        banks.append(spk.Gabor(size, filt, pad=size, device=dev)(img))
+    g = torch.cat(banks, dim=1).contiguous()  # [n, 8, 128, 128]
+    spk.threshold(g, 0.0)
+    x = spk.code(g, C3_T)  # [n, 30, 8, 128, 128]
+    rng = np.random.Generator(np.random.PCG64(seed))
+    k = np.clip(rng.normal(0.8, 0.05, (C3_COUT, 8, C3_K, C3_K)), 0, 1).astype(np.float32)
+    return x, k
+
+
+def c3_bench(spk, dev, steps):
+    import torch
+    x, k = c3_inputs(spk, dev)
+    n = x.shape[0]
+    conv = spk.Conv(8, C3_COUT, C3_K, pad=3, device=dev)
+    conv.kernel = torch.from_numpy(k).to(dev)
+    conv.stdpconfig = [spk.STDPConfig(0.004, -0.003)]
+    plan = spk.LayerPlan(conv, [(C3_T, C3_S, C3_S)] * n, C3_TH, 5, 10, 2, 2, tune=True)
+    packed = x.reshape(-1)
+    bufs = plan.buffers()
+    plan.run(packed, buffers=bufs)  # tunes the region height
+    spk.call("spk_kernel_timer", 1)
+    dt = timed(lambda: plan.run(packed, buffers=bufs), steps, 1)
+    total_ms, launches = C.c_double(0), C.c_int64(0)
+    spk.call("spk_kernel_timer_read", C.byref(total_ms), C.byref(launches))
+    spk.call("spk_kernel_timer", 0)
+    kern_s = total_ms.value / 1e3 / max(launches.value, 1)
+    flop = 2.0 * C3_COUT * C3_S * C3_S * 8 * C3_K * C3_K * n * C3_T
+    res = {"workload": "512 bar images 128x128, Gabor 4x2 -> 8 ch, T=30, conv 8->64 7x7 pad 3, threshold 20, "
+                       "inhibit, convwta r5 k10, pool 2x2, STDP",
+           "value": n * C3_T / dt, "unit": "images*timesteps/s", "ms_per_step": dt * 1e3, "steps": steps,
+           "dtype": "f16-mfma/f32-acc",
+           "roofline": {"bound": "mfma", "achieved": flop / kern_s / 1e12, "peak": PEAK_FP16 / 1e12,
+                        "unit": "TFLOP/s", "frac": flop / kern_s / PEAK_FP16, "traffic": None,
+                        "kernel": "fused_region", "kernel_ms": kern_s * 1e3}}
+    # precision against the FP64 conv on a few samples
+    m = 8
+    xs = x[:m].contiguous()
+    p64 = torch.empty((m, C3_T, C3_COUT, C3_S, C3_S), dtype=torch.float64, device=dev)
+    kd = torch.from_numpy(k).to(dev)
+    spk.call("spk_conv", spk._ptr(xs), 4, spk._ptr(kd), 9, spk._ptr(p64), 10, m * C3_T, 8, C3_S, C3_S, C3_COUT,
+             C3_K, C3_K, 1, 1, 3, 3, 3, 3, spk._stream(xs))
+    cp = spk.Conv(8, C3_COUT, C3_K, pad=3, device=dev)
+    cp.kernel = kd.clone()
+    p16 = torch.empty((m, C3_T, C3_COUT, C3_S, C3_S), dtype=torch.float32, device=dev)
+    spk.layer_step(cp, xs, C3_TH, 5, 10, learn=False, potentials=p16)
+    res["precision"] = {"samples": m,
+                        "threshold_flip_rate_vs_fp64": ((p16.double() > C3_TH) != (p64 > C3_TH)).double().mean().item(),
+                        "max_abs_potential_error_vs_fp64": (p16.double() - p64).abs().max().item()}
+    return res
+
+
+# Batching variant: 4096 stroke images of uneven size S in {32..256} and T in 8..32, DoG 4 ch,
+# conv 4->32 5x5 pad 2, threshold 12, inhibit, convwta r2 k4, pool 2x2, STDP; one plan,
+# one captured graph per step.
+C4_N, C4_SIZES, C4_COUT, C4_TH = 4096, (32, 48, 64, 96, 128, 192, 256), 32, 12.0
+
+
+def c4_inputs(spk, dev, n=C4_N, seed=4):
+    import torch
+    from spyker_amd import synthetic
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = rng.choice(C4_SIZES, n)
+    times = rng.integers(8, 33, n)
+    shapes = [(int(t), int(s), int(s)) for t, s in zip(times, sizes)]
+    d3 = spk.DoG(1, [spk.DoGFilter(1, 2), spk.DoGFilter(2, 1)], pad=1, device=dev)
+    d7 = spk.DoG(3, [spk.DoGFilter(1, 2), spk.DoGFilter(2, 1)], pad=3, device=dev)
+    coded = [None] * n
+    for s in C4_SIZES:
+        idx = np.nonzero(sizes == s)[0]
+        if len(idx) == 0:
+            continue
+        img = torch.from_numpy(synthetic.strokes(len(idx), int(s), seed=seed + int(s)).astype(np.float32) / 255)
+        img = img.to(dev).reshape(len(idx), 1, int(s), int(s))
+        f = torch.cat([d3(img), d7(img)], dim=1).contiguous()
+        spk.threshold(f, 0.01)
+        for t in np.unique(times[idx]):
+            sel = idx[times[idx] == t]
+            pos = np.searchsorted(idx, sel)
+            x = spk.code(f[torch.from_numpy(pos).to(dev)].contiguous(), int(t))
+            for j, i in enumerate(sel):
+                coded[i] = x[j]
+    rng2 = np.random.Generator(np.random.PCG64(seed))
+    k = np.clip(rng2.normal(0.8, 0.05, (C4_COUT, 4, 5, 5)), 0, 1).astype(np.float32)
+    return shapes, coded, k
+
+
+def c4_bench(spk, dev, steps):
+    import torch
+    shapes, coded, k = c4_inputs(spk, dev)
+    conv = spk.Conv(4, C4_COUT, 5, pad=2, device=dev)
+    conv.kernel = torch.from_numpy(k).to(dev)
+    conv.stdpconfig = [spk.STDPConfig(0.004, -0.003)]
+    plan = spk.LayerPlan(conv, shapes, C4_TH, 2, 4, 2, 2, tune=True)
+    packed = plan.pack(coded)
+    del coded
+    bufs = plan.buffers()
+    plan.run(packed, buffers=bufs)  # tunes the region height, configures the kernels
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        plan.run(packed, buffers=bufs)
+    torch.cuda.current_stream().wait_stream(side)
+    with torch.cuda.graph(g):
+        plan.run(packed, buffers=bufs)
+    dt = timed(g.replay, steps, 1)
+    spk.call("spk_kernel_timer", 1)
+    plan.run(packed, buffers=bufs)
+    torch.cuda.synchronize()
+    total_ms, launches = C.c_double(0), C.c_int64(0)
+    spk.call("spk_kernel_timer_read", C.byref(total_ms), C.byref(launches))
+    spk.call("spk_kernel_timer", 0)
+    kern_s = total_ms.value / 1e3 / max(launches.value, 1)
+    its = sum(t for t, _, _ in shapes)
+    flop = sum(2.0 * C4_COUT * h * w * 4 * 25 * t for t, h, w in shapes)
+    return {"workload": "4096 stroke images, S in {32..256}, T in 8..32, DoG 4 ch, conv 4->32 5x5 pad 2, "
+                        "threshold 12, inhibit, convwta r2 k4, pool 2x2, STDP; one plan, graph-replayed step",
+            "value": its / dt, "unit": "images*timesteps/s", "ms_per_step": dt * 1e3, "steps": steps,
+            "images_timesteps": its, "dtype": "f16-mfma/f32-acc",
+            "bytes_in_out_pooled": [plan.in_bytes, plan.out_bytes, plan.pool_bytes],
+            "roofline": {"bound": "mfma", "achieved": flop / kern_s / 1e12, "peak": PEAK_FP16 / 1e12,
+                         "unit": "TFLOP/s", "frac": flop / kern_s / PEAK_FP16, "traffic": None,
+                         "kernel": "fused_region", "kernel_ms": kern_s * 1e3},
+            "step_hbm_bytes_alg": plan.in_bytes + plan.out_bytes + plan.pool_bytes}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -241,6 +392,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-sparse", action="store_true")
     ap.add_argument("--sparse-only", action="store_true", help="time only the sparse event-stream workload")
+    ap.add_argument("--only", choices=["c3", "c4", "sparse"], help="time only this extra workload")
+    ap.add_argument("--no-extra", action="store_true", help="skip the precision and batching workloads")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (HBM counter calibration under rocprofv3)")
     a = ap.parse_args()
@@ -263,7 +416,11 @@ def main():
 
     dev = torch.device("cuda", 0)
     if a.sparse_only:
-        print(json.dumps({"sparse": sparse_bench(spk, dev, a.steps, a.warmup, not a.no_cpu)}))
+        a.only = "sparse"
+    if a.only:
+        fns = {"c3": lambda: c3_bench(spk, dev, a.steps), "c4": lambda: c4_bench(spk, dev, a.steps),
+               "sparse": lambda: sparse_bench(spk, dev, a.steps, a.warmup, not a.no_cpu)}
+        print(json.dumps({a.only: fns[a.only]()}))
         return
     x, k2 = make_inputs(dev)
 
@@ -313,10 +470,13 @@ def main():
     line["precision"] = precision(spk, x, k2, dev)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(x.cpu().numpy(), k2, a.cpu_samples)
+    del x
+    torch.cuda.empty_cache()
     if not a.no_sparse:
-        del x
-        torch.cuda.empty_cache()
-        line["sparse"] = sparse_bench(spk, dev, 5, 1, not a.no_cpu)
+        line["sparse"] = guarded(lambda: sparse_bench(spk, dev, 5, 1, not a.no_cpu))
+    if not a.no_extra:
+        line["precision_variant"] = guarded(lambda: c3_bench(spk, dev, 3))
+        line["batching_variant"] = guarded(lambda: c4_bench(spk, dev, 3))
     print(json.dumps(line))
 
 

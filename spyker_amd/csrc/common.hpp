@@ -71,6 +71,9 @@ inline hipStream_t S(void *stream) { return reinterpret_cast<hipStream_t>(stream
 
 const char *type_name(int32_t type);
 
+// Frees the layer plans the uniform fused path keeps per shape (region.hip).
+void clear_plan_cache();
+
 // --------------------------------------------------------------- launch --
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }

@@ -30,7 +30,9 @@
 // (hipGraph) of a run replays it.  Region height and chunking are chosen by
 // timing the candidates on the first run when the plan was made with tuning.
 #include <algorithm>
+#include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "fused.hpp"
@@ -849,6 +851,37 @@ LayerPlan *make_plan(int64_t n, const int32_t *shapes, int64_t cin, int64_t cout
     return P.release();
 }
 
+namespace
+{
+// The plans of the uniform path, kept per (shape, stream, device) so repeated
+// calls allocate nothing; a stream's calls are ordered, so its plan's
+// buffers are never in use by two calls at once.
+struct PlanKey
+{
+    std::vector<int64_t> v;
+    bool operator==(const PlanKey &o) const { return v == o.v; }
+};
+struct PlanCache
+{
+    std::mutex mu;
+    std::vector<std::pair<PlanKey, std::unique_ptr<LayerPlan>>> plans;  // most recent last
+};
+PlanCache &plan_cache()
+{
+    static PlanCache *c = new PlanCache;
+    return *c;
+}
+constexpr size_t PLAN_CACHE = 4;
+}  // namespace
+
+void clear_plan_cache()
+{
+    PlanCache &c = plan_cache();
+    std::lock_guard<std::mutex> lock(c.mu);
+    if (!c.plans.empty()) SPK_HIP(hipDeviceSynchronize());
+    c.plans.clear();
+}
+
 // Uniform batches whose sample does not fit the LDS whole (spk_stdp_layer_fused).
 void region_layer_step(const uint8_t *in, int64_t n, int64_t t, int64_t cin, int64_t h, int64_t w, float *kernel,
                        int64_t cout, int64_t kh, int64_t kw, int64_t pt, int64_t pl, int64_t pb, int64_t pr,
@@ -856,14 +889,41 @@ void region_layer_step(const uint8_t *in, int64_t n, int64_t t, int64_t cin, int
                        uint8_t *out, uint8_t *pooled, int64_t pkh, int64_t pkw, int64_t psh, int64_t psw,
                        spk_winner *winners, int64_t *counts, float *pot, hipStream_t s)
 {
-    std::vector<int32_t> shapes(size_t(3 * n));
-    for (int64_t i = 0; i < n; ++i)
-        shapes[size_t(3 * i)] = int32_t(t), shapes[size_t(3 * i + 1)] = int32_t(h), shapes[size_t(3 * i + 2)] = int32_t(w);
-    std::unique_ptr<LayerPlan> P(make_plan(n, shapes.data(), cin, cout, kh, kw, pt, pl, pb, pr, threshold, ry, rx,
-                                           count, pooled ? pkh : 0, pkw, psh, psw, false, in));
-    SPK_ARG(P->in_bytes == n * t * cin * h * w, "Fused layer: input is not packed contiguously.");
+    int dev = 0;
+    SPK_HIP(hipGetDevice(&dev));
+    int64_t thbits = 0;
+    std::memcpy(&thbits, &threshold, sizeof(thbits));
+    const int64_t fast = (reinterpret_cast<uintptr_t>(in) & 3) == 0;
+    PlanKey key{{n, t, cin, h, w, cout, kh, kw, pt, pl, pb, pr, thbits, ry, rx, count, pooled ? pkh : 0, pkw, psh,
+                 psw, fast, int64_t(reinterpret_cast<uintptr_t>(s)), dev}};
+    PlanCache &c = plan_cache();
+    std::lock_guard<std::mutex> lock(c.mu);
+    LayerPlan *P = nullptr;
+    for (size_t i = 0; i < c.plans.size(); ++i)
+        if (c.plans[i].first == key)
+        {
+            std::rotate(c.plans.begin() + long(i), c.plans.begin() + long(i) + 1, c.plans.end());
+            P = c.plans.back().second.get();
+            break;
+        }
+    if (!P)
+    {
+        std::vector<int32_t> shapes(size_t(3 * n));
+        for (int64_t i = 0; i < n; ++i)
+            shapes[size_t(3 * i)] = int32_t(t), shapes[size_t(3 * i + 1)] = int32_t(h),
+                               shapes[size_t(3 * i + 2)] = int32_t(w);
+        std::unique_ptr<LayerPlan> made(make_plan(n, shapes.data(), cin, cout, kh, kw, pt, pl, pb, pr, threshold, ry,
+                                                  rx, count, pooled ? pkh : 0, pkw, psh, psw, false, in));
+        SPK_ARG(made->in_bytes == n * t * cin * h * w, "Fused layer: input is not packed contiguously.");
+        if (c.plans.size() == PLAN_CACHE)
+        {
+            SPK_HIP(hipDeviceSynchronize());  // the oldest plan's buffers may still be in use
+            c.plans.erase(c.plans.begin());
+        }
+        c.plans.emplace_back(std::move(key), std::move(made));
+        P = c.plans.back().second.get();
+    }
     P->run(in, kernel, cfg, out, pooled, winners, counts, pot, s);
-    SPK_HIP(hipStreamSynchronize(s));  // before the plan's buffers are freed
 }
 }  // namespace spk
 

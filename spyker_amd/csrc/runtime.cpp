@@ -162,6 +162,7 @@ int32_t spk_cache_clear(void)
             c.cached_bytes -= kv.first;
         }
         c.free.clear();
+        clear_plan_cache();
     });
 }
 }
