@@ -64,9 +64,36 @@ struct RegionGeom
     int32_t wrows, wcols, pitch, win_units, tc;
     int32_t stage_bytes;         // tmax * sh * sw * cin_pad, 16-aligned
     int32_t fast;                // dword staging (every w % 4 == 0, 4-aligned inputs)
+    int32_t pair;                // cin <= 4: K groups of two taps x 4 channels, 8-byte window units
     int64_t items;
     float th;
 };
+
+// cin <= 4: K groups of two taps x 4 channels when that needs at most 8 K-steps.
+inline bool use_pair(int64_t cin, int64_t kh, int64_t kw) { return cin <= 4 && cdiv(kh * ((kw + 1) / 2), 4) <= 8; }
+
+// Weight fragments of the paired K groups: lane l of K step s holds output
+// channel rt*16 + l%16 and group 4s + l/16 = (ky, p); half j is channel j%4
+// of tap (ky, 2p + j/4), 0 past the kernel.
+__global__ void prep_fragments_pair(const float *__restrict__ k, half8 *__restrict__ frag, int32_t cout, int32_t cin,
+                                    int32_t kh, int32_t kw, int32_t groups, int32_t steps, int32_t row_tiles)
+{
+    const int32_t total = row_tiles * steps * 64, np = (kw + 1) / 2;
+    for (int32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x)
+    {
+        const int32_t lane = e % 64, s = (e / 64) % steps, rt = e / (64 * steps);
+        const int32_t o = rt * 16 + lane % 16, grp = 4 * s + lane / 16, ky = grp / np, p = grp % np;
+        half8 v;
+        for (int j = 0; j < 8; ++j)
+        {
+            const int32_t kx = 2 * p + j / 4, c = j % 4;
+            float x = 0.0f;
+            if (o < cout && grp < groups && c < cin && kx < kw) x = k[((int64_t(o) * cin + c) * kh + ky) * kw + kx];
+            v[j] = _Float16(x);
+        }
+        frag[e] = v;
+    }
+}
 
 // LDS bytes past the stage and windows: any/argmax partials, step table, tap table.
 inline int64_t region_fixed(int32_t waves, int32_t nct, int32_t steps)
@@ -74,7 +101,7 @@ inline int64_t region_fixed(int32_t waves, int32_t nct, int32_t steps)
     return int64_t(waves) * COLS * 12 + int64_t(nct) * COLS * 4 + int64_t(steps) * 4 * 16;
 }
 
-template <int S, bool DIAG>
+template <int S, bool DIAG, bool PAIR>
 __global__ void __launch_bounds__(64 * REGION_WAVES)
     fused_region(const uint8_t *__restrict__ in, const half8 *__restrict__ frag, const RegionInst *__restrict__ inst,
                  const int2 *__restrict__ items, Summary *__restrict__ summary, float *__restrict__ pot_out,
@@ -92,7 +119,11 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, q = lane >> 4, j = lane & 15;
     const int32_t cgw = w % g.cg, ct = w / g.cg;
     const int32_t o0 = cgw * 32 + 4 * q;  // the lane's first channel; rows r*16 + i follow
-    const int32_t planes = g.cin_pad / 8;
+    // K groups: 8 input channels at one tap, or (PAIR, cin <= 4) the 4
+    // channels of taps (ky, 2p) and (ky, 2p + 1), whose window units (8 bytes
+    // a pixel) are adjacent
+    const int32_t planes = PAIR ? 1 : g.cin_pad / 8, np = (g.kw + 1) / 2;
+    constexpr int32_t UB = PAIR ? 8 : 16;  // bytes of one window unit
 
     // weight fragments and window offsets, resident for the whole kernel;
     // groups past the real K (zero weights) reuse group 0's operands
@@ -103,13 +134,25 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
     {
         a[0][s] = frag[((cgw * 2 + 0) * S + s) * 64 + lane];
         a[1][s] = frag[((cgw * 2 + 1) * S + s) * 64 + lane];
-        const int32_t grp = 4 * s + q, gg = grp < g.groups ? grp : 0, tap = gg / planes, c8 = gg % planes;
-        offs[s] = ((tap / g.kw) * g.pitch + c8 * g.wcols + tap % g.kw) * 16;
+        const int32_t grp = 4 * s + q, gg = grp < g.groups ? grp : 0;
+        if (PAIR)
+            offs[s] = ((gg / np) * g.pitch + 2 * (gg % np)) * UB;
+        else
+        {
+            const int32_t tap = gg / planes, c8 = gg % planes;
+            offs[s] = ((tap / g.kw) * g.pitch + c8 * g.wcols + tap % g.kw) * UB;
+        }
     }
     for (int32_t e = threadIdx.x; e < 4 * S; e += blockDim.x)
     {
-        const int32_t gg = e < g.groups ? e : 0, tap = gg / planes, c8 = gg % planes;
-        gtab[e] = make_int4(tap / g.kw - g.pt, tap % g.kw - g.pl, c8 * 8, 0);
+        const int32_t gg = e < g.groups ? e : 0;
+        if (PAIR)
+            gtab[e] = make_int4(gg / np - g.pt, 2 * (gg % np) - g.pl, 0, 0);
+        else
+        {
+            const int32_t tap = gg / planes, c8 = gg % planes;
+            gtab[e] = make_int4(tap / g.kw - g.pt, tap % g.kw - g.pl, c8 * 8, 0);
+        }
     }
     const int32_t win_step = g.win_units * 16;
 
@@ -215,15 +258,24 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
             {
                 const int32_t c8 = u % planes, px = u / planes, iy = iy_lo + px / ow, ix = ix_lo + px % ow;
                 const uint8_t *sp = stage + ((iy - sy0) * g.sw + ix - sx0) * g.cin_pad + c8 * 8;
+                uint8_t *fp = fs_out + I.fs_off + (int64_t(iy) * I.w + ix) * g.cin_pad + c8 * 8;
                 uint32_t lo = tb, hi = tb, seen_lo = 0, seen_hi = 0;
-                for (int32_t tt = 0; tt < I.t; ++tt)
+                if (PAIR)
                 {
-                    const uint2 v = *reinterpret_cast<const uint2 *>(sp + tt * g.sh * g.sw * g.cin_pad);
-                    first_nonzero(lo, seen_lo, v.x, tt);
-                    first_nonzero(hi, seen_hi, v.y, tt);
+                    for (int32_t tt = 0; tt < I.t; ++tt)
+                        first_nonzero(lo, seen_lo, *reinterpret_cast<const uint32_t *>(sp + tt * g.sh * g.sw * 4), tt);
+                    *reinterpret_cast<uint32_t *>(fp) = lo;
                 }
-                *reinterpret_cast<uint2 *>(fs_out + I.fs_off + (int64_t(iy) * I.w + ix) * g.cin_pad + c8 * 8) =
-                    make_uint2(lo, hi);
+                else
+                {
+                    for (int32_t tt = 0; tt < I.t; ++tt)
+                    {
+                        const uint2 v = *reinterpret_cast<const uint2 *>(sp + tt * g.sh * g.sw * g.cin_pad);
+                        first_nonzero(lo, seen_lo, v.x, tt);
+                        first_nonzero(hi, seen_hi, v.y, tt);
+                    }
+                    *reinterpret_cast<uint2 *>(fp) = make_uint2(lo, hi);
+                }
             }
         }
 
@@ -232,7 +284,7 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
         {
             const int32_t oy = oy0 + grp0 + ct, ox = ox0 + j;
             const bool pvalid = oy < I.ho && ox < I.wo;
-            const uint8_t *wb = reinterpret_cast<const uint8_t *>(win16) + (ct * g.pitch + j) * 16;
+            const uint8_t *wb = reinterpret_cast<const uint8_t *>(win16) + (ct * g.pitch + j) * UB;
             uint32_t mask[2][4];
             int32_t mxb[2][4];
 #pragma unroll
@@ -256,9 +308,19 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
                     {
                         const int32_t iy = sy0 + grp0 + r;
                         uint2 raw = make_uint2(0, 0);
-                        if (colv && iy >= 0 && iy < I.h)
-                            raw = *reinterpret_cast<const uint2 *>(sp0 + ((t0 + ts) * g.sh + r) * g.sw * g.cin_pad);
-                        win16[ts * g.win_units + r * g.pitch + wcu] = widen(raw);
+                        const uint8_t *sp = sp0 + ((t0 + ts) * g.sh + r) * g.sw * g.cin_pad;
+                        if (PAIR)
+                        {
+                            if (colv && iy >= 0 && iy < I.h) raw.x = *reinterpret_cast<const uint32_t *>(sp);
+                            const uint4 hv = widen(raw);
+                            reinterpret_cast<uint2 *>(win16)[ts * 2 * g.win_units + r * g.pitch + wcu] =
+                                make_uint2(hv.x, hv.y);
+                        }
+                        else
+                        {
+                            if (colv && iy >= 0 && iy < I.h) raw = *reinterpret_cast<const uint2 *>(sp);
+                            win16[ts * g.win_units + r * g.pitch + wcu] = widen(raw);
+                        }
                         ts += wdq, r += wdr;
                         if (r >= g.wrows) r -= g.wrows, ++ts;
                     }
@@ -269,9 +331,21 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
                 // (loads past the chunk re-read its last step, unused)
                 half8 b0[S], b1[S];
                 float4v c0, c1, n0, n1;
-                load_b(b0, wb, offs);
+                auto load = [&](half8 (&b)[S], const uint8_t *base) {
+                    if (PAIR)
+#pragma unroll
+                        for (int s = 0; s < S; ++s)
+                        {
+                            const uint2 lo = *reinterpret_cast<const uint2 *>(base + offs[s]);
+                            const uint2 hi = *reinterpret_cast<const uint2 *>(base + offs[s] + 8);
+                            b[s] = __builtin_bit_cast(half8, make_uint4(lo.x, lo.y, hi.x, hi.y));
+                        }
+                    else
+                        load_b(b, base, offs);
+                };
+                load(b0, wb);
                 mfma_chain(a, b0, c0, c1);
-                load_b(b0, wb + min(1, tc - 1) * win_step, offs);
+                load(b0, wb + min(1, tc - 1) * win_step);
                 auto update = [&](int32_t k, const float4v &u0, const float4v &u1) {
                     step_update(u0, u1, g.th, mask, mxb);
                     if (DIAG && pvalid)
@@ -288,10 +362,10 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
                 };
                 for (int32_t k = 0; k < tc; k += 2)
                 {
-                    load_b(b1, wb + min(k + 2, tc - 1) * win_step, offs);
+                    load(b1, wb + min(k + 2, tc - 1) * win_step);
                     if (k + 1 < tc) mfma_chain(a, b0, n0, n1);
                     update(k, c0, c1);
-                    load_b(b0, wb + min(k + 3, tc - 1) * win_step, offs);
+                    load(b0, wb + min(k + 3, tc - 1) * win_step);
                     if (k + 2 < tc) mfma_chain(a, b1, c0, c1);
                     if (k + 1 < tc) update(k + 1, n0, n1);
                 }
@@ -319,9 +393,16 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
                     const int4 te = gtab[4 * s + q];
                     const int32_t iy = oy + te.x, ix = ox + te.y;
                     uint2 raw = make_uint2(0, 0);
-                    if (st >= 0 && iy >= 0 && iy < I.h && ix >= 0 && ix < I.w)
-                        raw = *reinterpret_cast<const uint2 *>(
-                            stage + ((st * g.sh + iy - sy0) * g.sw + ix - sx0) * g.cin_pad + te.z);
+                    const bool row = st >= 0 && iy >= 0 && iy < I.h;
+                    const uint8_t *sp = stage + ((st * g.sh + iy - sy0) * g.sw + ix - sx0) * g.cin_pad + te.z;
+                    if (PAIR)
+                    {
+                        // the two taps' 4 channels, each 0 outside the input
+                        if (row && ix >= 0 && ix < I.w) raw.x = *reinterpret_cast<const uint32_t *>(sp);
+                        if (row && ix + 1 >= 0 && ix + 1 < I.w) raw.y = *reinterpret_cast<const uint32_t *>(sp + 4);
+                    }
+                    else if (row && ix >= 0 && ix < I.w)
+                        raw = *reinterpret_cast<const uint2 *>(sp);
                     b[s] = __builtin_bit_cast(half8, widen(raw));
                 }
             };
@@ -452,67 +533,115 @@ __global__ void __launch_bounds__(1024)
     for (int32_t r = k + int32_t(threadIdx.x); r < count; r += blockDim.x) wkey[s * count + r] = -1;
 }
 
-// Spikes [t][cout][ho][wo] and pooled spikes of every instance from the
-// summaries; one block per (instance, step), steps past an instance's t idle.
-__global__ void __launch_bounds__(256)
-    region_write(const Summary *__restrict__ summary, const RegionInst *__restrict__ inst, int32_t tmax, int32_t cout,
-                 uint8_t *__restrict__ out, uint8_t *__restrict__ pooled, int32_t pkh, int32_t pkw, int32_t psh,
-                 int32_t psw)
+// Spikes [t][cout][ho][wo] of every instance from the summaries.  Grid
+// (chunk, instance): a thread owns 4 consecutive pixels (one when hw % 4 !=
+// 0 or the instance's spikes are not 4-aligned), reads their summaries once
+// and writes every step's channel planes, one 32-bit store per plane.
+constexpr int WRITE_THREADS = 256;
+
+__global__ void __launch_bounds__(WRITE_THREADS)
+    region_spikes(const Summary *__restrict__ summary, const RegionInst *__restrict__ inst, int32_t cout,
+                  uint8_t *__restrict__ out)
 {
-    const int64_t s = blockIdx.x / tmax;
-    const int32_t step = blockIdx.x % tmax;
-    const RegionInst I = inst[s];
-    if (step >= I.t) return;
+    const RegionInst I = inst[blockIdx.y];
     const int32_t hw = I.ho * I.wo;
-    const Summary *sm = summary + I.sum_off;
-    auto act = [&](int32_t p) {
-        const Summary e = sm[p];
-        return (e.mask >> step) & 1u ? e.c : -1;
-    };
-    if (out)
+    uint8_t *base = out + I.out_off;
+    const bool quad = hw % 4 == 0 && (reinterpret_cast<uintptr_t>(base) & 3) == 0;
+    const int32_t per = quad ? 4 : 1, units = hw / per;
+    const int32_t u = int32_t(blockIdx.x) * WRITE_THREADS + int32_t(threadIdx.x);
+    if (u >= units) return;
+    const Summary *sm = summary + I.sum_off + int64_t(u) * per;
+    int32_t c[4];
+    uint32_t m[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
     {
-        uint8_t *dst = out + I.out_off + int64_t(step) * cout * hw;
-        if (hw % 4 == 0 && (reinterpret_cast<uintptr_t>(dst) & 3) == 0)
-        {
-            // four pixels of one channel plane per 32-bit store
-            const int32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
-            for (int32_t qd = lane; qd < hw / 4; qd += 64)
-            {
-                const int32_t a0 = act(4 * qd), a1 = act(4 * qd + 1), a2 = act(4 * qd + 2), a3 = act(4 * qd + 3);
-                for (int32_t c = wave; c < cout; c += waves)
-                    *reinterpret_cast<uint32_t *>(dst + int64_t(c) * hw + 4 * qd) =
-                        uint32_t(a0 == c) | uint32_t(a1 == c) << 8 | uint32_t(a2 == c) << 16 | uint32_t(a3 == c) << 24;
-            }
-        }
-        else
-            for (int32_t p = threadIdx.x; p < hw; p += blockDim.x)
-            {
-                const int32_t a0 = act(p);
-                for (int32_t c = 0; c < cout; ++c) dst[int64_t(c) * hw + p] = a0 == c;
-            }
+        const Summary e = k < per ? sm[k] : Summary{-1, 0, 0.0f, 0.0f};
+        c[k] = e.c, m[k] = e.mask;
     }
-    if (pooled)
+    const int64_t plane = hw;
+    for (int32_t step = 0; step < I.t; ++step)
     {
-        const int32_t phw = I.pho * I.pwo;
-        uint8_t *dst = pooled + I.pool_off + int64_t(step) * cout * phw;
-        for (int32_t pos = threadIdx.x; pos < phw; pos += blockDim.x)
+        int32_t a[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) a[k] = (m[k] >> step) & 1u ? c[k] : -1;
+        uint8_t *d = base + int64_t(step) * cout * plane + int64_t(u) * per;
+        if (quad)
+            for (int32_t ch = 0; ch < cout; ++ch)
+                *reinterpret_cast<uint32_t *>(d + ch * plane) = uint32_t(a[0] == ch) | uint32_t(a[1] == ch) << 8 |
+                                                               uint32_t(a[2] == ch) << 16 | uint32_t(a[3] == ch) << 24;
+        else
+            for (int32_t ch = 0; ch < cout; ++ch) d[ch * plane] = a[0] == ch;
+    }
+}
+
+// Pooled spikes [t][cout][pho][pwo].  With windows of at most 4 pixels, at
+// most 64 channels and 4-aligned planes, a thread owns 4 consecutive pooled
+// positions: their windows' summaries stay in registers and each step builds
+// one 64-bit channel mask per position (a pixel has at most one active
+// channel), then writes one 32-bit store per channel plane.  Otherwise a
+// thread owns one position and compares channel by channel.
+__global__ void __launch_bounds__(WRITE_THREADS)
+    region_pooled(const Summary *__restrict__ summary, const RegionInst *__restrict__ inst, int32_t cout,
+                  uint8_t *__restrict__ pooled, int32_t pkh, int32_t pkw, int32_t psh, int32_t psw)
+{
+    const RegionInst I = inst[blockIdx.y];
+    const int32_t phw = I.pho * I.pwo, win = pkh * pkw;
+    uint8_t *base = pooled + I.pool_off;
+    const Summary *sm = summary + I.sum_off;
+    const bool quad = win <= 4 && cout <= 64 && phw % 4 == 0 && (reinterpret_cast<uintptr_t>(base) & 3) == 0;
+    const int32_t u = int32_t(blockIdx.x) * WRITE_THREADS + int32_t(threadIdx.x);
+    auto load = [&](int32_t pos, int32_t k, int32_t &c, uint32_t &m) {
+        const int32_t py = pos / I.pwo, px = pos % I.pwo, y = py * psh + k / pkw, x = px * psw + k % pkw;
+        const bool in = k < win && y < I.ho && x < I.wo;
+        const Summary e = in ? sm[y * I.wo + x] : Summary{-1, 0, 0.0f, 0.0f};
+        c = e.c, m = e.mask;
+    };
+    if (quad)
+    {
+        if (u >= phw / 4) return;
+        int32_t wc[4][4];
+        uint32_t wm[4][4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) load(4 * u + p, k, wc[p][k], wm[p][k]);
+        uint8_t *d = base + 4 * int64_t(u);
+        for (int32_t step = 0; step < I.t; ++step, d += int64_t(cout) * phw)
         {
-            const int32_t py = pos / I.pwo, px = pos % I.pwo;
-            int32_t wa[16];
-            const int32_t win = pkh * pkw;
+            uint64_t cm[4];
 #pragma unroll
-            for (int k = 0; k < 16; ++k)
+            for (int p = 0; p < 4; ++p)
             {
-                const int32_t y = py * psh + k / pkw, x = px * psw + k % pkw;
-                wa[k] = (k < win && y < I.ho && x < I.wo) ? act(y * I.wo + x) : -1;
-            }
-            for (int32_t c = 0; c < cout; ++c)
-            {
-                bool m = false;
+                cm[p] = 0;
 #pragma unroll
-                for (int k = 0; k < 16; ++k) m |= wa[k] == c;
-                dst[int64_t(c) * phw + pos] = m;
+                for (int k = 0; k < 4; ++k)
+                    cm[p] |= uint64_t((wm[p][k] >> step) & 1u) << (wc[p][k] & 63);
             }
+            for (int32_t ch = 0; ch < cout; ++ch)
+                *reinterpret_cast<uint32_t *>(d + int64_t(ch) * phw) =
+                    uint32_t((cm[0] >> ch) & 1) | uint32_t((cm[1] >> ch) & 1) << 8 |
+                    uint32_t((cm[2] >> ch) & 1) << 16 | uint32_t((cm[3] >> ch) & 1) << 24;
+        }
+        return;
+    }
+    if (u >= phw) return;
+    int32_t wc[16];
+    uint32_t wm[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) load(u, k, wc[k], wm[k]);
+    uint8_t *d = base + u;
+    for (int32_t step = 0; step < I.t; ++step, d += int64_t(cout) * phw)
+    {
+        int32_t a[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) a[k] = (wm[k] >> step) & 1u ? wc[k] : -1;
+        for (int32_t ch = 0; ch < cout; ++ch)
+        {
+            bool hit = false;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) hit |= a[k] == ch;
+            d[int64_t(ch) * phw] = hit;
         }
     }
 }
@@ -562,31 +691,42 @@ struct RegionConfig
     int2 *items = nullptr;  // device table [g.items] of (instance, region)
 };
 
-template <int S, bool DIAG>
+template <int S, bool DIAG, bool PAIR>
 void launch_region(const uint8_t *in, const half8 *frag, const RegionInst *inst, const RegionConfig &rc,
                    Summary *summary, float *pot, uint8_t *fs, int grid, hipStream_t s)
 {
     static bool configured = false;  // per instantiation; the attribute is per function
     if (!configured)
     {
-        SPK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&fused_region<S, DIAG>),
+        SPK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&fused_region<S, DIAG, PAIR>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, LDS_BYTES));
         configured = true;
     }
-    fused_region<S, DIAG><<<dim3(grid), dim3(64 * rc.g.waves), rc.lds, s>>>(in, frag, inst, rc.items, summary, pot,
-                                                                          fs, rc.g);
+    fused_region<S, DIAG, PAIR><<<dim3(grid), dim3(64 * rc.g.waves), rc.lds, s>>>(in, frag, inst, rc.items,
+                                                                                summary, pot, fs, rc.g);
     SPK_LAUNCHED();
 }
 
 void dispatch_region(const uint8_t *in, const half8 *frag, const RegionInst *inst, const RegionConfig &rc,
                      Summary *summary, float *pot, uint8_t *fs, int grid, hipStream_t s)
 {
+    if (rc.g.pair)
+        switch (rc.g.steps)
+        {
+#define CASE(k)                                                                              \
+    case k:                                                                                  \
+        return pot ? launch_region<k, true, true>(in, frag, inst, rc, summary, pot, fs, grid, s) \
+                   : launch_region<k, false, true>(in, frag, inst, rc, summary, pot, fs, grid, s);
+            CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+        default: SPK_ARG(false, "Fused layer: K-steps out of range.");
+        }
     switch (rc.g.steps)
     {
-#define CASE(k)                                                                        \
-    case k:                                                                            \
-        return pot ? launch_region<k, true>(in, frag, inst, rc, summary, pot, fs, grid, s) \
-                   : launch_region<k, false>(in, frag, inst, rc, summary, pot, fs, grid, s);
+#define CASE(k)                                                                               \
+    case k:                                                                                   \
+        return pot ? launch_region<k, true, false>(in, frag, inst, rc, summary, pot, fs, grid, s) \
+                   : launch_region<k, false, false>(in, frag, inst, rc, summary, pot, fs, grid, s);
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
         CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
 #undef CASE
@@ -624,7 +764,7 @@ struct LayerPlan
     bool pool = false, tune = false, tuned = false, fast = false;
     float th = 0;
     std::vector<RegionInst> host;
-    int64_t in_bytes = 0, out_bytes = 0, pool_bytes = 0, pixels = 0, fs_bytes = 0;
+    int64_t in_bytes = 0, out_bytes = 0, pool_bytes = 0, pixels = 0, fs_bytes = 0, max_hw = 0, max_phw = 0;
     std::vector<std::unique_ptr<RegionConfig>> configs;
     std::vector<std::unique_ptr<DeviceBuffer<int2>>> item_tables;
     int chosen = 0, grid = 0;
@@ -650,20 +790,23 @@ bool region_config(const LayerPlan &P, int32_t m, bool fast, RegionGeom &g, size
     g = RegionGeom{};
     g.fast = fast;
     g.cin = P.cin, g.cout = P.cout, g.kh = P.kh, g.kw = P.kw, g.pt = P.pt, g.pl = P.pl;
-    g.cin_pad = int32_t(cdiv(P.cin, 8) * 8);
-    const int32_t planes = g.cin_pad / 8;
-    g.groups = P.kh * P.kw * planes;
+    const int32_t np = (P.kw + 1) / 2;
+    g.pair = use_pair(P.cin, P.kh, P.kw);
+    g.cin_pad = g.pair ? 4 : int32_t(cdiv(P.cin, 8) * 8);
+    const int32_t planes = g.pair ? 1 : g.cin_pad / 8;
+    g.groups = g.pair ? P.kh * np : P.kh * P.kw * planes;
     g.steps = int32_t(cdiv(g.groups, 4));
+    const int32_t kspan = g.pair ? 2 * np : P.kw;  // window columns a pixel's taps span
     g.cg = int32_t(cdiv(P.cout, 32));
     g.nct = std::max(1, REGION_WAVES / g.cg);
     g.waves = g.cg * g.nct;
     g.rh = g.nct * m;
     g.sh = g.rh + P.kh - 1;
-    g.sw = int32_t(cdiv(RW + P.kw - 1 + 3, 4) * 4);
+    g.sw = int32_t(cdiv(RW + kspan - 1 + 3, 4) * 4);
     g.wrows = g.nct + P.kh - 1;
-    g.wcols = RW + P.kw - 1;
+    g.wcols = RW + kspan - 1;
     g.pitch = planes * g.wcols;
-    g.win_units = g.wrows * g.pitch;
+    g.win_units = g.pair ? int32_t(cdiv(g.wrows * g.pitch, 2)) : g.wrows * g.pitch;
     g.th = P.th;
     const int64_t stage = cdiv(int64_t(P.tmax) * g.sh * g.sw * g.cin_pad, 16) * 16;
     const int64_t fixed = stage + region_fixed(g.waves, g.nct, g.steps);
@@ -707,8 +850,13 @@ void LayerPlan::run(const uint8_t *in, float *kernel, const spk_stdp_config *cfg
                     spk_winner *winners, int64_t *counts, float *pot, hipStream_t s)
 {
     const RegionGeom &g0 = configs[0]->g;
-    prep_fragments<<<dim3(unsigned(cdiv(int64_t(g0.cg) * 2 * g0.steps * 64, 256))), 256, 0, s>>>(
-        kernel, frag.p, g0.cout, g0.cin, g0.kh, g0.kw, g0.cin_pad, g0.groups, g0.steps, g0.cg * 2);
+    const dim3 fgrid(unsigned(cdiv(int64_t(g0.cg) * 2 * g0.steps * 64, 256)));
+    if (g0.pair)
+        prep_fragments_pair<<<fgrid, 256, 0, s>>>(kernel, frag.p, g0.cout, g0.cin, g0.kh, g0.kw, g0.groups, g0.steps,
+                                                  g0.cg * 2);
+    else
+        prep_fragments<<<fgrid, 256, 0, s>>>(kernel, frag.p, g0.cout, g0.cin, g0.kh, g0.kw, g0.cin_pad, g0.groups,
+                                             g0.steps, g0.cg * 2);
     SPK_LAUNCHED();
     if (tune && !tuned) choose(in, s);
     const bool learn = cfg != nullptr;
@@ -729,10 +877,16 @@ void LayerPlan::run(const uint8_t *in, float *kernel, const spk_stdp_config *cfg
     region_select<<<dim3(unsigned(n)), 1024, 0, s>>>(summary.p, inst.p, ry, rx, count, val.p, chan.p, winners, counts,
                                                      wkey.p);
     SPK_LAUNCHED();
-    if (out || pooled)
+    if (out)
     {
-        region_write<<<dim3(unsigned(n * tmax)), 256, 0, s>>>(summary.p, inst.p, tmax, cout, out,
-                                                              pool ? pooled : nullptr, pkh, pkw, psh, psw);
+        region_spikes<<<dim3(unsigned(cdiv(max_hw, WRITE_THREADS)), unsigned(n)), WRITE_THREADS, 0, s>>>(
+            summary.p, inst.p, cout, out);
+        SPK_LAUNCHED();
+    }
+    if (pool && pooled)
+    {
+        region_pooled<<<dim3(unsigned(cdiv(max_phw, WRITE_THREADS)), unsigned(n)), WRITE_THREADS, 0, s>>>(
+            summary.p, inst.p, cout, pooled, pkh, pkw, psh, psw);
         SPK_LAUNCHED();
     }
     if (learn)
@@ -751,7 +905,8 @@ LayerPlan *make_plan(int64_t n, const int32_t *shapes, int64_t cin, int64_t cout
                      int64_t pl, int64_t pb, int64_t pr, double threshold, int64_t ry, int64_t rx, int64_t count,
                      int64_t pkh, int64_t pkw, int64_t psh, int64_t psw, bool tune, const uint8_t *in_ptr_hint)
 {
-    SPK_ARG(n >= 1 && n < (1LL << 31) && shapes, "Fused layer: no instances.");
+    SPK_ARG(n >= 1 && shapes, "Fused layer: no instances.");
+    SPK_ARG(n <= 65535, "Fused layer: at most 65535 instances per plan.");
     SPK_ARG(cin >= 1 && cout >= 2 && kh >= 1 && kw >= 1, "Fused layer: invalid shapes.");
     SPK_ARG(pt >= 0 && pl >= 0 && pb >= 0 && pr >= 0, "Pad items are incorrect.");
     SPK_ARG(cout <= 256, "Fused layer: at most 256 output channels.");
@@ -770,7 +925,7 @@ LayerPlan *make_plan(int64_t n, const int32_t *shapes, int64_t cin, int64_t cout
         SPK_ARG(pkh * pkw <= 16, "Fused layer: pool windows of at most 16 pixels.");
         P->pkh = int32_t(pkh), P->pkw = int32_t(pkw), P->psh = int32_t(psh), P->psw = int32_t(psw);
     }
-    const int64_t cin_pad = cdiv(cin, 8) * 8;
+    const int64_t cin_pad = use_pair(cin, kh, kw) ? 4 : cdiv(cin, 8) * 8;
     P->host.resize(size_t(n));
     int64_t in_b = 0, out_b = 0, pool_b = 0, pix = 0, fs_b = 0;
     for (int64_t i = 0; i < n; ++i)
@@ -794,6 +949,7 @@ LayerPlan *make_plan(int64_t n, const int32_t *shapes, int64_t cin, int64_t cout
         in_b += t * cin * h * w, out_b += t * cout * ho * wo, pool_b += t * cout * pho * pwo;
         pix += ho * wo, fs_b += h * w * cin_pad;
         P->tmax = std::max(P->tmax, int32_t(t));
+        P->max_hw = std::max(P->max_hw, ho * wo), P->max_phw = std::max(P->max_phw, pho * pwo);
     }
     P->in_bytes = in_b, P->out_bytes = out_b, P->pool_bytes = pool_b, P->pixels = pix, P->fs_bytes = fs_b;
     // dword staging: every row a whole number of aligned dwords (then every
