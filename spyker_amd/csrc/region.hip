@@ -533,6 +533,22 @@ __global__ void __launch_bounds__(1024)
     for (int32_t r = k + int32_t(threadIdx.x); r < count; r += blockDim.x) wkey[s * count + r] = -1;
 }
 
+// Writes channel planes 0 .. cout-1 (at most 64) of 4 consecutive positions:
+// bit ch of cm[p] is position p's byte in plane ch.  Byte g of the four masks
+// side by side holds channels 8g .. 8g+7 of the 4 positions, so plane 8g + i
+// is that dword's bit i of every byte.
+__device__ __forceinline__ void store_planes(uint8_t *d, int64_t plane, int32_t cout, const uint64_t (&cm)[4])
+{
+    for (int32_t g = 0; 8 * g < cout; ++g)
+    {
+        const uint32_t x = uint32_t((cm[0] >> (8 * g)) & 0xFF) | uint32_t((cm[1] >> (8 * g)) & 0xFF) << 8 |
+                           uint32_t((cm[2] >> (8 * g)) & 0xFF) << 16 | uint32_t((cm[3] >> (8 * g)) & 0xFF) << 24;
+        const int32_t n = min(8, cout - 8 * g);
+        for (int32_t i = 0; i < n; ++i)
+            *reinterpret_cast<uint32_t *>(d + (8 * g + i) * plane) = (x >> i) & 0x01010101u;
+    }
+}
+
 // Spikes [t][cout][ho][wo] of every instance from the summaries.  Grid
 // (chunk, instance): a thread owns 4 consecutive pixels (one when hw % 4 !=
 // 0 or the instance's spikes are not 4-aligned), reads their summaries once
@@ -566,7 +582,14 @@ __global__ void __launch_bounds__(WRITE_THREADS)
 #pragma unroll
         for (int k = 0; k < 4; ++k) a[k] = (m[k] >> step) & 1u ? c[k] : -1;
         uint8_t *d = base + int64_t(step) * cout * plane + int64_t(u) * per;
-        if (quad)
+        if (quad && cout <= 64)
+        {
+            uint64_t cm[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) cm[k] = a[k] >= 0 ? uint64_t(1) << a[k] : 0;
+            store_planes(d, plane, cout, cm);
+        }
+        else if (quad)
             for (int32_t ch = 0; ch < cout; ++ch)
                 *reinterpret_cast<uint32_t *>(d + ch * plane) = uint32_t(a[0] == ch) | uint32_t(a[1] == ch) << 8 |
                                                                uint32_t(a[2] == ch) << 16 | uint32_t(a[3] == ch) << 24;
@@ -618,10 +641,7 @@ __global__ void __launch_bounds__(WRITE_THREADS)
                 for (int k = 0; k < 4; ++k)
                     cm[p] |= uint64_t((wm[p][k] >> step) & 1u) << (wc[p][k] & 63);
             }
-            for (int32_t ch = 0; ch < cout; ++ch)
-                *reinterpret_cast<uint32_t *>(d + int64_t(ch) * phw) =
-                    uint32_t((cm[0] >> ch) & 1) | uint32_t((cm[1] >> ch) & 1) << 8 |
-                    uint32_t((cm[2] >> ch) & 1) << 16 | uint32_t((cm[3] >> ch) & 1) << 24;
+            store_planes(d, phw, cout, cm);
         }
         return;
     }
