@@ -40,6 +40,8 @@
 //     a second at c*'s first firing step yields convwta's t0 value.  The
 //     gathered passes issue the same MFMA sequence on the same operands as
 //     the step loop, so they reproduce its potentials bit for bit.
+#include <mutex>
+
 #include "fused.hpp"
 
 namespace spk
@@ -574,6 +576,53 @@ KernelTimer &timer()
     return *t;
 }
 
+SideStream::SideStream(hipStream_t main_, bool use) : main(main_), stream(main_)
+{
+    if (!use) return;
+    // one non-blocking side stream per device, at the default priority
+    static std::mutex mu;
+    static std::vector<hipStream_t> streams;
+    int dev = 0;
+    SPK_HIP(hipGetDevice(&dev));
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        if (int(streams.size()) <= dev) streams.resize(size_t(dev) + 1, nullptr);
+        if (!streams[size_t(dev)]) SPK_HIP(hipStreamCreateWithFlags(&streams[size_t(dev)], hipStreamNonBlocking));
+        stream = streams[size_t(dev)];
+    }
+    hipEvent_t fork;
+    SPK_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    SPK_HIP(hipEventRecord(fork, main));
+    SPK_HIP(hipStreamWaitEvent(stream, fork, 0));
+    SPK_HIP(hipEventDestroy(fork));
+}
+
+void SideStream::join()
+{
+    if (stream == main || joined) return;
+    hipEvent_t done;
+    SPK_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+    SPK_HIP(hipEventRecord(done, stream));
+    SPK_HIP(hipStreamWaitEvent(main, done, 0));
+    SPK_HIP(hipEventDestroy(done));
+    joined = true;
+}
+
+SideStream::~SideStream()
+{
+    if (stream != main && !joined)
+    {
+        // an error path: still order the call's stream after the side work
+        hipEvent_t done;
+        if (hipEventCreateWithFlags(&done, hipEventDisableTiming) == hipSuccess)
+        {
+            (void)hipEventRecord(done, stream);
+            (void)hipStreamWaitEvent(main, done, 0);
+            (void)hipEventDestroy(done);
+        }
+    }
+}
+
 namespace
 {
 template <int S, bool DIAG>
@@ -740,17 +789,21 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
         SPK_LAUNCHED();
         dispatch_layer(g.steps, (const uint8_t *)in, frag.as<half8>(), summary.as<Summary>(), (float *)potentials,
                        learn ? fsm.as<uint8_t>() : nullptr, g, lds, s);
-        fused_select<<<dim3(unsigned(n)), 256, size_t(hw) * 8, s>>>(summary.as<Summary>(), g.t, g.ho, g.wo,
-                                                                    int32_t(ry), int32_t(rx), int32_t(count), winners,
-                                                                    counts, wkey.as<int32_t>());
-        SPK_LAUNCHED();
+        // the writer needs only the summaries: it runs on a side stream,
+        // overlapping convwta, the records and the STDP chains (HBM-bound
+        // beside latency-bound), and the call's stream waits for it at the end
+        SideStream side(s, out || pooled);
         if (out || pooled)
         {
-            fused_write<<<dim3(unsigned(n * t)), 256, size_t(hw) * 2, s>>>(
+            fused_write<<<dim3(unsigned(n * t)), 256, size_t(hw) * 2, side.stream>>>(
                 summary.as<Summary>(), (uint8_t *)out, (uint8_t *)pooled, g.t, g.cout, g.ho, g.wo, int32_t(pkh),
                 int32_t(pkw), int32_t(psh), int32_t(psw), int32_t(pho), int32_t(pwo));
             SPK_LAUNCHED();
         }
+        fused_select<<<dim3(unsigned(n)), 256, size_t(hw) * 8, s>>>(summary.as<Summary>(), g.t, g.ho, g.wo,
+                                                                    int32_t(ry), int32_t(rx), int32_t(count), winners,
+                                                                    counts, wkey.as<int32_t>());
+        SPK_LAUNCHED();
         if (learn)
         {
             const int64_t ksz = cin * kh * kw;
@@ -764,6 +817,7 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
                 (float *)kernel, bits.as<uint32_t>(), wkey.as<int32_t>(), fp, cfg.c[0]);
             SPK_LAUNCHED();
         }
+        side.join();
     });
 }
 }

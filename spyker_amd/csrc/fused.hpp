@@ -281,6 +281,21 @@ struct KernelTimer
 };
 KernelTimer &timer();
 
+// Work forked from a call's stream onto the device's side stream (the
+// caller's stream when `use` is false) and joined back by join(), or by the
+// destructor on an error path.  Fork and join are event record/wait pairs,
+// so a stream capture of the call follows them.
+struct SideStream
+{
+    hipStream_t main, stream;
+    bool joined = false;
+    SideStream(hipStream_t main, bool use);
+    SideStream(const SideStream &) = delete;
+    SideStream &operator=(const SideStream &) = delete;
+    void join();
+    ~SideStream();
+};
+
 // The layer step over output regions (region.hip) for a uniform batch whose
 // sample does not fit the LDS whole; same arguments as spk_stdp_layer_fused,
 // cfg = null without learning.

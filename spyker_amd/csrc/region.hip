@@ -894,21 +894,24 @@ void LayerPlan::run(const uint8_t *in, float *kernel, const spk_stdp_config *cfg
         SPK_HIP(hipEventRecord(e1, s));
         tm.events.emplace_back(e0, e1);
     }
-    region_select<<<dim3(unsigned(n)), 1024, 0, s>>>(summary.p, inst.p, ry, rx, count, val.p, chan.p, winners, counts,
-                                                     wkey.p);
-    SPK_LAUNCHED();
+    // the writers need only the summaries: they run on the side stream
+    // beside convwta, the records and the STDP chains
+    SideStream side(s, out || (pool && pooled));
     if (out)
     {
-        region_spikes<<<dim3(unsigned(cdiv(max_hw, WRITE_THREADS)), unsigned(n)), WRITE_THREADS, 0, s>>>(
+        region_spikes<<<dim3(unsigned(cdiv(max_hw, WRITE_THREADS)), unsigned(n)), WRITE_THREADS, 0, side.stream>>>(
             summary.p, inst.p, cout, out);
         SPK_LAUNCHED();
     }
     if (pool && pooled)
     {
-        region_pooled<<<dim3(unsigned(cdiv(max_phw, WRITE_THREADS)), unsigned(n)), WRITE_THREADS, 0, s>>>(
+        region_pooled<<<dim3(unsigned(cdiv(max_phw, WRITE_THREADS)), unsigned(n)), WRITE_THREADS, 0, side.stream>>>(
             summary.p, inst.p, cout, pooled, pkh, pkw, psh, psw);
         SPK_LAUNCHED();
     }
+    region_select<<<dim3(unsigned(n)), 1024, 0, s>>>(summary.p, inst.p, ry, rx, count, val.p, chan.p, winners, counts,
+                                                     wkey.p);
+    SPK_LAUNCHED();
     if (learn)
     {
         region_records<<<dim3(unsigned(cdiv(stdp.slots, 4))), 256, 0, s>>>(fs.p, summary.p, inst.p, wkey.p, stdp,
@@ -918,6 +921,7 @@ void LayerPlan::run(const uint8_t *in, float *kernel, const spk_stdp_config *cfg
                      s>>>(kernel, bits.p, wkey.p, stdp, *cfg);
         SPK_LAUNCHED();
     }
+    side.join();
 }
 
 // Makes a plan; shapes: host [n][3] of (t, h, w).
