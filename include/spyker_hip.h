@@ -205,6 +205,24 @@ int32_t spk_rank_fcstdp(const void *in, void *kernel, int32_t kernel_type, const
                         int64_t fin, int64_t fout, const spk_stdp_config *configs, int64_t nconfigs,
                         const spk_winner *winners, const int64_t *counts, int64_t maxcount, void *stream);
 
+/* -- readout: first-spike times and back (cpu/time.cpp, cuda/time.cu) -------- */
+
+/* Core::rank_gather (cuda_rank_gather, impl.h:260-262): out[i][j] = the first
+ * step s with in[i][s][j] > threshold, t where none.  in [n][t][len], out
+ * [n][len]; types U8, U16, I32, F32 (integer inputs take the threshold
+ * truncated toward zero, clamped into their range). */
+int32_t spk_rank_gather(const void *in, int32_t in_type, void *out, int32_t out_type, int64_t n, int64_t t,
+                        int64_t len, double threshold, void *stream);
+/* Core::rank_scatter (cuda_rank_scatter, impl.h:264-266): out[i][s][j] =
+ * (s >= in[i][j]) with s an unsigned 32-bit step index compared in the input's
+ * type.  in [n][len], out [n][t][len]; types U8, U16, I32, F32. */
+int32_t spk_rank_scatter(const void *in, int32_t in_type, void *out, int32_t out_type, int64_t n, int64_t len,
+                         int64_t t, void *stream);
+/* Core::rate_gather (cuda_rate_gather, impl.h:303-305): out[i][j] =
+ * in[i][t-1][j], converted (float to integer truncated and clamped). */
+int32_t spk_rate_gather(const void *in, int32_t in_type, void *out, int32_t out_type, int64_t n, int64_t t,
+                        int64_t len, void *stream);
+
 /* -- sparse event path (Core::sparse_*, impl.h:328-356; sparse/*.cpp) ------- */
 
 /* A sparse spike tensor of shape [n][t][c][h][w] holds, for every sample i

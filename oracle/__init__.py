@@ -164,6 +164,48 @@ class _Lib:
             self._check(rc, "fcwta")
         return win.reshape(n, count), cnt
 
+    def rank_gather(self, x, th=0):
+        """x: [N, T, ...] uint8 or float32 -> int32 [N, ...]: the first step whose value is > th
+        (th in the input's type), T where none (cpu/time.cpp:40-64)."""
+        n, t = x.shape[:2]
+        ln = int(np.prod(x.shape[2:]))
+        out = np.zeros((n,) + x.shape[2:], np.int32)
+        if x.dtype == np.uint8:
+            x = np.ascontiguousarray(x)
+            rc = self._f("rank_gather_u8")(_ptr(x), I64(n), I64(t), I64(ln), C.c_uint8(int(th)), _ptr(out))
+        else:
+            x = np.ascontiguousarray(x, dtype=np.float32)
+            rc = self._f("rank_gather_f32")(_ptr(x), I64(n), I64(t), I64(ln), F32(th), _ptr(out))
+        if self.prefix == "ref_":
+            self._check(rc, "rank_gather")
+        return out
+
+    def rank_scatter(self, x, t):
+        """x: [N, ...] uint8 or float32 times -> uint8 [N, t, ...]: step s spikes where s >= x
+        (cpu/time.cpp:66-84)."""
+        n = x.shape[0]
+        ln = int(np.prod(x.shape[1:]))
+        out = np.zeros((n, t) + x.shape[1:], np.uint8)
+        if x.dtype == np.uint8:
+            x = np.ascontiguousarray(x)
+            rc = self._f("rank_scatter_u8")(_ptr(x), I64(n), I64(ln), I64(t), _ptr(out))
+        else:
+            x = np.ascontiguousarray(x, dtype=np.float32)
+            rc = self._f("rank_scatter_f32")(_ptr(x), I64(n), I64(ln), I64(t), _ptr(out))
+        if self.prefix == "ref_":
+            self._check(rc, "rank_scatter")
+        return out
+
+    def rate_gather(self, x):
+        """x: [N, T, ...] float32 -> float32 [N, ...]: the last step (cpu/time.cpp:86-92)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        n, t = x.shape[:2]
+        out = np.zeros((n,) + x.shape[2:], np.float32)
+        rc = self._f("rate_gather_f32")(_ptr(x), I64(n), I64(t), I64(int(np.prod(x.shape[2:]))), _ptr(out))
+        if self.prefix == "ref_":
+            self._check(rc, "rate_gather")
+        return out
+
     def pool(self, x, kernel, stride=None, pad=(0, 0, 0, 0)):
         """x: [..., H, W] uint8 or float32; pad (top, left, bottom, right)."""
         kh, kw = (kernel, kernel) if np.isscalar(kernel) else kernel

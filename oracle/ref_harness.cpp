@@ -362,4 +362,35 @@ int ref_rank_fcwta_u8(const uint8_t *x, int64_t n, int64_t t, int64_t f, int64_t
         put(r, count, winners, counts);
     });
 }
+
+int ref_rank_gather_u8(const uint8_t *in, int64_t n, int64_t t, int64_t len, uint8_t threshold, int32_t *out)
+{
+    return guard([&] {
+        cpu_rank_gather(Dyn3((void *)in, Type::U8, n, t, len), Dyn2(out, Type::I32, n, len), Scalar(threshold));
+    });
+}
+
+int ref_rank_gather_f32(const float *in, int64_t n, int64_t t, int64_t len, float threshold, int32_t *out)
+{
+    return guard([&] {
+        cpu_rank_gather(Dyn3((void *)in, Type::F32, n, t, len), Dyn2(out, Type::I32, n, len), Scalar(threshold));
+    });
+}
+
+int ref_rank_scatter_f32(const float *in, int64_t n, int64_t len, int64_t t, uint8_t *out)
+{
+    return guard([&] { cpu_rank_scatter(Dyn2((void *)in, Type::F32, n, len), Dyn3(out, Type::U8, n, t, len)); });
+}
+
+int ref_rank_scatter_u8(const uint8_t *in, int64_t n, int64_t len, int64_t t, uint8_t *out)
+{
+    return guard([&] { cpu_rank_scatter(Dyn2((void *)in, Type::U8, n, len), Dyn3(out, Type::U8, n, t, len)); });
+}
+
+int ref_rate_gather_f32(const float *in, int64_t n, int64_t t, int64_t len, float *out)
+{
+    return guard([&] {
+        cpu_rate_gather(Dyn3((void *)in, Type::F32, n, t, len), Dyn2(out, Type::F32, n, len), Scalar(0.0f));
+    });
+}
 }

@@ -953,3 +953,56 @@ void orc_rank_fcwta_u8(const uint8_t *x, int64_t n, int64_t t, int64_t f, int64_
     }
     free(temp);
 }
+
+/* ------------------------------------------------------------------------ */
+/* readout: first-spike times and back (src/spyker/cpu/time.cpp)           */
+/* ------------------------------------------------------------------------ */
+
+/* rank gather, time.cpp:40-64: out[i][j] = the first step t with
+ * in[i][t][j] > threshold, T if none (threshold cast to the input type). */
+void orc_rank_gather_u8(const uint8_t *in, int64_t n, int64_t t, int64_t len, uint8_t threshold, int32_t *out)
+{
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < len; ++j)
+        {
+            int32_t f = (int32_t)t;
+            for (int64_t s = t - 1; s >= 0; --s)
+                if (in[(i * t + s) * len + j] > threshold) f = (int32_t)s;
+            out[i * len + j] = f;
+        }
+}
+
+void orc_rank_gather_f32(const float *in, int64_t n, int64_t t, int64_t len, float threshold, int32_t *out)
+{
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < len; ++j)
+        {
+            int32_t f = (int32_t)t;
+            for (int64_t s = t - 1; s >= 0; --s)
+                if (in[(i * t + s) * len + j] > threshold) f = (int32_t)s;
+            out[i * len + j] = f;
+        }
+}
+
+/* rank scatter, time.cpp:66-84: out[i][s][j] = (s >= in[i][j]), the step
+ * index an unsigned 32-bit value compared in the input's type. */
+void orc_rank_scatter_f32(const float *in, int64_t n, int64_t len, int64_t t, uint8_t *out)
+{
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t s = 0; s < t; ++s)
+            for (int64_t j = 0; j < len; ++j) out[(i * t + s) * len + j] = (float)(uint32_t)s >= in[i * len + j];
+}
+
+void orc_rank_scatter_u8(const uint8_t *in, int64_t n, int64_t len, int64_t t, uint8_t *out)
+{
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t s = 0; s < t; ++s)
+            for (int64_t j = 0; j < len; ++j) out[(i * t + s) * len + j] = (uint32_t)s >= (uint32_t)in[i * len + j];
+}
+
+/* rate gather, time.cpp:86-92: the last step, copied. */
+void orc_rate_gather_f32(const float *in, int64_t n, int64_t t, int64_t len, float *out)
+{
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < len; ++j) out[i * len + j] = in[(i * t + t - 1) * len + j];
+}

@@ -136,6 +136,13 @@ int32_t orc_sparse_stdp(const int64_t *off, const int32_t *ev, int64_t n, int64_
 void orc_rank_fcwta_u8(const uint8_t *x, int64_t n, int64_t t, int64_t f, int64_t radius, int64_t count,
                        orc_winner *winners, int64_t *counts);
 
+/* readout (cpu/time.cpp) */
+void orc_rank_gather_u8(const uint8_t *in, int64_t n, int64_t t, int64_t len, uint8_t threshold, int32_t *out);
+void orc_rank_gather_f32(const float *in, int64_t n, int64_t t, int64_t len, float threshold, int32_t *out);
+void orc_rank_scatter_f32(const float *in, int64_t n, int64_t len, int64_t t, uint8_t *out);
+void orc_rank_scatter_u8(const uint8_t *in, int64_t n, int64_t len, int64_t t, uint8_t *out);
+void orc_rate_gather_f32(const float *in, int64_t n, int64_t t, int64_t len, float *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,13 +29,16 @@ from ._lib import call
 
 __all__ = [
     "DoGFilter", "GaborFilter", "STDPConfig", "Winner", "Winners", "DoG", "Gabor", "Conv", "FC", "threshold",
-    "fire", "code", "pool", "inhibit", "convwta", "fcwta", "conv", "fc", "pad", "random_seed", "expand2", "expand4",
+    "fire", "code", "pool", "inhibit", "convwta", "fcwta", "conv", "fc", "pad", "gather", "scatter", "random_seed",
+    "expand2", "expand4",
     "layer_step", "LayerPlan", "sparse", "SparseTensor", "to_sparse",
 ]
 
 _TYPES = {torch.uint8: _lib.U8, torch.float16: _lib.F16, torch.float32: _lib.F32, torch.float64: _lib.F64,
-          torch.int8: _lib.I8, torch.int16: _lib.I16, torch.int32: _lib.I32, torch.int64: _lib.I64}
-_DTYPES = {"u8": torch.uint8, "f16": torch.float16, "f32": torch.float32, "f64": torch.float64}
+          torch.int8: _lib.I8, torch.int16: _lib.I16, torch.int32: _lib.I32, torch.int64: _lib.I64,
+          torch.uint16: _lib.U16}
+_DTYPES = {"u8": torch.uint8, "u16": torch.uint16, "i8": torch.int8, "i16": torch.int16, "i32": torch.int32,
+           "i64": torch.int64, "f16": torch.float16, "f32": torch.float32, "f64": torch.float64}
 
 
 def _dtype(d):
@@ -346,6 +349,37 @@ def fc(array: torch.Tensor, kernel: torch.Tensor) -> torch.Tensor:
         raise RuntimeError("Input and kernel shapes don't match.")
     out = torch.empty((n, t, fout), dtype=kernel.dtype, device=a.device)
     call("spk_fc", _ptr(a), _type(a), _ptr(kernel), _ptr(out), n * t, fin, fout, _stream(a))
+    return out
+
+
+def gather(array: torch.Tensor, threshold: float = 0.0, dtype="u8", code: str = "rank") -> torch.Tensor:
+    """First spike step of every element (rank), or the last step (rate): [N, T, ...] -> [N, ...]
+    (src/python/spyker/module.py:1106-1142, src/spyker/opers.cpp:554-575)."""
+    _dev(array)
+    if code not in ("rank", "rate"):
+        raise RuntimeError("Coding type %s is not supported." % code)
+    a = array if array.dim() >= 3 else array.reshape((1,) * (3 - array.dim()) + tuple(array.shape))
+    n, t = a.shape[0], a.shape[1]
+    out = torch.empty((n,) + tuple(a.shape[2:]), dtype=_dtype(dtype), device=a.device)
+    ln = out[0].numel() if n else 0
+    if code == "rank":
+        call("spk_rank_gather", _ptr(a), _type(a), _ptr(out), _type(out), n, t, ln, float(threshold), _stream(a))
+    else:
+        call("spk_rate_gather", _ptr(a), _type(a), _ptr(out), _type(out), n, t, ln, _stream(a))
+    return out
+
+
+def scatter(array: torch.Tensor, time: int, dtype="u8") -> torch.Tensor:
+    """Spike trains from first-spike times: [N, ...] -> [N, time, ...], step s spikes where
+    s >= the time (src/python/spyker/module.py:1145-1174, src/spyker/opers.cpp:577-598)."""
+    _dev(array)
+    if time <= 0:
+        raise RuntimeError("Input time can't be less than 1.")
+    a = array if array.dim() >= 2 else array.reshape(1, -1)
+    n = a.shape[0]
+    out = torch.empty((n, int(time)) + tuple(a.shape[1:]), dtype=_dtype(dtype), device=a.device)
+    ln = a[0].numel() if n else 0
+    call("spk_rank_scatter", _ptr(a), _type(a), _ptr(out), _type(out), n, ln, int(time), _stream(a))
     return out
 
 

@@ -57,6 +57,9 @@ SIGNATURES = {
     "spk_rank_code": [_p, _i32, _p, _i32, _i64, _i64, _i64, _i32, _p],
     "spk_rank_convstdp": [_p, _p, _i32, _p] + [_i64] * 10 + [_p, _i64, _p, _p, _i64] + [_i64] * 6 + [_p],
     "spk_rank_fcstdp": [_p, _p, _i32, _p, _i64, _i64, _i64, _i64, _p, _i64, _p, _p, _i64, _p],
+    "spk_rank_gather": [_p, _i32, _p, _i32, _i64, _i64, _i64, _f64, _p],
+    "spk_rank_scatter": [_p, _i32, _p, _i32, _i64, _i64, _i64, _p],
+    "spk_rate_gather": [_p, _i32, _p, _i32, _i64, _i64, _i64, _p],
     "spk_sparse_release": [_p],
     "spk_sparse_from_dense": [_p, _i32, _i64, _i64, _i64, _i64, _i64, _f64, _p, _p],
     "spk_sparse_code": [_p, _i32, _i64, _i64, _i64, _i64, _i64, _i32, _p, _p],

@@ -124,6 +124,22 @@ def main():
     v["layer_kernel_after"] = R.convstdp(spk, kern, so, [(0.004, -0.003)], w, c, 1, (2, 2, 2, 2))
     v["layer_pool"] = R.pool(so, 2, 2, (0, 0, 0, 0))
 
+    # readout (cpu/time.cpp): first-spike times of integer and float inputs, times back to
+    # cumulative spikes, and the rate code's last step
+    rd = (rng.random((4, 9, 3, 6, 5)) < 0.25).astype(np.uint8) * rng.integers(1, 5, (4, 9, 3, 6, 5)).astype(np.uint8)
+    v["readout_in_u8"] = rd
+    for th in (0, 2):
+        v["rank_gather_u8_th%d" % th] = R.rank_gather(rd, th)
+    rf = rng.normal(0.0, 1.0, (4, 9, 40)).astype(np.float32)
+    v["readout_in_f32"] = rf
+    v["rank_gather_f32_th05"] = R.rank_gather(rf, 0.5)
+    v["rate_gather_f32"] = R.rate_gather(rf)
+    times_u8 = rng.integers(0, 12, (4, 3, 6, 5)).astype(np.uint8)
+    times_f32 = (rng.random((4, 40)) * 12 - 1.5).astype(np.float32)
+    v["scatter_in_u8"], v["scatter_in_f32"] = times_u8, times_f32
+    v["rank_scatter_u8_T10"] = R.rank_scatter(times_u8, 10)
+    v["rank_scatter_f32_T10"] = R.rank_scatter(times_f32, 10)
+
     np.savez_compressed(OUT, **v)
     print("wrote", OUT, os.path.getsize(OUT), "bytes,", len(v), "arrays")
 

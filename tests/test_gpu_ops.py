@@ -203,3 +203,59 @@ def test_layer_reference_vector(dev, golden):
     conv.stdp(x, win, out)
     assert np.array_equal(H(conv.kernel), g["layer_kernel_after"])
     assert np.array_equal(H(spk.pool(out, 2)), g["layer_pool"])
+
+
+def test_readout_gather_scatter(dev, golden, orc):
+    """rank/rate gather and rank scatter (cpu/time.cpp) against the reference vectors and the
+    oracle, over input and output types."""
+    rd, rf = golden["readout_in_u8"], golden["readout_in_f32"]
+    for th in (0, 2):
+        for dt in ("u8", "u16", "i32", "f32"):
+            got = H(spk.gather(T(rd, dev), th, dt)).astype(np.int64)
+            assert np.array_equal(got, golden["rank_gather_u8_th%d" % th]), (th, dt)
+    assert np.array_equal(H(spk.gather(T(rf, dev), 0.5, "i32")), golden["rank_gather_f32_th05"])
+    assert np.array_equal(H(spk.gather(T(rf, dev), code="rate", dtype="f32")), golden["rate_gather_f32"])
+    tu, tf = golden["scatter_in_u8"], golden["scatter_in_f32"]
+    assert np.array_equal(H(spk.scatter(T(tu, dev), 10)), golden["rank_scatter_u8_T10"])
+    assert np.array_equal(H(spk.scatter(T(tf, dev), 10)), golden["rank_scatter_f32_T10"])
+    assert np.array_equal(H(spk.scatter(T(tf, dev), 10, "f32")), golden["rank_scatter_f32_T10"].astype(np.float32))
+    # larger, and a round trip through the layer's own spikes
+    rng = np.random.default_rng(41)
+    s = np.maximum.accumulate((rng.random((16, 15, 30, 14, 14)) < 0.05).astype(np.uint8), axis=1)
+    first = H(spk.gather(T(s, dev)))
+    assert np.array_equal(first.astype(np.int32), orc.rank_gather(s))
+    assert np.array_equal(H(spk.scatter(T(first, dev), 15)), s)
+    neg = np.array([[-3, 0, 2, 20]], np.int32)  # negative times compare as large unsigned values
+    assert np.array_equal(H(spk.scatter(T(neg, dev), 4))[0, :, 0], np.zeros(4, np.uint8))
+
+
+def test_two_layer_forward_features(dev, orc):
+    """The inference forward the examples feed to a classifier
+    (examples/rezghi_mnist_dense.py:82-85): pool(fire(conv1(x), th1), 2),
+    pool(fire(conv2(.), th2), 3), then T - gather(.), flattened; equal to the
+    oracle's composition of the same reference functions."""
+    from spyker_amd import synthetic
+    n, t = 6, 15
+    img = synthetic.strokes(n, 28, seed=43).astype(np.float32) / 255
+    d3 = spk.DoG(1, [spk.DoGFilter(1, 2), spk.DoGFilter(2, 1)], pad=1, device=dev)
+    d7 = spk.DoG(3, [spk.DoGFilter(1, 2), spk.DoGFilter(2, 1)], pad=3, device=dev)
+    xi = T(img, dev).reshape(n, 1, 28, 28)
+    f = torch.cat([d3(xi), d7(xi)], dim=1).contiguous()
+    spk.threshold(f, 0.01)
+    x = spk.code(f, t)
+    rng = np.random.default_rng(44)
+    k1 = np.clip(rng.normal(0.5, 0.02, (20, 4, 5, 5)), 0, 1).astype(np.float32)
+    k2 = np.clip(rng.normal(0.5, 0.02, (40, 20, 3, 3)), 0, 1).astype(np.float32)
+    c1, c2 = spk.Conv(4, 20, 5, pad=2, device=dev), spk.Conv(20, 40, 3, pad=1, device=dev)
+    c1.kernel, c2.kernel = T(k1, dev), T(k2, dev)
+    a = spk.pool(spk.fire(c1(x), 16.0), 2)
+    a = spk.pool(spk.fire(c2(a), 5.0), 3)
+    feats = (t - spk.gather(a).to(torch.int32)).flatten(1)
+    xo = H(x)
+    o = orc.conv(xo.reshape(n * t, 4, 28, 28), k1, 1, (2, 2, 2, 2)).reshape(n, t, 20, 28, 28)
+    o = orc.pool(orc.fire(o, 16.0), 2, 2)
+    o = orc.conv(o.reshape(n * t, 20, 14, 14), k2, 1, (1, 1, 1, 1)).reshape(n, t, 40, 14, 14)
+    o = orc.pool(orc.fire(o, 5.0), 3, 3)
+    want = (t - orc.rank_gather(o)).reshape(n, -1)
+    assert 0 < (want > 0).mean() < 1
+    assert np.array_equal(H(feats), want)

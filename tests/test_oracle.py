@@ -266,3 +266,27 @@ def test_oracle_vs_live_reference(orc, ref):
     assert np.array_equal(orc.inhibit(th), ref.inhibit(th))
     a, b = orc.convwta(orc.inhibit(th), 1, 4), ref.convwta(ref.inhibit(th), 1, 4)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_readout_gather_scatter(orc, golden):
+    """rank/rate gather and rank scatter (cpu/time.cpp): reference vectors, and the
+    round trips that fix them: scatter(t) is the cumulative spike train of t, and
+    gather of a cumulative {0,1} train is its first spike."""
+    rd, rf = golden["readout_in_u8"], golden["readout_in_f32"]
+    for th in (0, 2):
+        assert np.array_equal(orc.rank_gather(rd, th), golden["rank_gather_u8_th%d" % th])
+    assert np.array_equal(orc.rank_gather(rf, 0.5), golden["rank_gather_f32_th05"])
+    assert np.array_equal(orc.rate_gather(rf), golden["rate_gather_f32"])
+    tu, tf = golden["scatter_in_u8"], golden["scatter_in_f32"]
+    assert np.array_equal(orc.rank_scatter(tu, 10), golden["rank_scatter_u8_T10"])
+    assert np.array_equal(orc.rank_scatter(tf, 10), golden["rank_scatter_f32_T10"])
+    # closed forms
+    first = np.where((rd > 2).any(axis=1), (rd > 2).argmax(axis=1), rd.shape[1])
+    assert np.array_equal(orc.rank_gather(rd, 2), first)
+    steps = np.arange(10).reshape(1, 10, 1)
+    assert np.array_equal(orc.rank_scatter(tf, 10), (steps >= tf[:, None, :]).astype(np.uint8))
+    # round trips
+    cum = orc.rank_scatter(tu, 10)
+    assert np.array_equal(orc.rank_gather(cum), np.minimum(tu, 10))
+    assert np.array_equal(orc.rank_scatter(orc.rank_gather(cum).astype(np.uint8), 10), cum)
+    assert np.array_equal(orc.rate_gather(rf), rf[:, -1])
