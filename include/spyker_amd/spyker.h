@@ -251,6 +251,8 @@ public:
     Tensor operator()(Tensor input);
 };
 
+class SparseTensor;
+
 /// 2D convolution layer with rank STDP.
 class Conv
 {
@@ -271,6 +273,11 @@ public:
     Tensor operator()(Tensor input, Tensor output);
     Tensor operator()(Tensor input);
     void stdp(Tensor input, const Winners &winners, Tensor output);
+    /// Sparse conv of spike lists (opers.cpp:866-871); the layer and the lists on the GPU.
+    SparseTensor operator()(SparseTensor input, F64 threshold);
+    /// STDP from spike lists (opers.cpp:882-889): potentiate where the input's last event
+    /// is at a step <= the winner's, the sparse path's F32 rule.
+    void stdp(SparseTensor input, const Winners &winners);
     Device device() const { return _device; }
     Expand2 stride() const { return _stride; }
     Expand4 pad() const { return _pad; }
@@ -317,6 +324,62 @@ Tensor pool(Tensor input, Tensor output, Expand2 kernel, Expand2 stride = 0, Exp
 Tensor inhibit(Tensor input, F64 threshold = 0, bool inplace = true);
 Winners fcwta(Tensor input, Size radius, Size count, F64 threshold = 0);
 Winners convwta(Tensor input, Expand2 radius, Size count, F64 threshold = 0);
+
+// -- readout (opers.h:662-699) ----------------------------------------------
+
+/// First spike step of every element (Code::Rank), or the last step (Code::Rate):
+/// [N, T, ...] -> [N, ...].
+Tensor gather(Tensor input, F64 threshold = 0, Type type = Type::U8, Code code = Code::Rank);
+Tensor gather(Tensor input, Tensor output, F64 threshold = 0, Code code = Code::Rank);
+/// Spike trains from first-spike times: [N, ...] -> [N, time, ...].
+Tensor scatter(Tensor input, Size time, Type type = Type::U8);
+Tensor scatter(Tensor input, Tensor output);
+
+// -- sparse event path (utils.h:369-420, opers.h:763-828) ----------------------
+
+/// Spike lists of a [N, T, C, H, W] tensor on the GPU: per sample and step the
+/// spikes that are new at that step (base.h:493-528), C, H, W <= 255.
+class SparseTensor
+{
+    std::shared_ptr<void> _data;  // the spk_sparse descriptor; its lists are released with it
+    Device _device;
+
+public:
+    SparseTensor();
+    /// Events at each element's first step with a value > threshold (U8 or F32 input).
+    explicit SparseTensor(Tensor tensor, F64 threshold = 0);
+    /// Adopts a descriptor filled by the C ABI (the Sparse functions use it).
+    SparseTensor(std::shared_ptr<void> descriptor, Device device);
+
+    explicit operator bool() const { return bool(_data); }
+    void *data() const { return _data.get(); }
+    Size dims() const;
+    /// Number of events.
+    Size numel() const;
+    Size bytes() const;
+    Shape shape() const;
+    Size shape(Size index) const;
+    F64 sparsity() const;
+    Device device() const { return _device; }
+    SparseTensor copy() const;
+    /// Cumulative U8 spikes [N, T, C, H, W].
+    Tensor dense() const;
+};
+
+namespace Sparse
+{
+SparseTensor conv(SparseTensor input, Tensor kernel, F64 threshold, Expand2 stride = 1, Expand4 pad = 0);
+SparseTensor pad(SparseTensor input, Expand4 pad);
+SparseTensor code(Tensor input, Size time, bool sort = true);
+Tensor gather(SparseTensor input, Type type = Type::U8);
+Tensor gather(SparseTensor input, Tensor output);
+SparseTensor pool(SparseTensor input, Expand2 kernel, Expand2 stride = 0, Expand4 pad = 0);
+SparseTensor inhibit(SparseTensor input);
+Winners convwta(SparseTensor input, Expand2 radius, Size count);
+/// No single reference call: the winners of fcwta(input.dense()), as conv
+/// coordinates with their spike step (Winner{0, t, z, y, x}).
+Winners fcwta(SparseTensor input, Size radius, Size count);
+}  // namespace Sparse
 
 // -- fused layer step (no single counterpart in the reference) -------------
 

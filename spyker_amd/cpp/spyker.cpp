@@ -834,6 +834,258 @@ void FC::stdp(Tensor input, const Winners &winners, Tensor output)
           "Interface::STDP");
 }
 
+// -- readout (opers.cpp:554-598) ---------------------------------------------
+
+Tensor gather(Tensor input, F64 threshold, Type type, Code code)
+{
+    input = atLeast(input, 3, "3D");
+    Shape s = input.shape();
+    s.erase(s.begin() + 1);
+    return gather(input, Tensor(input.device(), type, s), threshold, code);
+}
+
+Tensor gather(Tensor input, Tensor output, F64 threshold, Code code)
+{
+    input = atLeast(input, 3, "3D"), output = atLeast(output, 2, "2D");
+    onGpu({input, output}, "Interface::Gather");
+    Shape s = input.shape();
+    s.erase(s.begin() + 1);
+    REQUIRE(output.shape() == s, "Interface::Gather", "Tensor shapes are not compatible.");
+    const Size n = input.shape(0), t = input.shape(1), len = n ? output.numel() / n : 0;
+    if (code == Code::Rank)
+        check(spk_rank_gather(input.data(), ty(input.type()), output.data(), ty(output.type()), n, t, len, threshold,
+                              nullptr),
+              "Interface::Gather");
+    else
+        check(spk_rate_gather(input.data(), ty(input.type()), output.data(), ty(output.type()), n, t, len, nullptr),
+              "Interface::Gather");
+    return output;
+}
+
+Tensor scatter(Tensor input, Size time, Type type)
+{
+    input = atLeast(input, 2, "2D");
+    REQUIRE(time > 0, "Interface::Scatter", "Input time can't be less than 1.");
+    Shape s = input.shape();
+    s.insert(s.begin() + 1, time);
+    return scatter(input, Tensor(input.device(), type, s));
+}
+
+Tensor scatter(Tensor input, Tensor output)
+{
+    input = atLeast(input, 2, "2D"), output = atLeast(output, 3, "3D");
+    onGpu({input, output}, "Interface::Scatter");
+    Shape s = input.shape();
+    s.insert(s.begin() + 1, output.shape(1));
+    REQUIRE(output.shape() == s, "Interface::Scatter", "Tensor shapes are not compatible.");
+    const Size n = input.shape(0), len = n ? input.numel() / n : 0;
+    check(spk_rank_scatter(input.data(), ty(input.type()), output.data(), ty(output.type()), n, len, output.shape(1),
+                           nullptr),
+          "Interface::Scatter");
+    return output;
+}
+
+// -- sparse event path (utils.cpp:206-257, opers.cpp:866-889,982-1043) ----------
+
+namespace
+{
+std::shared_ptr<void> newSparse()
+{
+    return std::shared_ptr<void>(new spk_sparse{}, [](void *p) {
+        auto *d = static_cast<spk_sparse *>(p);
+        (void)spk_sparse_release(d);
+        delete d;
+    });
+}
+
+const spk_sparse &desc(const SparseTensor &t, const char *scope)
+{
+    REQUIRE(bool(t), scope, "Sparse can't be used without initialization.");
+    return *static_cast<const spk_sparse *>(t.data());
+}
+
+spk_sparse *out(std::shared_ptr<void> &d) { return static_cast<spk_sparse *>(d.get()); }
+}  // namespace
+
+SparseTensor::SparseTensor() {}
+
+SparseTensor::SparseTensor(std::shared_ptr<void> descriptor, Device device) : _data(descriptor), _device(device) {}
+
+SparseTensor::SparseTensor(Tensor tensor, F64 threshold) : _device(tensor.device())
+{
+    tensor = exactly(tensor, 5, "5D");
+    onGpu({tensor}, "Interface::Sparse");
+    auto d = newSparse();
+    check(spk_sparse_from_dense(tensor.data(), ty(tensor.type()), tensor.shape(0), tensor.shape(1), tensor.shape(2),
+                                tensor.shape(3), tensor.shape(4), threshold, out(d), nullptr),
+          "Interface::Sparse");
+    _data = d;
+}
+
+Size SparseTensor::dims() const { return _data ? 5 : 0; }
+
+Size SparseTensor::numel() const { return desc(*this, "Interface::Sparse").total; }
+
+Size SparseTensor::bytes() const
+{
+    const spk_sparse &d = desc(*this, "Interface::Sparse");
+    return (d.n * d.t + 1) * 8 + d.total * 4;
+}
+
+Shape SparseTensor::shape() const
+{
+    if (!_data) return {};
+    const spk_sparse &d = desc(*this, "Interface::Sparse");
+    return {d.n, d.t, d.c, d.h, d.w};
+}
+
+Size SparseTensor::shape(Size index) const
+{
+    const Shape s = shape();
+    REQUIRE(index >= 0 && index < Size(s.size()), "Interface::Sparse", "Given index is out of range.");
+    return s[size_t(index)];
+}
+
+F64 SparseTensor::sparsity() const
+{
+    const Shape s = shape();
+    Size total = 1;
+    for (Size v : s) total *= v;
+    return F64(numel()) / F64(total);
+}
+
+SparseTensor SparseTensor::copy() const { return Sparse::pad(*this, 0); }
+
+Tensor SparseTensor::dense() const
+{
+    const spk_sparse &d = desc(*this, "Interface::Sparse");
+    Tensor output(_device, Type::U8, shape());
+    check(spk_sparse_to_dense(&d, output.data(), nullptr), "Interface::Sparse");
+    return output;
+}
+
+namespace Sparse
+{
+SparseTensor conv(SparseTensor input, Tensor kernel, F64 threshold, Expand2 stride, Expand4 pad)
+{
+    const spk_sparse &d = desc(input, "Interface::Conv");
+    checkItems("Stride", stride.get(), {1, 1}), checkItems("Pad", pad.get(), {0, 0, 0, 0});
+    kernel = to4(kernel);
+    onGpu({kernel}, "Interface::Conv");
+    REQUIRE(kernel.shape(1) == d.c, "Interface::Conv", "Input and kernel shapes are not compatible.");
+    auto o = newSparse();
+    check(spk_sparse_conv(&d, kernel.data(), ty(kernel.type()), kernel.shape(0), kernel.shape(2), kernel.shape(3),
+                          threshold, stride[0], stride[1], pad[0], pad[1], pad[2], pad[3], out(o), nullptr),
+          "Interface::Conv");
+    return SparseTensor(o, input.device());
+}
+
+SparseTensor pad(SparseTensor input, Expand4 pad)
+{
+    const spk_sparse &d = desc(input, "Interface::Pad");
+    checkItems("Pad", pad.get(), {0, 0, 0, 0});
+    auto o = newSparse();
+    check(spk_sparse_pad(&d, pad[0], pad[1], pad[2], pad[3], out(o), nullptr), "Interface::Pad");
+    return SparseTensor(o, input.device());
+}
+
+SparseTensor code(Tensor input, Size time, bool sort)
+{
+    if (input.dims() == 3) input = input.reshape({1, input.shape(0), input.shape(1), input.shape(2)});
+    input = to4(input);
+    onGpu({input}, "Interface::RateCvt");
+    REQUIRE(time > 0, "Interface::RateCvt", "Input time can't be less than 1.");
+    auto o = newSparse();
+    check(spk_sparse_code(input.data(), ty(input.type()), input.shape(0), input.shape(1), input.shape(2),
+                          input.shape(3), time, sort ? 1 : 0, out(o), nullptr),
+          "Interface::RateCvt");
+    return SparseTensor(o, input.device());
+}
+
+Tensor gather(SparseTensor input, Type type)
+{
+    const Shape s = input.shape();
+    REQUIRE(s.size() == 5, "Interface::Gather", "Sparse can't be used without initialization.");
+    return gather(input, Tensor(input.device(), type, {s[0], s[2], s[3], s[4]}));
+}
+
+Tensor gather(SparseTensor input, Tensor output)
+{
+    const spk_sparse &d = desc(input, "Interface::Gather");
+    onGpu({output}, "Interface::Gather");
+    REQUIRE(output.shape() == (Shape{d.n, d.c, d.h, d.w}), "Interface::Gather", "Tensor shapes are not compatible.");
+    check(spk_sparse_gather(&d, output.data(), ty(output.type()), nullptr), "Interface::Gather");
+    return output;
+}
+
+SparseTensor pool(SparseTensor input, Expand2 kernel, Expand2 stride, Expand4 pad)
+{
+    const spk_sparse &d = desc(input, "Interface::Pool");
+    if (stride.get() == Shape{0, 0}) stride = kernel;
+    checkItems("Kernel", kernel.get(), {1, 1}), checkItems("Stride", stride.get(), {1, 1});
+    checkItems("Pad", pad.get(), {0, 0, 0, 0});
+    auto o = newSparse();
+    check(spk_sparse_pool(&d, kernel[0], kernel[1], stride[0], stride[1], pad[0], pad[1], pad[2], pad[3], out(o),
+                          nullptr),
+          "Interface::Pool");
+    return SparseTensor(o, input.device());
+}
+
+SparseTensor inhibit(SparseTensor input)
+{
+    const spk_sparse &d = desc(input, "Interface::Inhibit");
+    auto o = newSparse();
+    check(spk_sparse_inhibit(&d, out(o), nullptr), "Interface::Inhibit");
+    return SparseTensor(o, input.device());
+}
+
+Winners convwta(SparseTensor input, Expand2 radius, Size count)
+{
+    const spk_sparse &d = desc(input, "Interface::ConvWTA");
+    checkItems("Radius", radius.get(), {0, 0});
+    REQUIRE(count > 0, "Interface::ConvWTA", "Winners count can't be less than 1.");
+    Tensor data(input.device(), Type::I64, {d.n, count, 5}), counts(input.device(), Type::I64, {d.n});
+    check(spk_sparse_convwta(&d, radius[0], radius[1], count, data.data<spk_winner>(), counts.data<int64_t>(),
+                             nullptr),
+          "Interface::ConvWTA");
+    return download(data, counts);
+}
+
+Winners fcwta(SparseTensor input, Size radius, Size count)
+{
+    const spk_sparse &d = desc(input, "Interface::FCWTA");
+    REQUIRE(radius >= 0, "Interface::FCWTA", "Radius can't be less than 0.");
+    REQUIRE(count > 0, "Interface::FCWTA", "Winners count can't be less than 1.");
+    Tensor data(input.device(), Type::I64, {d.n, count, 5}), counts(input.device(), Type::I64, {d.n});
+    check(spk_sparse_fcwta(&d, radius, count, data.data<spk_winner>(), counts.data<int64_t>(), nullptr),
+          "Interface::FCWTA");
+    return download(data, counts);
+}
+}  // namespace Sparse
+
+SparseTensor Conv::operator()(SparseTensor input, F64 threshold)
+{
+    REQUIRE(_init, "Interface::Conv", "Layer is used before initializing.");
+    REQUIRE(input.device() == _device, "Interface::Conv", "Input device is not the same as layer's.");
+    return Sparse::conv(input, kernel, threshold, _stride, _pad);
+}
+
+void Conv::stdp(SparseTensor input, const Winners &winners)
+{
+    REQUIRE(!stdpconfig.empty(), "Interface::STDP", "At least one STDP Config must be given.");
+    REQUIRE(!winners.empty(), "Interface::STDP", "Winners can't be empty.");
+    REQUIRE(_init, "Interface::STDP", "Layer is used before initializing.");
+    REQUIRE(input.device() == _device, "Interface::Conv", "Input device is not the same as layer's.");
+    const spk_sparse &d = desc(input, "Interface::STDP");
+    onGpu({kernel}, "Interface::STDP");
+    const DeviceWinners w = upload(winners, _device);
+    const auto cfg = configs(stdpconfig);
+    check(spk_sparse_convstdp(&d, kernel.data(), ty(kernel.type()), kernel.shape(0), kernel.shape(2),
+                              kernel.shape(3), cfg.data(), Size(cfg.size()), w.data.data<spk_winner>(),
+                              w.counts.data<int64_t>(), w.cap, _pad[0], _pad[1], _pad[2], _pad[3], nullptr),
+          "Interface::STDP");
+}
+
 LayerStep layerStep(Conv &layer, Tensor input, F64 threshold, Expand2 radius, Size count, Expand2 pool,
                     Expand2 poolStride, bool learn)
 {

@@ -7,6 +7,10 @@
 //                          code, DoG, conv, threshold, inhibit, convwta,
 //                          fire, Conv::stdp, pool and layerStep on the GPU,
 //                          writes the results to DIR for the Python test
+//   api_test sparse DIR    reads DIR/{events.u8,k.f32,shape.txt}, runs the
+//                          sparse path (SparseTensor, Conv on lists, Sparse::
+//                          inhibit/convwta/pool/gather/fcwta, Conv::stdp on
+//                          lists) and the dense readout, writes the results
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -151,6 +155,42 @@ int layer(const std::string &dir)
     if (failures == 0) std::cout << "layer ok\n";
     return failures ? 1 : 0;
 }
+int sparse(const std::string &dir)
+{
+    Size n, t, h, w, cout, k, pad;
+    double th;
+    {
+        std::ifstream f(dir + "/shape.txt");
+        f >> n >> t >> h >> w >> cout >> k >> pad >> th;
+        if (!f) throw std::runtime_error("cannot read shape.txt");
+    }
+    const Device gpu(Kind::CUDA);
+    Tensor ev = readFile(dir + "/events.u8", Type::U8, {n, t, 2, h, w}).to(gpu);
+    SparseTensor s(ev);
+    expect(s.dims() == 5 && s.shape() == (Shape{n, t, 2, h, w}) && s.numel() > 0, "sparse tensor shape");
+    Conv layer(gpu, 2, cout, k, 1, pad);
+    layer.kernel = readFile(dir + "/k.f32", Type::F32, {cout, 2, k, k}).to(gpu);
+    layer.stdpconfig = {STDPConfig(0.004, -0.003)};
+    SparseTensor out = Sparse::inhibit(layer(s, th));
+    Winners win = Sparse::convwta(out, 2, 5);
+    layer.stdp(s, win);
+    writeFile(dir + "/out_dense.out", out.dense());
+    writeFile(dir + "/out_gather.out", Sparse::gather(out, Type::I32));
+    writeFile(dir + "/pooled_dense.out", Sparse::pool(out, 2, 0, 1).dense());
+    writeFile(dir + "/kernel.out", layer.kernel);
+    writeWinners(dir + "/winners.out", win);
+    writeWinners(dir + "/fcwta.out", Sparse::fcwta(out, 0, 8));
+    // dense readout of the densified output: first-spike times and back
+    Tensor d = out.dense();
+    Tensor times = gather(d, 0, Type::I32);
+    writeFile(dir + "/times.out", times);
+    writeFile(dir + "/scatter.out", scatter(times, t));
+    expect(s.copy().numel() == s.numel(), "sparse copy");
+    expectThrow([&] { Sparse::pool(out, 2, 1); }, "Stride must be equal to kernel", "sparse pool stride");
+    expectThrow([&] { Sparse::conv(s, layer.kernel, th, 2, pad); }, "Stride must be 1", "sparse conv stride");
+    if (failures == 0) std::cout << "sparse ok\n";
+    return failures ? 1 : 0;
+}
 }  // namespace
 
 int main(int argc, char **argv)
@@ -159,7 +199,8 @@ int main(int argc, char **argv)
     {
         if (argc >= 2 && std::string(argv[1]) == "host") return host();
         if (argc >= 3 && std::string(argv[1]) == "layer") return layer(argv[2]);
-        std::cerr << "usage: api_test host | api_test layer DIR\n";
+        if (argc >= 3 && std::string(argv[1]) == "sparse") return sparse(argv[2]);
+        std::cerr << "usage: api_test host | api_test layer DIR | api_test sparse DIR\n";
         return 2;
     }
     catch (const std::exception &e)

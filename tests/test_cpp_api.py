@@ -33,7 +33,10 @@ def test_cpp_library_exports_the_interface():
     for name in ["Spyker::Conv::stdp(", "Spyker::Conv::Conv(", "Spyker::DoG::DoG(", "Spyker::Gabor::Gabor(",
                  "Spyker::FC::stdp(", "Spyker::code(", "Spyker::fire(", "Spyker::threshold(", "Spyker::inhibit(",
                  "Spyker::convwta(", "Spyker::fcwta(", "Spyker::pool(", "Spyker::conv(", "Spyker::fc(",
-                 "Spyker::pad(", "Spyker::layerStep(", "Spyker::Tensor::to("]:
+                 "Spyker::pad(", "Spyker::layerStep(", "Spyker::Tensor::to(", "Spyker::gather(", "Spyker::scatter(",
+                 "Spyker::SparseTensor::SparseTensor(", "Spyker::SparseTensor::dense(", "Spyker::Sparse::conv(",
+                 "Spyker::Sparse::pool(", "Spyker::Sparse::inhibit(", "Spyker::Sparse::convwta(",
+                 "Spyker::Sparse::gather(", "Spyker::Sparse::code("]:
         assert name in out, name
 
 
@@ -97,3 +100,45 @@ def test_cpp_layer_matches_oracle(tmp_path, orc):
     assert np.array_equal(rd("fused_pooled.out", np.uint8, (n, t, cout, ho // poolk, ho // poolk)),
                           orc.pool(oout, poolk))
     assert _winners(os.path.join(d, "fused_winners.out")) == want
+
+
+@pytest.mark.gpu
+def test_cpp_sparse_matches_oracle(tmp_path, orc):
+    from spyker_amd import synthetic
+    n, t, h, w, cout, k, pad, th = 3, 12, 40, 40, 24, 5, 2, 8.0
+    ev = synthetic.events(n, t, h, seed=22)
+    kern = np.clip(np.random.default_rng(22).normal(0.8, 0.05, (cout, 2, k, k)), 0, 1).astype(np.float32)
+    d = str(tmp_path)
+    ev.tofile(os.path.join(d, "events.u8"))
+    kern.tofile(os.path.join(d, "k.f32"))
+    with open(os.path.join(d, "shape.txt"), "w") as f:
+        f.write("%d %d %d %d %d %d %d %.17g\n" % (n, t, h, w, cout, k, pad, th))
+    r = subprocess.run([_driver(), "sparse", d], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "sparse ok" in r.stdout, r.stdout + r.stderr
+
+    def rd(name, dtype, shape):
+        return np.fromfile(os.path.join(d, name), dtype=dtype).reshape(shape)
+
+    s = orc.sparse_from_dense(ev)
+    out = orc.sparse_inhibit(orc.sparse_conv(s, kern, th, (pad,) * 4))
+    dense = orc.sparse_to_dense(out)
+    assert dense.any()
+    assert np.array_equal(rd("out_dense.out", np.uint8, dense.shape), dense)
+    assert np.array_equal(rd("out_gather.out", np.int32, (n, cout, h, w)), orc.sparse_gather(out).astype(np.int32))
+    pooled = orc.sparse_to_dense(orc.sparse_pool(out, 2, (1, 1, 1, 1)))
+    assert np.array_equal(rd("pooled_dense.out", np.uint8, pooled.shape), pooled)
+    ow, oc = orc.sparse_convwta(out, 2, 5)
+    want = [[tuple(int(ow[i, j][f]) for f in "ctzyx") for j in range(oc[i])] for i in range(n)]
+    assert _winners(os.path.join(d, "winners.out")) == want
+    assert np.array_equal(rd("kernel.out", np.float32, kern.shape),
+                          orc.sparse_stdp(s, kern, [(0.004, -0.003)], ow, oc, (pad,) * 4))
+    fw, fc = orc.fcwta(dense.reshape(n, t, -1), 0, 8)
+    fw = fw.reshape(n, 8)
+    first = orc.sparse_gather(out).reshape(n, -1)
+    got = _winners(os.path.join(d, "fcwta.out"))
+    for i in range(n):
+        assert [((z * h + y) * w + x, tt) for (_, tt, z, y, x) in got[i]] == \
+            [(int(fw[i, j]["x"]), int(first[i, fw[i, j]["x"]])) for j in range(fc[i])]
+    times = orc.rank_gather(dense)
+    assert np.array_equal(rd("times.out", np.int32, times.shape), times)
+    assert np.array_equal(rd("scatter.out", np.uint8, dense.shape), dense)
