@@ -298,11 +298,30 @@ def c3_bench(spk, dev, steps):
              C3_K, C3_K, 1, 1, 3, 3, 3, 3, spk._stream(xs))
     cp = spk.Conv(8, C3_COUT, C3_K, pad=3, device=dev)
     cp.kernel = kd.clone()
+    cp.stdpconfig = [spk.STDPConfig(0.004, -0.003)]
     p16 = torch.empty((m, C3_T, C3_COUT, C3_S, C3_S), dtype=torch.float32, device=dev)
-    spk.layer_step(cp, xs, C3_TH, 5, 10, learn=False, potentials=p16)
-    res["precision"] = {"samples": m,
-                        "threshold_flip_rate_vs_fp64": ((p16.double() > C3_TH) != (p64 > C3_TH)).double().mean().item(),
-                        "max_abs_potential_error_vs_fp64": (p16.double() - p64).abs().max().item()}
+    out16, win16, _ = spk.layer_step(cp, xs, C3_TH, 5, 10, learn=True, potentials=p16)
+    flips = ((p16.double() > C3_TH) != (p64 > C3_TH)).double().mean().item()
+    perr = (p16.double() - p64).abs().max().item()
+    del p64, p16
+    # the FP32 plain path (the reference's semantics) on the same samples and weights
+    cf = spk.Conv(8, C3_COUT, C3_K, pad=3, device=dev)
+    cf.kernel = kd.clone()
+    cf.stdpconfig = [spk.STDPConfig(0.004, -0.003)]
+    pot = cf(xs)
+    spk.threshold(pot, C3_TH)
+    spk.inhibit(pot)
+    winp = spk.convwta(pot, 5, 10)
+    outp = spk.fire(pot)
+    del pot
+    cf.stdp(xs, winp, outp)
+    wf, wp = win16.data[:, :, 2:].cpu(), winp.data[:, :, 2:].cpu()
+    cf_, cp_ = win16.counts.cpu(), winp.counts.cpu()
+    same = sum(int(cf_[i] == cp_[i] and bool((wf[i, :cf_[i]] == wp[i, :cp_[i]]).all())) for i in range(m))
+    res["precision"] = {"samples": m, "threshold_flip_rate_vs_fp64": flips, "max_abs_potential_error_vs_fp64": perr,
+                        "spike_mismatch_rate_vs_fp32_plain": (out16 != outp).double().mean().item(),
+                        "samples_with_identical_winners_vs_fp32_plain": same,
+                        "max_abs_weight_diff_vs_fp32_plain_after_step": (cp.kernel - cf.kernel).abs().max().item()}
     return res
 
 
