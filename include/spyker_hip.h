@@ -170,10 +170,39 @@ int32_t spk_fc(const void *in, int32_t in_type, const void *kernel, void *out, i
 int32_t spk_dog(const void *in, const void *kernel, void *out, int64_t planes, int64_t h, int64_t w, int64_t count,
                 int64_t k, int64_t pt, int64_t pl, int64_t pb, int64_t pr, void *stream);
 
+/* Core::cuda_log(Dyn4 input, Dyn4 kernel, Dyn4 output, Len4 pad), impl.h:208:
+ * in F32 [planes][h][w], kernel F32 [2][count][k][k], out F32
+ * [planes][2][count][ho][wo] with d = g1 - g2: [0] = max(d, 0), [1] = max(-d, 0). */
+int32_t spk_log(const void *in, const void *kernel, void *out, int64_t planes, int64_t h, int64_t w, int64_t count,
+                int64_t k, int64_t pt, int64_t pl, int64_t pb, int64_t pr, void *stream);
+
 /* Core::cuda_gabor(Dyn4 input, Dyn4 kernel, Dyn4 output, Len4 pad), impl.h:204:
  * in F32 [planes][h][w], kernel F32 [count][k][k], out F32 [planes][count][ho][wo]. */
 int32_t spk_gabor(const void *in, const void *kernel, void *out, int64_t planes, int64_t h, int64_t w, int64_t count,
                   int64_t k, int64_t pt, int64_t pl, int64_t pb, int64_t pr, void *stream);
+
+/* -- whitening ------------------------------------------------------------ */
+
+/* Core::zca_fit(Dyn2 input, Dyn1 mean, Dyn2 trans, Scalar epsilon, bool
+ * transform), impl.h:212 (the reference runs it on the host, cpu/zca.cpp:66-105):
+ * in F32 [rows][cols] (whitened in place when transform != 0), mean F32
+ * [cols], trans F32 [cols][cols] = U diag(1 / sqrt(max(0, l) + eps)) U^T of
+ * the covariance (divisor rows - 1), eps taken as F32.  Computed in FP64 on
+ * the device (rocBLAS/rocSOLVER, opened on first use); rows >= 2,
+ * cols <= 16384.  Synchronizes `stream` (the eigensolver's status is read). */
+int32_t spk_zca_fit(void *in, void *mean, void *trans, int64_t rows, int64_t cols, double epsilon, int32_t transform,
+                    void *stream);
+
+/* Core::zca_trans(Dyn2 input, Dyn1 mean, Dyn2 trans), impl.h:216
+ * (cpu/zca.cpp:107-121): in F32 [rows][cols] <- (in - mean) . T in place, T
+ * the symmetric matrix of trans's row-major upper triangle; FP64 products,
+ * rounded once to F32.  cols <= 16384. */
+int32_t spk_zca_trans(void *in, const void *mean, const void *trans, int64_t rows, int64_t cols, void *stream);
+
+/* Core::zca_split(Dyn2 input, Dyn3 output), impl.h:220 (cpu/zca.cpp:141-156):
+ * in [rows][cols] -> out [rows][2][cols] = max(x, 0), max(-x, 0) with C++'s
+ * conversions of -x to the element type.  Types: all eleven. */
+int32_t spk_zca_split(const void *in, void *out, int32_t type, int64_t rows, int64_t cols, void *stream);
 
 /* -- coding --------------------------------------------------------------- */
 
@@ -223,7 +252,7 @@ int32_t spk_rank_scatter(const void *in, int32_t in_type, void *out, int32_t out
 int32_t spk_rate_gather(const void *in, int32_t in_type, void *out, int32_t out_type, int64_t n, int64_t t,
                         int64_t len, void *stream);
 
-/* -- sparse event path (Core::sparse_*, impl.h:328-356; sparse/*.cpp) ------- */
+/* -- sparse event path (Core::sparse_*, impl.h:328-356; src/spyker/sparse/) ------ */
 
 /* A sparse spike tensor of shape [n][t][c][h][w] holds, for every sample i
  * and step j, the list of spikes that are new at step j (base.h:493-528).

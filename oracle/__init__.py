@@ -420,6 +420,56 @@ class Oracle(_Lib):
                              I64(pl), I64(pb), I64(pr), _ptr(out), I32(os.cpu_count() or 1))
         return out
 
+    def log(self, x, kernel, pad):
+        """x: [N, C, H, W] float32, kernel [2, count, k, k] float32 -> [N, C*2*count, Ho, Wo]."""
+        x = np.ascontiguousarray(x, np.float32)
+        kernel = np.ascontiguousarray(kernel, np.float32)
+        n, c, h, w = x.shape
+        count, k = kernel.shape[1], kernel.shape[2]
+        pt, pl, pb, pr = pad
+        ho, wo = h + pt + pb - k + 1, w + pl + pr - k + 1
+        out = np.zeros((n, c * 2 * count, ho, wo), np.float32)
+        self.lib.orc_log_f32(_ptr(x), I64(n * c), I64(h), I64(w), _ptr(kernel), I64(count), I64(k), I64(pt),
+                             I64(pl), I64(pb), I64(pr), _ptr(out), I32(os.cpu_count() or 1))
+        return out
+
+    # -- ZCA (cpu/zca.cpp:66-156), numpy FP64 restatement ------------------
+    # The reference computes in its F32 through BLAS/LAPACK and is not built
+    # here (no BLAS/LAPACK in its build recipe); these are the same steps in
+    # FP64, pinned by the mathematics in tests/test_oracle.py.
+
+    @staticmethod
+    def zca_fit(x, epsilon):
+        """x: [rows, cols] -> (mean FP64 [cols], trans FP64 [cols, cols]), zca.cpp:66-97:
+        mean of the rows; sigma = mid^T mid / (rows - 1); trans = U diag(1 / sqrt(max(0, l) + eps)) U^T,
+        eps taken as the F32 value."""
+        x = np.asarray(x, np.float64).reshape(x.shape[0], -1)
+        mean = x.mean(axis=0)
+        mid = x - mean
+        sigma = mid.T @ mid / (x.shape[0] - 1)
+        lam, u = np.linalg.eigh(sigma)
+        s = 1.0 / np.sqrt(np.maximum(0.0, lam) + float(np.float32(epsilon)))
+        return mean, (u * s) @ u.T
+
+    @staticmethod
+    def zca_trans(x, mean, trans):
+        """(x - mean) . T in FP64, T the symmetric matrix of trans's row-major upper
+        triangle (zca.cpp:107-121, rusym)."""
+        x = np.asarray(x, np.float64).reshape(x.shape[0], -1)
+        t = np.triu(np.asarray(trans, np.float64))
+        t = t + np.triu(t, 1).T
+        return (x - np.asarray(mean, np.float64)) @ t
+
+    @staticmethod
+    def zca_split(x):
+        """[rows, ...] -> [rows, 2, ...] of max(x, 0) and max(-x, 0) in the element type
+        (zca.cpp:141-156; -x wraps for the integer types as C++'s conversion does)."""
+        x = np.asarray(x)
+        with np.errstate(over="ignore"):
+            neg = (-x).astype(x.dtype)
+        zero = x.dtype.type(0)
+        return np.stack([np.where(x < zero, zero, x), np.where(neg < zero, zero, neg)], axis=1)
+
     def gabor(self, x, kernel, pad):
         x = np.ascontiguousarray(x, np.float32)
         kernel = np.ascontiguousarray(kernel, np.float32)

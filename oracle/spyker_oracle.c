@@ -567,6 +567,23 @@ static void normalize(double *v, int64_t n)
 }
 
 /* src/spyker/cpu/init.cpp:51-62 */
+/* cpu/dog.cpp:62-84: the conv writes [planes][2][count][ho][wo]; then, per
+ * plane, d = [0] - [1] gives [0] = max(d, 0) and [1] = max(-d, 0). */
+void orc_log_f32(const float *in, int64_t planes, int64_t h, int64_t w, const float *kernel, int64_t count, int64_t k,
+                 int64_t pt, int64_t pl, int64_t pb, int64_t pr, float *out, int32_t threads)
+{
+    int64_t ho = h + pt + pb - k + 1, wo = w + pl + pr - k + 1, len = count * ho * wo;
+    orc_conv_f32(in, 0, planes, 1, h, w, kernel, 2 * count, k, k, 1, 1, pt, pl, pb, pr, out, threads);
+    for (int64_t i = 0; i < planes; ++i)
+        for (int64_t j = 0; j < len; ++j)
+        {
+            float *a = out + (i * 2) * len + j, *b = out + (i * 2 + 1) * len + j;
+            float d = *a - *b, e = -d;
+            *a = d < 0.0f ? 0.0f : d; /* std::max<T>(diff, 0) */
+            *b = e < 0.0f ? 0.0f : e; /* std::max<T>(-diff, 0) */
+        }
+}
+
 void orc_gaussian_kernel(double *kernel, int64_t k, double std)
 {
     int64_t size = k / 2;

@@ -99,6 +99,8 @@ void orc_gabor_f32(const float *in, int64_t planes, int64_t h, int64_t w, const 
                    int64_t k, int64_t pt, int64_t pl, int64_t pb, int64_t pr, float *out, int32_t threads);
 
 /* -- host kernel generators (src/spyker/cpu/init.cpp:38-77) --------------- */
+void orc_log_f32(const float *in, int64_t planes, int64_t h, int64_t w, const float *kernel, int64_t count, int64_t k,
+                 int64_t pt, int64_t pl, int64_t pb, int64_t pr, float *out, int32_t threads);
 void orc_gaussian_kernel(double *kernel, int64_t k, double std);
 void orc_gabor_kernel(double *kernel, int64_t k, double sigma, double theta, double gamma, double lambda, double psi);
 

@@ -7,6 +7,7 @@ Every computation runs in libspyker_hip.so through its C ABI
 (include/spyker_hip.h); nothing falls back to the CPU.
 
     DoG, Gabor, Conv, FC            module.py:174,253,537,665
+    LoG, ZCA                        module.py:332,400
     threshold, fire, code           module.py:923,1072,993
     pool, inhibit, convwta, fcwta   module.py:1177,1230,1293,1262
     conv, fc, pad                   module.py:835,801,882
@@ -28,8 +29,8 @@ from . import _lib
 from ._lib import call
 
 __all__ = [
-    "DoGFilter", "GaborFilter", "STDPConfig", "Winner", "Winners", "DoG", "Gabor", "Conv", "FC", "threshold",
-    "fire", "code", "pool", "inhibit", "convwta", "fcwta", "conv", "fc", "pad", "gather", "scatter", "random_seed",
+    "DoGFilter", "GaborFilter", "STDPConfig", "Winner", "Winners", "DoG", "LoG", "ZCA", "Gabor", "Conv", "FC",
+    "threshold", "fire", "code", "pool", "inhibit", "convwta", "fcwta", "conv", "fc", "pad", "gather", "scatter", "random_seed",
     "expand2", "expand4",
     "layer_step", "LayerPlan", "sparse", "SparseTensor", "to_sparse",
 ]
@@ -446,6 +447,113 @@ class DoG:
         out = torch.empty((n, c * count, ho, wo), dtype=self.kernel.dtype, device=a.device)
         call("spk_dog", _ptr(a), _ptr(self.kernel), _ptr(out), n * c, h, w, count, k, pt, pl, pb, pr, _stream(a))
         return out
+
+
+class LoG:
+    """Laplacian-of-Gaussian filter bank (src/spyker/opers.cpp:773-802): each std s gives the
+    Gaussian pair (s / sqrt 2, s * sqrt 2); an input channel yields [on, off] x count channels."""
+
+    def __init__(self, size: int, stds: Sequence[float], pad=0, device=None, dtype="f32"):
+        if size <= 0:
+            raise RuntimeError("Input size can't be less than 1.")
+        stds = [float(s) for s in stds]
+        if not stds:
+            raise RuntimeError("Input stds count can't be less then 1.")
+        k = 2 * size + 1
+        ker = np.zeros((2, len(stds), k, k), np.float64)
+        for i, s in enumerate(stds):
+            ker[0, i] = _host_kernel("spk_gaussian_kernel", (k, k), k, s / math.sqrt(2))
+            ker[1, i] = _host_kernel("spk_gaussian_kernel", (k, k), k, s * math.sqrt(2))
+        self._pad = expand4(pad)
+        self.kernel = torch.from_numpy(ker).to(_dtype(dtype)).to(_device(device))
+
+    def __call__(self, array: torch.Tensor) -> torch.Tensor:
+        _dev(array)
+        a = _to_f32(_exact(array, 4)).contiguous()
+        n, c, h, w = a.shape
+        count, k = self.kernel.shape[1], self.kernel.shape[2]
+        pt, pl, pb, pr = self._pad
+        ho, wo = h + pt + pb - k + 1, w + pl + pr - k + 1
+        if ho <= 0 or wo <= 0:
+            raise RuntimeError("Input and kernel shapes are not compatible.")
+        if self.kernel.dtype != torch.float32:
+            raise RuntimeError("LoG: the kernel must be F32.")
+        out = torch.empty((n, c * 2 * count, ho, wo), dtype=torch.float32, device=a.device)
+        call("spk_log", _ptr(a), _ptr(self.kernel), _ptr(out), n * c, h, w, count, k, pt, pl, pb, pr, _stream(a))
+        return out
+
+
+def _least2(t):
+    if t.dim() == 0:
+        raise RuntimeError("Input dimensions couldn't be viewed as at least 2D.")
+    return t.reshape(1, -1) if t.dim() == 1 else t
+
+
+def _zca_input(t):
+    _dev(t)
+    if t.dtype != torch.float32:
+        raise RuntimeError("Input data type must be F32.")
+    return t
+
+
+class ZCA:
+    """ZCA whitening (src/spyker/opers.cpp:805-831, src/python/spyker/module.py:400-530).
+
+    ``fit`` learns ``mean`` [features] and ``transform`` [features, features]
+    (F32 device tensors, computed in FP64) from the rows of an F32 array;
+    calling the module whitens arrays in place (or copies with inplace=False).
+    """
+
+    def __init__(self):
+        self.mean = None
+        self.transform = None
+
+    def fit(self, array: torch.Tensor, epsilon: float, transform: bool = False) -> "ZCA":
+        if array.dim() < 2:
+            raise RuntimeError("Input dimensions must be at least 2D.")
+        a = _zca_input(array)
+        rows, cols = a.shape[0], math.prod(a.shape[1:])
+        self.mean = torch.empty(cols, dtype=torch.float32, device=a.device)
+        self.transform = torch.empty((cols, cols), dtype=torch.float32, device=a.device)
+        call("spk_zca_fit", _ptr(a), _ptr(self.mean), _ptr(self.transform), rows, cols, float(epsilon),
+             int(bool(transform)), _stream(a))
+        return self
+
+    def __call__(self, *array: torch.Tensor, inplace: bool = True):
+        if self.mean is None or self.transform is None:
+            raise RuntimeError("Layer is not initialized.")
+        if not inplace:
+            array = [x.clone() for x in array]
+        for x in array:
+            a = _least2(_zca_input(x))
+            rows, cols = a.shape[0], math.prod(a.shape[1:])
+            if cols != self.mean.numel() or self.transform.shape != (cols, cols):
+                raise RuntimeError("Input and mean shapes are not compatible.")
+            if self.mean.device != a.device or self.transform.device != a.device:
+                raise RuntimeError("Tensor devices are not compatible.")
+            call("spk_zca_trans", _ptr(a), _ptr(self.mean), _ptr(self.transform), rows, cols, _stream(a))
+        return tuple(array) if len(array) > 1 else array[0]
+
+    @staticmethod
+    def split(array: torch.Tensor) -> torch.Tensor:
+        a = _least2(_dev(array))
+        shape = list(a.shape)
+        shape.insert(1, 2)
+        out = torch.empty(shape, dtype=a.dtype, device=a.device)
+        call("spk_zca_split", _ptr(a), _ptr(out), _type(a), a.shape[0], math.prod(a.shape[1:]), _stream(a))
+        return out
+
+    def save(self, path: str) -> None:
+        np.savez(path, mean=self.mean.cpu().numpy(), transform=self.transform.cpu().numpy())
+
+    def load(self, path: str, device=None) -> None:
+        data = np.load(path)
+        mean, trans = np.asarray(data["mean"], np.float32), np.asarray(data["transform"], np.float32)
+        if mean.ndim != 1 or trans.shape != (mean.shape[0], mean.shape[0]):
+            raise RuntimeError("Input and mean shapes are not compatible.")
+        dev = self.mean.device if self.mean is not None else _device(device)
+        self.mean = torch.from_numpy(mean).to(dev)
+        self.transform = torch.from_numpy(np.ascontiguousarray(trans)).to(dev)
 
 
 class Gabor:

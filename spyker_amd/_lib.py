@@ -54,6 +54,10 @@ SIGNATURES = {
     "spk_fc": [_p, _i32, _p, _p, _i64, _i64, _i64, _p],
     "spk_dog": [_p, _p, _p] + [_i64] * 9 + [_p],
     "spk_gabor": [_p, _p, _p] + [_i64] * 9 + [_p],
+    "spk_log": [_p, _p, _p] + [_i64] * 9 + [_p],
+    "spk_zca_fit": [_p, _p, _p, _i64, _i64, _f64, _i32, _p],
+    "spk_zca_trans": [_p, _p, _p, _i64, _i64, _p],
+    "spk_zca_split": [_p, _p, _i32, _i64, _i64, _p],
     "spk_rank_code": [_p, _i32, _p, _i32, _i64, _i64, _i64, _i32, _p],
     "spk_rank_convstdp": [_p, _p, _i32, _p] + [_i64] * 10 + [_p, _i64, _p, _p, _i64] + [_i64] * 6 + [_p],
     "spk_rank_fcstdp": [_p, _p, _i32, _p, _i64, _i64, _i64, _i64, _p, _i64, _p, _p, _i64, _p],

@@ -148,6 +148,21 @@ __global__ void dog_combine(const float *__restrict__ mid, float *__restrict__ o
     }
 }
 
+// LoG on/off pair (cpu/dog.cpp:62-84): with d = conv(std/sqrt2) - conv(std*sqrt2),
+// on = max(d, 0) and off = max(-d, 0) in std::max's form (a < b ? b : a, so a
+// zero difference keeps its sign).  mid and out are [planes][2][len].
+__global__ void log_combine(const float *__restrict__ mid, float *__restrict__ out, int64_t planes, int64_t len)
+{
+    const int64_t total = planes * len;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += int64_t(gridDim.x) * blockDim.x)
+    {
+        const int64_t p = i / len, j = i % len;
+        const float d = mid[(p * 2) * len + j] - mid[(p * 2 + 1) * len + j], e = -d;
+        out[(p * 2) * len + j] = d < 0.0f ? 0.0f : d;
+        out[(p * 2 + 1) * len + j] = e < 0.0f ? 0.0f : e;
+    }
+}
+
 // FC: kt is [fin][fout]; out[r][o] = fma chain over f in order.
 template <typename I>
 __global__ void fc_kernel(const I *__restrict__ in, const float *__restrict__ kt, float *__restrict__ out,
@@ -271,6 +286,22 @@ int32_t spk_dog(const void *in, const void *kernel, void *out, int64_t planes, i
         Scratch mid(planes * 2 * len * int64_t(sizeof(float)), s);
         conv_run<float, float>(in, (const float *)kernel, mid.ptr, planes, g, s);
         dog_combine<<<flat_grid(planes * len, 256), 256, 0, s>>>(mid.as<float>(), (float *)out, planes, len);
+        SPK_LAUNCHED();
+    });
+}
+
+int32_t spk_log(const void *in, const void *kernel, void *out, int64_t planes, int64_t h, int64_t w, int64_t count,
+                int64_t k, int64_t pt, int64_t pl, int64_t pb, int64_t pr, void *stream)
+{
+    return guard([&] {
+        check_conv(planes, 1, h, w, 2 * count, k, k, 1, 1, pt, pl, pb, pr);
+        if (planes == 0) return;
+        ConvGeom g = geometry(1, h, w, 2 * count, k, k, 1, 1, pt, pl, pb, pr);
+        hipStream_t s = S(stream);
+        const int64_t len = count * int64_t(g.ho) * g.wo;
+        Scratch mid(planes * 2 * len * int64_t(sizeof(float)), s);
+        conv_run<float, float>(in, (const float *)kernel, mid.ptr, planes, g, s);
+        log_combine<<<flat_grid(planes * len, 256), 256, 0, s>>>(mid.as<float>(), (float *)out, planes, len);
         SPK_LAUNCHED();
     });
 }

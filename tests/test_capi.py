@@ -91,6 +91,22 @@ def test_host_kernel_generators_match_reference(lib, golden):
     assert np.array_equal(_host(lib, "spk_normal_kernel", (64,), 64, 0.8, 0.05), golden["normal_seed7"])
 
 
+def test_log_kernel_bank(golden):
+    """LoG(size, stds) holds the Gaussians of std/sqrt 2 and std*sqrt 2 (opers.cpp:773-788)."""
+    import math
+    import spyker_amd as spk
+    lg = spk.LoG(3, [math.sqrt(2), 2 * math.sqrt(2)], device="cpu", dtype="f64")
+    k = lg.kernel.numpy()
+    assert k.shape == (2, 2, 7, 7)
+    assert np.array_equal(k[0, 0], golden["gauss_7_1"])  # sqrt2 / sqrt2 == 1 exactly
+    assert np.array_equal(k[0, 1], golden["gauss_7_2"])  # 2 sqrt2 / sqrt2 == 2 exactly
+    np.testing.assert_allclose(k[1, 0], golden["gauss_7_2"], rtol=1e-14, atol=0)
+    with pytest.raises(RuntimeError):
+        spk.LoG(0, [1.0], device="cpu")
+    with pytest.raises(RuntimeError):
+        spk.LoG(2, [], device="cpu")
+
+
 def test_python_front_end_shape_rules():
     import spyker_amd as spk
     assert spk.expand2(3) == (3, 3) and spk.expand2([2, 5]) == (2, 5)

@@ -139,6 +139,17 @@ def test_dog_gabor(dev, orc):
     assert np.array_equal(H(d(T(u8, dev))), orc.dog(u8.astype(np.float32), H(d.kernel), (3, 3, 3, 3)))
 
 
+def test_log(dev, orc):
+    rng = np.random.default_rng(9)
+    img = rng.random((4, 2, 28, 28)).astype(np.float32)
+    lg = spk.LoG(3, [1.0, 2.0, 3.0], pad=(3, 2, 1, 3), device=dev)
+    got = H(lg(T(img, dev)))
+    assert got.shape == (4, 2 * 2 * 3, 28 + 4 - 6, 28 + 5 - 6)
+    assert np.array_equal(got, orc.log(img, H(lg.kernel), (3, 2, 1, 3)))
+    u8 = (rng.random((1, 28, 28)) * 255).astype(np.uint8)  # 3-D input is one sample (to4)
+    assert np.array_equal(H(lg(T(u8, dev))), orc.log(u8.astype(np.float32)[None], H(lg.kernel), (3, 2, 1, 3)))
+
+
 def test_rank_code(dev, golden, orc):
     x = golden["code_in"]
     for t in (1, 15, 30):

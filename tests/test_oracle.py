@@ -243,6 +243,85 @@ def test_dog_gabor_against_float64(orc, golden):
     np.testing.assert_allclose(got, ref, atol=2e-5)
 
 
+def test_log_against_float64(orc, golden):
+    """LoG (cpu/dog.cpp:62-84): on - off is the Gaussian difference of an FP64
+    conv, on and off are non-negative and never both non-zero, and the two
+    halves land in [plane][2][count] order."""
+    rng = np.random.default_rng(8)
+    img = rng.random((3, 2, 19, 23)).astype(np.float32)
+    g1, g2 = golden["gauss_7_1"], golden["gauss_7_2"]
+    ker = np.stack([np.stack([g1, g2]), np.stack([g2, g1])]).astype(np.float32)  # [2, 2, 7, 7]
+    got = orc.log(img, ker, (3, 2, 1, 3))
+    assert got.shape == (3, 2 * 2 * 2, 17, 22)
+    mid = torch.nn.functional.conv2d(torch.nn.functional.pad(torch.from_numpy(img).double().reshape(6, 1, 19, 23),
+                                                              (2, 3, 3, 1)),
+                                     torch.from_numpy(ker.reshape(4, 1, 7, 7)).double()).numpy()
+    d = (mid[:, :2] - mid[:, 2:]).reshape(3, 2, 2, 17, 22)
+    g = got.reshape(3, 2, 2, 2, 17, 22)
+    on, off = g[:, :, 0], g[:, :, 1]
+    assert (on >= 0).all() and (off >= 0).all() and not (on * off).any()
+    np.testing.assert_allclose(on - off, d, atol=2e-6)
+    # filter 1 is filter 0 with the Gaussians swapped: its on is filter 0's off
+    np.testing.assert_allclose(on[:, :, 1], off[:, :, 0], atol=2e-6)
+    # DoG is LoG's on half
+    assert np.array_equal(orc.dog(img, ker, (3, 2, 1, 3)).reshape(3, 2, 2, 17, 22), on)
+
+
+def test_zca_fit_closed_forms(orc):
+    """ZCA fit (cpu/zca.cpp:66-97) against what the mathematics fixes."""
+    # one feature: mean, the (rows - 1) divisor and eps as F32
+    m, t = orc.zca_fit(np.array([[1.0], [2.0], [3.0], [4.0]]), 0.1)
+    assert m[0] == 2.5
+    np.testing.assert_allclose(t[0, 0], 1 / np.sqrt(5 / 3 + float(np.float32(0.1))), rtol=1e-14)
+    assert abs(t[0, 0] - 1 / np.sqrt(5 / 3 + 0.1)) > 1e-11  # F32 eps, not the double
+    # correlated features: trans is the unique symmetric positive-definite
+    # matrix whose square inverts the textbook covariance + eps I
+    rng = np.random.default_rng(10)
+    x = rng.normal(size=(500, 6)) @ rng.normal(size=(6, 6)) + rng.normal(size=6)
+    eps = 1e-3
+    m, t = orc.zca_fit(x, eps)
+    np.testing.assert_allclose(m, x.mean(axis=0), rtol=1e-13, atol=1e-13)
+    cov = np.cov(x, rowvar=False, ddof=1) + float(np.float32(eps)) * np.eye(6)
+    assert np.allclose(t, t.T, rtol=0, atol=1e-12)
+    assert np.linalg.eigvalsh(t).min() > 0
+    np.testing.assert_allclose(t @ t @ cov, np.eye(6), atol=1e-9)
+    # the transform whitens: the covariance of (x - mean) trans is cov0 (cov0 + eps I)^-1
+    y = orc.zca_trans(x, m, t)
+    cov0 = np.cov(x, rowvar=False, ddof=1)
+    np.testing.assert_allclose(np.cov(y, rowvar=False, ddof=1), cov0 @ np.linalg.inv(cov), atol=1e-9)
+    # a constant feature: its eigenvalue (0, or a rounding-level negative) clamps to 0 -> 1/sqrt(eps)
+    xc = x.copy()
+    xc[:, 2] = 7.0
+    m, t = orc.zca_fit(xc, 0.25)
+    assert m[2] == 7.0
+    np.testing.assert_allclose(t[2, 2], 2.0, rtol=1e-12)
+    np.testing.assert_allclose(np.delete(t[2], 2), 0, atol=1e-12)
+
+
+def test_zca_trans_reads_the_upper_triangle(orc):
+    rng = np.random.default_rng(11)
+    x = rng.normal(size=(40, 5))
+    a = rng.normal(size=(5, 5))
+    sym = a + a.T
+    mean = rng.normal(size=5)
+    np.testing.assert_allclose(orc.zca_trans(x, mean, sym), (x - mean) @ sym, rtol=1e-13, atol=1e-13)
+    garbage = sym.copy()
+    garbage[np.tril_indices(5, -1)] = 99.0
+    assert np.array_equal(orc.zca_trans(x, mean, garbage), orc.zca_trans(x, mean, sym))
+
+
+def test_zca_split_closed_forms(orc):
+    rng = np.random.default_rng(12)
+    x = rng.normal(size=(3, 4, 5)).astype(np.float32)
+    s = orc.zca_split(x)
+    assert s.shape == (3, 2, 4, 5)
+    assert np.array_equal(s[:, 0] - s[:, 1], x) and not (s[:, 0] * s[:, 1]).any()
+    assert (s >= 0).all()
+    # unsigned -x wraps (C++: T(-x)), signed -(-128) stays -128 in int8
+    assert orc.zca_split(np.array([[3, 0, 255]], np.uint8)).tolist() == [[[3, 0, 255], [253, 0, 1]]]
+    assert orc.zca_split(np.array([[-128, 5, -7]], np.int8)).tolist() == [[[0, 5, 0], [0, 0, 7]]]
+
+
 def test_host_kernels(orc, golden):
     for kk, sd in [(3, 1.0), (3, 2.0), (7, 1.0), (7, 2.0)]:
         g = orc.gaussian_kernel(kk, sd)
