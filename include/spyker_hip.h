@@ -212,6 +212,36 @@ int32_t spk_zca_split(const void *in, void *out, int32_t type, int64_t rows, int
 int32_t spk_rank_code(const void *in, int32_t in_type, void *out, int32_t out_type, int64_t n, int64_t len,
                       int64_t time, int32_t sort, void *stream);
 
+/* Core::cuda_rate_code(Dyn2 input, Dyn3 output, bool sort), impl.h:295, run
+ * as the reference's CPU algorithm (cpu/code.cpp:134-243): in F32 [n][len] ->
+ * out U8 or F32 [n][time][len], the spike count of each element so far.
+ * Element j's rank-code time r (latency, or sorted when sort=1) selects row
+ * min(r, time) of the Poisson table (spk_poisson_table with time + 1 rows);
+ * sample i draws from std::mt19937(rnd + i), rnd one draw of the generator
+ * that spk_random_seed seeds, taken when the call is made.  len <= 16384,
+ * time <= 4096, n < 65536.  The first call for a `time` builds and uploads
+ * its table (synchronous); later calls are asynchronous. */
+int32_t spk_rate_code(const void *in, int32_t in_type, void *out, int32_t out_type, int64_t n, int64_t len,
+                      int64_t time, int32_t sort, void *stream);
+
+/* Core::poisson_create(Size time, Size bins), impl.h:190 (cpu/init.cpp:120-149),
+ * on the host: table [rows][bins] U16, row i the intervals of mean i + 1.
+ * Each row's contents are the reference's; its order is a fixed permutation
+ * (the reference's std::rand shuffle differs from run to run). */
+int32_t spk_poisson_table(uint16_t *table, int64_t rows, int64_t bins);
+
+/* Core::cuda_poisson_clear, impl.h:126: frees the device tables that
+ * spk_rate_code caches per (device, time); waits for each device first. */
+int32_t spk_poisson_clear(void);
+
+/* Core::cuda_rate_fire(Dyn3 input, Dyn3 output, Scalar threshold), impl.h:299
+ * (cpu/fire.cpp:84-114): in [n][t][len] -> out [n][t][len] spike counts; per
+ * element a spike counts when value + threshold < x, and value becomes x
+ * (value starts at 0, threshold converted to the input type).  Types (in and
+ * out): U8, U16, I32, F32. */
+int32_t spk_rate_fire(const void *in, int32_t in_type, void *out, int32_t out_type, int64_t n, int64_t t,
+                      int64_t len, double threshold, void *stream);
+
 /* -- learning ------------------------------------------------------------- */
 
 /* Core::cuda_rank_convstdp(Dyn5 input, Dyn4 kernel, Dyn5 output, configs,

@@ -196,6 +196,33 @@ class _Lib:
             self._check(rc, "rank_scatter")
         return out
 
+    def rate_code(self, x, time, sort, rnd, table):
+        """x: [N, L] float32 -> [N, time, L] uint8 spike counts of the Poisson rate code
+        (cpu/code.cpp:168-230): sample i draws from std::mt19937(rnd + i), intervals from
+        `table` [rows, bins] uint16 at the element's rank-code time row."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        table = np.ascontiguousarray(table, dtype=np.uint16)
+        n, l = x.shape
+        out = np.zeros((n, time, l), np.uint8)
+        if self.prefix == "ref_":
+            assert table.shape[1] == 1024
+            rc = self._f("rate_code_f32")(_ptr(x), I64(n), I64(l), I64(time), I32(int(sort)), C.c_uint64(rnd),
+                                          _ptr(table), I64(table.shape[0]), _ptr(out))
+        else:
+            rc = self._f("rate_code_f32")(_ptr(x), I64(n), I64(l), I64(time), I32(int(sort)), C.c_uint64(rnd),
+                                          _ptr(table), I64(table.shape[0]), I64(table.shape[1]), _ptr(out))
+        if rc != 0:
+            raise RuntimeError("rate_code: a spike-time row is outside the table")
+        return out
+
+    def rate_fire(self, x, threshold):
+        """x: [N, T, ...] float32 -> uint8 [N, T, ...] spike counts (cpu/fire.cpp:84-114)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        n, t = x.shape[:2]
+        out = np.zeros(x.shape, np.uint8)
+        self._f("rate_fire_f32")(_ptr(x), I64(n), I64(t), I64(int(np.prod(x.shape[2:]))), F32(threshold), _ptr(out))
+        return out
+
     def rate_gather(self, x):
         """x: [N, T, ...] float32 -> float32 [N, ...]: the last step (cpu/time.cpp:86-92)."""
         x = np.ascontiguousarray(x, dtype=np.float32)
@@ -376,6 +403,12 @@ class Oracle(_Lib):
         super().__init__()
         self.lib.orc_stdp_update.restype = C.c_float
 
+    def mt_stream(self, seed, n):
+        """The first n outputs of std::mt19937(seed)."""
+        out = np.zeros(n, np.uint32)
+        self.lib.orc_mt_stream(C.c_uint32(seed), I64(n), _ptr(out))
+        return out
+
     def stdp_update(self, w, potentiate, config):
         cfg = configs_array([config])
         return self.lib.orc_stdp_update(F32(w), I32(int(potentiate)), _ptr(cfg))
@@ -490,6 +523,13 @@ class Reference(_Lib):
     def normal_kernel(self, n, mean, std, seed):
         out = np.zeros(n, np.float64)
         self._check(self.lib.ref_normal_kernel(_ptr(out), I64(n), F64(mean), F64(std), I64(seed)), "normal_kernel")
+        return out
+
+    def poisson_create(self, rows, bins=1024):
+        """Core::poisson_create (cpu/init.cpp:120-149): [rows, bins] uint16, rows in the
+        reference's (std::rand-shuffled) order."""
+        out = np.zeros((rows, bins), np.uint16)
+        self._check(self.lib.ref_poisson_create(I64(rows), I64(bins), _ptr(out)), "poisson_create")
         return out
 
 

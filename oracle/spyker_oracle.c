@@ -103,25 +103,146 @@ void orc_rank_code_sort_f32(const float *in, int64_t n, int64_t len, int64_t tim
     }
 }
 
-/* src/spyker/cpu/code.cpp:258-269 (latency); the subtraction from `end` is
- * contracted into one fused multiply-add in the reference build. */
+/* src/spyker/cpu/code.cpp:43-55 (latency times of one sample); the
+ * subtraction from `end` is contracted into one fused multiply-add in the
+ * reference build. */
+static void latency_times(const float *x, int64_t len, int64_t time, uint16_t *times)
+{
+    float mn = x[0], mx = x[0];
+    for (int64_t i = 0; i < len; ++i)
+    {
+        mn = x[i] < mn ? x[i] : mn; /* std::min(a,b) = b < a ? b : a */
+        mx = mx < x[i] ? x[i] : mx; /* std::max(a,b) = a < b ? b : a */
+    }
+    float end = (float)time + EPS, scale = end / (mx - mn);
+    for (int64_t i = 0; i < len; ++i) times[i] = f32_to_u16(fmaf(-scale, x[i] - mn, end));
+}
+
+/* src/spyker/cpu/code.cpp:258-269 (latency) */
 void orc_rank_code_latency_f32(const float *in, int64_t n, int64_t len, int64_t time, uint8_t *out)
 {
     uint16_t *times = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)len);
     for (int64_t s = 0; s < n; ++s)
     {
-        const float *x = in + s * len;
-        float mn = x[0], mx = x[0];
-        for (int64_t i = 0; i < len; ++i)
-        {
-            mn = x[i] < mn ? x[i] : mn; /* std::min(a,b) = b < a ? b : a */
-            mx = mx < x[i] ? x[i] : mx; /* std::max(a,b) = a < b ? b : a */
-        }
-        float end = (float)time + EPS, scale = end / (mx - mn);
-        for (int64_t i = 0; i < len; ++i) times[i] = f32_to_u16(fmaf(-scale, x[i] - mn, end));
+        latency_times(in + s * len, len, time, times);
         times_to_spikes(times, len, time, out + s * time * len);
     }
     free(times);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Rate (Poisson) coding                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* MT19937 (Matsumoto & Nishimura 1998), the std::mt19937 the reference draws
+ * from (code.cpp:140 `using Random = std::mt19937`): 32-bit words, n = 624,
+ * m = 397, seeded as the standard's seed(value). */
+typedef struct
+{
+    uint32_t x[624];
+    int p;
+} orc_mt;
+
+void orc_mt_seed(orc_mt *g, uint32_t seed)
+{
+    g->x[0] = seed;
+    for (int i = 1; i < 624; ++i) g->x[i] = 1812433253u * (g->x[i - 1] ^ (g->x[i - 1] >> 30)) + (uint32_t)i;
+    g->p = 624;
+}
+
+uint32_t orc_mt_next(orc_mt *g)
+{
+    if (g->p >= 624)
+    {
+        for (int k = 0; k < 624; ++k)
+        {
+            uint32_t y = (g->x[k] & 0x80000000u) | (g->x[(k + 1) % 624] & 0x7fffffffu);
+            g->x[k] = g->x[(k + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+        }
+        g->p = 0;
+    }
+    uint32_t z = g->x[g->p++];
+    z ^= z >> 11;
+    z ^= (z << 7) & 0x9d2c5680u;
+    z ^= (z << 15) & 0xefc60000u;
+    z ^= z >> 18;
+    return z;
+}
+
+/* The first n outputs of std::mt19937(seed). */
+void orc_mt_stream(uint32_t seed, int64_t n, uint32_t *out)
+{
+    orc_mt g;
+    orc_mt_seed(&g, seed);
+    for (int64_t i = 0; i < n; ++i) out[i] = orc_mt_next(&g);
+}
+
+/* code.cpp:168-191 for one sample: with rows[j] the spike-time row of
+ * element j, draw(j) = table[rows[j]][next() & (bins - 1)]; every element
+ * draws once in order, then at each step t, in element order, an element
+ * whose next time equals t counts a spike and draws its next interval; the
+ * output at step t is the count so far.  times/values are U16 as there. */
+static void poisson_sample(const uint16_t *rows, int64_t len, int64_t time, const uint16_t *table, int64_t bins,
+                           orc_mt *g, uint16_t *times, uint16_t *values, uint8_t *out)
+{
+    for (int64_t j = 0; j < len; ++j) values[j] = 0;
+    for (int64_t j = 0; j < len; ++j) times[j] = table[rows[j] * bins + (orc_mt_next(g) & (uint32_t)(bins - 1))];
+    for (int64_t t = 0; t < time; ++t)
+    {
+        for (int64_t j = 0; j < len; ++j)
+            if (times[j] == t)
+            {
+                values[j] += 1;
+                times[j] = (uint16_t)(times[j] + table[rows[j] * bins + (orc_mt_next(g) & (uint32_t)(bins - 1))]);
+            }
+        for (int64_t j = 0; j < len; ++j) out[t * len + j] = (uint8_t)values[j];
+    }
+}
+
+/* code.cpp:192-230 (poisson, poissort) with the table given: sample i draws
+ * from std::mt19937(rnd + i) (rnd = the shared generator's draw for the
+ * call), its rows are the rank-code times (latency or sorted).  Every row
+ * index must be below `rows`; -1 is returned otherwise. */
+int32_t orc_rate_code_f32(const float *in, int64_t n, int64_t len, int64_t time, int32_t sort, uint64_t rnd,
+                          const uint16_t *table, int64_t rows, int64_t bins, uint8_t *out)
+{
+    uint16_t *r = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)len * 3);
+    uint16_t *times = r + len, *values = r + 2 * len;
+    int32_t rc = 0;
+    for (int64_t i = 0; i < n && rc == 0; ++i)
+    {
+        if (sort)
+            orc_rank_times_sort_f32(in + i * len, len, time, r);
+        else
+            latency_times(in + i * len, len, time, r);
+        for (int64_t j = 0; j < len; ++j)
+            if (r[j] >= rows) rc = -1;
+        if (rc) break;
+        orc_mt g;
+        orc_mt_seed(&g, (uint32_t)(rnd + (uint64_t)i));
+        poisson_sample(r, len, time, table, bins, &g, times, values, out + i * time * len);
+    }
+    free(r);
+    return rc;
+}
+
+/* src/spyker/cpu/fire.cpp:84-114 (rate fire): per element, over the steps,
+ * a spike is counted when value + threshold < x, and value becomes x; the
+ * output at step t is the count so far. */
+void orc_rate_fire_f32(const float *in, int64_t n, int64_t t, int64_t len, float threshold, uint8_t *out)
+{
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < len; ++j)
+        {
+            float value = 0.0f;
+            uint8_t spikes = 0;
+            for (int64_t s = 0; s < t; ++s)
+            {
+                const float x = in[(i * t + s) * len + j];
+                if (value + threshold < x) ++spikes, value = x;
+                out[(i * t + s) * len + j] = spikes;
+            }
+        }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -617,7 +738,7 @@ void orc_gabor_kernel(double *kernel, int64_t k, double sigma, double theta, dou
 }
 
 /* ------------------------------------------------------------------------ */
-/* sparse event path (src/spyker/sparse/*.cpp)                              */
+/* sparse event path (src/spyker/sparse/)                                    */
 /* ------------------------------------------------------------------------ */
 /*
  * A sparse tensor of shape [n][t][c][h][w] holds, for every (sample i, step

@@ -101,10 +101,14 @@ void orc_gabor_f32(const float *in, int64_t planes, int64_t h, int64_t w, const 
 /* -- host kernel generators (src/spyker/cpu/init.cpp:38-77) --------------- */
 void orc_log_f32(const float *in, int64_t planes, int64_t h, int64_t w, const float *kernel, int64_t count, int64_t k,
                  int64_t pt, int64_t pl, int64_t pb, int64_t pr, float *out, int32_t threads);
+void orc_mt_stream(uint32_t seed, int64_t n, uint32_t *out);
+int32_t orc_rate_code_f32(const float *in, int64_t n, int64_t len, int64_t time, int32_t sort, uint64_t rnd,
+                          const uint16_t *table, int64_t rows, int64_t bins, uint8_t *out);
+void orc_rate_fire_f32(const float *in, int64_t n, int64_t t, int64_t len, float threshold, uint8_t *out);
 void orc_gaussian_kernel(double *kernel, int64_t k, double std);
 void orc_gabor_kernel(double *kernel, int64_t k, double sigma, double theta, double gamma, double lambda, double psi);
 
-/* -- sparse event path (src/spyker/sparse/*.cpp) ------------------------ */
+/* -- sparse event path (src/spyker/sparse/) ------------------------------ */
 /* A list set: off[n*t + 1] and events packed z << 16 | y << 8 | x; list
  * (i, j) is [off[i*t+j], off[i*t+j+1]).  Producers return the event total,
  * or -1 when more than `cap` events would be written. */

@@ -233,26 +233,44 @@ def threshold(array: torch.Tensor, threshold: float, value: float = 0.0, inplace
     return array
 
 
-def fire(array: torch.Tensor, threshold: float = 0.0, dtype="u8") -> torch.Tensor:
-    """Spikes where x > threshold (src/spyker/opers.cpp:522-552)."""
+def _coding(code):
+    """Coding type from its name, any case (src/bind/bind.cpp:63-69)."""
+    c = str(code).lower()
+    if c not in ("rank", "rate"):
+        raise RuntimeError("Unknown codeing given.")
+    return c
+
+
+def fire(array: torch.Tensor, threshold: float = 0.0, dtype="u8", code: str = "rank") -> torch.Tensor:
+    """Spikes where x > threshold (rank), or spike counts over the steps of
+    [N, T, ...] input (rate) (src/spyker/opers.cpp:522-552)."""
     _dev(array)
     out = torch.empty(array.shape, dtype=_dtype(dtype), device=array.device)
-    call("spk_rank_fire", _ptr(array), _type(array), _ptr(out), _type(out), array.numel(), float(threshold),
-         _stream(array))
+    if _coding(code) == "rank":
+        call("spk_rank_fire", _ptr(array), _type(array), _ptr(out), _type(out), array.numel(), float(threshold),
+             _stream(array))
+        return out
+    if array.dim() < 3:
+        raise RuntimeError("Input dimensions couldn't be viewed as at least 3D.")
+    n, t = array.shape[0], array.shape[1]
+    call("spk_rate_fire", _ptr(array), _type(array), _ptr(out), _type(out), n, t, math.prod(array.shape[2:]),
+         float(threshold), _stream(array))
     return out
 
 
-def code(array: torch.Tensor, time: int, sort: bool = True, dtype="u8") -> torch.Tensor:
-    """Rank-order code into cumulative spikes [N, time, ...] (src/spyker/opers.cpp:488-511)."""
+def code(array: torch.Tensor, time: int, sort: bool = True, dtype="u8", code: str = "rank") -> torch.Tensor:
+    """Rank-order code into cumulative spikes, or rate (Poisson) code into
+    spike counts, [N, time, ...] (src/spyker/opers.cpp:488-511)."""
     _dev(array)
+    c = _coding(code)
     a = _least(array, 2)
     if time <= 0:
         raise RuntimeError("Input time can't be less than 1.")
     shape = [a.shape[0], int(time)] + list(a.shape[1:])
     out = torch.empty(shape, dtype=_dtype(dtype), device=array.device)
     n = a.shape[0]
-    call("spk_rank_code", _ptr(a), _type(a), _ptr(out), _type(out), n, a.numel() // max(n, 1), int(time),
-         int(bool(sort)), _stream(array))
+    call("spk_rank_code" if c == "rank" else "spk_rate_code", _ptr(a), _type(a), _ptr(out), _type(out), n,
+         a.numel() // max(n, 1), int(time), int(bool(sort)), _stream(array))
     return out
 
 
@@ -357,8 +375,7 @@ def gather(array: torch.Tensor, threshold: float = 0.0, dtype="u8", code: str = 
     """First spike step of every element (rank), or the last step (rate): [N, T, ...] -> [N, ...]
     (src/python/spyker/module.py:1106-1142, src/spyker/opers.cpp:554-575)."""
     _dev(array)
-    if code not in ("rank", "rate"):
-        raise RuntimeError("Coding type %s is not supported." % code)
+    code = _coding(code)
     a = array if array.dim() >= 3 else array.reshape((1,) * (3 - array.dim()) + tuple(array.shape))
     n, t = a.shape[0], a.shape[1]
     out = torch.empty((n,) + tuple(a.shape[2:]), dtype=_dtype(dtype), device=a.device)

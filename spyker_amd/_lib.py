@@ -55,6 +55,10 @@ SIGNATURES = {
     "spk_dog": [_p, _p, _p] + [_i64] * 9 + [_p],
     "spk_gabor": [_p, _p, _p] + [_i64] * 9 + [_p],
     "spk_log": [_p, _p, _p] + [_i64] * 9 + [_p],
+    "spk_rate_code": [_p, _i32, _p, _i32, _i64, _i64, _i64, _i32, _p],
+    "spk_poisson_table": [_p, _i64, _i64],
+    "spk_poisson_clear": [],
+    "spk_rate_fire": [_p, _i32, _p, _i32, _i64, _i64, _i64, _f64, _p],
     "spk_zca_fit": [_p, _p, _p, _i64, _i64, _f64, _i32, _p],
     "spk_zca_trans": [_p, _p, _p, _i64, _i64, _p],
     "spk_zca_split": [_p, _p, _i32, _i64, _i64, _p],

@@ -565,13 +565,13 @@ Tensor code(Tensor input, Tensor output, Size time, bool sort, Code code)
     input = atLeast(input, 2, "2D"), output = atLeast(output, 3, "3D");
     onGpu({input, output}, "Interface::RankCode");
     REQUIRE(time > 0, "Interface::RankCode", "Input time can't be less than 1.");
-    REQUIRE(code == Code::Rank, "Interface::RankCode", "This backend implements rank coding only.");
     Shape s = input.shape();
     s.insert(s.begin() + 1, time);
     REQUIRE(output.shape() == s, "Interface::RankCode", "Tensor shapes are not compatible.");
     const Size n = input.shape(0);
-    check(spk_rank_code(input.data(), ty(input.type()), output.data(), ty(output.type()), n, input.numel() / n, time,
-                        sort ? 1 : 0, nullptr),
+    auto coder = code == Code::Rank ? spk_rank_code : spk_rate_code;
+    check(coder(input.data(), ty(input.type()), output.data(), ty(output.type()), n, input.numel() / n, time,
+                sort ? 1 : 0, nullptr),
           "Interface::RankCode");
     return output;
 }
@@ -586,9 +586,17 @@ Tensor fire(Tensor input, Tensor output, F64 threshold, Code code)
 {
     onGpu({input, output}, "Interface::Fire");
     REQUIRE(output.shape() == input.shape(), "Interface::Fire", "Tensor shapes are not compatible.");
-    REQUIRE(code == Code::Rank, "Interface::Fire", "This backend implements rank coding only.");
-    check(spk_rank_fire(input.data(), ty(input.type()), output.data(), ty(output.type()), input.numel(), threshold,
-                        nullptr),
+    if (code == Code::Rank)
+    {
+        check(spk_rank_fire(input.data(), ty(input.type()), output.data(), ty(output.type()), input.numel(),
+                            threshold, nullptr),
+              "Interface::Fire");
+        return output;
+    }
+    // opers.cpp:540-550: [N][T][...] as [N][T][-1]
+    REQUIRE(input.dims() >= 3, "Interface::Fire", "Input dimensions couldn't be viewed as at least 3D.");
+    check(spk_rate_fire(input.data(), ty(input.type()), output.data(), ty(output.type()), input.shape(0),
+                        input.shape(1), input.numel() / (input.shape(0) * input.shape(1)), threshold, nullptr),
           "Interface::Fire");
     return output;
 }

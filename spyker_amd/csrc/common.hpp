@@ -73,6 +73,10 @@ const char *type_name(int32_t type);
 
 // Frees the layer plans the uniform fused path keeps per shape (region.hip).
 void clear_plan_cache();
+// One draw of the shared host generator (Generator() of the reference).
+uint64_t generator_draw();
+// [rows][bins] Poisson interval table (Core::poisson_create), host memory.
+void poisson_rows(uint16_t *table, int64_t rows, int64_t bins);
 
 // --------------------------------------------------------------- launch --
 

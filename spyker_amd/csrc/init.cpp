@@ -5,6 +5,7 @@
 // The reference is built with FMA contraction; the products it contracts
 // are written here as explicit std::fma so this file gives the same doubles
 // whatever it is compiled with (it is compiled with -ffp-contract=off).
+#include <algorithm>
 #include <cmath>
 #include <mutex>
 #include <random>
@@ -24,9 +25,70 @@ void normalize(double *v, int64_t n)
 }
 }  // namespace
 
+namespace spk
+{
+// Generator() of the reference (src/spyker/base.h:577): one draw of the shared
+// std::mt19937, e.g. the per-call seed base of the rate code (cpu/code.cpp:195).
+uint64_t generator_draw()
+{
+    std::lock_guard<std::mutex> lock(g_rng_mu);
+    return uint64_t(g_rng());
+}
+
+// Core::poisson_create (src/spyker/cpu/init.cpp:120-149).  Row i is the
+// interval table of lambda = i + 1: round(ztp(j, lambda) * bins) copies of
+// every interval j in [max(1, lambda - range), lambda + range], range =
+// ceil(scale * sqrt(lambda)) with scale 5 in the first row and 4 after, the
+// rest of the row lambda; ztp is the zero-truncated Poisson probability
+// lambda^j / (j! (e^lambda - 1)) in F32, accumulated as there.  The reference
+// then permutes each row with std::random_shuffle, whose std::rand state
+// (shared, and touched from an OpenMP loop) differs from run to run; here each
+// row is permuted by a Fisher-Yates pass over std::mt19937(5489 + row), the
+// same in every process.  A draw picks a uniform position of its row, so the
+// permutation leaves the code's distribution unchanged.
+void poisson_rows(uint16_t *table, int64_t rows, int64_t bins)
+{
+    auto ztp = [](uint16_t k, float a) {
+        float p = 1 / (std::exp(a) - 1);
+        for (int64_t i = 1; i <= k; ++i) p *= a / float(i);
+        return p;
+    };
+    for (int64_t i = 0; i < rows; ++i)
+    {
+        const float scale = i < 1 ? 5 : 4;
+        const uint16_t lambda = uint16_t(i + 1);
+        const uint16_t range = uint16_t(std::ceil(scale * std::sqrt(double(lambda))));
+        const uint16_t start = uint16_t(std::max(1, int(lambda) - int(range)));
+        const uint16_t end = uint16_t(lambda + range);
+        uint16_t *row = table + i * bins;
+        int64_t index = 0;
+        for (uint16_t j = start; j <= end; ++j)
+        {
+            int64_t count = int64_t(std::nearbyint(ztp(j, float(lambda)) * float(bins)));
+            if (index + count > bins) count = bins - index;
+            std::fill(row + index, row + index + count, j);
+            index += count;
+        }
+        std::fill(row + index, row + bins, lambda);
+        std::mt19937 shuffle(uint32_t(5489u + uint64_t(i)));
+        for (int64_t j = bins - 1; j > 0; --j) std::swap(row[j], row[shuffle() % uint64_t(j + 1)]);
+    }
+}
+}  // namespace spk
+
 using namespace spk;
 
 extern "C" {
+
+// Core::poisson_create (src/spyker/impl.h:190): [rows][bins] U16 on the host.
+int32_t spk_poisson_table(uint16_t *table, int64_t rows, int64_t bins)
+{
+    return guard([&] {
+        SPK_ARG(rows >= 1 && rows <= 65535, "Poisson table: rows must be in [1, 65535].");
+        SPK_ARG(bins >= 1, "Poisson table: bins can't be less than 1.");
+        poisson_rows(table, rows, bins);
+    });
+}
 
 // Core::random_seed (src/spyker/impl.cpp:43)
 int32_t spk_random_seed(int64_t seed)

@@ -89,6 +89,29 @@ __global__ void rate_gather_kernel(const I *__restrict__ in, O *__restrict__ out
     }
 }
 
+// rate fire (fire.cpp:84-114; cuda/fire.cu:82-96): per element over the steps,
+// value + threshold < x counts a spike (in the output type) and keeps x.
+template <typename I, typename O>
+__global__ void rate_fire_kernel(const I *__restrict__ in, O *__restrict__ out, int64_t n, int32_t t, int64_t len,
+                                 I th)
+{
+    for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < n * len;
+         idx += int64_t(gridDim.x) * blockDim.x)
+    {
+        const int64_t i = idx / len, j = idx % len;
+        const I *x = in + i * t * len + j;
+        O *o = out + i * t * len + j;
+        O spikes = O(0);
+        I value = I(0);
+        for (int32_t s = 0; s < t; ++s)
+        {
+            const I v = x[int64_t(s) * len];
+            if (value + th < v) spikes = O(spikes + O(1)), value = v;
+            o[int64_t(s) * len] = spikes;
+        }
+    }
+}
+
 // Calls f with a value of the C type of `type` (U8, U16, I32 or F32).
 template <typename F>
 void with_type(int32_t type, const char *what, F &&f)
@@ -143,6 +166,25 @@ int32_t spk_rank_scatter(const void *in, int32_t in_type, void *out, int32_t out
                 if (n * len == 0) return;
                 rank_scatter_kernel<I, O><<<flat_grid(n * len, 256), 256, 0, s>>>((const I *)in, (O *)out, n, len,
                                                                                   int32_t(t));
+                SPK_LAUNCHED();
+            });
+        });
+    });
+}
+
+int32_t spk_rate_fire(const void *in, int32_t in_type, void *out, int32_t out_type, int64_t n, int64_t t,
+                      int64_t len, double threshold, void *stream)
+{
+    return guard([&] {
+        SPK_ARG(n >= 0 && t >= 1 && len >= 0 && t < INT32_MAX, "Fire: input shape is invalid.");
+        hipStream_t s = S(stream);
+        with_type(in_type, "Fire", [&](auto i0) {
+            using I = decltype(i0);
+            with_type(out_type, "Fire", [&](auto o0) {
+                using O = decltype(o0);
+                if (n * len == 0) return;
+                rate_fire_kernel<I, O><<<flat_grid(n * len, 256), 256, 0, s>>>(
+                    (const I *)in, (O *)out, n, int32_t(t), len, to_input<I>(threshold));
                 SPK_LAUNCHED();
             });
         });

@@ -140,6 +140,23 @@ def main():
     v["rank_scatter_u8_T10"] = R.rank_scatter(times_u8, 10)
     v["rank_scatter_f32_T10"] = R.rank_scatter(times_f32, 10)
 
+    # rate coding (cpu/code.cpp:134-243, cpu/init.cpp:120-149, cpu/fire.cpp:84-114): the
+    # table's per-row contents, and the code itself run on a given table (rows past T repeat
+    # row T, as the device reads them); rnd + i wraps past 2^32 within the batch
+    v["poisson_rows_sorted_T40"] = np.sort(R.poisson_create(40), axis=1)
+    xr = rng.random((4, 300)).astype(np.float32)
+    xr[xr < 0.3] = 0
+    base = R.poisson_create(16)
+    table = np.concatenate([base] + [base[-1:]] * 40)
+    v["rate_in"], v["rate_table"] = xr, table
+    v["rate_code_latency_T15"] = R.rate_code(xr, 15, False, 2**32 - 2, table)
+    v["rate_code_sort_T15"] = R.rate_code(xr, 15, True, 2**32 - 2, table)
+    rp = np.cumsum(rng.random((3, 12, 50)).astype(np.float32) * 0.3, axis=1).astype(np.float32)
+    rp[:, 5:7] -= 0.5  # drops below the last value fire nothing
+    v["rate_fire_in"] = rp
+    v["rate_fire_th01"] = R.rate_fire(rp, 0.1)
+    v["rate_fire_th0"] = R.rate_fire(rp, 0.0)
+
     np.savez_compressed(OUT, **v)
     print("wrote", OUT, os.path.getsize(OUT), "bytes,", len(v), "arrays")
 

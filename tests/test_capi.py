@@ -107,6 +107,18 @@ def test_log_kernel_bank(golden):
         spk.LoG(2, [], device="cpu")
 
 
+def test_poisson_table_rows_match_reference(lib, golden):
+    """spk_poisson_table (Core::poisson_create, cpu/init.cpp:120-149): every row holds the
+    reference's intervals; only their order (the reference's std::rand shuffle) differs."""
+    t = np.zeros((40, 1024), np.uint16)
+    assert lib.spk_poisson_table(t.ctypes.data_as(C.c_void_p), C.c_int64(40), C.c_int64(1024)) == 0
+    assert np.array_equal(np.sort(t, axis=1), golden["poisson_rows_sorted_T40"])
+    u = np.zeros((41, 1024), np.uint16)
+    assert lib.spk_poisson_table(u.ctypes.data_as(C.c_void_p), C.c_int64(41), C.c_int64(1024)) == 0
+    assert np.array_equal(u[:40], t)  # a row does not depend on the table's length
+    assert (t >= 1).all() and not all((np.diff(r.astype(np.int32)) >= 0).all() for r in t)  # shuffled
+
+
 def test_python_front_end_shape_rules():
     import spyker_amd as spk
     assert spk.expand2(3) == (3, 3) and spk.expand2([2, 5]) == (2, 5)

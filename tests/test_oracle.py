@@ -322,6 +322,41 @@ def test_zca_split_closed_forms(orc):
     assert orc.zca_split(np.array([[-128, 5, -7]], np.int8)).tolist() == [[[0, 5, 0], [0, 0, 7]]]
 
 
+def test_mt19937_stream(orc):
+    """The oracle's MT19937 against the standard's known answer (the 10000th output of
+    std::mt19937 default-seeded with 5489 is 4123659995) and numpy's independent MT19937
+    under the same seeding."""
+    assert orc.mt_stream(5489, 10000)[-1] == 4123659995
+    for seed in (0, 1, 12345, 2**32 - 1):
+        bg = np.random.MT19937(0)
+        bg._legacy_seeding(seed)
+        assert np.array_equal(orc.mt_stream(seed, 2000), bg.random_raw(2000).astype(np.uint32))
+
+
+def test_rate_code_against_reference_vectors(orc, golden):
+    """The Poisson rate code (cpu/code.cpp:168-230) on the reference's own run with the same
+    table, both time orders; the seed base wraps past 2^32 inside the batch."""
+    x, table = golden["rate_in"], golden["rate_table"]
+    for sort, key in ((False, "rate_code_latency_T15"), (True, "rate_code_sort_T15")):
+        got = orc.rate_code(x, 15, sort, 2**32 - 2, table)
+        assert np.array_equal(got, golden[key]), key
+        steps = np.diff(got.astype(np.int32), axis=1)
+        assert ((steps == 0) | (steps == 1)).all()  # counts grow by at most one spike a step
+    # a table with too few rows is refused rather than read past
+    with pytest.raises(RuntimeError):
+        orc.rate_code(x, 15, True, 7, table[:16])
+
+
+def test_rate_fire_against_reference_vectors(orc, golden):
+    x = golden["rate_fire_in"]
+    for th, key in ((0.1, "rate_fire_th01"), (0.0, "rate_fire_th0")):
+        assert np.array_equal(orc.rate_fire(x, th), golden[key]), key
+    # closed form on a ramp: x_t = t * 0.25 fires every step above the threshold gap
+    ramp = (np.arange(8, dtype=np.float32) * 0.25).reshape(1, 8, 1)
+    assert orc.rate_fire(ramp, 0.1).ravel().tolist() == [0, 1, 2, 3, 4, 5, 6, 7]
+    assert orc.rate_fire(ramp, 0.3).ravel().tolist() == [0, 0, 1, 1, 2, 2, 3, 3]
+
+
 def test_host_kernels(orc, golden):
     for kk, sd in [(3, 1.0), (3, 2.0), (7, 1.0), (7, 2.0)]:
         g = orc.gaussian_kernel(kk, sd)
