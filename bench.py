@@ -400,6 +400,117 @@ def c4_bench(spk, dev, steps):
             "step_hbm_bytes_alg": plan.in_bytes + plan.out_bytes + plan.pool_bytes}
 
 
+# Front ends before the layer: ZCA whitening of 50000 CIFAR-shaped samples (3x32x32 = 3072
+# features, FP64 on the device), the LoG bank on the 1024 28x28 images (sizes 3, stds 1 and 2),
+# the Poisson rate code of the 1024 DoG maps (4x28x28, T=15) and rate fire on layer-2 shaped
+# potentials [1024, 15, 30*14*14].
+ZCA_ROWS, ZCA_COLS, ZCA_EPS = 50000, 3072, 1e-2
+PEAK_FP64 = 78.6e12
+
+
+def _event_time(fn, reps):
+    """Mean device time of fn on the current stream (HIP events), after one untimed call."""
+    import torch
+    fn()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    s.record()
+    for _ in range(reps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / 1e3 / reps
+
+
+def front_end_bench(spk, dev, cpu):
+    import torch
+    from spyker_amd import synthetic
+    import oracle
+    orc = oracle.Oracle() if cpu else None
+    threads = os.cpu_count() or 1
+    out = {}
+
+    # ZCA: correlated synthetic data (rank-256 signal + noise), fit then whiten in place
+    g = torch.Generator(device=dev).manual_seed(7)
+    basis = torch.randn(256, ZCA_COLS, device=dev, generator=g) / 16
+    x = (torch.randn(ZCA_ROWS, 256, device=dev, generator=g) @ basis
+         + 0.1 * torch.randn(ZCA_ROWS, ZCA_COLS, device=dev, generator=g)).contiguous()
+    z = spk.ZCA()
+    t0 = time.perf_counter()
+    z.fit(x, ZCA_EPS)
+    torch.cuda.synchronize()
+    fit_s = time.perf_counter() - t0
+    y = x.clone()
+    apply_s = _event_time(lambda: z(y), 3)
+    flop = 2.0 * ZCA_ROWS * ZCA_COLS * ZCA_COLS
+    out["zca"] = {"workload": "%d x %d F32 (CIFAR-shaped), eps %g, FP64 inside" % (ZCA_ROWS, ZCA_COLS, ZCA_EPS),
+                  "fit_s": fit_s, "apply_ms": apply_s * 1e3, "value": ZCA_ROWS / apply_s, "unit": "samples/s",
+                  "roofline": {"bound": "mfma", "achieved": flop / apply_s / 1e12, "peak": PEAK_FP64 / 1e12,
+                               "unit": "TFLOP/s", "frac": flop / apply_s / PEAK_FP64,
+                               "kernel": "apply: rocBLAS dgemm + centring/narrowing kernels"}}
+    if cpu:
+        xs = x[:5000].cpu().numpy()
+        t0 = time.perf_counter()
+        m, tr = orc.zca_fit(xs, ZCA_EPS)
+        orc.zca_trans(xs, m, tr)
+        dt = time.perf_counter() - t0
+        out["zca"]["cpu_baseline"] = {"value": 5000 / dt, "unit": "samples/s (fit + apply)", "cores": threads,
+                                      "kind": "port", "sample": "5000 x 3072: the oracle's numpy FP64 fit and "
+                                      "apply (host BLAS/LAPACK threads)"}
+    del x, y, basis, z
+    torch.cuda.empty_cache()
+
+    img = torch.from_numpy(synthetic.strokes(N, 28, seed=2).astype(np.float32) / 255).to(dev).reshape(N, 1, 28, 28)
+    lg = spk.LoG(3, [1.0, 2.0], pad=3, device=dev)
+    log_s = _event_time(lambda: lg(img), 10)
+    lflop = 2.0 * N * 4 * 28 * 28 * 49
+    out["log"] = {"workload": "1024 x 28x28, LoG size 3 (7x7), stds 1 and 2, pad 3 -> [1024, 4, 28, 28]",
+                  "ms": log_s * 1e3, "value": N / log_s, "unit": "images/s",
+                  "roofline": {"bound": "hbm", "achieved": N * 28 * 28 * 4 * (1 + 4 + 4 * 2) / log_s / 1e9,
+                               "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                               "frac": N * 28 * 28 * 4 * (1 + 4 + 4 * 2) / log_s / PEAK_HBM,
+                               "flop_rate_tflops": lflop / log_s / 1e12}}
+    if cpu:
+        ih, kh = img.cpu().numpy(), lg.kernel.cpu().numpy()
+        t0 = time.perf_counter()
+        orc.log(ih, kh, (3, 3, 3, 3))
+        dt = time.perf_counter() - t0
+        out["log"]["cpu_baseline"] = {"value": N / dt, "unit": "images/s", "cores": threads, "kind": "port",
+                                      "sample": "the same 1024 images (oracle C port, OpenMP)"}
+
+    d3 = spk.DoG(1, [spk.DoGFilter(1, 2), spk.DoGFilter(2, 1)], pad=1, device=dev)
+    d7 = spk.DoG(3, [spk.DoGFilter(1, 2), spk.DoGFilter(2, 1)], pad=3, device=dev)
+    filt = torch.cat([d3(img), d7(img)], dim=1).contiguous()
+    spk.threshold(filt, 0.01)
+    ln = 4 * 28 * 28
+    rate_s = _event_time(lambda: spk.code(filt, T_STEPS, code="rate"), 10)
+    rbytes = N * ln * (4 + 2 + 2 + T_STEPS)  # F32 in, U16 times out and back in, U8 counts per step
+    out["rate_code"] = {"workload": "1024 DoG maps 4x28x28, T=15, sorted order (the default), U8 counts",
+                        "ms": rate_s * 1e3, "value": N * T_STEPS / rate_s, "unit": "images*timesteps/s",
+                        "roofline": {"bound": "hbm", "achieved": rbytes / rate_s / 1e9, "peak": PEAK_HBM / 1e9,
+                                     "unit": "GB/s", "frac": rbytes / rate_s / PEAK_HBM,
+                                     "kernel": "rank-time sort + rate_code_kernel"}}
+    if cpu:
+        fh = filt.cpu().numpy().reshape(N, ln)[:256]
+        tab = np.zeros((T_STEPS + 1, 1024), np.uint16)
+        spk._lib.load().spk_poisson_table(tab.ctypes.data_as(C.c_void_p), C.c_int64(T_STEPS + 1), C.c_int64(1024))
+        t0 = time.perf_counter()
+        orc.rate_code(fh, T_STEPS, False, 12345, tab)
+        dt = time.perf_counter() - t0
+        out["rate_code"]["cpu_baseline"] = {"value": 256 * T_STEPS / dt, "unit": "images*timesteps/s", "cores": 1,
+                                            "kind": "port",
+                                            "sample": "256 of the maps (oracle C port, one thread)"}
+
+    pot = torch.rand((N, T_STEPS, CIN * H * H), device=dev).cumsum(1).contiguous()
+    fire_s = _event_time(lambda: spk.fire(pot, 0.5, code="rate"), 10)
+    fbytes = pot.numel() * (4 + 1)
+    out["rate_fire"] = {"workload": "[1024, 15, 30x14x14] F32 potentials -> U8 counts, threshold 0.5",
+                        "ms": fire_s * 1e3, "value": N * T_STEPS / fire_s, "unit": "images*timesteps/s",
+                        "roofline": {"bound": "hbm", "achieved": fbytes / fire_s / 1e9, "peak": PEAK_HBM / 1e9,
+                                     "unit": "GB/s", "frac": fbytes / fire_s / PEAK_HBM, "kernel": "rate_fire_kernel"}}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -411,7 +522,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-sparse", action="store_true")
     ap.add_argument("--sparse-only", action="store_true", help="time only the sparse event-stream workload")
-    ap.add_argument("--only", choices=["c3", "c4", "sparse"], help="time only this extra workload")
+    ap.add_argument("--only", choices=["c3", "c4", "sparse", "front"], help="time only this extra workload")
     ap.add_argument("--no-extra", action="store_true", help="skip the precision and batching workloads")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (HBM counter calibration under rocprofv3)")
@@ -438,7 +549,8 @@ def main():
         a.only = "sparse"
     if a.only:
         fns = {"c3": lambda: c3_bench(spk, dev, a.steps), "c4": lambda: c4_bench(spk, dev, a.steps),
-               "sparse": lambda: sparse_bench(spk, dev, a.steps, a.warmup, not a.no_cpu)}
+               "sparse": lambda: sparse_bench(spk, dev, a.steps, a.warmup, not a.no_cpu),
+               "front": lambda: front_end_bench(spk, dev, not a.no_cpu)}
         print(json.dumps({a.only: fns[a.only]()}))
         return
     x, k2 = make_inputs(dev)
@@ -496,6 +608,7 @@ def main():
     if not a.no_extra:
         line["precision_variant"] = guarded(lambda: c3_bench(spk, dev, 3))
         line["batching_variant"] = guarded(lambda: c4_bench(spk, dev, 3))
+        line["front_end"] = guarded(lambda: front_end_bench(spk, dev, not a.no_cpu))
     print(json.dumps(line))
 
 

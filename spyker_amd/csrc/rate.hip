@@ -104,7 +104,33 @@ __device__ void ensure(uint32_t *mt, uint32_t *buf, int64_t cap, int64_t pos, in
     }
 }
 
-template <typename O>
+// Exclusive prefix of v over the workgroup's threads in thread order, and its total.
+__device__ __forceinline__ int32_t block_exclusive(int32_t v, int32_t *wsum, int32_t &total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int32_t incl = v;
+    for (int d = 1; d < 64; d <<= 1)
+    {
+        const int32_t u = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += u;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    int32_t before = 0;
+    total = 0;
+    for (int k = 0; k < RATE_THREADS / 64; ++k)
+    {
+        before += k < wave ? wsum[k] : 0;
+        total += wsum[k];
+    }
+    return before + incl - v;
+}
+
+// Thread k owns the contiguous elements [k * per, (k + 1) * per), so the
+// draws of a step are ranked by one scan of per-thread counts and each thread
+// issues its table reads back to back.  ROWS_IN_LDS keeps the rows (spike
+// times) in LDS, when they fit, instead of re-reading them from HBM.
+template <typename O, bool ROWS_IN_LDS>
 __global__ void __launch_bounds__(RATE_THREADS)
     rate_code_kernel(const uint16_t *__restrict__ rows_g, const uint16_t *__restrict__ table, O *__restrict__ out,
                      int64_t len, int32_t t, uint64_t rnd)
@@ -113,15 +139,26 @@ __global__ void __launch_bounds__(RATE_THREADS)
     __shared__ int32_t wsum[RATE_THREADS / 64];
     const int64_t cap = len + MT_N;
     uint32_t *mt = lds, *buf = lds + MT_N;
-    uint16_t *times = reinterpret_cast<uint16_t *>(buf + cap), *values = times + len;
+    uint16_t *times = reinterpret_cast<uint16_t *>(buf + cap), *values = times + len, *rows_l = values + len;
     const int64_t s = blockIdx.x;
     const uint16_t *rows = rows_g + s * len;
     O *o = out + s * int64_t(t) * len;
     const uint32_t last = uint32_t(t);  // the table's last row
-    auto interval = [&](int64_t j, uint32_t word) {
-        const uint32_t r = rows[j] < last ? rows[j] : last;
-        return table[int64_t(r) * POISSON_BINS + (word & (POISSON_BINS - 1))];
+    if constexpr (ROWS_IN_LDS)
+        for (int64_t j = threadIdx.x; j < len; j += RATE_THREADS) rows_l[j] = rows[j];
+    // table position of element j's next interval for stream word `word`
+    auto slot = [&](int64_t j, uint32_t word) {
+        uint32_t r;
+        if constexpr (ROWS_IN_LDS)
+            r = rows_l[j];
+        else
+            r = rows[j];
+        r = r < last ? r : last;
+        return r * uint32_t(POISSON_BINS) + (word & (POISSON_BINS - 1));
     };
+    const int64_t per = (len + RATE_THREADS - 1) / RATE_THREADS;
+    const int64_t j0 = int64_t(threadIdx.x) * per < len ? int64_t(threadIdx.x) * per : len;
+    const int64_t j1 = j0 + per < len ? j0 + per : len;
 
     if (threadIdx.x == 0)
     {
@@ -138,43 +175,50 @@ __global__ void __launch_bounds__(RATE_THREADS)
 
     int64_t base = 0, filled = 0, pos = 0;
     ensure(mt, buf, cap, 0, len, base, filled);
-    for (int64_t j = threadIdx.x; j < len; j += RATE_THREADS)
+    for (int64_t j = j0; j < j1; ++j)
     {
         values[j] = 0;
-        times[j] = interval(j, buf[j - base]);
+        times[j] = table[slot(j, buf[j - base])];
     }
     pos = len;
     __syncthreads();
 
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int32_t step = 0; step < t; ++step)
     {
         ensure(mt, buf, cap, pos, pos + len, base, filled);
-        int64_t run = 0;
-        for (int64_t c = 0; c < len; c += RATE_THREADS)
+        const uint16_t now = uint16_t(step);
+        int32_t cnt = 0;
+        for (int64_t j = j0; j < j1; ++j) cnt += times[j] == now ? 1 : 0;
+        int32_t total;
+        int64_t at = pos + block_exclusive(cnt, wsum, total) - base;
+        // eight elements at a time: stream words and table positions first, then
+        // the eight table reads together, then the updates
+        for (int64_t c = j0; c < j1; c += 8)
         {
-            const int64_t j = c + threadIdx.x;
-            const bool fires = j < len && times[j] == uint16_t(step);
-            const uint64_t ball = __ballot(fires);
-            if (lane == 0) wsum[wave] = __popcll(ball);
-            __syncthreads();
-            int32_t before = 0, total = 0;
-            for (int k = 0; k < RATE_THREADS / 64; ++k)
+            uint32_t idx[8];
+            bool fire[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
             {
-                before += k < wave ? wsum[k] : 0;
-                total += wsum[k];
+                const int64_t j = c + k;
+                fire[k] = j < j1 && times[j] == now;
+                idx[k] = fire[k] ? slot(j, buf[at]) : 0u;
+                at += fire[k] ? 1 : 0;
             }
-            if (fires)
-            {
-                const int64_t at = pos + run + before + __popcll(ball & ((1ull << lane) - 1));
-                values[j] = uint16_t(values[j] + 1);
-                times[j] = uint16_t(times[j] + interval(j, buf[at - base]));
-            }
-            run += total;
-            __syncthreads();
+            uint16_t gap[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) gap[k] = fire[k] ? table[idx[k]] : uint16_t(0);
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (fire[k])
+                {
+                    values[c + k] = uint16_t(values[c + k] + 1);
+                    times[c + k] = uint16_t(times[c + k] + gap[k]);
+                }
         }
+        __syncthreads();
         for (int64_t j = threadIdx.x; j < len; j += RATE_THREADS) o[int64_t(step) * len + j] = O(values[j]);
-        pos += run;
+        pos += total;
     }
 }
 
@@ -216,7 +260,22 @@ const uint16_t *device_table(int64_t time)
     return d;
 }
 
-size_t rate_lds(int64_t len) { return size_t(4 * (MT_N + len + MT_N) + 4 * len); }
+// mt + stream window (u32), times + values (+ rows) (u16)
+size_t rate_lds(int64_t len, bool rows) { return size_t(4 * (MT_N + len + MT_N) + (rows ? 6 : 4) * len); }
+
+template <typename O>
+void launch_rate(const uint16_t *rows, const uint16_t *table, O *out, int64_t n, int64_t len, int32_t t,
+                 uint64_t rnd, hipStream_t s)
+{
+    const bool in_lds = rate_lds(len, true) <= 131072;
+    const size_t lds = rate_lds(len, in_lds);
+    auto kernel = in_lds ? &rate_code_kernel<O, true> : &rate_code_kernel<O, false>;
+    if (lds > 65536)
+        SPK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    int(lds)));
+    kernel<<<dim3(unsigned(n)), RATE_THREADS, lds, s>>>(rows, table, out, len, t, rnd);
+    SPK_LAUNCHED();
+}
 }  // namespace
 }  // namespace spk
 
@@ -244,24 +303,10 @@ int32_t spk_rate_code(const void *in, int32_t in_type, void *out, int32_t out_ty
         const uint16_t *table = device_table(time);
         Scratch rows(n * len * int64_t(sizeof(uint16_t)), s);
         rank_times_launch((const float *)in, n, len, time, sort != 0, rows.as<uint16_t>(), s);
-        const size_t lds = rate_lds(len);
         if (out_type == SPK_U8)
-        {
-            if (lds > 65536)
-                SPK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&rate_code_kernel<uint8_t>),
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-            rate_code_kernel<uint8_t><<<dim3(unsigned(n)), RATE_THREADS, lds, s>>>(
-                rows.as<uint16_t>(), table, (uint8_t *)out, len, int32_t(time), rnd);
-        }
+            launch_rate<uint8_t>(rows.as<uint16_t>(), table, (uint8_t *)out, n, len, int32_t(time), rnd, s);
         else
-        {
-            if (lds > 65536)
-                SPK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&rate_code_kernel<float>),
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-            rate_code_kernel<float><<<dim3(unsigned(n)), RATE_THREADS, lds, s>>>(rows.as<uint16_t>(), table,
-                                                                                 (float *)out, len, int32_t(time), rnd);
-        }
-        SPK_LAUNCHED();
+            launch_rate<float>(rows.as<uint16_t>(), table, (float *)out, n, len, int32_t(time), rnd, s);
     });
 }
 

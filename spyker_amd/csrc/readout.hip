@@ -90,24 +90,46 @@ __global__ void rate_gather_kernel(const I *__restrict__ in, O *__restrict__ out
 }
 
 // rate fire (fire.cpp:84-114; cuda/fire.cu:82-96): per element over the steps,
-// value + threshold < x counts a spike (in the output type) and keeps x.
-template <typename I, typename O>
+// value + threshold < x counts a spike (in the output type) and keeps x.  A
+// thread walks W consecutive elements through the steps (W = 4 when the row
+// length allows it, so each step is one 4-wide load and store per thread).
+template <typename I, typename O, int W>
 __global__ void rate_fire_kernel(const I *__restrict__ in, O *__restrict__ out, int64_t n, int32_t t, int64_t len,
                                  I th)
 {
-    for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < n * len;
+    const int64_t groups = len / W;
+    for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < n * groups;
          idx += int64_t(gridDim.x) * blockDim.x)
     {
-        const int64_t i = idx / len, j = idx % len;
+        const int64_t i = idx / groups, j = (idx % groups) * W;
         const I *x = in + i * t * len + j;
         O *o = out + i * t * len + j;
-        O spikes = O(0);
-        I value = I(0);
+        O spikes[W];
+        I value[W];
+#pragma unroll
+        for (int k = 0; k < W; ++k) spikes[k] = O(0), value[k] = I(0);
         for (int32_t s = 0; s < t; ++s)
         {
-            const I v = x[int64_t(s) * len];
-            if (value + th < v) spikes = O(spikes + O(1)), value = v;
-            o[int64_t(s) * len] = spikes;
+            I v[W];
+            O r[W];
+            if constexpr (W == 4 && sizeof(I) == 4)
+                *reinterpret_cast<uint4 *>(v) = *reinterpret_cast<const uint4 *>(x + int64_t(s) * len);
+            else
+#pragma unroll
+                for (int k = 0; k < W; ++k) v[k] = x[int64_t(s) * len + k];
+#pragma unroll
+            for (int k = 0; k < W; ++k)
+            {
+                if (value[k] + th < v[k]) spikes[k] = O(spikes[k] + O(1)), value[k] = v[k];
+                r[k] = spikes[k];
+            }
+            if constexpr (W == 4 && sizeof(O) == 1)
+                *reinterpret_cast<uint32_t *>(o + int64_t(s) * len) = *reinterpret_cast<const uint32_t *>(r);
+            else if constexpr (W == 4 && sizeof(O) == 4)
+                *reinterpret_cast<uint4 *>(o + int64_t(s) * len) = *reinterpret_cast<const uint4 *>(r);
+            else
+#pragma unroll
+                for (int k = 0; k < W; ++k) o[int64_t(s) * len + k] = r[k];
         }
     }
 }
@@ -183,8 +205,15 @@ int32_t spk_rate_fire(const void *in, int32_t in_type, void *out, int32_t out_ty
             with_type(out_type, "Fire", [&](auto o0) {
                 using O = decltype(o0);
                 if (n * len == 0) return;
-                rate_fire_kernel<I, O><<<flat_grid(n * len, 256), 256, 0, s>>>(
-                    (const I *)in, (O *)out, n, int32_t(t), len, to_input<I>(threshold));
+                // 4-wide when rows keep 16-byte input and 4-element output alignment
+                const bool wide = len % 4 == 0 && reinterpret_cast<uintptr_t>(in) % 16 == 0 &&
+                                  reinterpret_cast<uintptr_t>(out) % (4 * sizeof(O)) == 0;
+                if (wide)
+                    rate_fire_kernel<I, O, 4><<<flat_grid(n * len / 4, 256), 256, 0, s>>>(
+                        (const I *)in, (O *)out, n, int32_t(t), len, to_input<I>(threshold));
+                else
+                    rate_fire_kernel<I, O, 1><<<flat_grid(n * len, 256), 256, 0, s>>>(
+                        (const I *)in, (O *)out, n, int32_t(t), len, to_input<I>(threshold));
                 SPK_LAUNCHED();
             });
         });

@@ -102,6 +102,11 @@ def test_rate_fire(dev, orc, golden):
     rng = np.random.default_rng(6)
     x = np.cumsum(rng.random((4, 20, 3, 5, 5)).astype(np.float32), axis=1)
     assert np.array_equal(H(spk.fire(T(x, dev), 0.7, code="rate")), orc.rate_fire(x, 0.7))
+    # rows of a multiple of 4 elements take the 4-wide path (F32 in, U8 and F32 out)
+    x4 = np.cumsum(rng.random((5, 17, 2, 64)).astype(np.float32), axis=1)
+    ref4 = orc.rate_fire(x4, 0.6)
+    assert np.array_equal(H(spk.fire(T(x4, dev), 0.6, code="rate")), ref4)
+    assert np.array_equal(H(spk.fire(T(x4, dev), 0.6, dtype="f32", code="rate")), ref4.astype(np.float32))
     # integer input: value + threshold < x in integer arithmetic (threshold converted to U8)
     u = np.cumsum(rng.integers(0, 3, (3, 16, 40)), axis=1).clip(0, 255).astype(np.uint8)
     assert np.array_equal(H(spk.fire(T(u, dev), 2, code="rate")), orc.rate_fire(u.astype(np.float32), 2.0))
