@@ -251,6 +251,42 @@ public:
     Tensor operator()(Tensor input);
 };
 
+/// Laplacian of Gaussian filter bank (src/spyker/opers.h:296-335): each std s
+/// gives the Gaussians of s / sqrt 2 and s * sqrt 2; an input channel yields
+/// [on, off] x count channels.
+class LoG
+{
+    bool _init = false;
+    Device _device;
+    Expand4 _pad;
+
+public:
+    Tensor kernel;
+
+    LoG() {}
+    LoG(Size size, const std::vector<F64> &stds, Expand4 pad = 0, Type type = Type::F32);
+    LoG(Device device, Size size, const std::vector<F64> &stds, Expand4 pad = 0, Type type = Type::F32);
+    Tensor operator()(Tensor input, Tensor output);
+    Tensor operator()(Tensor input);
+    Device device() const { return _device; }
+};
+
+/// ZCA whitening (src/spyker/opers.h:339-375).  `fit` learns the F32 mean and
+/// transform (FP64 inside) from the rows of an F32 device tensor; calling the
+/// module whitens in place unless `inplace` is false.
+class ZCA
+{
+public:
+    Tensor mean;
+    Tensor transform;
+
+    ZCA() {}
+    ZCA &fit(Tensor input, F64 epsilon, bool transform = false);
+    Tensor operator()(Tensor input, bool inplace = true);
+    static Tensor split(Tensor input, Tensor output);
+    static Tensor split(Tensor input);
+};
+
 class SparseTensor;
 
 /// 2D convolution layer with rank STDP.

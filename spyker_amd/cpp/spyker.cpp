@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <sstream>
@@ -703,6 +704,104 @@ Tensor DoG::operator()(Tensor input)
     REQUIRE(_init, "Interface::DoG", "Layer is used before initializing.");
     REQUIRE(input.device() == _device, "Interface::DoG", "Input device is not the same as layer's.");
     return dog(input, kernel, _pad);
+}
+
+LoG::LoG(Size size, const std::vector<F64> &stds, Expand4 pad, Type type)
+    : LoG(Device(Kind::CPU), size, stds, pad, type)
+{
+}
+
+LoG::LoG(Device device, Size size, const std::vector<F64> &stds, Expand4 pad, Type type)
+    : _init(true), _device(device), _pad(pad)
+{
+    REQUIRE(size > 0, "Interface::LoG", "Input size can't be less than 1.");
+    REQUIRE(!stds.empty(), "Interface::LoG", "Input stds count can't be less then 1.");
+    checkItems("Pad", _pad.get(), {0, 0, 0, 0});
+    const Size count = Size(stds.size()), k = 2 * size + 1;
+    Tensor host(Type::F64, {2, count, k, k});
+    for (Size i = 0; i < count; ++i)
+    {
+        check(spk_gaussian_kernel(host[0][i].data<double>(), k, stds[size_t(i)] / std::sqrt(2)), "Interface::LoG");
+        check(spk_gaussian_kernel(host[1][i].data<double>(), k, stds[size_t(i)] * std::sqrt(2)), "Interface::LoG");
+    }
+    kernel = host.to(type).to(_device);
+}
+
+Tensor LoG::operator()(Tensor input, Tensor output)
+{
+    REQUIRE(_init, "Interface::LoG", "Layer is used before initializing.");
+    REQUIRE(input.device() == _device, "Interface::LoG", "Input device is not the same as layer's.");
+    input = to4(input).to(kernel.type()), output = to4(output);
+    onGpu({input, kernel, output}, "Interface::LoG");
+    REQUIRE(kernel.type() == Type::F32 && output.type() == Type::F32, "Interface::LoG",
+            "Kernel and output must be F32 on this backend.");
+    Shape s = filterShape(input.shape(), 2 * kernel.shape(1), kernel.shape(2), kernel.shape(3), _pad.get(),
+                          "Shape::LoG");
+    REQUIRE(output.shape() == s, "Interface::LoG", "Tensor shapes are not compatible.");
+    check(spk_log(input.data(), kernel.data(), output.data(), input.shape(0) * input.shape(1), input.shape(2),
+                  input.shape(3), kernel.shape(1), kernel.shape(2), _pad[0], _pad[1], _pad[2], _pad[3], nullptr),
+          "Interface::LoG");
+    return output;
+}
+
+Tensor LoG::operator()(Tensor input)
+{
+    REQUIRE(_init, "Interface::LoG", "Layer is used before initializing.");
+    Tensor in = to4(input);
+    Tensor output(in.device(), kernel.type(),
+                  filterShape(in.shape(), 2 * kernel.shape(1), kernel.shape(2), kernel.shape(3), _pad.get(),
+                              "Shape::LoG"));
+    return (*this)(in, output);
+}
+
+ZCA &ZCA::fit(Tensor input, F64 epsilon, bool transform_)
+{
+    REQUIRE(input.dims() >= 2, "Interface::ZCA", "Input dimensions must be at least 2D.");
+    onGpu({input}, "Interface::ZCA");
+    REQUIRE(input.type() == Type::F32, "Interface::ZCA", "Input data type must be F32.");
+    const Size rows = input.shape(0), cols = input.numel() / rows;
+    mean = Tensor(input.device(), Type::F32, {cols});
+    transform = Tensor(input.device(), Type::F32, {cols, cols});
+    check(spk_zca_fit(input.data(), mean.data(), transform.data(), rows, cols, epsilon, transform_ ? 1 : 0, nullptr),
+          "Interface::ZCA");
+    return *this;
+}
+
+Tensor ZCA::operator()(Tensor input, bool inplace)
+{
+    REQUIRE(bool(mean), "Interface::ZCA", "Layer is not initialized.");
+    REQUIRE(bool(transform), "Interface::ZCA", "Layer is not initialized.");
+    if (!inplace) input = input.copy();
+    Tensor in = atLeast(input, 2, "2D");
+    onGpu({in, mean, transform}, "Interface::ZCA");
+    REQUIRE(in.type() == Type::F32, "Interface::ZCA", "Input data type must be F32.");
+    const Size rows = in.shape(0), cols = rows ? in.numel() / rows : 0;
+    REQUIRE(mean.numel() == cols && transform.shape() == Shape({cols, cols}), "Shape::ZCA",
+            "Input and mean shapes are not compatible.");
+    check(spk_zca_trans(in.data(), mean.data(), transform.data(), rows, cols, nullptr), "Interface::ZCA");
+    return input;
+}
+
+Tensor ZCA::split(Tensor input, Tensor output)
+{
+    Tensor in = atLeast(input, 2, "2D");
+    onGpu({in, output}, "Interface::ZCA");
+    REQUIRE(in.type() == output.type(), "Interface::ZCA", "Input and output type must have the same type.");
+    Shape s = in.shape();
+    s.insert(s.begin() + 1, 2);
+    REQUIRE(output.numel() == in.numel() * 2 && output.shape(0) == in.shape(0), "Interface::ZCA",
+            "Output shape is incorrect.");
+    const Size rows = in.shape(0), cols = rows ? in.numel() / rows : 0;
+    check(spk_zca_split(in.data(), output.data(), ty(in.type()), rows, cols, nullptr), "Interface::ZCA");
+    return output;
+}
+
+Tensor ZCA::split(Tensor input)
+{
+    Tensor in = atLeast(input, 2, "2D");
+    Shape s = in.shape();
+    s.insert(s.begin() + 1, 2);
+    return split(in, Tensor(in.device(), in.type(), s));
 }
 
 Gabor::Gabor(Size size, const std::vector<GaborFilter> &filters, Expand4 pad, Type type)

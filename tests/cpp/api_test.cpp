@@ -11,6 +11,10 @@
 //                          sparse path (SparseTensor, Conv on lists, Sparse::
 //                          inhibit/convwta/pool/gather/fcwta, Conv::stdp on
 //                          lists) and the dense readout, writes the results
+//   api_test front DIR     reads DIR/{img.f32,z.f32,pot.f32,shape.txt}, runs
+//                          LoG, ZCA (fit, apply, split), the Poisson rate
+//                          code after randomSeed and rate fire, writes the
+//                          results
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -191,6 +195,46 @@ int sparse(const std::string &dir)
     if (failures == 0) std::cout << "sparse ok\n";
     return failures ? 1 : 0;
 }
+int front(const std::string &dir)
+{
+    Size n, h, w, rows, cols, t, seed;
+    {
+        std::ifstream f(dir + "/shape.txt");
+        f >> n >> h >> w >> rows >> cols >> t >> seed;
+        if (!f) throw std::runtime_error("cannot read shape.txt");
+    }
+    const Device gpu(Kind::CUDA);
+    Tensor img = readFile(dir + "/img.f32", Type::F32, {n, 1, h, w}).to(gpu);
+    LoG lg(gpu, 3, {1.0, 2.0}, 3);
+    Tensor l = lg(img);
+    expect(l.shape() == (Shape{n, 4, h, w}), "log shape");
+    writeFile(dir + "/log.out", l);
+    writeFile(dir + "/log_kernel.out", lg.kernel);
+
+    Tensor z = readFile(dir + "/z.f32", Type::F32, {rows, cols}).to(gpu);
+    ZCA zca;
+    zca.fit(z, 1e-2);
+    Tensor white = zca(z, false);
+    expect(white.data() != z.data(), "zca copy when not in place");
+    writeFile(dir + "/zca_mean.out", zca.mean);
+    writeFile(dir + "/zca_trans.out", zca.transform);
+    writeFile(dir + "/zca_white.out", white);
+    Tensor sp = ZCA::split(white);
+    expect(sp.shape() == (Shape{rows, 2, cols}), "zca split shape");
+    writeFile(dir + "/zca_split.out", sp);
+
+    randomSeed(seed);
+    Tensor maps = l.reshape({n, 4 * h * w});
+    writeFile(dir + "/rate_in.out", maps);
+    writeFile(dir + "/rate.out", code(maps, t, true, Type::U8, Code::Rate));
+
+    Tensor pot = readFile(dir + "/pot.f32", Type::F32, {n, t, 4, h, w}).to(gpu);
+    writeFile(dir + "/rate_fire.out", fire(pot, 0.25, Type::U8, Code::Rate));
+    expectThrow([&] { ZCA().fit(img.to(Type::F64), 0.1); }, "Input data type must be F32.", "zca type");
+    expectThrow([&] { ZCA()(z); }, "Layer is not initialized.", "zca before fit");
+    if (failures == 0) std::cout << "front ok\n";
+    return failures ? 1 : 0;
+}
 }  // namespace
 
 int main(int argc, char **argv)
@@ -200,7 +244,8 @@ int main(int argc, char **argv)
         if (argc >= 2 && std::string(argv[1]) == "host") return host();
         if (argc >= 3 && std::string(argv[1]) == "layer") return layer(argv[2]);
         if (argc >= 3 && std::string(argv[1]) == "sparse") return sparse(argv[2]);
-        std::cerr << "usage: api_test host | api_test layer DIR | api_test sparse DIR\n";
+        if (argc >= 3 && std::string(argv[1]) == "front") return front(argv[2]);
+        std::cerr << "usage: api_test host | api_test layer DIR | api_test sparse DIR | api_test front DIR\n";
         return 2;
     }
     catch (const std::exception &e)

@@ -36,7 +36,8 @@ def test_cpp_library_exports_the_interface():
                  "Spyker::pad(", "Spyker::layerStep(", "Spyker::Tensor::to(", "Spyker::gather(", "Spyker::scatter(",
                  "Spyker::SparseTensor::SparseTensor(", "Spyker::SparseTensor::dense(", "Spyker::Sparse::conv(",
                  "Spyker::Sparse::pool(", "Spyker::Sparse::inhibit(", "Spyker::Sparse::convwta(",
-                 "Spyker::Sparse::gather(", "Spyker::Sparse::code("]:
+                 "Spyker::Sparse::gather(", "Spyker::Sparse::code(", "Spyker::LoG::LoG(", "Spyker::ZCA::fit(",
+                 "Spyker::ZCA::split("]:
         assert name in out, name
 
 
@@ -142,3 +143,49 @@ def test_cpp_sparse_matches_oracle(tmp_path, orc):
     times = orc.rank_gather(dense)
     assert np.array_equal(rd("times.out", np.int32, times.shape), times)
     assert np.array_equal(rd("scatter.out", np.uint8, dense.shape), dense)
+
+
+@pytest.mark.gpu
+def test_cpp_front_ends_match_oracle(tmp_path, orc):
+    """LoG, ZCA, the Poisson rate code and rate fire through the C++ interface."""
+    import ctypes as C
+    from spyker_amd import _lib
+    n, h, w, rows, cols, t, seed = 6, 28, 28, 800, 48, 15, 4242
+    rng = np.random.default_rng(23)
+    img = rng.random((n, 1, h, w)).astype(np.float32)
+    z = (rng.normal(size=(rows, cols)) @ rng.normal(size=(cols, cols)) * 0.3).astype(np.float32)
+    pot = np.cumsum(rng.random((n, t, 4, h, w)).astype(np.float32) * 0.3, axis=1).astype(np.float32)
+    d = str(tmp_path)
+    img.tofile(os.path.join(d, "img.f32"))
+    z.tofile(os.path.join(d, "z.f32"))
+    pot.tofile(os.path.join(d, "pot.f32"))
+    with open(os.path.join(d, "shape.txt"), "w") as f:
+        f.write("%d %d %d %d %d %d %d\n" % (n, h, w, rows, cols, t, seed))
+    r = subprocess.run([_driver(), "front", d], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "front ok" in r.stdout, r.stdout + r.stderr
+
+    def rd(name, dtype, shape):
+        return np.fromfile(os.path.join(d, name), dtype=dtype).reshape(shape)
+
+    lk = rd("log_kernel.out", np.float32, (2, 2, 7, 7))
+    assert np.array_equal(rd("log.out", np.float32, (n, 4, h, w)), orc.log(img, lk, (3, 3, 3, 3)))
+    m64, t64 = orc.zca_fit(z.astype(np.float64), 1e-2)
+    m, tr = rd("zca_mean.out", np.float32, (cols,)), rd("zca_trans.out", np.float32, (cols, cols))
+    assert (np.abs(m - m64) <= np.spacing(np.abs(m64).astype(np.float32))).all()
+    assert (np.abs(tr - t64) <= 4 * 2.0 ** -24 * np.abs(t64) + 1e-9 * np.abs(t64).max()).all()
+    white = rd("zca_white.out", np.float32, (rows, cols))
+    ref = orc.zca_trans(z, m, tr)
+    bound = np.abs(z.astype(np.float64) - m) @ np.abs(tr.astype(np.float64))
+    assert (np.abs(white - ref) <= 2.0 ** -24 * np.abs(ref) + 1e-12 * bound).all()
+    assert np.array_equal(rd("zca_split.out", np.float32, (rows, 2, cols)), orc.zca_split(white))
+    maps = rd("rate_in.out", np.float32, (n, 4 * h * w))
+    table = np.zeros((t + 1, 1024), np.uint16)
+    assert _lib.load().spk_poisson_table(table.ctypes.data_as(C.c_void_p), C.c_int64(t + 1), C.c_int64(1024)) == 0
+    above = min(int((row > row.min()).sum()) for row in maps)
+    extra = max(0, min(65536, int((maps.shape[1] - 1) * (t + 1) / max(1, above)) + 2) - (t + 1))
+    table = np.concatenate([table, np.repeat(table[-1:], extra, axis=0)])
+    bg = np.random.MT19937(0)
+    bg._legacy_seeding(seed)
+    rnd = int(bg.random_raw(1)[0])
+    assert np.array_equal(rd("rate.out", np.uint8, (n, t, 4 * h * w)), orc.rate_code(maps, t, True, rnd, table))
+    assert np.array_equal(rd("rate_fire.out", np.uint8, pot.shape), orc.rate_fire(pot, 0.25))
