@@ -25,21 +25,24 @@
 //   - wave w owns output channels [32w, 32w+32) as two 16-row MFMA tiles;
 //     its FP16 weight fragments for all K stay in VGPRs for the whole kernel;
 //   - the sample's input for all T steps is staged once in LDS as U8
-//     [t][y][x][cin_pad] (channel innermost);
-//   - columns are 16 row-major output pixels.  For a column tile, the FP16
-//     windows of a chunk of steps are widened once for the block into
-//     [step][row][cin_pad/8 planes of padded width], row pitch = wo mod 16 so
-//     the 16 pixels of one B fragment fall in distinct LDS banks; every
-//     wave then runs the chunk's steps without barriers, loading the next
-//     step's B fragments while it works on the current one;
+//     [t][y][x][cin_pad] (channel innermost) inside a zero border of the
+//     conv's padding, so every tap of every output pixel is in the stage;
+//   - columns are 16 row-major output pixels.  A lane's B fragment is 8
+//     bytes of the stage (8 channels at one tap of its pixel) zero-extended
+//     to 16 bits: the FP16 subnormals b * 2^-24, which the MFMA multiplies
+//     exactly, so the potentials come out exactly 2^-24 times the sums with
+//     b as FP16 integers (the threshold is scaled to match).  Every wave
+//     runs a tile's T steps without barriers, loading the bytes of step k+2
+//     while the MFMAs of step k+1 run;
 //   - per step and (channel, pixel) a lane only shifts "v > th" into a mask
 //     and keeps the running maximum;
 //   - at the end of a tile, cnt = popcount(OR of all masks) gives inhibit's
-//     step ts = T - max(cnt, 1); one MFMA pass over B fragments gathered
-//     from the stage at each pixel's own ts yields the kept channel c*, and
-//     a second at c*'s first firing step yields convwta's t0 value.  The
-//     gathered passes issue the same MFMA sequence on the same operands as
-//     the step loop, so they reproduce its potentials bit for bit.
+//     step ts = T - max(cnt, 1); one MFMA pass with each lane reading its
+//     pixel's bytes at that pixel's own ts yields the kept channel c*, and a
+//     second at c*'s first firing step yields convwta's t0 value.  The
+//     passes issue the same MFMA sequence on the same operands as the step
+//     loop, so they reproduce its potentials bit for bit.  Two barriers a
+//     tile exchange the per-pixel partials between waves.
 #include <mutex>
 
 #include "fused.hpp"
@@ -52,97 +55,145 @@ struct FusedGeom
 {
     int32_t n, t, cin, h, w, cout, kh, kw, pt, pl, ho, wo;
     int32_t ph, pw, cin_pad, groups, steps, waves, tiles;
-    int32_t stage_bytes;  // bytes of the staged sample ([t][h][w][cin_pad] U8, 16-aligned)
-    int32_t rows;         // window rows of a column tile (at most)
-    int32_t pitch;        // 16-byte units per window row: cin_pad/8 planes of pw, padded to = wo (mod 16)
-    int32_t win_units;    // 16-byte units of one step's FP16 window (rows * pitch)
-    int32_t tc;           // steps whose windows are widened together
+    int32_t stage_bytes;  // bytes of the staged sample ([t][ph][pw][cin_pad] U8, 16-aligned)
+    int32_t sstep;        // bytes of one staged step (ph * pw * cin_pad)
     int32_t stage_fast;   // staging by quad transposes (hw % 4 == 0, aligned input, cin_pad/4 | block quads)
-    float th;
+    float th;             // the conv threshold times 2^-24 (potentials are computed times 2^-24)
 };
 
-// Where K-group grp (8 input channels at one tap) of a pixel lives: byte
-// offset in the stage relative to the pixel's own position, the tap's
-// row/column offset (for the bounds test) and its unit offset in a window.
-// Groups past the real K (zero weights) reuse group 0's operands.
-struct TapEntry
-{
-    int32_t soff, dy, dx, woff;
-};
-
-__device__ __forceinline__ TapEntry tap_entry(const FusedGeom &g, int32_t grp)
+// Byte offset of K-group grp (8 input channels at one tap) in the stage,
+// relative to the output pixel's own position (the top-left tap of its
+// window in the zero-bordered stage).  Groups past the real K (zero weights)
+// reuse group 0's operands.
+__device__ __forceinline__ int32_t tap_off(const FusedGeom &g, int32_t grp)
 {
     const int32_t cg = g.cin_pad / 8;
-    const int32_t gg = grp < g.groups ? grp : 0, tap = gg / cg, c8 = gg % cg, ky = tap / g.kw, kx = tap % g.kw;
-    const int32_t dy = ky - g.pt, dx = kx - g.pl;
-    return TapEntry{(dy * g.w + dx) * g.cin_pad + c8 * 8, dy, dx, ky * g.pitch + c8 * g.pw + kx};
+    const int32_t gg = grp < g.groups ? grp : 0, tap = gg / cg, c8 = gg % cg;
+    return ((tap / g.kw) * g.pw + tap % g.kw) * g.cin_pad + c8 * 8;
 }
 
-
-// Writes input byte i of the sample's [t][cin][h][w] tensor to its
-// [t][y][x][c] slot of the (unpadded, channel-innermost) U8 stage.
-__device__ __forceinline__ void stage_bytes(uint8_t *stage, const FusedGeom &g, int32_t i, uint8_t v, int32_t chw,
-                                            int32_t hwi)
+// B fragments of one step: the lane's 8 bytes of K-group 4s+q at its pixel.
+template <int S>
+__device__ __forceinline__ void load_raw(uint2 (&r)[S], const uint8_t *base, const int32_t (&off)[S])
 {
-    const int32_t tt = i / chw, r = i - tt * chw, c = r / hwi, pix = r - c * hwi;
-    stage[(tt * hwi + pix) * g.cin_pad + c] = v;
+#pragma unroll
+    for (int s = 0; s < S; ++s) r[s] = *reinterpret_cast<const uint2 *>(base + off[s]);
 }
 
-template <int S, bool DIAG>
-__global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ in, const half8 *__restrict__ frag,
-                                                   Summary *__restrict__ summary, float *__restrict__ pot_out,
-                                                   uint8_t *__restrict__ fs_out, FusedGeom g)
+// The MFMA chain of one step on raw spike bytes (as FP16 subnormals), for
+// the RT 16-row tiles of a wave.
+template <int RT, int S>
+__device__ __forceinline__ void mfma_sub(const half8 (&a)[RT][S], const uint2 (&r)[S], float4v (&c)[RT])
 {
-    extern __shared__ __align__(16) uint8_t lds[];
-    uint8_t *stage = lds;                                                    // [t][h][w][cin_pad] U8
-    uint4 *win16 = reinterpret_cast<uint4 *>(lds + g.stage_bytes);           // [tc][rows][pitch] FP16 x 8
-    uint4 *patch = win16 + g.tc * g.win_units;                               // [4S][COLS] gathered B fragments
-    uint32_t *any_s = reinterpret_cast<uint32_t *>(patch + 4 * S * COLS);    // [MAX_WAVES][COLS]
-    float *red_v = reinterpret_cast<float *>(any_s + MAX_WAVES * COLS);      // [MAX_WAVES][COLS]
-    int32_t *red_i = reinterpret_cast<int32_t *>(red_v + MAX_WAVES * COLS);  // [MAX_WAVES][COLS]
-    int32_t *tsel = red_i + MAX_WAVES * COLS;                                // [COLS]
-    int4 *gtab = reinterpret_cast<int4 *>(tsel + COLS);                      // [4S] TapEntry
-
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, q = lane >> 4, j = lane & 15;
-    const int32_t hw = g.ho * g.wo, hwi = g.h * g.w, cg = g.cin_pad / 8;
-    const int32_t o0 = w * 32 + 4 * q;  // the lane's first channel; rows r*16 + i follow
-
-    // weight fragments and window offsets, resident for the whole kernel
-    half8 a[2][S];
-    int32_t offs[S];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) c[rt] = float4v{0, 0, 0, 0};
 #pragma unroll
     for (int s = 0; s < S; ++s)
     {
-        a[0][s] = frag[((w * 2 + 0) * S + s) * 64 + lane];
-        a[1][s] = frag[((w * 2 + 1) * S + s) * 64 + lane];
-        offs[s] = tap_entry(g, 4 * s + q).woff * 16;
+        const half8 b = __builtin_bit_cast(half8, subnormals(r[s]));
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) c[rt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[rt][s], b, c[rt], 0, 0, 0);
     }
-    for (int32_t e = threadIdx.x; e < 4 * S; e += blockDim.x)
-    {
-        const TapEntry te = tap_entry(g, e);
-        gtab[e] = make_int4(te.soff, te.dy, te.dx, te.woff);
-    }
-    const int32_t win_step = g.win_units * 16;  // bytes of one step's window
+}
 
-    // window build: a thread owns one unit of a window row (channel plane
-    // c8, padded column col) and walks the (step, row) pairs
-    const int32_t row_units = cg * g.pw, lanes_per = int32_t(blockDim.x) / row_units;
-    const int32_t cu = int32_t(threadIdx.x) % row_units, slot = int32_t(threadIdx.x) / row_units;
-    const int32_t bc8 = cu / g.pw, bix = cu % g.pw - g.pl;
-    const bool bcol = bix >= 0 && bix < g.w;
+// The update of every value of one step (see value_update).
+template <int RT>
+__device__ __forceinline__ void update_all(const float4v (&u)[RT], float th, uint32_t (&mask)[RT][4],
+                                           int32_t (&mxb)[RT][4])
+{
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) value_update(u[rt][i], th, mask[rt][i], mxb[rt][i]);
+}
+
+// One step of a tile, interleaved for a single wave: the MFMA chain of step
+// k+1 on its raw bytes `rin`, and between its MFMAs the update of step k's
+// values `u` and the loads of step k+2's bytes into `rout`.  The conversions,
+// value updates and addresses ride along with the MFMAs, so one wave keeps
+// the matrix core and the VALU busy together instead of alternating between
+// an MFMA phase and a VALU phase.  Value v is updated after MFMA number
+// RT + v * (S*RT - RT) / (4*RT): the first fragment's MFMAs give the previous
+// chain's last results time to land.
+template <int RT, int S>
+__device__ __forceinline__ void step_iter(const half8 (&a)[RT][S], const uint2 (&rin)[S], float4v (&c)[RT],
+                                          const float4v (&u)[RT], float th, uint32_t (&mask)[RT][4],
+                                          int32_t (&mxb)[RT][4], uint2 (&rout)[S], const uint8_t *next,
+                                          const int32_t (&toff)[S])
+{
+    constexpr int V = 4 * RT, SPAN = S * RT - RT;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) c[rt] = float4v{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+    {
+        const half8 b = __builtin_bit_cast(half8, subnormals(rin[s]));
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+        {
+            c[rt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[rt][s], b, c[rt], 0, 0, 0);
+#pragma unroll
+            for (int v = 0; v < V; ++v)
+                if (SPAN > 0 && RT + (v * SPAN) / V == s * RT + rt)
+                    value_update(u[v / 4][v % 4], th, mask[v / 4][v % 4], mxb[v / 4][v % 4]);
+        }
+        rout[s] = *reinterpret_cast<const uint2 *>(next + toff[s]);
+    }
+    if (SPAN <= 0) update_all(u, th, mask, mxb);
+    // keep the next iteration's conversions below this one's loads
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// RT 16-row tiles per wave: 4 (64 channels, one wave per SIMD, with the
+// accumulation registers of the whole file) halves the byte conversions and
+// LDS reads per MFMA against 2 (32 channels, two waves per SIMD).
+template <int RT>
+constexpr int layer_threads()
+{
+    return RT == 4 ? 256 : 512;
+}
+
+template <int RT, int S, bool DIAG>
+__global__ void __launch_bounds__(layer_threads<RT>())
+    fused_layer(const uint8_t *__restrict__ in, const half8 *__restrict__ frag, Summary *__restrict__ summary,
+                float *__restrict__ pot_out, uint8_t *__restrict__ fs_out, FusedGeom g)
+{
+    extern __shared__ __align__(16) uint8_t lds[];
+    uint8_t *stage = lds;                                                       // [t][ph][pw][cin_pad] U8
+    uint32_t *any_s = reinterpret_cast<uint32_t *>(lds + g.stage_bytes);        // [2][MAX_WAVES][COLS]
+    float *red_v = reinterpret_cast<float *>(any_s + 2 * MAX_WAVES * COLS);     // [2][MAX_WAVES][COLS]
+    int32_t *red_i = reinterpret_cast<int32_t *>(red_v + 2 * MAX_WAVES * COLS);  // [2][MAX_WAVES][COLS]
+
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, q = lane >> 4, j = lane & 15;
+    const int32_t hw = g.ho * g.wo, hwi = g.h * g.w, cg = g.cin_pad / 8;
+    const int32_t o0 = w * 16 * RT + 4 * q;  // the lane's first channel; value (r, i) is channel o0 + 16r + i
+
+    // weight fragments and tap offsets, resident for the whole kernel
+    half8 a[RT][S];
+    int32_t toff[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+    {
+#pragma unroll
+        for (int r = 0; r < RT; ++r) a[r][s] = frag[((w * RT + r) * S + s) * 64 + lane];
+        toff[s] = tap_off(g, 4 * s + q);
+    }
+    // the zero border and the padded channels are written once; every sample
+    // rewrites only the interior
+    for (int32_t i = threadIdx.x; i < g.stage_bytes / 16; i += blockDim.x)
+        reinterpret_cast<uint4 *>(stage)[i] = make_uint4(0, 0, 0, 0);
 
     for (int32_t n = blockIdx.x; n < g.n; n += gridDim.x)
     {
         __syncthreads();
-        // stage the sample as [t][y][x][cin_pad] bytes
+        // stage the sample as [t][y][x][cin_pad] bytes inside the zero border
         if (g.stage_fast)
         {
             // a lane loads one dword (4 pixels of channel c) of the
             // [t][cin][h][w] input; the 4 lanes of a quad hold channels
             // 4*c4 .. 4*c4+3 of the same 4 pixels and transpose the 4x4 bytes
             // (two DPP exchanges + v_perm), after which lane l4 holds the 4
-            // channels of pixel l4: one dword store per lane.  With
-            // cin_pad / 4 dividing 16, the 64 stores of a wave hit 64 banks.
+            // channels of pixel l4: one dword store per lane
             const int32_t D = g.cin_pad / 4, npq = hwi / 4, step = int32_t(blockDim.x) / 4 / D;
             const int32_t Q = int32_t(threadIdx.x) >> 2, l4 = int32_t(threadIdx.x) & 3, c4 = Q % D;
             const int32_t c = 4 * c4 + l4;
@@ -155,10 +206,17 @@ __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ i
             uint32_t *dst = reinterpret_cast<uint32_t *>(stage);
             // lin = tt * npq + pq walks the (step, pixel quad) pairs by `step`
             // (<= npq, so pq wraps at most once per move); off is the byte
-            // offset of the lane's dword, channel c of step tt
+            // offset of the lane's dword, channel c of step tt.  U loads are
+            // in flight per lane before the first store.
             int32_t lin = Q / D, pq = lin, off = (c * npq + pq) * 4;
             const int32_t total = g.t * npq, wrap_add = (g.cin - 1) * npq * 4;
-            constexpr int U = 8;
+            // the store side walks the same pairs as (tt, pixel row, column)
+            // of the lane's own pixel 4*pq + l4, moving 4*step pixels a time:
+            // dy rows and dx columns, one more row when the column wraps, and
+            // back h rows into the next step when the pixel passes h*w
+            int32_t sq = pq, st = 0, sy = (4 * pq + l4) / g.w, sx = (4 * pq + l4) % g.w;
+            const int32_t dy = 4 * step / g.w, dx = 4 * step % g.w;
+            constexpr int U = 16;
             for (; lin < total; lin += U * step)
             {
                 uint32_t v[U];
@@ -178,25 +236,30 @@ __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ i
                     const uint32_t y = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(x), 0xB1, 0xF, 0xF, true));  // lane^1
                     const uint32_t r1 = __builtin_amdgcn_perm(y, x, sel1);
                     const uint32_t y2 = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(r1), 0x4E, 0xF, 0xF, true));  // lane^2
-                    const int32_t li = lin + u * step;
-                    if (li < total) dst[(4 * li + l4) * D + c4] = __builtin_amdgcn_perm(y2, r1, sel2);
+                    if (lin + u * step < total)
+                        dst[((st * g.ph + sy + g.pt) * g.pw + sx + g.pl) * D + c4] = __builtin_amdgcn_perm(y2, r1, sel2);
+                    sq += step, sy += dy, sx += dx;
+                    sy = sx >= g.w ? sy + 1 : sy;
+                    sx = sx >= g.w ? sx - g.w : sx;
+                    const bool wrap = sq >= npq;
+                    sq = wrap ? sq - npq : sq;
+                    st = wrap ? st + 1 : st;
+                    sy = wrap ? sy - g.h : sy;
                 }
             }
         }
         else
         {
-            // any other shape: zero (for the padded channels), then scatter
-            // the [t][cin][h][w] input byte by byte
-            if (g.cin != g.cin_pad)
-            {
-                uint4 *z = reinterpret_cast<uint4 *>(stage);
-                for (int32_t i = threadIdx.x; i < g.stage_bytes / 16; i += blockDim.x) z[i] = make_uint4(0, 0, 0, 0);
-                __syncthreads();
-            }
+            // any other shape: scatter the [t][cin][h][w] input byte by byte
+            // (the padded channels stay 0)
             const uint8_t *src = in + int64_t(n) * g.t * g.cin * hwi;
             const int32_t chw = g.cin * hwi, total = g.t * chw;
             for (int32_t i = int32_t(threadIdx.x); i < total; i += int32_t(blockDim.x))
-                stage_bytes(stage, g, i, src[i], chw, hwi);
+            {
+                const int32_t tt = i / chw, r = i - tt * chw, c = r / hwi, pix = r - c * hwi, yy = pix / g.w;
+                const int32_t xx = pix - yy * g.w;
+                stage[((tt * g.ph + yy + g.pt) * g.pw + xx + g.pl) * g.cin_pad + c] = src[i];
+            }
         }
         __syncthreads();
         if (fs_out)
@@ -206,11 +269,12 @@ __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ i
             const uint32_t tb = uint32_t(g.t) * 0x01010101u;
             for (int32_t u = threadIdx.x; u < hwi * cg; u += blockDim.x)
             {
-                const uint8_t *sp = stage + u * 8;
+                const int32_t pix = u / cg, c8 = u - pix * cg, py = pix / g.w, px = pix - py * g.w;
+                const uint8_t *sp = stage + ((py + g.pt) * g.pw + px + g.pl) * g.cin_pad + c8 * 8;
                 uint32_t lo = tb, hi = tb, seen_lo = 0, seen_hi = 0;
                 for (int32_t tt = 0; tt < g.t; ++tt)
                 {
-                    const uint2 v = *reinterpret_cast<const uint2 *>(sp + tt * hwi * g.cin_pad);
+                    const uint2 v = *reinterpret_cast<const uint2 *>(sp + tt * g.sstep);
                     first_nonzero(lo, seen_lo, v.x, tt);
                     first_nonzero(hi, seen_hi, v.y, tt);
                 }
@@ -220,180 +284,152 @@ __global__ void __launch_bounds__(512) fused_layer(const uint8_t *__restrict__ i
 
         for (int32_t tile = 0; tile < g.tiles; ++tile)
         {
+            // reduction buffers alternate between tiles, so two barriers a
+            // tile order every write after the reads of the same buffer
+            const int32_t buf = (tile & 1) * MAX_WAVES * COLS;
             const int32_t p = tile * COLS + j;
             const bool pvalid = p < hw;
             const int32_t pc = pvalid ? p : hw - 1, y = pc / g.wo, x = pc % g.wo;
-            // the tile's window: padded rows [y_first, y_first + rows), all padded columns
-            const int32_t y_first = (tile * COLS) / g.wo;
-            const int32_t rows = min(tile * COLS + COLS - 1, hw - 1) / g.wo - y_first + g.kh;
-            const uint8_t *wb = reinterpret_cast<const uint8_t *>(win16) + ((y - y_first) * g.pitch + x) * 16;
-            uint32_t mask[2][4];
-            int32_t mxb[2][4];
+            const uint8_t *pb = stage + (y * g.pw + x) * g.cin_pad;  // the pixel's window at step 0
+            uint32_t mask[RT][4];
+            int32_t mxb[RT][4];
 #pragma unroll
-            for (int r = 0; r < 2; ++r)
+            for (int r = 0; r < RT; ++r)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) mask[r][i] = 0, mxb[r][i] = 0;
 
-            for (int32_t t0 = 0; t0 < g.t; t0 += g.tc)
+            // the steps, no barriers: each iteration runs the MFMAs of step
+            // k+1 with the update of step k and the loads of step k+2 (loads
+            // past the last step re-read it, unused)
             {
-                const int32_t tc = min(g.tc, g.t - t0);
-                // widen the windows of steps [t0, t0 + tc) once for the block:
-                // a thread slot takes steps slot, slot + lanes_per, ... and walks
-                // their rows; rows in the zero pad ([0, r_lo) and [r_hi, rows))
-                // and columns outside the input are written 0
-                if (slot < lanes_per)
-                {
-                    const int32_t r_lo = max(0, g.pt - y_first), r_hi = min(rows, g.h + g.pt - y_first);
-                    const int32_t row_bytes = g.w * g.cin_pad;
-                    for (int32_t ts = slot; ts < tc; ts += lanes_per)
-                    {
-                        const uint8_t *sp =
-                            stage + (((t0 + ts) * g.h + y_first - g.pt) * g.w + bix) * g.cin_pad + bc8 * 8;
-                        uint4 *dp = win16 + ts * g.win_units + cu;
-                        for (int32_t r = 0; r < rows; ++r)
-                        {
-                            uint2 raw = make_uint2(0, 0);
-                            if (bcol && r >= r_lo && r < r_hi)
-                                raw = *reinterpret_cast<const uint2 *>(sp + r * row_bytes);
-                            dp[r * g.pitch] = widen(raw);
-                        }
-                    }
-                }
-                __syncthreads();
-                // the chunk's steps without barriers: the loads of step k+2 and
-                // the MFMAs of step k+1 are issued before the update of step k
-                // (loads past the chunk re-read its last step, unused)
-                half8 b0[S], b1[S];
-                float4v c0, c1, n0, n1;
-                load_b(b0, wb, offs);
-                mfma_chain(a, b0, c0, c1);
-                load_b(b0, wb + min(1, tc - 1) * win_step, offs);
-                auto update = [&](int32_t k, const float4v &u0, const float4v &u1) {
-                    step_update(u0, u1, g.th, mask, mxb);
+                const int32_t last = g.t - 1;
+                uint2 rA[S], rB[S];
+                float4v cA[RT], cB[RT];
+                auto diag = [&](int32_t k, const float4v (&u)[RT]) {
                     if (DIAG && pvalid)
 #pragma unroll
-                        for (int r = 0; r < 2; ++r)
+                        for (int r = 0; r < RT; ++r)
 #pragma unroll
                             for (int i = 0; i < 4; ++i)
                             {
                                 const int32_t o = o0 + r * 16 + i;
                                 if (o < g.cout)
-                                    pot_out[((int64_t(n) * g.t + t0 + k) * g.cout + o) * hw + p] =
-                                        r == 0 ? u0[i] : u1[i];
+                                    pot_out[((int64_t(n) * g.t + k) * g.cout + o) * hw + p] = u[r][i] * TWO_POW_24;
                             }
                 };
-                for (int32_t k = 0; k < tc; k += 2)
+                load_raw(rA, pb, toff);
+                load_raw(rB, pb + min(1, last) * g.sstep, toff);
+                mfma_sub(a, rA, cA);
+                // pairs of iterations with no branch between them, so the
+                // loads of one stay ahead of the MFMAs of the next
+                int32_t k = 0;
+                for (; k + 2 < g.t; k += 2)
                 {
-                    load_b(b1, wb + min(k + 2, tc - 1) * win_step, offs);
-                    if (k + 1 < tc) mfma_chain(a, b0, n0, n1);
-                    update(k, c0, c1);
-                    load_b(b0, wb + min(k + 3, tc - 1) * win_step, offs);
-                    if (k + 2 < tc) mfma_chain(a, b1, c0, c1);
-                    if (k + 1 < tc) update(k + 1, n0, n1);
+                    // cA: step k; rB: the bytes of step k+1
+                    step_iter(a, rB, cB, cA, g.th, mask, mxb, rA, pb + (k + 2) * g.sstep, toff);
+                    diag(k, cA);
+                    step_iter(a, rA, cA, cB, g.th, mask, mxb, rB, pb + min(k + 3, last) * g.sstep, toff);
+                    diag(k + 1, cB);
                 }
-                if (t0 + tc < g.t) __syncthreads();  // windows free for the next chunk
+                if (k + 1 < g.t)
+                {
+                    step_iter(a, rB, cB, cA, g.th, mask, mxb, rA, pb + last * g.sstep, toff);
+                    diag(k, cA);
+                    update_all(cB, g.th, mask, mxb);
+                    diag(k + 1, cB);
+                }
+                else
+                {
+                    update_all(cA, g.th, mask, mxb);
+                    diag(k, cA);
+                }
             }
 
             // steps at which some channel fired, per pixel (bit T-1-t for step t)
             {
                 uint32_t any = 0;
 #pragma unroll
-                for (int r = 0; r < 2; ++r)
+                for (int r = 0; r < RT; ++r)
 #pragma unroll
                     for (int i = 0; i < 4; ++i) any |= mask[r][i];
                 any |= __shfl_xor(any, 16, 64);
                 any |= __shfl_xor(any, 32, 64);
-                if (lane < COLS) any_s[w * COLS + lane] = any;
+                if (lane < COLS) any_s[buf + w * COLS + lane] = any;
             }
             __syncthreads();
-            // the B fragments of the tile's pixels, each at its own step st
-            // (the same bytes the step loop's window held for that step)
-            auto gather = [&](int32_t st) {
-                for (int32_t grp = int32_t(threadIdx.x) >> 4; grp < 4 * S; grp += int32_t(blockDim.x) >> 4)
-                {
-                    const int4 te = gtab[grp];
-                    const int32_t iy = y + te.y, ix = x + te.z;
-                    uint2 raw = make_uint2(0, 0);
-                    if (st >= 0 && iy >= 0 && iy < g.h && ix >= 0 && ix < g.w)
-                        raw = *reinterpret_cast<const uint2 *>(stage + (st * hwi + y * g.w + x) * g.cin_pad + te.x);
-                    patch[grp * COLS + j] = widen(raw);
-                }
-            };
+            // pass 1: the potentials at inhibit's step ts = T - max(cnt, 1),
+            // each lane reading its pixel's operands at that pixel's own ts
+            // (the same bytes the step loop read there, so the same sums); the
+            // kept channel is the first maximum of the thresholded values
+            // (inhibit.cpp:64-85).  The lane's channels ascend with (r, i), so
+            // strict '>' keeps the first maximum; padded channels have zero
+            // weights and value 0, which never beats a lower channel.
+            float4v gv[RT];
             {
-                uint32_t any = 0;
-                for (int32_t ow = 0; ow < g.waves; ++ow) any |= any_s[ow * COLS + j];
-                const int32_t cnt = __builtin_popcount(any);
-                gather(g.t - (cnt > 1 ? cnt : 1));
-            }
-            __syncthreads();
-            int32_t poffs[S];
-#pragma unroll
-            for (int s = 0; s < S; ++s) poffs[s] = (s * 64 + lane) * 16;
-            // pass 1: the potentials at inhibit's step; the kept channel is the
-            // first maximum of the thresholded values (inhibit.cpp:64-85).  The
-            // lane's channels ascend with (r, i), so strict '>' keeps the first
-            // maximum; padded channels have zero weights and value 0, which never
-            // beats a lower channel.
-            float4v g0, g1;
-            {
-                half8 b[S];
-                load_b(b, reinterpret_cast<const uint8_t *>(patch), poffs);
-                mfma_chain(a, b, g0, g1);
+                uint32_t all = 0;
+                for (int32_t ow = 0; ow < g.waves; ++ow) all |= any_s[buf + ow * COLS + j];
+                const int32_t cnt = __builtin_popcount(all);
+                uint2 r[S];
+                load_raw(r, pb + (g.t - (cnt > 1 ? cnt : 1)) * g.sstep, toff);
+                mfma_sub(a, r, gv);
                 float bv = -1.0f;
                 int32_t bi = INT32_MAX;
 #pragma unroll
-                for (int r = 0; r < 2; ++r)
+                for (int r2 = 0; r2 < RT; ++r2)
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
                     {
-                        const float v = r == 0 ? g0[i] : g1[i];
+                        const float v = gv[r2][i];
                         const float xp = v > g.th ? v : 0.0f;
                         const bool take = xp > bv;
                         bv = take ? xp : bv;
-                        bi = take ? o0 + r * 16 + i : bi;
+                        bi = take ? o0 + r2 * 16 + i : bi;
                     }
                 combine(bv, bi, __shfl_xor(bv, 16, 64), __shfl_xor(bi, 16, 64));
                 combine(bv, bi, __shfl_xor(bv, 32, 64), __shfl_xor(bi, 32, 64));
-                if (lane < COLS) red_v[w * COLS + lane] = bv, red_i[w * COLS + lane] = bi;
+                if (lane < COLS) red_v[buf + w * COLS + lane] = bv, red_i[buf + w * COLS + lane] = bi;
             }
             __syncthreads();
             int32_t cs;
             {
-                float m = red_v[j];
-                cs = red_i[j];
-                for (int32_t ow = 1; ow < g.waves; ++ow) combine(m, cs, red_v[ow * COLS + j], red_i[ow * COLS + j]);
+                float m = red_v[buf + j];
+                cs = red_i[buf + j];
+                for (int32_t ow = 1; ow < g.waves; ++ow)
+                    combine(m, cs, red_v[buf + ow * COLS + j], red_i[buf + ow * COLS + j]);
             }
-            // the lane holding (c*, pixel) publishes c*'s first firing step
-            const bool own = (cs >> 5) == w && ((cs >> 2) & 3) == q;
+            // the lane holding (c*, pixel) has its fire mask and maximum; the
+            // lanes of the same pixel in its wave take c*'s first firing step
+            // from it
+            const int32_t cr = (cs % (16 * RT)) / 16, cq = (cs >> 2) & 3, ci = cs & 3;
+            const bool own = cs / (16 * RT) == w && cq == q;
             uint32_t om = 0;
             int32_t ox = 0;
 #pragma unroll
-            for (int r = 0; r < 2; ++r)
+            for (int r = 0; r < RT; ++r)
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    if (((cs >> 4) & 1) == r && (cs & 3) == i) om = mask[r][i], ox = mxb[r][i];
-            if (own) tsel[j] = om ? g.t - 32 + int32_t(__builtin_clz(om)) : -1;
-            __syncthreads();
-            gather(tsel[j]);
-            __syncthreads();
+                    if (cr == r && ci == i) om = mask[r][i], ox = mxb[r][i];
+            const int32_t my_t0 = om ? g.t - 32 + int32_t(__builtin_clz(om)) : 0;
+            const int32_t t0 = __shfl(my_t0, cq * 16 + j, 64);
             // pass 2: the value of c* at its first firing step (convwta's t0 value)
             float first = 0.0f;
             if (__builtin_amdgcn_ballot_w64(own && om != 0))
             {
-                half8 b[S];
-                load_b(b, reinterpret_cast<const uint8_t *>(patch), poffs);
-                mfma_chain(a, b, g0, g1);
+                uint2 r[S];
+                load_raw(r, pb + t0 * g.sstep, toff);
+                mfma_sub(a, r, gv);
 #pragma unroll
-                for (int r = 0; r < 2; ++r)
+                for (int r2 = 0; r2 < RT; ++r2)
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
-                        if (((cs >> 4) & 1) == r && (cs & 3) == i) first = r == 0 ? g0[i] : g1[i];
+                        if (cr == r2 && ci == i) first = gv[r2][i];
             }
             if (own && pvalid)
             {
                 const uint32_t fwd = om ? __builtin_bitreverse32(om) >> (32 - g.t) : 0u;
                 summary[int64_t(n) * hw + p] =
-                    Summary{cs, fwd, om ? first : 0.0f, om ? __builtin_bit_cast(float, ox) : 0.0f};
+                    Summary{cs, fwd, om ? first * TWO_POW_24 : 0.0f,
+                            om ? __builtin_bit_cast(float, ox) * TWO_POW_24 : 0.0f};
             }
         }
     }
@@ -625,16 +661,15 @@ SideStream::~SideStream()
 
 namespace
 {
-template <int S, bool DIAG>
+template <int RT, int S, bool DIAG>
 void launch_layer(const uint8_t *in, const half8 *frag, Summary *summary, float *pot, uint8_t *fs, const FusedGeom &g,
-                  size_t lds,
-                  hipStream_t s)
+                  size_t lds, hipStream_t s)
 {
     static bool configured = false;  // per instantiation; the attribute is per function
     if (!configured)
     {
-        SPK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&fused_layer<S, DIAG>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(lds > 65536 ? 163840 : 65536)));
+        SPK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&fused_layer<RT, S, DIAG>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 163840));
         configured = true;
     }
     int dev = 0, cus = 0;
@@ -649,7 +684,7 @@ void launch_layer(const uint8_t *in, const half8 *frag, Summary *summary, float 
         SPK_HIP(hipEventCreate(&e1));
         SPK_HIP(hipEventRecord(e0, s));
     }
-    fused_layer<S, DIAG><<<dim3(grid), dim3(64 * g.waves), lds, s>>>(in, frag, summary, pot, fs, g);
+    fused_layer<RT, S, DIAG><<<dim3(grid), dim3(64 * g.waves), lds, s>>>(in, frag, summary, pot, fs, g);
     SPK_LAUNCHED();
     if (tm.on)
     {
@@ -658,6 +693,7 @@ void launch_layer(const uint8_t *in, const half8 *frag, Summary *summary, float 
     }
 }
 
+template <int RT>
 void dispatch_layer(int steps, const uint8_t *in, const half8 *frag, Summary *summary, float *pot, uint8_t *fs,
                     const FusedGeom &g, size_t lds, hipStream_t s)
 {
@@ -665,8 +701,8 @@ void dispatch_layer(int steps, const uint8_t *in, const half8 *frag, Summary *su
     {
 #define CASE(k) \
     case k:                                                                                 \
-        return pot ? launch_layer<k, true>(in, frag, summary, pot, fs, g, lds, s)             \
-                   : launch_layer<k, false>(in, frag, summary, pot, fs, g, lds, s);
+        return pot ? launch_layer<RT, k, true>(in, frag, summary, pot, fs, g, lds, s)         \
+                   : launch_layer<RT, k, false>(in, frag, summary, pot, fs, g, lds, s);
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
         CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
 #undef CASE
@@ -743,23 +779,22 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
         g.cin_pad = int32_t(cdiv(cin, 8) * 8);
         g.groups = int32_t(kh * kw * (g.cin_pad / 8));
         g.steps = int32_t(cdiv(g.groups, 4));
-        g.waves = int32_t(cdiv(cout, 32));
+        // 32 channels a wave, two waves per SIMD (measured faster on the layer-2
+        // workload than 64 channels a wave with one wave per SIMD)
+        const int rt = 2;
+        g.waves = int32_t(cdiv(cout, 16 * rt));
         g.tiles = int32_t(cdiv(ho * wo, COLS));
-        g.stage_bytes = int32_t(cdiv(int64_t(t) * h * w * g.cin_pad, 16) * 16);
-        // a tile of COLS row-major pixels spans at most ceil((COLS-1)/wo)+1 rows
-        g.rows = int32_t(cdiv(COLS - 1, wo) + kh);
-        // row pitch = wo (mod 16) puts 16 consecutive output pixels in 16
-        // distinct 16-byte bank groups
-        const int64_t row_units = int64_t(g.pw) * (g.cin_pad / 8);
-        g.pitch = int32_t(row_units + ((wo - row_units) % 16 + 16) % 16);
-        g.win_units = g.rows * g.pitch;
-        g.th = float(threshold);
+        // the zero-bordered stage holds every tap of every output pixel
+        const int64_t sstep = int64_t(g.ph) * g.pw * g.cin_pad;
+        const int64_t stage_bytes = cdiv(int64_t(t) * sstep, 16) * 16;
+        // potentials come out of the MFMAs times 2^-24 (spike bytes enter as
+        // FP16 subnormals); the threshold is scaled to match (exact)
+        g.th = float(threshold) * (1.0f / TWO_POW_24);
+        SPK_ARG(threshold_scales(threshold), "Fused layer: the threshold is too small to scale exactly.");
         SPK_ARG(g.steps <= MAX_STEPS, "Fused layer: Cin*Kh*Kw too large (K <= 512 after padding).");
-        // stage, gathered fragments, any/argmax partials, step table, tap table
-        const size_t fixed = size_t(g.stage_bytes) + size_t(g.steps) * 4 * COLS * 16 + MAX_WAVES * COLS * 12 +
-                             COLS * 4 + size_t(g.steps) * 4 * 16;
-        const int64_t room = (163840 - int64_t(fixed)) / (int64_t(g.win_units) * 16);
-        if (room < 1 || row_units > 64 * g.waves || ho * wo > 8192 || fixed > 163840)
+        // stage + any/argmax partials (two buffers each)
+        const int64_t lds = stage_bytes + 3 * 2 * MAX_WAVES * COLS * 4;
+        if (lds > 163840 || ho * wo > 8192)
         {
             // the sample does not fit the LDS whole: the region path
             const StdpConfigs rcfg = learn ? pack_configs(configs, nconfigs) : StdpConfigs{};
@@ -768,12 +803,9 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
                               (uint8_t *)pooled, pkh, pkw, psh, psw, winners, counts, (float *)potentials, S(stream));
             return;
         }
-        const int64_t chunks = cdiv(t, room < t ? room : t);
-        g.tc = int32_t(cdiv(t, chunks));
+        g.sstep = int32_t(sstep), g.stage_bytes = int32_t(stage_bytes);
         g.stage_fast = (h * w) % 4 == 0 && (reinterpret_cast<uintptr_t>(in) & 3) == 0 &&
                        (16 * g.waves) % (g.cin_pad / 4) == 0 && 16 * g.waves / (g.cin_pad / 4) <= h * w / 4;
-        const size_t lds = fixed + size_t(g.tc) * g.win_units * 16;
-        SPK_ARG(lds <= 163840, "Fused layer: the staged sample (" << lds << " bytes) exceeds the 160 KiB LDS.");
         SPK_ARG(int64_t(cout) * ho * wo < (1LL << 31), "Fused layer: output plane too large.");
 
         const StdpConfigs cfg = learn ? pack_configs(configs, nconfigs) : StdpConfigs{};
@@ -781,14 +813,15 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
 
         hipStream_t s = S(stream);
         const int64_t hw = ho * wo, hwi = h * w;
-        Scratch frag(int64_t(g.waves) * 2 * g.steps * 64 * 16, s), summary(n * hw * int64_t(sizeof(Summary)), s);
+        const int64_t row_tiles = int64_t(g.waves) * rt;
+        Scratch frag(row_tiles * g.steps * 64 * 16, s), summary(n * hw * int64_t(sizeof(Summary)), s);
         Scratch wkey(n * count * int64_t(sizeof(int32_t)), s), fsm(learn ? n * hwi * g.cin_pad : 0, s);
-        prep_fragments<<<dim3(unsigned(cdiv(int64_t(g.waves) * 2 * g.steps * 64, 256))), 256, 0, s>>>(
+        prep_fragments<<<dim3(unsigned(cdiv(row_tiles * g.steps * 64, 256))), 256, 0, s>>>(
             (const float *)kernel, frag.as<half8>(), g.cout, g.cin, g.kh, g.kw, g.cin_pad, g.groups, g.steps,
-            g.waves * 2);
+            int32_t(row_tiles));
         SPK_LAUNCHED();
-        dispatch_layer(g.steps, (const uint8_t *)in, frag.as<half8>(), summary.as<Summary>(), (float *)potentials,
-                       learn ? fsm.as<uint8_t>() : nullptr, g, lds, s);
+        dispatch_layer<2>(g.steps, (const uint8_t *)in, frag.as<half8>(), summary.as<Summary>(), (float *)potentials,
+                          learn ? fsm.as<uint8_t>() : nullptr, g, size_t(lds), s);
         // the writer needs only the summaries: it runs on a side stream,
         // overlapping convwta, the records and the STDP chains (HBM-bound
         // beside latency-bound), and the call's stream waits for it at the end

@@ -1,5 +1,5 @@
 // Device pieces shared by the fused layer kernels (fused.hip, region.hip):
-// FP16 fragment types, the per-pixel summary, U8 -> FP16 widening, the
+// FP16 fragment types, the per-pixel summary, U8 -> FP16 operands, the
 // per-step fire/maximum update, the MFMA chain, and the STDP records/chains.
 #pragma once
 
@@ -51,16 +51,25 @@ __global__ void prep_fragments(const float *__restrict__ k, half8 *__restrict__ 
     }
 }
 
-__device__ __forceinline__ uint4 widen(uint2 raw)
+// Spike bytes -> FP16 MFMA operands: byte b zero-extended to 16 bits is the
+// FP16 subnormal b * 2^-24.  The MFMA takes subnormal operands unflushed, so
+// a product with an FP16 weight is exactly 2^-24 times the product with b as
+// an FP16 integer, and a conv sum comes out as 2^-24 times the potential (to
+// within FP32 rounding, identically for identical operands); kernels compare
+// against the threshold times 2^-24 and scale their outputs back by 2^24.
+__device__ __forceinline__ uint4 subnormals(uint2 raw)
 {
-    // bytes -> FP16 exactly: 0x64 above byte b is the half 1024 + b, minus 1024
-    const half2v bias = {_Float16(1024.0f), _Float16(1024.0f)};
-    const uint32_t hi = 0x64646464u;
-    uint32_t w[4] = {__builtin_amdgcn_perm(hi, raw.x, 0x04010400u), __builtin_amdgcn_perm(hi, raw.x, 0x04030402u),
-                     __builtin_amdgcn_perm(hi, raw.y, 0x04010400u), __builtin_amdgcn_perm(hi, raw.y, 0x04030402u)};
-#pragma unroll
-    for (int i = 0; i < 4; ++i) w[i] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(half2v, w[i]) - bias);
-    return make_uint4(w[0], w[1], w[2], w[3]);
+    return make_uint4(__builtin_amdgcn_perm(0u, raw.x, 0x04010400u), __builtin_amdgcn_perm(0u, raw.x, 0x04030402u),
+                      __builtin_amdgcn_perm(0u, raw.y, 0x04010400u), __builtin_amdgcn_perm(0u, raw.y, 0x04030402u));
+}
+
+constexpr float TWO_POW_24 = 16777216.0f;
+
+// The threshold (as FP32) times 2^-24 is exact: 0, or far from underflow.
+inline bool threshold_scales(double th)
+{
+    const float f = float(th);
+    return f == 0.0f || double(f * (1.0f / TWO_POW_24)) * double(TWO_POW_24) == double(f);
 }
 
 __device__ __forceinline__ void combine(float &v, int32_t &i, float ov, int32_t oi)
@@ -80,25 +89,28 @@ __device__ __forceinline__ void first_nonzero(uint32_t &res, uint32_t &seen, uin
     res = (res & ~fresh) | (uint32_t(tt) * 0x01010101u & fresh);
 }
 
-// One step of one lane's 8 (channel, pixel) values: shift "v > th" into the
-// masks (newest step in bit 0, via the carry input of v_addc) and keep the
-// running maximum as a signed compare of the float bits, which orders every
-// positive value correctly; the maximum is only read when a value fired,
-// and then it is the largest fired value (> th >= 0).
+// One (channel, pixel) value of a step: shift "v > th" into its mask (newest
+// step in bit 0) and keep the running maximum as a signed compare of the float
+// bits, which orders every positive value correctly; the maximum is only read
+// when a value fired, and then it is the largest fired value (> th >= 0).
+// th - v has its sign bit set exactly when v > th (an exact difference never
+// rounds across zero, and x - x is +0), and v_alignbit shifts that bit in:
+// two VALU operations with no SGPR in between, so consecutive values
+// pipeline instead of waiting on a compare result.
+__device__ __forceinline__ void value_update(float v, float th, uint32_t &mask, int32_t &mxb)
+{
+    mask = __builtin_amdgcn_alignbit(mask, __builtin_bit_cast(uint32_t, th - v), 31);
+    mxb = max(mxb, __builtin_bit_cast(int32_t, v));
+}
+
+// One step of one lane's 8 (channel, pixel) values (two 16-row tiles).
 __device__ __forceinline__ void step_update(const float4v &acc0, const float4v &acc1, float th,
                                             uint32_t (&mask)[2][4], int32_t (&mxb)[2][4])
 {
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
+    for (int i = 0; i < 4; ++i) value_update(acc0[i], th, mask[0][i], mxb[0][i]);
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-        {
-            const float v = r == 0 ? acc0[i] : acc1[i];
-            const uint64_t fired = __builtin_amdgcn_ballot_w64(v > th);
-            uint64_t carry;
-            asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(mask[r][i]), "=s"(carry) : "v"(mask[r][i]), "s"(fired));
-            mxb[r][i] = max(mxb[r][i], __builtin_bit_cast(int32_t, v));
-        }
+    for (int i = 0; i < 4; ++i) value_update(acc1[i], th, mask[1][i], mxb[1][i]);
 }
 
 template <int S>

@@ -295,7 +295,7 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
             for (int32_t t0 = 0; t0 < I.t; t0 += g.tc)
             {
                 const int32_t tc = min(g.tc, I.t - t0);
-                // widen the windows of steps [t0, t0 + tc): stage rows
+                // the FP16 windows of steps [t0, t0 + tc): stage rows
                 // grp0 .. grp0 + wrows, columns xoff .. xoff + wcols; units
                 // outside the input are 0
                 if (wslot < wslots)
@@ -312,14 +312,14 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
                         if (PAIR)
                         {
                             if (colv && iy >= 0 && iy < I.h) raw.x = *reinterpret_cast<const uint32_t *>(sp);
-                            const uint4 hv = widen(raw);
+                            const uint4 hv = subnormals(raw);
                             reinterpret_cast<uint2 *>(win16)[ts * 2 * g.win_units + r * g.pitch + wcu] =
                                 make_uint2(hv.x, hv.y);
                         }
                         else
                         {
                             if (colv && iy >= 0 && iy < I.h) raw = *reinterpret_cast<const uint2 *>(sp);
-                            win16[ts * g.win_units + r * g.pitch + wcu] = widen(raw);
+                            win16[ts * g.win_units + r * g.pitch + wcu] = subnormals(raw);
                         }
                         ts += wdq, r += wdr;
                         if (r >= g.wrows) r -= g.wrows, ++ts;
@@ -357,7 +357,7 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
                                 const int32_t o = o0 + r * 16 + i;
                                 if (o < g.cout)
                                     pot_out[((int64_t(itm.x) * I.t + t0 + k) * g.cout + o) * (I.ho * I.wo) +
-                                            oy * I.wo + ox] = r == 0 ? u0[i] : u1[i];
+                                            oy * I.wo + ox] = (r == 0 ? u0[i] : u1[i]) * TWO_POW_24;
                             }
                 };
                 for (int32_t k = 0; k < tc; k += 2)
@@ -403,7 +403,7 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
                     }
                     else if (row && ix >= 0 && ix < I.w)
                         raw = *reinterpret_cast<const uint2 *>(sp);
-                    b[s] = __builtin_bit_cast(half8, widen(raw));
+                    b[s] = __builtin_bit_cast(half8, subnormals(raw));
                 }
             };
             int32_t ts;
@@ -473,7 +473,8 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
             {
                 const uint32_t fwd = om ? __builtin_bitreverse32(om) >> (32 - I.t) : 0u;
                 summary[I.sum_off + int64_t(oy) * I.wo + ox] =
-                    Summary{cs, fwd, om ? first : 0.0f, om ? __builtin_bit_cast(float, oxb) : 0.0f};
+                    Summary{cs, fwd, om ? first * TWO_POW_24 : 0.0f,
+                            om ? __builtin_bit_cast(float, oxb) * TWO_POW_24 : 0.0f};
             }
         }
     }
@@ -827,7 +828,7 @@ bool region_config(const LayerPlan &P, int32_t m, bool fast, RegionGeom &g, size
     g.wcols = RW + kspan - 1;
     g.pitch = planes * g.wcols;
     g.win_units = g.pair ? int32_t(cdiv(g.wrows * g.pitch, 2)) : g.wrows * g.pitch;
-    g.th = P.th;
+    g.th = P.th * (1.0f / TWO_POW_24);  // potentials are computed times 2^-24
     const int64_t stage = cdiv(int64_t(P.tmax) * g.sh * g.sw * g.cin_pad, 16) * 16;
     const int64_t fixed = stage + region_fixed(g.waves, g.nct, g.steps);
     const int64_t room = (LDS_BYTES - fixed) / (int64_t(g.win_units) * 16);
@@ -935,6 +936,7 @@ LayerPlan *make_plan(int64_t n, const int32_t *shapes, int64_t cin, int64_t cout
     SPK_ARG(pt >= 0 && pl >= 0 && pb >= 0 && pr >= 0, "Pad items are incorrect.");
     SPK_ARG(cout <= 256, "Fused layer: at most 256 output channels.");
     SPK_ARG(threshold >= 0.0, "Fused layer: the conv threshold must be >= 0.");
+    SPK_ARG(threshold_scales(threshold), "Fused layer: the threshold is too small to scale exactly.");
     SPK_ARG(ry >= 0 && rx >= 0 && count >= 1, "Fused layer: invalid convwta parameters.");
     SPK_ARG(kh + RW + kw <= 64, "Fused layer: kernel too large for the region path.");
     auto P = std::make_unique<LayerPlan>();
