@@ -72,25 +72,61 @@ __device__ __forceinline__ int32_t tap_off(const FusedGeom &g, int32_t grp)
     return ((tap / g.kw) * g.pw + tap % g.kw) * g.cin_pad + c8 * 8;
 }
 
-// B fragments of one step: the lane's 8 bytes of K-group 4s+q at its pixel.
-template <int S>
-__device__ __forceinline__ void load_raw(uint2 (&r)[S], const uint8_t *base, const int32_t (&off)[S])
+// The bytes of one step's B fragments for a lane.  K-step s takes K-group
+// 4s+q (q = lane / 16) by default.  PAIRED (channels padded to a multiple of
+// 16, so 16-byte units never straddle a tap) lets K-steps 2m and 2m+1 share
+// one 16-byte read: unit u = 4m+q, whose low 8 bytes are K-step 2m's group 2u
+// and high 8 bytes K-step 2m+1's group 2u+1; an odd last K-step reads its 8
+// bytes alone.  prep_fragments lays the weights out in the same order.
+template <int S, bool PAIRED>
+struct StepBytes
+{
+    static constexpr int P = PAIRED ? S / 2 : 0, L = S - 2 * P;
+    uint4 p[P > 0 ? P : 1];
+    uint2 l[L > 0 ? L : 1];
+    __device__ __forceinline__ uint2 frag(int s) const
+    {
+        if (s < 2 * P) return (s & 1) ? make_uint2(p[s / 2].z, p[s / 2].w) : make_uint2(p[s / 2].x, p[s / 2].y);
+        return l[s - 2 * P];
+    }
+};
+
+// Stage offsets of a lane's reads relative to its pixel (see StepBytes).
+template <int S, bool PAIRED>
+struct StepOffsets
+{
+    static constexpr int P = StepBytes<S, PAIRED>::P, L = StepBytes<S, PAIRED>::L;
+    int32_t p[P > 0 ? P : 1], l[L > 0 ? L : 1];
+    __device__ __forceinline__ StepOffsets(const FusedGeom &g, int32_t q)
+    {
+#pragma unroll
+        for (int i = 0; i < P; ++i) p[i] = tap_off(g, 2 * (4 * i + q));
+#pragma unroll
+        for (int i = 0; i < L; ++i) l[i] = tap_off(g, 4 * (2 * P + i) + q);
+    }
+};
+
+template <int S, bool PAIRED>
+__device__ __forceinline__ void load_raw(StepBytes<S, PAIRED> &r, const uint8_t *base,
+                                         const StepOffsets<S, PAIRED> &o)
 {
 #pragma unroll
-    for (int s = 0; s < S; ++s) r[s] = *reinterpret_cast<const uint2 *>(base + off[s]);
+    for (int i = 0; i < StepBytes<S, PAIRED>::P; ++i) r.p[i] = *reinterpret_cast<const uint4 *>(base + o.p[i]);
+#pragma unroll
+    for (int i = 0; i < StepBytes<S, PAIRED>::L; ++i) r.l[i] = *reinterpret_cast<const uint2 *>(base + o.l[i]);
 }
 
 // The MFMA chain of one step on raw spike bytes (as FP16 subnormals), for
 // the RT 16-row tiles of a wave.
-template <int RT, int S>
-__device__ __forceinline__ void mfma_sub(const half8 (&a)[RT][S], const uint2 (&r)[S], float4v (&c)[RT])
+template <int RT, int S, bool PAIRED>
+__device__ __forceinline__ void mfma_sub(const half8 (&a)[RT][S], const StepBytes<S, PAIRED> &r, float4v (&c)[RT])
 {
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt) c[rt] = float4v{0, 0, 0, 0};
 #pragma unroll
     for (int s = 0; s < S; ++s)
     {
-        const half8 b = __builtin_bit_cast(half8, subnormals(r[s]));
+        const half8 b = __builtin_bit_cast(half8, subnormals(r.frag(s)));
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt) c[rt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[rt][s], b, c[rt], 0, 0, 0);
     }
@@ -115,19 +151,19 @@ __device__ __forceinline__ void update_all(const float4v (&u)[RT], float th, uin
 // an MFMA phase and a VALU phase.  Value v is updated after MFMA number
 // RT + v * (S*RT - RT) / (4*RT): the first fragment's MFMAs give the previous
 // chain's last results time to land.
-template <int RT, int S>
-__device__ __forceinline__ void step_iter(const half8 (&a)[RT][S], const uint2 (&rin)[S], float4v (&c)[RT],
+template <int RT, int S, bool PAIRED>
+__device__ __forceinline__ void step_iter(const half8 (&a)[RT][S], const StepBytes<S, PAIRED> &rin, float4v (&c)[RT],
                                           const float4v (&u)[RT], float th, uint32_t (&mask)[RT][4],
-                                          int32_t (&mxb)[RT][4], uint2 (&rout)[S], const uint8_t *next,
-                                          const int32_t (&toff)[S])
+                                          int32_t (&mxb)[RT][4], StepBytes<S, PAIRED> &rout, const uint8_t *next,
+                                          const StepOffsets<S, PAIRED> &o)
 {
-    constexpr int V = 4 * RT, SPAN = S * RT - RT;
+    constexpr int V = 4 * RT, SPAN = S * RT - RT, P = StepBytes<S, PAIRED>::P;
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt) c[rt] = float4v{0, 0, 0, 0};
 #pragma unroll
     for (int s = 0; s < S; ++s)
     {
-        const half8 b = __builtin_bit_cast(half8, subnormals(rin[s]));
+        const half8 b = __builtin_bit_cast(half8, subnormals(rin.frag(s)));
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
         {
@@ -137,7 +173,9 @@ __device__ __forceinline__ void step_iter(const half8 (&a)[RT][S], const uint2 (
                 if (SPAN > 0 && RT + (v * SPAN) / V == s * RT + rt)
                     value_update(u[v / 4][v % 4], th, mask[v / 4][v % 4], mxb[v / 4][v % 4]);
         }
-        rout[s] = *reinterpret_cast<const uint2 *>(next + toff[s]);
+        // the read that fed this fragment, for the step after next
+        if (s < 2 * P && (s & 1)) rout.p[s / 2] = *reinterpret_cast<const uint4 *>(next + o.p[s / 2]);
+        if (s >= 2 * P) rout.l[s - 2 * P] = *reinterpret_cast<const uint2 *>(next + o.l[s - 2 * P]);
     }
     if (SPAN <= 0) update_all(u, th, mask, mxb);
     // keep the next iteration's conversions below this one's loads
@@ -153,7 +191,7 @@ constexpr int layer_threads()
     return RT == 4 ? 256 : 512;
 }
 
-template <int RT, int S, bool DIAG>
+template <int RT, int S, bool PAIRED, bool DIAG>
 __global__ void __launch_bounds__(layer_threads<RT>())
     fused_layer(const uint8_t *__restrict__ in, const half8 *__restrict__ frag, Summary *__restrict__ summary,
                 float *__restrict__ pot_out, uint8_t *__restrict__ fs_out, FusedGeom g)
@@ -170,14 +208,11 @@ __global__ void __launch_bounds__(layer_threads<RT>())
 
     // weight fragments and tap offsets, resident for the whole kernel
     half8 a[RT][S];
-    int32_t toff[S];
 #pragma unroll
     for (int s = 0; s < S; ++s)
-    {
 #pragma unroll
         for (int r = 0; r < RT; ++r) a[r][s] = frag[((w * RT + r) * S + s) * 64 + lane];
-        toff[s] = tap_off(g, 4 * s + q);
-    }
+    const StepOffsets<S, PAIRED> toff(g, q);
     // the zero border and the padded channels are written once; every sample
     // rewrites only the interior
     for (int32_t i = threadIdx.x; i < g.stage_bytes / 16; i += blockDim.x)
@@ -303,7 +338,7 @@ __global__ void __launch_bounds__(layer_threads<RT>())
             // past the last step re-read it, unused)
             {
                 const int32_t last = g.t - 1;
-                uint2 rA[S], rB[S];
+                StepBytes<S, PAIRED> rA, rB;
                 float4v cA[RT], cB[RT];
                 auto diag = [&](int32_t k, const float4v (&u)[RT]) {
                     if (DIAG && pvalid)
@@ -369,7 +404,7 @@ __global__ void __launch_bounds__(layer_threads<RT>())
                 uint32_t all = 0;
                 for (int32_t ow = 0; ow < g.waves; ++ow) all |= any_s[buf + ow * COLS + j];
                 const int32_t cnt = __builtin_popcount(all);
-                uint2 r[S];
+                StepBytes<S, PAIRED> r;
                 load_raw(r, pb + (g.t - (cnt > 1 ? cnt : 1)) * g.sstep, toff);
                 mfma_sub(a, r, gv);
                 float bv = -1.0f;
@@ -415,7 +450,7 @@ __global__ void __launch_bounds__(layer_threads<RT>())
             float first = 0.0f;
             if (__builtin_amdgcn_ballot_w64(own && om != 0))
             {
-                uint2 r[S];
+                StepBytes<S, PAIRED> r;
                 load_raw(r, pb + t0 * g.sstep, toff);
                 mfma_sub(a, r, gv);
 #pragma unroll
@@ -661,14 +696,14 @@ SideStream::~SideStream()
 
 namespace
 {
-template <int RT, int S, bool DIAG>
+template <int RT, int S, bool PAIRED, bool DIAG>
 void launch_layer(const uint8_t *in, const half8 *frag, Summary *summary, float *pot, uint8_t *fs, const FusedGeom &g,
                   size_t lds, hipStream_t s)
 {
     static bool configured = false;  // per instantiation; the attribute is per function
     if (!configured)
     {
-        SPK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&fused_layer<RT, S, DIAG>),
+        SPK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&fused_layer<RT, S, PAIRED, DIAG>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, 163840));
         configured = true;
     }
@@ -684,7 +719,7 @@ void launch_layer(const uint8_t *in, const half8 *frag, Summary *summary, float 
         SPK_HIP(hipEventCreate(&e1));
         SPK_HIP(hipEventRecord(e0, s));
     }
-    fused_layer<RT, S, DIAG><<<dim3(grid), dim3(64 * g.waves), lds, s>>>(in, frag, summary, pot, fs, g);
+    fused_layer<RT, S, PAIRED, DIAG><<<dim3(grid), dim3(64 * g.waves), lds, s>>>(in, frag, summary, pot, fs, g);
     SPK_LAUNCHED();
     if (tm.on)
     {
@@ -694,15 +729,18 @@ void launch_layer(const uint8_t *in, const half8 *frag, Summary *summary, float 
 }
 
 template <int RT>
-void dispatch_layer(int steps, const uint8_t *in, const half8 *frag, Summary *summary, float *pot, uint8_t *fs,
-                    const FusedGeom &g, size_t lds, hipStream_t s)
+void dispatch_layer(int steps, bool paired, const uint8_t *in, const half8 *frag, Summary *summary, float *pot,
+                    uint8_t *fs, const FusedGeom &g, size_t lds, hipStream_t s)
 {
     switch (steps)
     {
 #define CASE(k) \
-    case k:                                                                                 \
-        return pot ? launch_layer<RT, k, true>(in, frag, summary, pot, fs, g, lds, s)         \
-                   : launch_layer<RT, k, false>(in, frag, summary, pot, fs, g, lds, s);
+    case k:                                                                                        \
+        if (paired)                                                                                \
+            return pot ? launch_layer<RT, k, true, true>(in, frag, summary, pot, fs, g, lds, s)      \
+                       : launch_layer<RT, k, true, false>(in, frag, summary, pot, fs, g, lds, s);    \
+        return pot ? launch_layer<RT, k, false, true>(in, frag, summary, pot, fs, g, lds, s)         \
+                   : launch_layer<RT, k, false, false>(in, frag, summary, pot, fs, g, lds, s);
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
         CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
 #undef CASE
@@ -816,12 +854,14 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
         const int64_t row_tiles = int64_t(g.waves) * rt;
         Scratch frag(row_tiles * g.steps * 64 * 16, s), summary(n * hw * int64_t(sizeof(Summary)), s);
         Scratch wkey(n * count * int64_t(sizeof(int32_t)), s), fsm(learn ? n * hwi * g.cin_pad : 0, s);
+        // pairs of K-steps share 16-byte reads when 16-byte units stay inside a tap
+        const bool paired = g.cin_pad % 16 == 0 && g.steps >= 2;
         prep_fragments<<<dim3(unsigned(cdiv(row_tiles * g.steps * 64, 256))), 256, 0, s>>>(
             (const float *)kernel, frag.as<half8>(), g.cout, g.cin, g.kh, g.kw, g.cin_pad, g.groups, g.steps,
-            int32_t(row_tiles));
+            int32_t(row_tiles), paired ? 1 : 0);
         SPK_LAUNCHED();
-        dispatch_layer<2>(g.steps, (const uint8_t *)in, frag.as<half8>(), summary.as<Summary>(), (float *)potentials,
-                          learn ? fsm.as<uint8_t>() : nullptr, g, size_t(lds), s);
+        dispatch_layer<2>(g.steps, paired, (const uint8_t *)in, frag.as<half8>(), summary.as<Summary>(),
+                          (float *)potentials, learn ? fsm.as<uint8_t>() : nullptr, g, size_t(lds), s);
         // the writer needs only the summaries: it runs on a side stream,
         // overlapping convwta, the records and the STDP chains (HBM-bound
         // beside latency-bound), and the call's stream waits for it at the end

@@ -29,15 +29,19 @@ struct Summary
 // [row tile][K step][lane][8] FP16 fragments of the FP32 masters [cout][cin][kh][kw]:
 // lane l of K step s holds output channel rt*16 + l%16 and K group 4s + l/16
 // (8 input channels c8*8.. of one tap); groups past the real K are zero.
+// With `paired`, K-steps 2m and 2m+1 (below an odd last one) hold the groups
+// 2u and 2u+1 of unit u = 4m + l/16 instead, the order in which the whole-
+// sample kernel reads its 16-byte units.
 __global__ void prep_fragments(const float *__restrict__ k, half8 *__restrict__ frag, int32_t cout, int32_t cin,
                                int32_t kh, int32_t kw, int32_t cin_pad, int32_t groups, int32_t steps,
-                               int32_t row_tiles)
+                               int32_t row_tiles, int32_t paired)
 {
     const int32_t total = row_tiles * steps * 64;
     for (int32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x)
     {
         const int32_t lane = e % 64, s = (e / 64) % steps, rt = e / (64 * steps);
-        const int32_t o = rt * 16 + lane % 16, grp = 4 * s + lane / 16;
+        const int32_t o = rt * 16 + lane % 16;
+        const int32_t grp = paired && s < 2 * (steps / 2) ? 2 * (4 * (s / 2) + lane / 16) + (s & 1) : 4 * s + lane / 16;
         const int32_t cg = cin_pad / 8, tap = grp / cg, c8 = grp % cg;
         half8 v;
         for (int j = 0; j < 8; ++j)

@@ -877,7 +877,7 @@ void LayerPlan::run(const uint8_t *in, float *kernel, const spk_stdp_config *cfg
                                                   g0.cg * 2);
     else
         prep_fragments<<<fgrid, 256, 0, s>>>(kernel, frag.p, g0.cout, g0.cin, g0.kh, g0.kw, g0.cin_pad, g0.groups,
-                                             g0.steps, g0.cg * 2);
+                                             g0.steps, g0.cg * 2, 0);
     SPK_LAUNCHED();
     if (tune && !tuned) choose(in, s);
     const bool learn = cfg != nullptr;
