@@ -527,115 +527,60 @@ __global__ void __launch_bounds__(256) fused_select(const Summary *__restrict__ 
     for (int32_t r = k + int32_t(threadIdx.x); r < count; r += blockDim.x) wkey[s * count + r] = -1;
 }
 
+// Zeroes bytes [0, total) of dst, 16-byte stores where aligned.
+__device__ __forceinline__ void zero_bytes(uint8_t *dst, int64_t total)
+{
+    const int64_t head = int64_t((16 - (reinterpret_cast<uintptr_t>(dst) & 15)) & 15);
+    const int64_t h = head < total ? head : total;
+    if (int64_t(threadIdx.x) < h) dst[threadIdx.x] = 0;
+    uint4 *body = reinterpret_cast<uint4 *>(dst + h);
+    const int64_t units = (total - h) / 16;
+    for (int64_t i = threadIdx.x; i < units; i += blockDim.x) body[i] = make_uint4(0, 0, 0, 0);
+    for (int64_t i = h + units * 16 + threadIdx.x; i < total; i += blockDim.x) dst[i] = 0;
+}
+
 // Spikes [n][t][cout][ho][wo] and (optionally) their max-pool, from the
-// summaries.  One block per (sample, step); each thread writes 16 bytes.
+// summaries.  A block per CU walks the (sample, step) items, leaving the
+// CUs' other wave slots to the STDP kernels that run beside it.  An item's
+// output is all zero
+// but for at most one 1 per pixel (its kept channel, when that fires at this
+// step), so the block zeroes its spike planes and pooled planes with wide
+// stores, then each pixel stores its 1 into the spike plane of its channel
+// and into every pool window that contains it (the max of a window of 0/1
+// bytes is 1 exactly when one of its pixels is 1; pixels of one window store
+// the same value).  The barrier (a workgroup-scope release and acquire, so
+// the zero stores have completed in the block's L2 first) orders every zero
+// before every 1.
 __global__ void __launch_bounds__(256) fused_write(const Summary *__restrict__ summary, uint8_t *__restrict__ out,
                                                    uint8_t *__restrict__ pooled, int32_t t, int32_t cout, int32_t ho,
                                                    int32_t wo, int32_t pkh, int32_t pkw, int32_t psh, int32_t psw,
-                                                   int32_t pho, int32_t pwo)
+                                                   int32_t pho, int32_t pwo, int64_t nt)
 {
-    extern __shared__ __align__(16) uint8_t lds[];
-    int16_t *act = reinterpret_cast<int16_t *>(lds);  // active channel per pixel at this step, or -1
-    const int32_t hw = ho * wo;
-    const int64_t ns = blockIdx.x / t;
-    const int32_t step = blockIdx.x % t;
-    for (int32_t p = threadIdx.x; p < hw; p += blockDim.x)
+    const int32_t hw = ho * wo, phw = pho * pwo;
+    for (int64_t it = blockIdx.x; it < nt; it += gridDim.x)
     {
-        const Summary e = summary[ns * hw + p];
-        act[p] = (e.mask >> step) & 1u ? int16_t(e.c) : int16_t(-1);
-    }
-    __syncthreads();
-    const int64_t total_out = int64_t(cout) * hw;
-    if (out && hw % 4 == 0 && (reinterpret_cast<uintptr_t>(out + int64_t(blockIdx.x) * total_out) & 3) == 0)
-    {
-        // four pixels of one channel plane per 32-bit store, consecutive
-        // threads on consecutive pixels of the same plane
-        uint8_t *dst = out + int64_t(blockIdx.x) * total_out;
-        const int32_t quads = hw / 4, lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
-        for (int32_t qd = lane; qd < quads; qd += 64)
+        const int64_t ns = it / t;
+        const int32_t step = int32_t(it % t);
+        uint8_t *dst = out ? out + it * cout * hw : nullptr;
+        uint8_t *pdst = pooled ? pooled + it * cout * phw : nullptr;
+        if (dst) zero_bytes(dst, int64_t(cout) * hw);
+        if (pdst) zero_bytes(pdst, int64_t(cout) * phw);
+        __syncthreads();
+        for (int32_t p = threadIdx.x; p < hw; p += blockDim.x)
         {
-            // the lane's four pixels keep their active channels in registers
-            // while the wave walks the channel planes
-            const uint2 a2 = *reinterpret_cast<const uint2 *>(act + 4 * qd);
-            const int32_t a0 = int16_t(a2.x & 0xFFFF), a1 = int16_t(a2.x >> 16);
-            const int32_t a2v = int16_t(a2.y & 0xFFFF), a3v = int16_t(a2.y >> 16);
-            uint8_t *d = dst + 4 * qd;
-            for (int32_t c = wave; c < cout; c += waves)
+            const Summary e = summary[ns * hw + p];
+            if (!((e.mask >> step) & 1u)) continue;
+            if (dst) dst[int64_t(e.c) * hw + p] = 1;
+            if (pdst)
             {
-                const uint32_t word = uint32_t(a0 == c) | uint32_t(a1 == c) << 8 | uint32_t(a2v == c) << 16 |
-                                      uint32_t(a3v == c) << 24;
-                *reinterpret_cast<uint32_t *>(d + int64_t(c) * hw) = word;
+                // windows (py, px) with py*psh <= y < py*psh + pkh, likewise x
+                const int32_t y = p / wo, x = p % wo;
+                const int32_t py0 = y >= pkh ? (y - pkh) / psh + 1 : 0, py1 = min(y / psh, pho - 1);
+                const int32_t px0 = x >= pkw ? (x - pkw) / psw + 1 : 0, px1 = min(x / psw, pwo - 1);
+                for (int32_t py = py0; py <= py1; ++py)
+                    for (int32_t px = px0; px <= px1; ++px) pdst[int64_t(e.c) * phw + py * pwo + px] = 1;
             }
         }
-    }
-    else if (out)
-    {
-        // bytes [g0, g0 + total) of out, walked in 16-byte aligned chunks
-        const int64_t total = total_out;
-        const int64_t g0 = int64_t(blockIdx.x) * total;
-        const int64_t head = int64_t(reinterpret_cast<uintptr_t>(out + g0) & 15);
-        for (int64_t c0 = int64_t(threadIdx.x) * 16 - head; c0 < total; c0 += int64_t(blockDim.x) * 16)
-        {
-            uint8_t v[16];
-            const int64_t start = c0 < 0 ? 0 : c0;
-            int32_t c = int32_t(start / hw), p = int32_t(start % hw);
-#pragma unroll
-            for (int b = 0; b < 16; ++b)
-            {
-                const int64_t i = c0 + b;
-                v[b] = (i >= 0 && i < total && act[p] == c) ? 1 : 0;
-                if (i >= 0 && ++p == hw) p = 0, ++c;
-            }
-            uint8_t *d = out + g0 + c0;
-            if (c0 >= 0 && c0 + 16 <= total)
-                *reinterpret_cast<uint4 *>(d) = *reinterpret_cast<uint4 *>(v);
-            else
-                for (int b = 0; b < 16; ++b)
-                    if (c0 + b >= 0 && c0 + b < total) d[b] = v[b];
-        }
-    }
-    if (pooled)
-    {
-        const int32_t phw = pho * pwo, total = cout * phw, win = pkh * pkw;
-        uint8_t *dst = pooled + int64_t(blockIdx.x) * total;
-        if (win <= 16)
-        {
-            // each thread owns one pooled position (its window's active
-            // channels in registers) and a strided set of channels
-            const int32_t P = min(phw, int32_t(blockDim.x)), C = int32_t(blockDim.x) / P;
-            const int32_t clane = int32_t(threadIdx.x) / P;
-            for (int32_t pos = int32_t(threadIdx.x) % P; clane < C && pos < phw; pos += P)
-            {
-                const int32_t py = pos / pwo, px = pos % pwo;
-                int32_t wa[16];
-#pragma unroll
-                for (int k = 0; k < 16; ++k)
-                {
-                    const int32_t y = py * psh + k / pkw, x = px * psw + k % pkw;
-                    wa[k] = (k < win && y < ho && x < wo) ? act[y * wo + x] : -1;
-                }
-                for (int32_t c = clane; c < cout; c += C)
-                {
-                    bool m = false;
-#pragma unroll
-                    for (int k = 0; k < 16; ++k) m |= wa[k] == c;
-                    dst[int64_t(c) * phw + pos] = m;
-                }
-            }
-        }
-        else
-            for (int32_t i = threadIdx.x; i < total; i += blockDim.x)
-            {
-                const int32_t c = i / phw, r = i % phw, py = r / pwo, px = r % pwo;
-                uint8_t m = 0;
-                for (int32_t a = 0; a < pkh; ++a)
-                    for (int32_t b = 0; b < pkw; ++b)
-                    {
-                        const int32_t y = py * psh + a, x = px * psw + b;
-                        if (y < ho && x < wo && act[y * wo + x] == c) m = 1;
-                    }
-                dst[i] = m;
-            }
     }
 }
 
@@ -868,9 +813,12 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
         SideStream side(s, out || pooled);
         if (out || pooled)
         {
-            fused_write<<<dim3(unsigned(n * t)), 256, size_t(hw) * 2, side.stream>>>(
+            int dev = 0, cus = 0;
+            SPK_HIP(hipGetDevice(&dev));
+            SPK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            fused_write<<<dim3(unsigned(n * t < cus ? n * t : cus)), 256, 0, side.stream>>>(
                 summary.as<Summary>(), (uint8_t *)out, (uint8_t *)pooled, g.t, g.cout, g.ho, g.wo, int32_t(pkh),
-                int32_t(pkw), int32_t(psh), int32_t(psw), int32_t(pho), int32_t(pwo));
+                int32_t(pkw), int32_t(psh), int32_t(psw), int32_t(pho), int32_t(pwo), n * t);
             SPK_LAUNCHED();
         }
         fused_select<<<dim3(unsigned(n)), 256, size_t(hw) * 8, s>>>(summary.as<Summary>(), g.t, g.ho, g.wo,
