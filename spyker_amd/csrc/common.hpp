@@ -73,6 +73,15 @@ const char *type_name(int32_t type);
 
 // Frees the layer plans the uniform fused path keeps per shape (region.hip).
 void clear_plan_cache();
+// Compute units of the current device (queried once per device).
+int cu_count();
+// A grow-only device workspace of at least `bytes` for the calls of one
+// entry point (`owner`) on one stream of the current device.  Calls on a
+// stream run in order, so they share it without waiting; it grows only
+// outside a stream capture (a capture that needs more is refused, so warm
+// the call up first), and spk_cache_clear frees it.
+void *stream_workspace(int owner, hipStream_t stream, int64_t bytes);
+void clear_workspaces();
 // One draw of the shared host generator (Generator() of the reference).
 uint64_t generator_draw();
 // [rows][bins] Poisson interval table (Core::poisson_create), host memory.

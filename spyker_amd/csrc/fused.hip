@@ -595,48 +595,46 @@ KernelTimer &timer()
 SideStream::SideStream(hipStream_t main_, bool use) : main(main_), stream(main_)
 {
     if (!use) return;
-    // one non-blocking side stream per device, at the default priority
+    // one non-blocking side stream and a fork and a join event per device,
+    // at the default priority; an event is recorded again by every call (a
+    // wait takes the record that precedes it)
     static std::mutex mu;
     static std::vector<hipStream_t> streams;
+    static std::vector<hipEvent_t> forks, joins;
     int dev = 0;
     SPK_HIP(hipGetDevice(&dev));
     {
         std::lock_guard<std::mutex> lock(mu);
-        if (int(streams.size()) <= dev) streams.resize(size_t(dev) + 1, nullptr);
-        if (!streams[size_t(dev)]) SPK_HIP(hipStreamCreateWithFlags(&streams[size_t(dev)], hipStreamNonBlocking));
-        stream = streams[size_t(dev)];
+        if (int(streams.size()) <= dev)
+        {
+            streams.resize(size_t(dev) + 1, nullptr);
+            forks.resize(size_t(dev) + 1, nullptr);
+            joins.resize(size_t(dev) + 1, nullptr);
+        }
+        if (!streams[size_t(dev)])
+        {
+            SPK_HIP(hipStreamCreateWithFlags(&streams[size_t(dev)], hipStreamNonBlocking));
+            SPK_HIP(hipEventCreateWithFlags(&forks[size_t(dev)], hipEventDisableTiming));
+            SPK_HIP(hipEventCreateWithFlags(&joins[size_t(dev)], hipEventDisableTiming));
+        }
+        stream = streams[size_t(dev)], done = joins[size_t(dev)];
+        SPK_HIP(hipEventRecord(forks[size_t(dev)], main));
+        SPK_HIP(hipStreamWaitEvent(stream, forks[size_t(dev)], 0));
     }
-    hipEvent_t fork;
-    SPK_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
-    SPK_HIP(hipEventRecord(fork, main));
-    SPK_HIP(hipStreamWaitEvent(stream, fork, 0));
-    SPK_HIP(hipEventDestroy(fork));
 }
 
 void SideStream::join()
 {
     if (stream == main || joined) return;
-    hipEvent_t done;
-    SPK_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
     SPK_HIP(hipEventRecord(done, stream));
     SPK_HIP(hipStreamWaitEvent(main, done, 0));
-    SPK_HIP(hipEventDestroy(done));
     joined = true;
 }
 
 SideStream::~SideStream()
 {
-    if (stream != main && !joined)
-    {
-        // an error path: still order the call's stream after the side work
-        hipEvent_t done;
-        if (hipEventCreateWithFlags(&done, hipEventDisableTiming) == hipSuccess)
-        {
-            (void)hipEventRecord(done, stream);
-            (void)hipStreamWaitEvent(main, done, 0);
-            (void)hipEventDestroy(done);
-        }
-    }
+    // an error path: still order the call's stream after the side work
+    if (stream != main && !joined && hipEventRecord(done, stream) == hipSuccess) (void)hipStreamWaitEvent(main, done, 0);
 }
 
 namespace
@@ -652,9 +650,7 @@ void launch_layer(const uint8_t *in, const half8 *frag, Summary *summary, float 
                                     hipFuncAttributeMaxDynamicSharedMemorySize, 163840));
         configured = true;
     }
-    int dev = 0, cus = 0;
-    SPK_HIP(hipGetDevice(&dev));
-    SPK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int cus = cu_count();
     const int grid = g.n < cus ? g.n : cus;
     KernelTimer &tm = timer();
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -796,46 +792,51 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
 
         hipStream_t s = S(stream);
         const int64_t hw = ho * wo, hwi = h * w;
-        const int64_t row_tiles = int64_t(g.waves) * rt;
-        Scratch frag(row_tiles * g.steps * 64 * 16, s), summary(n * hw * int64_t(sizeof(Summary)), s);
-        Scratch wkey(n * count * int64_t(sizeof(int32_t)), s), fsm(learn ? n * hwi * g.cin_pad : 0, s);
+        const int64_t row_tiles = int64_t(g.waves) * rt, ksz = cin * kh * kw;
+        const FusedStdp fp{n * count, int32_t(count), g.cin, g.cin_pad, g.h, g.w, g.kh, g.kw, g.pt, g.pl,
+                           g.wo, int32_t(hw), g.t, int32_t(cdiv(ksz, 32))};
+        // the call's buffers, carved from the stream's workspace: FP16 weight
+        // fragments, summaries, winner keys, first-spike map, STDP bit records
+        const int64_t sizes[5] = {row_tiles * g.steps * 64 * 16, n * hw * int64_t(sizeof(Summary)),
+                                  n * count * int64_t(sizeof(int32_t)), learn ? n * hwi * g.cin_pad : 0,
+                                  learn ? n * count * fp.words * int64_t(sizeof(uint32_t)) : 0};
+        int64_t offs[6] = {0};
+        for (int i = 0; i < 5; ++i) offs[i + 1] = offs[i] + cdiv(sizes[i], 256) * 256;
+        uint8_t *ws = static_cast<uint8_t *>(stream_workspace(0, s, offs[5]));
+        half8 *frag = reinterpret_cast<half8 *>(ws + offs[0]);
+        Summary *summary = reinterpret_cast<Summary *>(ws + offs[1]);
+        int32_t *wkey = reinterpret_cast<int32_t *>(ws + offs[2]);
+        uint8_t *fsm = learn ? ws + offs[3] : nullptr;
+        uint32_t *bits = reinterpret_cast<uint32_t *>(ws + offs[4]);
         // pairs of K-steps share 16-byte reads when 16-byte units stay inside a tap
         const bool paired = g.cin_pad % 16 == 0 && g.steps >= 2;
         prep_fragments<<<dim3(unsigned(cdiv(row_tiles * g.steps * 64, 256))), 256, 0, s>>>(
-            (const float *)kernel, frag.as<half8>(), g.cout, g.cin, g.kh, g.kw, g.cin_pad, g.groups, g.steps,
-            int32_t(row_tiles), paired ? 1 : 0);
+            (const float *)kernel, frag, g.cout, g.cin, g.kh, g.kw, g.cin_pad, g.groups, g.steps, int32_t(row_tiles),
+            paired ? 1 : 0);
         SPK_LAUNCHED();
-        dispatch_layer<2>(g.steps, paired, (const uint8_t *)in, frag.as<half8>(), summary.as<Summary>(),
-                          (float *)potentials, learn ? fsm.as<uint8_t>() : nullptr, g, size_t(lds), s);
+        dispatch_layer<2>(g.steps, paired, (const uint8_t *)in, frag, summary, (float *)potentials, fsm, g,
+                          size_t(lds), s);
         // the writer needs only the summaries: it runs on a side stream,
         // overlapping convwta, the records and the STDP chains (HBM-bound
         // beside latency-bound), and the call's stream waits for it at the end
         SideStream side(s, out || pooled);
         if (out || pooled)
         {
-            int dev = 0, cus = 0;
-            SPK_HIP(hipGetDevice(&dev));
-            SPK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            const int cus = cu_count();
             fused_write<<<dim3(unsigned(n * t < cus ? n * t : cus)), 256, 0, side.stream>>>(
-                summary.as<Summary>(), (uint8_t *)out, (uint8_t *)pooled, g.t, g.cout, g.ho, g.wo, int32_t(pkh),
+                summary, (uint8_t *)out, (uint8_t *)pooled, g.t, g.cout, g.ho, g.wo, int32_t(pkh),
                 int32_t(pkw), int32_t(psh), int32_t(psw), int32_t(pho), int32_t(pwo), n * t);
             SPK_LAUNCHED();
         }
-        fused_select<<<dim3(unsigned(n)), 256, size_t(hw) * 8, s>>>(summary.as<Summary>(), g.t, g.ho, g.wo,
-                                                                    int32_t(ry), int32_t(rx), int32_t(count), winners,
-                                                                    counts, wkey.as<int32_t>());
+        fused_select<<<dim3(unsigned(n)), 256, size_t(hw) * 8, s>>>(summary, g.t, g.ho, g.wo, int32_t(ry),
+                                                                    int32_t(rx), int32_t(count), winners, counts, wkey);
         SPK_LAUNCHED();
         if (learn)
         {
-            const int64_t ksz = cin * kh * kw;
-            const FusedStdp fp{n * count, int32_t(count), g.cin, g.cin_pad, g.h, g.w, g.kh, g.kw, g.pt, g.pl,
-                               g.wo, int32_t(hw), g.t, int32_t(cdiv(ksz, 32))};
-            Scratch bits(n * count * fp.words * int64_t(sizeof(uint32_t)), s);
-            fused_records<<<dim3(unsigned(cdiv(n * count, 4))), 256, 0, s>>>(
-                fsm.as<uint8_t>(), summary.as<Summary>(), wkey.as<int32_t>(), fp, bits.as<uint32_t>());
+            fused_records<<<dim3(unsigned(cdiv(n * count, 4))), 256, 0, s>>>(fsm, summary, wkey, fp, bits);
             SPK_LAUNCHED();
             fused_stdp<<<dim3(unsigned(cout), unsigned(cdiv(ksz, STDP_THREADS))), STDP_THREADS, 0, s>>>(
-                (float *)kernel, bits.as<uint32_t>(), wkey.as<int32_t>(), fp, cfg.c[0]);
+                (float *)kernel, bits, wkey, fp, cfg.c[0]);
             SPK_LAUNCHED();
         }
         side.join();

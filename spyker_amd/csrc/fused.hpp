@@ -304,6 +304,7 @@ KernelTimer &timer();
 struct SideStream
 {
     hipStream_t main, stream;
+    hipEvent_t done = nullptr;
     bool joined = false;
     SideStream(hipStream_t main, bool use);
     SideStream(const SideStream &) = delete;

@@ -5,8 +5,12 @@
 // handed out again on the next request of that size; cache_clear returns
 // the cached blocks to the driver.  Per-call scratch does not use it: it
 // comes from the stream-ordered pool (hipMallocAsync), so a kernel still
-// reading a block never sees it reused by another stream.
+// reading a block never sees it reused by another stream.  Entry points that
+// run every training step keep a grow-only workspace per stream instead
+// (stream_workspace), which costs no allocation call once warm and lets a
+// stream capture record the call without memory nodes.
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <unordered_map>
 #include <vector>
@@ -41,9 +45,89 @@ const char *type_name(int32_t type)
     return (type >= 0 && type <= 10) ? names[type] : "unknown";
 }
 
+namespace
+{
+struct Workspaces
+{
+    std::mutex mu;
+    std::map<std::tuple<int, int, hipStream_t>, std::pair<void *, int64_t>> map;  // (device, owner, stream)
+};
+Workspaces &workspaces()
+{
+    static Workspaces *w = new Workspaces;
+    return *w;
+}
+}  // namespace
+
+int cu_count()
+{
+    static std::mutex mu;
+    static std::vector<int> cus;
+    int dev = 0;
+    SPK_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if (int(cus.size()) <= dev) cus.resize(size_t(dev) + 1, 0);
+    if (!cus[size_t(dev)]) SPK_HIP(hipDeviceGetAttribute(&cus[size_t(dev)], hipDeviceAttributeMultiprocessorCount, dev));
+    return cus[size_t(dev)];
+}
+
+void *stream_workspace(int owner, hipStream_t stream, int64_t bytes)
+{
+    int dev = 0;
+    SPK_HIP(hipGetDevice(&dev));
+    Workspaces &w = workspaces();
+    std::lock_guard<std::mutex> lock(w.mu);
+    auto &e = w.map[std::make_tuple(dev, owner, stream)];
+    if (e.second >= bytes) return e.first;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    SPK_HIP(hipStreamIsCapturing(stream, &cap));
+    SPK_ARG(cap == hipStreamCaptureStatusNone,
+            "The workspace of this call must grow, which a stream capture cannot do: run the call once before "
+            "capturing it.");
+    // the old block may still be in use by work enqueued on the stream
+    if (e.first)
+    {
+        SPK_HIP(hipStreamSynchronize(stream));
+        SPK_HIP(hipFree(e.first));
+        e.first = nullptr, e.second = 0;
+    }
+    SPK_HIP(hipMalloc(&e.first, size_t(bytes)));
+    e.second = bytes;
+    return e.first;
+}
+
+void clear_workspaces()
+{
+    Workspaces &w = workspaces();
+    std::lock_guard<std::mutex> lock(w.mu);
+    for (auto &kv : w.map)
+        if (kv.second.first) SPK_HIP(hipFree(kv.second.first));
+    w.map.clear();
+}
+
 Scratch::Scratch(int64_t bytes_, hipStream_t stream_) : bytes(bytes_), stream(stream_)
 {
-    if (bytes > 0) SPK_HIP(hipMallocAsync(&ptr, size_t(bytes), stream));
+    if (bytes <= 0) return;
+    // the device's default pool keeps what is freed instead of returning it
+    // to the driver at each synchronization, so steady-state calls do not
+    // map memory again
+    static std::mutex mu;
+    static std::vector<char> done;
+    int dev = 0;
+    SPK_HIP(hipGetDevice(&dev));
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        if (int(done.size()) <= dev) done.resize(size_t(dev) + 1, 0);
+        if (!done[size_t(dev)])
+        {
+            hipMemPool_t pool;
+            SPK_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
+            uint64_t keep = UINT64_MAX;
+            SPK_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+            done[size_t(dev)] = 1;
+        }
+    }
+    SPK_HIP(hipMallocAsync(&ptr, size_t(bytes), stream));
 }
 
 Scratch::~Scratch()
@@ -163,6 +247,7 @@ int32_t spk_cache_clear(void)
         }
         c.free.clear();
         clear_plan_cache();
+        clear_workspaces();
     });
 }
 }
