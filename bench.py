@@ -46,7 +46,7 @@ PEAK_HBM = 8.0e12
 # HBM bytes one full-batch fused_layer launch moves on this workload, from rocprofv3
 # FETCH_SIZE (x2.0, the factor a 512 MiB device copy reads as) + WRITE_SIZE (x1.0):
 # profiles/fused_layer_c2_traffic.txt.  Algorithmic: 90.3 MB input + 9.6 MB summaries/first-spike map.
-TRAFFIC_FUSED_LAYER = 104.8e6
+TRAFFIC_FUSED_LAYER = 101.5e6
 
 
 def make_inputs(dev, n=N, seed=2):
