@@ -835,8 +835,8 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
         {
             fused_records<<<dim3(unsigned(cdiv(n * count, 4))), 256, 0, s>>>(fsm, summary, wkey, fp, bits);
             SPK_LAUNCHED();
-            fused_stdp<<<dim3(unsigned(cout), unsigned(cdiv(ksz, STDP_THREADS))), STDP_THREADS, 0, s>>>(
-                (float *)kernel, bits, wkey, fp, cfg.c[0]);
+            launch_fused_stdp(dim3(unsigned(cout), unsigned(cdiv(ksz, STDP_THREADS))), s, (float *)kernel, bits, wkey,
+                              fp, cfg.c[0]);
             SPK_LAUNCHED();
         }
         side.join();

@@ -3,6 +3,7 @@
 // per-step fire/maximum update, the MFMA chain, and the STDP records/chains.
 #pragma once
 
+#include <cmath>
 #include <vector>
 
 #include "stdp.hpp"
@@ -181,6 +182,49 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// How the STDP clamp max(lower, min(v, upper)) may be evaluated.  The
+// reference's selects (v < upper ? v : upper, then lower < hi ? hi : lower)
+// differ from v_min_f64/v_max_f64 only on ties between a +0 and a -0, so
+// the min/max instructions are exact when upper != 0 and lower != 0
+// (CLAMP_MINMAX); with lower == +0 the max may return -0 where the select
+// returns +0, which adding +0 corrects (CLAMP_MINMAX_P0, -0 + +0 = +0 and
+// x + 0 = x otherwise); any other bound keeps the selects (CLAMP_SELECT).
+enum ClampMode
+{
+    CLAMP_SELECT = 0,
+    CLAMP_MINMAX = 1,
+    CLAMP_MINMAX_P0 = 2
+};
+
+inline int clamp_mode(const spk_stdp_config &c)
+{
+    if (c.upper == 0.0) return CLAMP_SELECT;
+    if (c.lower != 0.0) return CLAMP_MINMAX;
+    return std::signbit(c.lower) ? CLAMP_SELECT : CLAMP_MINMAX_P0;
+}
+
+// The update of stdp_update with the clamp evaluated as MODE allows and the
+// stabilize choice fixed at compile time.
+template <int MODE, bool STAB>
+__device__ __forceinline__ float stdp_step(float w, double rate, double lower, double upper)
+{
+    const double old = double(w);
+    double value = STAB ? (old - lower) * (upper - old) : 1.0;
+    value = __builtin_fma(rate, value, old);
+    double r;
+    if (MODE == CLAMP_SELECT)
+    {
+        const double hi = value < upper ? value : upper;
+        r = lower < hi ? hi : lower;
+    }
+    else
+    {
+        r = __builtin_fmax(__builtin_fmin(value, upper), lower);
+        if (MODE == CLAMP_MINMAX_P0) r = r + 0.0;
+    }
+    return float(r);
+}
+
 constexpr int STDP_THREADS = 256;
 constexpr int STDP_LIST = 1024;  // winners staged in LDS per pass
 
@@ -191,6 +235,7 @@ constexpr int STDP_LIST = 1024;  // winners staged in LDS per pass
 // winner slots, one block-wide scan lists the slots naming the channel in
 // slot order, their bit words are staged in LDS, and every thread runs the
 // chain of its weight 32 updates at a time from a register of bits.
+template <int MODE, bool STAB>
 __global__ void __launch_bounds__(STDP_THREADS)
     fused_stdp(float *__restrict__ kernel, const uint32_t *__restrict__ bits, const int32_t *__restrict__ wkey,
                FusedStdp p, spk_stdp_config cfg)
@@ -275,13 +320,36 @@ __global__ void __launch_bounds__(STDP_THREADS)
                 for (int u = 0; u < 32; ++u)
                     b |= ((lbits[min(q0 + u, mm - 1) * sw + my_word] >> (i & 31)) & 1u) << u;
                 const int32_t n_up = min(32, mm - q0);
-                for (int u = 0; u < n_up; ++u) wr = stdp_update(wr, (b >> u) & 1u, cfg);
+                if (n_up == 32)
+                {
+#pragma unroll
+                    for (int u = 0; u < 32; ++u)
+                        wr = stdp_step<MODE, STAB>(wr, (b >> u) & 1u ? cfg.positive : cfg.negative, cfg.lower,
+                                                   cfg.upper);
+                }
+                else
+                    for (int u = 0; u < n_up; ++u)
+                        wr = stdp_step<MODE, STAB>(wr, (b >> u) & 1u ? cfg.positive : cfg.negative, cfg.lower,
+                                                   cfg.upper);
             }
         __syncthreads();
     }
     if (live) kz[i] = wr;
 }
 
+
+// Launches fused_stdp for the clamp and stabilize choice of cfg.
+inline void launch_fused_stdp(dim3 grid, hipStream_t s, float *kernel, const uint32_t *bits, const int32_t *wkey,
+                              const FusedStdp &p, const spk_stdp_config &cfg)
+{
+    const int mode = clamp_mode(cfg);
+    const bool stab = cfg.stabilize != 0;
+#define STDP_CASE(M, B) \
+    if (mode == M && stab == B) fused_stdp<M, B><<<grid, STDP_THREADS, 0, s>>>(kernel, bits, wkey, p, cfg);
+    STDP_CASE(CLAMP_SELECT, false) STDP_CASE(CLAMP_SELECT, true) STDP_CASE(CLAMP_MINMAX, false)
+    STDP_CASE(CLAMP_MINMAX, true) STDP_CASE(CLAMP_MINMAX_P0, false) STDP_CASE(CLAMP_MINMAX_P0, true)
+#undef STDP_CASE
+}
 
 }  // namespace
 }  // namespace spk

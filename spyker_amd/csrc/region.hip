@@ -918,8 +918,8 @@ void LayerPlan::run(const uint8_t *in, float *kernel, const spk_stdp_config *cfg
         region_records<<<dim3(unsigned(cdiv(stdp.slots, 4))), 256, 0, s>>>(fs.p, summary.p, inst.p, wkey.p, stdp,
                                                                           pt, pl, bits.p);
         SPK_LAUNCHED();
-        fused_stdp<<<dim3(unsigned(cout), unsigned(cdiv(int64_t(cin) * kh * kw, STDP_THREADS))), STDP_THREADS, 0,
-                     s>>>(kernel, bits.p, wkey.p, stdp, *cfg);
+        launch_fused_stdp(dim3(unsigned(cout), unsigned(cdiv(int64_t(cin) * kh * kw, STDP_THREADS))), s, kernel,
+                          bits.p, wkey.p, stdp, *cfg);
         SPK_LAUNCHED();
     }
     side.join();

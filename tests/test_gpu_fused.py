@@ -26,7 +26,7 @@ def H(t):
     return t.cpu().numpy()
 
 
-def oracle_step(orc, x, k, th, radius, count, pad, pool):
+def oracle_step(orc, x, k, th, radius, count, pad, pool, config=(0.004, -0.003)):
     n, t, cin, h, w = x.shape
     cout, _, kh, kw = k.shape
     pt, pl, pb, pr = pad
@@ -35,7 +35,7 @@ def oracle_step(orc, x, k, th, radius, count, pad, pool):
     pot = orc.inhibit(orc.threshold(pot, th))
     win, cnt = orc.convwta(pot, radius, count)
     out = orc.fire(pot)
-    k2 = orc.convstdp(x, k, out, [(0.004, -0.003)], win, cnt, 1, pad)
+    k2 = orc.convstdp(x, k, out, [config], win, cnt, 1, pad)
     pooled = orc.pool(out, pool[0], pool[1]) if pool else None
     return out, win, cnt, k2, pooled
 
@@ -81,6 +81,32 @@ def test_fused_exact_on_dyadic_weights(dev, orc, case):
     assert np.array_equal(H(conv.kernel), o_k)
     if pool:
         assert np.array_equal(H(pooled), o_pool)
+
+
+@pytest.mark.parametrize("config", [
+    # positive, negative, stabilize, lower, upper
+    (0.3, -0.2, True, 0.0, 1.0),      # lower +0: min/max then +0
+    (0.3, -0.2, False, -0.5, 2.0),    # nonzero bounds: min/max
+    (0.3, -0.25, True, 0.7, 0.8),     # nonzero bounds, clamps hit
+    (0.004, -0.003, True, -1.0, 0.0), # upper 0: the reference's selects
+    (0.2, -0.2, True, -0.0, 1.0),     # lower -0: the reference's selects
+])
+def test_fused_stdp_bounds(dev, orc, config):
+    """The fused STDP chains against the oracle under every clamp form."""
+    n, t, cin, h, w, cout, k, pad = 6, 15, 30, 14, 14, 250, 3, (1, 1, 1, 1)
+    rng = np.random.default_rng(17)
+    x = spikes(rng, n, t, cin, h, w, 0.5)
+    kern = (rng.integers(40, 64, (cout, cin, k, k)) / 64).astype(np.float32)
+    pot = orc.conv(x.reshape(n * t, cin, h, w), kern, 1, pad)
+    th = float(np.quantile(pot, 0.9)) + 0.5 / 64
+    conv = spk.Conv(cin, cout, k, pad=pad, device=dev)
+    conv.kernel = T(kern, dev)
+    conv.stdpconfig = [spk.STDPConfig(*config)]
+    out, wn, _ = spk.layer_step(conv, T(x, dev), th, 1, 8)
+    o_out, o_win, o_cnt, o_k, _ = oracle_step(orc, x, kern, th, 1, 8, pad, None, config)
+    assert o_cnt.sum() > 0
+    check_winners(wn, o_win, o_cnt)
+    assert np.array_equal(H(conv.kernel).view(np.uint32), o_k.view(np.uint32))
 
 
 def test_fused_precision_vs_fp64(dev, orc):
