@@ -59,7 +59,23 @@ struct FusedGeom
     int32_t sstep;        // bytes of one staged step (ph * pw * cin_pad)
     int32_t stage_fast;   // staging by quad transposes (hw % 4 == 0, aligned input, cin_pad/4 | block quads)
     float th;             // the conv threshold times 2^-24 (potentials are computed times 2^-24)
+    int32_t pkh, pkw, psh, psw, pho, pwo;  // max-pool of the spikes (pho = 0: none)
+    int32_t zp;                            // bytes of a sample's pooled spikes each tile zeroes
 };
+
+// Zeroes bytes [lo, hi) of dst with the block's threads: 16-byte stores on
+// the aligned middle, byte stores at the two ends.
+__device__ __forceinline__ void zero_slice(uint8_t *dst, int64_t lo, int64_t hi)
+{
+    if (lo >= hi) return;
+    int64_t head = int64_t((16 - (reinterpret_cast<uintptr_t>(dst + lo) & 15)) & 15);
+    head = head < hi - lo ? head : hi - lo;
+    if (int64_t(threadIdx.x) < head) dst[lo + threadIdx.x] = 0;
+    const int64_t body = lo + head, units = (hi - body) / 16;
+    uint4 *b = reinterpret_cast<uint4 *>(dst + body);
+    for (int64_t i = threadIdx.x; i < units; i += blockDim.x) b[i] = make_uint4(0, 0, 0, 0);
+    for (int64_t i = body + units * 16 + threadIdx.x; i < hi; i += blockDim.x) dst[i] = 0;
+}
 
 // Byte offset of K-group grp (8 input channels at one tap) in the stage,
 // relative to the output pixel's own position (the top-left tap of its
@@ -194,13 +210,15 @@ constexpr int layer_threads()
 template <int RT, int S, bool PAIRED, bool DIAG>
 __global__ void __launch_bounds__(layer_threads<RT>())
     fused_layer(const uint8_t *__restrict__ in, const half8 *__restrict__ frag, Summary *__restrict__ summary,
-                float *__restrict__ pot_out, uint8_t *__restrict__ fs_out, FusedGeom g)
+                float *__restrict__ pot_out, uint8_t *__restrict__ fs_out, uint8_t *__restrict__ out,
+                uint8_t *__restrict__ pooled, FusedGeom g)
 {
     extern __shared__ __align__(16) uint8_t lds[];
     uint8_t *stage = lds;                                                       // [t][ph][pw][cin_pad] U8
     uint32_t *any_s = reinterpret_cast<uint32_t *>(lds + g.stage_bytes);        // [2][MAX_WAVES][COLS]
     float *red_v = reinterpret_cast<float *>(any_s + 2 * MAX_WAVES * COLS);     // [2][MAX_WAVES][COLS]
     int32_t *red_i = reinterpret_cast<int32_t *>(red_v + 2 * MAX_WAVES * COLS);  // [2][MAX_WAVES][COLS]
+    uint2 *smap = reinterpret_cast<uint2 *>(red_i + 2 * MAX_WAVES * COLS);       // [ho*wo] {c*, fire mask}
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, q = lane >> 4, j = lane & 15;
     const int32_t hw = g.ho * g.wo, hwi = g.h * g.w, cg = g.cin_pad / 8;
@@ -317,8 +335,28 @@ __global__ void __launch_bounds__(layer_threads<RT>())
             }
         }
 
+        // the sample's spike and pooled blocks: all 0 but the kept channels'
+        // 1s.  The aligned 16-byte units of the spike block are zeroed a few
+        // at a time inside the step loops (so the stores stream out while the
+        // MFMAs run instead of bursting at once on every CU), the pooled
+        // block a slice a tile; after the last tile the block's edge bytes
+        // are zeroed and a barrier orders every zero before the 1s are
+        // scattered.
+        const int64_t s_bytes = int64_t(g.t) * g.cout * hw, p_bytes = int64_t(g.t) * g.cout * g.pho * g.pwo;
+        uint8_t *s_out = out ? out + int64_t(n) * s_bytes : nullptr;
+        uint8_t *p_out = pooled ? pooled + int64_t(n) * p_bytes : nullptr;
+        int64_t s_head = 0, s_units = 0, s_upt = 0;  // edge bytes, 16-byte units, units a tile
+        if (s_out)
+        {
+            s_head = min(int64_t((16 - (reinterpret_cast<uintptr_t>(s_out) & 15)) & 15), s_bytes);
+            s_units = (s_bytes - s_head) / 16;
+            s_upt = (s_units + g.tiles - 1) / g.tiles;
+        }
+        uint4 *s_body = reinterpret_cast<uint4 *>(s_out + s_head);
         for (int32_t tile = 0; tile < g.tiles; ++tile)
         {
+            if (p_out) zero_slice(p_out, int64_t(tile) * g.zp, min(int64_t(tile + 1) * g.zp, p_bytes));
+            const int64_t u0 = int64_t(tile) * s_upt, u1 = min(u0 + s_upt, s_units);
             // reduction buffers alternate between tiles, so two barriers a
             // tile order every write after the reads of the same buffer
             const int32_t buf = (tile & 1) * MAX_WAVES * COLS;
@@ -358,11 +396,14 @@ __global__ void __launch_bounds__(layer_threads<RT>())
                 // pairs of iterations with no branch between them, so the
                 // loads of one stay ahead of the MFMAs of the next
                 int32_t k = 0;
+                int64_t zu = u0 + int32_t(threadIdx.x);  // the thread's next zero unit of this tile
                 for (; k + 2 < g.t; k += 2)
                 {
                     // cA: step k; rB: the bytes of step k+1
                     step_iter(a, rB, cB, cA, g.th, mask, mxb, rA, pb + (k + 2) * g.sstep, toff);
                     diag(k, cA);
+                    if (zu < u1) s_body[zu] = make_uint4(0, 0, 0, 0);
+                    zu += blockDim.x;
                     step_iter(a, rA, cA, cB, g.th, mask, mxb, rB, pb + min(k + 3, last) * g.sstep, toff);
                     diag(k + 1, cB);
                 }
@@ -378,6 +419,7 @@ __global__ void __launch_bounds__(layer_threads<RT>())
                     update_all(cA, g.th, mask, mxb);
                     diag(k, cA);
                 }
+                for (; zu < u1; zu += blockDim.x) s_body[zu] = make_uint4(0, 0, 0, 0);
             }
 
             // steps at which some channel fired, per pixel (bit T-1-t for step t)
@@ -465,6 +507,36 @@ __global__ void __launch_bounds__(layer_threads<RT>())
                 summary[int64_t(n) * hw + p] =
                     Summary{cs, fwd, om ? first * TWO_POW_24 : 0.0f,
                             om ? __builtin_bit_cast(float, ox) * TWO_POW_24 : 0.0f};
+                smap[p] = make_uint2(uint32_t(cs), fwd);
+            }
+        }
+        if (s_out || p_out)
+        {
+            if (s_out)
+            {
+                if (int64_t(threadIdx.x) < s_head) s_out[threadIdx.x] = 0;
+                for (int64_t i = s_head + s_units * 16 + threadIdx.x; i < s_bytes; i += blockDim.x) s_out[i] = 0;
+            }
+            __syncthreads();
+            // each pixel's 1s: its kept channel at every step it fires, in
+            // its spike plane and in every pool window that holds the pixel
+            // (the max of a window of 0/1 bytes is 1 exactly when one of its
+            // pixels is; pixels of a window store the same value)
+            for (int32_t q2 = threadIdx.x; q2 < hw; q2 += blockDim.x)
+            {
+                const uint2 e = smap[q2];
+                const int32_t y = q2 / g.wo, x = q2 % g.wo;
+                const int32_t py0 = y >= g.pkh ? (y - g.pkh) / g.psh + 1 : 0, py1 = min(y / max(g.psh, 1), g.pho - 1);
+                const int32_t px0 = x >= g.pkw ? (x - g.pkw) / g.psw + 1 : 0, px1 = min(x / max(g.psw, 1), g.pwo - 1);
+                for (uint32_t m = e.y; m; m &= m - 1)
+                {
+                    const int32_t st = int32_t(__builtin_ctz(m));
+                    if (s_out) s_out[(int64_t(st) * g.cout + e.x) * hw + q2] = 1;
+                    if (p_out)
+                        for (int32_t py = py0; py <= py1; ++py)
+                            for (int32_t px = px0; px <= px1; ++px)
+                                p_out[((int64_t(st) * g.cout + e.x) * g.pho + py) * g.pwo + px] = 1;
+                }
             }
         }
     }
@@ -527,63 +599,6 @@ __global__ void __launch_bounds__(256) fused_select(const Summary *__restrict__ 
     for (int32_t r = k + int32_t(threadIdx.x); r < count; r += blockDim.x) wkey[s * count + r] = -1;
 }
 
-// Zeroes bytes [0, total) of dst, 16-byte stores where aligned.
-__device__ __forceinline__ void zero_bytes(uint8_t *dst, int64_t total)
-{
-    const int64_t head = int64_t((16 - (reinterpret_cast<uintptr_t>(dst) & 15)) & 15);
-    const int64_t h = head < total ? head : total;
-    if (int64_t(threadIdx.x) < h) dst[threadIdx.x] = 0;
-    uint4 *body = reinterpret_cast<uint4 *>(dst + h);
-    const int64_t units = (total - h) / 16;
-    for (int64_t i = threadIdx.x; i < units; i += blockDim.x) body[i] = make_uint4(0, 0, 0, 0);
-    for (int64_t i = h + units * 16 + threadIdx.x; i < total; i += blockDim.x) dst[i] = 0;
-}
-
-// Spikes [n][t][cout][ho][wo] and (optionally) their max-pool, from the
-// summaries.  A block per CU walks the (sample, step) items, leaving the
-// CUs' other wave slots to the STDP kernels that run beside it.  An item's
-// output is all zero
-// but for at most one 1 per pixel (its kept channel, when that fires at this
-// step), so the block zeroes its spike planes and pooled planes with wide
-// stores, then each pixel stores its 1 into the spike plane of its channel
-// and into every pool window that contains it (the max of a window of 0/1
-// bytes is 1 exactly when one of its pixels is 1; pixels of one window store
-// the same value).  The barrier (a workgroup-scope release and acquire, so
-// the zero stores have completed in the block's L2 first) orders every zero
-// before every 1.
-__global__ void __launch_bounds__(256) fused_write(const Summary *__restrict__ summary, uint8_t *__restrict__ out,
-                                                   uint8_t *__restrict__ pooled, int32_t t, int32_t cout, int32_t ho,
-                                                   int32_t wo, int32_t pkh, int32_t pkw, int32_t psh, int32_t psw,
-                                                   int32_t pho, int32_t pwo, int64_t nt)
-{
-    const int32_t hw = ho * wo, phw = pho * pwo;
-    for (int64_t it = blockIdx.x; it < nt; it += gridDim.x)
-    {
-        const int64_t ns = it / t;
-        const int32_t step = int32_t(it % t);
-        uint8_t *dst = out ? out + it * cout * hw : nullptr;
-        uint8_t *pdst = pooled ? pooled + it * cout * phw : nullptr;
-        if (dst) zero_bytes(dst, int64_t(cout) * hw);
-        if (pdst) zero_bytes(pdst, int64_t(cout) * phw);
-        __syncthreads();
-        for (int32_t p = threadIdx.x; p < hw; p += blockDim.x)
-        {
-            const Summary e = summary[ns * hw + p];
-            if (!((e.mask >> step) & 1u)) continue;
-            if (dst) dst[int64_t(e.c) * hw + p] = 1;
-            if (pdst)
-            {
-                // windows (py, px) with py*psh <= y < py*psh + pkh, likewise x
-                const int32_t y = p / wo, x = p % wo;
-                const int32_t py0 = y >= pkh ? (y - pkh) / psh + 1 : 0, py1 = min(y / psh, pho - 1);
-                const int32_t px0 = x >= pkw ? (x - pkw) / psw + 1 : 0, px1 = min(x / psw, pwo - 1);
-                for (int32_t py = py0; py <= py1; ++py)
-                    for (int32_t px = px0; px <= px1; ++px) pdst[int64_t(e.c) * phw + py * pwo + px] = 1;
-            }
-        }
-    }
-}
-
 }  // namespace
 
 KernelTimer &timer()
@@ -640,8 +655,8 @@ SideStream::~SideStream()
 namespace
 {
 template <int RT, int S, bool PAIRED, bool DIAG>
-void launch_layer(const uint8_t *in, const half8 *frag, Summary *summary, float *pot, uint8_t *fs, const FusedGeom &g,
-                  size_t lds, hipStream_t s)
+void launch_layer(const uint8_t *in, const half8 *frag, Summary *summary, float *pot, uint8_t *fs, uint8_t *out,
+                  uint8_t *pooled, const FusedGeom &g, size_t lds, hipStream_t s)
 {
     static bool configured = false;  // per instantiation; the attribute is per function
     if (!configured)
@@ -660,7 +675,8 @@ void launch_layer(const uint8_t *in, const half8 *frag, Summary *summary, float 
         SPK_HIP(hipEventCreate(&e1));
         SPK_HIP(hipEventRecord(e0, s));
     }
-    fused_layer<RT, S, PAIRED, DIAG><<<dim3(grid), dim3(64 * g.waves), lds, s>>>(in, frag, summary, pot, fs, g);
+    fused_layer<RT, S, PAIRED, DIAG><<<dim3(grid), dim3(64 * g.waves), lds, s>>>(in, frag, summary, pot, fs, out,
+                                                                                pooled, g);
     SPK_LAUNCHED();
     if (tm.on)
     {
@@ -671,17 +687,17 @@ void launch_layer(const uint8_t *in, const half8 *frag, Summary *summary, float 
 
 template <int RT>
 void dispatch_layer(int steps, bool paired, const uint8_t *in, const half8 *frag, Summary *summary, float *pot,
-                    uint8_t *fs, const FusedGeom &g, size_t lds, hipStream_t s)
+                    uint8_t *fs, uint8_t *out, uint8_t *pooled, const FusedGeom &g, size_t lds, hipStream_t s)
 {
     switch (steps)
     {
 #define CASE(k) \
     case k:                                                                                        \
         if (paired)                                                                                \
-            return pot ? launch_layer<RT, k, true, true>(in, frag, summary, pot, fs, g, lds, s)      \
-                       : launch_layer<RT, k, true, false>(in, frag, summary, pot, fs, g, lds, s);    \
-        return pot ? launch_layer<RT, k, false, true>(in, frag, summary, pot, fs, g, lds, s)         \
-                   : launch_layer<RT, k, false, false>(in, frag, summary, pot, fs, g, lds, s);
+            return pot ? launch_layer<RT, k, true, true>(in, frag, summary, pot, fs, out, pooled, g, lds, s)      \
+                       : launch_layer<RT, k, true, false>(in, frag, summary, pot, fs, out, pooled, g, lds, s);    \
+        return pot ? launch_layer<RT, k, false, true>(in, frag, summary, pot, fs, out, pooled, g, lds, s)         \
+                   : launch_layer<RT, k, false, false>(in, frag, summary, pot, fs, out, pooled, g, lds, s);
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
         CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
 #undef CASE
@@ -771,8 +787,8 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
         g.th = float(threshold) * (1.0f / TWO_POW_24);
         SPK_ARG(threshold_scales(threshold), "Fused layer: the threshold is too small to scale exactly.");
         SPK_ARG(g.steps <= MAX_STEPS, "Fused layer: Cin*Kh*Kw too large (K <= 512 after padding).");
-        // stage + any/argmax partials (two buffers each)
-        const int64_t lds = stage_bytes + 3 * 2 * MAX_WAVES * COLS * 4;
+        // stage + any/argmax partials (two buffers each) + the per-pixel map of kept channels and fire masks
+        const int64_t lds = stage_bytes + 3 * 2 * MAX_WAVES * COLS * 4 + ho * wo * 8;
         if (lds > 163840 || ho * wo > 8192)
         {
             // the sample does not fit the LDS whole: the region path
@@ -783,6 +799,12 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
             return;
         }
         g.sstep = int32_t(sstep), g.stage_bytes = int32_t(stage_bytes);
+        g.pkh = pooled ? int32_t(pkh) : 1, g.pkw = pooled ? int32_t(pkw) : 1;
+        g.psh = pooled ? int32_t(psh) : 1, g.psw = pooled ? int32_t(psw) : 1;
+        g.pho = int32_t(pho), g.pwo = int32_t(pwo);
+        // each tile zeroes 1/tiles of the sample's pooled block, in 16-byte multiples
+        SPK_ARG(t * cout * ho * wo < (1LL << 31), "Fused layer: a sample's spikes exceed 2 GiB.");
+        g.zp = int32_t(cdiv(cdiv(t * cout * pho * pwo, g.tiles), 16) * 16);
         g.stage_fast = (h * w) % 4 == 0 && (reinterpret_cast<uintptr_t>(in) & 3) == 0 &&
                        (16 * g.waves) % (g.cin_pad / 4) == 0 && 16 * g.waves / (g.cin_pad / 4) <= h * w / 4;
         SPK_ARG(int64_t(cout) * ho * wo < (1LL << 31), "Fused layer: output plane too large.");
@@ -814,20 +836,9 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
             (const float *)kernel, frag, g.cout, g.cin, g.kh, g.kw, g.cin_pad, g.groups, g.steps, int32_t(row_tiles),
             paired ? 1 : 0);
         SPK_LAUNCHED();
-        dispatch_layer<2>(g.steps, paired, (const uint8_t *)in, frag, summary, (float *)potentials, fsm, g,
-                          size_t(lds), s);
-        // the writer needs only the summaries: it runs on a side stream,
-        // overlapping convwta, the records and the STDP chains (HBM-bound
-        // beside latency-bound), and the call's stream waits for it at the end
-        SideStream side(s, out || pooled);
-        if (out || pooled)
-        {
-            const int cus = cu_count();
-            fused_write<<<dim3(unsigned(n * t < cus ? n * t : cus)), 256, 0, side.stream>>>(
-                summary, (uint8_t *)out, (uint8_t *)pooled, g.t, g.cout, g.ho, g.wo, int32_t(pkh),
-                int32_t(pkw), int32_t(psh), int32_t(psw), int32_t(pho), int32_t(pwo), n * t);
-            SPK_LAUNCHED();
-        }
+        // the kernel also writes the spikes and pooled spikes of each sample
+        dispatch_layer<2>(g.steps, paired, (const uint8_t *)in, frag, summary, (float *)potentials, fsm,
+                          (uint8_t *)out, (uint8_t *)pooled, g, size_t(lds), s);
         fused_select<<<dim3(unsigned(n)), 256, size_t(hw) * 8, s>>>(summary, g.t, g.ho, g.wo, int32_t(ry),
                                                                     int32_t(rx), int32_t(count), winners, counts, wkey);
         SPK_LAUNCHED();
@@ -839,7 +850,6 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
                               fp, cfg.c[0]);
             SPK_LAUNCHED();
         }
-        side.join();
     });
 }
 }
