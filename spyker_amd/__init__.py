@@ -691,8 +691,9 @@ def layer_step(conv: "Conv", array: torch.Tensor, threshold: float, radius, coun
         psh, psw = expand2(pool_stride if pool_stride is not None else pool_kernel)
         pooled = torch.empty((n, t, cout, (ho - pkh) // psh + 1, (wo - pkw) // psw + 1), dtype=torch.uint8,
                              device=a.device)
-    data = torch.zeros((n, count, 5), dtype=torch.int64, device=a.device)
-    cnt = torch.zeros((n,), dtype=torch.int64, device=a.device)
+    # the step writes every winner slot (zeros past a sample's count) and every count
+    data = torch.empty((n, count, 5), dtype=torch.int64, device=a.device)
+    cnt = torch.empty((n,), dtype=torch.int64, device=a.device)
     if learn:
         cfg, ncfg = _configs(conv.stdpconfig)
     else:

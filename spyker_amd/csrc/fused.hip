@@ -596,7 +596,79 @@ __global__ void __launch_bounds__(256) fused_select(const Summary *__restrict__ 
         __syncthreads();
     }
     if (threadIdx.x == 0) counts[s] = k;
-    for (int32_t r = k + int32_t(threadIdx.x); r < count; r += blockDim.x) wkey[s * count + r] = -1;
+    for (int32_t r = k + int32_t(threadIdx.x); r < count; r += blockDim.x)
+        wkey[s * count + r] = -1, win[s * count + r] = spk_winner{0, 0, 0, 0, 0};
+}
+
+// fused_select with one wave per sample, for planes of at most 64 * VPL
+// pixels: pixel p = lane + 64 i is the lane's value i, each round's first
+// maximum is a wave reduction with no barrier, and the unused winner slots
+// are zeroed.  Same arithmetic and tie order as fused_select.
+template <int VPL>
+__global__ void __launch_bounds__(256) fused_select_wave(const Summary *__restrict__ summary, int32_t t, int32_t ho,
+                                                         int32_t wo, int32_t ry, int32_t rx, int32_t count,
+                                                         spk_winner *__restrict__ win, int64_t *__restrict__ counts,
+                                                         int32_t *__restrict__ wkey, int64_t n)
+{
+    const int64_t s = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (s >= n) return;
+    const int lane = threadIdx.x & 63;
+    const int32_t hw = ho * wo;
+    const Summary *sm = summary + s * hw;
+    Summary e[VPL];
+    float m = 0.0f;
+#pragma unroll
+    for (int i = 0; i < VPL; ++i)
+    {
+        const int32_t p = lane + 64 * i;
+        e[i] = p < hw ? sm[p] : Summary{-1, 0, 0.0f, 0.0f};
+        m = m < e[i].max ? e[i].max : m;
+    }
+    m = wave_max(m);
+    const float diff = (m - 0.0f) * 2.0f;
+    float val[VPL];
+    int32_t chan[VPL], py[VPL], px[VPL];
+#pragma unroll
+    for (int i = 0; i < VPL; ++i)
+    {
+        const int32_t p = lane + 64 * i;
+        chan[i] = e[i].c;
+        val[i] = p >= hw ? -INFINITY
+                         : e[i].mask ? float(t - int32_t(__builtin_ctz(e[i].mask)) - 1) + (e[i].first - 0.0f) / diff
+                                     : -FLT_MAX;
+        // slots past the plane sit far outside every box (and hold channel -1)
+        py[i] = p < hw ? p / wo : -(1 << 24), px[i] = p < hw ? p % wo : 0;
+    }
+    int32_t k = 0;
+    for (; k < count; ++k)
+    {
+        ArgMax a{-INFINITY, INT32_MAX};
+#pragma unroll
+        for (int i = 0; i < VPL; ++i)
+            if (better(val[i], chan[i] * hw + lane + 64 * i, a.v, a.i)) a.v = val[i], a.i = chan[i] * hw + lane + 64 * i;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1)
+        {
+            const float ov = __shfl_xor(a.v, d, 64);
+            const int32_t oi = __shfl_xor(a.i, d, 64);
+            if (better(ov, oi, a.v, a.i)) a.v = ov, a.i = oi;
+        }
+        if (fabsf(a.v - (-FLT_MAX)) < 1e-5f) break;
+        const int32_t z = a.i / hw, y = (a.i % hw) / wo, x = (a.i % hw) % wo;
+        if (lane == 0) win[s * count + k] = spk_winner{0, 0, z, y, x}, wkey[s * count + k] = z << 16 | (a.i % hw);
+        // a select on bitwise conditions: the short-circuit form of this test
+        // was compiled with the box condition dropped (only the channel test
+        // masked) by this toolchain
+#pragma unroll
+        for (int i = 0; i < VPL; ++i)
+        {
+            const bool drop = (chan[i] == z) | ((abs(py[i] - y) <= ry) & (abs(px[i] - x) <= rx));
+            val[i] = drop ? -FLT_MAX : val[i];
+        }
+    }
+    if (lane == 0) counts[s] = k;
+    for (int32_t r = k + lane; r < count; r += 64)
+        wkey[s * count + r] = -1, win[s * count + r] = spk_winner{0, 0, 0, 0, 0};
 }
 
 }  // namespace
@@ -839,8 +911,19 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
         // the kernel also writes the spikes and pooled spikes of each sample
         dispatch_layer<2>(g.steps, paired, (const uint8_t *)in, frag, summary, (float *)potentials, fsm,
                           (uint8_t *)out, (uint8_t *)pooled, g, size_t(lds), s);
-        fused_select<<<dim3(unsigned(n)), 256, size_t(hw) * 8, s>>>(summary, g.t, g.ho, g.wo, int32_t(ry),
-                                                                    int32_t(rx), int32_t(count), winners, counts, wkey);
+        // convwta: one wave per sample when the plane fits a wave's registers
+        if (hw <= 256)
+            fused_select_wave<4><<<dim3(unsigned(cdiv(n, 4))), 256, 0, s>>>(summary, g.t, g.ho, g.wo, int32_t(ry),
+                                                                             int32_t(rx), int32_t(count), winners,
+                                                                             counts, wkey, n);
+        else if (hw <= 1024)
+            fused_select_wave<16><<<dim3(unsigned(cdiv(n, 4))), 256, 0, s>>>(summary, g.t, g.ho, g.wo, int32_t(ry),
+                                                                              int32_t(rx), int32_t(count), winners,
+                                                                              counts, wkey, n);
+        else
+            fused_select<<<dim3(unsigned(n)), 256, size_t(hw) * 8, s>>>(summary, g.t, g.ho, g.wo, int32_t(ry),
+                                                                        int32_t(rx), int32_t(count), winners, counts,
+                                                                        wkey);
         SPK_LAUNCHED();
         if (learn)
         {

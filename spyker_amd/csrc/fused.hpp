@@ -311,14 +311,19 @@ __global__ void __launch_bounds__(STDP_THREADS)
         for (int32_t q = threadIdx.x; q < mm * sw; q += STDP_THREADS)
             lbits[q] = bits[int64_t(list[q / sw]) * p.words + word0 + q % sw];
         __syncthreads();
+        // bits of 32 updates of this weight from the staged words
+        auto gather = [&](int32_t q0) {
+            uint32_t b = 0;
+#pragma unroll
+            for (int u = 0; u < 32; ++u) b |= ((lbits[min(q0 + u, mm - 1) * sw + my_word] >> (i & 31)) & 1u) << u;
+            return b;
+        };
+        uint32_t b = live ? gather(0) : 0u;
         if (live)
             for (int32_t q0 = 0; q0 < mm; q0 += 32)
             {
-                // bits of the next 32 updates of this weight, then the chain
-                uint32_t b = 0;
-#pragma unroll
-                for (int u = 0; u < 32; ++u)
-                    b |= ((lbits[min(q0 + u, mm - 1) * sw + my_word] >> (i & 31)) & 1u) << u;
+                // the next chunk's bits are read while this chunk's chain runs
+                const uint32_t b_next = q0 + 32 < mm ? gather(q0 + 32) : 0u;
                 const int32_t n_up = min(32, mm - q0);
                 if (n_up == 32)
                 {
@@ -331,6 +336,7 @@ __global__ void __launch_bounds__(STDP_THREADS)
                     for (int u = 0; u < n_up; ++u)
                         wr = stdp_step<MODE, STAB>(wr, (b >> u) & 1u ? cfg.positive : cfg.negative, cfg.lower,
                                                    cfg.upper);
+                b = b_next;
             }
         __syncthreads();
     }

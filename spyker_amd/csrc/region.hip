@@ -531,7 +531,8 @@ __global__ void __launch_bounds__(1024)
         __syncthreads();
     }
     if (threadIdx.x == 0) counts[s] = k;
-    for (int32_t r = k + int32_t(threadIdx.x); r < count; r += blockDim.x) wkey[s * count + r] = -1;
+    for (int32_t r = k + int32_t(threadIdx.x); r < count; r += blockDim.x)
+        wkey[s * count + r] = -1, win[s * count + r] = spk_winner{0, 0, 0, 0, 0};
 }
 
 // Writes channel planes 0 .. cout-1 (at most 64) of 4 consecutive positions:
