@@ -45,8 +45,9 @@ PEAK_FP16 = 2.5e15
 PEAK_HBM = 8.0e12
 # HBM bytes one full-batch fused_layer launch moves on this workload, from rocprofv3
 # FETCH_SIZE (x2.0, the factor a 512 MiB device copy reads as) + WRITE_SIZE (x1.0):
-# profiles/fused_layer_c2_traffic.txt.  Algorithmic: 90.3 MB input + 9.6 MB summaries/first-spike map.
-TRAFFIC_FUSED_LAYER = 101.5e6
+# profiles/fused_layer_c2_traffic.txt.  Algorithmic: 90.3 MB input + 9.6 MB summaries/first-spike map
+# + 814.0 MB spikes and pooled spikes (the kernel writes them) = 914.0 MB.
+TRAFFIC_FUSED_LAYER = 945.6e6
 
 
 def make_inputs(dev, n=N, seed=2):
