@@ -515,8 +515,10 @@ def front_end_bench(spk, dev, cpu):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # a step takes well under a millisecond: enough warm-up for the clocks and
+    # the allocator to settle, and enough steps for a steady mean
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--cpu-samples", type=int, default=192)
     ap.add_argument("--no-cpu", action="store_true")

@@ -230,12 +230,46 @@ __global__ void __launch_bounds__(layer_threads<RT>())
     for (int s = 0; s < S; ++s)
 #pragma unroll
         for (int r = 0; r < RT; ++r) a[r][s] = frag[((w * RT + r) * S + s) * 64 + lane];
+    // wait for the fragments here: with their loads still pending where the
+    // tile loop (global stores, no global loads) is first entered, the
+    // compiler drains every outstanding store before each sample's tiles
+    __builtin_amdgcn_s_waitcnt(0);
     const StepOffsets<S, PAIRED> toff(g, q);
     // the zero border and the padded channels are written once; every sample
     // rewrites only the interior
     for (int32_t i = threadIdx.x; i < g.stage_bytes / 16; i += blockDim.x)
         reinterpret_cast<uint4 *>(stage)[i] = make_uint4(0, 0, 0, 0);
 
+    // each sample's 1s: its kept channel at every step it fires, in its
+    // spike plane and in every pool window that holds the pixel (the max of a
+    // window of 0/1 bytes is 1 exactly when one of its pixels is; pixels of a
+    // window store the same value).  One (step, pixel) pair a thread, from
+    // smap; the sample's zeros were stored before the last barrier.
+    auto scatter = [&](int32_t m) {
+        const int64_t s_bytes = int64_t(g.t) * g.cout * hw, p_bytes = int64_t(g.t) * g.cout * g.pho * g.pwo;
+        uint8_t *s_out = out ? out + int64_t(m) * s_bytes : nullptr;
+        uint8_t *p_out = pooled ? pooled + int64_t(m) * p_bytes : nullptr;
+        int32_t st = int32_t(threadIdx.x) / hw, q2 = int32_t(threadIdx.x) - st * hw;
+        const int32_t bst = int32_t(blockDim.x) / hw, bq = int32_t(blockDim.x) - bst * hw;
+        for (; st < g.t; st += bst, q2 += bq)
+        {
+            if (q2 >= hw) q2 -= hw, ++st;
+            if (st >= g.t) break;
+            const uint2 e = smap[q2];
+            if (!((e.y >> st) & 1u)) continue;
+            if (s_out) s_out[(int64_t(st) * g.cout + e.x) * hw + q2] = 1;
+            if (p_out)
+            {
+                const int32_t y = q2 / g.wo, x = q2 % g.wo;
+                const int32_t py0 = y >= g.pkh ? (y - g.pkh) / g.psh + 1 : 0, py1 = min(y / max(g.psh, 1), g.pho - 1);
+                const int32_t px0 = x >= g.pkw ? (x - g.pkw) / g.psw + 1 : 0, px1 = min(x / max(g.psw, 1), g.pwo - 1);
+                for (int32_t py = py0; py <= py1; ++py)
+                    for (int32_t px = px0; px <= px1; ++px)
+                        p_out[((int64_t(st) * g.cout + e.x) * g.pho + py) * g.pwo + px] = 1;
+            }
+        }
+    };
+    int32_t prev = -1;  // the sample whose 1s are not stored yet
     for (int32_t n = blockIdx.x; n < g.n; n += gridDim.x)
     {
         __syncthreads();
@@ -334,14 +368,18 @@ __global__ void __launch_bounds__(layer_threads<RT>())
                 *reinterpret_cast<uint2 *>(fs_out + (int64_t(n) * hwi * cg + u) * 8) = make_uint2(lo, hi);
             }
         }
+        // the previous sample's 1s, after this sample's staging so that its
+        // loads do not wait behind these stores; smap still holds that
+        // sample's map (the first tile's end rewrites it, after two barriers)
+        if (prev >= 0 && (out || pooled)) scatter(prev);
 
         // the sample's spike and pooled blocks: all 0 but the kept channels'
         // 1s.  The aligned 16-byte units of the spike block are zeroed a few
         // at a time inside the step loops (so the stores stream out while the
         // MFMAs run instead of bursting at once on every CU), the pooled
-        // block a slice a tile; after the last tile the block's edge bytes
-        // are zeroed and a barrier orders every zero before the 1s are
-        // scattered.
+        // block a slice a tile, the block's edge bytes after the last tile;
+        // the 1s are scattered after the next barriers (at the next sample's
+        // start, or after the last sample).
         const int64_t s_bytes = int64_t(g.t) * g.cout * hw, p_bytes = int64_t(g.t) * g.cout * g.pho * g.pwo;
         uint8_t *s_out = out ? out + int64_t(n) * s_bytes : nullptr;
         uint8_t *p_out = pooled ? pooled + int64_t(n) * p_bytes : nullptr;
@@ -510,35 +548,17 @@ __global__ void __launch_bounds__(layer_threads<RT>())
                 smap[p] = make_uint2(uint32_t(cs), fwd);
             }
         }
-        if (s_out || p_out)
+        if (s_out)
         {
-            if (s_out)
-            {
-                if (int64_t(threadIdx.x) < s_head) s_out[threadIdx.x] = 0;
-                for (int64_t i = s_head + s_units * 16 + threadIdx.x; i < s_bytes; i += blockDim.x) s_out[i] = 0;
-            }
-            __syncthreads();
-            // each pixel's 1s: its kept channel at every step it fires, in
-            // its spike plane and in every pool window that holds the pixel
-            // (the max of a window of 0/1 bytes is 1 exactly when one of its
-            // pixels is; pixels of a window store the same value)
-            for (int32_t q2 = threadIdx.x; q2 < hw; q2 += blockDim.x)
-            {
-                const uint2 e = smap[q2];
-                const int32_t y = q2 / g.wo, x = q2 % g.wo;
-                const int32_t py0 = y >= g.pkh ? (y - g.pkh) / g.psh + 1 : 0, py1 = min(y / max(g.psh, 1), g.pho - 1);
-                const int32_t px0 = x >= g.pkw ? (x - g.pkw) / g.psw + 1 : 0, px1 = min(x / max(g.psw, 1), g.pwo - 1);
-                for (uint32_t m = e.y; m; m &= m - 1)
-                {
-                    const int32_t st = int32_t(__builtin_ctz(m));
-                    if (s_out) s_out[(int64_t(st) * g.cout + e.x) * hw + q2] = 1;
-                    if (p_out)
-                        for (int32_t py = py0; py <= py1; ++py)
-                            for (int32_t px = px0; px <= px1; ++px)
-                                p_out[((int64_t(st) * g.cout + e.x) * g.pho + py) * g.pwo + px] = 1;
-                }
-            }
+            if (int64_t(threadIdx.x) < s_head) s_out[threadIdx.x] = 0;
+            for (int64_t i = s_head + s_units * 16 + threadIdx.x; i < s_bytes; i += blockDim.x) s_out[i] = 0;
         }
+        prev = n;
+    }
+    if (prev >= 0 && (out || pooled))
+    {
+        __syncthreads();
+        scatter(prev);
     }
 }
 
@@ -743,8 +763,8 @@ void launch_layer(const uint8_t *in, const half8 *frag, Summary *summary, float 
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (tm.on)
     {
-        SPK_HIP(hipEventCreate(&e0));
-        SPK_HIP(hipEventCreate(&e1));
+        SPK_HIP(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));
+        SPK_HIP(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence));
         SPK_HIP(hipEventRecord(e0, s));
     }
     fused_layer<RT, S, PAIRED, DIAG><<<dim3(grid), dim3(64 * g.waves), lds, s>>>(in, frag, summary, pot, fs, out,

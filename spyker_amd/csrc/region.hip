@@ -852,8 +852,8 @@ void LayerPlan::choose(const uint8_t *in, hipStream_t s)
     // run that follows rewrites)
     std::vector<float> ms(configs.size(), 0.0f);
     hipEvent_t e0, e1;
-    SPK_HIP(hipEventCreate(&e0));
-    SPK_HIP(hipEventCreate(&e1));
+    SPK_HIP(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));
+    SPK_HIP(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence));
     for (size_t c = 0; c < configs.size(); ++c)
         for (int rep = 0; rep < 2; ++rep)  // the first launch configures and warms
         {
@@ -886,8 +886,8 @@ void LayerPlan::run(const uint8_t *in, float *kernel, const spk_stdp_config *cfg
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (tm.on)
     {
-        SPK_HIP(hipEventCreate(&e0));
-        SPK_HIP(hipEventCreate(&e1));
+        SPK_HIP(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));
+        SPK_HIP(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence));
         SPK_HIP(hipEventRecord(e0, s));
     }
     dispatch_region(in, frag.p, inst.p, *configs[chosen], summary.p, pot, learn ? fs.p : nullptr, grid, s);
