@@ -193,14 +193,22 @@ enum ClampMode
 {
     CLAMP_SELECT = 0,
     CLAMP_MINMAX = 1,
-    CLAMP_MINMAX_P0 = 2
+    CLAMP_MINMAX_P0 = 2,
+    CLAMP_MED3_F32 = 3
 };
 
+// When both bounds are F32 values with lower <= upper, the clamp moves after
+// the rounding to F32: rounding to nearest is monotone and leaves the bounds
+// unchanged, so clamp-then-round equals round-then-clamp, and one v_med3_f32
+// does the clamp.  v_med3_f32 (like v_max) orders -0 below +0, so a lower
+// bound of +0 turns -0 into +0 as the min/max-then-+0 form does.
 inline int clamp_mode(const spk_stdp_config &c)
 {
     if (c.upper == 0.0) return CLAMP_SELECT;
-    if (c.lower != 0.0) return CLAMP_MINMAX;
-    return std::signbit(c.lower) ? CLAMP_SELECT : CLAMP_MINMAX_P0;
+    if (c.lower == 0.0 && std::signbit(c.lower)) return CLAMP_SELECT;
+    if (double(float(c.lower)) == c.lower && double(float(c.upper)) == c.upper && c.lower <= c.upper)
+        return CLAMP_MED3_F32;
+    return c.lower != 0.0 ? CLAMP_MINMAX : CLAMP_MINMAX_P0;
 }
 
 // The update of stdp_update with the clamp evaluated as MODE allows and the
@@ -211,6 +219,7 @@ __device__ __forceinline__ float stdp_step(float w, double rate, double lower, d
     const double old = double(w);
     double value = STAB ? (old - lower) * (upper - old) : 1.0;
     value = __builtin_fma(rate, value, old);
+    if (MODE == CLAMP_MED3_F32) return __builtin_amdgcn_fmed3f(float(value), float(lower), float(upper));
     double r;
     if (MODE == CLAMP_SELECT)
     {
@@ -354,6 +363,7 @@ inline void launch_fused_stdp(dim3 grid, hipStream_t s, float *kernel, const uin
     if (mode == M && stab == B) fused_stdp<M, B><<<grid, STDP_THREADS, 0, s>>>(kernel, bits, wkey, p, cfg);
     STDP_CASE(CLAMP_SELECT, false) STDP_CASE(CLAMP_SELECT, true) STDP_CASE(CLAMP_MINMAX, false)
     STDP_CASE(CLAMP_MINMAX, true) STDP_CASE(CLAMP_MINMAX_P0, false) STDP_CASE(CLAMP_MINMAX_P0, true)
+    STDP_CASE(CLAMP_MED3_F32, false) STDP_CASE(CLAMP_MED3_F32, true)
 #undef STDP_CASE
 }
 

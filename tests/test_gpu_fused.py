@@ -109,6 +109,30 @@ def test_fused_stdp_bounds(dev, orc, config):
     assert np.array_equal(H(conv.kernel).view(np.uint32), o_k.view(np.uint32))
 
 
+def test_fused_stdp_negative_zero_weights(dev, orc):
+    """-0 weights under a +0 lower bound: the reference's clamp returns +0, so
+    the F32 clamp (after the rounding) must order -0 below +0."""
+    n, t, cin, h, w, cout, k, pad = 6, 15, 30, 14, 14, 250, 3, (1, 1, 1, 1)
+    rng = np.random.default_rng(18)
+    x = spikes(rng, n, t, cin, h, w, 0.5)
+    kern = (rng.integers(40, 64, (cout, cin, k, k)) / 64).astype(np.float32)
+    kern[:, :3] = -0.0
+    pot = orc.conv(x.reshape(n * t, cin, h, w), kern, 1, pad)
+    th = float(np.quantile(pot, 0.9)) + 0.5 / 64
+    for config in [(0.3, -0.2, True, 0.0, 1.0), (0.3, -0.2, False, 0.0, 1.0)]:
+        conv = spk.Conv(cin, cout, k, pad=pad, device=dev)
+        conv.kernel = T(kern, dev)
+        conv.stdpconfig = [spk.STDPConfig(*config)]
+        _, wn, _ = spk.layer_step(conv, T(x, dev), th, 1, 8)
+        _, o_win, o_cnt, o_k, _ = oracle_step(orc, x, kern, th, 1, 8, pad, None, config)
+        assert o_cnt.sum() > 0
+        check_winners(wn, o_win, o_cnt)
+        got = H(conv.kernel)
+        assert np.array_equal(got.view(np.uint32), o_k.view(np.uint32)), config
+        # some -0 weights were updated to +0 (the case under test)
+        assert ((o_k[:, :3] == 0) & ~np.signbit(o_k[:, :3])).any()
+
+
 def test_fused_precision_vs_fp64(dev, orc):
     n, t, cin, h, w, cout = 8, 15, 30, 14, 14, 250
     rng = np.random.default_rng(12)
