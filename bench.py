@@ -47,7 +47,7 @@ PEAK_HBM = 8.0e12
 # FETCH_SIZE (x2.0, the factor a 512 MiB device copy reads as) + WRITE_SIZE (x1.0):
 # profiles/fused_layer_c2_traffic.txt.  Algorithmic: 90.3 MB input + 9.6 MB summaries/first-spike map
 # + 814.0 MB spikes and pooled spikes (the kernel writes them) = 914.0 MB.
-TRAFFIC_FUSED_LAYER = 945.6e6
+TRAFFIC_FUSED_LAYER = 943.6e6
 
 
 def make_inputs(dev, n=N, seed=2):

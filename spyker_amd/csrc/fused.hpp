@@ -194,7 +194,8 @@ enum ClampMode
     CLAMP_SELECT = 0,
     CLAMP_MINMAX = 1,
     CLAMP_MINMAX_P0 = 2,
-    CLAMP_MED3_F32 = 3
+    CLAMP_MED3_F32 = 3,
+    CLAMP_MED3_F32_L0 = 4  // CLAMP_MED3_F32 with lower == +0: old - lower is old itself (also for -0)
 };
 
 // When both bounds are F32 values with lower <= upper, the clamp moves after
@@ -207,7 +208,7 @@ inline int clamp_mode(const spk_stdp_config &c)
     if (c.upper == 0.0) return CLAMP_SELECT;
     if (c.lower == 0.0 && std::signbit(c.lower)) return CLAMP_SELECT;
     if (double(float(c.lower)) == c.lower && double(float(c.upper)) == c.upper && c.lower <= c.upper)
-        return CLAMP_MED3_F32;
+        return c.lower == 0.0 ? CLAMP_MED3_F32_L0 : CLAMP_MED3_F32;
     return c.lower != 0.0 ? CLAMP_MINMAX : CLAMP_MINMAX_P0;
 }
 
@@ -217,9 +218,10 @@ template <int MODE, bool STAB>
 __device__ __forceinline__ float stdp_step(float w, double rate, double lower, double upper)
 {
     const double old = double(w);
-    double value = STAB ? (old - lower) * (upper - old) : 1.0;
+    double value = STAB ? (MODE == CLAMP_MED3_F32_L0 ? old : old - lower) * (upper - old) : 1.0;
     value = __builtin_fma(rate, value, old);
-    if (MODE == CLAMP_MED3_F32) return __builtin_amdgcn_fmed3f(float(value), float(lower), float(upper));
+    if (MODE == CLAMP_MED3_F32 || MODE == CLAMP_MED3_F32_L0)
+        return __builtin_amdgcn_fmed3f(float(value), float(lower), float(upper));
     double r;
     if (MODE == CLAMP_SELECT)
     {
@@ -363,7 +365,8 @@ inline void launch_fused_stdp(dim3 grid, hipStream_t s, float *kernel, const uin
     if (mode == M && stab == B) fused_stdp<M, B><<<grid, STDP_THREADS, 0, s>>>(kernel, bits, wkey, p, cfg);
     STDP_CASE(CLAMP_SELECT, false) STDP_CASE(CLAMP_SELECT, true) STDP_CASE(CLAMP_MINMAX, false)
     STDP_CASE(CLAMP_MINMAX, true) STDP_CASE(CLAMP_MINMAX_P0, false) STDP_CASE(CLAMP_MINMAX_P0, true)
-    STDP_CASE(CLAMP_MED3_F32, false) STDP_CASE(CLAMP_MED3_F32, true)
+    STDP_CASE(CLAMP_MED3_F32, false) STDP_CASE(CLAMP_MED3_F32, true) STDP_CASE(CLAMP_MED3_F32_L0, false)
+    STDP_CASE(CLAMP_MED3_F32_L0, true)
 #undef STDP_CASE
 }
 
