@@ -46,8 +46,9 @@ PEAK_HBM = 8.0e12
 # HBM bytes one full-batch fused_layer launch moves on this workload, from rocprofv3
 # FETCH_SIZE (x2.0, the factor a 512 MiB device copy reads as) + WRITE_SIZE (x1.0):
 # profiles/fused_layer_c2_traffic.txt.  Algorithmic: 90.3 MB input + 9.6 MB summaries/first-spike map
-# + 814.0 MB spikes and pooled spikes (the kernel writes them) = 914.0 MB.
-TRAFFIC_FUSED_LAYER = 943.6e6
+# + 814.0 MB spikes and pooled spikes (the kernel writes them) = 914.0 MB; the spike lines that get 1s
+# are written twice (non-temporal zeros, then the 1s).
+TRAFFIC_FUSED_LAYER = 979.2e6
 
 
 def make_inputs(dev, n=N, seed=2):

@@ -393,7 +393,8 @@ int32_t spk_sparse_convstdp(const spk_sparse *in, void *kernel, int32_t kernel_t
  * arithmetic.  in: U8 [n][t][cin][h][w]; kernel: F32 [cout][cin][kh][kw]
  * (updated in place when nconfigs > 0); out: U8 [n][t][cout][ho][wo] or NULL
  * (required for learning); pooled: U8 [n][t][cout][pho][pwo] or NULL;
- * winners/counts: device [n][count], [n]; potentials: NULL, or F32
+ * winners/counts: device [n][count], [n], every slot written (zeros past a
+ * sample's count, so they need no clearing); potentials: NULL, or F32
  * [n][t][cout][ho][wo] receiving the raw FP16-path potentials (a check
  * output).  Limits: t <= 32, 2 <= cout <= 256, cin*kh*kw <= 512 after
  * padding cin to 8, one sample's input staged in 160 KiB of LDS, ho*wo <=

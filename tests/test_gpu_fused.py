@@ -61,6 +61,9 @@ def check_winners(wn, win, cnt):
     (4, 12, 3, 11, 9, 40, 3, (0, 1, 2, 0), 4.0, (1, 2), 4, None, True),
     # isolated spikes: potentials rise and fall, fire masks have gaps
     (4, 10, 16, 12, 12, 48, 3, (1, 1, 1, 1), 6.0, 1, 6, (2, 2), False),
+    # a plane of 1296 pixels: convwta by the block-per-sample kernel (the
+    # smaller planes above take the wave-per-sample one)
+    (2, 8, 4, 36, 36, 32, 3, (1, 1, 1, 1), 5.0, 2, 6, (2, 2), True),
 ])
 def test_fused_exact_on_dyadic_weights(dev, orc, case):
     n, t, cin, h, w, cout, k, pad, th, radius, count, pool, cumulative = case
