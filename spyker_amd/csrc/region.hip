@@ -547,14 +547,15 @@ __device__ __forceinline__ void store_planes(uint8_t *d, int64_t plane, int32_t 
                            uint32_t((cm[2] >> (8 * g)) & 0xFF) << 16 | uint32_t((cm[3] >> (8 * g)) & 0xFF) << 24;
         const int32_t n = min(8, cout - 8 * g);
         for (int32_t i = 0; i < n; ++i)
-            *reinterpret_cast<uint32_t *>(d + (8 * g + i) * plane) = (x >> i) & 0x01010101u;
+            __builtin_nontemporal_store((x >> i) & 0x01010101u, reinterpret_cast<uint32_t *>(d + (8 * g + i) * plane));
     }
 }
 
 // Spikes [t][cout][ho][wo] of every instance from the summaries.  Grid
 // (chunk, instance): a thread owns 4 consecutive pixels (one when hw % 4 !=
 // 0 or the instance's spikes are not 4-aligned), reads their summaries once
-// and writes every step's channel planes, one 32-bit store per plane.
+// and writes every step's channel planes, one 32-bit store per plane.  The
+// 32-bit stores are non-temporal: nothing here reads the output again.
 constexpr int WRITE_THREADS = 256;
 
 __global__ void __launch_bounds__(WRITE_THREADS)
@@ -593,8 +594,9 @@ __global__ void __launch_bounds__(WRITE_THREADS)
         }
         else if (quad)
             for (int32_t ch = 0; ch < cout; ++ch)
-                *reinterpret_cast<uint32_t *>(d + ch * plane) = uint32_t(a[0] == ch) | uint32_t(a[1] == ch) << 8 |
-                                                               uint32_t(a[2] == ch) << 16 | uint32_t(a[3] == ch) << 24;
+                __builtin_nontemporal_store(uint32_t(a[0] == ch) | uint32_t(a[1] == ch) << 8 |
+                                                uint32_t(a[2] == ch) << 16 | uint32_t(a[3] == ch) << 24,
+                                            reinterpret_cast<uint32_t *>(d + ch * plane));
         else
             for (int32_t ch = 0; ch < cout; ++ch) d[ch * plane] = a[0] == ch;
     }
