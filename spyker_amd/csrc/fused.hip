@@ -629,6 +629,29 @@ __global__ void __launch_bounds__(256) fused_select(const Summary *__restrict__ 
         wkey[s * count + r] = -1, win[s * count + r] = spk_winner{0, 0, 0, 0, 0};
 }
 
+// The first maximum of (v, then smallest i) over the wave, in every lane:
+// butterflies over quads (quad_perm), eights (row_half_mirror) and rows
+// (row_mirror), then rows 0..3 chained by row_bcast15/row_bcast31 into lane
+// 63, read back as a uniform value.  All 64 lanes must be active.
+__device__ __forceinline__ void wave_argmax_dpp(float &v, int32_t &i)
+{
+    auto step = [&](auto ctrl, auto rows) {
+        const int32_t vb = __builtin_bit_cast(int32_t, v);
+        const float ov = __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(vb, vb, decltype(ctrl)::value,
+                                                                               decltype(rows)::value, 0xF, false));
+        const int32_t oi = __builtin_amdgcn_update_dpp(i, i, decltype(ctrl)::value, decltype(rows)::value, 0xF, false);
+        if (better(ov, oi, v, i)) v = ov, i = oi;
+    };
+    step(std::integral_constant<int, 0xB1>{}, std::integral_constant<int, 0xF>{});   // lane ^ 1
+    step(std::integral_constant<int, 0x4E>{}, std::integral_constant<int, 0xF>{});   // lane ^ 2
+    step(std::integral_constant<int, 0x141>{}, std::integral_constant<int, 0xF>{});  // 7 - lane within 8
+    step(std::integral_constant<int, 0x140>{}, std::integral_constant<int, 0xF>{});  // 15 - lane within 16
+    step(std::integral_constant<int, 0x142>{}, std::integral_constant<int, 0xA>{});  // rows 1, 3 take 15, 47
+    step(std::integral_constant<int, 0x143>{}, std::integral_constant<int, 0xC>{});  // rows 2, 3 take 31
+    v = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int32_t, v), 63));
+    i = __builtin_amdgcn_readlane(i, 63);
+}
+
 // fused_select with one wave per sample, for planes of at most 64 * VPL
 // pixels: pixel p = lane + 64 i is the lane's value i, each round's first
 // maximum is a wave reduction with no barrier, and the unused winner slots
@@ -675,13 +698,7 @@ __global__ void __launch_bounds__(256) fused_select_wave(const Summary *__restri
 #pragma unroll
         for (int i = 0; i < VPL; ++i)
             if (better(val[i], chan[i] * hw + lane + 64 * i, a.v, a.i)) a.v = val[i], a.i = chan[i] * hw + lane + 64 * i;
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1)
-        {
-            const float ov = __shfl_xor(a.v, d, 64);
-            const int32_t oi = __shfl_xor(a.i, d, 64);
-            if (better(ov, oi, a.v, a.i)) a.v = ov, a.i = oi;
-        }
+        wave_argmax_dpp(a.v, a.i);
         if (fabsf(a.v - (-FLT_MAX)) < 1e-5f) break;
         const int32_t z = a.i / hw, y = (a.i % hw) / wo, x = (a.i % hw) % wo;
         if (lane == 0) win[s * count + k] = spk_winner{0, 0, z, y, x}, wkey[s * count + k] = z << 16 | (a.i % hw);
