@@ -437,6 +437,18 @@ int32_t spk_layer_plan_run(spk_layer_plan *plan, const void *in, void *kernel, i
                            const spk_stdp_config *configs, int64_t nconfigs, void *out, void *pooled,
                            spk_winner *winners, int64_t *counts, void *stream);
 int32_t spk_layer_plan_destroy(spk_layer_plan *plan);
+/* The plan's candidate tilings (no reference counterpart: the reference picks
+ * cuDNN algorithms per shape at run time, src/spyker/cuda/conv.cu:161-178).
+ * Candidate i has rows[i] output rows per region and covers the longest
+ * instance's steps in chunks[i] passes of the FP16 windows.  *count gets the
+ * number of candidates (at most capacity are written), *chosen the one runs
+ * use.  spk_layer_plan_select fixes candidate index for every later run (a
+ * caller that tuned once can restore its choice); a plan made with tune != 0
+ * and never run has every candidate, one made with tune = 0 only the first. */
+int32_t spk_layer_plan_candidates(const spk_layer_plan *plan, int32_t *rows, int32_t *chunks, int32_t capacity,
+                                  int32_t *count, int32_t *chosen);
+int32_t spk_layer_plan_select(spk_layer_plan *plan, int32_t index);
+
 
 /* HIP-event timing of the main kernel of spk_stdp_layer_fused, recorded on
  * the launch stream around each launch while enabled (no reference

@@ -233,6 +233,21 @@ class _Lib:
             self._check(rc, "rate_gather")
         return out
 
+    def pad(self, x, pad, value=0):
+        """x: [..., H, W] uint8 or float32 -> [..., H+pt+pb, W+pl+pr], the border set to value."""
+        pt, pl, pb, pr = pad
+        h, w = x.shape[-2:]
+        planes = int(np.prod(x.shape[:-2]))
+        kind = "u8" if x.dtype == np.uint8 else "f32"
+        x = np.ascontiguousarray(x, dtype=np.uint8 if kind == "u8" else np.float32)
+        out = np.zeros(x.shape[:-2] + (h + pt + pb, w + pl + pr), x.dtype)
+        val = C.c_uint8(int(value)) if kind == "u8" else F32(value)
+        rc = self._f("pad_" + kind)(_ptr(x), I64(planes), I64(h), I64(w), _ptr(out), I64(pt), I64(pl), I64(pb),
+                                    I64(pr), val)
+        if self.prefix == "ref_":
+            self._check(rc, "pad")
+        return out
+
     def pool(self, x, kernel, stride=None, pad=(0, 0, 0, 0)):
         """x: [..., H, W] uint8 or float32; pad (top, left, bottom, right)."""
         kh, kw = (kernel, kernel) if np.isscalar(kernel) else kernel

@@ -159,6 +159,26 @@ int ref_rank_fcwta_f32(const float *x, int64_t n, int64_t t, int64_t f, int64_t 
     });
 }
 
+// cpu/pool.cpp:41-51 (Core::cpu_pad, impl.h:236): planes of [h][w] into
+// [h+pt+pb][w+pl+pr], the border filled with value
+int ref_pad_u8(const uint8_t *in, int64_t planes, int64_t h, int64_t w, uint8_t *out, int64_t pt, int64_t pl,
+               int64_t pb, int64_t pr, uint8_t value)
+{
+    return guard([&] {
+        cpu_pad(Dyn3((void *)in, Type::U8, planes, h, w), Dyn3(out, Type::U8, planes, h + pt + pb, w + pl + pr),
+                Len4(pt, pl, pb, pr), Scalar(value));
+    });
+}
+
+int ref_pad_f32(const float *in, int64_t planes, int64_t h, int64_t w, float *out, int64_t pt, int64_t pl, int64_t pb,
+                int64_t pr, float value)
+{
+    return guard([&] {
+        cpu_pad(Dyn3((void *)in, Type::F32, planes, h, w), Dyn3(out, Type::F32, planes, h + pt + pb, w + pl + pr),
+                Len4(pt, pl, pb, pr), Scalar(value));
+    });
+}
+
 int ref_rank_pool_u8(const uint8_t *in, int64_t planes, int64_t h, int64_t w, uint8_t *out, int64_t kh, int64_t kw,
                      int64_t sh, int64_t sw, int64_t pt, int64_t pl, int64_t pb, int64_t pr)
 {

@@ -447,6 +447,18 @@ void orc_pad_u8(const uint8_t *in, int64_t planes, int64_t h, int64_t w, uint8_t
     }
 }
 
+void orc_pad_f32(const float *in, int64_t planes, int64_t h, int64_t w, float *out, int64_t pt, int64_t pl, int64_t pb,
+                 int64_t pr, float value)
+{
+    int64_t oh = h + pt + pb, ow = w + pl + pr;
+    for (int64_t i = 0; i < planes; ++i)
+    {
+        for (int64_t j = 0; j < oh * ow; ++j) out[i * oh * ow + j] = value;
+        for (int64_t j = 0; j < h; ++j)
+            memcpy(out + (i * oh + j + pt) * ow + pl, in + (i * h + j) * w, (size_t)w * sizeof(float));
+    }
+}
+
 /* src/spyker/cpu/pool.cpp:53-86: zero pad, window max seeded with the type's
  * lowest value. */
 #define POOL_BODY(T, LOWEST)                                                                           \

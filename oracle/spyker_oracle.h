@@ -60,6 +60,8 @@ void orc_rank_fcwta_f32(const float *x, int64_t n, int64_t t, int64_t f, int64_t
 /* -- pooling / padding (src/spyker/cpu/pool.cpp:41-86) ------------------- */
 void orc_pad_u8(const uint8_t *in, int64_t planes, int64_t h, int64_t w, uint8_t *out, int64_t pt, int64_t pl,
                 int64_t pb, int64_t pr, uint8_t value);
+void orc_pad_f32(const float *in, int64_t planes, int64_t h, int64_t w, float *out, int64_t pt, int64_t pl, int64_t pb,
+                 int64_t pr, float value);
 void orc_rank_pool_u8(const uint8_t *in, int64_t planes, int64_t h, int64_t w, uint8_t *out, int64_t kh, int64_t kw,
                       int64_t sh, int64_t sw, int64_t pt, int64_t pl, int64_t pb, int64_t pr);
 void orc_rank_pool_f32(const float *in, int64_t planes, int64_t h, int64_t w, float *out, int64_t kh, int64_t kw,

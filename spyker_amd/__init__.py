@@ -790,6 +790,20 @@ class LayerPlan:
         cnt = torch.zeros((len(self),), dtype=torch.int64, device=self.device)
         return out, pooled, data, cnt
 
+    def candidates(self):
+        """(rows per region, time chunks) of each candidate tiling, and the index runs use."""
+        cap = 16
+        rows, chunks = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+        count, chosen = C.c_int32(0), C.c_int32(0)
+        call("spk_layer_plan_candidates", self._h, C.c_void_p(rows.ctypes.data), C.c_void_p(chunks.ctypes.data), cap,
+             C.byref(count), C.byref(chosen))
+        k = min(count.value, cap)
+        return list(zip(rows[:k].tolist(), chunks[:k].tolist())), chosen.value
+
+    def select(self, index: int) -> None:
+        """Runs use candidate ``index`` from now on (no tuning run)."""
+        call("spk_layer_plan_select", self._h, int(index))
+
     def run(self, packed: torch.Tensor, learn: bool = True, buffers=None):
         """One step on a packed input; returns (spikes, pooled-or-None, Winners)."""
         _dev(packed)

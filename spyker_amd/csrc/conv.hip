@@ -191,9 +191,11 @@ template <typename I, typename Acc>
 void conv_run(const void *in, const float *kernel, void *out, int64_t n, const ConvGeom &g, hipStream_t s)
 {
     const int32_t taps = g.cin * g.kh * g.kw;
-    Scratch kt(int64_t(taps) * g.cout_pad * int64_t(sizeof(float)), s);
-    transpose_weights<<<flat_grid(int64_t(taps) * g.cout_pad, 256), 256, 0, s>>>(kernel, kt.as<float>(), g.cout,
-                                                                                  taps, g.cout_pad);
+    // the transposed weights live in the stream's workspace (calls on a stream
+    // run in order), so a captured conv records no allocation
+    float *kt = static_cast<float *>(stream_workspace(1, s, int64_t(taps) * g.cout_pad * int64_t(sizeof(float))));
+    transpose_weights<<<flat_grid(int64_t(taps) * g.cout_pad, 256), 256, 0, s>>>(kernel, kt, g.cout, taps,
+                                                                                  g.cout_pad);
     SPK_LAUNCHED();
     for (int64_t n0 = 0; n0 < n; n0 += 65535)
     {
@@ -201,7 +203,7 @@ void conv_run(const void *in, const float *kernel, void *out, int64_t n, const C
         dim3 grid(unsigned(g.tiles), unsigned(g.cout_pad / COT), unsigned(nb));
         const I *src = (const I *)in + n0 * g.cin * int64_t(g.h) * g.w;
         Acc *dst = (Acc *)out + n0 * g.cout * int64_t(g.ho) * g.wo;
-        conv_direct<I, Acc><<<grid, THREADS, 0, s>>>(src, kt.as<float>(), dst, g);
+        conv_direct<I, Acc><<<grid, THREADS, 0, s>>>(src, kt, dst, g);
         SPK_LAUNCHED();
     }
 }

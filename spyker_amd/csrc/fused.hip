@@ -864,6 +864,22 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
                              void *potentials, void *stream)
 {
     return guard([&] {
+        stdp_layer_fused(in, n, t, cin, h, w, kernel, cout, kh, kw, pt, pl, pb, pr, threshold, ry, rx, count, configs,
+                         nconfigs, out, pooled, pkh, pkw, psh, psw, winners, counts, potentials, stream, nullptr, 0);
+    });
+}
+}
+
+namespace spk
+{
+void stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, int64_t h, int64_t w, void *kernel,
+                      int64_t cout, int64_t kh, int64_t kw, int64_t pt, int64_t pl, int64_t pb, int64_t pr,
+                      double threshold, int64_t ry, int64_t rx, int64_t count, const spk_stdp_config *configs,
+                      int64_t nconfigs, void *out, void *pooled, int64_t pkh, int64_t pkw, int64_t psh, int64_t psw,
+                      spk_winner *winners, int64_t *counts, void *potentials, void *stream, uint8_t *scratch,
+                      int64_t scratch_bytes)
+{
+    {
         SPK_ARG(n >= 1 && t >= 1 && cin >= 1 && h >= 1 && w >= 1 && cout >= 2 && kh >= 1 && kw >= 1,
                 "Fused layer: invalid shapes.");
         SPK_ARG(pt >= 0 && pl >= 0 && pb >= 0 && pr >= 0, "Pad items are incorrect.");
@@ -909,6 +925,8 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
         const int64_t lds = stage_bytes + 3 * 2 * MAX_WAVES * COLS * 4 + ho * wo * 8;
         if (lds > 163840 || ho * wo > 8192)
         {
+            SPK_ARG(!scratch, "Fused layer: a sample that does not fit the LDS whole takes the region path, whose "
+                              "plan allocates its buffers; use the allocating entry point for it.");
             // the sample does not fit the LDS whole: the region path
             const StdpConfigs rcfg = learn ? pack_configs(configs, nconfigs) : StdpConfigs{};
             region_layer_step((const uint8_t *)in, n, t, cin, h, w, (float *)kernel, cout, kh, kw, pt, pl, pb, pr,
@@ -942,7 +960,10 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
                                   learn ? n * count * fp.words * int64_t(sizeof(uint32_t)) : 0};
         int64_t offs[6] = {0};
         for (int i = 0; i < 5; ++i) offs[i + 1] = offs[i] + cdiv(sizes[i], 256) * 256;
-        uint8_t *ws = static_cast<uint8_t *>(stream_workspace(0, s, offs[5]));
+        SPK_ARG(!scratch || offs[5] <= scratch_bytes,
+                "Fused layer: this call needs " << offs[5] << " bytes of scratch, more than the " << scratch_bytes
+                                                << " available; split the batch.");
+        uint8_t *ws = scratch ? scratch : static_cast<uint8_t *>(stream_workspace(0, s, offs[5]));
         half8 *frag = reinterpret_cast<half8 *>(ws + offs[0]);
         Summary *summary = reinterpret_cast<Summary *>(ws + offs[1]);
         int32_t *wkey = reinterpret_cast<int32_t *>(ws + offs[2]);
@@ -979,6 +1000,6 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
                               fp, cfg.c[0]);
             SPK_LAUNCHED();
         }
-    });
+    }
 }
-}
+}  // namespace spk

@@ -403,6 +403,16 @@ struct SideStream
 // The layer step over output regions (region.hip) for a uniform batch whose
 // sample does not fit the LDS whole; same arguments as spk_stdp_layer_fused,
 // cfg = null without learning.
+// The fused layer step (spk_stdp_layer_fused).  scratch = null: its buffers
+// come from the stream's workspace; otherwise from scratch[0, scratch_bytes),
+// and a call that needs more, or the region path, is refused.
+void stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, int64_t h, int64_t w, void *kernel,
+                      int64_t cout, int64_t kh, int64_t kw, int64_t pt, int64_t pl, int64_t pb, int64_t pr,
+                      double threshold, int64_t ry, int64_t rx, int64_t count, const spk_stdp_config *configs,
+                      int64_t nconfigs, void *out, void *pooled, int64_t pkh, int64_t pkw, int64_t psh, int64_t psw,
+                      spk_winner *winners, int64_t *counts, void *potentials, void *stream, uint8_t *scratch,
+                      int64_t scratch_bytes);
+
 void region_layer_step(const uint8_t *in, int64_t n, int64_t t, int64_t cin, int64_t h, int64_t w, float *kernel,
                        int64_t cout, int64_t kh, int64_t kw, int64_t pt, int64_t pl, int64_t pb, int64_t pr,
                        double threshold, int64_t ry, int64_t rx, int64_t count, const spk_stdp_config *cfg,

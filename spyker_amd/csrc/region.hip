@@ -1046,10 +1046,20 @@ struct PlanKey
     std::vector<int64_t> v;
     bool operator==(const PlanKey &o) const { return v == o.v; }
 };
+struct CachedPlan
+{
+    PlanKey key;
+    std::unique_ptr<LayerPlan> plan;
+    bool captured = false;  // a stream capture has recorded a run of it
+};
+// A plan that a capture has used is baked into the captured graph (its tables
+// and work buffers are kernel arguments), so it is never destroyed: eviction
+// and cache_clear move it to `retired`, kept until the process ends.
 struct PlanCache
 {
     std::mutex mu;
-    std::vector<std::pair<PlanKey, std::unique_ptr<LayerPlan>>> plans;  // most recent last
+    std::vector<CachedPlan> plans;  // most recent last
+    std::vector<std::unique_ptr<LayerPlan>> retired;
 };
 PlanCache &plan_cache()
 {
@@ -1064,6 +1074,8 @@ void clear_plan_cache()
     PlanCache &c = plan_cache();
     std::lock_guard<std::mutex> lock(c.mu);
     if (!c.plans.empty()) SPK_HIP(hipDeviceSynchronize());
+    for (auto &e : c.plans)
+        if (e.captured) c.retired.push_back(std::move(e.plan));
     c.plans.clear();
 }
 
@@ -1083,16 +1095,21 @@ void region_layer_step(const uint8_t *in, int64_t n, int64_t t, int64_t cin, int
                  psw, fast, int64_t(reinterpret_cast<uintptr_t>(s)), dev}};
     PlanCache &c = plan_cache();
     std::lock_guard<std::mutex> lock(c.mu);
-    LayerPlan *P = nullptr;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    SPK_HIP(hipStreamIsCapturing(s, &cap));
+    CachedPlan *E = nullptr;
     for (size_t i = 0; i < c.plans.size(); ++i)
-        if (c.plans[i].first == key)
+        if (c.plans[i].key == key)
         {
             std::rotate(c.plans.begin() + long(i), c.plans.begin() + long(i) + 1, c.plans.end());
-            P = c.plans.back().second.get();
+            E = &c.plans.back();
             break;
         }
-    if (!P)
+    if (!E)
     {
+        SPK_ARG(cap == hipStreamCaptureStatusNone,
+                "Fused layer: this shape's plan must be made, which a stream capture cannot do: run the call once on "
+                "the capturing stream before capturing it.");
         std::vector<int32_t> shapes(size_t(3 * n));
         for (int64_t i = 0; i < n; ++i)
             shapes[size_t(3 * i)] = int32_t(t), shapes[size_t(3 * i + 1)] = int32_t(h),
@@ -1103,11 +1120,14 @@ void region_layer_step(const uint8_t *in, int64_t n, int64_t t, int64_t cin, int
         if (c.plans.size() == PLAN_CACHE)
         {
             SPK_HIP(hipDeviceSynchronize());  // the oldest plan's buffers may still be in use
+            if (c.plans.front().captured) c.retired.push_back(std::move(c.plans.front().plan));
             c.plans.erase(c.plans.begin());
         }
-        c.plans.emplace_back(std::move(key), std::move(made));
-        P = c.plans.back().second.get();
+        c.plans.push_back(CachedPlan{std::move(key), std::move(made), false});
+        E = &c.plans.back();
     }
+    if (cap != hipStreamCaptureStatusNone) E->captured = true;
+    LayerPlan *P = E->plan.get();
     P->run(in, kernel, cfg, out, pooled, winners, counts, pot, s);
 }
 }  // namespace spk
@@ -1163,9 +1183,46 @@ int32_t spk_layer_plan_run(spk_layer_plan *handle, const void *in, void *kernel,
         int dev = 0;
         SPK_HIP(hipGetDevice(&dev));
         SPK_ARG(dev == plan->device, "Fused layer: the plan belongs to another device.");
+        if (plan->tune && !plan->tuned)
+        {
+            hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+            SPK_HIP(hipStreamIsCapturing(S(stream), &cap));
+            SPK_ARG(cap == hipStreamCaptureStatusNone,
+                    "Fused layer: a tuning plan times its candidates on its first run, which a stream capture cannot "
+                    "do: run it once before capturing it.");
+        }
         const StdpConfigs cfg = nconfigs > 0 ? pack_configs(configs, nconfigs) : StdpConfigs{};
         plan->run((const uint8_t *)in, (float *)kernel, nconfigs > 0 ? &cfg.c[0] : nullptr, (uint8_t *)out,
                   (uint8_t *)pooled, winners, counts, nullptr, S(stream));
+    });
+}
+
+int32_t spk_layer_plan_candidates(const spk_layer_plan *handle, int32_t *rows, int32_t *chunks, int32_t capacity,
+                                  int32_t *count, int32_t *chosen)
+{
+    return guard([&] {
+        SPK_ARG(handle, "Plan is null.");
+        const LayerPlan *plan = reinterpret_cast<const LayerPlan *>(handle);
+        const int32_t k = int32_t(plan->configs.size());
+        for (int32_t i = 0; i < k && i < capacity; ++i)
+        {
+            const RegionGeom &g = plan->configs[size_t(i)]->g;
+            if (rows) rows[i] = g.rh;
+            if (chunks) chunks[i] = int32_t(cdiv(plan->tmax, g.tc));
+        }
+        if (count) *count = k;
+        if (chosen) *chosen = plan->chosen;
+    });
+}
+
+int32_t spk_layer_plan_select(spk_layer_plan *handle, int32_t index)
+{
+    return guard([&] {
+        SPK_ARG(handle, "Plan is null.");
+        LayerPlan *plan = reinterpret_cast<LayerPlan *>(handle);
+        SPK_ARG(index >= 0 && index < int32_t(plan->configs.size()), "Plan candidate index out of range.");
+        plan->chosen = index;
+        plan->tuned = true;
     });
 }
 

@@ -8,7 +8,8 @@
 // reading a block never sees it reused by another stream.  Entry points that
 // run every training step keep a grow-only workspace per stream instead
 // (stream_workspace), which costs no allocation call once warm and lets a
-// stream capture record the call without memory nodes.
+// stream capture record the call without memory nodes; a block a capture
+// has used is kept for the life of the process.
 #include <map>
 #include <tuple>
 #include <mutex>
@@ -47,10 +48,21 @@ const char *type_name(int32_t type)
 
 namespace
 {
+// A workspace block that a stream capture has referenced is baked into the
+// captured graph's kernel arguments, so it must stay valid for as long as
+// the graph may be replayed: such a block is never freed (growing moves it
+// to `retired`, and cache_clear keeps it) until the process ends.
+struct Workspace
+{
+    void *ptr = nullptr;
+    int64_t bytes = 0;
+    bool captured = false;
+};
 struct Workspaces
 {
     std::mutex mu;
-    std::map<std::tuple<int, int, hipStream_t>, std::pair<void *, int64_t>> map;  // (device, owner, stream)
+    std::map<std::tuple<int, int, hipStream_t>, Workspace> map;  // (device, owner, stream)
+    std::vector<void *> retired;                                  // captured blocks, never freed
 };
 Workspaces &workspaces()
 {
@@ -75,34 +87,49 @@ void *stream_workspace(int owner, hipStream_t stream, int64_t bytes)
 {
     int dev = 0;
     SPK_HIP(hipGetDevice(&dev));
-    Workspaces &w = workspaces();
-    std::lock_guard<std::mutex> lock(w.mu);
-    auto &e = w.map[std::make_tuple(dev, owner, stream)];
-    if (e.second >= bytes) return e.first;
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     SPK_HIP(hipStreamIsCapturing(stream, &cap));
-    SPK_ARG(cap == hipStreamCaptureStatusNone,
-            "The workspace of this call must grow, which a stream capture cannot do: run the call once before "
-            "capturing it.");
-    // the old block may still be in use by work enqueued on the stream
-    if (e.first)
+    Workspaces &w = workspaces();
+    std::lock_guard<std::mutex> lock(w.mu);
+    Workspace &e = w.map[std::make_tuple(dev, owner, stream)];
+    if (e.bytes < bytes)
     {
-        SPK_HIP(hipStreamSynchronize(stream));
-        SPK_HIP(hipFree(e.first));
-        e.first = nullptr, e.second = 0;
+        SPK_ARG(cap == hipStreamCaptureStatusNone,
+                "The workspace of this call must grow, which a stream capture cannot do: run the call once on the "
+                "capturing stream before capturing it.");
+        if (e.ptr)
+        {
+            if (e.captured)
+                w.retired.push_back(e.ptr);  // a captured graph may still read it
+            else
+            {
+                // the old block may still be in use by work enqueued on the stream
+                SPK_HIP(hipStreamSynchronize(stream));
+                SPK_HIP(hipFree(e.ptr));
+            }
+            e = Workspace{};
+        }
+        SPK_HIP(hipMalloc(&e.ptr, size_t(bytes)));
+        e.bytes = bytes;
     }
-    SPK_HIP(hipMalloc(&e.first, size_t(bytes)));
-    e.second = bytes;
-    return e.first;
+    if (cap != hipStreamCaptureStatusNone) e.captured = true;
+    return e.ptr;
 }
 
 void clear_workspaces()
 {
     Workspaces &w = workspaces();
     std::lock_guard<std::mutex> lock(w.mu);
-    for (auto &kv : w.map)
-        if (kv.second.first) SPK_HIP(hipFree(kv.second.first));
-    w.map.clear();
+    for (auto it = w.map.begin(); it != w.map.end();)
+    {
+        if (it->second.captured)
+        {
+            ++it;
+            continue;
+        }
+        if (it->second.ptr) SPK_HIP(hipFree(it->second.ptr));
+        it = w.map.erase(it);
+    }
 }
 
 Scratch::Scratch(int64_t bytes_, hipStream_t stream_) : bytes(bytes_), stream(stream_)

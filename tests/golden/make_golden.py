@@ -157,6 +157,17 @@ def main():
     v["rate_fire_th01"] = R.rate_fire(rp, 0.1)
     v["rate_fire_th0"] = R.rate_fire(rp, 0.0)
 
+    # dense pad (cpu/pool.cpp:41-51): U8 and F32 planes, symmetric and
+    # asymmetric pads, a zero and a nonzero fill value (own stream of draws)
+    rp2 = np.random.Generator(np.random.PCG64(2025))
+    pu8 = rp2.integers(0, 256, (2, 3, 5, 7)).astype(np.uint8)
+    pf32 = rp2.normal(0.0, 1.0, (3, 4, 6, 5)).astype(np.float32)
+    v["pad_in_u8"], v["pad_in_f32"] = pu8, pf32
+    v["pad_u8_p1111_v0"] = R.pad(pu8, (1, 1, 1, 1), 0)
+    v["pad_u8_p2031_v7"] = R.pad(pu8, (2, 0, 3, 1), 7)
+    v["pad_f32_p0102_v0"] = R.pad(pf32, (0, 1, 0, 2), 0.0)
+    v["pad_f32_p3210_vm1p5"] = R.pad(pf32, (3, 2, 1, 0), -1.5)
+
     np.savez_compressed(OUT, **v)
     print("wrote", OUT, os.path.getsize(OUT), "bytes,", len(v), "arrays")
 

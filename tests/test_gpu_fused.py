@@ -64,6 +64,11 @@ def check_winners(wn, win, cnt):
     # a plane of 1296 pixels: convwta by the block-per-sample kernel (the
     # smaller planes above take the wave-per-sample one)
     (2, 8, 4, 36, 36, 32, 3, (1, 1, 1, 1), 5.0, 2, 6, (2, 2), True),
+    # the longest step counts the whole-sample kernel takes: 32-bit fire masks
+    # at full width, with gaps (isolated spikes) and cumulative
+    (3, 32, 8, 12, 12, 64, 3, (1, 1, 1, 1), 10.0, 1, 6, (2, 2), True),
+    (3, 32, 8, 12, 12, 64, 3, (1, 1, 1, 1), 6.0, 1, 6, (2, 2), False),
+    (3, 24, 16, 10, 10, 96, 3, (1, 1, 1, 1), 10.0, 2, 5, (2, 2), True),
 ])
 def test_fused_exact_on_dyadic_weights(dev, orc, case):
     n, t, cin, h, w, cout, k, pad, th, radius, count, pool, cumulative = case
@@ -166,13 +171,14 @@ def test_fused_inference_only_leaves_weights(dev, orc):
     assert H(out).sum() > 0
 
 
-@pytest.mark.skip(reason="a stream capture of spk_stdp_layer_fused on a 1024-sample batch faulted the GPU on its "
-                         "first replay and the cause is not established; capture goes through LayerPlan, which "
-                         "allocates nothing while it runs")
 def test_fused_graph_replay(dev):
     """Replays of a captured layer step (the layer stack of a training loop
     in a hipGraph) give the eager steps' spikes, pooled spikes, winners and
-    weights, step after step, the weights changing in place each step."""
+    weights, step after step, the weights changing in place each step.
+
+    The capture runs on the stream that was warmed, so the stream's workspace
+    already has its size and the capture records no allocation; the workspace
+    a capture used is kept for the life of the process."""
     rng = np.random.default_rng(41)
     n, t, cin, h, w, cout = 12, 15, 30, 14, 14, 250
     x = T(spikes(rng, n, t, cin, h, w), dev)
@@ -192,7 +198,7 @@ def test_fused_graph_replay(dev):
     conv.kernel.copy_(T(kern, dev))
     torch.cuda.synchronize()
     g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g):
+    with torch.cuda.graph(g, stream=side):
         out, wn, pooled = spk.layer_step(conv, x, 10.0, 1, 8, 3, 3)
     conv.kernel.copy_(T(kern, dev))
     torch.cuda.synchronize()

@@ -89,6 +89,22 @@ def test_pool(dev, golden, pad):
     assert np.array_equal(H(spk.pool(T(p, dev), 3, 2, pad)), golden["pool_f32_k3s2_p" + key])
 
 
+@pytest.mark.parametrize("kind,pad,value,key", [
+    ("u8", (1, 1, 1, 1), 0, "pad_u8_p1111_v0"), ("u8", (2, 0, 3, 1), 7, "pad_u8_p2031_v7"),
+    ("f32", (0, 1, 0, 2), 0.0, "pad_f32_p0102_v0"), ("f32", (3, 2, 1, 0), -1.5, "pad_f32_p3210_vm1p5")])
+def test_pad(dev, golden, orc, kind, pad, value, key):
+    """Dense pad (cuda_pad, impl.h:236) against the reference's cpu_pad vectors
+    (cpu/pool.cpp:41-51), and a larger plane set against the oracle."""
+    x = golden["pad_in_" + kind]
+    got = H(spk.pad(T(x, dev), pad, value))
+    assert np.array_equal(got.reshape(golden[key].shape), golden[key])
+    rng = np.random.default_rng(9)
+    big = (rng.integers(0, 256, (3, 5, 7, 33, 41)).astype(np.uint8) if kind == "u8"
+           else rng.normal(0, 1, (3, 5, 7, 33, 41)).astype(np.float32))
+    got = H(spk.pad(T(big, dev), pad, value))
+    assert np.array_equal(got, orc.pad(big, pad, value))
+
+
 @pytest.mark.parametrize("case", [
     # n, t, cin, h, w, cout, kh, kw, stride, pad
     (4, 3, 4, 28, 28, 30, 5, 5, 1, (2, 2, 2, 2)),

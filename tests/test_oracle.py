@@ -133,6 +133,21 @@ def test_pool(orc, golden, pad):
     assert np.array_equal(got.reshape(ref.shape), ref)
 
 
+PAD_CASES = [("u8", (1, 1, 1, 1), 0, "pad_u8_p1111_v0"), ("u8", (2, 0, 3, 1), 7, "pad_u8_p2031_v7"),
+             ("f32", (0, 1, 0, 2), 0.0, "pad_f32_p0102_v0"), ("f32", (3, 2, 1, 0), -1.5, "pad_f32_p3210_vm1p5")]
+
+
+@pytest.mark.parametrize("kind,pad,value,key", PAD_CASES)
+def test_pad(orc, golden, kind, pad, value, key):
+    x = golden["pad_in_" + kind]
+    got = orc.pad(x, pad, value)
+    assert np.array_equal(got, golden[key])
+    # closed form: torch constant pad of the last two dimensions
+    pt, pl, pb, pr = pad
+    ref = torch.nn.functional.pad(torch.from_numpy(x.astype(np.float32)), (pl, pr, pt, pb), value=float(value))
+    assert np.array_equal(got.astype(np.float32), ref.numpy())
+
+
 def test_convstdp(orc, golden):
     g = golden
     k = orc.convstdp(g["stdp_in"], g["stdp_kernel"], g["stdp_out"], [(0.004, -0.003)], g["stdp_win"], g["stdp_cnt"],
