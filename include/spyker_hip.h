@@ -406,6 +406,24 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
                              int64_t pkw, int64_t psh, int64_t psw, spk_winner *winners, int64_t *counts,
                              void *potentials, void *stream);
 
+/* spk_stdp_layer_fused for callers that cannot allocate device memory (code
+ * under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same arguments and results
+ * as spk_stdp_layer_fused; the caller still passes every input and output.
+ * The step's own buffers come from a 16 MiB device array that the library
+ * holds on each GPU it runs on until the process ends, so the call makes no
+ * allocation.  It runs only the whole-sample path: a call whose buffers need
+ * more than 16 MiB (at the layer-2 shape, about 1700 samples) or whose
+ * samples do not fit the LDS whole fails with SPK_ERR_ARGUMENT.  Every call
+ * on a GPU shares that GPU's array: calls that overlap on one GPU (from two
+ * streams or threads) race. */
+int32_t spk_stdp_layer_fused_noalloc(const void *in, int64_t n, int64_t t, int64_t cin, int64_t h, int64_t w,
+                                     void *kernel, int64_t cout, int64_t kh, int64_t kw, int64_t pt, int64_t pl,
+                                     int64_t pb, int64_t pr, double threshold, int64_t ry, int64_t rx, int64_t count,
+                                     const spk_stdp_config *configs, int64_t nconfigs, void *out, void *pooled,
+                                     int64_t pkh, int64_t pkw, int64_t psh, int64_t psw, spk_winner *winners,
+                                     int64_t *counts, void *potentials, void *stream);
+
 /* -- layer plans: batches of uneven instances ------------------------------- */
 
 /* No single reference counterpart: the fused step above for a batch of n

@@ -82,6 +82,8 @@ SIGNATURES = {
     "spk_sparse_convstdp": [_p, _p, _i32, _i64, _i64, _i64, _p, _i64, _p, _p, _i64] + [_i64] * 4 + [_p],
     "spk_stdp_layer_fused": [_p] + [_i64] * 5 + [_p] + [_i64] * 7 + [_f64] + [_i64] * 3 + [_p, _i64, _p, _p]
                             + [_i64] * 4 + [_p, _p, _p, _p],
+    "spk_stdp_layer_fused_noalloc": [_p] + [_i64] * 5 + [_p] + [_i64] * 7 + [_f64] + [_i64] * 3 + [_p, _i64, _p, _p]
+                                    + [_i64] * 4 + [_p, _p, _p, _p],
     "spk_layer_plan_create": [_p, _i64, _p] + [_i64] * 8 + [_f64] + [_i64] * 7 + [_i32],
     "spk_layer_plan_layout": [_p, _p, _p, _p, _p],
     "spk_layer_plan_run": [_p, _p, _p, _i32, _p, _i64, _p, _p, _p, _p, _p],
