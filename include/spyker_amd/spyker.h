@@ -434,7 +434,9 @@ struct LayerStep
 ///     output = fire(pot); layer.stdp(input, winners, output); pooled = pool(output, pool, poolStride)
 /// on FP16 MFMA with FP32 accumulation and the FP32 master weights in
 /// `layer.kernel`.  `learn = false` leaves the weights unchanged.  A pool
-/// kernel of 0 skips pooling.
+/// kernel of 0 skips pooling.  A layer outside the fused step's domain
+/// (stride other than 1, more than 32 steps or 256 output channels, Cin*Kh*Kw
+/// above 512, a negative threshold) runs the same calls on the FP32 plain path.
 LayerStep layerStep(Conv &layer, Tensor input, F64 threshold, Expand2 radius, Size count, Expand2 pool = 0,
                     Expand2 poolStride = 0, bool learn = true);
 }  // namespace Spyker

@@ -396,9 +396,12 @@ int32_t spk_sparse_convstdp(const spk_sparse *in, void *kernel, int32_t kernel_t
  * winners/counts: device [n][count], [n], every slot written (zeros past a
  * sample's count, so they need no clearing); potentials: NULL, or F32
  * [n][t][cout][ho][wo] receiving the raw FP16-path potentials (a check
- * output).  Limits: t <= 32, 2 <= cout <= 256, cin*kh*kw <= 512 after
- * padding cin to 8, one sample's input staged in 160 KiB of LDS, ho*wo <=
- * 8192.  No counterpart in the reference as one call. */
+ * output).  Domain (SPK_ERR_UNSUPPORTED outside it, so a caller can run the
+ * plain path instead): t <= 32, 2 <= cout <= 256, cin*kh*kw <= 512 after
+ * padding cin to 8, threshold >= 0.  A sample whose input fits 160 KiB of
+ * LDS whole (and ho*wo <= 8192) runs the whole-sample kernel; any other runs
+ * the region kernel with a cached plan (ho*wo <= 65536, pool windows <= 16
+ * pixels).  No counterpart in the reference as one call. */
 int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, int64_t h, int64_t w, void *kernel,
                              int64_t cout, int64_t kh, int64_t kw, int64_t pt, int64_t pl, int64_t pb, int64_t pr,
                              double threshold, int64_t ry, int64_t rx, int64_t count,

@@ -661,14 +661,18 @@ class Conv:
 
 def layer_step(conv: "Conv", array: torch.Tensor, threshold: float, radius, count: int, pool_kernel=None,
                pool_stride=None, learn: bool = True, potentials: torch.Tensor = None):
-    """One fused training step of a conv layer on FP16 MFMA:
+    """One training step of a conv layer:
 
         pot = inhibit(threshold(conv(array), threshold)); winners = convwta(pot, radius, count)
         out = fire(pot); conv.stdp(array, winners, out); pooled = pool(out, pool_kernel, pool_stride)
 
-    in a handful of launches that never materialise the FP32 potentials.
-    Returns (out, winners, pooled-or-None).  ``potentials``, if given, is an
-    F32 tensor shaped like ``out`` that receives the raw FP16-path potentials.
+    On the fused FP16-MFMA path (spk_stdp_layer_fused) in a handful of
+    launches that never materialise the FP32 potentials; a layer outside that
+    path's domain (stride other than 1, more than 32 steps or 256 output
+    channels, Cin*Kh*Kw above 512, a negative threshold, ...) runs the same
+    step on the FP32 plain path instead.  Returns (out, winners,
+    pooled-or-None).  ``potentials``, if given, is an F32 tensor shaped like
+    ``out`` that receives the raw conv potentials of whichever path ran.
     """
     _dev(array)
     a = _exact(array, 5)
@@ -676,37 +680,56 @@ def layer_step(conv: "Conv", array: torch.Tensor, threshold: float, radius, coun
         raise RuntimeError("Fused layer: spikes must be u8.")
     if conv.kernel.dtype != torch.float32:
         raise RuntimeError("Fused layer: the master weights must be f32.")
-    if conv._stride != (1, 1):
-        raise RuntimeError("Fused layer: stride must be 1.")
     n, t, cin, h, w = a.shape
     cout, _, kh, kw = conv.kernel.shape
+    sy, sx = conv._stride
     pt, pl, pb, pr = conv._pad
-    ho, wo = h + pt + pb - kh + 1, w + pl + pr - kw + 1
+    ho, wo = (h + pt + pb - kh) // sy + 1, (w + pl + pr - kw) // sx + 1
     ry, rx = expand2(radius)
-    out = torch.empty((n, t, cout, ho, wo), dtype=torch.uint8, device=a.device)
-    pooled = None
     pkh = pkw = psh = psw = 0
     if pool_kernel is not None:
         pkh, pkw = expand2(pool_kernel)
         psh, psw = expand2(pool_stride if pool_stride is not None else pool_kernel)
-        pooled = torch.empty((n, t, cout, (ho - pkh) // psh + 1, (wo - pkw) // psw + 1), dtype=torch.uint8,
-                             device=a.device)
-    # the step writes every winner slot (zeros past a sample's count) and every count
-    data = torch.empty((n, count, 5), dtype=torch.int64, device=a.device)
-    cnt = torch.empty((n,), dtype=torch.int64, device=a.device)
-    if learn:
-        cfg, ncfg = _configs(conv.stdpconfig)
-    else:
-        cfg, ncfg = None, 0
     if potentials is not None:
         _dev(potentials)
-        if potentials.dtype != torch.float32 or tuple(potentials.shape) != tuple(out.shape):
+        if potentials.dtype != torch.float32 or tuple(potentials.shape) != (n, t, cout, ho, wo):
             raise RuntimeError("potentials must be f32 shaped like the output")
-    call("spk_stdp_layer_fused", _ptr(a), n, t, cin, h, w, _ptr(conv.kernel), cout, kh, kw, pt, pl, pb, pr,
-         float(threshold), ry, rx, int(count), cfg, ncfg, _ptr(out),
-         _ptr(pooled) if pooled is not None else None, pkh, pkw, psh, psw, _ptr(data), _ptr(cnt),
-         _ptr(potentials) if potentials is not None else None, _stream(a))
-    return out, Winners(data, cnt), pooled
+    if (sy, sx) == (1, 1):
+        out = torch.empty((n, t, cout, ho, wo), dtype=torch.uint8, device=a.device)
+        pooled = None
+        if pool_kernel is not None:
+            pooled = torch.empty((n, t, cout, (ho - pkh) // psh + 1, (wo - pkw) // psw + 1), dtype=torch.uint8,
+                                 device=a.device)
+        # the step writes every winner slot (zeros past a sample's count) and every count
+        data = torch.empty((n, count, 5), dtype=torch.int64, device=a.device)
+        cnt = torch.empty((n,), dtype=torch.int64, device=a.device)
+        cfg, ncfg = _configs(conv.stdpconfig) if learn else (None, 0)
+        try:
+            call("spk_stdp_layer_fused", _ptr(a), n, t, cin, h, w, _ptr(conv.kernel), cout, kh, kw, pt, pl, pb, pr,
+                 float(threshold), ry, rx, int(count), cfg, ncfg, _ptr(out),
+                 _ptr(pooled) if pooled is not None else None, pkh, pkw, psh, psw, _ptr(data), _ptr(cnt),
+                 _ptr(potentials) if potentials is not None else None, _stream(a))
+            return out, Winners(data, cnt), pooled
+        except _lib.SpykerError as e:
+            if e.code != _lib.ERR_UNSUPPORTED:
+                raise
+    return _plain_layer_step(conv, a, float(threshold), radius, count, pool_kernel, pool_stride, learn, potentials)
+
+
+def _plain_layer_step(conv, a, th, radius, count, pool_kernel, pool_stride, learn, potentials):
+    """The step as the reference runs it, one FP32 plain-path call per stage."""
+    pot = conv(a)
+    if potentials is not None:
+        potentials.copy_(pot)
+    threshold(pot, th)
+    inhibit(pot)
+    win = convwta(pot, radius, count)
+    out = fire(pot)
+    del pot
+    if learn:
+        conv.stdp(a, win, out)
+    pooled = pool(out, pool_kernel, pool_stride) if pool_kernel is not None else None
+    return out, win, pooled
 
 
 class LayerPlan:

@@ -118,11 +118,23 @@ def load():
     return lib
 
 
+# spk_status values (include/spyker_hip.h)
+ERR_ARGUMENT, ERR_UNSUPPORTED, ERR_RUNTIME, ERR_CONFIG = -1, -2, -3, -4
+
+
+class SpykerError(RuntimeError):
+    """A nonzero status from the library: the message and the spk_status code."""
+
+    def __init__(self, msg, code):
+        super().__init__(msg)
+        self.code = code
+
+
 def call(name, *args):
-    """Calls one entry point; a nonzero status raises RuntimeError with the library's message."""
+    """Calls one entry point; a nonzero status raises SpykerError (a RuntimeError) with the library's message."""
     lib = load()
     rc = getattr(lib, name)(*args)
     if rc != 0:
         msg = lib.spk_last_error().decode(errors="replace")
-        raise RuntimeError("%s failed (%d): %s" % (name, rc, msg))
+        raise SpykerError("%s failed (%d): %s" % (name, rc, msg), rc)
     return rc

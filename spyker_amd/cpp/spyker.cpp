@@ -1193,6 +1193,25 @@ void Conv::stdp(SparseTensor input, const Winners &winners)
           "Interface::STDP");
 }
 
+namespace
+{
+// The step as the reference runs it, one plain-path call per stage (layers
+// outside the fused step's domain).
+LayerStep plainStep(Conv &layer, Tensor input, F64 th, Expand2 radius, Size count, Expand2 pool, Expand2 poolStride,
+                    bool learn)
+{
+    LayerStep r;
+    Tensor pot = layer(input);
+    threshold(pot, th);
+    inhibit(pot);
+    r.winners = convwta(pot, radius, count);
+    r.output = fire(pot);
+    if (learn) layer.stdp(input, r.winners, r.output);
+    if (pool.get() != Shape{0, 0}) r.pooled = Spyker::pool(r.output, pool, poolStride);
+    return r;
+}
+}  // namespace
+
 LayerStep layerStep(Conv &layer, Tensor input, F64 threshold, Expand2 radius, Size count, Expand2 pool,
                     Expand2 poolStride, bool learn)
 {
@@ -1200,31 +1219,35 @@ LayerStep layerStep(Conv &layer, Tensor input, F64 threshold, Expand2 radius, Si
     onGpu({input, layer.kernel}, "Interface::LayerStep");
     REQUIRE(input.type() == Type::U8, "Interface::LayerStep", "Input spikes must be u8.");
     REQUIRE(layer.kernel.type() == Type::F32, "Interface::LayerStep", "The master weights must be f32.");
-    REQUIRE(layer.stride().get() == (Shape{1, 1}), "Interface::LayerStep", "Stride must be 1.");
     REQUIRE(!learn || !layer.stdpconfig.empty(), "Interface::LayerStep", "At least one STDP Config must be given.");
     checkItems("Radius", radius.get(), {0, 0});
     REQUIRE(count > 0, "Interface::LayerStep", "Winners count can't be less than 1.");
+    if (pool.get() != Shape{0, 0} && poolStride.get() == Shape{0, 0}) poolStride = pool;
+    // the fused FP16-MFMA step; a layer outside its domain (stride, step
+    // count, channels, K, threshold) runs on the plain path
+    if (layer.stride().get() != (Shape{1, 1}))
+        return plainStep(layer, input, threshold, radius, count, pool, poolStride, learn);
     const Expand4 pad = layer.pad();
     const Shape os = convShape(input.shape(), layer.kernel.shape(), {1, 1}, pad.get());
     LayerStep r;
     r.output = Tensor(input.device(), Type::U8, os);
     if (pool.get() != Shape{0, 0})
     {
-        if (poolStride.get() == Shape{0, 0}) poolStride = pool;
         checkItems("Kernel", pool.get(), {1, 1}), checkItems("Stride", poolStride.get(), {1, 1});
         r.pooled = Tensor(input.device(), Type::U8, poolShape(os, pool.get(), poolStride.get(), {0, 0, 0, 0}));
     }
     const Size n = input.shape(0);
     Tensor data(input.device(), Type::I64, {n, count, 5}), counts(input.device(), Type::I64, {n});
     const auto cfg = configs(layer.stdpconfig);
-    check(spk_stdp_layer_fused(input.data(), n, input.shape(1), input.shape(2), input.shape(3), input.shape(4),
-                               layer.kernel.data(), layer.kernel.shape(0), layer.kernel.shape(2),
-                               layer.kernel.shape(3), pad[0], pad[1], pad[2], pad[3], threshold, radius[0],
-                               radius[1], count, learn ? cfg.data() : nullptr, learn ? Size(cfg.size()) : 0,
-                               r.output.data(), r.pooled ? r.pooled.data() : nullptr, pool[0], pool[1],
-                               poolStride[0], poolStride[1], data.data<spk_winner>(), counts.data<int64_t>(), nullptr,
-                               nullptr),
-          "Interface::LayerStep");
+    const int32_t status = spk_stdp_layer_fused(
+        input.data(), n, input.shape(1), input.shape(2), input.shape(3), input.shape(4), layer.kernel.data(),
+        layer.kernel.shape(0), layer.kernel.shape(2), layer.kernel.shape(3), pad[0], pad[1], pad[2], pad[3], threshold,
+        radius[0], radius[1], count, learn ? cfg.data() : nullptr, learn ? Size(cfg.size()) : 0, r.output.data(),
+        r.pooled ? r.pooled.data() : nullptr, pool[0], pool[1], poolStride[0], poolStride[1],
+        data.data<spk_winner>(), counts.data<int64_t>(), nullptr, nullptr);
+    if (status == SPK_ERR_UNSUPPORTED)
+        return plainStep(layer, input, threshold, radius, count, pool, poolStride, learn);
+    check(status, "Interface::LayerStep");
     r.winners = download(data, counts);
     return r;
 }

@@ -36,6 +36,8 @@ void set_last_error(const std::string &msg);
     } while (0)
 
 #define SPK_ARG(cond, msg) SPK_REQUIRE(cond, SPK_ERR_ARGUMENT, msg)
+// outside what a device path implements (a caller may route the call elsewhere)
+#define SPK_UNSUP(cond, msg) SPK_REQUIRE(cond, SPK_ERR_UNSUPPORTED, msg)
 
 #define SPK_HIP(expr)                                                                                       \
     do                                                                                                      \

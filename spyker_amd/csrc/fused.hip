@@ -883,9 +883,9 @@ void stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, int64_t
         SPK_ARG(n >= 1 && t >= 1 && cin >= 1 && h >= 1 && w >= 1 && cout >= 2 && kh >= 1 && kw >= 1,
                 "Fused layer: invalid shapes.");
         SPK_ARG(pt >= 0 && pl >= 0 && pb >= 0 && pr >= 0, "Pad items are incorrect.");
-        SPK_ARG(t <= 32, "Fused layer: at most 32 time steps.");
-        SPK_ARG(cout <= 32 * MAX_WAVES, "Fused layer: at most " << 32 * MAX_WAVES << " output channels.");
-        SPK_ARG(threshold >= 0.0, "Fused layer: the conv threshold must be >= 0.");
+        SPK_UNSUP(t <= 32, "Fused layer: at most 32 time steps.");
+        SPK_UNSUP(cout <= 32 * MAX_WAVES, "Fused layer: at most " << 32 * MAX_WAVES << " output channels.");
+        SPK_UNSUP(threshold >= 0.0, "Fused layer: the conv threshold must be >= 0.");
         SPK_ARG(ry >= 0 && rx >= 0 && count >= 1, "Fused layer: invalid convwta parameters.");
         SPK_ARG(winners && counts, "Fused layer: winner buffers are required.");
         const int64_t ho = h + pt + pb - kh + 1, wo = w + pl + pr - kw + 1;
@@ -919,8 +919,8 @@ void stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, int64_t
         // potentials come out of the MFMAs times 2^-24 (spike bytes enter as
         // FP16 subnormals); the threshold is scaled to match (exact)
         g.th = float(threshold) * (1.0f / TWO_POW_24);
-        SPK_ARG(threshold_scales(threshold), "Fused layer: the threshold is too small to scale exactly.");
-        SPK_ARG(g.steps <= MAX_STEPS, "Fused layer: Cin*Kh*Kw too large (K <= 512 after padding).");
+        SPK_UNSUP(threshold_scales(threshold), "Fused layer: the threshold is too small to scale exactly.");
+        SPK_UNSUP(g.steps <= MAX_STEPS, "Fused layer: Cin*Kh*Kw too large (K <= 512 after padding).");
         // stage + any/argmax partials (two buffers each) + the per-pixel map of kept channels and fire masks
         const int64_t lds = stage_bytes + 3 * 2 * MAX_WAVES * COLS * 4 + ho * wo * 8;
         if (lds > 163840 || ho * wo > 8192)

@@ -937,11 +937,11 @@ LayerPlan *make_plan(int64_t n, const int32_t *shapes, int64_t cin, int64_t cout
     SPK_ARG(n <= 65535, "Fused layer: at most 65535 instances per plan.");
     SPK_ARG(cin >= 1 && cout >= 2 && kh >= 1 && kw >= 1, "Fused layer: invalid shapes.");
     SPK_ARG(pt >= 0 && pl >= 0 && pb >= 0 && pr >= 0, "Pad items are incorrect.");
-    SPK_ARG(cout <= 256, "Fused layer: at most 256 output channels.");
-    SPK_ARG(threshold >= 0.0, "Fused layer: the conv threshold must be >= 0.");
-    SPK_ARG(threshold_scales(threshold), "Fused layer: the threshold is too small to scale exactly.");
+    SPK_UNSUP(cout <= 256, "Fused layer: at most 256 output channels.");
+    SPK_UNSUP(threshold >= 0.0, "Fused layer: the conv threshold must be >= 0.");
+    SPK_UNSUP(threshold_scales(threshold), "Fused layer: the threshold is too small to scale exactly.");
     SPK_ARG(ry >= 0 && rx >= 0 && count >= 1, "Fused layer: invalid convwta parameters.");
-    SPK_ARG(kh + RW + kw <= 64, "Fused layer: kernel too large for the region path.");
+    SPK_UNSUP(kh + RW + kw <= 64, "Fused layer: kernel too large for the region path.");
     auto P = std::make_unique<LayerPlan>();
     SPK_HIP(hipGetDevice(&P->device));
     P->n = n, P->cin = int32_t(cin), P->cout = int32_t(cout), P->kh = int32_t(kh), P->kw = int32_t(kw);
@@ -951,7 +951,7 @@ LayerPlan *make_plan(int64_t n, const int32_t *shapes, int64_t cin, int64_t cout
     if (P->pool)
     {
         SPK_ARG(pkh >= 1 && pkw >= 1 && psh >= 1 && psw >= 1, "Pool: kernel and stride must be positive.");
-        SPK_ARG(pkh * pkw <= 16, "Fused layer: pool windows of at most 16 pixels.");
+        SPK_UNSUP(pkh * pkw <= 16, "Fused layer: pool windows of at most 16 pixels.");
         P->pkh = int32_t(pkh), P->pkw = int32_t(pkw), P->psh = int32_t(psh), P->psw = int32_t(psw);
     }
     const int64_t cin_pad = use_pair(cin, kh, kw) ? 4 : cdiv(cin, 8) * 8;
@@ -960,11 +960,12 @@ LayerPlan *make_plan(int64_t n, const int32_t *shapes, int64_t cin, int64_t cout
     for (int64_t i = 0; i < n; ++i)
     {
         const int64_t t = shapes[3 * i], h = shapes[3 * i + 1], w = shapes[3 * i + 2];
-        SPK_ARG(t >= 1 && t <= 32, "Fused layer: every instance needs 1 to 32 time steps.");
+        SPK_ARG(t >= 1, "Fused layer: every instance needs at least one time step.");
+        SPK_UNSUP(t <= 32, "Fused layer: at most 32 time steps per instance.");
         SPK_ARG(h >= 1 && w >= 1, "Fused layer: invalid instance shape.");
         const int64_t ho = h + pt + pb - kh + 1, wo = w + pl + pr - kw + 1;
         SPK_ARG(ho >= 1 && wo >= 1, "Input and kernel shapes are not compatible.");
-        SPK_ARG(ho * wo <= 65536, "Fused layer: at most 65536 output pixels per instance.");
+        SPK_UNSUP(ho * wo <= 65536, "Fused layer: at most 65536 output pixels per instance.");
         int64_t pho = 0, pwo = 0;
         if (P->pool)
         {
@@ -1011,9 +1012,9 @@ LayerPlan *make_plan(int64_t n, const int32_t *shapes, int64_t cin, int64_t cout
         P->configs.push_back(std::move(rc));
         if (!tune) break;  // without tuning: the smallest region that fits
     }
-    SPK_ARG(!P->configs.empty(), "Fused layer: the region stage does not fit the 160 KiB LDS (T * Cin too large).");
+    SPK_UNSUP(!P->configs.empty(), "Fused layer: the region stage does not fit the 160 KiB LDS (T * Cin too large).");
     const RegionGeom &g = P->configs[0]->g;
-    SPK_ARG(g.steps <= MAX_STEPS, "Fused layer: Cin*Kh*Kw too large (K <= 512 after padding).");
+    SPK_UNSUP(g.steps <= MAX_STEPS, "Fused layer: Cin*Kh*Kw too large (K <= 512 after padding).");
     int cus = 0;
     SPK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, P->device));
     int64_t min_items = g.items;

@@ -209,3 +209,37 @@ def test_fused_graph_replay(dev):
         assert torch.equal(out, e[0]) and torch.equal(pooled, e[1])
         assert torch.equal(wn.data, e[2]) and torch.equal(wn.counts, e[3])
         assert torch.equal(conv.kernel, e[4])
+
+
+@pytest.mark.parametrize("case", [
+    # n, t, cin, h, w, cout, k, stride, pad, th: each outside the fused step's domain
+    (3, 10, 4, 20, 20, 16, 3, 2, (1, 1, 1, 1), 3.0),     # stride 2
+    (2, 40, 4, 12, 12, 16, 3, 1, (1, 1, 1, 1), 3.0),     # 40 time steps
+    (2, 8, 4, 10, 10, 300, 3, 1, (1, 1, 1, 1), 3.0),     # 300 output channels
+    (2, 6, 64, 8, 8, 32, 3, 1, (1, 1, 1, 1), 20.0),      # Cin*Kh*Kw = 576
+    (2, 8, 4, 12, 12, 16, 3, 1, (1, 1, 1, 1), -0.5),     # a negative threshold
+])
+def test_layer_step_falls_back_to_the_plain_path(dev, orc, case):
+    """Outside the fused step's domain, layer_step runs the FP32 plain path,
+    which matches the oracle's step bit for bit."""
+    n, t, cin, h, w, cout, k, stride, pad, th = case
+    rng = np.random.default_rng(19)
+    x = spikes(rng, n, t, cin, h, w, 0.5)
+    kern = np.clip(rng.normal(0.8, 0.05, (cout, cin, k, k)), 0, 1).astype(np.float32)
+    conv = spk.Conv(cin, cout, k, stride=stride, pad=pad, device=dev)
+    conv.kernel = T(kern, dev)
+    conv.stdpconfig = [spk.STDPConfig(0.004, -0.003)]
+    ho, wo = (h + pad[0] + pad[2] - k) // stride + 1, (w + pad[1] + pad[3] - k) // stride + 1
+    pots = torch.empty((n, t, cout, ho, wo), dtype=torch.float32, device=dev)
+    out, wn, pooled = spk.layer_step(conv, T(x, dev), th, 1, 4, 2, 2, potentials=pots)
+    pot = orc.conv(x.reshape(n * t, cin, h, w), kern, stride, pad).reshape(n, t, cout, ho, wo)
+    assert np.array_equal(H(pots), pot)
+    pot = orc.inhibit(orc.threshold(pot, th))
+    o_win, o_cnt = orc.convwta(pot, 1, 4)
+    o_out = orc.fire(pot)
+    o_k = orc.convstdp(x, kern, o_out, [(0.004, -0.003)], o_win, o_cnt, stride, pad)
+    assert o_cnt.sum() > 0
+    assert np.array_equal(H(out), o_out)
+    check_winners(wn, o_win, o_cnt)
+    assert np.array_equal(H(conv.kernel), o_k)
+    assert np.array_equal(H(pooled), orc.pool(o_out, 2, 2))
