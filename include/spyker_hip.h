@@ -82,9 +82,13 @@ typedef struct spk_stdp_config
 const char *spk_last_error(void);
 /* Library version as MAJOR*10000 + MINOR*100 + PATCH. */
 int32_t spk_version(void);
-/* Core::cuda_device_count / cuda_set_device / cuda_sync (src/spyker/impl.h:75-86). */
+/* Core::cuda_device_count / cuda_set_device / cuda_current_device / cuda_sync
+ * (src/spyker/impl.h:75-80), and cuda_memory_total / cuda_memory_free
+ * (impl.h:88-90) as one query of the current device. */
 int32_t spk_device_count(int32_t *count);
 int32_t spk_set_device(int32_t device);
+int32_t spk_current_device(int32_t *device);
+int32_t spk_memory_info(int64_t *free_bytes, int64_t *total_bytes);
 int32_t spk_synchronize(void *stream);
 
 /* Caching allocator, Core::cuda_alloc/dealloc (src/spyker/cuda/memory.cu:139-155)
@@ -111,10 +115,15 @@ int32_t spk_threshold(void *x, int32_t type, int64_t n, double threshold, double
 int32_t spk_rank_fire(const void *in, int32_t in_type, void *out, int32_t out_type, int64_t n, double threshold,
                       void *stream);
 
-/* Core::cuda_cast for one dense buffer (src/spyker/impl.h:160), used by
- * the host layer for Tensor::to(type).  Pairs: U8->F32, U8->F16, F32->F16,
- * F16->F32, F32->F64, F64->F32, F32->U8 (truncating). */
+/* Core::cuda_cast(Size size, Dyn input, Dyn output), src/spyker/impl.h:160:
+ * out[i] = (out type) in[i] for any two of the 11 element types, as
+ * cpu_cast's std::copy converts (src/spyker/cpu/memory.cpp:54-62); used for
+ * Tensor::to(type).  Float to integer truncates. */
 int32_t spk_cast(const void *in, int32_t in_type, void *out, int32_t out_type, int64_t n, void *stream);
+
+/* Core::cuda_fill(Size size, Dyn input, Scalar value), src/spyker/impl.h:165:
+ * x[i] = value converted to the element type (cpu/memory.cpp:64-70). */
+int32_t spk_fill(void *x, int32_t type, int64_t n, double value, void *stream);
 
 /* Core::cuda_pad(Dyn3 input, Dyn3 output, Len4 pad, Scalar value), impl.h:236:
  * planes of [h][w] into [h+pt+pb][w+pl+pr] filled with value.  Types: U8, F32. */

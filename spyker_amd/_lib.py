@@ -35,6 +35,8 @@ SIGNATURES = {
     "spk_version": [],
     "spk_device_count": [_p],
     "spk_set_device": [_i32],
+    "spk_current_device": [_p],
+    "spk_memory_info": [_p, _p],
     "spk_synchronize": [_p],
     "spk_alloc": [_p, _i64],
     "spk_dealloc": [_p],
@@ -45,6 +47,7 @@ SIGNATURES = {
     "spk_threshold": [_p, _i32, _i64, _f64, _f64, _p],
     "spk_rank_fire": [_p, _i32, _p, _i32, _i64, _f64, _p],
     "spk_cast": [_p, _i32, _p, _i32, _i64, _p],
+    "spk_fill": [_p, _i32, _i64, _f64, _p],
     "spk_pad": [_p, _p, _i32, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _f64, _p],
     "spk_rank_pool": [_p, _p, _i32, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _p],
     "spk_rank_inhibit": [_p, _i32, _i64, _i64, _i64, _i64, _f64, _p],

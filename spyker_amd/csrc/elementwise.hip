@@ -1,4 +1,4 @@
-// Elementwise stages: threshold, rank fire, cast, pad.
+// Elementwise stages: threshold, rank fire, cast, fill, pad.
 //
 // threshold  src/spyker/cpu/fire.cpp:41-57   (x <= th ? value : x)
 // rank fire  src/spyker/cpu/fire.cpp:67-82   (x > th)
@@ -99,6 +99,35 @@ void cast_t(const void *in, void *out, int64_t n, hipStream_t s)
     cast_kernel<I, O><<<flat_grid(n, 256), 256, 0, s>>>((const I *)in, (O *)out, n);
     SPK_LAUNCHED();
 }
+
+template <typename T>
+__global__ void fill_kernel(T *__restrict__ x, int64_t n, T value)
+{
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) x[i] = value;
+}
+
+// Calls f with a value of the element type `type` names (the reference's
+// IfType switch, src/spyker/base.h:583-776).
+template <typename F>
+void with_type(int32_t type, F &&f)
+{
+    switch (type)
+    {
+    case SPK_I8: return f(int8_t{});
+    case SPK_I16: return f(int16_t{});
+    case SPK_I32: return f(int32_t{});
+    case SPK_I64: return f(int64_t{});
+    case SPK_U8: return f(uint8_t{});
+    case SPK_U16: return f(uint16_t{});
+    case SPK_U32: return f(uint32_t{});
+    case SPK_U64: return f(uint64_t{});
+    case SPK_F16: return f(_Float16{});
+    case SPK_F32: return f(float{});
+    case SPK_F64: return f(double{});
+    default: SPK_ARG(false, "Data type " << type << " is not recognized.");
+    }
+}
 }  // namespace
 }  // namespace spk
 
@@ -143,15 +172,23 @@ int32_t spk_cast(const void *in, int32_t in_type, void *out, int32_t out_type, i
         SPK_ARG(n >= 0, "Input size can't be negative.");
         if (n == 0) return;
         hipStream_t s = S(stream);
-        if (in_type == SPK_U8 && out_type == SPK_F32) return cast_t<uint8_t, float>(in, out, n, s);
-        if (in_type == SPK_U8 && out_type == SPK_F16) return cast_t<uint8_t, _Float16>(in, out, n, s);
-        if (in_type == SPK_F32 && out_type == SPK_F16) return cast_t<float, _Float16>(in, out, n, s);
-        if (in_type == SPK_F16 && out_type == SPK_F32) return cast_t<_Float16, float>(in, out, n, s);
-        if (in_type == SPK_F32 && out_type == SPK_F64) return cast_t<float, double>(in, out, n, s);
-        if (in_type == SPK_F64 && out_type == SPK_F32) return cast_t<double, float>(in, out, n, s);
-        if (in_type == SPK_F32 && out_type == SPK_U8) return cast_t<float, uint8_t>(in, out, n, s);
-        SPK_REQUIRE(false, SPK_ERR_UNSUPPORTED,
-                    "Cast: data types " << type_name(in_type) << " -> " << type_name(out_type) << " are not supported.");
+        with_type(in_type, [&](auto a) {
+            with_type(out_type, [&](auto b) { cast_t<decltype(a), decltype(b)>(in, out, n, s); });
+        });
+    });
+}
+
+int32_t spk_fill(void *x, int32_t type, int64_t n, double value, void *stream)
+{
+    return guard([&] {
+        SPK_ARG(n >= 0, "Input size can't be negative.");
+        if (n == 0) return;
+        hipStream_t s = S(stream);
+        with_type(type, [&](auto a) {
+            using T = decltype(a);
+            fill_kernel<T><<<flat_grid(n, 256), 256, 0, s>>>((T *)x, n, T(value));
+            SPK_LAUNCHED();
+        });
     });
 }
 

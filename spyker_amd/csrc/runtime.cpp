@@ -182,6 +182,25 @@ int32_t spk_device_count(int32_t *count)
 
 int32_t spk_set_device(int32_t device) { return guard([&] { SPK_HIP(hipSetDevice(device)); }); }
 
+int32_t spk_current_device(int32_t *device)
+{
+    return guard([&] {
+        int d = 0;
+        SPK_HIP(hipGetDevice(&d));
+        *device = d;
+    });
+}
+
+int32_t spk_memory_info(int64_t *free_bytes, int64_t *total_bytes)
+{
+    return guard([&] {
+        size_t f = 0, t = 0;
+        SPK_HIP(hipMemGetInfo(&f, &t));
+        if (free_bytes) *free_bytes = int64_t(f);
+        if (total_bytes) *total_bytes = int64_t(t);
+    });
+}
+
 int32_t spk_synchronize(void *stream) { return guard([&] { SPK_HIP(hipStreamSynchronize(S(stream))); }); }
 
 int32_t spk_alloc(void **ptr, int64_t bytes)
