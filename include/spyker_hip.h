@@ -472,12 +472,16 @@ int32_t spk_layer_plan_destroy(spk_layer_plan *plan);
  * Candidate i has rows[i] output rows per region and covers the longest
  * instance's steps in chunks[i] passes of the FP16 windows.  *count gets the
  * number of candidates (at most capacity are written), *chosen the one runs
- * use.  spk_layer_plan_select fixes candidate index for every later run (a
- * caller that tuned once can restore its choice); a plan made with tune != 0
- * and never run has every candidate, one made with tune = 0 only the first. */
+ * use, *writes the write mode: 1 = the region kernel writes the spikes (and
+ * pooled spikes whose windows lie inside one group of its rows), 0 = separate
+ * writer kernels.  A plan made with tune != 0 times every candidate in both
+ * modes on its first run and keeps the fastest; otherwise it uses the first
+ * candidate and a mode chosen from the shape.  spk_layer_plan_select fixes
+ * candidate `index` and mode `writes` (-1 keeps the mode) for every later
+ * run, for a caller that tuned once and restores its choice. */
 int32_t spk_layer_plan_candidates(const spk_layer_plan *plan, int32_t *rows, int32_t *chunks, int32_t capacity,
-                                  int32_t *count, int32_t *chosen);
-int32_t spk_layer_plan_select(spk_layer_plan *plan, int32_t index);
+                                  int32_t *count, int32_t *chosen, int32_t *writes);
+int32_t spk_layer_plan_select(spk_layer_plan *plan, int32_t index, int32_t writes);
 
 
 /* HIP-event timing of the main kernel of spk_stdp_layer_fused, recorded on

@@ -101,8 +101,9 @@ class _Lib:
             self._f("rank_code_latency_f32")(_ptr(x), I64(n), I64(l), I64(time), _ptr(out))
         return out
 
-    def threshold(self, x, th, value=0.0):
-        x = np.array(x, dtype=np.float32, copy=True)
+    def threshold(self, x, th, value=0.0, inplace=False):
+        """inplace: x (a C-contiguous float32 array) is updated and returned."""
+        x = x if inplace else np.array(x, dtype=np.float32, copy=True)
         rc = self._f("threshold_f32")(_ptr(x), I64(x.size), F32(th), F32(value))
         if self.prefix == "ref_":
             self._check(rc, "threshold")
@@ -116,9 +117,9 @@ class _Lib:
             self._check(rc, "fire")
         return out
 
-    def inhibit(self, x, th=0.0):
-        """x: [N, T, C, H, W] float32 -> inhibited copy."""
-        x = np.array(x, dtype=np.float32, copy=True)
+    def inhibit(self, x, th=0.0, inplace=False):
+        """x: [N, T, C, H, W] float32 -> inhibited copy (or x itself, updated, when inplace)."""
+        x = x if inplace else np.array(x, dtype=np.float32, copy=True)
         n, t, c = x.shape[:3]
         hw = int(np.prod(x.shape[3:]))
         if self.prefix == "ref_":

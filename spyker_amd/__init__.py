@@ -814,18 +814,19 @@ class LayerPlan:
         return out, pooled, data, cnt
 
     def candidates(self):
-        """(rows per region, time chunks) of each candidate tiling, and the index runs use."""
+        """(rows per region, time chunks) of each candidate tiling, the index runs use, and the
+        write mode (1: the region kernel writes the outputs, 0: separate writer kernels)."""
         cap = 16
         rows, chunks = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
-        count, chosen = C.c_int32(0), C.c_int32(0)
+        count, chosen, writes = C.c_int32(0), C.c_int32(0), C.c_int32(0)
         call("spk_layer_plan_candidates", self._h, C.c_void_p(rows.ctypes.data), C.c_void_p(chunks.ctypes.data), cap,
-             C.byref(count), C.byref(chosen))
+             C.byref(count), C.byref(chosen), C.byref(writes))
         k = min(count.value, cap)
-        return list(zip(rows[:k].tolist(), chunks[:k].tolist())), chosen.value
+        return list(zip(rows[:k].tolist(), chunks[:k].tolist())), chosen.value, writes.value
 
-    def select(self, index: int) -> None:
-        """Runs use candidate ``index`` from now on (no tuning run)."""
-        call("spk_layer_plan_select", self._h, int(index))
+    def select(self, index: int, writes: int = -1) -> None:
+        """Runs use candidate ``index`` and write mode ``writes`` (-1 keeps it) from now on."""
+        call("spk_layer_plan_select", self._h, int(index), int(writes))
 
     def run(self, packed: torch.Tensor, learn: bool = True, buffers=None):
         """One step on a packed input; returns (spikes, pooled-or-None, Winners)."""

@@ -90,8 +90,8 @@ SIGNATURES = {
     "spk_layer_plan_create": [_p, _i64, _p] + [_i64] * 8 + [_f64] + [_i64] * 7 + [_i32],
     "spk_layer_plan_layout": [_p, _p, _p, _p, _p],
     "spk_layer_plan_run": [_p, _p, _p, _i32, _p, _i64, _p, _p, _p, _p, _p],
-    "spk_layer_plan_candidates": [_p, _p, _p, _i32, _p, _p],
-    "spk_layer_plan_select": [_p, _i32],
+    "spk_layer_plan_candidates": [_p, _p, _p, _i32, _p, _p, _p],
+    "spk_layer_plan_select": [_p, _i32, _i32],
     "spk_layer_plan_destroy": [_p],
     "spk_kernel_timer": [_i32],
     "spk_kernel_timer_read": [_p, _p],

@@ -3,8 +3,8 @@
 // step count (one persistent launch per layer).
 //
 // The semantics are those of the whole-sample kernel (fused.hip): conv ->
-// threshold -> inhibit -> fire summary per output pixel, then convwta, the
-// spike/pool writer and STDP on the FP32 masters, in the reference's batch
+// threshold -> inhibit -> fire summary per output pixel, the spikes and pooled
+// spikes, then convwta and STDP on the FP32 masters, in the reference's batch
 // order (src/spyker/opers.cpp:408-693).  What changes is the unit of work:
 //
 //   - a work item is (instance, region), a region being RH output rows x 16
@@ -22,7 +22,11 @@
 //     stage at the pixel's own step, the same bytes and the same MFMA
 //     sequence as the step loop, so they reproduce its potentials exactly;
 //   - the region also writes the first-spike map of the input pixels it owns
-//     (a partition of the input), for the STDP records.
+//     (a partition of the input), for the STDP records;
+//   - the spike block of a group of rows is zeroed by 16-byte non-temporal
+//     stores spread over its step loop (and the pooled block when the pool
+//     windows lie inside the group), and the group's end stores the kept
+//     channels' 1s over them: at most one 1 per pixel and step.
 //
 // A plan (spk_layer_plan_*) fixes the shapes once: instance and item tables
 // and every work buffer are allocated when it is made, so running it
@@ -41,6 +45,7 @@ namespace spk
 {
 namespace
 {
+using u32x4 = __attribute__((ext_vector_type(4))) uint32_t;
 constexpr int RW = 16;            // region width: one column tile per region row
 constexpr int REGION_WAVES = 8;   // waves per block
 constexpr int LDS_BYTES = 163840;
@@ -67,6 +72,17 @@ struct RegionGeom
     int32_t pair;                // cin <= 4: K groups of two taps x 4 channels, 8-byte window units
     int64_t items;
     float th;
+};
+
+// The step's outputs the region kernel writes itself: spikes [t][cout][ho][wo]
+// and, when every pool window lies inside one group of region rows
+// (non-overlapping windows, nct a multiple of pkh, 16 a multiple of pkw),
+// pooled spikes [t][cout][pho][pwo].  Either pointer may be null.
+struct RegionOut
+{
+    uint8_t *out;
+    uint8_t *pooled;
+    int32_t pkh, pkw;
 };
 
 // cin <= 4: K groups of two taps x 4 channels when that needs at most 8 K-steps.
@@ -98,14 +114,14 @@ __global__ void prep_fragments_pair(const float *__restrict__ k, half8 *__restri
 // LDS bytes past the stage and windows: any/argmax partials, step table, tap table.
 inline int64_t region_fixed(int32_t waves, int32_t nct, int32_t steps)
 {
-    return int64_t(waves) * COLS * 12 + int64_t(nct) * COLS * 4 + int64_t(steps) * 4 * 16;
+    return int64_t(waves) * COLS * 12 + int64_t(nct) * COLS * 12 + int64_t(steps) * 4 * 16;
 }
 
 template <int S, bool DIAG, bool PAIR>
 __global__ void __launch_bounds__(64 * REGION_WAVES)
     fused_region(const uint8_t *__restrict__ in, const half8 *__restrict__ frag, const RegionInst *__restrict__ inst,
                  const int2 *__restrict__ items, Summary *__restrict__ summary, float *__restrict__ pot_out,
-                 uint8_t *__restrict__ fs_out, RegionGeom g)
+                 uint8_t *__restrict__ fs_out, RegionOut ro, RegionGeom g)
 {
     extern __shared__ __align__(16) uint8_t lds[];
     uint8_t *stage = lds;                                                          // [t][sh][sw][cin_pad]
@@ -114,7 +130,9 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
     float *red_v = reinterpret_cast<float *>(any_s + g.waves * COLS);             // [waves][COLS]
     int32_t *red_i = reinterpret_cast<int32_t *>(red_v + g.waves * COLS);         // [waves][COLS]
     int32_t *tsel = red_i + g.waves * COLS;                                       // [nct][COLS]
-    int4 *gtab = reinterpret_cast<int4 *>(tsel + g.nct * COLS);                   // [4S] (dy, dx, byte, 0)
+    int32_t *sel_c = tsel + g.nct * COLS;                                         // [nct][COLS] kept channel
+    uint32_t *sel_m = reinterpret_cast<uint32_t *>(sel_c + g.nct * COLS);         // [nct][COLS] its steps
+    int4 *gtab = reinterpret_cast<int4 *>(sel_m + g.nct * COLS);                  // [4S] (dy, dx, byte, 0)
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, q = lane >> 4, j = lane & 15;
     const int32_t cgw = w % g.cg, ct = w / g.cg;
@@ -176,6 +194,20 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
     // blockDim-th pair and walks the (step, stage row) pairs
     const int32_t broles = g.sw * g.cin;
 
+    // output roles: each step of a group's step loop, thread z < cout * nct
+    // zeroes the 16 spike bytes of (channel, row) z, and the next cout *
+    // nct / pkh threads the pooled bytes of (channel, pooled row); the
+    // group's end stores the 1s over them.  A spike block is all zero but for
+    // at most one 1 per pixel and step (its kept channel), so the zero stores
+    // stream out between the MFMAs and only the 1s remain for the end.
+    const int32_t zs = ro.out ? g.cout * g.nct : 0;
+    const int32_t prows = ro.pooled ? g.nct / ro.pkh : 0, zp = g.cout * prows;
+    const int32_t zrole = int32_t(threadIdx.x);
+    const bool zspk = zrole < zs, zpool = zrole >= zs && zrole < zs + zp;
+    const int32_t zc = zspk ? zrole / g.nct : (zrole - zs) / max(prows, 1);
+    const int32_t zr = zspk ? zrole % g.nct : (zrole - zs) % max(prows, 1);
+    const int32_t pbytes = RW / max(ro.pkw, 1);  // pooled bytes of a region row
+
     // blocks of one XCD (dispatched round-robin over 8) take consecutive
     // items, so neighbouring regions share their halo in that XCD's L2
     const int64_t nb = gridDim.x;
@@ -189,6 +221,10 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
         const int32_t sy0 = oy0 - g.pt, sx0 = (ox0 - g.pl) & ~3;  // stage origin (input coordinates)
         const int32_t xoff = ox0 - g.pl - sx0;
         const uint8_t *src = in + I.in_off;
+        uint8_t *const out_i = ro.out ? ro.out + I.out_off : nullptr;
+        uint8_t *const pool_i = ro.pooled ? ro.pooled + I.pool_off : nullptr;
+        const bool spk16 = out_i && I.wo % 16 == 0 && (reinterpret_cast<uintptr_t>(out_i) & 15) == 0;
+        const bool pool8 = pool_i && pbytes == 8 && I.pwo % 8 == 0 && (reinterpret_cast<uintptr_t>(pool_i) & 7) == 0;
         __syncthreads();  // the previous item's stage and windows are free
 
         // stage the input rectangle [sy0, sy0 + sh) x [sx0, sx0 + sw), all steps
@@ -329,25 +365,81 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
                 // the chunk's steps without barriers: the loads of step k+2 and
                 // the MFMAs of step k+1 are issued before the update of step k
                 // (loads past the chunk re-read its last step, unused)
-                half8 b0[S], b1[S];
+                // B operands: up to 8 K-steps two whole steps of buffers
+                // (b0, b1), loaded two steps ahead of their use; longer chains
+                // (a whole step of buffers spills at 13 K-steps) a ring of R
+                // registers, operand s of a step in slot s % R, loaded as soon
+                // as the slot's previous reader has issued, R MFMA pairs or
+                // more ahead of its use
+                constexpr bool RING = S > 8;
+                constexpr int R = RING ? 4 : S;
+                half8 b0[S], b1[RING ? 1 : S];
                 float4v c0, c1, n0, n1;
-                auto load = [&](half8 (&b)[S], const uint8_t *base) {
+                auto load1 = [&](half8 &dst, int s, const uint8_t *base) {
                     if (PAIR)
-#pragma unroll
-                        for (int s = 0; s < S; ++s)
-                        {
-                            const uint2 lo = *reinterpret_cast<const uint2 *>(base + offs[s]);
-                            const uint2 hi = *reinterpret_cast<const uint2 *>(base + offs[s] + 8);
-                            b[s] = __builtin_bit_cast(half8, make_uint4(lo.x, lo.y, hi.x, hi.y));
-                        }
+                    {
+                        const uint2 lo = *reinterpret_cast<const uint2 *>(base + offs[s]);
+                        const uint2 hi = *reinterpret_cast<const uint2 *>(base + offs[s] + 8);
+                        dst = __builtin_bit_cast(half8, make_uint4(lo.x, lo.y, hi.x, hi.y));
+                    }
                     else
-                        load_b(b, base, offs);
+                        dst = *reinterpret_cast<const half8 *>(base + offs[s]);
                 };
-                load(b0, wb);
-                mfma_chain(a, b0, c0, c1);
-                load(b0, wb + min(1, tc - 1) * win_step);
+                auto load = [&](half8(&b)[S], const uint8_t *base) {
+#pragma unroll
+                    for (int s = 0; s < S; ++s) load1(b[s], s, base);
+                };
+                // the ring's chain: the MFMAs of the step at `cur` into (u0,
+                // u1), loading ahead (past the step's operands: the step at `next`)
+                auto chain = [&](float4v &u0, float4v &u1, const uint8_t *cur, const uint8_t *next) {
+                    u0 = float4v{0, 0, 0, 0}, u1 = float4v{0, 0, 0, 0};
+#pragma unroll
+                    for (int s = 0; s < S; ++s)
+                    {
+                        u0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0][s], b0[s % R], u0, 0, 0, 0);
+                        u1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1][s], b0[s % R], u1, 0, 0, 0);
+                        // slot s % R is next read by operand s + R of this
+                        // step, or, past the step's last reader of the slot,
+                        // by operand s % R of the next step
+                        if (s + R < S)
+                            load1(b0[s % R], s + R, cur);
+                        else
+                            load1(b0[s % R], s % R, next);
+                    }
+                };
+                auto step_base = [&](int32_t k) { return wb + min(k, tc - 1) * win_step; };
+                if (RING)
+                {
+#pragma unroll
+                    for (int s = 0; s < R; ++s) load1(b0[s], s, wb);
+                    chain(c0, c1, step_base(0), step_base(1));
+                }
+                else
+                {
+                    load(b0, wb);
+                    mfma_chain(a, b0, c0, c1);
+                    load(b0, step_base(1));
+                }
                 auto update = [&](int32_t k, const float4v &u0, const float4v &u1) {
                     step_update(u0, u1, g.th, mask, mxb);
+                    const int64_t st = t0 + k;
+                    if (zspk && oy0 + grp0 + zr < I.ho)
+                    {
+                        uint8_t *d = out_i + ((st * g.cout + zc) * I.ho + oy0 + grp0 + zr) * I.wo + ox0;
+                        if (spk16)
+                            *reinterpret_cast<u32x4 *>(d) = u32x4{0u, 0u, 0u, 0u};
+                        else
+                            for (int32_t x = 0; x < RW && ox0 + x < I.wo; ++x) d[x] = 0;
+                    }
+                    if (zpool && (oy0 + grp0) / ro.pkh + zr < I.pho)
+                    {
+                        const int32_t px0 = ox0 / ro.pkw;
+                        uint8_t *d = pool_i + ((st * g.cout + zc) * I.pho + (oy0 + grp0) / ro.pkh + zr) * I.pwo + px0;
+                        if (pool8 && px0 + 8 <= I.pwo)
+                            *reinterpret_cast<uint64_t *>(d) = 0;
+                        else
+                            for (int32_t x = 0; x < pbytes && px0 + x < I.pwo; ++x) d[x] = 0;
+                    }
                     if (DIAG && pvalid)
 #pragma unroll
                         for (int r = 0; r < 2; ++r)
@@ -360,15 +452,30 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
                                             oy * I.wo + ox] = (r == 0 ? u0[i] : u1[i]) * TWO_POW_24;
                             }
                 };
-                for (int32_t k = 0; k < tc; k += 2)
-                {
-                    load(b1, wb + min(k + 2, tc - 1) * win_step);
-                    if (k + 1 < tc) mfma_chain(a, b0, n0, n1);
-                    update(k, c0, c1);
-                    load(b0, wb + min(k + 3, tc - 1) * win_step);
-                    if (k + 2 < tc) mfma_chain(a, b1, c0, c1);
-                    if (k + 1 < tc) update(k + 1, n0, n1);
-                }
+                if (!RING)
+                    // the loads of step k+2 and the MFMAs of step k+1 are
+                    // issued before step k's update, which reads step k's
+                    // results (loads past the chunk re-read its last step)
+                    for (int32_t k = 0; k < tc; k += 2)
+                    {
+                        if constexpr (!RING)
+                        {
+                            load(b1, step_base(k + 2));
+                            if (k + 1 < tc) mfma_chain(a, b0, n0, n1);
+                            update(k, c0, c1);
+                            load(b0, step_base(k + 3));
+                            if (k + 2 < tc) mfma_chain(a, b1, c0, c1);
+                            if (k + 1 < tc) update(k + 1, n0, n1);
+                        }
+                    }
+                else
+                    // long chains: one accumulator pair (the other wave of the
+                    // SIMD covers the wait for the last MFMA's result)
+                    for (int32_t k = 0; k < tc; ++k)
+                    {
+                        update(k, c0, c1);
+                        if (k + 1 < tc) chain(c0, c1, step_base(k + 1), step_base(k + 2));
+                    }
                 if (t0 + tc < I.t) __syncthreads();  // windows free for the next chunk
             }
 
@@ -386,24 +493,36 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
             __syncthreads();
             // the lane's B operands for pixel (oy, ox) at step st, from the
             // stage: the bytes the step loop's window held for that step
-            auto gather = [&](half8 (&b)[S], int32_t st) {
+            auto gather1 = [&](int s, int32_t st) {
+                const int4 te = gtab[4 * s + q];
+                const int32_t iy = oy + te.x, ix = ox + te.y;
+                uint2 raw = make_uint2(0, 0);
+                const bool row = st >= 0 && iy >= 0 && iy < I.h;
+                const uint8_t *sp = stage + ((st * g.sh + iy - sy0) * g.sw + ix - sx0) * g.cin_pad + te.z;
+                if (PAIR)
+                {
+                    // the two taps' 4 channels, each 0 outside the input
+                    if (row && ix >= 0 && ix < I.w) raw.x = *reinterpret_cast<const uint32_t *>(sp);
+                    if (row && ix + 1 >= 0 && ix + 1 < I.w) raw.y = *reinterpret_cast<const uint32_t *>(sp + 4);
+                }
+                else if (row && ix >= 0 && ix < I.w)
+                    raw = *reinterpret_cast<const uint2 *>(sp);
+                return __builtin_bit_cast(half8, subnormals(raw));
+            };
+            // the step loop's MFMA sequence on the gathered operands at step
+            // st, gathered GR K-steps ahead of their use
+            auto gchain = [&](int32_t st, float4v &u0, float4v &u1) {
+                constexpr int GR = S < 4 ? S : 4;
+                half8 gb[GR];
+#pragma unroll
+                for (int s = 0; s < GR; ++s) gb[s] = gather1(s, st);
+                u0 = float4v{0, 0, 0, 0}, u1 = float4v{0, 0, 0, 0};
 #pragma unroll
                 for (int s = 0; s < S; ++s)
                 {
-                    const int4 te = gtab[4 * s + q];
-                    const int32_t iy = oy + te.x, ix = ox + te.y;
-                    uint2 raw = make_uint2(0, 0);
-                    const bool row = st >= 0 && iy >= 0 && iy < I.h;
-                    const uint8_t *sp = stage + ((st * g.sh + iy - sy0) * g.sw + ix - sx0) * g.cin_pad + te.z;
-                    if (PAIR)
-                    {
-                        // the two taps' 4 channels, each 0 outside the input
-                        if (row && ix >= 0 && ix < I.w) raw.x = *reinterpret_cast<const uint32_t *>(sp);
-                        if (row && ix + 1 >= 0 && ix + 1 < I.w) raw.y = *reinterpret_cast<const uint32_t *>(sp + 4);
-                    }
-                    else if (row && ix >= 0 && ix < I.w)
-                        raw = *reinterpret_cast<const uint2 *>(sp);
-                    b[s] = __builtin_bit_cast(half8, subnormals(raw));
+                    u0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[0][s], gb[s % GR], u0, 0, 0, 0);
+                    u1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[1][s], gb[s % GR], u1, 0, 0, 0);
+                    if (s + GR < S) gb[s % GR] = gather1(s + GR, st);
                 }
             };
             int32_t ts;
@@ -417,9 +536,7 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
             // first maximum of the thresholded values (inhibit.cpp:64-85)
             float4v g0, g1;
             {
-                half8 b[S];
-                gather(b, ts);
-                mfma_chain(a, b, g0, g1);
+                gchain(ts, g0, g1);
                 float bv = -1.0f;
                 int32_t bi = INT32_MAX;
 #pragma unroll
@@ -460,21 +577,38 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
             float first = 0.0f;
             if (__builtin_amdgcn_ballot_w64(own && om != 0))
             {
-                half8 b[S];
-                gather(b, tsel[ct * COLS + j]);
-                mfma_chain(a, b, g0, g1);
+                gchain(tsel[ct * COLS + j], g0, g1);
 #pragma unroll
                 for (int r = 0; r < 2; ++r)
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
                         if (((cs >> 4) & 1) == r && (cs & 3) == i) first = r == 0 ? g0[i] : g1[i];
             }
-            if (own && pvalid)
+            if (own)
             {
                 const uint32_t fwd = om ? __builtin_bitreverse32(om) >> (32 - I.t) : 0u;
-                summary[I.sum_off + int64_t(oy) * I.wo + ox] =
-                    Summary{cs, fwd, om ? first * TWO_POW_24 : 0.0f,
-                            om ? __builtin_bit_cast(float, oxb) * TWO_POW_24 : 0.0f};
+                if (pvalid)
+                    summary[I.sum_off + int64_t(oy) * I.wo + ox] =
+                        Summary{cs, fwd, om ? first * TWO_POW_24 : 0.0f,
+                                om ? __builtin_bit_cast(float, oxb) * TWO_POW_24 : 0.0f};
+                sel_c[ct * COLS + j] = cs, sel_m[ct * COLS + j] = pvalid ? fwd : 0u;
+            }
+            if (zs + zp > 0)
+            {
+                // the group's 1s: pixel (row, column) at step t in its kept
+                // channel's plane and pooled window, over the zeros (the
+                // barrier orders every zero store of the block before them)
+                __syncthreads();
+                const int32_t per_row = COLS * I.t;
+                for (int32_t e = threadIdx.x; e < g.nct * per_row; e += blockDim.x)
+                {
+                    const int32_t r = e / per_row, px = (e % per_row) / I.t, st = (e % per_row) % I.t;
+                    if (!((sel_m[r * COLS + px] >> st) & 1u)) continue;
+                    const int32_t c = sel_c[r * COLS + px], py = oy0 + grp0 + r, pxx = ox0 + px;
+                    if (out_i) out_i[((int64_t(st) * g.cout + c) * I.ho + py) * I.wo + pxx] = 1;
+                    if (pool_i && py / ro.pkh < I.pho && pxx / ro.pkw < I.pwo)
+                        pool_i[((int64_t(st) * g.cout + c) * I.pho + py / ro.pkh) * I.pwo + pxx / ro.pkw] = 1;
+                }
             }
         }
     }
@@ -551,7 +685,8 @@ __device__ __forceinline__ void store_planes(uint8_t *d, int64_t plane, int32_t 
     }
 }
 
-// Spikes [t][cout][ho][wo] of every instance from the summaries.  Grid
+// Spikes [t][cout][ho][wo] of every instance from the summaries, when the
+// region kernel does not write them.  Grid
 // (chunk, instance): a thread owns 4 consecutive pixels (one when hw % 4 !=
 // 0 or the instance's spikes are not 4-aligned), reads their summaries once
 // and writes every step's channel planes, one 32-bit store per plane.  The
@@ -717,7 +852,7 @@ struct RegionConfig
 
 template <int S, bool DIAG, bool PAIR>
 void launch_region(const uint8_t *in, const half8 *frag, const RegionInst *inst, const RegionConfig &rc,
-                   Summary *summary, float *pot, uint8_t *fs, int grid, hipStream_t s)
+                   Summary *summary, float *pot, uint8_t *fs, const RegionOut &ro, int grid, hipStream_t s)
 {
     static bool configured = false;  // per instantiation; the attribute is per function
     if (!configured)
@@ -727,20 +862,20 @@ void launch_region(const uint8_t *in, const half8 *frag, const RegionInst *inst,
         configured = true;
     }
     fused_region<S, DIAG, PAIR><<<dim3(grid), dim3(64 * rc.g.waves), rc.lds, s>>>(in, frag, inst, rc.items,
-                                                                                summary, pot, fs, rc.g);
+                                                                                summary, pot, fs, ro, rc.g);
     SPK_LAUNCHED();
 }
 
 void dispatch_region(const uint8_t *in, const half8 *frag, const RegionInst *inst, const RegionConfig &rc,
-                     Summary *summary, float *pot, uint8_t *fs, int grid, hipStream_t s)
+                     Summary *summary, float *pot, uint8_t *fs, const RegionOut &ro, int grid, hipStream_t s)
 {
     if (rc.g.pair)
         switch (rc.g.steps)
         {
 #define CASE(k)                                                                              \
     case k:                                                                                  \
-        return pot ? launch_region<k, true, true>(in, frag, inst, rc, summary, pot, fs, grid, s) \
-                   : launch_region<k, false, true>(in, frag, inst, rc, summary, pot, fs, grid, s);
+        return pot ? launch_region<k, true, true>(in, frag, inst, rc, summary, pot, fs, ro, grid, s) \
+                   : launch_region<k, false, true>(in, frag, inst, rc, summary, pot, fs, ro, grid, s);
             CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE
         default: SPK_ARG(false, "Fused layer: K-steps out of range.");
@@ -749,8 +884,8 @@ void dispatch_region(const uint8_t *in, const half8 *frag, const RegionInst *ins
     {
 #define CASE(k)                                                                               \
     case k:                                                                                   \
-        return pot ? launch_region<k, true, false>(in, frag, inst, rc, summary, pot, fs, grid, s) \
-                   : launch_region<k, false, false>(in, frag, inst, rc, summary, pot, fs, grid, s);
+        return pot ? launch_region<k, true, false>(in, frag, inst, rc, summary, pot, fs, ro, grid, s) \
+                   : launch_region<k, false, false>(in, frag, inst, rc, summary, pot, fs, ro, grid, s);
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
         CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
 #undef CASE
@@ -792,6 +927,7 @@ struct LayerPlan
     std::vector<std::unique_ptr<RegionConfig>> configs;
     std::vector<std::unique_ptr<DeviceBuffer<int2>>> item_tables;
     int chosen = 0, grid = 0;
+    bool fuse_out = true;  // the region kernel writes the outputs (else the separate writers)
     DeviceBuffer<RegionInst> inst;
     DeviceBuffer<half8> frag;
     DeviceBuffer<Summary> summary;
@@ -803,7 +939,11 @@ struct LayerPlan
 
     void run(const uint8_t *in, float *kernel, const spk_stdp_config *cfg, uint8_t *out, uint8_t *pooled,
              spk_winner *winners, int64_t *counts, float *pot, hipStream_t s);
-    void choose(const uint8_t *in, hipStream_t s);
+    void choose(const uint8_t *in, uint8_t *out, uint8_t *pooled, hipStream_t s);
+    void write_step(const uint8_t *in, const RegionConfig &rc, bool fused, uint8_t *out, uint8_t *pooled, float *pot,
+                    uint8_t *fsp, hipStream_t s, bool writers);
+    void launch_writers(const RegionConfig &rc, bool fused, uint8_t *out, uint8_t *pooled, hipStream_t ws);
+    bool pool_in_kernel(const RegionConfig &rc, bool fused, const uint8_t *pooled) const;
 };
 
 namespace
@@ -847,27 +987,65 @@ bool region_config(const LayerPlan &P, int32_t m, bool fast, RegionGeom &g, size
 }
 }  // namespace
 
-void LayerPlan::choose(const uint8_t *in, hipStream_t s)
+void LayerPlan::choose(const uint8_t *in, uint8_t *out, uint8_t *pooled, hipStream_t s)
 {
-    // time each candidate's region kernel on this input and keep the fastest
-    // (the kernel only writes summaries and the first-spike map, which the
-    // run that follows rewrites)
-    std::vector<float> ms(configs.size(), 0.0f);
+    // time every candidate with the outputs written inside the region kernel
+    // and with the separate writers, on this input, and keep the fastest (the
+    // run that follows rewrites the summaries, map and outputs)
+    float best = INFINITY;
     hipEvent_t e0, e1;
     SPK_HIP(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));
     SPK_HIP(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence));
-    for (size_t c = 0; c < configs.size(); ++c)
-        for (int rep = 0; rep < 2; ++rep)  // the first launch configures and warms
+    for (int fused = 1; fused >= 0; --fused)
+        for (size_t c = 0; c < configs.size(); ++c)
         {
-            SPK_HIP(hipEventRecord(e0, s));
-            dispatch_region(in, frag.p, inst.p, *configs[c], summary.p, nullptr, fs.p, grid, s);
-            SPK_HIP(hipEventRecord(e1, s));
-            SPK_HIP(hipEventSynchronize(e1));
-            SPK_HIP(hipEventElapsedTime(&ms[c], e0, e1));
+            float ms = 0.0f;
+            for (int rep = 0; rep < 2; ++rep)  // the first launch configures and warms
+            {
+                SPK_HIP(hipEventRecord(e0, s));
+                write_step(in, *configs[c], fused != 0, out, pooled, nullptr, fs.p, s, true);
+                SPK_HIP(hipEventRecord(e1, s));
+                SPK_HIP(hipEventSynchronize(e1));
+                SPK_HIP(hipEventElapsedTime(&ms, e0, e1));
+            }
+            if (ms < best) best = ms, chosen = int(c), fuse_out = fused != 0;
         }
     (void)hipEventDestroy(e0), (void)hipEventDestroy(e1);
-    chosen = int(std::min_element(ms.begin(), ms.end()) - ms.begin());
     tuned = true;
+}
+
+// The region kernel with config rc and the step's outputs: written inside the
+// kernel (the pool only when its windows lie inside one group of region
+// rows; nct is the same for every candidate), or by the separate writers, on
+// `s` after it when `writers` (else the caller launches them on a side stream).
+bool LayerPlan::pool_in_kernel(const RegionConfig &rc, bool fused, const uint8_t *pooled) const
+{
+    return fused && pool && pooled && psh == pkh && psw == pkw && rc.g.nct % pkh == 0 && RW % pkw == 0;
+}
+
+void LayerPlan::write_step(const uint8_t *in, const RegionConfig &rc, bool fused, uint8_t *out, uint8_t *pooled,
+                           float *pot, uint8_t *fsp, hipStream_t s, bool writers)
+{
+    const RegionOut ro{fused ? out : nullptr, pool_in_kernel(rc, fused, pooled) ? pooled : nullptr, pkh, pkw};
+    dispatch_region(in, frag.p, inst.p, rc, summary.p, pot, fsp, ro, grid, s);
+    if (writers) launch_writers(rc, fused, out, pooled, s);  // else the caller launches them on its side stream
+}
+
+void LayerPlan::launch_writers(const RegionConfig &rc, bool fused, uint8_t *out, uint8_t *pooled, hipStream_t ws)
+{
+    const bool pool_in = pool_in_kernel(rc, fused, pooled);
+    if (out && !fused)
+    {
+        region_spikes<<<dim3(unsigned(cdiv(max_hw, WRITE_THREADS)), unsigned(n)), WRITE_THREADS, 0, ws>>>(
+            summary.p, inst.p, cout, out);
+        SPK_LAUNCHED();
+    }
+    if (pool && pooled && !pool_in)
+    {
+        region_pooled<<<dim3(unsigned(cdiv(max_phw, WRITE_THREADS)), unsigned(n)), WRITE_THREADS, 0, ws>>>(
+            summary.p, inst.p, cout, pooled, pkh, pkw, psh, psw);
+        SPK_LAUNCHED();
+    }
 }
 
 void LayerPlan::run(const uint8_t *in, float *kernel, const spk_stdp_config *cfg, uint8_t *out, uint8_t *pooled,
@@ -882,7 +1060,7 @@ void LayerPlan::run(const uint8_t *in, float *kernel, const spk_stdp_config *cfg
         prep_fragments<<<fgrid, 256, 0, s>>>(kernel, frag.p, g0.cout, g0.cin, g0.kh, g0.kw, g0.cin_pad, g0.groups,
                                              g0.steps, g0.cg * 2, 0);
     SPK_LAUNCHED();
-    if (tune && !tuned) choose(in, s);
+    if (tune && !tuned) choose(in, out, pooled, s);
     const bool learn = cfg != nullptr;
     KernelTimer &tm = timer();
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -892,27 +1070,18 @@ void LayerPlan::run(const uint8_t *in, float *kernel, const spk_stdp_config *cfg
         SPK_HIP(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence));
         SPK_HIP(hipEventRecord(e0, s));
     }
-    dispatch_region(in, frag.p, inst.p, *configs[chosen], summary.p, pot, learn ? fs.p : nullptr, grid, s);
+    const RegionConfig &rc = *configs[chosen];
+    write_step(in, rc, fuse_out, out, pooled, pot, learn ? fs.p : nullptr, s, false);
     if (tm.on)
     {
         SPK_HIP(hipEventRecord(e1, s));
         tm.events.emplace_back(e0, e1);
     }
-    // the writers need only the summaries: they run on the side stream
-    // beside convwta, the records and the STDP chains
-    SideStream side(s, out || (pool && pooled));
-    if (out)
-    {
-        region_spikes<<<dim3(unsigned(cdiv(max_hw, WRITE_THREADS)), unsigned(n)), WRITE_THREADS, 0, side.stream>>>(
-            summary.p, inst.p, cout, out);
-        SPK_LAUNCHED();
-    }
-    if (pool && pooled)
-    {
-        region_pooled<<<dim3(unsigned(cdiv(max_phw, WRITE_THREADS)), unsigned(n)), WRITE_THREADS, 0, side.stream>>>(
-            summary.p, inst.p, cout, pooled, pkh, pkw, psh, psw);
-        SPK_LAUNCHED();
-    }
+    // outputs the kernel does not write: the separate writers on the side
+    // stream, beside convwta, the records and the STDP chains
+    const bool sep = (out && !fuse_out) || (pool && pooled && !pool_in_kernel(rc, fuse_out, pooled));
+    SideStream side(s, sep);
+    if (sep) launch_writers(rc, fuse_out, out, pooled, side.stream);
     region_select<<<dim3(unsigned(n)), 1024, 0, s>>>(summary.p, inst.p, ry, rx, count, val.p, chan.p, winners, counts,
                                                      wkey.p);
     SPK_LAUNCHED();
@@ -1015,6 +1184,11 @@ LayerPlan *make_plan(int64_t n, const int32_t *shapes, int64_t cin, int64_t cout
     SPK_UNSUP(!P->configs.empty(), "Fused layer: the region stage does not fit the 160 KiB LDS (T * Cin too large).");
     const RegionGeom &g = P->configs[0]->g;
     SPK_UNSUP(g.steps <= MAX_STEPS, "Fused layer: Cin*Kh*Kw too large (K <= 512 after padding).");
+    // without tuning: outputs written inside the region kernel when a step's
+    // MFMA chain (2 * steps MFMAs of 16 cycles a wave) outlasts the issue of
+    // its 16-byte output segments (about one a cycle for cout * nct segments
+    // a block), else by the separate writers, whose stores coalesce
+    P->fuse_out = 32 * g.steps >= g.cout * g.nct;
     int cus = 0;
     SPK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, P->device));
     int64_t min_items = g.items;
@@ -1199,7 +1373,7 @@ int32_t spk_layer_plan_run(spk_layer_plan *handle, const void *in, void *kernel,
 }
 
 int32_t spk_layer_plan_candidates(const spk_layer_plan *handle, int32_t *rows, int32_t *chunks, int32_t capacity,
-                                  int32_t *count, int32_t *chosen)
+                                  int32_t *count, int32_t *chosen, int32_t *writes)
 {
     return guard([&] {
         SPK_ARG(handle, "Plan is null.");
@@ -1213,16 +1387,19 @@ int32_t spk_layer_plan_candidates(const spk_layer_plan *handle, int32_t *rows, i
         }
         if (count) *count = k;
         if (chosen) *chosen = plan->chosen;
+        if (writes) *writes = plan->fuse_out ? 1 : 0;
     });
 }
 
-int32_t spk_layer_plan_select(spk_layer_plan *handle, int32_t index)
+int32_t spk_layer_plan_select(spk_layer_plan *handle, int32_t index, int32_t writes)
 {
     return guard([&] {
         SPK_ARG(handle, "Plan is null.");
         LayerPlan *plan = reinterpret_cast<LayerPlan *>(handle);
         SPK_ARG(index >= 0 && index < int32_t(plan->configs.size()), "Plan candidate index out of range.");
+        SPK_ARG(writes >= -1 && writes <= 1, "Plan write mode must be -1, 0 or 1.");
         plan->chosen = index;
+        if (writes >= 0) plan->fuse_out = writes == 1;
         plan->tuned = true;
     });
 }

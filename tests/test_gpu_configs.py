@@ -113,17 +113,18 @@ def test_precision_variant_layer_exact(dev, orc):
 
     # through a plan, every candidate tiling
     plan = spk.LayerPlan(conv, [(C3_T, C3_S, C3_S)] * n, th, 5, 10, 2, 2, tune=True)
-    cands, _ = plan.candidates()
+    cands, _, _ = plan.candidates()
     assert len(cands) >= 2 and all(ch >= 2 for _, ch in cands), cands
     bufs = plan.buffers()
     for i in range(len(cands)):
-        conv.kernel.copy_(T(kern, dev))
-        plan.select(i)
-        out, pooled, wn = plan.run(xs.reshape(-1), buffers=bufs)
-        assert np.array_equal(H(out).reshape(o_out.shape), o_out), cands[i]
-        assert np.array_equal(H(pooled).reshape(o_pool.shape), o_pool), cands[i]
-        check_winners(wn, o_win, o_cnt)
-        assert np.array_equal(H(conv.kernel), o_k), cands[i]
+        for writes in (1, 0):  # outputs written by the region kernel, then by the separate writers
+            conv.kernel.copy_(T(kern, dev))
+            plan.select(i, writes)
+            out, pooled, wn = plan.run(xs.reshape(-1), buffers=bufs)
+            assert np.array_equal(H(out).reshape(o_out.shape), o_out), (cands[i], writes)
+            assert np.array_equal(H(pooled).reshape(o_pool.shape), o_pool), (cands[i], writes)
+            check_winners(wn, o_win, o_cnt)
+            assert np.array_equal(H(conv.kernel), o_k), (cands[i], writes)
 
 
 # -- the batching variant's extremes -----------------------------------------
@@ -161,7 +162,7 @@ def test_batching_variant_long_steps(dev, orc, shapes):
     conv.kernel = T(kern, dev)
     conv.stdpconfig = [spk.STDPConfig(0.004, -0.003)]
     plan = spk.LayerPlan(conv, shapes, th, radius, count, *pool, tune=True)
-    cands, _ = plan.candidates()
+    cands, _, _ = plan.candidates()
     assert max(ch for _, ch in cands) >= 2, cands
     packed = plan.pack(xd)
     # the oracle: every instance's forward pass on the original weights, then
@@ -175,14 +176,15 @@ def test_batching_variant_long_steps(dev, orc, shapes):
     assert sum(r[2].sum() for r in ref) > 0
     bufs = plan.buffers()
     for c in range(len(cands)):
-        conv.kernel.copy_(T(kern, dev))
-        plan.select(c)
-        out, pooled, wn = plan.run(packed, buffers=bufs)
-        for i, (o_out, o_win, o_cnt, o_pool) in enumerate(ref):
-            assert np.array_equal(H(plan.spikes(out, i)), o_out[0]), (cands[c], i)
-            assert np.array_equal(H(plan.pooled(pooled, i)), o_pool[0]), (cands[c], i)
-            check_winners(wn, o_win, o_cnt, i)
-        assert np.array_equal(H(conv.kernel), o_k), cands[c]
+        for writes in (1, 0):
+            conv.kernel.copy_(T(kern, dev))
+            plan.select(c, writes)
+            out, pooled, wn = plan.run(packed, buffers=bufs)
+            for i, (o_out, o_win, o_cnt, o_pool) in enumerate(ref):
+                assert np.array_equal(H(plan.spikes(out, i)), o_out[0]), (cands[c], writes, i)
+                assert np.array_equal(H(plan.pooled(pooled, i)), o_pool[0]), (cands[c], writes, i)
+                check_winners(wn, o_win, o_cnt, i)
+            assert np.array_equal(H(conv.kernel), o_k), (cands[c], writes)
 
 
 # -- precision on weights away from N(0.8, 0.05) -------------------------------

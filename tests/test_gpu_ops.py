@@ -286,3 +286,31 @@ def test_two_layer_forward_features(dev, orc):
     want = (t - orc.rank_gather(o)).reshape(n, -1)
     assert 0 < (want > 0).mean() < 1
     assert np.array_equal(H(feats), want)
+
+
+# spk_type ordinal -> numpy type (the reference's Type order, src/spyker/utils.h:96-117)
+ALL_TYPES = [np.int8, np.int16, np.int32, np.int64, np.uint8, np.uint16, np.uint32, np.uint64, np.float16,
+             np.float32, np.float64]
+
+
+def _raw(a, dev):
+    """A device byte buffer holding numpy array a."""
+    return torch.from_numpy(np.ascontiguousarray(a).view(np.uint8).copy()).to(dev)
+
+
+def test_cast_and_fill_every_type(dev):
+    """cuda_cast / cuda_fill for every element type (cpu/memory.cpp:54-70: std::copy's
+    conversion and std::fill), on values every type holds exactly."""
+    vals = np.array([0, 1, 2, 3, 7, 100, 120, 5, 64, 127, 9, 33, 17], np.float64)
+    s = torch.cuda.current_stream().cuda_stream
+    for i, ti in enumerate(ALL_TYPES):
+        src = _raw(vals.astype(ti), dev)
+        for o, to in enumerate(ALL_TYPES):
+            dst = torch.zeros(len(vals) * np.dtype(to).itemsize, dtype=torch.uint8, device=dev)
+            spk.call("spk_cast", spk._ptr(src), i, spk._ptr(dst), o, len(vals), s)
+            got = H(dst).view(to)
+            assert np.array_equal(got, vals.astype(ti).astype(to)), (ti, to)
+        buf = torch.zeros(1000 * np.dtype(ti).itemsize, dtype=torch.uint8, device=dev)
+        spk.call("spk_fill", spk._ptr(buf), i, 999, 42.0, s)
+        got = H(buf).view(ti)
+        assert (got[:999] == ti(42)).all() and got[999] == 0, ti
