@@ -42,13 +42,14 @@ FLOP_PER_IT = 2.0 * COUT * H * H * CIN * K * K          # dense time-unrolled co
 BYTES_PER_IT = CIN * H * H + COUT * H * H + COUT * 4 * 4  # U8 spikes in, spikes out, pooled out
 PEAK_FP32 = 157.3e12
 PEAK_FP16 = 2.5e15
+PEAK_INT8 = 5.0e15
 PEAK_HBM = 8.0e12
 # HBM bytes one full-batch fused_layer launch moves on this workload, from rocprofv3
 # FETCH_SIZE (x2.0, the factor a 512 MiB device copy reads as) + WRITE_SIZE (x1.0):
 # profiles/fused_layer_c2_traffic.txt.  Algorithmic: 90.3 MB input + 9.6 MB summaries/first-spike map
 # + 814.0 MB spikes and pooled spikes (the kernel writes them) = 914.0 MB; the spike lines that get 1s
 # are written twice (non-temporal zeros, then the 1s).
-TRAFFIC_FUSED_LAYER = 979.2e6
+TRAFFIC_FUSED_LAYER = {"fp16": 979.2e6, "int8": None}
 
 
 def make_inputs(dev, n=N, seed=2):
@@ -99,7 +100,7 @@ def timed(fn, steps, warmup):
 
 
 def precision(spk, x, k2, dev, samples=64):
-    """FP16 path vs the FP64 check conv and vs the FP32 plain step."""
+    """The fused path vs the FP64 check conv and vs the FP32 plain step."""
     import torch
     from spyker_amd import _lib
     xs = x[:samples].contiguous()
@@ -634,6 +635,8 @@ def main():
     ap.add_argument("--sparse-only", action="store_true", help="time only the sparse event-stream workload")
     ap.add_argument("--only", choices=["c3", "c4", "sparse", "front"], help="time only this extra workload")
     ap.add_argument("--no-extra", action="store_true", help="skip the precision and batching workloads")
+    ap.add_argument("--arith", choices=["int8", "fp16"], default="int8",
+                    help="the whole-sample kernel's arithmetic (int8 fixed point, the library default, or fp16)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (HBM counter calibration under rocprofv3)")
     a = ap.parse_args()
@@ -655,6 +658,7 @@ def main():
     import spyker_amd as spk
 
     dev = torch.device("cuda", 0)
+    spk.set_fused_arith(a.arith)
     if a.sparse_only:
         a.only = "sparse"
     if a.only:
@@ -678,6 +682,14 @@ def main():
     spk.call("spk_kernel_timer", 0)
     kern_s = total_ms.value / 1e3 / max(launches.value, 1)
     achieved = FLOP_PER_IT * N * T_STEPS / kern_s
+    if a.arith == "int8":
+        # spike bytes times 16-bit fixed-point weights, split into a high and a low int8 byte:
+        # two int8 MFMA products per multiply-add of the conv, I32 accumulation
+        dtype, peak, kname = "u8*i16-fixed (int8 mfma, i32 acc)", PEAK_INT8, "fused_layer8"
+        path = "fused int8-MFMA layer step (16-bit fixed-point weights), FP32 master weights"
+    else:
+        dtype, peak, kname = "f16-mfma/f32-acc", PEAK_FP16, "fused_layer"
+        path = "fused FP16-MFMA layer step, FP32 master weights"
 
     line = {
         "metric": "images*timesteps/s, conv+fire+wta+STDP layer, 1 GPU",
@@ -690,17 +702,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f16-mfma/f32-acc",
+        "dtype": dtype,
         "data": "synthetic",
         "config": {"workload": "layer 2 of the two-layer stroke-image STDP network: batch 1024 of 28x28 images, "
                                "T=15, conv 30->250 3x3 pad 1 on 14x14 spikes, threshold 10, inhibit, convwta r1 k8, "
                                "STDP, pool 3x3 s3",
-                   "path": "fused FP16-MFMA layer step, FP32 master weights"},
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP16 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP16, "traffic": TRAFFIC_FUSED_LAYER, "traffic_unit": "bytes/launch",
-                     "kernel": "fused_layer", "kernel_ms": kern_s * 1e3, "launches_timed": launches.value,
+                   "path": path},
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": TRAFFIC_FUSED_LAYER[a.arith], "traffic_unit": "bytes/launch",
+                     "kernel": kname, "kernel_ms": kern_s * 1e3, "launches_timed": launches.value,
                      "flop_per_launch": FLOP_PER_IT * N * T_STEPS},
     }
+    if a.arith == "int8":
+        line["roofline"]["note"] = ("achieved counts the conv's own multiply-adds; the int8 path issues two int8 "
+                                    "products for each (high and low weight byte), so its ceiling for this conv "
+                                    "is half the int8 peak, the FP16 peak")
+        line["roofline"]["frac_of_two_product_ceiling"] = achieved / (PEAK_INT8 / 2)
     if not a.no_plain:
         conv_p = spk.Conv(CIN, COUT, K, pad=1, device=dev)
         conv_p.kernel = torch.from_numpy(k2).to(dev)

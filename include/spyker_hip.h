@@ -418,6 +418,17 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
                              int64_t pkw, int64_t psh, int64_t psw, spk_winner *winners, int64_t *counts,
                              void *potentials, void *stream);
 
+/* The arithmetic of spk_stdp_layer_fused's whole-sample kernel (no reference
+ * counterpart): mode 0 (the default) runs the conv as int8 MFMAs on the spike
+ * bytes with the weights in 16-bit fixed point (w rounded to a multiple of
+ * 2^-e, e the largest with max|w| * 2^e <= 32639, chosen per call), whose
+ * potentials are exact sums of the rounded weights; shapes whose window rows
+ * need more than 8 K-steps of 64 bytes use FP16 operands instead.  Mode 1
+ * uses FP16 operands (weights rounded to FP16) everywhere.  Both accumulate
+ * exactly what they sum in their own format; samples too large for the LDS
+ * (the region kernel) always use FP16.  Process-wide. */
+int32_t spk_set_fused_arith(int32_t mode);
+
 /* spk_stdp_layer_fused for callers that cannot allocate device memory (code
  * under stream capture, tight serving loops, harnesses that forbid
  * hipMalloc).  No counterpart in the reference.  Same arguments and results

@@ -666,7 +666,7 @@ def layer_step(conv: "Conv", array: torch.Tensor, threshold: float, radius, coun
         pot = inhibit(threshold(conv(array), threshold)); winners = convwta(pot, radius, count)
         out = fire(pot); conv.stdp(array, winners, out); pooled = pool(out, pool_kernel, pool_stride)
 
-    On the fused FP16-MFMA path (spk_stdp_layer_fused) in a handful of
+    On the fused MFMA path (spk_stdp_layer_fused) in a handful of
     launches that never materialise the FP32 potentials; a layer outside that
     path's domain (stride other than 1, more than 32 steps or 256 output
     channels, Cin*Kh*Kw above 512, a negative threshold, ...) runs the same
@@ -714,6 +714,15 @@ def layer_step(conv: "Conv", array: torch.Tensor, threshold: float, radius, coun
             if e.code != _lib.ERR_UNSUPPORTED:
                 raise
     return _plain_layer_step(conv, a, float(threshold), radius, count, pool_kernel, pool_stride, learn, potentials)
+
+
+def set_fused_arith(mode: str) -> None:
+    """The fused step's whole-sample arithmetic: "int8" (the default: int8 MFMAs on the
+    spike bytes, 16-bit fixed-point weights) or "fp16" (FP16 operands)."""
+    modes = {"int8": 0, "fp16": 1}
+    if mode not in modes:
+        raise RuntimeError("Fused arithmetic must be 'int8' or 'fp16'.")
+    call("spk_set_fused_arith", modes[mode])
 
 
 def _plain_layer_step(conv, a, th, radius, count, pool_kernel, pool_stride, learn, potentials):

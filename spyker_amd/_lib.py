@@ -93,6 +93,7 @@ SIGNATURES = {
     "spk_layer_plan_candidates": [_p, _p, _p, _i32, _p, _p, _p],
     "spk_layer_plan_select": [_p, _i32, _i32],
     "spk_layer_plan_destroy": [_p],
+    "spk_set_fused_arith": [_i32],
     "spk_kernel_timer": [_i32],
     "spk_kernel_timer_read": [_p, _p],
     "spk_random_seed": [_i64],

@@ -45,6 +45,8 @@
 //     tile exchange the per-pixel partials between waves.
 #include <mutex>
 
+#include <atomic>
+
 #include "fused.hpp"
 
 namespace spk
@@ -59,6 +61,8 @@ struct FusedGeom
     int32_t sstep;        // bytes of one staged step (ph * pw * cin_pad)
     int32_t stage_fast;   // staging by quad transposes (hw % 4 == 0, aligned input, cin_pad/4 | block quads)
     float th;             // the conv threshold times 2^-24 (potentials are computed times 2^-24)
+    float thf;            // the conv threshold (the int8 kernel scales it to its fixed point)
+    int32_t upr, units;   // int8 kernel: 16-byte K-units per window row, and in all
     int32_t pkh, pkw, psh, psw, pho, pwo;  // max-pool of the spikes (pho = 0: none)
     int32_t zp;                            // bytes of a sample's pooled spikes each tile zeroes
 };
@@ -216,6 +220,143 @@ constexpr int layer_threads()
     return RT == 4 ? 256 : 512;
 }
 
+// Stages sample n of the [n][t][cin][h][w] input as [t][y][x][cin_pad] bytes
+// inside the zero border of the conv's padding (written once per kernel).
+__device__ __forceinline__ void stage_sample(const uint8_t *__restrict__ in, int32_t n, const FusedGeom &g,
+                                             uint8_t *stage)
+{
+    const int32_t hwi = g.h * g.w;
+    if (g.stage_fast)
+    {
+        // a lane loads one dword (4 pixels of channel c) of the
+        // [t][cin][h][w] input; the 4 lanes of a quad hold channels
+        // 4*c4 .. 4*c4+3 of the same 4 pixels and transpose the 4x4 bytes
+        // (two DPP exchanges + v_perm), after which lane l4 holds the 4
+        // channels of pixel l4: one dword store per lane
+        const int32_t D = g.cin_pad / 4, npq = hwi / 4, step = int32_t(blockDim.x) / 4 / D;
+        const int32_t Q = int32_t(threadIdx.x) >> 2, l4 = int32_t(threadIdx.x) & 3, c4 = Q % D;
+        const int32_t c = 4 * c4 + l4;
+        const uint32_t sel1 = (l4 & 1) ? 0x03070105u : 0x06020400u;
+        const uint32_t sel2 = (l4 & 2) ? 0x03020706u : 0x05040100u;
+        // raw buffer loads of the sample: 32-bit offsets, and offsets past
+        // its end (the walk's tail) read 0
+        const __amdgpu_buffer_rsrc_t src = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<uint8_t *>(in + int64_t(n) * g.t * g.cin * hwi), 0, g.t * g.cin * hwi, 0x00020000);
+        uint32_t *dst = reinterpret_cast<uint32_t *>(stage);
+        // lin = tt * npq + pq walks the (step, pixel quad) pairs by `step`
+        // (<= npq, so pq wraps at most once per move); off is the byte
+        // offset of the lane's dword, channel c of step tt.  U loads are
+        // in flight per lane before the first store.
+        int32_t lin = Q / D, pq = lin, off = (c * npq + pq) * 4;
+        const int32_t total = g.t * npq, wrap_add = (g.cin - 1) * npq * 4;
+        // the store side walks the same pairs as (tt, pixel row, column)
+        // of the lane's own pixel 4*pq + l4, moving 4*step pixels a time:
+        // dy rows and dx columns, one more row when the column wraps, and
+        // back h rows into the next step when the pixel passes h*w
+        int32_t sq = pq, st = 0, sy = (4 * pq + l4) / g.w, sx = (4 * pq + l4) % g.w;
+        const int32_t dy = 4 * step / g.w, dx = 4 * step % g.w;
+        constexpr int U = 16;
+        for (; lin < total; lin += U * step)
+        {
+            uint32_t v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+            {
+                v[u] = uint32_t(__builtin_amdgcn_raw_buffer_load_b32(src, off, 0, 0));
+                pq += step, off += step * 4;
+                const bool wrap = pq >= npq;
+                pq = wrap ? pq - npq : pq;
+                off = wrap ? off + wrap_add : off;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+            {
+                const uint32_t x = c < g.cin ? v[u] : 0u;
+                const uint32_t y = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(x), 0xB1, 0xF, 0xF, true));  // lane^1
+                const uint32_t r1 = __builtin_amdgcn_perm(y, x, sel1);
+                const uint32_t y2 = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(r1), 0x4E, 0xF, 0xF, true));  // lane^2
+                if (lin + u * step < total)
+                    dst[((st * g.ph + sy + g.pt) * g.pw + sx + g.pl) * D + c4] = __builtin_amdgcn_perm(y2, r1, sel2);
+                sq += step, sy += dy, sx += dx;
+                sy = sx >= g.w ? sy + 1 : sy;
+                sx = sx >= g.w ? sx - g.w : sx;
+                const bool wrap = sq >= npq;
+                sq = wrap ? sq - npq : sq;
+                st = wrap ? st + 1 : st;
+                sy = wrap ? sy - g.h : sy;
+            }
+        }
+    }
+    else
+    {
+        // any other shape: scatter the [t][cin][h][w] input byte by byte
+        // (the padded channels stay 0)
+        const uint8_t *src = in + int64_t(n) * g.t * g.cin * hwi;
+        const int32_t chw = g.cin * hwi, total = g.t * chw;
+        for (int32_t i = int32_t(threadIdx.x); i < total; i += int32_t(blockDim.x))
+        {
+            const int32_t tt = i / chw, r = i - tt * chw, c = r / hwi, pix = r - c * hwi, yy = pix / g.w;
+            const int32_t xx = pix - yy * g.w;
+            stage[((tt * g.ph + yy + g.pt) * g.pw + xx + g.pl) * g.cin_pad + c] = src[i];
+        }
+    }
+}
+
+// The first spiking step of every staged input byte of sample n (t when it
+// never spikes), 8 channels of one pixel per thread: [n][h*w][cin_pad].
+__device__ __forceinline__ void first_spike_map(const uint8_t *stage, int32_t n, const FusedGeom &g,
+                                                uint8_t *__restrict__ fs_out)
+{
+    const int32_t hwi = g.h * g.w, cg = g.cin_pad / 8;
+    const uint32_t tb = uint32_t(g.t) * 0x01010101u;
+    for (int32_t u = threadIdx.x; u < hwi * cg; u += blockDim.x)
+    {
+        const int32_t pix = u / cg, c8 = u - pix * cg, py = pix / g.w, px = pix - py * g.w;
+        const uint8_t *sp = stage + ((py + g.pt) * g.pw + px + g.pl) * g.cin_pad + c8 * 8;
+        uint32_t lo = tb, hi = tb, seen_lo = 0, seen_hi = 0;
+        for (int32_t tt = 0; tt < g.t; ++tt)
+        {
+            const uint2 v = *reinterpret_cast<const uint2 *>(sp + tt * g.sstep);
+            first_nonzero(lo, seen_lo, v.x, tt);
+            first_nonzero(hi, seen_hi, v.y, tt);
+        }
+        *reinterpret_cast<uint2 *>(fs_out + (int64_t(n) * hwi * cg + u) * 8) = make_uint2(lo, hi);
+    }
+}
+
+// each sample's 1s: its kept channel at every step it fires, in its
+// spike plane and in every pool window that holds the pixel (the max of a
+// window of 0/1 bytes is 1 exactly when one of its pixels is; pixels of a
+// window store the same value).  One (step, pixel) pair a thread, from
+// smap; the sample's zeros were stored before the last barrier.
+__device__ __forceinline__ void scatter_sample(int32_t m, const uint2 *smap, uint8_t *__restrict__ out,
+                                               uint8_t *__restrict__ pooled, const FusedGeom &g)
+{
+    const int32_t hw = g.ho * g.wo;
+    const int64_t s_bytes = int64_t(g.t) * g.cout * hw, p_bytes = int64_t(g.t) * g.cout * g.pho * g.pwo;
+    uint8_t *s_out = out ? out + int64_t(m) * s_bytes : nullptr;
+    uint8_t *p_out = pooled ? pooled + int64_t(m) * p_bytes : nullptr;
+    int32_t st = int32_t(threadIdx.x) / hw, q2 = int32_t(threadIdx.x) - st * hw;
+    const int32_t bst = int32_t(blockDim.x) / hw, bq = int32_t(blockDim.x) - bst * hw;
+    for (; st < g.t; st += bst, q2 += bq)
+    {
+        if (q2 >= hw) q2 -= hw, ++st;
+        if (st >= g.t) break;
+        const uint2 e = smap[q2];
+        if (!((e.y >> st) & 1u)) continue;
+        if (s_out) s_out[(int64_t(st) * g.cout + e.x) * hw + q2] = 1;
+        if (p_out)
+        {
+            const int32_t y = q2 / g.wo, x = q2 % g.wo;
+            const int32_t py0 = y >= g.pkh ? (y - g.pkh) / g.psh + 1 : 0, py1 = min(y / max(g.psh, 1), g.pho - 1);
+            const int32_t px0 = x >= g.pkw ? (x - g.pkw) / g.psw + 1 : 0, px1 = min(x / max(g.psw, 1), g.pwo - 1);
+            for (int32_t py = py0; py <= py1; ++py)
+                for (int32_t px = px0; px <= px1; ++px)
+                    p_out[((int64_t(st) * g.cout + e.x) * g.pho + py) * g.pwo + px] = 1;
+        }
+    }
+}
+
 template <int RT, int S, bool PAIRED, bool DIAG>
 __global__ void __launch_bounds__(layer_threads<RT>())
     fused_layer(const uint8_t *__restrict__ in, const half8 *__restrict__ frag, Summary *__restrict__ summary,
@@ -249,138 +390,17 @@ __global__ void __launch_bounds__(layer_threads<RT>())
     for (int32_t i = threadIdx.x; i < g.stage_bytes / 16; i += blockDim.x)
         reinterpret_cast<uint4 *>(stage)[i] = make_uint4(0, 0, 0, 0);
 
-    // each sample's 1s: its kept channel at every step it fires, in its
-    // spike plane and in every pool window that holds the pixel (the max of a
-    // window of 0/1 bytes is 1 exactly when one of its pixels is; pixels of a
-    // window store the same value).  One (step, pixel) pair a thread, from
-    // smap; the sample's zeros were stored before the last barrier.
-    auto scatter = [&](int32_t m) {
-        const int64_t s_bytes = int64_t(g.t) * g.cout * hw, p_bytes = int64_t(g.t) * g.cout * g.pho * g.pwo;
-        uint8_t *s_out = out ? out + int64_t(m) * s_bytes : nullptr;
-        uint8_t *p_out = pooled ? pooled + int64_t(m) * p_bytes : nullptr;
-        int32_t st = int32_t(threadIdx.x) / hw, q2 = int32_t(threadIdx.x) - st * hw;
-        const int32_t bst = int32_t(blockDim.x) / hw, bq = int32_t(blockDim.x) - bst * hw;
-        for (; st < g.t; st += bst, q2 += bq)
-        {
-            if (q2 >= hw) q2 -= hw, ++st;
-            if (st >= g.t) break;
-            const uint2 e = smap[q2];
-            if (!((e.y >> st) & 1u)) continue;
-            if (s_out) s_out[(int64_t(st) * g.cout + e.x) * hw + q2] = 1;
-            if (p_out)
-            {
-                const int32_t y = q2 / g.wo, x = q2 % g.wo;
-                const int32_t py0 = y >= g.pkh ? (y - g.pkh) / g.psh + 1 : 0, py1 = min(y / max(g.psh, 1), g.pho - 1);
-                const int32_t px0 = x >= g.pkw ? (x - g.pkw) / g.psw + 1 : 0, px1 = min(x / max(g.psw, 1), g.pwo - 1);
-                for (int32_t py = py0; py <= py1; ++py)
-                    for (int32_t px = px0; px <= px1; ++px)
-                        p_out[((int64_t(st) * g.cout + e.x) * g.pho + py) * g.pwo + px] = 1;
-            }
-        }
-    };
     int32_t prev = -1;  // the sample whose 1s are not stored yet
     for (int32_t n = blockIdx.x; n < g.n; n += gridDim.x)
     {
         __syncthreads();
-        // stage the sample as [t][y][x][cin_pad] bytes inside the zero border
-        if (g.stage_fast)
-        {
-            // a lane loads one dword (4 pixels of channel c) of the
-            // [t][cin][h][w] input; the 4 lanes of a quad hold channels
-            // 4*c4 .. 4*c4+3 of the same 4 pixels and transpose the 4x4 bytes
-            // (two DPP exchanges + v_perm), after which lane l4 holds the 4
-            // channels of pixel l4: one dword store per lane
-            const int32_t D = g.cin_pad / 4, npq = hwi / 4, step = int32_t(blockDim.x) / 4 / D;
-            const int32_t Q = int32_t(threadIdx.x) >> 2, l4 = int32_t(threadIdx.x) & 3, c4 = Q % D;
-            const int32_t c = 4 * c4 + l4;
-            const uint32_t sel1 = (l4 & 1) ? 0x03070105u : 0x06020400u;
-            const uint32_t sel2 = (l4 & 2) ? 0x03020706u : 0x05040100u;
-            // raw buffer loads of the sample: 32-bit offsets, and offsets past
-            // its end (the walk's tail) read 0
-            const __amdgpu_buffer_rsrc_t src = __builtin_amdgcn_make_buffer_rsrc(
-                const_cast<uint8_t *>(in + int64_t(n) * g.t * g.cin * hwi), 0, g.t * g.cin * hwi, 0x00020000);
-            uint32_t *dst = reinterpret_cast<uint32_t *>(stage);
-            // lin = tt * npq + pq walks the (step, pixel quad) pairs by `step`
-            // (<= npq, so pq wraps at most once per move); off is the byte
-            // offset of the lane's dword, channel c of step tt.  U loads are
-            // in flight per lane before the first store.
-            int32_t lin = Q / D, pq = lin, off = (c * npq + pq) * 4;
-            const int32_t total = g.t * npq, wrap_add = (g.cin - 1) * npq * 4;
-            // the store side walks the same pairs as (tt, pixel row, column)
-            // of the lane's own pixel 4*pq + l4, moving 4*step pixels a time:
-            // dy rows and dx columns, one more row when the column wraps, and
-            // back h rows into the next step when the pixel passes h*w
-            int32_t sq = pq, st = 0, sy = (4 * pq + l4) / g.w, sx = (4 * pq + l4) % g.w;
-            const int32_t dy = 4 * step / g.w, dx = 4 * step % g.w;
-            constexpr int U = 16;
-            for (; lin < total; lin += U * step)
-            {
-                uint32_t v[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u)
-                {
-                    v[u] = uint32_t(__builtin_amdgcn_raw_buffer_load_b32(src, off, 0, 0));
-                    pq += step, off += step * 4;
-                    const bool wrap = pq >= npq;
-                    pq = wrap ? pq - npq : pq;
-                    off = wrap ? off + wrap_add : off;
-                }
-#pragma unroll
-                for (int u = 0; u < U; ++u)
-                {
-                    const uint32_t x = c < g.cin ? v[u] : 0u;
-                    const uint32_t y = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(x), 0xB1, 0xF, 0xF, true));  // lane^1
-                    const uint32_t r1 = __builtin_amdgcn_perm(y, x, sel1);
-                    const uint32_t y2 = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(r1), 0x4E, 0xF, 0xF, true));  // lane^2
-                    if (lin + u * step < total)
-                        dst[((st * g.ph + sy + g.pt) * g.pw + sx + g.pl) * D + c4] = __builtin_amdgcn_perm(y2, r1, sel2);
-                    sq += step, sy += dy, sx += dx;
-                    sy = sx >= g.w ? sy + 1 : sy;
-                    sx = sx >= g.w ? sx - g.w : sx;
-                    const bool wrap = sq >= npq;
-                    sq = wrap ? sq - npq : sq;
-                    st = wrap ? st + 1 : st;
-                    sy = wrap ? sy - g.h : sy;
-                }
-            }
-        }
-        else
-        {
-            // any other shape: scatter the [t][cin][h][w] input byte by byte
-            // (the padded channels stay 0)
-            const uint8_t *src = in + int64_t(n) * g.t * g.cin * hwi;
-            const int32_t chw = g.cin * hwi, total = g.t * chw;
-            for (int32_t i = int32_t(threadIdx.x); i < total; i += int32_t(blockDim.x))
-            {
-                const int32_t tt = i / chw, r = i - tt * chw, c = r / hwi, pix = r - c * hwi, yy = pix / g.w;
-                const int32_t xx = pix - yy * g.w;
-                stage[((tt * g.ph + yy + g.pt) * g.pw + xx + g.pl) * g.cin_pad + c] = src[i];
-            }
-        }
+        stage_sample(in, n, g, stage);
         __syncthreads();
-        if (fs_out)
-        {
-            // first spiking step of every staged input byte (t when it never
-            // spikes), 8 channels of one pixel per thread: [n][h*w][cin_pad]
-            const uint32_t tb = uint32_t(g.t) * 0x01010101u;
-            for (int32_t u = threadIdx.x; u < hwi * cg; u += blockDim.x)
-            {
-                const int32_t pix = u / cg, c8 = u - pix * cg, py = pix / g.w, px = pix - py * g.w;
-                const uint8_t *sp = stage + ((py + g.pt) * g.pw + px + g.pl) * g.cin_pad + c8 * 8;
-                uint32_t lo = tb, hi = tb, seen_lo = 0, seen_hi = 0;
-                for (int32_t tt = 0; tt < g.t; ++tt)
-                {
-                    const uint2 v = *reinterpret_cast<const uint2 *>(sp + tt * g.sstep);
-                    first_nonzero(lo, seen_lo, v.x, tt);
-                    first_nonzero(hi, seen_hi, v.y, tt);
-                }
-                *reinterpret_cast<uint2 *>(fs_out + (int64_t(n) * hwi * cg + u) * 8) = make_uint2(lo, hi);
-            }
-        }
+        if (fs_out) first_spike_map(stage, n, g, fs_out);
         // the previous sample's 1s, after this sample's staging so that its
         // loads do not wait behind these stores; smap still holds that
         // sample's map (the first tile's end rewrites it, after two barriers)
-        if (prev >= 0 && (out || pooled)) scatter(prev);
+        if (prev >= 0 && (out || pooled)) scatter_sample(prev, smap, out, pooled, g);
 
         // the sample's spike and pooled blocks: all 0 but the kept channels'
         // 1s.  The aligned 16-byte units of the spike block are zeroed a few
@@ -567,7 +587,402 @@ __global__ void __launch_bounds__(layer_threads<RT>())
     if (prev >= 0 && (out || pooled))
     {
         __syncthreads();
-        scatter(prev);
+        scatter_sample(prev, smap, out, pooled, g);
+    }
+}
+
+// -- the fixed-point int8 form of the whole-sample kernel --------------------
+//
+// Spikes are 0 or 1, so the stage's bytes can feed v_mfma_i32_16x16x64_i8 as
+// they are, with no conversion.  The weights become fixed point: w_q =
+// rint(w * 2^e) with the largest e that keeps every |w_q| <= 32639 (chosen per
+// call from the kernel's max |w|), split into two int8 planes w_q = 256 * hi +
+// lo (lo in [-128, 127]); a potential is the exact integer 256 * (spikes . hi)
+// + (spikes . lo) times 2^-e.  Against FP16 weights this rounds each weight
+// to 2^-e / 2 <= max|w| / 65278 (FP16: 2^-11 of the weight), and any sum of
+// weights exact at that scale (the dyadic test weights) is exact.  The fire
+// test "pot > th" is the integer test pot_q > floor(th * 2^e), exact.
+//
+// K-units: a window row (kw pixels x cin_pad channels) is contiguous in the
+// stage; unit u covers 16 bytes of row u / upr at byte 16 * (u % upr), bytes
+// past the row (into the next pixel) carrying zero weights.  K-step s: lane
+// group q takes unit 4s + q (the i8 MFMA's lane l holds K bytes 16 * (l / 16)
+// .. +16 of row or column l % 16).
+typedef int32_t int4v __attribute__((ext_vector_type(4)));
+
+constexpr int MAX_STEPS8 = 8;  // K-steps of 64 bytes -> 512
+
+// The fixed-point scale: the largest e with max|w| * 2^e <= 32639.
+__device__ __forceinline__ int32_t fixed_exponent(float m)
+{
+    if (!(m > 0.0f)) return 0;
+    if (!(m <= 3.0e38f)) return -120;
+    int x = 0;
+    (void)frexpf(m, &x);  // m = f * 2^x, f in [0.5, 1)
+    int32_t e = 15 - x;
+    if (ldexpf(m, e) > 32639.0f) --e;
+    return e < -120 ? -120 : (e > 120 ? 120 : e);
+}
+
+// pot_q > floor(th * 2^e) <=> pot_q * 2^-e > th for integers pot_q; thresholds
+// past 2^30 (no potential reaches them) clamp there.
+__device__ __forceinline__ int32_t threshold_fixed(float th, int32_t e)
+{
+    const float x = floorf(ldexpf(th, e));
+    return x >= 1073741824.0f ? 1073741824 : int32_t(x);
+}
+
+__device__ __forceinline__ int32_t unit_off(const FusedGeom &g, int32_t u)
+{
+    const int32_t uu = u < g.units ? u : 0;
+    return (uu / g.upr) * g.pw * g.cin_pad + (uu % g.upr) * 16;
+}
+
+// [row tile][K step][plane][lane] int8 fragments of the FP32 masters (plane 0
+// = hi, 1 = lo), and hdr[0] = e.  Every block reduces max|w| itself, so one
+// launch needs no second pass.
+__global__ void __launch_bounds__(1024)
+    prep_fragments8(const float *__restrict__ k, int4v *__restrict__ frag, int32_t *__restrict__ hdr, int32_t cout,
+                    int32_t cin, int32_t kh, int32_t kw, int32_t cin_pad, int32_t upr, int32_t units, int32_t steps,
+                    int32_t row_tiles)
+{
+    __shared__ float part[16];
+    const int64_t nw = int64_t(cout) * cin * kh * kw;
+    // four independent maxima over 16-byte loads keep many loads in flight
+    float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f;
+    int64_t i0 = 0;
+    if ((reinterpret_cast<uintptr_t>(k) & 15) == 0)
+    {
+        const float4 *k4 = reinterpret_cast<const float4 *>(k);
+        const int64_t n4 = nw / 4;
+#pragma unroll 4
+        for (int64_t i = threadIdx.x; i < n4; i += blockDim.x)
+        {
+            const float4 v = k4[i];
+            m0 = fmaxf(m0, fabsf(v.x)), m1 = fmaxf(m1, fabsf(v.y)), m2 = fmaxf(m2, fabsf(v.z)),
+            m3 = fmaxf(m3, fabsf(v.w));
+        }
+        i0 = n4 * 4;
+    }
+    for (int64_t i = i0 + threadIdx.x; i < nw; i += blockDim.x) m0 = fmaxf(m0, fabsf(k[i]));
+    float m = wave_max(fmaxf(fmaxf(m0, m1), fmaxf(m2, m3)));
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    m = part[0];
+    for (int i = 1; i < int(blockDim.x >> 6); ++i) m = fmaxf(m, part[i]);
+    const int32_t e = fixed_exponent(m);
+    if (blockIdx.x == 0 && threadIdx.x == 0) hdr[0] = e;
+    const int32_t total = row_tiles * steps * 2 * 64;
+    for (int32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += gridDim.x * blockDim.x)
+    {
+        const int32_t lane = idx % 64, p = (idx / 64) % 2, s = (idx / 128) % steps, rt = idx / (128 * steps);
+        const int32_t o = rt * 16 + lane % 16, u = 4 * s + lane / 16, ky = u / upr;
+        uint32_t word[4] = {0, 0, 0, 0};
+        for (int b = 0; b < 16; ++b)
+        {
+            const int32_t pos = (u % upr) * 16 + b, kx = pos / cin_pad, c = pos % cin_pad;
+            float x = 0.0f;
+            if (o < cout && u < units && kx < kw && c < cin) x = k[((int64_t(o) * cin + c) * kh + ky) * kw + kx];
+            const int32_t q = int32_t(rintf(ldexpf(x, e)));
+            const int32_t hi = (q + 128) >> 8, lo = q - hi * 256;
+            word[b / 4] |= uint32_t((p == 0 ? hi : lo) & 0xFF) << (8 * (b % 4));
+        }
+        frag[idx] = int4v{int32_t(word[0]), int32_t(word[1]), int32_t(word[2]), int32_t(word[3])};
+    }
+}
+
+template <bool WIDE>
+__device__ __forceinline__ int4v load16(const uint8_t *p)
+{
+    if (WIDE) return *reinterpret_cast<const int4v *>(p);
+    const uint2 lo = *reinterpret_cast<const uint2 *>(p), hi = *reinterpret_cast<const uint2 *>(p + 8);
+    return int4v{int32_t(lo.x), int32_t(lo.y), int32_t(hi.x), int32_t(hi.y)};
+}
+
+template <int S8, bool WIDE>
+__device__ __forceinline__ void load_units(int4v (&b)[S8], const uint8_t *base, const int32_t (&off)[S8])
+{
+#pragma unroll
+    for (int s = 0; s < S8; ++s) b[s] = load16<WIDE>(base + off[s]);
+}
+
+// One step's MFMAs: hi and lo planes into their own accumulators.
+template <int RT, int S8>
+__device__ __forceinline__ void chain8(const int4v (&a)[RT][S8][2], const int4v (&b)[S8], int4v (&ch)[RT],
+                                       int4v (&cl)[RT])
+{
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) ch[rt] = int4v{0, 0, 0, 0}, cl[rt] = int4v{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < S8; ++s)
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+        {
+            ch[rt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[rt][s][0], b[s], ch[rt], 0, 0, 0);
+            cl[rt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[rt][s][1], b[s], cl[rt], 0, 0, 0);
+        }
+}
+
+// value_update on an exact fixed-point potential: the sign of th - v is "v >
+// th" (no overflow: |v| < 2^24, th <= 2^30).
+__device__ __forceinline__ void value_update8(int32_t hi, int32_t lo, int32_t th, uint32_t &mask, int32_t &mx)
+{
+    const int32_t v = hi * 256 + lo;
+    mask = __builtin_amdgcn_alignbit(mask, uint32_t(th - v), 31);
+    mx = max(mx, v);
+}
+
+template <int RT>
+__device__ __forceinline__ void update_all8(const int4v (&uh)[RT], const int4v (&ul)[RT], int32_t th,
+                                            uint32_t (&mask)[RT][4], int32_t (&mx)[RT][4])
+{
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) value_update8(uh[rt][i], ul[rt][i], th, mask[rt][i], mx[rt][i]);
+}
+
+// step_iter of the int8 kernel: the MFMA pairs of step k+1 on `rin`, step
+// k's value updates spread between them, and the loads of step k+2's units
+// into `rout` right after the MFMAs that read the same unit of `rin`.
+template <int RT, int S8, bool WIDE>
+__device__ __forceinline__ void step_iter8(const int4v (&a)[RT][S8][2], const int4v (&rin)[S8], int4v (&ch)[RT],
+                                           int4v (&cl)[RT], const int4v (&uh)[RT], const int4v (&ul)[RT], int32_t th,
+                                           uint32_t (&mask)[RT][4], int32_t (&mx)[RT][4], int4v (&rout)[S8],
+                                           const uint8_t *next, const int32_t (&off)[S8])
+{
+    constexpr int V = 4 * RT, SPAN = S8 * RT - RT;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) ch[rt] = int4v{0, 0, 0, 0}, cl[rt] = int4v{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < S8; ++s)
+    {
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+        {
+            ch[rt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[rt][s][0], rin[s], ch[rt], 0, 0, 0);
+            cl[rt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[rt][s][1], rin[s], cl[rt], 0, 0, 0);
+#pragma unroll
+            for (int v = 0; v < V; ++v)
+                if (SPAN > 0 && RT + (v * SPAN) / V == s * RT + rt)
+                    value_update8(uh[v / 4][v % 4], ul[v / 4][v % 4], th, mask[v / 4][v % 4], mx[v / 4][v % 4]);
+        }
+        rout[s] = load16<WIDE>(next + off[s]);
+    }
+    if (SPAN <= 0) update_all8(uh, ul, th, mask, mx);
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// The whole-sample kernel in fixed-point int8 arithmetic: fused_layer's
+// structure (persistent blocks over samples, staging, 16-pixel tiles, the
+// steps without barriers, the two gathered tile-end passes, the in-loop zero
+// stores and the scatter of 1s) with i8 MFMAs on the raw stage bytes.
+template <int RT, int S8, bool WIDE, bool DIAG>
+__global__ void __launch_bounds__(64 * MAX_WAVES)
+    fused_layer8(const uint8_t *__restrict__ in, const int4v *__restrict__ frag, const int32_t *__restrict__ hdr,
+                 Summary *__restrict__ summary, float *__restrict__ pot_out, uint8_t *__restrict__ fs_out,
+                 uint8_t *__restrict__ out, uint8_t *__restrict__ pooled, FusedGeom g)
+{
+    extern __shared__ __align__(16) uint8_t lds[];
+    uint8_t *stage = lds;                                                       // [t][ph][pw][cin_pad] U8
+    uint32_t *any_s = reinterpret_cast<uint32_t *>(lds + g.stage_bytes);        // [2][MAX_WAVES][COLS]
+    float *red_v = reinterpret_cast<float *>(any_s + 2 * MAX_WAVES * COLS);     // [2][MAX_WAVES][COLS]
+    int32_t *red_i = reinterpret_cast<int32_t *>(red_v + 2 * MAX_WAVES * COLS);  // [2][MAX_WAVES][COLS]
+    uint2 *smap = reinterpret_cast<uint2 *>(red_i + 2 * MAX_WAVES * COLS);       // [ho*wo] {c*, fire mask}
+
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, q = lane >> 4, j = lane & 15;
+    const int32_t hw = g.ho * g.wo;
+    const int32_t o0 = w * 16 * RT + 4 * q;  // the lane's first channel; value (r, i) is channel o0 + 16r + i
+
+    int4v a[RT][S8][2];
+#pragma unroll
+    for (int s = 0; s < S8; ++s)
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+            for (int p = 0; p < 2; ++p) a[r][s][p] = frag[(((w * RT + r) * S8 + s) * 2 + p) * 64 + lane];
+    const int32_t e = hdr[0];
+    __builtin_amdgcn_s_waitcnt(0);  // as in fused_layer: no pending fragment loads in the tile loop
+    const float scale = ldexpf(1.0f, -e);
+    const int32_t th = threshold_fixed(g.thf, e);
+    int32_t off[S8];
+#pragma unroll
+    for (int s = 0; s < S8; ++s) off[s] = unit_off(g, 4 * s + q);
+    for (int32_t i = threadIdx.x; i < g.stage_bytes / 16; i += blockDim.x)
+        reinterpret_cast<uint4 *>(stage)[i] = make_uint4(0, 0, 0, 0);
+
+    int32_t prev = -1;  // the sample whose 1s are not stored yet
+    for (int32_t n = blockIdx.x; n < g.n; n += gridDim.x)
+    {
+        __syncthreads();
+        stage_sample(in, n, g, stage);
+        __syncthreads();
+        if (fs_out) first_spike_map(stage, n, g, fs_out);
+        if (prev >= 0 && (out || pooled)) scatter_sample(prev, smap, out, pooled, g);
+
+        const int64_t s_bytes = int64_t(g.t) * g.cout * hw, p_bytes = int64_t(g.t) * g.cout * g.pho * g.pwo;
+        uint8_t *s_out = out ? out + int64_t(n) * s_bytes : nullptr;
+        uint8_t *p_out = pooled ? pooled + int64_t(n) * p_bytes : nullptr;
+        int64_t s_head = 0, s_units = 0, s_upt = 0;  // edge bytes, 16-byte units, units a tile
+        if (s_out)
+        {
+            s_head = min(int64_t((16 - (reinterpret_cast<uintptr_t>(s_out) & 15)) & 15), s_bytes);
+            s_units = (s_bytes - s_head) / 16;
+            s_upt = (s_units + g.tiles - 1) / g.tiles;
+        }
+        uint4 *s_body = reinterpret_cast<uint4 *>(s_out + s_head);
+        for (int32_t tile = 0; tile < g.tiles; ++tile)
+        {
+            if (p_out) zero_slice(p_out, int64_t(tile) * g.zp, min(int64_t(tile + 1) * g.zp, p_bytes));
+            const int64_t u0 = int64_t(tile) * s_upt, u1 = min(u0 + s_upt, s_units);
+            const int32_t buf = (tile & 1) * MAX_WAVES * COLS;
+            const int32_t p = tile * COLS + j;
+            const bool pvalid = p < hw;
+            const int32_t pc = pvalid ? p : hw - 1, y = pc / g.wo, x = pc % g.wo;
+            const uint8_t *pb = stage + (y * g.pw + x) * g.cin_pad;  // the pixel's window at step 0
+            uint32_t mask[RT][4];
+            int32_t mx[RT][4];
+#pragma unroll
+            for (int r = 0; r < RT; ++r)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) mask[r][i] = 0, mx[r][i] = 0;
+
+            {
+                const int32_t last = g.t - 1;
+                int4v rA[S8], rB[S8];
+                int4v hA[RT], lA[RT], hB[RT], lB[RT];
+                auto diag = [&](int32_t k, const int4v (&uh)[RT], const int4v (&ul)[RT]) {
+                    if (DIAG && pvalid)
+#pragma unroll
+                        for (int r = 0; r < RT; ++r)
+#pragma unroll
+                            for (int i = 0; i < 4; ++i)
+                            {
+                                const int32_t o = o0 + r * 16 + i;
+                                if (o < g.cout)
+                                    pot_out[((int64_t(n) * g.t + k) * g.cout + o) * hw + p] =
+                                        float(uh[r][i] * 256 + ul[r][i]) * scale;
+                            }
+                };
+                load_units<S8, WIDE>(rA, pb, off);
+                load_units<S8, WIDE>(rB, pb + min(1, last) * g.sstep, off);
+                chain8(a, rA, hA, lA);
+                int32_t k = 0;
+                int64_t zu = u0 + int32_t(threadIdx.x);
+                for (; k + 2 < g.t; k += 2)
+                {
+                    step_iter8<RT, S8, WIDE>(a, rB, hB, lB, hA, lA, th, mask, mx, rA, pb + (k + 2) * g.sstep, off);
+                    diag(k, hA, lA);
+                    if (zu < u1) zero16(s_body + zu);
+                    zu += blockDim.x;
+                    step_iter8<RT, S8, WIDE>(a, rA, hA, lA, hB, lB, th, mask, mx, rB,
+                                             pb + min(k + 3, last) * g.sstep, off);
+                    diag(k + 1, hB, lB);
+                }
+                if (k + 1 < g.t)
+                {
+                    step_iter8<RT, S8, WIDE>(a, rB, hB, lB, hA, lA, th, mask, mx, rA, pb + last * g.sstep, off);
+                    diag(k, hA, lA);
+                    update_all8(hB, lB, th, mask, mx);
+                    diag(k + 1, hB, lB);
+                }
+                else
+                {
+                    update_all8(hA, lA, th, mask, mx);
+                    diag(k, hA, lA);
+                }
+                for (; zu < u1; zu += blockDim.x) zero16(s_body + zu);
+            }
+
+            {
+                uint32_t any = 0;
+#pragma unroll
+                for (int r = 0; r < RT; ++r)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) any |= mask[r][i];
+                any |= __shfl_xor(any, 16, 64);
+                any |= __shfl_xor(any, 32, 64);
+                if (lane < COLS) any_s[buf + w * COLS + lane] = any;
+            }
+            __syncthreads();
+            // pass 1 at inhibit's step (see fused_layer); the thresholded
+            // values compare exactly as floats (|v| < 2^24)
+            int4v gh[RT], gl[RT];
+            {
+                uint32_t all = 0;
+                for (int32_t ow = 0; ow < g.waves; ++ow) all |= any_s[buf + ow * COLS + j];
+                const int32_t cnt = __builtin_popcount(all);
+                int4v r[S8];
+                load_units<S8, WIDE>(r, pb + (g.t - (cnt > 1 ? cnt : 1)) * g.sstep, off);
+                chain8(a, r, gh, gl);
+                float bv = -1.0f;
+                int32_t bi = INT32_MAX;
+#pragma unroll
+                for (int r2 = 0; r2 < RT; ++r2)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                    {
+                        const int32_t v = gh[r2][i] * 256 + gl[r2][i];
+                        const float xp = v > th ? float(v) : 0.0f;
+                        const bool take = xp > bv;
+                        bv = take ? xp : bv;
+                        bi = take ? o0 + r2 * 16 + i : bi;
+                    }
+                combine(bv, bi, __shfl_xor(bv, 16, 64), __shfl_xor(bi, 16, 64));
+                combine(bv, bi, __shfl_xor(bv, 32, 64), __shfl_xor(bi, 32, 64));
+                if (lane < COLS) red_v[buf + w * COLS + lane] = bv, red_i[buf + w * COLS + lane] = bi;
+            }
+            __syncthreads();
+            int32_t cs;
+            {
+                float m = red_v[buf + j];
+                cs = red_i[buf + j];
+                for (int32_t ow = 1; ow < g.waves; ++ow)
+                    combine(m, cs, red_v[buf + ow * COLS + j], red_i[buf + ow * COLS + j]);
+            }
+            const int32_t cr = (cs % (16 * RT)) / 16, cq = (cs >> 2) & 3, ci = cs & 3;
+            const bool own = cs / (16 * RT) == w && cq == q;
+            uint32_t om = 0;
+            int32_t ox = 0;
+#pragma unroll
+            for (int r = 0; r < RT; ++r)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (cr == r && ci == i) om = mask[r][i], ox = mx[r][i];
+            const int32_t my_t0 = om ? g.t - 32 + int32_t(__builtin_clz(om)) : 0;
+            const int32_t t0 = __shfl(my_t0, cq * 16 + j, 64);
+            // pass 2: the value of c* at its first firing step
+            int32_t first = 0;
+            if (__builtin_amdgcn_ballot_w64(own && om != 0))
+            {
+                int4v r[S8];
+                load_units<S8, WIDE>(r, pb + t0 * g.sstep, off);
+                chain8(a, r, gh, gl);
+#pragma unroll
+                for (int r2 = 0; r2 < RT; ++r2)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        if (cr == r2 && ci == i) first = gh[r2][i] * 256 + gl[r2][i];
+            }
+            if (own && pvalid)
+            {
+                const uint32_t fwd = om ? __builtin_bitreverse32(om) >> (32 - g.t) : 0u;
+                summary[int64_t(n) * hw + p] =
+                    Summary{cs, fwd, om ? float(first) * scale : 0.0f, om ? float(ox) * scale : 0.0f};
+                smap[p] = make_uint2(uint32_t(cs), fwd);
+            }
+        }
+        if (s_out)
+        {
+            if (int64_t(threadIdx.x) < s_head) s_out[threadIdx.x] = 0;
+            for (int64_t i = s_head + s_units * 16 + threadIdx.x; i < s_bytes; i += blockDim.x) s_out[i] = 0;
+        }
+        prev = n;
+    }
+    if (prev >= 0 && (out || pooled))
+    {
+        __syncthreads();
+        scatter_sample(prev, smap, out, pooled, g);
     }
 }
 
@@ -803,6 +1218,60 @@ void launch_layer(const uint8_t *in, const half8 *frag, Summary *summary, float 
     }
 }
 
+// The whole-sample kernel's arithmetic (spk_set_fused_arith): 0 = int8
+// fixed point where its K-units fit (else FP16), 1 = FP16 operands.
+std::atomic<int32_t> g_arith{0};
+
+template <int S8, bool WIDE, bool DIAG>
+void launch_layer8(const uint8_t *in, const int4v *frag, const int32_t *hdr, Summary *summary, float *pot,
+                   uint8_t *fs, uint8_t *out, uint8_t *pooled, const FusedGeom &g, size_t lds, hipStream_t s)
+{
+    static bool configured = false;
+    if (!configured)
+    {
+        SPK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&fused_layer8<2, S8, WIDE, DIAG>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 163840));
+        configured = true;
+    }
+    const int cus = cu_count();
+    const int grid = g.n < cus ? g.n : cus;
+    KernelTimer &tm = timer();
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (tm.on)
+    {
+        SPK_HIP(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));
+        SPK_HIP(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence));
+        SPK_HIP(hipEventRecord(e0, s));
+    }
+    fused_layer8<2, S8, WIDE, DIAG><<<dim3(grid), dim3(64 * g.waves), lds, s>>>(in, frag, hdr, summary, pot, fs, out,
+                                                                              pooled, g);
+    SPK_LAUNCHED();
+    if (tm.on)
+    {
+        SPK_HIP(hipEventRecord(e1, s));
+        tm.events.emplace_back(e0, e1);
+    }
+}
+
+void dispatch_layer8(int steps, bool wide, const uint8_t *in, const int4v *frag, const int32_t *hdr,
+                     Summary *summary, float *pot, uint8_t *fs, uint8_t *out, uint8_t *pooled, const FusedGeom &g,
+                     size_t lds, hipStream_t s)
+{
+    switch (steps)
+    {
+#define CASE(k)                                                                                                   \
+    case k:                                                                                                       \
+        if (wide)                                                                                                 \
+            return pot ? launch_layer8<k, true, true>(in, frag, hdr, summary, pot, fs, out, pooled, g, lds, s)     \
+                       : launch_layer8<k, true, false>(in, frag, hdr, summary, pot, fs, out, pooled, g, lds, s);  \
+        return pot ? launch_layer8<k, false, true>(in, frag, hdr, summary, pot, fs, out, pooled, g, lds, s)        \
+                   : launch_layer8<k, false, false>(in, frag, hdr, summary, pot, fs, out, pooled, g, lds, s);
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+    default: SPK_ARG(false, "Fused layer: int8 K-steps out of range.");
+    }
+}
+
 template <int RT>
 void dispatch_layer(int steps, bool paired, const uint8_t *in, const half8 *frag, Summary *summary, float *pot,
                     uint8_t *fs, uint8_t *out, uint8_t *pooled, const FusedGeom &g, size_t lds, hipStream_t s)
@@ -828,6 +1297,14 @@ void dispatch_layer(int steps, bool paired, const uint8_t *in, const half8 *frag
 using namespace spk;
 
 extern "C" {
+
+int32_t spk_set_fused_arith(int32_t mode)
+{
+    return guard([&] {
+        SPK_ARG(mode == 0 || mode == 1, "Fused layer: arithmetic mode must be 0 (int8 fixed point) or 1 (FP16).");
+        g_arith.store(mode);
+    });
+}
 
 int32_t spk_kernel_timer(int32_t enable)
 {
@@ -919,8 +1396,16 @@ void stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, int64_t
         // potentials come out of the MFMAs times 2^-24 (spike bytes enter as
         // FP16 subnormals); the threshold is scaled to match (exact)
         g.th = float(threshold) * (1.0f / TWO_POW_24);
-        SPK_UNSUP(threshold_scales(threshold), "Fused layer: the threshold is too small to scale exactly.");
-        SPK_UNSUP(g.steps <= MAX_STEPS, "Fused layer: Cin*Kh*Kw too large (K <= 512 after padding).");
+        g.thf = float(threshold);
+        // int8 K-units: 16 bytes of a window row (kw pixels x cin_pad channels)
+        g.upr = int32_t(cdiv(int64_t(kw) * g.cin_pad, 16)), g.units = int32_t(kh) * g.upr;
+        const int32_t steps8 = int32_t(cdiv(g.units, 4));
+        const bool use8 = g_arith.load() == 0 && steps8 <= MAX_STEPS8;
+        if (!use8)
+        {
+            SPK_UNSUP(threshold_scales(threshold), "Fused layer: the threshold is too small to scale exactly.");
+            SPK_UNSUP(g.steps <= MAX_STEPS, "Fused layer: Cin*Kh*Kw too large (K <= 512 after padding).");
+        }
         // stage + any/argmax partials (two buffers each) + the per-pixel map of kept channels and fire masks
         const int64_t lds = stage_bytes + 3 * 2 * MAX_WAVES * COLS * 4 + ho * wo * 8;
         if (lds > 163840 || ho * wo > 8192)
@@ -953,9 +1438,11 @@ void stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, int64_t
         const int64_t row_tiles = int64_t(g.waves) * rt, ksz = cin * kh * kw;
         const FusedStdp fp{n * count, int32_t(count), g.cin, g.cin_pad, g.h, g.w, g.kh, g.kw, g.pt, g.pl,
                            g.wo, int32_t(hw), g.t, int32_t(cdiv(ksz, 32))};
-        // the call's buffers, carved from the stream's workspace: FP16 weight
-        // fragments, summaries, winner keys, first-spike map, STDP bit records
-        const int64_t sizes[5] = {row_tiles * g.steps * 64 * 16, n * hw * int64_t(sizeof(Summary)),
+        // the call's buffers, carved from the stream's workspace: weight
+        // fragments (FP16, or a 256-byte header and int8 planes), summaries,
+        // winner keys, first-spike map, STDP bit records
+        const int64_t frag_bytes = use8 ? 256 + row_tiles * steps8 * 2 * 64 * 16 : row_tiles * g.steps * 64 * 16;
+        const int64_t sizes[5] = {frag_bytes, n * hw * int64_t(sizeof(Summary)),
                                   n * count * int64_t(sizeof(int32_t)), learn ? n * hwi * g.cin_pad : 0,
                                   learn ? n * count * fp.words * int64_t(sizeof(uint32_t)) : 0};
         int64_t offs[6] = {0};
@@ -969,15 +1456,32 @@ void stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, int64_t
         int32_t *wkey = reinterpret_cast<int32_t *>(ws + offs[2]);
         uint8_t *fsm = learn ? ws + offs[3] : nullptr;
         uint32_t *bits = reinterpret_cast<uint32_t *>(ws + offs[4]);
-        // pairs of K-steps share 16-byte reads when 16-byte units stay inside a tap
-        const bool paired = g.cin_pad % 16 == 0 && g.steps >= 2;
-        prep_fragments<<<dim3(unsigned(cdiv(row_tiles * g.steps * 64, 256))), 256, 0, s>>>(
-            (const float *)kernel, frag, g.cout, g.cin, g.kh, g.kw, g.cin_pad, g.groups, g.steps, int32_t(row_tiles),
-            paired ? 1 : 0);
-        SPK_LAUNCHED();
-        // the kernel also writes the spikes and pooled spikes of each sample
-        dispatch_layer<2>(g.steps, paired, (const uint8_t *)in, frag, summary, (float *)potentials, fsm,
-                          (uint8_t *)out, (uint8_t *)pooled, g, size_t(lds), s);
+        if (use8)
+        {
+            // the scale exponent in the first 256 bytes, the fragments after
+            int32_t *hdr = reinterpret_cast<int32_t *>(ws + offs[0]);
+            int4v *frag8 = reinterpret_cast<int4v *>(ws + offs[0] + 256);
+            const int64_t entries = row_tiles * steps8 * 2 * 64;
+            prep_fragments8<<<dim3(unsigned(std::min<int64_t>(cdiv(entries, 1024), 32))), 1024, 0, s>>>(
+                (const float *)kernel, frag8, hdr, g.cout, g.cin, g.kh, g.kw, g.cin_pad, g.upr, g.units, steps8,
+                int32_t(row_tiles));
+            SPK_LAUNCHED();
+            // the kernel also writes the spikes and pooled spikes of each sample
+            dispatch_layer8(steps8, g.cin_pad % 16 == 0, (const uint8_t *)in, frag8, hdr, summary,
+                            (float *)potentials, fsm, (uint8_t *)out, (uint8_t *)pooled, g, size_t(lds), s);
+        }
+        else
+        {
+            // pairs of K-steps share 16-byte reads when 16-byte units stay inside a tap
+            const bool paired = g.cin_pad % 16 == 0 && g.steps >= 2;
+            prep_fragments<<<dim3(unsigned(cdiv(row_tiles * g.steps * 64, 256))), 256, 0, s>>>(
+                (const float *)kernel, frag, g.cout, g.cin, g.kh, g.kw, g.cin_pad, g.groups, g.steps,
+                int32_t(row_tiles), paired ? 1 : 0);
+            SPK_LAUNCHED();
+            // the kernel also writes the spikes and pooled spikes of each sample
+            dispatch_layer<2>(g.steps, paired, (const uint8_t *)in, frag, summary, (float *)potentials, fsm,
+                              (uint8_t *)out, (uint8_t *)pooled, g, size_t(lds), s);
+        }
         // convwta: one wave per sample when the plane fits a wave's registers
         if (hw <= 256)
             fused_select_wave<4><<<dim3(unsigned(cdiv(n, 4))), 256, 0, s>>>(summary, g.t, g.ho, g.wo, int32_t(ry),

@@ -9,6 +9,8 @@ Two bars:
     K * 2^-11 * max|w| of FP64 and the threshold-flip rate stays <= 0.1 %.
 """
 
+import contextlib
+
 import numpy as np
 import pytest
 import torch
@@ -16,6 +18,18 @@ import torch
 import spyker_amd as spk
 
 pytestmark = pytest.mark.gpu
+
+ARITH = ["int8", "fp16"]
+
+
+@contextlib.contextmanager
+def arith(mode):
+    """The whole-sample kernel's arithmetic for the duration of a block (int8 is the default)."""
+    spk.set_fused_arith(mode)
+    try:
+        yield
+    finally:
+        spk.set_fused_arith("int8")
 
 
 def T(a, dev):
@@ -70,7 +84,8 @@ def check_winners(wn, win, cnt):
     (3, 32, 8, 12, 12, 64, 3, (1, 1, 1, 1), 6.0, 1, 6, (2, 2), False),
     (3, 24, 16, 10, 10, 96, 3, (1, 1, 1, 1), 10.0, 2, 5, (2, 2), True),
 ])
-def test_fused_exact_on_dyadic_weights(dev, orc, case):
+@pytest.mark.parametrize("mode", ARITH)
+def test_fused_exact_on_dyadic_weights(dev, orc, case, mode):
     n, t, cin, h, w, cout, k, pad, th, radius, count, pool, cumulative = case
     rng = np.random.default_rng(11)
     x = spikes(rng, n, t, cin, h, w, 0.5 if cumulative else 1.0, cumulative)
@@ -81,7 +96,8 @@ def test_fused_exact_on_dyadic_weights(dev, orc, case):
     conv = spk.Conv(cin, cout, k, pad=pad, device=dev)
     conv.kernel = T(kern, dev)
     conv.stdpconfig = [spk.STDPConfig(0.004, -0.003)]
-    out, wn, pooled = spk.layer_step(conv, T(x, dev), th, radius, count, *(pool if pool else (None, None)))
+    with arith(mode):
+        out, wn, pooled = spk.layer_step(conv, T(x, dev), th, radius, count, *(pool if pool else (None, None)))
     o_out, o_win, o_cnt, o_k, o_pool = oracle_step(orc, x, kern, th, radius, count, pad, pool)
     assert o_cnt.sum() > 0
     assert np.array_equal(H(out), o_out)
@@ -141,7 +157,8 @@ def test_fused_stdp_negative_zero_weights(dev, orc):
         assert ((o_k[:, :3] == 0) & ~np.signbit(o_k[:, :3])).any()
 
 
-def test_fused_precision_vs_fp64(dev, orc):
+@pytest.mark.parametrize("mode", ARITH)
+def test_fused_precision_vs_fp64(dev, orc, mode):
     n, t, cin, h, w, cout = 8, 15, 30, 14, 14, 250
     rng = np.random.default_rng(12)
     x = spikes(rng, n, t, cin, h, w, 0.5)
@@ -152,7 +169,8 @@ def test_fused_precision_vs_fp64(dev, orc):
     conv.kernel = T(kern, dev)
     conv.stdpconfig = [spk.STDPConfig(0.004, -0.003)]
     pots = torch.empty((n, t, cout, h, w), dtype=torch.float32, device=dev)
-    spk.layer_step(conv, T(x, dev), th, 1, 8, learn=False, potentials=pots)
+    with arith(mode):
+        spk.layer_step(conv, T(x, dev), th, 1, 8, learn=False, potentials=pots)
     p16 = H(pots).astype(np.float64)
     K = cin * 9
     assert np.abs(p16 - pot64).max() <= K * 2.0 ** -11 * kern.max()

@@ -75,12 +75,17 @@ def make_conv(dev, kern, pad):
 
 @pytest.mark.parametrize("tune", [False, True])
 def test_region_equals_whole_sample_kernel(dev, tune):
+    """Both kernels in FP16 arithmetic sum the same operands in the same MFMA order."""
     n, t, cin, h, w, cout = 8, 15, 30, 14, 14, 250
     rng = np.random.default_rng(31)
     x = spikes(rng, (n, t, cin, h, w))
     kern = np.clip(rng.normal(0.8, 0.05, (cout, cin, 3, 3)), 0, 1).astype(np.float32)
     c1, c2 = make_conv(dev, kern, 1), make_conv(dev, kern, 1)
-    out1, wn1, pool1 = spk.layer_step(c1, T(x, dev), 10.0, 1, 8, 3, 3)
+    spk.set_fused_arith("fp16")
+    try:
+        out1, wn1, pool1 = spk.layer_step(c1, T(x, dev), 10.0, 1, 8, 3, 3)
+    finally:
+        spk.set_fused_arith("int8")
     plan = spk.LayerPlan(c2, [(t, h, w)] * n, 10.0, 1, 8, 3, 3, tune=tune)
     out2, pool2, wn2 = plan.run(T(x, dev).reshape(-1))
     assert H(wn1.counts).sum() > 0
