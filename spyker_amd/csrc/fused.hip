@@ -509,12 +509,14 @@ __global__ void __launch_bounds__(layer_threads<RT>())
             // strict '>' keeps the first maximum; padded channels have zero
             // weights and value 0, which never beats a lower channel.
             float4v gv[RT];
+            int32_t ti;  // inhibit's step at this pixel
             {
                 uint32_t all = 0;
                 for (int32_t ow = 0; ow < g.waves; ++ow) all |= any_s[buf + ow * COLS + j];
                 const int32_t cnt = __builtin_popcount(all);
+                ti = g.t - (cnt > 1 ? cnt : 1);
                 StepBytes<S, PAIRED> r;
-                load_raw(r, pb + (g.t - (cnt > 1 ? cnt : 1)) * g.sstep, toff);
+                load_raw(r, pb + ti * g.sstep, toff);
                 mfma_sub(a, r, gv);
                 float bv = -1.0f;
                 int32_t bi = INT32_MAX;
@@ -548,16 +550,20 @@ __global__ void __launch_bounds__(layer_threads<RT>())
             const bool own = cs / (16 * RT) == w && cq == q;
             uint32_t om = 0;
             int32_t ox = 0;
+            float first = 0.0f;  // c*'s value at inhibit's step, from pass 1
 #pragma unroll
             for (int r = 0; r < RT; ++r)
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    if (cr == r && ci == i) om = mask[r][i], ox = mxb[r][i];
+                    if (cr == r && ci == i) om = mask[r][i], ox = mxb[r][i], first = gv[r][i];
             const int32_t my_t0 = om ? g.t - 32 + int32_t(__builtin_clz(om)) : 0;
             const int32_t t0 = __shfl(my_t0, cq * 16 + j, 64);
-            // pass 2: the value of c* at its first firing step (convwta's t0 value)
-            float first = 0.0f;
-            if (__builtin_amdgcn_ballot_w64(own && om != 0))
+            // pass 2: the value of c* at its first firing step (convwta's t0
+            // value).  Where that step is inhibit's step the value is pass 1's
+            // (the same operands and sums): with cumulative spikes and
+            // non-negative weights the potentials never fall, so c* fires
+            // first exactly at inhibit's step and no lane needs the pass.
+            if (__builtin_amdgcn_ballot_w64(own && pvalid && om != 0 && my_t0 != ti))
             {
                 StepBytes<S, PAIRED> r;
                 load_raw(r, pb + t0 * g.sstep, toff);
@@ -908,12 +914,14 @@ __global__ void __launch_bounds__(64 * MAX_WAVES)
             // pass 1 at inhibit's step (see fused_layer); the thresholded
             // values compare exactly as floats (|v| < 2^24)
             int4v gh[RT], gl[RT];
+            int32_t ti;  // inhibit's step at this pixel
             {
                 uint32_t all = 0;
                 for (int32_t ow = 0; ow < g.waves; ++ow) all |= any_s[buf + ow * COLS + j];
                 const int32_t cnt = __builtin_popcount(all);
+                ti = g.t - (cnt > 1 ? cnt : 1);
                 int4v r[S8];
-                load_units<S8, WIDE>(r, pb + (g.t - (cnt > 1 ? cnt : 1)) * g.sstep, off);
+                load_units<S8, WIDE>(r, pb + ti * g.sstep, off);
                 chain8(a, r, gh, gl);
                 float bv = -1.0f;
                 int32_t bi = INT32_MAX;
@@ -943,17 +951,17 @@ __global__ void __launch_bounds__(64 * MAX_WAVES)
             const int32_t cr = (cs % (16 * RT)) / 16, cq = (cs >> 2) & 3, ci = cs & 3;
             const bool own = cs / (16 * RT) == w && cq == q;
             uint32_t om = 0;
-            int32_t ox = 0;
+            int32_t ox = 0, first = 0;  // first: c*'s value at inhibit's step, from pass 1
 #pragma unroll
             for (int r = 0; r < RT; ++r)
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    if (cr == r && ci == i) om = mask[r][i], ox = mx[r][i];
+                    if (cr == r && ci == i) om = mask[r][i], ox = mx[r][i], first = gh[r][i] * 256 + gl[r][i];
             const int32_t my_t0 = om ? g.t - 32 + int32_t(__builtin_clz(om)) : 0;
             const int32_t t0 = __shfl(my_t0, cq * 16 + j, 64);
-            // pass 2: the value of c* at its first firing step
-            int32_t first = 0;
-            if (__builtin_amdgcn_ballot_w64(own && om != 0))
+            // pass 2: the value of c* at its first firing step, run only where
+            // that step is not inhibit's step (as in fused_layer)
+            if (__builtin_amdgcn_ballot_w64(own && pvalid && om != 0 && my_t0 != ti))
             {
                 int4v r[S8];
                 load_units<S8, WIDE>(r, pb + t0 * g.sstep, off);
