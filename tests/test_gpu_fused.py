@@ -1,4 +1,4 @@
-"""The fused FP16-MFMA layer step.
+"""The fused layer step (int8 fixed-point and FP16 MFMA arithmetic).
 
 Two bars:
   * structure: with weights exactly representable in FP16 and potentials
@@ -227,6 +227,56 @@ def test_fused_graph_replay(dev):
         assert torch.equal(out, e[0]) and torch.equal(pooled, e[1])
         assert torch.equal(wn.data, e[2]) and torch.equal(wn.counts, e[3])
         assert torch.equal(conv.kernel, e[4])
+
+
+def test_two_layer_stack_graph_replay(dev):
+    """The two-layer stroke-image network's training step for layer 2 (layer
+    1 forward: conv 4->30 5x5 pad 2, fire 15, pool 2; then the layer-2 step:
+    conv 30->250 3x3 pad 1, threshold 10, inhibit, convwta r1 k8, fire,
+    STDP, pool 3x3) captured as one hipGraph: every replay gives the eager
+    steps' layer-1 spikes, spikes, pooled spikes, winners and weights."""
+    from spyker_amd import synthetic
+    n = 12
+    img = torch.from_numpy(synthetic.strokes(n, 28, seed=43).astype(np.float32) / 255).to(dev).reshape(n, 1, 28, 28)
+    d3 = spk.DoG(1, [spk.DoGFilter(1, 2), spk.DoGFilter(2, 1)], pad=1, device=dev)
+    d7 = spk.DoG(3, [spk.DoGFilter(1, 2), spk.DoGFilter(2, 1)], pad=3, device=dev)
+    filt = torch.cat([d3(img), d7(img)], dim=1).contiguous()
+    spk.threshold(filt, 0.01)
+    coded = spk.code(filt, 15)  # [n, 15, 4, 28, 28]
+    rng = np.random.default_rng(44)
+    k1 = T(np.clip(rng.normal(0.8, 0.05, (30, 4, 5, 5)), 0, 1).astype(np.float32), dev)
+    k2 = np.clip(rng.normal(0.8, 0.05, (250, 30, 3, 3)), 0, 1).astype(np.float32)
+    conv2 = spk.Conv(30, 250, 3, pad=1, device=dev)
+    conv2.kernel = T(k2, dev)
+    conv2.stdpconfig = [spk.STDPConfig(0.004, -0.003)]
+
+    def stack():
+        l1 = spk.pool(spk.fire(spk.conv(coded, k1, 1, 2), 15.0), 2)
+        out, wn, pooled = spk.layer_step(conv2, l1, 10.0, 1, 8, 3, 3)
+        return l1, out, wn, pooled
+
+    eager = []
+    for _ in range(3):
+        l1, out, wn, pooled = stack()
+        eager.append([v.clone() for v in (l1, out, pooled, wn.data, wn.counts, conv2.kernel)])
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        stack()  # warm the capture stream's workspaces
+    torch.cuda.current_stream().wait_stream(side)
+    conv2.kernel.copy_(T(k2, dev))
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=side):
+        l1, out, wn, pooled = stack()
+    conv2.kernel.copy_(T(k2, dev))
+    torch.cuda.synchronize()
+    for e in eager:
+        g.replay()
+        torch.cuda.synchronize()
+        assert e[4].sum() > 0
+        for got, want in zip((l1, out, pooled, wn.data, wn.counts, conv2.kernel), e):
+            assert torch.equal(got, want)
 
 
 @pytest.mark.parametrize("case", [
