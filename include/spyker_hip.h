@@ -139,30 +139,39 @@ int32_t spk_rank_pool(const void *in, void *out, int32_t type, int64_t planes, i
                       int64_t kw, int64_t sh, int64_t sw, int64_t pt, int64_t pl, int64_t pb, int64_t pr, void *stream);
 
 /* Core::cuda_rank_inhibit(Dyn4 input, Scalar threshold), impl.h:272:
- * x is [n][t][c][hw], in place.  Types: F32. */
+ * x is [n][t][c][hw], in place, compared and scaled in its own type with the
+ * threshold converted to it (cpu/inhibit.cpp:41-121).  Types: F16, F32, F64. */
 int32_t spk_rank_inhibit(void *x, int32_t type, int64_t n, int64_t t, int64_t c, int64_t hw, double threshold,
                          void *stream);
 
 /* Core::cuda_rank_convwta(Dyn5 input, Len2 radius, Size count, Scalar
  * threshold), impl.h:280: x is [n][t][c][h][w]; winners (device) receives
- * n*count records, counts (device) n entries.  Types: F32, U8. */
+ * n*count records, counts (device) n entries.  Types: F16, F32, F64 (values
+ * computed in that type, the reference's Limits<T>::min() for unfired
+ * neurons: the smallest positive normal for F16, cpu/wta.cpp:44-64 and
+ * base.h:761-774), U8. */
 int32_t spk_rank_convwta(const void *x, int32_t type, int64_t n, int64_t t, int64_t c, int64_t h, int64_t w,
                          int64_t ry, int64_t rx, int64_t count, double threshold, spk_winner *winners,
                          int64_t *counts, void *stream);
 
 /* Core::cuda_rank_fcwta(Dyn3 input, Size radius, Size count, Scalar
- * threshold), impl.h:276: x is [n][t][f].  Types: F32, and U8 (the
- * reference's integer branch, value = t - first spike step, cpu/wta.cpp:66-85). */
+ * threshold), impl.h:276: x is [n][t][f].  Types: F16, F32, F64 (as
+ * spk_rank_convwta), and U8 (the reference's integer branch, value = t -
+ * first spike step, cpu/wta.cpp:66-85). */
 int32_t spk_rank_fcwta(const void *x, int32_t type, int64_t n, int64_t t, int64_t f, int64_t radius, int64_t count,
                        double threshold, spk_winner *winners, int64_t *counts, void *stream);
 
 /* -- contractions and filters --------------------------------------------- */
 
 /* Core::cuda_conv(Dyn4 input, Dyn4 kernel, Dyn4 output, Len2 stride, Len4
- * pad), impl.h:224: cross-correlation, zero pad.  in: U8 or F32
- * [n][cin][h][w]; kernel F32 [cout][cin][kh][kw]; out F32 (or F64 with
- * kernel F32, the high-precision check path).  FP32 accumulation is one fused
- * multiply-add per tap in (cin, ky, kx) order starting from +0. */
+ * pad), impl.h:224: cross-correlation, zero pad.  in [n][cin][h][w] is
+ * converted to the kernel's type first and out has the kernel's type
+ * (opers.cpp:408-421): kernel F32 (in U8, F16, F32 or F64; out F32, or F64
+ * for the high-precision check path), F16 (in U8, F16 or F32; out F16) or F64
+ * (in U8, F32 or F64; out F64), [cout][cin][kh][kw].  Accumulation is one
+ * fused multiply-add per tap in (cin, ky, kx) order starting from +0, in FP32
+ * for F32 and F16 kernels (an F16 output is that sum rounded once) and in
+ * FP64 for F64 kernels (and the check path). */
 int32_t spk_conv(const void *in, int32_t in_type, const void *kernel, int32_t kernel_type, void *out,
                  int32_t out_type, int64_t n, int64_t cin, int64_t h, int64_t w, int64_t cout, int64_t kh, int64_t kw,
                  int64_t sy, int64_t sx, int64_t pt, int64_t pl, int64_t pb, int64_t pr, void *stream);
@@ -255,8 +264,9 @@ int32_t spk_rate_fire(const void *in, int32_t in_type, void *out, int32_t out_ty
 
 /* Core::cuda_rank_convstdp(Dyn5 input, Dyn4 kernel, Dyn5 output, configs,
  * Winners, Len2 stride, Len4 pad), impl.h:284-289.  in/out U8 spikes
- * [n][t][cin][h][w] / [n][t][cout][ho][wo]; kernel F32 [cout][cin][kh][kw]
- * updated in place in (sample, winner) order; configs on the host; winners
+ * [n][t][cin][h][w] / [n][t][cout][ho][wo]; kernel F32, F16 or F64
+ * [cout][cin][kh][kw] updated in place in (sample, winner) order, each update
+ * in F64 rounded once to the kernel's type (cpu/stdp.cpp:121-137); configs on the host; winners
  * and counts on the device ([n][maxcount], [n]).  Winners with c < 0 are
  * skipped; any c >= nconfigs fails the call before any weight changes
  * (SPK_ERR_CONFIG). */
@@ -268,7 +278,7 @@ int32_t spk_rank_convstdp(const void *in, void *kernel, int32_t kernel_type, con
 
 /* Core::cuda_rank_fcstdp(Dyn3 input, Dyn2 kernel, Dyn3 output, configs,
  * Winners), impl.h:290-294.  in [n][t][fin] U8, out [n][t][fout] U8,
- * kernel F32 [fout][fin]. */
+ * kernel F32, F16 or F64 [fout][fin]. */
 int32_t spk_rank_fcstdp(const void *in, void *kernel, int32_t kernel_type, const void *out, int64_t n, int64_t t,
                         int64_t fin, int64_t fout, const spk_stdp_config *configs, int64_t nconfigs,
                         const spk_winner *winners, const int64_t *counts, int64_t maxcount, void *stream);

@@ -445,6 +445,22 @@ class Oracle(_Lib):
            I64(sy), I64(sx), I64(pt), I64(pl), I64(pb), I64(pr), _ptr(out), I32(threads or os.cpu_count() or 1))
         return out
 
+    def conv_f64k(self, x, kernel, stride=1, pad=(0, 0, 0, 0), threads=None):
+        """F64 kernel: x [N, Cin, H, W] uint8/float32/float64 converted to F64, F64 fma chain."""
+        kind = {np.dtype(np.uint8): 0, np.dtype(np.float32): 1, np.dtype(np.float64): 2}[x.dtype]
+        x = np.ascontiguousarray(x)
+        kernel = np.ascontiguousarray(kernel, np.float64)
+        n, cin, h, w = x.shape
+        cout, _, kh, kw = kernel.shape
+        sy, sx = (stride, stride) if np.isscalar(stride) else stride
+        pt, pl, pb, pr = pad
+        ho, wo = (h + pt + pb - kh) // sy + 1, (w + pl + pr - kw) // sx + 1
+        out = np.zeros((n, cout, ho, wo), np.float64)
+        self.lib.orc_conv_f64k(_ptr(x), I32(kind), I64(n), I64(cin), I64(h), I64(w), _ptr(kernel), I64(cout),
+                               I64(kh), I64(kw), I64(sy), I64(sx), I64(pt), I64(pl), I64(pb), I64(pr), _ptr(out),
+                               I32(threads or os.cpu_count() or 1))
+        return out
+
     def fc(self, x, kernel):
         is_u8 = x.dtype == np.uint8
         x = np.ascontiguousarray(x, np.uint8 if is_u8 else np.float32)
@@ -547,6 +563,74 @@ class Reference(_Lib):
         out = np.zeros((rows, bins), np.uint16)
         self._check(self.lib.ref_poisson_create(I64(rows), I64(bins), _ptr(out)), "poisson_create")
         return out
+
+
+
+# Spyker Type ordinals (src/spyker/utils.h:96-117) of the real numpy types
+REF_TYPE = {np.dtype(np.float16): 8, np.dtype(np.float32): 9, np.dtype(np.float64): 10}
+
+
+class ReferenceTyped(Reference):
+    """The reference's own CPU stages for F16, F32 or F64 values (ref_*_t)."""
+
+    def threshold(self, x, th, value=0.0):
+        x = np.array(x, copy=True)
+        self._check(self.lib.ref_threshold_t(_ptr(x), I32(REF_TYPE[x.dtype]), I64(x.size), F64(th), F64(value)),
+                    "threshold")
+        return x
+
+    def fire(self, x, th=0.0):
+        x = np.ascontiguousarray(x)
+        out = np.zeros(x.shape, np.uint8)
+        self._check(self.lib.ref_rank_fire_t(_ptr(x), I32(REF_TYPE[x.dtype]), _ptr(out), I64(x.size), F64(th)),
+                    "fire")
+        return out
+
+    def inhibit(self, x, th=0.0):
+        x = np.array(x, copy=True)
+        n, t, c = x.shape[:3]
+        hw = int(np.prod(x.shape[3:]))
+        self._check(self.lib.ref_rank_inhibit_t(_ptr(x), I32(REF_TYPE[x.dtype]), I64(n), I64(t), I64(c), I64(hw),
+                                                F64(th)), "inhibit")
+        return x
+
+    def convwta(self, x, radius, count, th=0.0):
+        ry, rx = (radius, radius) if np.isscalar(radius) else radius
+        x = np.ascontiguousarray(x)
+        n, t, c, h, w = x.shape
+        win = np.zeros(n * count, WINNER)
+        cnt = np.zeros(n, np.int64)
+        self._check(self.lib.ref_rank_convwta_t(_ptr(x), I32(REF_TYPE[x.dtype]), I64(n), I64(t), I64(c), I64(h),
+                                                I64(w), I64(ry), I64(rx), I64(count), F64(th), _ptr(win), _ptr(cnt)),
+                    "convwta")
+        return win.reshape(n, count), cnt
+
+    def fcwta(self, x, radius, count, th=0.0):
+        x = np.ascontiguousarray(x)
+        n, t, f = x.shape
+        win = np.zeros(n * count, WINNER)
+        cnt = np.zeros(n, np.int64)
+        self._check(self.lib.ref_rank_fcwta_t(_ptr(x), I32(REF_TYPE[x.dtype]), I64(n), I64(t), I64(f), I64(radius),
+                                              I64(count), F64(th), _ptr(win), _ptr(cnt)), "fcwta")
+        return win.reshape(n, count), cnt
+
+    def convstdp(self, spk_in, kernel, spk_out, configs, winners, counts, stride=1, pad=(0, 0, 0, 0)):
+        spk_in = np.ascontiguousarray(spk_in, np.uint8)
+        spk_out = np.ascontiguousarray(spk_out, np.uint8)
+        k = np.array(kernel, copy=True)
+        cfg = configs_array(configs)
+        win = np.ascontiguousarray(winners, WINNER)
+        cnt = np.ascontiguousarray(counts, np.int64)
+        n, t, cin, h, w = spk_in.shape
+        cout, _, kh, kw = k.shape
+        ho, wo = spk_out.shape[-2:]
+        sy, sx = (stride, stride) if np.isscalar(stride) else stride
+        rc = self.lib.ref_rank_convstdp_t(
+            _ptr(spk_in), I64(n), I64(t), I64(cin), I64(h), I64(w), _ptr(k), I32(REF_TYPE[k.dtype]), I64(cout),
+            I64(kh), I64(kw), _ptr(spk_out), I64(ho), I64(wo), _ptr(cfg), I64(len(cfg)), _ptr(win), _ptr(cnt),
+            I64(win.shape[1]), I64(sy), I64(sx), *[I64(p) for p in pad])
+        self._check(rc, "convstdp")
+        return k
 
 
 def reference_available():

@@ -222,6 +222,55 @@ int ref_rank_fcstdp_f32(const uint8_t *in, int64_t n, int64_t t, int64_t fin, fl
     });
 }
 
+// The same entry points for any real type (Type ordinals F16 = 8, F32 = 9,
+// F64 = 10, utils.h:96-117): the reference computes inhibit, wta, fire and
+// the STDP update in the tensor's own type (F16 through half_float::half).
+int ref_threshold_t(void *x, int32_t type, int64_t n, double threshold, double value)
+{
+    return guard([&] { cpu_threshold(Dyn1(x, Type(type), n), Scalar(threshold), Scalar(value)); });
+}
+
+int ref_rank_fire_t(const void *x, int32_t type, uint8_t *out, int64_t n, double threshold)
+{
+    return guard([&] { cpu_rank_fire(Dyn1((void *)x, Type(type), n), Dyn1(out, Type::U8, n), Scalar(threshold)); });
+}
+
+int ref_rank_inhibit_t(void *x, int32_t type, int64_t n, int64_t t, int64_t c, int64_t hw, double threshold)
+{
+    return guard([&] { cpu_rank_inhibit(Dyn4(x, Type(type), n, t, c, hw), Scalar(threshold)); });
+}
+
+int ref_rank_convwta_t(const void *x, int32_t type, int64_t n, int64_t t, int64_t c, int64_t h, int64_t w,
+                       int64_t ry, int64_t rx, int64_t count, double threshold, RefWinner *winners, int64_t *counts)
+{
+    return guard([&] {
+        auto r = cpu_rank_convwta(Dyn5((void *)x, Type(type), n, t, c, h, w), Len2(ry, rx), count, Scalar(threshold));
+        put(r, count, winners, counts);
+    });
+}
+
+int ref_rank_fcwta_t(const void *x, int32_t type, int64_t n, int64_t t, int64_t f, int64_t radius, int64_t count,
+                     double threshold, RefWinner *winners, int64_t *counts)
+{
+    return guard([&] {
+        auto r = cpu_rank_fcwta(Dyn3((void *)x, Type(type), n, t, f), radius, count, Scalar(threshold));
+        put(r, count, winners, counts);
+    });
+}
+
+int ref_rank_convstdp_t(const uint8_t *in, int64_t n, int64_t t, int64_t cin, int64_t h, int64_t w, void *kernel,
+                        int32_t ktype, int64_t cout, int64_t kh, int64_t kw, const uint8_t *out, int64_t ho,
+                        int64_t wo, const RefConfig *cfg, int64_t ncfg, const RefWinner *winners,
+                        const int64_t *counts, int64_t maxcount, int64_t sy, int64_t sx, int64_t pt, int64_t pl,
+                        int64_t pb, int64_t pr)
+{
+    return guard([&] {
+        cpu_rank_convstdp(Dyn5((void *)in, Type::U8, n, t, cin, h, w), Dyn4(kernel, Type(ktype), cout, cin, kh, kw),
+                          Dyn5((void *)out, Type::U8, n, t, cout, ho, wo), configs(cfg, ncfg),
+                          get(winners, counts, n, maxcount), Len2(sy, sx), Len4(pt, pl, pb, pr));
+    });
+}
+
 int ref_gaussian_kernel(double *kernel, int64_t k, double std)
 {
     return guard([&] { gaussian_kernel(Vec2<F64>(kernel, k, k), std); });

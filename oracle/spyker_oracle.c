@@ -652,6 +652,41 @@ void orc_conv_f64(const void *in, int32_t is_u8, int64_t n, int64_t cin, int64_t
                 }
 }
 
+/* F64 kernels (opers.cpp:408-421: the input converted to F64): one fused
+ * multiply-add per tap in (cin, ky, kx) order from +0, the order the device
+ * conv fixes.  in_kind: 0 = U8, 1 = F32, 2 = F64. */
+void orc_conv_f64k(const void *in, int32_t in_kind, int64_t n, int64_t cin, int64_t h, int64_t w,
+                   const double *kernel, int64_t cout, int64_t kh, int64_t kw, int64_t sy, int64_t sx, int64_t pt,
+                   int64_t pl, int64_t pb, int64_t pr, double *out, int32_t threads)
+{
+    int64_t ho = (h + pt + pb - kh) / sy + 1, wo = (w + pl + pr - kw) / sx + 1;
+#pragma omp parallel for collapse(2) num_threads(threads > 0 ? threads : 1)
+    for (int64_t s = 0; s < n; ++s)
+        for (int64_t o = 0; o < cout; ++o)
+            for (int64_t y = 0; y < ho; ++y)
+                for (int64_t x = 0; x < wo; ++x)
+                {
+                    double acc = 0.0;
+                    for (int64_t c = 0; c < cin; ++c)
+                        for (int64_t a = 0; a < kh; ++a)
+                        {
+                            int64_t iy = y * sy + a - pt;
+                            if (iy < 0 || iy >= h) continue;
+                            for (int64_t b = 0; b < kw; ++b)
+                            {
+                                int64_t ix = x * sx + b - pl;
+                                if (ix < 0 || ix >= w) continue;
+                                int64_t i = ((s * cin + c) * h + iy) * w + ix;
+                                double v = in_kind == 0   ? (double)((const uint8_t *)in)[i]
+                                           : in_kind == 1 ? (double)((const float *)in)[i]
+                                                          : ((const double *)in)[i];
+                                acc = fma(v, kernel[((o * cin + c) * kh + a) * kw + b], acc);
+                            }
+                        }
+                    out[((s * cout + o) * ho + y) * wo + x] = acc;
+                }
+}
+
 /* out[r][o] = sum_f in[r][f] * k[o][f] (src/spyker/cpu/fc.cpp semantics). */
 void orc_fc_f32(const void *in, int32_t is_u8, int64_t rows, int64_t fin, const float *kernel, int64_t fout,
                 float *out)

@@ -91,6 +91,9 @@ void orc_conv_f32(const void *in, int32_t is_u8, int64_t n, int64_t cin, int64_t
 void orc_conv_f64(const void *in, int32_t is_u8, int64_t n, int64_t cin, int64_t h, int64_t w, const float *kernel,
                   int64_t cout, int64_t kh, int64_t kw, int64_t sy, int64_t sx, int64_t pt, int64_t pl, int64_t pb,
                   int64_t pr, double *out, int32_t threads);
+void orc_conv_f64k(const void *in, int32_t in_kind, int64_t n, int64_t cin, int64_t h, int64_t w,
+                   const double *kernel, int64_t cout, int64_t kh, int64_t kw, int64_t sy, int64_t sx, int64_t pt,
+                   int64_t pl, int64_t pb, int64_t pr, double *out, int32_t threads);
 void orc_fc_f32(const void *in, int32_t is_u8, int64_t rows, int64_t fin, const float *kernel, int64_t fout,
                 float *out);
 /* DoG: conv with [2*count][1][k][k], then max(g1-g2, 0) (src/spyker/cpu/dog.cpp:47-60). */

@@ -666,20 +666,20 @@ def layer_step(conv: "Conv", array: torch.Tensor, threshold: float, radius, coun
         pot = inhibit(threshold(conv(array), threshold)); winners = convwta(pot, radius, count)
         out = fire(pot); conv.stdp(array, winners, out); pooled = pool(out, pool_kernel, pool_stride)
 
-    On the fused MFMA path (spk_stdp_layer_fused) in a handful of
-    launches that never materialise the FP32 potentials; a layer outside that
-    path's domain (stride other than 1, more than 32 steps or 256 output
-    channels, Cin*Kh*Kw above 512, a negative threshold, ...) runs the same
-    step on the FP32 plain path instead.  Returns (out, winners,
-    pooled-or-None).  ``potentials``, if given, is an F32 tensor shaped like
-    ``out`` that receives the raw conv potentials of whichever path ran.
+    On the fused MFMA path (spk_stdp_layer_fused, F32 master weights) in a
+    handful of launches that never materialise the FP32 potentials; a layer
+    outside that path's domain (F16 or F64 weights, stride other than 1, more
+    than 32 steps or 256 output channels, Cin*Kh*Kw above 512, a negative
+    threshold, ...) runs the same step on the plain path in the kernel's type
+    instead.  Returns (out, winners,
+    pooled-or-None).  ``potentials``, if given, is a tensor of the kernel's
+    type shaped like ``out`` that receives the raw conv potentials of
+    whichever path ran.
     """
     _dev(array)
     a = _exact(array, 5)
     if a.dtype != torch.uint8:
         raise RuntimeError("Fused layer: spikes must be u8.")
-    if conv.kernel.dtype != torch.float32:
-        raise RuntimeError("Fused layer: the master weights must be f32.")
     n, t, cin, h, w = a.shape
     cout, _, kh, kw = conv.kernel.shape
     sy, sx = conv._stride
@@ -692,9 +692,9 @@ def layer_step(conv: "Conv", array: torch.Tensor, threshold: float, radius, coun
         psh, psw = expand2(pool_stride if pool_stride is not None else pool_kernel)
     if potentials is not None:
         _dev(potentials)
-        if potentials.dtype != torch.float32 or tuple(potentials.shape) != (n, t, cout, ho, wo):
-            raise RuntimeError("potentials must be f32 shaped like the output")
-    if (sy, sx) == (1, 1):
+        if potentials.dtype != conv.kernel.dtype or tuple(potentials.shape) != (n, t, cout, ho, wo):
+            raise RuntimeError("potentials must have the kernel's type and the output's shape")
+    if (sy, sx) == (1, 1) and conv.kernel.dtype == torch.float32:
         out = torch.empty((n, t, cout, ho, wo), dtype=torch.uint8, device=a.device)
         pooled = None
         if pool_kernel is not None:
@@ -726,7 +726,7 @@ def set_fused_arith(mode: str) -> None:
 
 
 def _plain_layer_step(conv, a, th, radius, count, pool_kernel, pool_stride, learn, potentials):
-    """The step as the reference runs it, one FP32 plain-path call per stage."""
+    """The step as the reference runs it, one plain-path call per stage in the kernel's type."""
     pot = conv(a)
     if potentials is not None:
         potentials.copy_(pot)

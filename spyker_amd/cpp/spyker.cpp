@@ -1218,14 +1218,14 @@ LayerStep layerStep(Conv &layer, Tensor input, F64 threshold, Expand2 radius, Si
     input = to5(input);
     onGpu({input, layer.kernel}, "Interface::LayerStep");
     REQUIRE(input.type() == Type::U8, "Interface::LayerStep", "Input spikes must be u8.");
-    REQUIRE(layer.kernel.type() == Type::F32, "Interface::LayerStep", "The master weights must be f32.");
     REQUIRE(!learn || !layer.stdpconfig.empty(), "Interface::LayerStep", "At least one STDP Config must be given.");
     checkItems("Radius", radius.get(), {0, 0});
     REQUIRE(count > 0, "Interface::LayerStep", "Winners count can't be less than 1.");
     if (pool.get() != Shape{0, 0} && poolStride.get() == Shape{0, 0}) poolStride = pool;
-    // the fused FP16-MFMA step; a layer outside its domain (stride, step
-    // count, channels, K, threshold) runs on the plain path
-    if (layer.stride().get() != (Shape{1, 1}))
+    // the fused MFMA step (F32 master weights); a layer outside its domain
+    // (kernel type, stride, step count, channels, K, threshold) runs on the
+    // plain path in the kernel's type
+    if (layer.kernel.type() != Type::F32 || layer.stride().get() != (Shape{1, 1}))
         return plainStep(layer, input, threshold, radius, count, pool, poolStride, learn);
     const Expand4 pad = layer.pad();
     const Shape os = convShape(input.shape(), layer.kernel.shape(), {1, 1}, pad.get());

@@ -1,13 +1,18 @@
-// Direct FP32 cross-correlation for the plain path, plus DoG, Gabor and FC.
+// Direct cross-correlation for the plain path (F16, F32 and F64 kernels),
+// plus DoG, Gabor and FC.
 //
 // Semantics: src/spyker/cpu/conv.cpp (NCHW cross-correlation, zero pad
 // Len4{top,left,bottom,right}, stride), src/spyker/cpu/dog.cpp:39-96,
-// src/spyker/cpu/fc.cpp.  The reference's arithmetic lives in oneDNN/cuDNN;
-// this backend fixes one order for every output: acc = +0, then one fused
-// multiply-add per tap in (cin, ky, kx) nesting order.  Taps that fall in the
-// zero pad are included: fma(0, w, acc) == acc for finite w because acc is
-// never -0 (it starts at +0 and round-to-nearest turns exact cancellation
-// into +0), so the result equals the tap-skipping form of the CPU oracle.
+// src/spyker/cpu/fc.cpp.  As the reference does (opers.cpp:408-421), the
+// input is first converted to the kernel's type and the output has the
+// kernel's type.  The reference's arithmetic lives in oneDNN/cuDNN; this
+// backend fixes one order for every output: acc = +0, then one fused
+// multiply-add per tap in (cin, ky, kx) nesting order, in FP32 for F32 and
+// F16 kernels (an F16 output is that FP32 sum rounded once) and in FP64 for
+// F64 kernels.  Taps that fall in the zero pad are included: fma(0, w, acc)
+// == acc for finite w because acc is never -0 (it starts at +0 and
+// round-to-nearest turns exact cancellation into +0), so the result equals
+// the tap-skipping form of the CPU oracle.
 //
 // Layout: a block owns a TH x TW tile of output pixels of one image and 32
 // output channels.  The input window of the tile (with halo) is staged in LDS
@@ -53,30 +58,39 @@ ConvGeom geometry(int64_t cin, int64_t h, int64_t w, int64_t cout, int64_t kh, i
     return g;
 }
 
-// [cout][cin][kh][kw] -> [cin*kh*kw][cout_pad] (zero filled)
-__global__ void transpose_weights(const float *__restrict__ k, float *__restrict__ kt, int32_t cout, int32_t taps,
+// [cout][cin][kh][kw] -> [cin*kh*kw][cout_pad] (zero filled), widened to W
+// (exact: F16 -> F32, F32 -> F32, F64 -> F64)
+template <typename K, typename W>
+__global__ void transpose_weights(const K *__restrict__ k, W *__restrict__ kt, int32_t cout, int32_t taps,
                                   int32_t cout_pad)
 {
     const int64_t total = int64_t(taps) * cout_pad;
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += int64_t(gridDim.x) * blockDim.x)
     {
         int32_t o = int32_t(i % cout_pad), tap = int32_t(i / cout_pad);
-        kt[i] = o < cout ? k[int64_t(o) * taps + tap] : 0.0f;
+        kt[i] = o < cout ? W(k[int64_t(o) * taps + tap]) : W(0);
     }
 }
 
-template <typename I>
-__device__ __forceinline__ float load_f(const I *p, int64_t i)
+// An input element converted to the kernel's type K, held as the window type W.
+template <typename K, typename W, typename I>
+__device__ __forceinline__ W load_in(const I *p, int64_t i)
 {
-    return float(p[i]);
+    return W(K(p[i]));
 }
 
-// Acc = float (canonical FP32 path) or double (the FP64 check path).
-template <typename I, typename Acc>
-__global__ void __launch_bounds__(THREADS) conv_direct(const I *__restrict__ in, const float *__restrict__ kt,
-                                                       Acc *__restrict__ out, ConvGeom g)
+// The window type of a kernel type: F64 kernels keep F64 inputs.
+template <typename K>
+using WinT = typename std::conditional<std::is_same<K, double>::value, double, float>::type;
+
+// K: kernel type; Acc: float (F16 and F32 kernels) or double (F64 kernels
+// and the FP64 check path of F32 kernels); O: output type.
+template <typename I, typename K, typename Acc, typename O>
+__global__ void __launch_bounds__(THREADS) conv_direct(const I *__restrict__ in, const WinT<K> *__restrict__ kt,
+                                                       O *__restrict__ out, ConvGeom g)
 {
-    __shared__ float win[LDS_FLOATS];
+    using W = WinT<K>;
+    __shared__ W win[LDS_FLOATS];
     const int32_t tile = blockIdx.x, o0 = blockIdx.y * COT;
     const int64_t n = blockIdx.z;
     const int32_t ty0 = (tile / g.tiles_x) * g.th, tx0 = (tile % g.tiles_x) * g.tw;
@@ -101,21 +115,21 @@ __global__ void __launch_bounds__(THREADS) conv_direct(const I *__restrict__ in,
             const int32_t c = i / area, r = (i % area) / g.rw, q = i % g.rw;
             const int32_t iy = iy0 + r, ix = ix0 + q;
             win[i] = (iy >= 0 && iy < g.h && ix >= 0 && ix < g.w)
-                         ? load_f(src, (int64_t(c0 + c) * g.h + iy) * g.w + ix)
-                         : 0.0f;
+                         ? load_in<K, W>(src, (int64_t(c0 + c) * g.h + iy) * g.w + ix)
+                         : W(0);
         }
         __syncthreads();
         if (active)
         {
             for (int32_t c = 0; c < cc; ++c)
             {
-                const float *wrow = win + c * area + (ly * g.sy) * g.rw + lx * g.sx;
-                const float *kc = kt + (int64_t(c0 + c) * taps) * g.cout_pad + o0;
+                const W *wrow = win + c * area + (ly * g.sy) * g.rw + lx * g.sx;
+                const W *kc = kt + (int64_t(c0 + c) * taps) * g.cout_pad + o0;
                 for (int32_t a = 0; a < g.kh; ++a)
                     for (int32_t b = 0; b < g.kw; ++b)
                     {
-                        const float x = wrow[a * g.rw + b];
-                        const float *kk = kc + (a * g.kw + b) * g.cout_pad;
+                        const W x = wrow[a * g.rw + b];
+                        const W *kk = kc + (a * g.kw + b) * g.cout_pad;
 #pragma unroll
                         for (int j = 0; j < COT; ++j)
                         {
@@ -129,10 +143,10 @@ __global__ void __launch_bounds__(THREADS) conv_direct(const I *__restrict__ in,
         }
     }
     if (!active) return;
-    Acc *dst = out + (n * g.cout + o0) * int64_t(g.ho) * g.wo + int64_t(oy) * g.wo + ox;
+    O *dst = out + (n * g.cout + o0) * int64_t(g.ho) * g.wo + int64_t(oy) * g.wo + ox;
 #pragma unroll
     for (int j = 0; j < COT; ++j)
-        if (o0 + j < g.cout) dst[int64_t(j) * g.ho * g.wo] = acc[j];
+        if (o0 + j < g.cout) dst[int64_t(j) * g.ho * g.wo] = O(acc[j]);
 }
 
 // DoG combine: mid is [planes][2][count*ho*wo]; out = max(g1 - g2, 0) as
@@ -187,23 +201,24 @@ __global__ void transpose_fc(const float *__restrict__ k, float *__restrict__ kt
     }
 }
 
-template <typename I, typename Acc>
-void conv_run(const void *in, const float *kernel, void *out, int64_t n, const ConvGeom &g, hipStream_t s)
+template <typename I, typename K, typename Acc, typename O>
+void conv_run(const void *in, const void *kernel, void *out, int64_t n, const ConvGeom &g, hipStream_t s)
 {
+    using W = WinT<K>;
     const int32_t taps = g.cin * g.kh * g.kw;
     // the transposed weights live in the stream's workspace (calls on a stream
     // run in order), so a captured conv records no allocation
-    float *kt = static_cast<float *>(stream_workspace(1, s, int64_t(taps) * g.cout_pad * int64_t(sizeof(float))));
-    transpose_weights<<<flat_grid(int64_t(taps) * g.cout_pad, 256), 256, 0, s>>>(kernel, kt, g.cout, taps,
-                                                                                  g.cout_pad);
+    W *kt = static_cast<W *>(stream_workspace(1, s, int64_t(taps) * g.cout_pad * int64_t(sizeof(W))));
+    transpose_weights<K, W><<<flat_grid(int64_t(taps) * g.cout_pad, 256), 256, 0, s>>>((const K *)kernel, kt, g.cout,
+                                                                                        taps, g.cout_pad);
     SPK_LAUNCHED();
     for (int64_t n0 = 0; n0 < n; n0 += 65535)
     {
         const int64_t nb = n - n0 < 65535 ? n - n0 : 65535;
         dim3 grid(unsigned(g.tiles), unsigned(g.cout_pad / COT), unsigned(nb));
         const I *src = (const I *)in + n0 * g.cin * int64_t(g.h) * g.w;
-        Acc *dst = (Acc *)out + n0 * g.cout * int64_t(g.ho) * g.wo;
-        conv_direct<I, Acc><<<grid, THREADS, 0, s>>>(src, kt, dst, g);
+        O *dst = (O *)out + n0 * g.cout * int64_t(g.ho) * g.wo;
+        conv_direct<I, K, Acc, O><<<grid, THREADS, 0, s>>>(src, kt, dst, g);
         SPK_LAUNCHED();
     }
 }
@@ -232,16 +247,36 @@ int32_t spk_conv(const void *in, int32_t in_type, const void *kernel, int32_t ke
 {
     return guard([&] {
         check_conv(n, cin, h, w, cout, kh, kw, sy, sx, pt, pl, pb, pr);
-        SPK_REQUIRE(kernel_type == SPK_F32, SPK_ERR_UNSUPPORTED,
+        SPK_REQUIRE(kernel_type == SPK_F16 || kernel_type == SPK_F32 || kernel_type == SPK_F64, SPK_ERR_UNSUPPORTED,
                     "Conv: kernel data type " << type_name(kernel_type) << " is not supported.");
+        SPK_REQUIRE(out_type == kernel_type || (kernel_type == SPK_F32 && out_type == SPK_F64), SPK_ERR_UNSUPPORTED,
+                    "Kernel and output must have the same type.");
         if (n == 0) return;
         ConvGeom g = geometry(cin, h, w, cout, kh, kw, sy, sx, pt, pl, pb, pr);
         hipStream_t s = S(stream);
-        const float *k = (const float *)kernel;
-        if (in_type == SPK_U8 && out_type == SPK_F32) return conv_run<uint8_t, float>(in, k, out, n, g, s);
-        if (in_type == SPK_F32 && out_type == SPK_F32) return conv_run<float, float>(in, k, out, n, g, s);
-        if (in_type == SPK_U8 && out_type == SPK_F64) return conv_run<uint8_t, double>(in, k, out, n, g, s);
-        if (in_type == SPK_F32 && out_type == SPK_F64) return conv_run<float, double>(in, k, out, n, g, s);
+        using H = _Float16;
+        if (kernel_type == SPK_F32)
+        {
+            if (in_type == SPK_U8 && out_type == SPK_F32) return conv_run<uint8_t, float, float, float>(in, kernel, out, n, g, s);
+            if (in_type == SPK_F32 && out_type == SPK_F32) return conv_run<float, float, float, float>(in, kernel, out, n, g, s);
+            if (in_type == SPK_F16 && out_type == SPK_F32) return conv_run<H, float, float, float>(in, kernel, out, n, g, s);
+            if (in_type == SPK_F64 && out_type == SPK_F32) return conv_run<double, float, float, float>(in, kernel, out, n, g, s);
+            // the FP64 check path: F32 weights and inputs, FP64 sums
+            if (in_type == SPK_U8 && out_type == SPK_F64) return conv_run<uint8_t, float, double, double>(in, kernel, out, n, g, s);
+            if (in_type == SPK_F32 && out_type == SPK_F64) return conv_run<float, float, double, double>(in, kernel, out, n, g, s);
+        }
+        else if (kernel_type == SPK_F16)
+        {
+            if (in_type == SPK_U8) return conv_run<uint8_t, H, float, H>(in, kernel, out, n, g, s);
+            if (in_type == SPK_F16) return conv_run<H, H, float, H>(in, kernel, out, n, g, s);
+            if (in_type == SPK_F32) return conv_run<float, H, float, H>(in, kernel, out, n, g, s);
+        }
+        else
+        {
+            if (in_type == SPK_U8) return conv_run<uint8_t, double, double, double>(in, kernel, out, n, g, s);
+            if (in_type == SPK_F32) return conv_run<float, double, double, double>(in, kernel, out, n, g, s);
+            if (in_type == SPK_F64) return conv_run<double, double, double, double>(in, kernel, out, n, g, s);
+        }
         SPK_REQUIRE(false, SPK_ERR_UNSUPPORTED,
                     "Conv: data types " << type_name(in_type) << " -> " << type_name(out_type) << " are not supported.");
     });
@@ -286,7 +321,7 @@ int32_t spk_dog(const void *in, const void *kernel, void *out, int64_t planes, i
         hipStream_t s = S(stream);
         const int64_t len = count * int64_t(g.ho) * g.wo;
         Scratch mid(planes * 2 * len * int64_t(sizeof(float)), s);
-        conv_run<float, float>(in, (const float *)kernel, mid.ptr, planes, g, s);
+        conv_run<float, float, float, float>(in, kernel, mid.ptr, planes, g, s);
         dog_combine<<<flat_grid(planes * len, 256), 256, 0, s>>>(mid.as<float>(), (float *)out, planes, len);
         SPK_LAUNCHED();
     });
@@ -302,7 +337,7 @@ int32_t spk_log(const void *in, const void *kernel, void *out, int64_t planes, i
         hipStream_t s = S(stream);
         const int64_t len = count * int64_t(g.ho) * g.wo;
         Scratch mid(planes * 2 * len * int64_t(sizeof(float)), s);
-        conv_run<float, float>(in, (const float *)kernel, mid.ptr, planes, g, s);
+        conv_run<float, float, float, float>(in, kernel, mid.ptr, planes, g, s);
         log_combine<<<flat_grid(planes * len, 256), 256, 0, s>>>(mid.as<float>(), (float *)out, planes, len);
         SPK_LAUNCHED();
     });
@@ -315,7 +350,7 @@ int32_t spk_gabor(const void *in, const void *kernel, void *out, int64_t planes,
         check_conv(planes, 1, h, w, count, k, k, 1, 1, pt, pl, pb, pr);
         if (planes == 0) return;
         ConvGeom g = geometry(1, h, w, count, k, k, 1, 1, pt, pl, pb, pr);
-        conv_run<float, float>(in, (const float *)kernel, out, planes, g, S(stream));
+        conv_run<float, float, float, float>(in, kernel, out, planes, g, S(stream));
     });
 }
 }

@@ -1002,7 +1002,7 @@ int32_t spk_sparse_convstdp(const spk_sparse *in, void *kernel, int32_t kernel_t
                                                                        bits.as<uint32_t>(), slot.as<int2>(),
                                                                        flag.as<int32_t>());
         SPK_LAUNCHED();
-        stdp_apply((float *)kernel, g, cfg, slot.as<int2>(), bits.as<uint32_t>(), flag.as<int32_t>(), false, true, s);
+        stdp_apply(kernel, SPK_F32, g, cfg, slot.as<int2>(), bits.as<uint32_t>(), flag.as<int32_t>(), false, true, s);
         int32_t f = 0;
         SPK_HIP(hipMemcpyAsync(&f, flag.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         SPK_HIP(hipStreamSynchronize(s));

@@ -129,9 +129,9 @@ __global__ void __launch_bounds__(256)
 // keys written by the record kernel (winner.z for conv, winner.x for FC),
 // stages the bits of the matching winners of each chunk in LDS, and applies
 // them in slot order -- (sample, rank) order -- with each weight in a register.
-template <bool SPARSE_RULE>
+template <typename K, bool SPARSE_RULE>
 __global__ void __launch_bounds__(APPLY_THREADS)
-    apply_records(float *__restrict__ kernel, StdpGeom g, StdpConfigs cfg, const int2 *__restrict__ slot,
+    apply_records(K *__restrict__ kernel, StdpGeom g, StdpConfigs cfg, const int2 *__restrict__ slot,
                   const uint32_t *__restrict__ bits, const int32_t *__restrict__ flag, int32_t by_x)
 {
     extern __shared__ uint32_t lds_bits[];  // [APPLY_THREADS][sw]
@@ -144,9 +144,9 @@ __global__ void __launch_bounds__(APPLY_THREADS)
     const int32_t key = blockIdx.x, w0 = blockIdx.y * SLICE;
     const int32_t ksz = by_x ? g.cin : g.cin * g.kh * g.kw, words = (ksz + 31) / 32;
     const int32_t nw = min(SLICE, ksz - w0), sw = (nw + 31) / 32, word0 = w0 / 32;
-    float *kz = kernel + int64_t(key) * ksz + w0;
+    K *kz = kernel + int64_t(key) * ksz + w0;
     const int32_t i = threadIdx.x;
-    float wr = i < nw ? kz[i] : 0.0f;
+    K wr = i < nw ? kz[i] : K(0);
 
     const int64_t total = g.n * g.maxcount;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -186,8 +186,10 @@ __global__ void __launch_bounds__(APPLY_THREADS)
                 const uint32_t cur = word;
                 const spk_stdp_config cf = lcfg[ci];
                 if (q + 1 < m) word = lds_bits[(q + 1) * sw + (i >> 5)], ci = cfgidx[q + 1];
-                wr = SPARSE_RULE ? sparse_stdp_update(wr, (cur >> (i & 31)) & 1u, cf)
-                                 : stdp_update(wr, (cur >> (i & 31)) & 1u, cf);
+                if constexpr (SPARSE_RULE)
+                    wr = sparse_stdp_update(wr, (cur >> (i & 31)) & 1u, cf);
+                else
+                    wr = stdp_update<K>(wr, (cur >> (i & 31)) & 1u, cf);
             }
         }
         __syncthreads();
@@ -230,9 +232,9 @@ int64_t stdp_bits_bytes(const StdpGeom &g, bool fc)
     return bits_words(g, fc) * int64_t(sizeof(uint32_t)) + g.n * g.maxcount * int64_t(sizeof(int2));
 }
 
-void stdp_launch(const uint8_t *in, float *kernel, const uint8_t *out, const StdpGeom &g, const StdpConfigs &cfg,
-                 const spk_winner *winners, const int64_t *counts, uint32_t *bits, int32_t *flag, bool fc,
-                 hipStream_t s)
+void stdp_launch(const uint8_t *in, void *kernel, int32_t ktype, const uint8_t *out, const StdpGeom &g,
+                 const StdpConfigs &cfg, const spk_winner *winners, const int64_t *counts, uint32_t *bits,
+                 int32_t *flag, bool fc, hipStream_t s)
 {
     const int64_t ksz = fc ? g.cin : int64_t(g.cin) * g.kh * g.kw;
     SPK_ARG(ksz < (1LL << 30), "STDP: kernel too large.");
@@ -244,24 +246,36 @@ void stdp_launch(const uint8_t *in, float *kernel, const uint8_t *out, const Std
     else
         conv_records<<<dim3(unsigned(slots)), 256, 0, s>>>(in, out, g, winners, counts, bits, slot, flag);
     SPK_LAUNCHED();
-    stdp_apply(kernel, g, cfg, slot, bits, flag, fc, false, s);
+    stdp_apply(kernel, ktype, g, cfg, slot, bits, flag, fc, false, s);
 }
 
-void stdp_apply(float *kernel, const StdpGeom &g, const StdpConfigs &cfg, const int2 *slot, const uint32_t *bits,
-                const int32_t *flag, bool fc, bool sparse_rule, hipStream_t s)
+template <typename K, bool SPARSE_RULE>
+static void apply_launch(void *kernel, const StdpGeom &g, const StdpConfigs &cfg, const int2 *slot,
+                         const uint32_t *bits, const int32_t *flag, bool fc, hipStream_t s)
 {
     const int64_t ksz = fc ? g.cin : int64_t(g.cin) * g.kh * g.kw;
     const int64_t sw = (ksz < SLICE ? (ksz + 31) / 32 : SLICE / 32);
     const size_t lds = size_t(APPLY_THREADS) * sw * sizeof(uint32_t);
-    const void *fn = sparse_rule ? reinterpret_cast<const void *>(&apply_records<true>)
-                                 : reinterpret_cast<const void *>(&apply_records<false>);
-    if (lds > 65536) SPK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    if (lds > 65536)
+        SPK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&apply_records<K, SPARSE_RULE>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     const dim3 grid(unsigned(g.cout), unsigned(cdiv(ksz, SLICE)));
-    if (sparse_rule)
-        apply_records<true><<<grid, APPLY_THREADS, lds, s>>>(kernel, g, cfg, slot, bits, flag, fc ? 1 : 0);
-    else
-        apply_records<false><<<grid, APPLY_THREADS, lds, s>>>(kernel, g, cfg, slot, bits, flag, fc ? 1 : 0);
+    apply_records<K, SPARSE_RULE><<<grid, APPLY_THREADS, lds, s>>>((K *)kernel, g, cfg, slot, bits, flag, fc ? 1 : 0);
     SPK_LAUNCHED();
+}
+
+void stdp_apply(void *kernel, int32_t ktype, const StdpGeom &g, const StdpConfigs &cfg, const int2 *slot,
+                const uint32_t *bits, const int32_t *flag, bool fc, bool sparse_rule, hipStream_t s)
+{
+    if (sparse_rule)
+    {
+        SPK_ARG(ktype == SPK_F32, "Sparse STDP: the kernel must be F32.");
+        return apply_launch<float, true>(kernel, g, cfg, slot, bits, flag, fc, s);
+    }
+    if (ktype == SPK_F32) return apply_launch<float, false>(kernel, g, cfg, slot, bits, flag, fc, s);
+    if (ktype == SPK_F16) return apply_launch<_Float16, false>(kernel, g, cfg, slot, bits, flag, fc, s);
+    if (ktype == SPK_F64) return apply_launch<double, false>(kernel, g, cfg, slot, bits, flag, fc, s);
+    SPK_REQUIRE(false, SPK_ERR_UNSUPPORTED, "STDP: kernel data type " << type_name(ktype) << " is not supported.");
 }
 }  // namespace spk
 
@@ -276,7 +290,7 @@ int32_t spk_rank_convstdp(const void *in, void *kernel, int32_t kernel_type, con
                           int64_t pb, int64_t pr, void *stream)
 {
     return guard([&] {
-        SPK_REQUIRE(kernel_type == SPK_F32, SPK_ERR_UNSUPPORTED,
+        SPK_REQUIRE(kernel_type == SPK_F16 || kernel_type == SPK_F32 || kernel_type == SPK_F64, SPK_ERR_UNSUPPORTED,
                     "STDP: kernel data type " << type_name(kernel_type) << " is not supported.");
         SPK_ARG(n > 0, "Winners can't be empty.");
         SPK_ARG(t >= 1 && cin >= 1 && h >= 1 && w >= 1 && cout >= 1 && kh >= 1 && kw >= 1, "STDP: invalid shapes.");
@@ -291,7 +305,7 @@ int32_t spk_rank_convstdp(const void *in, void *kernel, int32_t kernel_type, con
         hipStream_t s = S(stream);
         Scratch flag(sizeof(int32_t), s), bits(stdp_bits_bytes(g, false), s);
         clear_flag<<<1, 1, 0, s>>>(flag.as<int32_t>());
-        stdp_launch((const uint8_t *)in, (float *)kernel, (const uint8_t *)out, g, c, winners, counts,
+        stdp_launch((const uint8_t *)in, kernel, kernel_type, (const uint8_t *)out, g, c, winners, counts,
                     bits.as<uint32_t>(), flag.as<int32_t>(), false, s);
         check_flag(flag.as<int32_t>(), s);
     });
@@ -302,7 +316,7 @@ int32_t spk_rank_fcstdp(const void *in, void *kernel, int32_t kernel_type, const
                         const spk_winner *winners, const int64_t *counts, int64_t maxcount, void *stream)
 {
     return guard([&] {
-        SPK_REQUIRE(kernel_type == SPK_F32, SPK_ERR_UNSUPPORTED,
+        SPK_REQUIRE(kernel_type == SPK_F16 || kernel_type == SPK_F32 || kernel_type == SPK_F64, SPK_ERR_UNSUPPORTED,
                     "STDP: kernel data type " << type_name(kernel_type) << " is not supported.");
         SPK_ARG(n > 0, "Winners can't be empty.");
         SPK_ARG(t >= 1 && fin >= 1 && fout >= 1, "STDP: invalid shapes.");
@@ -313,7 +327,7 @@ int32_t spk_rank_fcstdp(const void *in, void *kernel, int32_t kernel_type, const
         hipStream_t s = S(stream);
         Scratch flag(sizeof(int32_t), s), bits(stdp_bits_bytes(g, true), s);
         clear_flag<<<1, 1, 0, s>>>(flag.as<int32_t>());
-        stdp_launch((const uint8_t *)in, (float *)kernel, (const uint8_t *)out, g, c, winners, counts,
+        stdp_launch((const uint8_t *)in, kernel, kernel_type, (const uint8_t *)out, g, c, winners, counts,
                     bits.as<uint32_t>(), flag.as<int32_t>(), true, s);
         check_flag(flag.as<int32_t>(), s);
     });
