@@ -126,7 +126,7 @@ int32_t spk_cast(const void *in, int32_t in_type, void *out, int32_t out_type, i
 int32_t spk_fill(void *x, int32_t type, int64_t n, double value, void *stream);
 
 /* Core::cuda_pad(Dyn3 input, Dyn3 output, Len4 pad, Scalar value), impl.h:236:
- * planes of [h][w] into [h+pt+pb][w+pl+pr] filled with value.  Types: U8, F32. */
+ * planes of [h][w] into [h+pt+pb][w+pl+pr] filled with value.  Types: U8, F16, F32, F64. */
 int32_t spk_pad(const void *in, void *out, int32_t type, int64_t planes, int64_t h, int64_t w, int64_t pt, int64_t pl,
                 int64_t pb, int64_t pr, double value, void *stream);
 
@@ -134,7 +134,8 @@ int32_t spk_pad(const void *in, void *out, int32_t type, int64_t planes, int64_t
 
 /* Core::cuda_rank_pool(Dyn4 input, Dyn4 output, Len2 kernel, Len2 stride,
  * Len4 pad), impl.h:268: per plane, zero pad then window max seeded with the
- * type's lowest value.  Types: U8, F32. */
+ * reference's Limits<T>::min() (the lowest value; for F16 the smallest
+ * positive normal, base.h:761-774).  Types: U8, F16, F32, F64. */
 int32_t spk_rank_pool(const void *in, void *out, int32_t type, int64_t planes, int64_t h, int64_t w, int64_t kh,
                       int64_t kw, int64_t sh, int64_t sw, int64_t pt, int64_t pl, int64_t pb, int64_t pr, void *stream);
 

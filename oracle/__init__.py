@@ -614,6 +614,31 @@ class ReferenceTyped(Reference):
                                               I64(count), F64(th), _ptr(win), _ptr(cnt)), "fcwta")
         return win.reshape(n, count), cnt
 
+    def pool(self, x, kernel, stride=None, pad=(0, 0, 0, 0)):
+        x = np.ascontiguousarray(x)
+        kh, kw = (kernel, kernel) if np.isscalar(kernel) else kernel
+        stride = kernel if stride is None else stride
+        sh, sw = (stride, stride) if np.isscalar(stride) else stride
+        pt, pl, pb, pr = pad
+        h, w = x.shape[-2:]
+        oh, ow = (h + pt + pb - kh) // sh + 1, (w + pl + pr - kw) // sw + 1
+        out = np.zeros(x.shape[:-2] + (oh, ow), x.dtype)
+        planes = int(np.prod(x.shape[:-2]))
+        self._check(self.lib.ref_rank_pool_t(_ptr(x), I32(REF_TYPE[x.dtype]), I64(planes), I64(h), I64(w), _ptr(out),
+                                             I64(kh), I64(kw), I64(sh), I64(sw), I64(pt), I64(pl), I64(pb), I64(pr)),
+                    "pool")
+        return out
+
+    def pad(self, x, pad, value=0.0):
+        x = np.ascontiguousarray(x)
+        pt, pl, pb, pr = pad
+        h, w = x.shape[-2:]
+        out = np.zeros(x.shape[:-2] + (h + pt + pb, w + pl + pr), x.dtype)
+        planes = int(np.prod(x.shape[:-2]))
+        self._check(self.lib.ref_pad_t(_ptr(x), I32(REF_TYPE[x.dtype]), I64(planes), I64(h), I64(w), _ptr(out),
+                                       I64(pt), I64(pl), I64(pb), I64(pr), F64(value)), "pad")
+        return out
+
     def convstdp(self, spk_in, kernel, spk_out, configs, winners, counts, stride=1, pad=(0, 0, 0, 0)):
         spk_in = np.ascontiguousarray(spk_in, np.uint8)
         spk_out = np.ascontiguousarray(spk_out, np.uint8)

@@ -271,6 +271,25 @@ int ref_rank_convstdp_t(const uint8_t *in, int64_t n, int64_t t, int64_t cin, in
     });
 }
 
+int ref_rank_pool_t(const void *in, int32_t type, int64_t planes, int64_t h, int64_t w, void *out, int64_t kh,
+                    int64_t kw, int64_t sh, int64_t sw, int64_t pt, int64_t pl, int64_t pb, int64_t pr)
+{
+    int64_t oh = (h + pt + pb - kh) / sh + 1, ow = (w + pl + pr - kw) / sw + 1;
+    return guard([&] {
+        cpu_rank_pool(Dyn4((void *)in, Type(type), 1, planes, h, w), Dyn4(out, Type(type), 1, planes, oh, ow),
+                      Len2(kh, kw), Len2(sh, sw), Len4(pt, pl, pb, pr));
+    });
+}
+
+int ref_pad_t(const void *in, int32_t type, int64_t planes, int64_t h, int64_t w, void *out, int64_t pt, int64_t pl,
+              int64_t pb, int64_t pr, double value)
+{
+    return guard([&] {
+        cpu_pad(Dyn3((void *)in, Type(type), planes, h, w), Dyn3(out, Type(type), planes, h + pt + pb, w + pl + pr),
+                Len4(pt, pl, pb, pr), Scalar(value));
+    });
+}
+
 int ref_gaussian_kernel(double *kernel, int64_t k, double std)
 {
     return guard([&] { gaussian_kernel(Vec2<F64>(kernel, k, k), std); });

@@ -65,6 +65,32 @@ def inhibit(x, th=0.0):
     return xs.reshape(x.shape)
 
 
+def pool(x, kernel, stride=None, pad=(0, 0, 0, 0)):
+    """cpu/pool.cpp:53-86 on [..., H, W]: zero pad, then each window's maximum
+    in row-major scan order (std::max keeps the earlier of equals) seeded with
+    Limits<T>::min() -- for F16 the smallest positive normal."""
+    dt = x.dtype.type
+    kh, kw = (kernel, kernel) if np.isscalar(kernel) else kernel
+    stride = kernel if stride is None else stride
+    sh, sw = (stride, stride) if np.isscalar(stride) else stride
+    pt, pl, pb, pr = pad
+    xp = np.pad(x, [(0, 0)] * (x.ndim - 2) + [(pt, pb), (pl, pr)])
+    h, w = xp.shape[-2:]
+    oh, ow = (h - kh) // sh + 1, (w - kw) // sw + 1
+    out = np.full(x.shape[:-2] + (oh, ow), lowest(dt), dt)
+    for a in range(kh):
+        for b in range(kw):
+            v = xp[..., a:a + (oh - 1) * sh + 1:sh, b:b + (ow - 1) * sw + 1:sw]
+            out = np.where(out < v, v, out)
+    return out
+
+
+def pad(x, pad, value=0.0):
+    """cpu/pool.cpp:41-51: the planes inside a border of value (in x's type)."""
+    pt, pl, pb, pr = pad
+    return np.pad(x, [(0, 0)] * (x.ndim - 2) + [(pt, pb), (pl, pr)], constant_values=x.dtype.type(value))
+
+
 def _values(x2, th):
     """wta.cpp:44-64 on one sample [T, L]: min and max over the sample,
     diff = (max - min) * 2, each neuron's value level + (x - min) / diff at its

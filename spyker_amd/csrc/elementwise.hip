@@ -207,6 +207,12 @@ int32_t spk_pad(const void *in, void *out, int32_t type, int64_t planes, int64_t
         else if (type == SPK_F32)
             pad_kernel<float><<<flat_grid(total, 256), 256, 0, s>>>((const float *)in, (float *)out, planes, h, w, pt,
                                                                      pl, oh, ow, float(value));
+        else if (type == SPK_F16)
+            pad_kernel<_Float16><<<flat_grid(total, 256), 256, 0, s>>>((const _Float16 *)in, (_Float16 *)out, planes,
+                                                                        h, w, pt, pl, oh, ow, _Float16(value));
+        else if (type == SPK_F64)
+            pad_kernel<double><<<flat_grid(total, 256), 256, 0, s>>>((const double *)in, (double *)out, planes, h, w,
+                                                                      pt, pl, oh, ow, value);
         else
             SPK_REQUIRE(false, SPK_ERR_UNSUPPORTED, "Pad: data type " << type_name(type) << " is not supported.");
         SPK_LAUNCHED();

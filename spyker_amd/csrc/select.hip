@@ -25,6 +25,18 @@ struct Lowest<float>
 {
     static __device__ float v() { return -FLT_MAX; }
 };
+template <>
+struct Lowest<double>
+{
+    static __device__ double v() { return -DBL_MAX; }
+};
+// Limits<half>::min() (base.h:772): the smallest positive normal, so a window
+// of zeros or negatives pools to 2^-14, as the reference's does
+template <>
+struct Lowest<_Float16>
+{
+    static __device__ _Float16 v() { return _Float16(6.103515625e-05f); }
+};
 
 template <typename T>
 __global__ void pool_kernel(const T *__restrict__ in, T *__restrict__ out, int64_t planes, int32_t h, int32_t w,
@@ -402,6 +414,14 @@ int32_t spk_rank_pool(const void *in, void *out, int32_t type, int64_t planes, i
         else if (type == SPK_F32)
             pool_kernel<float><<<flat_grid(total, 256), 256, 0, s>>>(
                 (const float *)in, (float *)out, planes, int32_t(h), int32_t(w), int32_t(oh), int32_t(ow),
+                int32_t(kh), int32_t(kw), int32_t(sh), int32_t(sw), int32_t(pt), int32_t(pl));
+        else if (type == SPK_F16)
+            pool_kernel<_Float16><<<flat_grid(total, 256), 256, 0, s>>>(
+                (const _Float16 *)in, (_Float16 *)out, planes, int32_t(h), int32_t(w), int32_t(oh), int32_t(ow),
+                int32_t(kh), int32_t(kw), int32_t(sh), int32_t(sw), int32_t(pt), int32_t(pl));
+        else if (type == SPK_F64)
+            pool_kernel<double><<<flat_grid(total, 256), 256, 0, s>>>(
+                (const double *)in, (double *)out, planes, int32_t(h), int32_t(w), int32_t(oh), int32_t(ow),
                 int32_t(kh), int32_t(kw), int32_t(sh), int32_t(sw), int32_t(pt), int32_t(pl));
         else
             SPK_REQUIRE(false, SPK_ERR_UNSUPPORTED, "Pool: data type " << type_name(type) << " is not supported.");

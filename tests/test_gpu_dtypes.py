@@ -157,3 +157,14 @@ def test_f64_to_f16_rounds_once(dev):
     x = T(vals, dev)
     spk.call("spk_cast", spk._ptr(x), spk._lib.F64, spk._ptr(out), spk._lib.F16, vals.size, spk._stream(x))
     assert np.array_equal(H(out), want)
+
+
+@pytest.mark.parametrize("dt", [np.float16, np.float64])
+def test_pool_pad(dev, dt):
+    rng = np.random.default_rng(76)
+    x = np.where(rng.random((2, 3, 4, 9, 11)) < 0.3, 0.0, rng.normal(0, 1, (2, 3, 4, 9, 11)))
+    x = np.where(rng.random(x.shape) < 0.1, -0.0, x).astype(dt)
+    for k, s, p in [(2, 2, (0, 0, 0, 0)), (3, 2, (1, 0, 2, 1))]:
+        got, want = H(spk.pool(T(x, dev), k, s, p)), typed.pool(x, k, s, p)
+        assert np.array_equal(got, want) and np.array_equal(np.signbit(got), np.signbit(want))
+    assert np.array_equal(H(spk.pad(T(x, dev), (2, 0, 1, 3), -1.5)), typed.pad(x, (2, 0, 1, 3), -1.5))

@@ -141,3 +141,25 @@ def test_conv_f16_rounds_the_f32_sum_once(orc):
     want = orc.conv(x, k.astype(np.float32), 1, (1, 1, 1, 1))  # exact: multiples of 1/64 below 2^11/64
     assert np.array_equal(got, want.astype(np.float16))
     assert np.array_equal(got.astype(np.float32), want)
+
+
+@pytest.mark.parametrize("dt", [np.float16, np.float32, np.float64])
+def test_typed_pool_pad(orc, dt):
+    rng = np.random.default_rng(65)
+    x = np.where(rng.random((3, 4, 9, 11)) < 0.3, 0.0, rng.normal(0, 1, (3, 4, 9, 11)))
+    x = np.where(rng.random(x.shape) < 0.1, -0.0, x).astype(dt)
+    for k, s, p in [(2, 2, (0, 0, 0, 0)), (3, 2, (1, 0, 2, 1))]:
+        got = typed.pool(x, k, s, p)
+        if dt == np.float32:
+            want = orc.pool(x, k, s, p)
+            assert np.array_equal(got, want) and np.array_equal(np.signbit(got), np.signbit(want))
+        elif oracle.reference_available():
+            want = oracle.ReferenceTyped().pool(x, k, s, p)
+            assert np.array_equal(got, want) and np.array_equal(np.signbit(got), np.signbit(want))
+    if dt == np.float16:
+        assert typed.pool(np.full((1, 2, 2), -1.0, dt), 2)[0, 0, 0] == np.float16(2.0 ** -14)
+    got = typed.pad(x, (2, 0, 1, 3), -1.5)
+    if dt == np.float32:
+        assert np.array_equal(got, orc.pad(x, (2, 0, 1, 3), -1.5))
+    elif oracle.reference_available():
+        assert np.array_equal(got, oracle.ReferenceTyped().pad(x, (2, 0, 1, 3), -1.5))
