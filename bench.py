@@ -713,6 +713,7 @@ def main():
                      "kernel": kname, "kernel_ms": kern_s * 1e3, "launches_timed": launches.value,
                      "flop_per_launch": FLOP_PER_IT * N * T_STEPS},
     }
+    line["config"]["channels_per_wave"] = spk.fused_tuned_channels(conv_f, T_STEPS, H, H)
     if a.arith == "int8":
         line["roofline"]["note"] = ("achieved counts the conv's own multiply-adds; the int8 path issues two int8 "
                                     "products for each (high and low weight byte), so its ceiling for this conv "

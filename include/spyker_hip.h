@@ -440,6 +440,27 @@ int32_t spk_stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, 
  * (the region kernel) always use FP16.  Process-wide. */
 int32_t spk_set_fused_arith(int32_t mode);
 
+/* The whole-sample kernel's output channels a wave, its tile shape (the
+ * reference picks its conv algorithm per shape at run time,
+ * cuda/conv.cu:93-104,161-178; this is the fused step's counterpart): 32
+ * (two waves a SIMD) or 64 (one wave a SIMD, half the operand reads a
+ * multiply).  With tuning on (mode 1, the default) the first call of each
+ * shape (T, Cin, H, W, Cout, Kh, Kw, pad, arithmetic) times both on its first
+ * min(n, 2 x CUs) samples and keeps the faster for later calls; the two give
+ * the same bytes.  No tuning under a stream capture, in the allocation-free
+ * entry point or with the potentials output (32 then, or the kept choice).
+ * Process-wide. */
+int32_t spk_set_fused_tuning(int32_t mode);
+
+/* Forces 32 or 64 channels a wave for every later call (0: the tuned choice,
+ * else 32).  Process-wide. */
+int32_t spk_set_fused_channels(int32_t channels);
+
+/* The tuned channels a wave for a shape under the current arithmetic, or 0
+ * where no call of that shape has been tuned. */
+int32_t spk_fused_tuned_channels(int64_t t, int64_t cin, int64_t h, int64_t w, int64_t cout, int64_t kh, int64_t kw,
+                                 int64_t pt, int64_t pl, int64_t pb, int64_t pr, int32_t *channels);
+
 /* spk_stdp_layer_fused for callers that cannot allocate device memory (code
  * under stream capture, tight serving loops, harnesses that forbid
  * hipMalloc).  No counterpart in the reference.  Same arguments and results

@@ -725,6 +725,26 @@ def set_fused_arith(mode: str) -> None:
     call("spk_set_fused_arith", modes[mode])
 
 
+def set_fused_tuning(on: bool) -> None:
+    """Whether the fused step times 32 and 64 output channels a wave on the
+    first call of each shape and keeps the faster (on by default)."""
+    call("spk_set_fused_tuning", 1 if on else 0)
+
+
+def set_fused_channels(channels: int) -> None:
+    """Forces the fused step's output channels a wave (32 or 64; 0 returns to the tuned choice)."""
+    call("spk_set_fused_channels", int(channels))
+
+
+def fused_tuned_channels(conv: "Conv", t: int, h: int, w: int) -> int:
+    """The tuned channels a wave for this layer on [*, t, Cin, h, w] input (0 if not tuned yet)."""
+    cout, cin, kh, kw = conv.kernel.shape
+    pt, pl, pb, pr = conv._pad
+    out = C.c_int32(0)
+    call("spk_fused_tuned_channels", t, cin, h, w, cout, kh, kw, pt, pl, pb, pr, C.byref(out))
+    return out.value
+
+
 def _plain_layer_step(conv, a, th, radius, count, pool_kernel, pool_stride, learn, potentials):
     """The step as the reference runs it, one plain-path call per stage in the kernel's type."""
     pot = conv(a)

@@ -94,6 +94,9 @@ SIGNATURES = {
     "spk_layer_plan_select": [_p, _i32, _i32],
     "spk_layer_plan_destroy": [_p],
     "spk_set_fused_arith": [_i32],
+    "spk_set_fused_tuning": [_i32],
+    "spk_set_fused_channels": [_i32],
+    "spk_fused_tuned_channels": [_i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _p],
     "spk_kernel_timer": [_i32],
     "spk_kernel_timer_read": [_p, _p],
     "spk_random_seed": [_i64],

@@ -43,9 +43,10 @@
 //     passes issue the same MFMA sequence on the same operands as the step
 //     loop, so they reproduce its potentials bit for bit.  Two barriers a
 //     tile exchange the per-pixel partials between waves.
-#include <mutex>
-
+#include <array>
 #include <atomic>
+#include <map>
+#include <mutex>
 
 #include "fused.hpp"
 
@@ -784,7 +785,7 @@ __device__ __forceinline__ void step_iter8(const int4v (&a)[RT][S8][2], const in
 // steps without barriers, the two gathered tile-end passes, the in-loop zero
 // stores and the scatter of 1s) with i8 MFMAs on the raw stage bytes.
 template <int RT, int S8, bool WIDE, bool DIAG>
-__global__ void __launch_bounds__(64 * MAX_WAVES)
+__global__ void __launch_bounds__(layer_threads<RT>())
     fused_layer8(const uint8_t *__restrict__ in, const int4v *__restrict__ frag, const int32_t *__restrict__ hdr,
                  Summary *__restrict__ summary, float *__restrict__ pot_out, uint8_t *__restrict__ fs_out,
                  uint8_t *__restrict__ out, uint8_t *__restrict__ pooled, FusedGeom g)
@@ -1230,14 +1231,14 @@ void launch_layer(const uint8_t *in, const half8 *frag, Summary *summary, float 
 // fixed point where its K-units fit (else FP16), 1 = FP16 operands.
 std::atomic<int32_t> g_arith{0};
 
-template <int S8, bool WIDE, bool DIAG>
+template <int RT, int S8, bool WIDE, bool DIAG>
 void launch_layer8(const uint8_t *in, const int4v *frag, const int32_t *hdr, Summary *summary, float *pot,
                    uint8_t *fs, uint8_t *out, uint8_t *pooled, const FusedGeom &g, size_t lds, hipStream_t s)
 {
     static bool configured = false;
     if (!configured)
     {
-        SPK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&fused_layer8<2, S8, WIDE, DIAG>),
+        SPK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&fused_layer8<RT, S8, WIDE, DIAG>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, 163840));
         configured = true;
     }
@@ -1251,7 +1252,7 @@ void launch_layer8(const uint8_t *in, const int4v *frag, const int32_t *hdr, Sum
         SPK_HIP(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence));
         SPK_HIP(hipEventRecord(e0, s));
     }
-    fused_layer8<2, S8, WIDE, DIAG><<<dim3(grid), dim3(64 * g.waves), lds, s>>>(in, frag, hdr, summary, pot, fs, out,
+    fused_layer8<RT, S8, WIDE, DIAG><<<dim3(grid), dim3(64 * g.waves), lds, s>>>(in, frag, hdr, summary, pot, fs, out,
                                                                               pooled, g);
     SPK_LAUNCHED();
     if (tm.on)
@@ -1261,6 +1262,8 @@ void launch_layer8(const uint8_t *in, const int4v *frag, const int32_t *hdr, Sum
     }
 }
 
+// DIAG (the potentials output) is built for 32 channels a wave only
+template <int RT>
 void dispatch_layer8(int steps, bool wide, const uint8_t *in, const int4v *frag, const int32_t *hdr,
                      Summary *summary, float *pot, uint8_t *fs, uint8_t *out, uint8_t *pooled, const FusedGeom &g,
                      size_t lds, hipStream_t s)
@@ -1270,10 +1273,10 @@ void dispatch_layer8(int steps, bool wide, const uint8_t *in, const int4v *frag,
 #define CASE(k)                                                                                                   \
     case k:                                                                                                       \
         if (wide)                                                                                                 \
-            return pot ? launch_layer8<k, true, true>(in, frag, hdr, summary, pot, fs, out, pooled, g, lds, s)     \
-                       : launch_layer8<k, true, false>(in, frag, hdr, summary, pot, fs, out, pooled, g, lds, s);  \
-        return pot ? launch_layer8<k, false, true>(in, frag, hdr, summary, pot, fs, out, pooled, g, lds, s)        \
-                   : launch_layer8<k, false, false>(in, frag, hdr, summary, pot, fs, out, pooled, g, lds, s);
+            return pot ? launch_layer8<RT, k, true, RT == 2>(in, frag, hdr, summary, pot, fs, out, pooled, g, lds, s) \
+                       : launch_layer8<RT, k, true, false>(in, frag, hdr, summary, pot, fs, out, pooled, g, lds, s); \
+        return pot ? launch_layer8<RT, k, false, RT == 2>(in, frag, hdr, summary, pot, fs, out, pooled, g, lds, s)    \
+                   : launch_layer8<RT, k, false, false>(in, frag, hdr, summary, pot, fs, out, pooled, g, lds, s);
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE
     default: SPK_ARG(false, "Fused layer: int8 K-steps out of range.");
@@ -1289,15 +1292,94 @@ void dispatch_layer(int steps, bool paired, const uint8_t *in, const half8 *frag
 #define CASE(k) \
     case k:                                                                                        \
         if (paired)                                                                                \
-            return pot ? launch_layer<RT, k, true, true>(in, frag, summary, pot, fs, out, pooled, g, lds, s)      \
+            return pot ? launch_layer<RT, k, true, RT == 2>(in, frag, summary, pot, fs, out, pooled, g, lds, s)   \
                        : launch_layer<RT, k, true, false>(in, frag, summary, pot, fs, out, pooled, g, lds, s);    \
-        return pot ? launch_layer<RT, k, false, true>(in, frag, summary, pot, fs, out, pooled, g, lds, s)         \
+        return pot ? launch_layer<RT, k, false, RT == 2>(in, frag, summary, pot, fs, out, pooled, g, lds, s)      \
                    : launch_layer<RT, k, false, false>(in, frag, summary, pot, fs, out, pooled, g, lds, s);
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
         CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
 #undef CASE
     default: SPK_ARG(false, "Fused layer: K-steps out of range.");
     }
+}
+// The whole-sample kernel's channels a wave: spk_set_fused_channels forces
+// 32 or 64 (0: the tuned choice, else 32); spk_set_fused_tuning(1) (the
+// default) times both on the first call of each shape and keeps the faster.
+std::atomic<int32_t> g_tune{1};
+std::atomic<int32_t> g_force_channels{0};
+
+using TuneKey = std::array<int64_t, 12>;
+struct Tuned
+{
+    std::mutex mu;
+    std::map<TuneKey, int> rt;
+};
+Tuned &tuned()
+{
+    static Tuned t;
+    return t;
+}
+
+TuneKey tune_key(bool use8, const FusedGeom &g)
+{
+    return TuneKey{use8 ? 8 : 16, g.t, g.cin, g.h, g.w, g.cout, g.kh, g.kw, g.pt, g.pl, g.ph, g.pw};
+}
+
+// The channels a wave (as rt = channels / 16) for this call.  Tuning runs
+// both choices on the first min(n, 2 * CUs) samples (outputs the call
+// rewrites afterwards), twice each after a warm run, timed with events;
+// the two produce the same bytes (each potential is the same chain of
+// MFMAs), so the choice is about time only.  No tuning under a stream
+// capture, for the allocation-free entry point or with the potentials output
+// (built for 32 channels a wave only).
+template <typename SetRt, typename Run>
+int choose_channels(bool use8, const FusedGeom &g, bool potentials, bool noalloc, hipStream_t s, SetRt &&set_rt,
+                    Run &&run)
+{
+    if (potentials) return 2;
+    const int forced = g_force_channels.load();
+    if (forced) return forced / 16;
+    const TuneKey key = tune_key(use8, g);
+    Tuned &t = tuned();
+    {
+        std::lock_guard<std::mutex> lock(t.mu);
+        auto it = t.rt.find(key);
+        if (it != t.rt.end()) return it->second;
+    }
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    SPK_HIP(hipStreamIsCapturing(s, &cap));
+    if (!g_tune.load() || noalloc || cap != hipStreamCaptureStatusNone) return 2;
+    FusedGeom gt = g;
+    gt.n = std::min<int32_t>(g.n, 2 * cu_count());
+    hipEvent_t e0, e1;
+    SPK_HIP(hipEventCreate(&e0));
+    SPK_HIP(hipEventCreate(&e1));
+    float best = 0.0f;
+    int choice = 2;
+    for (int r : {2, 4})
+    {
+        set_rt(r);
+        FusedGeom gr = gt;
+        gr.waves = g.waves, gr.stage_fast = g.stage_fast;
+        run(r, gr);  // warm
+        float ms = 0.0f;
+        for (int rep = 0; rep < 2; ++rep)
+        {
+            SPK_HIP(hipEventRecord(e0, s));
+            run(r, gr);
+            SPK_HIP(hipEventRecord(e1, s));
+            SPK_HIP(hipEventSynchronize(e1));
+            float m = 0.0f;
+            SPK_HIP(hipEventElapsedTime(&m, e0, e1));
+            ms = rep == 0 ? m : std::min(ms, m);
+        }
+        if (r == 2 || ms < best) best = ms, choice = r;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::lock_guard<std::mutex> lock(t.mu);
+    t.rt[key] = choice;
+    return choice;
 }
 }  // namespace
 }  // namespace spk
@@ -1311,6 +1393,41 @@ int32_t spk_set_fused_arith(int32_t mode)
     return guard([&] {
         SPK_ARG(mode == 0 || mode == 1, "Fused layer: arithmetic mode must be 0 (int8 fixed point) or 1 (FP16).");
         g_arith.store(mode);
+    });
+}
+
+int32_t spk_set_fused_tuning(int32_t mode)
+{
+    return guard([&] {
+        SPK_ARG(mode == 0 || mode == 1, "Fused layer: tuning mode must be 0 (off) or 1 (time on first call).");
+        g_tune.store(mode);
+    });
+}
+
+int32_t spk_set_fused_channels(int32_t channels)
+{
+    return guard([&] {
+        SPK_ARG(channels == 0 || channels == 32 || channels == 64,
+                "Fused layer: channels a wave must be 0 (tuned), 32 or 64.");
+        g_force_channels.store(channels);
+    });
+}
+
+int32_t spk_fused_tuned_channels(int64_t t, int64_t cin, int64_t h, int64_t w, int64_t cout, int64_t kh, int64_t kw,
+                                 int64_t pt, int64_t pl, int64_t pb, int64_t pr, int32_t *channels)
+{
+    return guard([&] {
+        SPK_ARG(channels, "Fused layer: channels must not be null.");
+        FusedGeom g{};
+        g.t = int32_t(t), g.cin = int32_t(cin), g.h = int32_t(h), g.w = int32_t(w), g.cout = int32_t(cout);
+        g.kh = int32_t(kh), g.kw = int32_t(kw), g.pt = int32_t(pt), g.pl = int32_t(pl);
+        g.ph = int32_t(h + pt + pb), g.pw = int32_t(w + pl + pr);
+        const int64_t cin_pad = cdiv(cin, 8) * 8, upr = cdiv(kw * cin_pad, 16);
+        const bool use8 = g_arith.load() == 0 && cdiv(kh * upr, 4) <= MAX_STEPS8;
+        Tuned &tt = tuned();
+        std::lock_guard<std::mutex> lock(tt.mu);
+        auto it = tt.rt.find(tune_key(use8, g));
+        *channels = it == tt.rt.end() ? 0 : 16 * it->second;
     });
 }
 
@@ -1393,10 +1510,6 @@ void stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, int64_t
         g.cin_pad = int32_t(cdiv(cin, 8) * 8);
         g.groups = int32_t(kh * kw * (g.cin_pad / 8));
         g.steps = int32_t(cdiv(g.groups, 4));
-        // 32 channels a wave, two waves per SIMD (measured faster on the layer-2
-        // workload than 64 channels a wave with one wave per SIMD)
-        const int rt = 2;
-        g.waves = int32_t(cdiv(cout, 16 * rt));
         g.tiles = int32_t(cdiv(ho * wo, COLS));
         // the zero-bordered stage holds every tap of every output pixel
         const int64_t sstep = int64_t(g.ph) * g.pw * g.cin_pad;
@@ -1434,8 +1547,15 @@ void stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, int64_t
         // each tile zeroes 1/tiles of the sample's pooled block, in 16-byte multiples
         SPK_ARG(t * cout * ho * wo < (1LL << 31), "Fused layer: a sample's spikes exceed 2 GiB.");
         g.zp = int32_t(cdiv(cdiv(t * cout * pho * pwo, g.tiles), 16) * 16);
-        g.stage_fast = (h * w) % 4 == 0 && (reinterpret_cast<uintptr_t>(in) & 3) == 0 &&
-                       (16 * g.waves) % (g.cin_pad / 4) == 0 && 16 * g.waves / (g.cin_pad / 4) <= h * w / 4;
+        // 16 * rt channels a wave: rt 2 (32 channels, two waves per SIMD, the
+        // default) or rt 4 (64 channels, one wave per SIMD); the tuner times
+        // both on the first call of a shape
+        auto set_rt = [&](int r) {
+            g.waves = int32_t(cdiv(cout, 16 * r));
+            g.stage_fast = (h * w) % 4 == 0 && (reinterpret_cast<uintptr_t>(in) & 3) == 0 &&
+                           (16 * g.waves) % (g.cin_pad / 4) == 0 && 16 * g.waves / (g.cin_pad / 4) <= h * w / 4;
+        };
+        set_rt(2);
         SPK_ARG(int64_t(cout) * ho * wo < (1LL << 31), "Fused layer: output plane too large.");
 
         const StdpConfigs cfg = learn ? pack_configs(configs, nconfigs) : StdpConfigs{};
@@ -1443,7 +1563,8 @@ void stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, int64_t
 
         hipStream_t s = S(stream);
         const int64_t hw = ho * wo, hwi = h * w;
-        const int64_t row_tiles = int64_t(g.waves) * rt, ksz = cin * kh * kw;
+        // fragment rows for either choice of channels a wave
+        const int64_t row_tiles = std::max<int64_t>(cdiv(cout, 32) * 2, cdiv(cout, 64) * 4), ksz = cin * kh * kw;
         const FusedStdp fp{n * count, int32_t(count), g.cin, g.cin_pad, g.h, g.w, g.kh, g.kw, g.pt, g.pl,
                            g.wo, int32_t(hw), g.t, int32_t(cdiv(ksz, 32))};
         // the call's buffers, carved from the stream's workspace: weight
@@ -1464,32 +1585,41 @@ void stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, int64_t
         int32_t *wkey = reinterpret_cast<int32_t *>(ws + offs[2]);
         uint8_t *fsm = learn ? ws + offs[3] : nullptr;
         uint32_t *bits = reinterpret_cast<uint32_t *>(ws + offs[4]);
-        if (use8)
-        {
-            // the scale exponent in the first 256 bytes, the fragments after
-            int32_t *hdr = reinterpret_cast<int32_t *>(ws + offs[0]);
-            int4v *frag8 = reinterpret_cast<int4v *>(ws + offs[0] + 256);
-            const int64_t entries = row_tiles * steps8 * 2 * 64;
-            prep_fragments8<<<dim3(unsigned(std::min<int64_t>(cdiv(entries, 1024), 32))), 1024, 0, s>>>(
-                (const float *)kernel, frag8, hdr, g.cout, g.cin, g.kh, g.kw, g.cin_pad, g.upr, g.units, steps8,
-                int32_t(row_tiles));
-            SPK_LAUNCHED();
-            // the kernel also writes the spikes and pooled spikes of each sample
-            dispatch_layer8(steps8, g.cin_pad % 16 == 0, (const uint8_t *)in, frag8, hdr, summary,
-                            (float *)potentials, fsm, (uint8_t *)out, (uint8_t *)pooled, g, size_t(lds), s);
-        }
-        else
-        {
-            // pairs of K-steps share 16-byte reads when 16-byte units stay inside a tap
-            const bool paired = g.cin_pad % 16 == 0 && g.steps >= 2;
-            prep_fragments<<<dim3(unsigned(cdiv(row_tiles * g.steps * 64, 256))), 256, 0, s>>>(
-                (const float *)kernel, frag, g.cout, g.cin, g.kh, g.kw, g.cin_pad, g.groups, g.steps,
-                int32_t(row_tiles), paired ? 1 : 0);
-            SPK_LAUNCHED();
-            // the kernel also writes the spikes and pooled spikes of each sample
-            dispatch_layer<2>(g.steps, paired, (const uint8_t *)in, frag, summary, (float *)potentials, fsm,
-                              (uint8_t *)out, (uint8_t *)pooled, g, size_t(lds), s);
-        }
+        // prepares the weight fragments for rt and runs the whole-sample kernel
+        // over the first gg.n samples
+        int32_t *hdr = reinterpret_cast<int32_t *>(ws + offs[0]);
+        int4v *frag8 = reinterpret_cast<int4v *>(ws + offs[0] + 256);
+        auto run_layer = [&](int r, const FusedGeom &gg) {
+            const int64_t rtiles = int64_t(gg.waves) * r;
+            if (use8)
+            {
+                // the scale exponent in the first 256 bytes, the fragments after
+                const int64_t entries = rtiles * steps8 * 2 * 64;
+                prep_fragments8<<<dim3(unsigned(std::min<int64_t>(cdiv(entries, 1024), 32))), 1024, 0, s>>>(
+                    (const float *)kernel, frag8, hdr, g.cout, g.cin, g.kh, g.kw, g.cin_pad, g.upr, g.units, steps8,
+                    int32_t(rtiles));
+                SPK_LAUNCHED();
+                // the kernel also writes the spikes and pooled spikes of each sample
+                (r == 4 ? dispatch_layer8<4> : dispatch_layer8<2>)(
+                    steps8, g.cin_pad % 16 == 0, (const uint8_t *)in, frag8, hdr, summary, (float *)potentials, fsm,
+                    (uint8_t *)out, (uint8_t *)pooled, gg, size_t(lds), s);
+            }
+            else
+            {
+                // pairs of K-steps share 16-byte reads when 16-byte units stay inside a tap
+                const bool paired = g.cin_pad % 16 == 0 && g.steps >= 2;
+                prep_fragments<<<dim3(unsigned(cdiv(rtiles * g.steps * 64, 256))), 256, 0, s>>>(
+                    (const float *)kernel, frag, g.cout, g.cin, g.kh, g.kw, g.cin_pad, g.groups, g.steps,
+                    int32_t(rtiles), paired ? 1 : 0);
+                SPK_LAUNCHED();
+                (r == 4 ? dispatch_layer<4> : dispatch_layer<2>)(g.steps, paired, (const uint8_t *)in, frag, summary,
+                                                                  (float *)potentials, fsm, (uint8_t *)out,
+                                                                  (uint8_t *)pooled, gg, size_t(lds), s);
+            }
+        };
+        const int rt = choose_channels(use8, g, potentials != nullptr, scratch != nullptr, s, set_rt, run_layer);
+        set_rt(rt);
+        run_layer(rt, g);
         // convwta: one wave per sample when the plane fits a wave's registers
         if (hw <= 256)
             fused_select_wave<4><<<dim3(unsigned(cdiv(n, 4))), 256, 0, s>>>(summary, g.t, g.ho, g.wo, int32_t(ry),

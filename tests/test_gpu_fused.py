@@ -229,6 +229,59 @@ def test_fused_graph_replay(dev):
         assert torch.equal(conv.kernel, e[4])
 
 
+@pytest.mark.parametrize("mode", ARITH)
+@pytest.mark.parametrize("shape", [
+    # n, t, cin, h, w, cout
+    (6, 15, 30, 14, 14, 250),
+    (5, 9, 4, 13, 11, 32),
+    (4, 24, 12, 10, 9, 100),
+])
+def test_channels_a_wave_give_the_same_bytes(dev, orc, mode, shape):
+    """32 and 64 output channels a wave (the tuner's two tile shapes) give the
+    same spikes, pooled spikes, winners and weights, equal to the oracle's."""
+    n, t, cin, h, w, cout = shape
+    rng = np.random.default_rng(81)
+    x = spikes(rng, n, t, cin, h, w, 0.35)
+    kern = (rng.integers(40, 64, (cout, cin, 3, 3)) / 64).astype(np.float32)
+    th = float(np.floor(0.3 * cin * 9 * 0.5))
+    got = []
+    with arith(mode):
+        for ch in (32, 64):
+            spk.set_fused_channels(ch)
+            try:
+                conv = spk.Conv(cin, cout, 3, pad=1, device=dev)
+                conv.kernel = T(kern, dev)
+                conv.stdpconfig = [spk.STDPConfig(0.004, -0.003)]
+                out, wn, pooled = spk.layer_step(conv, T(x, dev), th, 1, 5, 2, 2)
+                got.append((H(out), H(pooled), H(wn.data), H(wn.counts), H(conv.kernel)))
+            finally:
+                spk.set_fused_channels(0)
+    for a, b in zip(*got):
+        assert np.array_equal(a, b)
+    pot = orc.conv(x.reshape(n * t, cin, h, w), kern, 1, (1, 1, 1, 1)).reshape(n, t, cout, h, w)
+    pot = orc.inhibit(orc.threshold(pot, th))
+    ow, oc = orc.convwta(pot, 1, 5)
+    assert oc.sum() > 0 and np.array_equal(got[0][3], oc)
+    assert np.array_equal(got[0][0], orc.fire(pot))
+
+
+def test_tuner_keeps_a_choice_per_shape(dev):
+    rng = np.random.default_rng(82)
+    n, t, cin, h, w, cout = 40, 11, 8, 12, 12, 96
+    x = T(spikes(rng, n, t, cin, h, w, 0.3), dev)
+    conv = spk.Conv(cin, cout, 3, pad=1, device=dev)
+    conv.kernel = T((rng.integers(30, 64, (cout, cin, 3, 3)) / 64).astype(np.float32), dev)
+    conv.stdpconfig = [spk.STDPConfig(0.004, -0.003)]
+    spk.set_fused_tuning(True)
+    assert spk.fused_tuned_channels(conv, t, h, w) == 0
+    first = spk.layer_step(conv, x, 20.0, 1, 4, learn=False)
+    choice = spk.fused_tuned_channels(conv, t, h, w)
+    assert choice in (32, 64)
+    again = spk.layer_step(conv, x, 20.0, 1, 4, learn=False)
+    assert torch.equal(first[0], again[0]) and torch.equal(first[1].data, again[1].data)
+    assert spk.fused_tuned_channels(conv, t, h, w) == choice
+
+
 def test_two_layer_stack_graph_replay(dev):
     """The two-layer stroke-image network's training step for layer 2 (layer
     1 forward: conv 4->30 5x5 pad 2, fire 15, pool 2; then the layer-2 step:
