@@ -187,6 +187,20 @@ __device__ __forceinline__ float wave_max(float v)
     return v;
 }
 
+__device__ __forceinline__ float wave_sum(float v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ int32_t wave_sum_i32(int32_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
 // Block-wide min and max in the reference's std::min/std::max form
 // (min = v < min ? v : min; max = max < v ? v : max).
 __device__ __forceinline__ void block_minmax(const float *__restrict__ x, int64_t len, float &mn, float &mx, float *lds /*[32]*/)

@@ -223,8 +223,10 @@ constexpr int layer_threads()
 
 // Stages sample n of the [n][t][cin][h][w] input as [t][y][x][cin_pad] bytes
 // inside the zero border of the conv's padding (written once per kernel).
+// Every staged byte is XORed with each byte of `bias` (0, or 0x80 for the
+// int8 kernel, whose border holds 0x80 too).
 __device__ __forceinline__ void stage_sample(const uint8_t *__restrict__ in, int32_t n, const FusedGeom &g,
-                                             uint8_t *stage)
+                                             uint8_t *stage, uint32_t bias)
 {
     const int32_t hwi = g.h * g.w;
     if (g.stage_fast)
@@ -277,7 +279,8 @@ __device__ __forceinline__ void stage_sample(const uint8_t *__restrict__ in, int
                 const uint32_t r1 = __builtin_amdgcn_perm(y, x, sel1);
                 const uint32_t y2 = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(r1), 0x4E, 0xF, 0xF, true));  // lane^2
                 if (lin + u * step < total)
-                    dst[((st * g.ph + sy + g.pt) * g.pw + sx + g.pl) * D + c4] = __builtin_amdgcn_perm(y2, r1, sel2);
+                    dst[((st * g.ph + sy + g.pt) * g.pw + sx + g.pl) * D + c4] =
+                        __builtin_amdgcn_perm(y2, r1, sel2) ^ bias;
                 sq += step, sy += dy, sx += dx;
                 sy = sx >= g.w ? sy + 1 : sy;
                 sx = sx >= g.w ? sx - g.w : sx;
@@ -291,22 +294,23 @@ __device__ __forceinline__ void stage_sample(const uint8_t *__restrict__ in, int
     else
     {
         // any other shape: scatter the [t][cin][h][w] input byte by byte
-        // (the padded channels stay 0)
+        // (the padded channels keep the border's value)
         const uint8_t *src = in + int64_t(n) * g.t * g.cin * hwi;
         const int32_t chw = g.cin * hwi, total = g.t * chw;
         for (int32_t i = int32_t(threadIdx.x); i < total; i += int32_t(blockDim.x))
         {
             const int32_t tt = i / chw, r = i - tt * chw, c = r / hwi, pix = r - c * hwi, yy = pix / g.w;
             const int32_t xx = pix - yy * g.w;
-            stage[((tt * g.ph + yy + g.pt) * g.pw + xx + g.pl) * g.cin_pad + c] = src[i];
+            stage[((tt * g.ph + yy + g.pt) * g.pw + xx + g.pl) * g.cin_pad + c] = src[i] ^ uint8_t(bias);
         }
     }
 }
 
 // The first spiking step of every staged input byte of sample n (t when it
-// never spikes), 8 channels of one pixel per thread: [n][h*w][cin_pad].
+// never spikes), 8 channels of one pixel per thread: [n][h*w][cin_pad]; the
+// stage's bytes are the input's XORed with `bias`.
 __device__ __forceinline__ void first_spike_map(const uint8_t *stage, int32_t n, const FusedGeom &g,
-                                                uint8_t *__restrict__ fs_out)
+                                                uint8_t *__restrict__ fs_out, uint32_t bias)
 {
     const int32_t hwi = g.h * g.w, cg = g.cin_pad / 8;
     const uint32_t tb = uint32_t(g.t) * 0x01010101u;
@@ -318,8 +322,8 @@ __device__ __forceinline__ void first_spike_map(const uint8_t *stage, int32_t n,
         for (int32_t tt = 0; tt < g.t; ++tt)
         {
             const uint2 v = *reinterpret_cast<const uint2 *>(sp + tt * g.sstep);
-            first_nonzero(lo, seen_lo, v.x, tt);
-            first_nonzero(hi, seen_hi, v.y, tt);
+            first_nonzero(lo, seen_lo, v.x ^ bias, tt);
+            first_nonzero(hi, seen_hi, v.y ^ bias, tt);
         }
         *reinterpret_cast<uint2 *>(fs_out + (int64_t(n) * hwi * cg + u) * 8) = make_uint2(lo, hi);
     }
@@ -395,9 +399,9 @@ __global__ void __launch_bounds__(layer_threads<RT>())
     for (int32_t n = blockIdx.x; n < g.n; n += gridDim.x)
     {
         __syncthreads();
-        stage_sample(in, n, g, stage);
+        stage_sample(in, n, g, stage, 0u);
         __syncthreads();
-        if (fs_out) first_spike_map(stage, n, g, fs_out);
+        if (fs_out) first_spike_map(stage, n, g, fs_out, 0u);
         // the previous sample's 1s, after this sample's staging so that its
         // loads do not wait behind these stores; smap still holds that
         // sample's map (the first tile's end rewrites it, after two barriers)
@@ -600,15 +604,22 @@ __global__ void __launch_bounds__(layer_threads<RT>())
 
 // -- the fixed-point int8 form of the whole-sample kernel --------------------
 //
-// Spikes are 0 or 1, so the stage's bytes can feed v_mfma_i32_16x16x64_i8 as
-// they are, with no conversion.  The weights become fixed point: w_q =
-// rint(w * 2^e) with the largest e that keeps every |w_q| <= 32639 (chosen per
-// call from the kernel's max |w|), split into two int8 planes w_q = 256 * hi +
-// lo (lo in [-128, 127]); a potential is the exact integer 256 * (spikes . hi)
-// + (spikes . lo) times 2^-e.  Against FP16 weights this rounds each weight
-// to 2^-e / 2 <= max|w| / 65278 (FP16: 2^-11 of the weight), and any sum of
-// weights exact at that scale (the dyadic test weights) is exact.  The fire
-// test "pot > th" is the integer test pot_q > floor(th * 2^e), exact.
+// The stage's bytes feed v_mfma_i32_16x16x64_i8 with no conversion but a
+// bias: byte b is staged as b ^ 0x80, which the signed int8 MFMA reads as b -
+// 128 for every b in [0, 255] (the zero border holds 0x80, a true 0).  The
+// weights become fixed point: w_q = rint(w * 2^e), split into two int8 planes
+// w_q = 256 * hi + lo (lo in [-128, 127]), so the chain yields 256 * (b . hi)
+// + (b . lo) - 128 * A, A = sum of the channel's w_q: the exact integer
+// potential v_q less a per-channel constant.  The kernel compares against
+// thc = floor(th * 2^e) - 128 * A instead: d = thc - raw = th_q - v_q
+// (modulo 2^32, the accumulators' own wrap-around), whose sign bit is "v > th"
+// and whose minimum over the steps gives the maximum potential.  e is the
+// largest exponent that keeps every |w_q| <= 32639 and th_q - v_q inside
+// int32 for any input bytes (255 * the channel's positive and negative weight
+// sums, with one unit of rounding slack a weight), so the test is exact for
+// every U8 input.  Against FP16 weights this rounds each weight to 2^-e / 2
+// (FP16: 2^-11 of the weight), and any sum of weights exact at that scale
+// (the dyadic test weights) is exact.
 //
 // K-units: a window row (kw pixels x cin_pad channels) is contiguous in the
 // stage; unit u covers 16 bytes of row u / upr at byte 16 * (u % upr), bytes
@@ -632,11 +643,28 @@ __device__ __forceinline__ int32_t fixed_exponent(float m)
 }
 
 // pot_q > floor(th * 2^e) <=> pot_q * 2^-e > th for integers pot_q; thresholds
-// past 2^30 (no potential reaches them) clamp there.
+// past 2^30 clamp there (no potential reaches them: |pot_q| < 2^31 - 2^30 by
+// the choice of e).
 __device__ __forceinline__ int32_t threshold_fixed(float th, int32_t e)
 {
     const float x = floorf(ldexpf(th, e));
     return x >= 1073741824.0f ? 1073741824 : int32_t(x);
+}
+
+// Lowers e until the fire test's difference th_q - v_q stays inside int32 for
+// any U8 input: v_q lies in [-255 * N_q, 255 * P_q], P_q and N_q being the
+// largest per-channel sums of the positive and negative w_q (each w_q within
+// 1/2 of w * 2^e, so P_q <= P * 2^e + K / 2; K of slack covers that and the
+// float sums), and th_q in [0, 2^30].
+__device__ __forceinline__ int32_t bounded_exponent(int32_t e, float th, double pmax, double nmax, int32_t k)
+{
+    const double lim = 2147483647.0;
+    for (; e > -120; --e)
+    {
+        const double s = ldexp(1.0, e);
+        if (255.0 * (pmax * s + k) <= lim && double(threshold_fixed(th, e)) + 255.0 * (nmax * s + k) <= lim) break;
+    }
+    return e;
 }
 
 __device__ __forceinline__ int32_t unit_off(const FusedGeom &g, int32_t u)
@@ -645,16 +673,24 @@ __device__ __forceinline__ int32_t unit_off(const FusedGeom &g, int32_t u)
     return (uu / g.upr) * g.pw * g.cin_pad + (uu % g.upr) * 16;
 }
 
+// The int8 header: e, then from HDR8_THC on the biased threshold thc of each
+// of 256 output channels (th_q for channels past cout, whose weights are 0).
+constexpr int HDR8_THC = 256;
+constexpr int HDR8_BYTES = 4 * (HDR8_THC + 256);
+
 // [row tile][K step][plane][lane] int8 fragments of the FP32 masters (plane 0
-// = hi, 1 = lo), and hdr[0] = e.  Every block reduces max|w| itself, so one
-// launch needs no second pass.
+// = hi, 1 = lo), and the header.  Every block reduces max|w| and the
+// per-channel positive and negative weight sums itself, so one launch needs
+// no second pass; block 0 writes the header.
 __global__ void __launch_bounds__(1024)
-    prep_fragments8(const float *__restrict__ k, int4v *__restrict__ frag, int32_t *__restrict__ hdr, int32_t cout,
-                    int32_t cin, int32_t kh, int32_t kw, int32_t cin_pad, int32_t upr, int32_t units, int32_t steps,
-                    int32_t row_tiles)
+    prep_fragments8(const float *__restrict__ k, int4v *__restrict__ frag, int32_t *__restrict__ hdr, float th,
+                    int32_t cout, int32_t cin, int32_t kh, int32_t kw, int32_t cin_pad, int32_t upr, int32_t units,
+                    int32_t steps, int32_t row_tiles)
 {
-    __shared__ float part[16];
-    const int64_t nw = int64_t(cout) * cin * kh * kw;
+    __shared__ float part[16], ppart[16], npart[16];
+    const int32_t ksz = cin * kh * kw;
+    const int64_t nw = int64_t(cout) * ksz;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
     // four independent maxima over 16-byte loads keep many loads in flight
     float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f;
     int64_t i0 = 0;
@@ -672,18 +708,44 @@ __global__ void __launch_bounds__(1024)
         i0 = n4 * 4;
     }
     for (int64_t i = i0 + threadIdx.x; i < nw; i += blockDim.x) m0 = fmaxf(m0, fabsf(k[i]));
+    // per-channel sums of the positive and of the negative weights, one wave
+    // a channel, and their maxima over the channels
+    float pm = 0.0f, nm = 0.0f;
+    for (int32_t c = wv; c < cout; c += nwv)
+    {
+        float ps = 0.0f, ns = 0.0f;
+        for (int32_t i = lane; i < ksz; i += 64)
+        {
+            const float x = k[int64_t(c) * ksz + i];
+            ps += fmaxf(x, 0.0f), ns += fmaxf(-x, 0.0f);
+        }
+        pm = fmaxf(pm, wave_sum(ps)), nm = fmaxf(nm, wave_sum(ns));
+    }
     float m = wave_max(fmaxf(fmaxf(m0, m1), fmaxf(m2, m3)));
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    if (lane == 0) part[wv] = m, ppart[wv] = pm, npart[wv] = nm;
     __syncthreads();
-    m = part[0];
-    for (int i = 1; i < int(blockDim.x >> 6); ++i) m = fmaxf(m, part[i]);
-    const int32_t e = fixed_exponent(m);
-    if (blockIdx.x == 0 && threadIdx.x == 0) hdr[0] = e;
+    m = part[0], pm = ppart[0], nm = npart[0];
+    for (int i = 1; i < nwv; ++i) m = fmaxf(m, part[i]), pm = fmaxf(pm, ppart[i]), nm = fmaxf(nm, npart[i]);
+    const int32_t e = bounded_exponent(fixed_exponent(m), th, double(pm), double(nm), ksz);
+    if (blockIdx.x == 0)
+    {
+        // thc = th_q - 128 * A (modulo 2^32), A the channel's sum of w_q
+        const uint32_t thq = uint32_t(threshold_fixed(th, e));
+        for (int32_t c = wv; c < 256; c += nwv)
+        {
+            int32_t a = 0;
+            if (c < cout)
+                for (int32_t i = lane; i < ksz; i += 64) a += int32_t(rintf(ldexpf(k[int64_t(c) * ksz + i], e)));
+            a = wave_sum_i32(a);
+            if (lane == 0) hdr[HDR8_THC + c] = int32_t(thq - 128u * uint32_t(a));
+        }
+        if (threadIdx.x == 0) hdr[0] = e;
+    }
     const int32_t total = row_tiles * steps * 2 * 64;
     for (int32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += gridDim.x * blockDim.x)
     {
-        const int32_t lane = idx % 64, p = (idx / 64) % 2, s = (idx / 128) % steps, rt = idx / (128 * steps);
-        const int32_t o = rt * 16 + lane % 16, u = 4 * s + lane / 16, ky = u / upr;
+        const int32_t lane2 = idx % 64, p = (idx / 64) % 2, s = (idx / 128) % steps, rt = idx / (128 * steps);
+        const int32_t o = rt * 16 + lane2 % 16, u = 4 * s + lane2 / 16, ky = u / upr;
         uint32_t word[4] = {0, 0, 0, 0};
         for (int b = 0; b < 16; ++b)
         {
@@ -730,23 +792,32 @@ __device__ __forceinline__ void chain8(const int4v (&a)[RT][S8][2], const int4v 
         }
 }
 
-// value_update on an exact fixed-point potential: the sign of th - v is "v >
-// th" (no overflow: |v| < 2^24, th <= 2^30).
-__device__ __forceinline__ void value_update8(int32_t hi, int32_t lo, int32_t th, uint32_t &mask, int32_t &mx)
+// th_q - v_q for the biased chain's planes (hi, lo) and the channel's
+// biased threshold thc, computed modulo 2^32 like the accumulators: exact,
+// since the choice of e keeps th_q - v_q inside int32.
+__device__ __forceinline__ int32_t fixed_diff(int32_t hi, int32_t lo, int32_t thc)
 {
-    const int32_t v = hi * 256 + lo;
-    mask = __builtin_amdgcn_alignbit(mask, uint32_t(th - v), 31);
-    mx = max(mx, v);
+    return int32_t(uint32_t(thc) - (uint32_t(hi) * 256u + uint32_t(lo)));
+}
+
+// value_update on an exact fixed-point potential: d = th_q - v_q has its sign
+// bit set exactly when v > th, and the running minimum of d is th_q minus the
+// running maximum of v (started at v = 0).
+__device__ __forceinline__ void value_update8(int32_t hi, int32_t lo, int32_t thc, uint32_t &mask, int32_t &mn)
+{
+    const int32_t d = fixed_diff(hi, lo, thc);
+    mask = __builtin_amdgcn_alignbit(mask, uint32_t(d), 31);
+    mn = min(mn, d);
 }
 
 template <int RT>
-__device__ __forceinline__ void update_all8(const int4v (&uh)[RT], const int4v (&ul)[RT], int32_t th,
-                                            uint32_t (&mask)[RT][4], int32_t (&mx)[RT][4])
+__device__ __forceinline__ void update_all8(const int4v (&uh)[RT], const int4v (&ul)[RT], const int32_t (&thc)[RT][4],
+                                            uint32_t (&mask)[RT][4], int32_t (&mn)[RT][4])
 {
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) value_update8(uh[rt][i], ul[rt][i], th, mask[rt][i], mx[rt][i]);
+        for (int i = 0; i < 4; ++i) value_update8(uh[rt][i], ul[rt][i], thc[rt][i], mask[rt][i], mn[rt][i]);
 }
 
 // step_iter of the int8 kernel: the MFMA pairs of step k+1 on `rin`, step
@@ -754,9 +825,10 @@ __device__ __forceinline__ void update_all8(const int4v (&uh)[RT], const int4v (
 // into `rout` right after the MFMAs that read the same unit of `rin`.
 template <int RT, int S8, bool WIDE>
 __device__ __forceinline__ void step_iter8(const int4v (&a)[RT][S8][2], const int4v (&rin)[S8], int4v (&ch)[RT],
-                                           int4v (&cl)[RT], const int4v (&uh)[RT], const int4v (&ul)[RT], int32_t th,
-                                           uint32_t (&mask)[RT][4], int32_t (&mx)[RT][4], int4v (&rout)[S8],
-                                           const uint8_t *next, const int32_t (&off)[S8])
+                                           int4v (&cl)[RT], const int4v (&uh)[RT], const int4v (&ul)[RT],
+                                           const int32_t (&thc)[RT][4], uint32_t (&mask)[RT][4],
+                                           int32_t (&mn)[RT][4], int4v (&rout)[S8], const uint8_t *next,
+                                           const int32_t (&off)[S8])
 {
     constexpr int V = 4 * RT, SPAN = S8 * RT - RT;
 #pragma unroll
@@ -772,11 +844,12 @@ __device__ __forceinline__ void step_iter8(const int4v (&a)[RT][S8][2], const in
 #pragma unroll
             for (int v = 0; v < V; ++v)
                 if (SPAN > 0 && RT + (v * SPAN) / V == s * RT + rt)
-                    value_update8(uh[v / 4][v % 4], ul[v / 4][v % 4], th, mask[v / 4][v % 4], mx[v / 4][v % 4]);
+                    value_update8(uh[v / 4][v % 4], ul[v / 4][v % 4], thc[v / 4][v % 4], mask[v / 4][v % 4],
+                                  mn[v / 4][v % 4]);
         }
         rout[s] = load16<WIDE>(next + off[s]);
     }
-    if (SPAN <= 0) update_all8(uh, ul, th, mask, mx);
+    if (SPAN <= 0) update_all8(uh, ul, thc, mask, mn);
     __builtin_amdgcn_sched_barrier(0);
 }
 
@@ -809,22 +882,29 @@ __global__ void __launch_bounds__(layer_threads<RT>())
 #pragma unroll
             for (int p = 0; p < 2; ++p) a[r][s][p] = frag[(((w * RT + r) * S8 + s) * 2 + p) * 64 + lane];
     const int32_t e = hdr[0];
+    int32_t thc[RT][4];  // the biased thresholds of the lane's channels
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) thc[r][i] = hdr[HDR8_THC + o0 + r * 16 + i];
     __builtin_amdgcn_s_waitcnt(0);  // as in fused_layer: no pending fragment loads in the tile loop
     const float scale = ldexpf(1.0f, -e);
-    const int32_t th = threshold_fixed(g.thf, e);
+    const uint32_t thq = uint32_t(threshold_fixed(g.thf, e));
     int32_t off[S8];
 #pragma unroll
     for (int s = 0; s < S8; ++s) off[s] = unit_off(g, 4 * s + q);
+    // the border (and padded channels) hold the bias, a true 0
+    constexpr uint32_t BIAS = 0x80808080u;
     for (int32_t i = threadIdx.x; i < g.stage_bytes / 16; i += blockDim.x)
-        reinterpret_cast<uint4 *>(stage)[i] = make_uint4(0, 0, 0, 0);
+        reinterpret_cast<uint4 *>(stage)[i] = make_uint4(BIAS, BIAS, BIAS, BIAS);
 
     int32_t prev = -1;  // the sample whose 1s are not stored yet
     for (int32_t n = blockIdx.x; n < g.n; n += gridDim.x)
     {
         __syncthreads();
-        stage_sample(in, n, g, stage);
+        stage_sample(in, n, g, stage, BIAS);
         __syncthreads();
-        if (fs_out) first_spike_map(stage, n, g, fs_out);
+        if (fs_out) first_spike_map(stage, n, g, fs_out, BIAS);
         if (prev >= 0 && (out || pooled)) scatter_sample(prev, smap, out, pooled, g);
 
         const int64_t s_bytes = int64_t(g.t) * g.cout * hw, p_bytes = int64_t(g.t) * g.cout * g.pho * g.pwo;
@@ -848,11 +928,11 @@ __global__ void __launch_bounds__(layer_threads<RT>())
             const int32_t pc = pvalid ? p : hw - 1, y = pc / g.wo, x = pc % g.wo;
             const uint8_t *pb = stage + (y * g.pw + x) * g.cin_pad;  // the pixel's window at step 0
             uint32_t mask[RT][4];
-            int32_t mx[RT][4];
+            int32_t mn[RT][4];  // th_q - (running maximum of the potential, from 0)
 #pragma unroll
             for (int r = 0; r < RT; ++r)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) mask[r][i] = 0, mx[r][i] = 0;
+                for (int i = 0; i < 4; ++i) mask[r][i] = 0, mn[r][i] = int32_t(thq);
 
             {
                 const int32_t last = g.t - 1;
@@ -868,7 +948,8 @@ __global__ void __launch_bounds__(layer_threads<RT>())
                                 const int32_t o = o0 + r * 16 + i;
                                 if (o < g.cout)
                                     pot_out[((int64_t(n) * g.t + k) * g.cout + o) * hw + p] =
-                                        float(uh[r][i] * 256 + ul[r][i]) * scale;
+                                        float(int32_t(thq - uint32_t(fixed_diff(uh[r][i], ul[r][i], thc[r][i])))) *
+                                        scale;
                             }
                 };
                 load_units<S8, WIDE>(rA, pb, off);
@@ -878,24 +959,24 @@ __global__ void __launch_bounds__(layer_threads<RT>())
                 int64_t zu = u0 + int32_t(threadIdx.x);
                 for (; k + 2 < g.t; k += 2)
                 {
-                    step_iter8<RT, S8, WIDE>(a, rB, hB, lB, hA, lA, th, mask, mx, rA, pb + (k + 2) * g.sstep, off);
+                    step_iter8<RT, S8, WIDE>(a, rB, hB, lB, hA, lA, thc, mask, mn, rA, pb + (k + 2) * g.sstep, off);
                     diag(k, hA, lA);
                     if (zu < u1) zero16(s_body + zu);
                     zu += blockDim.x;
-                    step_iter8<RT, S8, WIDE>(a, rA, hA, lA, hB, lB, th, mask, mx, rB,
+                    step_iter8<RT, S8, WIDE>(a, rA, hA, lA, hB, lB, thc, mask, mn, rB,
                                              pb + min(k + 3, last) * g.sstep, off);
                     diag(k + 1, hB, lB);
                 }
                 if (k + 1 < g.t)
                 {
-                    step_iter8<RT, S8, WIDE>(a, rB, hB, lB, hA, lA, th, mask, mx, rA, pb + last * g.sstep, off);
+                    step_iter8<RT, S8, WIDE>(a, rB, hB, lB, hA, lA, thc, mask, mn, rA, pb + last * g.sstep, off);
                     diag(k, hA, lA);
-                    update_all8(hB, lB, th, mask, mx);
+                    update_all8(hB, lB, thc, mask, mn);
                     diag(k + 1, hB, lB);
                 }
                 else
                 {
-                    update_all8(hA, lA, th, mask, mx);
+                    update_all8(hA, lA, thc, mask, mn);
                     diag(k, hA, lA);
                 }
                 for (; zu < u1; zu += blockDim.x) zero16(s_body + zu);
@@ -913,7 +994,8 @@ __global__ void __launch_bounds__(layer_threads<RT>())
             }
             __syncthreads();
             // pass 1 at inhibit's step (see fused_layer); the thresholded
-            // values compare exactly as floats (|v| < 2^24)
+            // values compare as floats (exactly for |v_q| < 2^24, as with
+            // spike inputs in {0, 1})
             int4v gh[RT], gl[RT];
             int32_t ti;  // inhibit's step at this pixel
             {
@@ -931,8 +1013,8 @@ __global__ void __launch_bounds__(layer_threads<RT>())
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
                     {
-                        const int32_t v = gh[r2][i] * 256 + gl[r2][i];
-                        const float xp = v > th ? float(v) : 0.0f;
+                        const int32_t d = fixed_diff(gh[r2][i], gl[r2][i], thc[r2][i]);
+                        const float xp = d < 0 ? float(int32_t(thq - uint32_t(d))) : 0.0f;
                         const bool take = xp > bv;
                         bv = take ? xp : bv;
                         bi = take ? o0 + r2 * 16 + i : bi;
@@ -952,12 +1034,12 @@ __global__ void __launch_bounds__(layer_threads<RT>())
             const int32_t cr = (cs % (16 * RT)) / 16, cq = (cs >> 2) & 3, ci = cs & 3;
             const bool own = cs / (16 * RT) == w && cq == q;
             uint32_t om = 0;
-            int32_t ox = 0, first = 0;  // first: c*'s value at inhibit's step, from pass 1
+            int32_t ox = 0, first = 0;  // th_q - c*'s maximum; first: th_q - c*'s value at inhibit's step
 #pragma unroll
             for (int r = 0; r < RT; ++r)
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    if (cr == r && ci == i) om = mask[r][i], ox = mx[r][i], first = gh[r][i] * 256 + gl[r][i];
+                    if (cr == r && ci == i) om = mask[r][i], ox = mn[r][i], first = fixed_diff(gh[r][i], gl[r][i], thc[r][i]);
             const int32_t my_t0 = om ? g.t - 32 + int32_t(__builtin_clz(om)) : 0;
             const int32_t t0 = __shfl(my_t0, cq * 16 + j, 64);
             // pass 2: the value of c* at its first firing step, run only where
@@ -971,13 +1053,14 @@ __global__ void __launch_bounds__(layer_threads<RT>())
                 for (int r2 = 0; r2 < RT; ++r2)
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
-                        if (cr == r2 && ci == i) first = gh[r2][i] * 256 + gl[r2][i];
+                        if (cr == r2 && ci == i) first = fixed_diff(gh[r2][i], gl[r2][i], thc[r2][i]);
             }
             if (own && pvalid)
             {
                 const uint32_t fwd = om ? __builtin_bitreverse32(om) >> (32 - g.t) : 0u;
                 summary[int64_t(n) * hw + p] =
-                    Summary{cs, fwd, om ? float(first) * scale : 0.0f, om ? float(ox) * scale : 0.0f};
+                    Summary{cs, fwd, om ? float(int32_t(thq - uint32_t(first))) * scale : 0.0f,
+                            om ? float(int32_t(thq - uint32_t(ox))) * scale : 0.0f};
                 smap[p] = make_uint2(uint32_t(cs), fwd);
             }
         }
@@ -1568,9 +1651,9 @@ void stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, int64_t
         const FusedStdp fp{n * count, int32_t(count), g.cin, g.cin_pad, g.h, g.w, g.kh, g.kw, g.pt, g.pl,
                            g.wo, int32_t(hw), g.t, int32_t(cdiv(ksz, 32))};
         // the call's buffers, carved from the stream's workspace: weight
-        // fragments (FP16, or a 256-byte header and int8 planes), summaries,
+        // fragments (FP16, or the int8 header and planes), summaries,
         // winner keys, first-spike map, STDP bit records
-        const int64_t frag_bytes = use8 ? 256 + row_tiles * steps8 * 2 * 64 * 16 : row_tiles * g.steps * 64 * 16;
+        const int64_t frag_bytes = use8 ? HDR8_BYTES + row_tiles * steps8 * 2 * 64 * 16 : row_tiles * g.steps * 64 * 16;
         const int64_t sizes[5] = {frag_bytes, n * hw * int64_t(sizeof(Summary)),
                                   n * count * int64_t(sizeof(int32_t)), learn ? n * hwi * g.cin_pad : 0,
                                   learn ? n * count * fp.words * int64_t(sizeof(uint32_t)) : 0};
@@ -1588,15 +1671,15 @@ void stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, int64_t
         // prepares the weight fragments for rt and runs the whole-sample kernel
         // over the first gg.n samples
         int32_t *hdr = reinterpret_cast<int32_t *>(ws + offs[0]);
-        int4v *frag8 = reinterpret_cast<int4v *>(ws + offs[0] + 256);
+        int4v *frag8 = reinterpret_cast<int4v *>(ws + offs[0] + HDR8_BYTES);
         auto run_layer = [&](int r, const FusedGeom &gg) {
             const int64_t rtiles = int64_t(gg.waves) * r;
             if (use8)
             {
-                // the scale exponent in the first 256 bytes, the fragments after
+                // the header (scale exponent, biased thresholds), the fragments after
                 const int64_t entries = rtiles * steps8 * 2 * 64;
                 prep_fragments8<<<dim3(unsigned(std::min<int64_t>(cdiv(entries, 1024), 32))), 1024, 0, s>>>(
-                    (const float *)kernel, frag8, hdr, g.cout, g.cin, g.kh, g.kw, g.cin_pad, g.upr, g.units, steps8,
+                    (const float *)kernel, frag8, hdr, g.thf, g.cout, g.cin, g.kh, g.kw, g.cin_pad, g.upr, g.units, steps8,
                     int32_t(rtiles));
                 SPK_LAUNCHED();
                 // the kernel also writes the spikes and pooled spikes of each sample

@@ -107,6 +107,40 @@ def test_fused_exact_on_dyadic_weights(dev, orc, case, mode):
         assert np.array_equal(H(pooled), o_pool)
 
 
+@pytest.mark.parametrize("case", [
+    # n, t, cin, h, w, cout, k, pad, weight numerators lo..hi (of 64), byte density
+    (4, 8, 4, 12, 12, 32, 3, (1, 1, 1, 1), (-64, 64), 0.3),
+    # K = 270 with weights near 1: 255 * K * w * 2^15 sits just under 2^31,
+    # the edge of the int8 kernel's range for its scale
+    (3, 10, 30, 10, 10, 64, 3, (1, 1, 1, 1), (56, 64), 0.5),
+    (3, 6, 8, 9, 11, 48, 5, (2, 2, 2, 2), (-20, 64), 0.2),
+])
+@pytest.mark.parametrize("mode", ARITH)
+def test_fused_any_input_bytes(dev, orc, case, mode):
+    """U8 inputs are not limited to spikes: conv sees every byte value (the
+    reference casts them to float, opers.cpp:408-423), here 0..255 at random
+    steps, with weights of both signs; on dyadic weights every sum is exact,
+    so the fused step must equal the oracle bit for bit."""
+    n, t, cin, h, w, cout, k, pad, (lo, hi), dens = case
+    rng = np.random.default_rng(17)
+    x = (rng.integers(1, 256, (n, t, cin, h, w)) * (rng.random((n, t, cin, h, w)) < dens)).astype(np.uint8)
+    x[0, 0, 0, 0, :3] = 255
+    kern = (rng.integers(lo, hi, (cout, cin, k, k)) / 64).astype(np.float32)
+    pot = orc.conv(x.reshape(n * t, cin, h, w), kern, 1, pad)
+    th = float(np.quantile(pot, 0.9)) + 0.5 / 64
+    conv = spk.Conv(cin, cout, k, pad=pad, device=dev)
+    conv.kernel = T(kern, dev)
+    conv.stdpconfig = [spk.STDPConfig(0.004, -0.003)]
+    with arith(mode):
+        out, wn, pooled = spk.layer_step(conv, T(x, dev), th, 1, 5, 2, 2)
+    o_out, o_win, o_cnt, o_k, o_pool = oracle_step(orc, x, kern, th, 1, 5, pad, (2, 2))
+    assert o_cnt.sum() > 0
+    assert np.array_equal(H(out), o_out)
+    check_winners(wn, o_win, o_cnt)
+    assert np.array_equal(H(conv.kernel), o_k)
+    assert np.array_equal(H(pooled), o_pool)
+
+
 @pytest.mark.parametrize("config", [
     # positive, negative, stabilize, lower, upper
     (0.3, -0.2, True, 0.0, 1.0),      # lower +0: min/max then +0
