@@ -607,13 +607,14 @@ __global__ void __launch_bounds__(layer_threads<RT>())
 // The stage's bytes feed v_mfma_i32_16x16x64_i8 with no conversion but a
 // bias: byte b is staged as b ^ 0x80, which the signed int8 MFMA reads as b -
 // 128 for every b in [0, 255] (the zero border holds 0x80, a true 0).  The
-// weights become fixed point: w_q = rint(w * 2^e), split into two int8 planes
-// w_q = 256 * hi + lo (lo in [-128, 127]), so the chain yields 256 * (b . hi)
-// + (b . lo) - 128 * A, A = sum of the channel's w_q: the exact integer
-// potential v_q less a per-channel constant.  The kernel compares against
-// thc = floor(th * 2^e) - 128 * A instead: d = thc - raw = th_q - v_q
-// (modulo 2^32, the accumulators' own wrap-around), whose sign bit is "v > th"
-// and whose minimum over the steps gives the maximum potential.  e is the
+// weights become fixed point: w_q = rint(w * 2^e), and the fragments hold
+// -w_q split into two int8 planes 256 * hi + lo (lo in [-128, 127]).  The hi
+// chain starts at 0 and the lo chain at the channel's biased threshold thc =
+// floor(th * 2^e) - 128 * A (A = the channel's sum of w_q), so 256 * hi_sum +
+// lo_sum = thc - (v_q - 128 * A) = th_q - v_q =: d, v_q the exact integer
+// potential (modulo 2^32, the accumulators' own wrap-around).  The sign bit
+// of d is "v > th", and its minimum over the steps gives the maximum
+// potential.  e is the
 // largest exponent that keeps every |w_q| <= 32639 and th_q - v_q inside
 // int32 for any input bytes (255 * the channel's positive and negative weight
 // sums, with one unit of rounding slack a weight), so the test is exact for
@@ -678,85 +679,106 @@ __device__ __forceinline__ int32_t unit_off(const FusedGeom &g, int32_t u)
 constexpr int HDR8_THC = 256;
 constexpr int HDR8_BYTES = 4 * (HDR8_THC + 256);
 
-// [row tile][K step][plane][lane] int8 fragments of the FP32 masters (plane 0
-// = hi, 1 = lo), and the header.  Every block reduces max|w| and the
-// per-channel positive and negative weight sums itself, so one launch needs
-// no second pass; block 0 writes the header.
+// The int8 scale of the FP32 masters: e from max|w| and the largest
+// per-channel sums of the positive and of the negative weights (hdr[0]), and
+// every channel's threshold th_q (hdr[HDR8_THC + c]), from which
+// prep_fragments8 subtracts 128 * A.  One block, four threads a channel,
+// each reading every fourth pair of the channel's weights with all its loads
+// issued at once; the sums combine in a fixed order, so e is deterministic.
+constexpr int SCALE_CHUNK = 16;  // pairs of weights a thread loads at once
+
 __global__ void __launch_bounds__(1024)
-    prep_fragments8(const float *__restrict__ k, int4v *__restrict__ frag, int32_t *__restrict__ hdr, float th,
+    prep_scale8(const float *__restrict__ k, int32_t *__restrict__ hdr, float th, int32_t cout, int32_t ksz)
+{
+    __shared__ float part[16][3];
+    const int32_t sub = threadIdx.x & 3, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    float m = 0.0f, pm = 0.0f, nm = 0.0f;
+    const int32_t c = threadIdx.x >> 2;
+    if (c < cout)
+    {
+        const float *kc = k + int64_t(c) * ksz;
+        float ps = 0.0f, ns = 0.0f;
+        if ((reinterpret_cast<uintptr_t>(kc) & 7) == 0)
+            // chunks of SCALE_CHUNK pairs, each chunk's loads in flight together
+            for (int32_t r0 = 0; 8 * r0 < ksz; r0 += SCALE_CHUNK)
+            {
+                float2 x[SCALE_CHUNK];
+#pragma unroll
+                for (int r = 0; r < SCALE_CHUNK; ++r)
+                {
+                    const int32_t i = 2 * (sub + 4 * (r0 + r));
+                    x[r] = i + 1 < ksz ? *reinterpret_cast<const float2 *>(kc + i)
+                                       : make_float2(i < ksz ? kc[i] : 0.0f, 0.0f);
+                }
+#pragma unroll
+                for (int r = 0; r < SCALE_CHUNK; ++r)
+                {
+                    m = fmaxf(m, fmaxf(fabsf(x[r].x), fabsf(x[r].y)));
+                    ps += fmaxf(x[r].x, 0.0f) + fmaxf(x[r].y, 0.0f);
+                    ns += fmaxf(-x[r].x, 0.0f) + fmaxf(-x[r].y, 0.0f);
+                }
+            }
+        else
+            for (int32_t i = sub; i < ksz; i += 4)
+            {
+                const float x = kc[i];
+                m = fmaxf(m, fabsf(x)), ps += fmaxf(x, 0.0f), ns += fmaxf(-x, 0.0f);
+            }
+        ps += __shfl_xor(ps, 1, 64), ns += __shfl_xor(ns, 1, 64);
+        ps += __shfl_xor(ps, 2, 64), ns += __shfl_xor(ns, 2, 64);
+        pm = ps, nm = ns;
+    }
+    m = wave_max(m), pm = wave_max(pm), nm = wave_max(nm);
+    if (lane == 0) part[wv][0] = m, part[wv][1] = pm, part[wv][2] = nm;
+    __syncthreads();
+    __shared__ int32_t e_s;
+    if (threadIdx.x == 0)
+    {
+        for (int i = 0; i < int(blockDim.x >> 6); ++i)
+            m = fmaxf(m, part[i][0]), pm = fmaxf(pm, part[i][1]), nm = fmaxf(nm, part[i][2]);
+        const int32_t e = bounded_exponent(fixed_exponent(m), th, double(pm), double(nm), ksz);
+        e_s = e, hdr[0] = e;
+    }
+    __syncthreads();
+    if (threadIdx.x < 256) hdr[HDR8_THC + threadIdx.x] = threshold_fixed(th, e_s);
+}
+
+// [row tile][K step][plane][lane] int8 fragments of the negated fixed-point
+// weights -w_q = 256 * hi + lo (plane 0 = hi, 1 = lo) at the scale
+// prep_scale8 chose (hdr[0]); a thread writes both
+// planes of one (row tile, K step, lane), whose 16 weights all belong to one
+// output channel, and subtracts 128 times their fixed-point sum from that
+// channel's threshold (integer atomics: the result does not depend on their
+// order), leaving thc = th_q - 128 * A.
+__global__ void __launch_bounds__(64)
+    prep_fragments8(const float *__restrict__ k, int4v *__restrict__ frag, int32_t *__restrict__ hdr,
                     int32_t cout, int32_t cin, int32_t kh, int32_t kw, int32_t cin_pad, int32_t upr, int32_t units,
                     int32_t steps, int32_t row_tiles)
 {
-    __shared__ float part[16], ppart[16], npart[16];
-    const int32_t ksz = cin * kh * kw;
-    const int64_t nw = int64_t(cout) * ksz;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-    // four independent maxima over 16-byte loads keep many loads in flight
-    float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f;
-    int64_t i0 = 0;
-    if ((reinterpret_cast<uintptr_t>(k) & 15) == 0)
-    {
-        const float4 *k4 = reinterpret_cast<const float4 *>(k);
-        const int64_t n4 = nw / 4;
-#pragma unroll 4
-        for (int64_t i = threadIdx.x; i < n4; i += blockDim.x)
-        {
-            const float4 v = k4[i];
-            m0 = fmaxf(m0, fabsf(v.x)), m1 = fmaxf(m1, fabsf(v.y)), m2 = fmaxf(m2, fabsf(v.z)),
-            m3 = fmaxf(m3, fabsf(v.w));
-        }
-        i0 = n4 * 4;
-    }
-    for (int64_t i = i0 + threadIdx.x; i < nw; i += blockDim.x) m0 = fmaxf(m0, fabsf(k[i]));
-    // per-channel sums of the positive and of the negative weights, one wave
-    // a channel, and their maxima over the channels
-    float pm = 0.0f, nm = 0.0f;
-    for (int32_t c = wv; c < cout; c += nwv)
-    {
-        float ps = 0.0f, ns = 0.0f;
-        for (int32_t i = lane; i < ksz; i += 64)
-        {
-            const float x = k[int64_t(c) * ksz + i];
-            ps += fmaxf(x, 0.0f), ns += fmaxf(-x, 0.0f);
-        }
-        pm = fmaxf(pm, wave_sum(ps)), nm = fmaxf(nm, wave_sum(ns));
-    }
-    float m = wave_max(fmaxf(fmaxf(m0, m1), fmaxf(m2, m3)));
-    if (lane == 0) part[wv] = m, ppart[wv] = pm, npart[wv] = nm;
-    __syncthreads();
-    m = part[0], pm = ppart[0], nm = npart[0];
-    for (int i = 1; i < nwv; ++i) m = fmaxf(m, part[i]), pm = fmaxf(pm, ppart[i]), nm = fmaxf(nm, npart[i]);
-    const int32_t e = bounded_exponent(fixed_exponent(m), th, double(pm), double(nm), ksz);
-    if (blockIdx.x == 0)
-    {
-        // thc = th_q - 128 * A (modulo 2^32), A the channel's sum of w_q
-        const uint32_t thq = uint32_t(threshold_fixed(th, e));
-        for (int32_t c = wv; c < 256; c += nwv)
-        {
-            int32_t a = 0;
-            if (c < cout)
-                for (int32_t i = lane; i < ksz; i += 64) a += int32_t(rintf(ldexpf(k[int64_t(c) * ksz + i], e)));
-            a = wave_sum_i32(a);
-            if (lane == 0) hdr[HDR8_THC + c] = int32_t(thq - 128u * uint32_t(a));
-        }
-        if (threadIdx.x == 0) hdr[0] = e;
-    }
-    const int32_t total = row_tiles * steps * 2 * 64;
+    const int32_t e = hdr[0];
+    const int32_t total = row_tiles * steps * 64;
     for (int32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += gridDim.x * blockDim.x)
     {
-        const int32_t lane2 = idx % 64, p = (idx / 64) % 2, s = (idx / 128) % steps, rt = idx / (128 * steps);
-        const int32_t o = rt * 16 + lane2 % 16, u = 4 * s + lane2 / 16, ky = u / upr;
-        uint32_t word[4] = {0, 0, 0, 0};
+        const int32_t lane = idx % 64, s = (idx / 64) % steps, rt = idx / (64 * steps);
+        const int32_t o = rt * 16 + lane % 16, u = 4 * s + lane / 16, ky = u / upr;
+        uint32_t hw_[4] = {0, 0, 0, 0}, lw[4] = {0, 0, 0, 0};
+        int32_t sum = 0;
+#pragma unroll
         for (int b = 0; b < 16; ++b)
         {
             const int32_t pos = (u % upr) * 16 + b, kx = pos / cin_pad, c = pos % cin_pad;
             float x = 0.0f;
             if (o < cout && u < units && kx < kw && c < cin) x = k[((int64_t(o) * cin + c) * kh + ky) * kw + kx];
-            const int32_t q = int32_t(rintf(ldexpf(x, e)));
-            const int32_t hi = (q + 128) >> 8, lo = q - hi * 256;
-            word[b / 4] |= uint32_t((p == 0 ? hi : lo) & 0xFF) << (8 * (b % 4));
+            const int32_t q = int32_t(rintf(ldexpf(x, e))), nq = -q;
+            const int32_t hi = (nq + 128) >> 8, lo = nq - hi * 256;
+            hw_[b / 4] |= uint32_t(hi & 0xFF) << (8 * (b % 4));
+            lw[b / 4] |= uint32_t(lo & 0xFF) << (8 * (b % 4));
+            sum += q;
         }
-        frag[idx] = int4v{int32_t(word[0]), int32_t(word[1]), int32_t(word[2]), int32_t(word[3])};
+        const int64_t base = (int64_t(rt) * steps + s) * 2 * 64 + lane;
+        frag[base] = int4v{int32_t(hw_[0]), int32_t(hw_[1]), int32_t(hw_[2]), int32_t(hw_[3])};
+        frag[base + 64] = int4v{int32_t(lw[0]), int32_t(lw[1]), int32_t(lw[2]), int32_t(lw[3])};
+        if (sum != 0) atomicSub(reinterpret_cast<uint32_t *>(hdr + HDR8_THC + o), 128u * uint32_t(sum));
     }
 }
 
@@ -775,13 +797,15 @@ __device__ __forceinline__ void load_units(int4v (&b)[S8], const uint8_t *base, 
     for (int s = 0; s < S8; ++s) b[s] = load16<WIDE>(base + off[s]);
 }
 
-// One step's MFMAs: hi and lo planes into their own accumulators.
+// One step's MFMAs: hi and lo planes of -w_q into their own accumulators,
+// the lo chain started at the channels' biased thresholds thc, so 256 * ch +
+// cl = thc + raw(-w_q) = th_q - v_q (see fixed_d).
 template <int RT, int S8>
-__device__ __forceinline__ void chain8(const int4v (&a)[RT][S8][2], const int4v (&b)[S8], int4v (&ch)[RT],
-                                       int4v (&cl)[RT])
+__device__ __forceinline__ void chain8(const int4v (&a)[RT][S8][2], const int4v (&b)[S8], const int4v (&thc)[RT],
+                                       int4v (&ch)[RT], int4v (&cl)[RT])
 {
 #pragma unroll
-    for (int rt = 0; rt < RT; ++rt) ch[rt] = int4v{0, 0, 0, 0}, cl[rt] = int4v{0, 0, 0, 0};
+    for (int rt = 0; rt < RT; ++rt) ch[rt] = int4v{0, 0, 0, 0}, cl[rt] = thc[rt];
 #pragma unroll
     for (int s = 0; s < S8; ++s)
 #pragma unroll
@@ -792,32 +816,42 @@ __device__ __forceinline__ void chain8(const int4v (&a)[RT][S8][2], const int4v 
         }
 }
 
-// th_q - v_q for the biased chain's planes (hi, lo) and the channel's
-// biased threshold thc, computed modulo 2^32 like the accumulators: exact,
-// since the choice of e keeps th_q - v_q inside int32.
-__device__ __forceinline__ int32_t fixed_diff(int32_t hi, int32_t lo, int32_t thc)
+// th_q - v_q from a chain's accumulators (hi plane, and lo plane started at
+// thc), computed modulo 2^32 like the accumulators: exact, since the choice of
+// e keeps th_q - v_q inside int32.
+__device__ __forceinline__ int32_t fixed_d(int32_t hi, int32_t lo)
 {
-    return int32_t(uint32_t(thc) - (uint32_t(hi) * 256u + uint32_t(lo)));
+    return int32_t(uint32_t(hi) * 256u + uint32_t(lo));
 }
 
 // value_update on an exact fixed-point potential: d = th_q - v_q has its sign
-// bit set exactly when v > th, and the running minimum of d is th_q minus the
-// running maximum of v (started at v = 0).
-__device__ __forceinline__ void value_update8(int32_t hi, int32_t lo, int32_t thc, uint32_t &mask, int32_t &mn)
+// bit set exactly when v > th, the running minimum of d is th_q minus the
+// running maximum of v (started at v = 0), and fd keeps d of the first step
+// that fired (the mask was 0 before it; later steps leave it).
+__device__ __forceinline__ void value_update8(int32_t hi, int32_t lo, uint32_t &mask, int32_t &mn, int32_t &fd)
 {
-    const int32_t d = fixed_diff(hi, lo, thc);
+    const int32_t d = fixed_d(hi, lo);
     mask = __builtin_amdgcn_alignbit(mask, uint32_t(d), 31);
     mn = min(mn, d);
+    fd = mask <= 1u ? d : fd;
 }
 
+// The per-value state of a tile's step loop.
 template <int RT>
-__device__ __forceinline__ void update_all8(const int4v (&uh)[RT], const int4v (&ul)[RT], const int32_t (&thc)[RT][4],
-                                            uint32_t (&mask)[RT][4], int32_t (&mn)[RT][4])
+struct FireState8
+{
+    uint32_t mask[RT][4];  // bit (T-1-t): fired at step t
+    int32_t mn[RT][4];     // th_q - the running maximum of the potential (from 0)
+    int32_t fd[RT][4];     // th_q - the potential at the first step that fired
+};
+
+template <int RT>
+__device__ __forceinline__ void update_all8(const int4v (&uh)[RT], const int4v (&ul)[RT], FireState8<RT> &f)
 {
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) value_update8(uh[rt][i], ul[rt][i], thc[rt][i], mask[rt][i], mn[rt][i]);
+        for (int i = 0; i < 4; ++i) value_update8(uh[rt][i], ul[rt][i], f.mask[rt][i], f.mn[rt][i], f.fd[rt][i]);
 }
 
 // step_iter of the int8 kernel: the MFMA pairs of step k+1 on `rin`, step
@@ -826,13 +860,12 @@ __device__ __forceinline__ void update_all8(const int4v (&uh)[RT], const int4v (
 template <int RT, int S8, bool WIDE>
 __device__ __forceinline__ void step_iter8(const int4v (&a)[RT][S8][2], const int4v (&rin)[S8], int4v (&ch)[RT],
                                            int4v (&cl)[RT], const int4v (&uh)[RT], const int4v (&ul)[RT],
-                                           const int32_t (&thc)[RT][4], uint32_t (&mask)[RT][4],
-                                           int32_t (&mn)[RT][4], int4v (&rout)[S8], const uint8_t *next,
-                                           const int32_t (&off)[S8])
+                                           const int4v (&thc)[RT], FireState8<RT> &f, int4v (&rout)[S8],
+                                           const uint8_t *next, const int32_t (&off)[S8])
 {
     constexpr int V = 4 * RT, SPAN = S8 * RT - RT;
 #pragma unroll
-    for (int rt = 0; rt < RT; ++rt) ch[rt] = int4v{0, 0, 0, 0}, cl[rt] = int4v{0, 0, 0, 0};
+    for (int rt = 0; rt < RT; ++rt) ch[rt] = int4v{0, 0, 0, 0}, cl[rt] = thc[rt];
 #pragma unroll
     for (int s = 0; s < S8; ++s)
     {
@@ -844,12 +877,12 @@ __device__ __forceinline__ void step_iter8(const int4v (&a)[RT][S8][2], const in
 #pragma unroll
             for (int v = 0; v < V; ++v)
                 if (SPAN > 0 && RT + (v * SPAN) / V == s * RT + rt)
-                    value_update8(uh[v / 4][v % 4], ul[v / 4][v % 4], thc[v / 4][v % 4], mask[v / 4][v % 4],
-                                  mn[v / 4][v % 4]);
+                    value_update8(uh[v / 4][v % 4], ul[v / 4][v % 4], f.mask[v / 4][v % 4], f.mn[v / 4][v % 4],
+                                  f.fd[v / 4][v % 4]);
         }
         rout[s] = load16<WIDE>(next + off[s]);
     }
-    if (SPAN <= 0) update_all8(uh, ul, thc, mask, mn);
+    if (SPAN <= 0) update_all8(uh, ul, f);
     __builtin_amdgcn_sched_barrier(0);
 }
 
@@ -865,9 +898,11 @@ __global__ void __launch_bounds__(layer_threads<RT>())
 {
     extern __shared__ __align__(16) uint8_t lds[];
     uint8_t *stage = lds;                                                       // [t][ph][pw][cin_pad] U8
-    uint32_t *any_s = reinterpret_cast<uint32_t *>(lds + g.stage_bytes);        // [2][MAX_WAVES][COLS]
-    float *red_v = reinterpret_cast<float *>(any_s + 2 * MAX_WAVES * COLS);     // [2][MAX_WAVES][COLS]
-    int32_t *red_i = reinterpret_cast<int32_t *>(red_v + 2 * MAX_WAVES * COLS);  // [2][MAX_WAVES][COLS]
+    // the waves' partials of each pixel side by side, two buffers each:
+    // [2][COLS][MAX_WAVES], so a lane reads its pixel's with two 16-byte loads
+    uint32_t *any_s = reinterpret_cast<uint32_t *>(lds + g.stage_bytes);
+    float *red_v = reinterpret_cast<float *>(any_s + 2 * MAX_WAVES * COLS);
+    int32_t *red_i = reinterpret_cast<int32_t *>(red_v + 2 * MAX_WAVES * COLS);
     uint2 *smap = reinterpret_cast<uint2 *>(red_i + 2 * MAX_WAVES * COLS);       // [ho*wo] {c*, fire mask}
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, q = lane >> 4, j = lane & 15;
@@ -882,7 +917,7 @@ __global__ void __launch_bounds__(layer_threads<RT>())
 #pragma unroll
             for (int p = 0; p < 2; ++p) a[r][s][p] = frag[(((w * RT + r) * S8 + s) * 2 + p) * 64 + lane];
     const int32_t e = hdr[0];
-    int32_t thc[RT][4];  // the biased thresholds of the lane's channels
+    int4v thc[RT];  // the biased thresholds of the lane's channels: the lo chains' start
 #pragma unroll
     for (int r = 0; r < RT; ++r)
 #pragma unroll
@@ -897,6 +932,9 @@ __global__ void __launch_bounds__(layer_threads<RT>())
     constexpr uint32_t BIAS = 0x80808080u;
     for (int32_t i = threadIdx.x; i < g.stage_bytes / 16; i += blockDim.x)
         reinterpret_cast<uint4 *>(stage)[i] = make_uint4(BIAS, BIAS, BIAS, BIAS);
+    // partials of waves past g.waves stay neutral: no steps, value -1
+    for (int32_t i = threadIdx.x; i < 2 * MAX_WAVES * COLS; i += blockDim.x)
+        any_s[i] = 0, red_v[i] = -1.0f, red_i[i] = INT32_MAX;
 
     int32_t prev = -1;  // the sample whose 1s are not stored yet
     for (int32_t n = blockIdx.x; n < g.n; n += gridDim.x)
@@ -927,12 +965,11 @@ __global__ void __launch_bounds__(layer_threads<RT>())
             const bool pvalid = p < hw;
             const int32_t pc = pvalid ? p : hw - 1, y = pc / g.wo, x = pc % g.wo;
             const uint8_t *pb = stage + (y * g.pw + x) * g.cin_pad;  // the pixel's window at step 0
-            uint32_t mask[RT][4];
-            int32_t mn[RT][4];  // th_q - (running maximum of the potential, from 0)
+            FireState8<RT> f;
 #pragma unroll
             for (int r = 0; r < RT; ++r)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) mask[r][i] = 0, mn[r][i] = int32_t(thq);
+                for (int i = 0; i < 4; ++i) f.mask[r][i] = 0, f.mn[r][i] = int32_t(thq), f.fd[r][i] = 0;
 
             {
                 const int32_t last = g.t - 1;
@@ -948,35 +985,34 @@ __global__ void __launch_bounds__(layer_threads<RT>())
                                 const int32_t o = o0 + r * 16 + i;
                                 if (o < g.cout)
                                     pot_out[((int64_t(n) * g.t + k) * g.cout + o) * hw + p] =
-                                        float(int32_t(thq - uint32_t(fixed_diff(uh[r][i], ul[r][i], thc[r][i])))) *
-                                        scale;
+                                        float(int32_t(thq - uint32_t(fixed_d(uh[r][i], ul[r][i])))) * scale;
                             }
                 };
                 load_units<S8, WIDE>(rA, pb, off);
                 load_units<S8, WIDE>(rB, pb + min(1, last) * g.sstep, off);
-                chain8(a, rA, hA, lA);
+                chain8(a, rA, thc, hA, lA);
                 int32_t k = 0;
                 int64_t zu = u0 + int32_t(threadIdx.x);
                 for (; k + 2 < g.t; k += 2)
                 {
-                    step_iter8<RT, S8, WIDE>(a, rB, hB, lB, hA, lA, thc, mask, mn, rA, pb + (k + 2) * g.sstep, off);
+                    step_iter8<RT, S8, WIDE>(a, rB, hB, lB, hA, lA, thc, f, rA, pb + (k + 2) * g.sstep, off);
                     diag(k, hA, lA);
                     if (zu < u1) zero16(s_body + zu);
                     zu += blockDim.x;
-                    step_iter8<RT, S8, WIDE>(a, rA, hA, lA, hB, lB, thc, mask, mn, rB,
-                                             pb + min(k + 3, last) * g.sstep, off);
+                    step_iter8<RT, S8, WIDE>(a, rA, hA, lA, hB, lB, thc, f, rB, pb + min(k + 3, last) * g.sstep,
+                                             off);
                     diag(k + 1, hB, lB);
                 }
                 if (k + 1 < g.t)
                 {
-                    step_iter8<RT, S8, WIDE>(a, rB, hB, lB, hA, lA, thc, mask, mn, rA, pb + last * g.sstep, off);
+                    step_iter8<RT, S8, WIDE>(a, rB, hB, lB, hA, lA, thc, f, rA, pb + last * g.sstep, off);
                     diag(k, hA, lA);
-                    update_all8(hB, lB, thc, mask, mn);
+                    update_all8(hB, lB, f);
                     diag(k + 1, hB, lB);
                 }
                 else
                 {
-                    update_all8(hA, lA, thc, mask, mn);
+                    update_all8(hA, lA, f);
                     diag(k, hA, lA);
                 }
                 for (; zu < u1; zu += blockDim.x) zero16(s_body + zu);
@@ -987,25 +1023,44 @@ __global__ void __launch_bounds__(layer_threads<RT>())
 #pragma unroll
                 for (int r = 0; r < RT; ++r)
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) any |= mask[r][i];
+                    for (int i = 0; i < 4; ++i) any |= f.mask[r][i];
                 any |= __shfl_xor(any, 16, 64);
                 any |= __shfl_xor(any, 32, 64);
-                if (lane < COLS) any_s[buf + w * COLS + lane] = any;
+                if (lane < COLS) any_s[buf + lane * MAX_WAVES + w] = any;
             }
             __syncthreads();
-            // pass 1 at inhibit's step (see fused_layer); the thresholded
-            // values compare as floats (exactly for |v_q| < 2^24, as with
-            // spike inputs in {0, 1})
-            int4v gh[RT], gl[RT];
-            int32_t ti;  // inhibit's step at this pixel
+            // inhibit's kept channel c* at its step ti = T - max(cnt, 1): the
+            // first maximum of the thresholded values there.  A value fired
+            // at ti exactly when its mask says so.  When the pixel's fired
+            // steps (all channels) form one run up to the last step, ti is the
+            // first step any channel fired, so a value that fired at ti fired
+            // first there and fd holds it; otherwise a pass of the step
+            // loop's MFMA sequence at ti (each lane at its own pixel's step)
+            // gives the values, bit for bit.  The thresholded values compare
+            // as floats (exactly for |v_q| < 2^24, as with spike inputs).
             {
-                uint32_t all = 0;
-                for (int32_t ow = 0; ow < g.waves; ++ow) all |= any_s[buf + ow * COLS + j];
+                const uint4 a0 = *reinterpret_cast<const uint4 *>(any_s + buf + j * MAX_WAVES);
+                const uint4 a1 = *reinterpret_cast<const uint4 *>(any_s + buf + j * MAX_WAVES + 4);
+                const uint32_t all = (a0.x | a0.y | a0.z | a0.w) | (a1.x | a1.y | a1.z | a1.w);
                 const int32_t cnt = __builtin_popcount(all);
-                ti = g.t - (cnt > 1 ? cnt : 1);
-                int4v r[S8];
-                load_units<S8, WIDE>(r, pb + ti * g.sstep, off);
-                chain8(a, r, gh, gl);
+                const int32_t ti = g.t - (cnt > 1 ? cnt : 1), sh = g.t - 1 - ti;
+                const bool run = (all & (all + 1u)) == 0u;
+                int32_t dti[RT][4];
+#pragma unroll
+                for (int r = 0; r < RT; ++r)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) dti[r][i] = f.fd[r][i];
+                if (__builtin_amdgcn_ballot_w64(!run))
+                {
+                    int4v r[S8], gh[RT], gl[RT];
+                    load_units<S8, WIDE>(r, pb + ti * g.sstep, off);
+                    chain8(a, r, thc, gh, gl);
+#pragma unroll
+                    for (int r2 = 0; r2 < RT; ++r2)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                            if (!run) dti[r2][i] = fixed_d(gh[r2][i], gl[r2][i]);
+                }
                 float bv = -1.0f;
                 int32_t bi = INT32_MAX;
 #pragma unroll
@@ -1013,48 +1068,41 @@ __global__ void __launch_bounds__(layer_threads<RT>())
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
                     {
-                        const int32_t d = fixed_diff(gh[r2][i], gl[r2][i], thc[r2][i]);
-                        const float xp = d < 0 ? float(int32_t(thq - uint32_t(d))) : 0.0f;
+                        const float xp = (f.mask[r2][i] >> sh) & 1u ? float(int32_t(thq - uint32_t(dti[r2][i]))) : 0.0f;
                         const bool take = xp > bv;
                         bv = take ? xp : bv;
                         bi = take ? o0 + r2 * 16 + i : bi;
                     }
                 combine(bv, bi, __shfl_xor(bv, 16, 64), __shfl_xor(bi, 16, 64));
                 combine(bv, bi, __shfl_xor(bv, 32, 64), __shfl_xor(bi, 32, 64));
-                if (lane < COLS) red_v[buf + w * COLS + lane] = bv, red_i[buf + w * COLS + lane] = bi;
+                if (lane < COLS) red_v[buf + lane * MAX_WAVES + w] = bv, red_i[buf + lane * MAX_WAVES + w] = bi;
             }
             __syncthreads();
             int32_t cs;
             {
-                float m = red_v[buf + j];
-                cs = red_i[buf + j];
-                for (int32_t ow = 1; ow < g.waves; ++ow)
-                    combine(m, cs, red_v[buf + ow * COLS + j], red_i[buf + ow * COLS + j]);
+                // the waves' partials of the pixel, combined as a tree (the
+                // order (value desc, channel asc) is total)
+                const float4 v0 = *reinterpret_cast<const float4 *>(red_v + buf + j * MAX_WAVES);
+                const float4 v1 = *reinterpret_cast<const float4 *>(red_v + buf + j * MAX_WAVES + 4);
+                const int4 i0 = *reinterpret_cast<const int4 *>(red_i + buf + j * MAX_WAVES);
+                const int4 i1 = *reinterpret_cast<const int4 *>(red_i + buf + j * MAX_WAVES + 4);
+                float m0 = v0.x, m1 = v0.z, m2 = v1.x, m3 = v1.z;
+                int32_t c0 = i0.x, c1 = i0.z, c2 = i1.x, c3 = i1.z;
+                combine(m0, c0, v0.y, i0.y), combine(m1, c1, v0.w, i0.w);
+                combine(m2, c2, v1.y, i1.y), combine(m3, c3, v1.w, i1.w);
+                combine(m0, c0, m1, c1), combine(m2, c2, m3, c3);
+                combine(m0, c0, m2, c2);
+                cs = c0;
             }
             const int32_t cr = (cs % (16 * RT)) / 16, cq = (cs >> 2) & 3, ci = cs & 3;
             const bool own = cs / (16 * RT) == w && cq == q;
             uint32_t om = 0;
-            int32_t ox = 0, first = 0;  // th_q - c*'s maximum; first: th_q - c*'s value at inhibit's step
+            int32_t ox = 0, first = 0;  // th_q - c*'s maximum and its value at its first firing step
 #pragma unroll
             for (int r = 0; r < RT; ++r)
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    if (cr == r && ci == i) om = mask[r][i], ox = mn[r][i], first = fixed_diff(gh[r][i], gl[r][i], thc[r][i]);
-            const int32_t my_t0 = om ? g.t - 32 + int32_t(__builtin_clz(om)) : 0;
-            const int32_t t0 = __shfl(my_t0, cq * 16 + j, 64);
-            // pass 2: the value of c* at its first firing step, run only where
-            // that step is not inhibit's step (as in fused_layer)
-            if (__builtin_amdgcn_ballot_w64(own && pvalid && om != 0 && my_t0 != ti))
-            {
-                int4v r[S8];
-                load_units<S8, WIDE>(r, pb + t0 * g.sstep, off);
-                chain8(a, r, gh, gl);
-#pragma unroll
-                for (int r2 = 0; r2 < RT; ++r2)
-#pragma unroll
-                    for (int i = 0; i < 4; ++i)
-                        if (cr == r2 && ci == i) first = fixed_diff(gh[r2][i], gl[r2][i], thc[r2][i]);
-            }
+                    if (cr == r && ci == i) om = f.mask[r][i], ox = f.mn[r][i], first = f.fd[r][i];
             if (own && pvalid)
             {
                 const uint32_t fwd = om ? __builtin_bitreverse32(om) >> (32 - g.t) : 0u;
@@ -1677,9 +1725,11 @@ void stdp_layer_fused(const void *in, int64_t n, int64_t t, int64_t cin, int64_t
             if (use8)
             {
                 // the header (scale exponent, biased thresholds), the fragments after
-                const int64_t entries = rtiles * steps8 * 2 * 64;
-                prep_fragments8<<<dim3(unsigned(std::min<int64_t>(cdiv(entries, 1024), 32))), 1024, 0, s>>>(
-                    (const float *)kernel, frag8, hdr, g.thf, g.cout, g.cin, g.kh, g.kw, g.cin_pad, g.upr, g.units, steps8,
+                const int64_t entries = rtiles * steps8 * 64;
+                prep_scale8<<<1, 1024, 0, s>>>((const float *)kernel, hdr, g.thf, g.cout, int32_t(ksz));
+                SPK_LAUNCHED();
+                prep_fragments8<<<dim3(unsigned(cdiv(entries, 64))), 64, 0, s>>>(
+                    (const float *)kernel, frag8, hdr, g.cout, g.cin, g.kh, g.kw, g.cin_pad, g.upr, g.units, steps8,
                     int32_t(rtiles));
                 SPK_LAUNCHED();
                 // the kernel also writes the spikes and pooled spikes of each sample
