@@ -44,12 +44,19 @@ PEAK_FP32 = 157.3e12
 PEAK_FP16 = 2.5e15
 PEAK_INT8 = 5.0e15
 PEAK_HBM = 8.0e12
-# HBM bytes one full-batch fused_layer launch moves on this workload, from rocprofv3
+# HBM bytes one full-batch whole-sample launch moves on this workload, from rocprofv3
 # FETCH_SIZE (x2.0, the factor a 512 MiB device copy reads as) + WRITE_SIZE (x1.0):
-# profiles/fused_layer_c2_traffic.txt.  Algorithmic: 90.3 MB input + 9.6 MB summaries/first-spike map
-# + 814.0 MB spikes and pooled spikes (the kernel writes them) = 914.0 MB; the spike lines that get 1s
-# are written twice (non-temporal zeros, then the 1s).
-TRAFFIC_FUSED_LAYER = {"fp16": 979.2e6, "int8": None}
+# profiles/fused_layer8_c2_traffic.txt (int8) and profiles/fused_layer_c2_traffic.txt (fp16).
+# Algorithmic: 90.3 MB input + 9.6 MB summaries/first-spike map + 814.0 MB spikes and pooled spikes
+# (the kernel writes them) = 914.0 MB; the spike lines that get 1s are written twice (non-temporal
+# zeros, then the 1s).
+TRAFFIC_FUSED_LAYER = {"fp16": 979.2e6, "int8": 985.7e6}
+# The region kernel per launch on the precision workload, and the whole batching-workload step
+# (region kernel, writers, convwta), from the same counters: profiles/fused_region_traffic.txt.
+TRAFFIC_REGION_C3 = 26.27e9
+TRAFFIC_STEP_C4 = 85.2e9
+WARM_S = 0.3  # seconds of untimed fused steps before the timed ones (at least --warmup steps)
+METRIC = "images\u00b7timesteps/s, conv+fire+wta+STDP layer: FP16-batched vs FP32 plain, 1 GPU"
 
 
 def make_inputs(dev, n=N, seed=2):
@@ -389,7 +396,7 @@ def c3_bench(spk, dev, steps):
            "value": n * C3_T / dt, "unit": "images*timesteps/s", "ms_per_step": dt * 1e3, "steps": steps,
            "dtype": "f16-mfma/f32-acc",
            "roofline": {"bound": "mfma", "achieved": flop / kern_s / 1e12, "peak": PEAK_FP16 / 1e12,
-                        "unit": "TFLOP/s", "frac": flop / kern_s / PEAK_FP16, "traffic": None,
+                        "unit": "TFLOP/s", "frac": flop / kern_s / PEAK_FP16, "traffic": TRAFFIC_REGION_C3,
                         "kernel": "fused_region", "kernel_ms": kern_s * 1e3}}
     # precision against the FP64 conv on a few samples
     m = 8
@@ -500,7 +507,7 @@ def c4_bench(spk, dev, steps):
             "images_timesteps": its, "dtype": "f16-mfma/f32-acc",
             "bytes_in_out_pooled": [plan.in_bytes, plan.out_bytes, plan.pool_bytes],
             "roofline": {"bound": "hbm", "achieved": step_bytes / dt / 1e9, "peak": PEAK_HBM / 1e9,
-                         "unit": "GB/s", "frac": step_bytes / dt / PEAK_HBM, "traffic": None,
+                         "unit": "GB/s", "frac": step_bytes / dt / PEAK_HBM, "traffic": TRAFFIC_STEP_C4,
                          "kernel": "the whole step (its HBM floor: the U8 input, spikes and pooled spikes)",
                          "floor_ms": step_bytes / PEAK_HBM * 1e3},
             "region_kernel": {"achieved_tflops": flop / kern_s / 1e12, "frac_of_fp16_mfma_peak": flop / kern_s / PEAK_FP16,
@@ -669,18 +676,32 @@ def main():
         return
     x, k2 = make_inputs(dev)
 
-    conv_f = spk.Conv(CIN, COUT, K, pad=1, device=dev)
-    conv_f.kernel = torch.from_numpy(k2).to(dev)
-    conv_f.stdpconfig = [spk.STDPConfig(0.004, -0.003)]
-    for _ in range(a.warmup):
-        fused_step(spk, x, conv_f)
-    torch.cuda.synchronize()
-    spk.call("spk_kernel_timer", 1)
-    dt = timed(lambda: fused_step(spk, x, conv_f), a.steps, 0)
-    total_ms, launches = C.c_double(0), C.c_int64(0)
-    spk.call("spk_kernel_timer_read", C.byref(total_ms), C.byref(launches))
-    spk.call("spk_kernel_timer", 0)
-    kern_s = total_ms.value / 1e3 / max(launches.value, 1)
+    def fused_run(arith):
+        """Warm-up and timed fused steps in one arithmetic: (seconds a step, seconds a whole-sample launch,
+        launches timed, the conv)."""
+        spk.set_fused_arith(arith)
+        conv = spk.Conv(CIN, COUT, K, pad=1, device=dev)
+        conv.kernel = torch.from_numpy(k2).to(dev)
+        conv.stdpconfig = [spk.STDPConfig(0.004, -0.003)]
+        # W warm-up steps, and at least WARM_S seconds of them: a step is half a
+        # millisecond, and the GPU's clocks take a few hundred steps to settle
+        t_end = time.perf_counter() + WARM_S
+        i = 0
+        while i < a.warmup or time.perf_counter() < t_end:
+            fused_step(spk, x, conv)
+            i += 1
+            if i % 32 == 0:
+                torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        spk.call("spk_kernel_timer", 1)
+        d = timed(lambda: fused_step(spk, x, conv), a.steps, 0)
+        total, nl = C.c_double(0), C.c_int64(0)
+        spk.call("spk_kernel_timer_read", C.byref(total), C.byref(nl))
+        spk.call("spk_kernel_timer", 0)
+        return d, total.value / 1e3 / max(nl.value, 1), nl.value, conv
+
+    dt, kern_s, nlaunch, conv_f = fused_run(a.arith)
+    launches = C.c_int64(nlaunch)
     achieved = FLOP_PER_IT * N * T_STEPS / kern_s
     if a.arith == "int8":
         # spike bytes times 16-bit fixed-point weights, split into a high and a low int8 byte:
@@ -692,7 +713,7 @@ def main():
         path = "fused FP16-MFMA layer step, FP32 master weights"
 
     line = {
-        "metric": "images*timesteps/s, conv+fire+wta+STDP layer, 1 GPU",
+        "metric": json.loads('"%s"' % METRIC),
         "value": N * T_STEPS / dt,
         "unit": "images*timesteps/s",
         "n_gpus": 1,
@@ -714,6 +735,14 @@ def main():
                      "flop_per_launch": FLOP_PER_IT * N * T_STEPS},
     }
     line["config"]["channels_per_wave"] = spk.fused_tuned_channels(conv_f, T_STEPS, H, H)
+    # the other arithmetic of the same kernel, timed the same way on the same inputs
+    other = "fp16" if a.arith == "int8" else "int8"
+    dto, kso, _, _ = fused_run(other)
+    spk.set_fused_arith(a.arith)
+    line["other_arithmetic"] = {
+        "arith": other, "value": N * T_STEPS / dto, "ms_per_step": dto * 1e3, "kernel_ms": kso * 1e3,
+        "achieved_tflops": FLOP_PER_IT * N * T_STEPS / kso / 1e12,
+        "frac_of_fp16_mfma_peak": FLOP_PER_IT * N * T_STEPS / kso / PEAK_FP16}
     if a.arith == "int8":
         line["roofline"]["note"] = ("achieved counts the conv's own multiply-adds; the int8 path issues two int8 "
                                     "products for each (high and low weight byte), so its ceiling for this conv "
