@@ -49,6 +49,8 @@ using u32x4 = __attribute__((ext_vector_type(4))) uint32_t;
 constexpr int RW = 16;            // region width: one column tile per region row
 constexpr int REGION_WAVES = 8;   // waves per block
 constexpr int LDS_BYTES = 163840;
+constexpr int SU = 8;  // stage rows a thread loads together
+constexpr int WU = 4;  // window (step, row) pairs a thread converts together
 
 struct RegionInst
 {
@@ -238,21 +240,35 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
                 const bool colv = ix0 >= 0 && ix0 < I.w && sc < g.cin;
                 int32_t tt = sslot / g.sh, lr = sslot % g.sh;
                 const int32_t dq = sslots / g.sh, dr = sslots % g.sh;
-                for (int32_t r = sslot; r < srows; r += sslots)
+                // batches of SU rows a slot, the batch's loads in flight together
+                for (int32_t r0 = sslot; r0 < srows; r0 += SU * sslots)
                 {
-                    const int32_t iy = sy0 + lr;
-                    uint32_t x = 0;
-                    if (colv && iy >= 0 && iy < I.h)
-                        x = *reinterpret_cast<const uint32_t *>(src + ((int64_t(tt) * g.cin + sc) * I.h + iy) * I.w +
-                                                               ix0);
-                    // 4x4 byte transpose within the quad: lane l4 then holds
-                    // channels 4*c4 .. 4*c4+3 of pixel 4*lq + l4
-                    const uint32_t y = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(x), 0xB1, 0xF, 0xF, true));
-                    const uint32_t r1 = __builtin_amdgcn_perm(y, x, sel1);
-                    const uint32_t y2 = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(r1), 0x4E, 0xF, 0xF, true));
-                    dst[((tt * g.sh + lr) * g.sw + 4 * lq + l4) * D + c4] = __builtin_amdgcn_perm(y2, r1, sel2);
-                    tt += dq, lr += dr;
-                    if (lr >= g.sh) lr -= g.sh, ++tt;
+                    uint32_t v[SU];
+                    int32_t at[SU];
+#pragma unroll
+                    for (int u = 0; u < SU; ++u)
+                    {
+                        const int32_t iy = sy0 + lr;
+                        v[u] = 0;
+                        if (r0 + u * sslots < srows && colv && iy >= 0 && iy < I.h)
+                            v[u] = *reinterpret_cast<const uint32_t *>(src + ((int64_t(tt) * g.cin + sc) * I.h + iy) *
+                                                                                 I.w +
+                                                                       ix0);
+                        at[u] = ((tt * g.sh + lr) * g.sw + 4 * lq + l4) * D + c4;
+                        tt += dq, lr += dr;
+                        if (lr >= g.sh) lr -= g.sh, ++tt;
+                    }
+#pragma unroll
+                    for (int u = 0; u < SU; ++u)
+                    {
+                        // 4x4 byte transpose within the quad: lane l4 then holds
+                        // channels 4*c4 .. 4*c4+3 of pixel 4*lq + l4
+                        const uint32_t x = v[u];
+                        const uint32_t y = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(x), 0xB1, 0xF, 0xF, true));
+                        const uint32_t r1 = __builtin_amdgcn_perm(y, x, sel1);
+                        const uint32_t y2 = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(r1), 0x4E, 0xF, 0xF, true));
+                        if (r0 + u * sslots < srows) dst[at[u]] = __builtin_amdgcn_perm(y2, r1, sel2);
+                    }
                 }
             }
         }
@@ -340,25 +356,39 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
                     const bool colv = ix >= 0 && ix < I.w;
                     const uint8_t *sp0 = stage + ((grp0 * g.sw) + xoff + wcc) * g.cin_pad + wc8 * 8;
                     int32_t ts = wslot / g.wrows, r = wslot % g.wrows;
-                    for (int32_t pr = wslot; pr < tc * g.wrows; pr += wslots)
+                    const int32_t prs = tc * g.wrows;
+                    // batches of WU (step, row) pairs a slot, their stage reads together
+                    for (int32_t pr0 = wslot; pr0 < prs; pr0 += WU * wslots)
                     {
-                        const int32_t iy = sy0 + grp0 + r;
-                        uint2 raw = make_uint2(0, 0);
-                        const uint8_t *sp = sp0 + ((t0 + ts) * g.sh + r) * g.sw * g.cin_pad;
-                        if (PAIR)
+                        uint2 raw[WU];
+                        int32_t at[WU];
+#pragma unroll
+                        for (int u = 0; u < WU; ++u)
                         {
-                            if (colv && iy >= 0 && iy < I.h) raw.x = *reinterpret_cast<const uint32_t *>(sp);
-                            const uint4 hv = subnormals(raw);
-                            reinterpret_cast<uint2 *>(win16)[ts * 2 * g.win_units + r * g.pitch + wcu] =
-                                make_uint2(hv.x, hv.y);
+                            const int32_t iy = sy0 + grp0 + r;
+                            const uint8_t *sp = sp0 + ((t0 + ts) * g.sh + r) * g.sw * g.cin_pad;
+                            raw[u] = make_uint2(0, 0);
+                            if (pr0 + u * wslots < prs && colv && iy >= 0 && iy < I.h)
+                            {
+                                if (PAIR)
+                                    raw[u].x = *reinterpret_cast<const uint32_t *>(sp);
+                                else
+                                    raw[u] = *reinterpret_cast<const uint2 *>(sp);
+                            }
+                            at[u] = PAIR ? ts * 2 * g.win_units + r * g.pitch + wcu : ts * g.win_units + r * g.pitch + wcu;
+                            ts += wdq, r += wdr;
+                            if (r >= g.wrows) r -= g.wrows, ++ts;
                         }
-                        else
-                        {
-                            if (colv && iy >= 0 && iy < I.h) raw = *reinterpret_cast<const uint2 *>(sp);
-                            win16[ts * g.win_units + r * g.pitch + wcu] = subnormals(raw);
-                        }
-                        ts += wdq, r += wdr;
-                        if (r >= g.wrows) r -= g.wrows, ++ts;
+#pragma unroll
+                        for (int u = 0; u < WU; ++u)
+                            if (pr0 + u * wslots < prs)
+                            {
+                                const uint4 hv = subnormals(raw[u]);
+                                if (PAIR)
+                                    reinterpret_cast<uint2 *>(win16)[at[u]] = make_uint2(hv.x, hv.y);
+                                else
+                                    win16[at[u]] = hv;
+                            }
                     }
                 }
                 __syncthreads();
