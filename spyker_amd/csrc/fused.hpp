@@ -118,6 +118,25 @@ __device__ __forceinline__ void step_update(const float4v &acc0, const float4v &
     for (int i = 0; i < 4; ++i) value_update(acc1[i], th, mask[1][i], mxb[1][i]);
 }
 
+// step_update that also keeps each value at its first firing step (fv: the
+// mask was 0 before it; later steps leave it).
+__device__ __forceinline__ void step_update_first(const float4v &acc0, const float4v &acc1, float th,
+                                                  uint32_t (&mask)[2][4], int32_t (&mxb)[2][4], float (&fv)[2][4])
+{
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+    {
+        value_update(acc0[i], th, mask[0][i], mxb[0][i]);
+        fv[0][i] = mask[0][i] <= 1u ? acc0[i] : fv[0][i];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+    {
+        value_update(acc1[i], th, mask[1][i], mxb[1][i]);
+        fv[1][i] = mask[1][i] <= 1u ? acc1[i] : fv[1][i];
+    }
+}
+
 template <int S>
 __device__ __forceinline__ void load_b(half8 (&b)[S], const uint8_t *base, const int32_t (&offs)[S])
 {

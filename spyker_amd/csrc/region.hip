@@ -339,10 +339,11 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
             const uint8_t *wb = reinterpret_cast<const uint8_t *>(win16) + (ct * g.pitch + j) * UB;
             uint32_t mask[2][4];
             int32_t mxb[2][4];
+            float fv[2][4];  // each value at its first firing step
 #pragma unroll
             for (int r = 0; r < 2; ++r)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) mask[r][i] = 0, mxb[r][i] = 0;
+                for (int i = 0; i < 4; ++i) mask[r][i] = 0, mxb[r][i] = 0, fv[r][i] = 0.0f;
 
             for (int32_t t0 = 0; t0 < I.t; t0 += g.tc)
             {
@@ -451,7 +452,7 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
                     load(b0, step_base(1));
                 }
                 auto update = [&](int32_t k, const float4v &u0, const float4v &u1) {
-                    step_update(u0, u1, g.th, mask, mxb);
+                    step_update_first(u0, u1, g.th, mask, mxb, fv);
                     const int64_t st = t0 + k;
                     if (zspk && oy0 + grp0 + zr < I.ho)
                     {
@@ -555,18 +556,39 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
                     if (s + GR < S) gb[s % GR] = gather1(s + GR, st);
                 }
             };
+            // inhibit's kept channel c* at its step ts = T - max(cnt, 1).  When
+            // the pixel's fired steps (all channels) form one run up to the
+            // last step, ts is the first step any channel fired: a value that
+            // fired at ts fired first there (fv holds it) and any other is 0
+            // after the threshold.  Otherwise the step loop's MFMA sequence on
+            // the gathered operands at ts gives the values, bit for bit.
             int32_t ts;
+            bool run;
             {
-                uint32_t any = 0;
-                for (int32_t k = 0; k < g.cg; ++k) any |= any_s[(ct * g.cg + k) * COLS + j];
-                const int32_t cnt = __builtin_popcount(any);
+                uint32_t all = 0;
+                for (int32_t k = 0; k < g.cg; ++k) all |= any_s[(ct * g.cg + k) * COLS + j];
+                const int32_t cnt = __builtin_popcount(all);
                 ts = I.t - (cnt > 1 ? cnt : 1);
+                run = (all & (all + 1u)) == 0u;
             }
-            // pass 1: the potentials at inhibit's step; the kept channel is the
-            // first maximum of the thresholded values (inhibit.cpp:64-85)
-            float4v g0, g1;
+            // pass 1 where needed: the potentials at inhibit's step; the kept
+            // channel is the first maximum of the thresholded values
+            // (inhibit.cpp:64-85)
             {
-                gchain(ts, g0, g1);
+                float vt[2][4];
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) vt[r][i] = fv[r][i];
+                if (__builtin_amdgcn_ballot_w64(!run))
+                {
+                    float4v g0, g1;
+                    gchain(ts, g0, g1);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        if (!run) vt[0][i] = g0[i], vt[1][i] = g1[i];
+                }
+                const int32_t sh = I.t - 1 - ts;
                 float bv = -1.0f;
                 int32_t bi = INT32_MAX;
 #pragma unroll
@@ -574,8 +596,7 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
                     {
-                        const float v = r == 0 ? g0[i] : g1[i];
-                        const float xp = v > g.th ? v : 0.0f;
+                        const float xp = (mask[r][i] >> sh) & 1u ? vt[r][i] : 0.0f;
                         const bool take = xp > bv;
                         bv = take ? xp : bv;
                         bi = take ? o0 + r * 16 + i : bi;
@@ -592,28 +613,17 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
                 for (int32_t k = 1; k < g.cg; ++k)
                     combine(m, cs, red_v[(ct * g.cg + k) * COLS + j], red_i[(ct * g.cg + k) * COLS + j]);
             }
-            // the lane holding (c*, pixel) publishes c*'s first firing step
+            // the lane holding (c*, pixel) has c*'s fire mask, maximum and
+            // value at its first firing step (convwta's t0 value)
             const bool own = (cs >> 5) == cgw && ((cs >> 2) & 3) == q;
             uint32_t om = 0;
             int32_t oxb = 0;
+            float first = 0.0f;
 #pragma unroll
             for (int r = 0; r < 2; ++r)
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    if (((cs >> 4) & 1) == r && (cs & 3) == i) om = mask[r][i], oxb = mxb[r][i];
-            if (own) tsel[ct * COLS + j] = om ? I.t - 32 + int32_t(__builtin_clz(om)) : -1;
-            __syncthreads();
-            // pass 2: the value of c* at its first firing step (convwta's t0 value)
-            float first = 0.0f;
-            if (__builtin_amdgcn_ballot_w64(own && om != 0))
-            {
-                gchain(tsel[ct * COLS + j], g0, g1);
-#pragma unroll
-                for (int r = 0; r < 2; ++r)
-#pragma unroll
-                    for (int i = 0; i < 4; ++i)
-                        if (((cs >> 4) & 1) == r && (cs & 3) == i) first = r == 0 ? g0[i] : g1[i];
-            }
+                    if (((cs >> 4) & 1) == r && (cs & 3) == i) om = mask[r][i], oxb = mxb[r][i], first = fv[r][i];
             if (own)
             {
                 const uint32_t fwd = om ? __builtin_bitreverse32(om) >> (32 - I.t) : 0u;
