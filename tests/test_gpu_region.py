@@ -32,9 +32,9 @@ def H(t):
     return t.cpu().numpy()
 
 
-def spikes(rng, shape, p=0.5):
+def spikes(rng, shape, p=0.5, cumulative=True):
     s = (rng.random(shape) < p / shape[-4] * 3).astype(np.uint8)
-    return np.maximum.accumulate(s, axis=-4)
+    return np.maximum.accumulate(s, axis=-4) if cumulative else s
 
 
 def dyadic(rng, shape):
@@ -95,16 +95,20 @@ def test_region_equals_whole_sample_kernel(dev, tune):
 
 
 @pytest.mark.parametrize("case", [
-    # n, t, cin, h, w, cout, k, pad, radius, count, pool
-    (3, 10, 8, 40, 44, 64, 7, (3, 3, 3, 3), 5, 10, (2, 2)),    # dword staging, 2 channel groups
-    (2, 9, 8, 38, 42, 64, 7, (3, 3, 3, 3), 5, 10, (2, 2)),     # w % 4 != 0: byte staging
-    (3, 12, 4, 36, 40, 32, 5, (2, 1, 0, 3), 2, 4, (2, 2)),     # one channel group, asymmetric pad
-    (2, 7, 3, 33, 20, 100, 3, (1, 1, 1, 1), (1, 2), 6, None),  # 4 channel groups, ragged regions
+    # n, t, cin, h, w, cout, k, pad, radius, count, pool, cumulative input
+    (3, 10, 8, 40, 44, 64, 7, (3, 3, 3, 3), 5, 10, (2, 2), True),    # dword staging, 2 channel groups
+    (2, 9, 8, 38, 42, 64, 7, (3, 3, 3, 3), 5, 10, (2, 2), True),     # w % 4 != 0: byte staging
+    (3, 12, 4, 36, 40, 32, 5, (2, 1, 0, 3), 2, 4, (2, 2), True),     # one channel group, asymmetric pad
+    (2, 7, 3, 33, 20, 100, 3, (1, 1, 1, 1), (1, 2), 6, None, True),  # 4 channel groups, ragged regions
+    # isolated spikes: potentials rise and fall, so a pixel's fired steps
+    # need not run to the last step and the tile end's gathered pass runs
+    (3, 10, 8, 40, 44, 64, 7, (3, 3, 3, 3), 5, 10, (2, 2), False),
+    (3, 12, 4, 36, 40, 32, 5, (2, 1, 0, 3), 2, 4, (2, 2), False),
 ])
 def test_region_exact_on_dyadic_weights(dev, orc, case):
-    n, t, cin, h, w, cout, k, pad, radius, count, pool = case
+    n, t, cin, h, w, cout, k, pad, radius, count, pool, cumulative = case
     rng = np.random.default_rng(32)
-    x = spikes(rng, (n, t, cin, h, w))
+    x = spikes(rng, (n, t, cin, h, w), 0.5 if cumulative else 1.5, cumulative)
     kern = dyadic(rng, (cout, cin, k, k))
     pot = orc.conv(x[:1].reshape(t, cin, h, w), kern, 1, pad)
     th = float(np.quantile(pot, 0.9)) + 0.5 / 64
@@ -114,6 +118,12 @@ def test_region_exact_on_dyadic_weights(dev, orc, case):
     o_out, o_win, o_cnt, o_pool = oracle_forward(orc, x, kern, th, radius, count, pad, pool)
     o_k = orc.convstdp(x, kern, o_out, [(0.004, -0.003)], o_win, o_cnt, 1, pad)
     assert o_cnt.sum() > 0
+    # pixels whose fired steps (any channel) do not run to the last step take
+    # the tile end's gathered pass; the isolated-spike cases must have some
+    full = orc.conv(x.reshape(n * t, cin, h, w), kern, 1, pad).reshape(n, t, cout, -1)
+    fired = (full > th).any(axis=2)  # [n, t, pixels]
+    suffix = np.flip(np.logical_and.accumulate(np.flip(fired, 1), axis=1), 1)
+    assert (fired != suffix).any() != cumulative
     assert np.array_equal(H(out).reshape(o_out.shape), o_out)
     check_winners(wn, o_win, o_cnt)
     assert np.array_equal(H(conv.kernel), o_k)
