@@ -1305,10 +1305,17 @@ SideStream::SideStream(hipStream_t main_, bool use) : main(main_), stream(main_)
             SPK_HIP(hipEventCreateWithFlags(&forks[size_t(dev)], hipEventDisableTiming));
             SPK_HIP(hipEventCreateWithFlags(&joins[size_t(dev)], hipEventDisableTiming));
         }
-        stream = streams[size_t(dev)], done = joins[size_t(dev)];
-        SPK_HIP(hipEventRecord(forks[size_t(dev)], main));
-        SPK_HIP(hipStreamWaitEvent(stream, forks[size_t(dev)], 0));
+        stream = streams[size_t(dev)], fork = forks[size_t(dev)], done = joins[size_t(dev)];
+        SPK_HIP(hipEventRecord(fork, main));
+        SPK_HIP(hipStreamWaitEvent(stream, fork, 0));
     }
+}
+
+void SideStream::again()
+{
+    if (stream == main) return;
+    SPK_HIP(hipEventRecord(fork, main));
+    SPK_HIP(hipStreamWaitEvent(stream, fork, 0));
 }
 
 void SideStream::join()

@@ -410,11 +410,12 @@ KernelTimer &timer();
 struct SideStream
 {
     hipStream_t main, stream;
-    hipEvent_t done = nullptr;
+    hipEvent_t fork = nullptr, done = nullptr;
     bool joined = false;
     SideStream(hipStream_t main, bool use);
     SideStream(const SideStream &) = delete;
     SideStream &operator=(const SideStream &) = delete;
+    void again();  // the side stream waits for the work the main stream has now
     void join();
     ~SideStream();
 };

@@ -85,6 +85,9 @@ struct RegionOut
     uint8_t *out;
     uint8_t *pooled;
     int32_t pkh, pkw;
+    // [instance] max(0, every pixel's maximum), as FP32 bits (their unsigned
+    // order is the order of non-negative floats); the caller zeroes it
+    uint32_t *imax;
 };
 
 // cin <= 4: K groups of two taps x 4 channels when that needs at most 8 K-steps.
@@ -624,14 +627,17 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
                     if (((cs >> 4) & 1) == r && (cs & 3) == i) om = mask[r][i], oxb = mxb[r][i], first = fv[r][i];
+            const float pmax = om ? __builtin_bit_cast(float, oxb) * TWO_POW_24 : 0.0f;
             if (own)
             {
                 const uint32_t fwd = om ? __builtin_bitreverse32(om) >> (32 - I.t) : 0u;
-                if (pvalid)
-                    summary[I.sum_off + int64_t(oy) * I.wo + ox] =
-                        Summary{cs, fwd, om ? first * TWO_POW_24 : 0.0f,
-                                om ? __builtin_bit_cast(float, oxb) * TWO_POW_24 : 0.0f};
+                if (pvalid) summary[I.sum_off + int64_t(oy) * I.wo + ox] = Summary{cs, fwd, om ? first * TWO_POW_24 : 0.0f, pmax};
                 sel_c[ct * COLS + j] = cs, sel_m[ct * COLS + j] = pvalid ? fwd : 0u;
+            }
+            {
+                // the instance's maximum for convwta (-0 and below count as +0)
+                const float mv = wave_max(own && pvalid && pmax > 0.0f ? pmax : 0.0f);
+                if (lane == 0 && mv > 0.0f) atomicMax(ro.imax + itm.x, __builtin_bit_cast(uint32_t, mv));
             }
             if (zs + zp > 0)
             {
@@ -655,52 +661,107 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
 }
 
 // convwta over one instance's summaries (fused_select's rule, wta.cpp:49-64,
-// 128-150) with the candidates in global memory, for planes of any size.
-// One block per instance; wkey[slot] = channel << 16 | pixel, -1 past the
-// instance's winners.
+// 128-150) for planes of any size, one block per instance.  The candidate of
+// pixel p is (v, c * hw + p) with v = (T-1-t0) + first/diff (-FLT_MAX when it
+// never fired), diff = 2 * max(0, the instance's potentials) and c its kept
+// channel; a round's winner is the first maximum
+// over the candidates not excluded by an earlier winner (same channel, or
+// inside its box).  Each thread keeps the SEL_TOP best candidates of its
+// pixels in registers, ordered, from one pass; a round takes every thread's
+// first candidate that no winner excludes.  A candidate outside the list is
+// excluded or ranks below all of the list, so that is the thread's best
+// unless every listed candidate is excluded; then (if the thread has more
+// pixels than the list) it refills the list with its best SEL_TOP candidates
+// that no winner excludes.  Winners only accumulate, so the rule holds for
+// the refilled list in later rounds.  wkey[slot] = channel << 16 | pixel, -1
+// past the instance's winners.
+constexpr int SEL_TOP = 8;
+constexpr int SEL_MAX = 256;  // winners an instance can have on the region path
+constexpr int SEL_BATCH = 8;  // summaries a thread loads together
+
 __global__ void __launch_bounds__(1024)
-    region_select(const Summary *__restrict__ summary, const RegionInst *__restrict__ inst, int32_t ry, int32_t rx,
-                  int32_t count, float *__restrict__ val, int32_t *__restrict__ chan, spk_winner *__restrict__ win,
-                  int64_t *__restrict__ counts, int32_t *__restrict__ wkey)
+    region_select(const Summary *__restrict__ summary, const RegionInst *__restrict__ inst,
+                  const uint32_t *__restrict__ imax, int32_t ry, int32_t rx, int32_t count,
+                  spk_winner *__restrict__ win, int64_t *__restrict__ counts, int32_t *__restrict__ wkey)
 {
-    __shared__ float mm[16];
     __shared__ ArgMax am[16];
+    __shared__ int32_t wz[SEL_MAX], wy[SEL_MAX], wx[SEL_MAX];  // the winners so far
     const int64_t s = blockIdx.x;
     const RegionInst I = inst[s];
     const int32_t hw = I.ho * I.wo;
     const Summary *sm = summary + I.sum_off;
-    float *v = val + I.sum_off;
-    int32_t *ch = chan + I.sum_off;
-    float m = 0.0f;
-    for (int32_t p = threadIdx.x; p < hw; p += blockDim.x) m = m < sm[p].max ? sm[p].max : m;
-    m = wave_max(m);
-    if ((threadIdx.x & 63) == 0) mm[threadIdx.x >> 6] = m;
-    __syncthreads();
-    m = mm[0];
-    for (int i = 1; i < int(blockDim.x >> 6); ++i) m = m < mm[i] ? mm[i] : m;
+    const int32_t bd = int32_t(blockDim.x);
+    // the instance's maximum potential (from 0, as the reference's scan),
+    // which the region kernel recorded
+    const float m = __builtin_bit_cast(float, imax[s]);
     const float diff = (m - 0.0f) * 2.0f;
-    for (int32_t p = threadIdx.x; p < hw; p += blockDim.x)
-    {
-        const Summary e = sm[p];
-        ch[p] = e.c;
-        v[p] = e.mask ? float(I.t - int32_t(__builtin_ctz(e.mask)) - 1) + (e.first - 0.0f) / diff : -FLT_MAX;
-    }
-    __syncthreads();
+    auto value = [&](const Summary &e, int32_t p, float &v, int32_t &key) {
+        v = e.mask ? float(I.t - int32_t(__builtin_ctz(e.mask)) - 1) + (e.first - 0.0f) / diff : -FLT_MAX;
+        key = e.c * hw + p;
+    };
+    auto excluded = [&](int32_t key, int32_t k) {
+        const int32_t z = key / hw, p = key % hw, py = p / I.wo, px = p % I.wo;
+        for (int32_t r = 0; r < k; ++r)
+            if (wz[r] == z || (abs(py - wy[r]) <= ry && abs(px - wx[r]) <= rx)) return true;
+        return false;
+    };
+    float tv[SEL_TOP];
+    int32_t tk[SEL_TOP];
+    const int32_t mine = hw > int32_t(threadIdx.x) ? (hw - int32_t(threadIdx.x) + bd - 1) / bd : 0;  // its pixels
+    // the thread's SEL_TOP best candidates that none of the first k winners
+    // excludes, ordered
+    auto fill = [&](int32_t k) {
+#pragma unroll
+        for (int u = 0; u < SEL_TOP; ++u) tv[u] = -INFINITY, tk[u] = INT32_MAX;
+        for (int32_t q0 = 0; q0 < mine; q0 += SEL_BATCH)
+        {
+            Summary e[SEL_BATCH];
+#pragma unroll
+            for (int u = 0; u < SEL_BATCH; ++u)
+                e[u] = q0 + u < mine ? sm[int32_t(threadIdx.x) + (q0 + u) * bd] : Summary{0, 0, 0.0f, 0.0f};
+#pragma unroll
+            for (int u = 0; u < SEL_BATCH; ++u)
+            {
+                if (q0 + u >= mine) break;
+                float v;
+                int32_t key;
+                value(e[u], int32_t(threadIdx.x) + (q0 + u) * bd, v, key);
+                if (!better(v, key, tv[SEL_TOP - 1], tk[SEL_TOP - 1]) || (k > 0 && excluded(key, k))) continue;
+                // insertion into the ordered list
+#pragma unroll
+                for (int w = SEL_TOP - 1; w > 0; --w)
+                {
+                    const bool shift = better(v, key, tv[w - 1], tk[w - 1]);
+                    const bool here = !shift && better(v, key, tv[w], tk[w]);
+                    const float nv = shift ? tv[w - 1] : (here ? v : tv[w]);
+                    const int32_t nk = shift ? tk[w - 1] : (here ? key : tk[w]);
+                    tv[w] = nv, tk[w] = nk;
+                }
+                if (better(v, key, tv[0], tk[0])) tv[0] = v, tk[0] = key;
+            }
+        }
+    };
+    fill(0);
     int32_t k = 0;
     for (; k < count; ++k)
     {
         ArgMax a{-INFINITY, INT32_MAX};
-        for (int32_t p = threadIdx.x; p < hw; p += blockDim.x)
-            if (better(v[p], ch[p] * hw + p, a.v, a.i)) a.v = v[p], a.i = ch[p] * hw + p;
+        bool found = false;
+#pragma unroll
+        for (int u = 0; u < SEL_TOP; ++u)
+            if (!found && tk[u] != INT32_MAX && !excluded(tk[u], k)) a.v = tv[u], a.i = tk[u], found = true;
+        if (!found && mine > SEL_TOP)
+        {
+            fill(k);
+            if (tk[0] != INT32_MAX) a.v = tv[0], a.i = tk[0];
+        }
         a = block_argmax(a, am);
-        if (fabsf(a.v - (-FLT_MAX)) < 1e-5f) break;
+        if (fabsf(a.v - (-FLT_MAX)) < 1e-5f || a.i == INT32_MAX) break;
         const int32_t z = a.i / hw, y = (a.i % hw) / I.wo, x = (a.i % hw) % I.wo;
         if (threadIdx.x == 0)
-            win[s * count + k] = spk_winner{0, 0, z, y, x}, wkey[s * count + k] = z << 16 | (a.i % hw);
-        for (int32_t p = threadIdx.x; p < hw; p += blockDim.x)
         {
-            const int32_t py = p / I.wo, px = p % I.wo;
-            if (ch[p] == z || (abs(py - y) <= ry && abs(px - x) <= rx)) v[p] = -FLT_MAX;
+            win[s * count + k] = spk_winner{0, 0, z, y, x}, wkey[s * count + k] = z << 16 | (a.i % hw);
+            wz[k] = z, wy[k] = y, wx[k] = x;
         }
         __syncthreads();
     }
@@ -882,13 +943,27 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+__global__ void __launch_bounds__(256) zero_words(uint32_t *__restrict__ p, int64_t n)
+{
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
 // A candidate launch configuration of the region kernel.
 struct RegionConfig
 {
     RegionGeom g;
     size_t lds = 0;
+    int grid = 0;  // blocks: one a compute unit, at most one an item
     int2 *items = nullptr;  // device table [g.items] of (instance, region)
+    // the instances in PIPE_CHUNKS chunks of about equal region counts: chunk
+    // c is items [cut_item[c], cut_item[c+1]), instances [cut_inst[c], cut_inst[c+1])
+    std::vector<int64_t> cut_item, cut_inst;
 };
+
+// Chunks of instances a step's region kernel runs in when the separate
+// writers follow it: each chunk's writers overlap the next chunk's kernel.
+constexpr int PIPE_CHUNKS = 8;
 
 template <int S, bool DIAG, bool PAIR>
 void launch_region(const uint8_t *in, const half8 *frag, const RegionInst *inst, const RegionConfig &rc,
@@ -966,23 +1041,25 @@ struct LayerPlan
     int64_t in_bytes = 0, out_bytes = 0, pool_bytes = 0, pixels = 0, fs_bytes = 0, max_hw = 0, max_phw = 0;
     std::vector<std::unique_ptr<RegionConfig>> configs;
     std::vector<std::unique_ptr<DeviceBuffer<int2>>> item_tables;
-    int chosen = 0, grid = 0;
+    int chosen = 0;
     bool fuse_out = true;  // the region kernel writes the outputs (else the separate writers)
     DeviceBuffer<RegionInst> inst;
     DeviceBuffer<half8> frag;
     DeviceBuffer<Summary> summary;
     DeviceBuffer<uint8_t> fs;
-    DeviceBuffer<int32_t> wkey, chan;
+    DeviceBuffer<int32_t> wkey;
+    DeviceBuffer<uint32_t> imax;  // [n] the instances' maximum potentials (RegionOut::imax)
     DeviceBuffer<uint32_t> bits;
-    DeviceBuffer<float> val;
     FusedStdp stdp{};
 
     void run(const uint8_t *in, float *kernel, const spk_stdp_config *cfg, uint8_t *out, uint8_t *pooled,
              spk_winner *winners, int64_t *counts, float *pot, hipStream_t s);
     void choose(const uint8_t *in, uint8_t *out, uint8_t *pooled, hipStream_t s);
     void write_step(const uint8_t *in, const RegionConfig &rc, bool fused, uint8_t *out, uint8_t *pooled, float *pot,
-                    uint8_t *fsp, hipStream_t s, bool writers);
-    void launch_writers(const RegionConfig &rc, bool fused, uint8_t *out, uint8_t *pooled, hipStream_t ws);
+                    uint8_t *fsp, hipStream_t s, SideStream *side);
+    void launch_select(spk_winner *winners, int64_t *counts, hipStream_t s, int64_t i0, int64_t i1);
+    void launch_writers(bool fused, const RegionConfig &rc, uint8_t *out, uint8_t *pooled, hipStream_t ws, int64_t i0,
+                        int64_t i1);
     bool pool_in_kernel(const RegionConfig &rc, bool fused, const uint8_t *pooled) const;
 };
 
@@ -1043,7 +1120,11 @@ void LayerPlan::choose(const uint8_t *in, uint8_t *out, uint8_t *pooled, hipStre
             for (int rep = 0; rep < 2; ++rep)  // the first launch configures and warms
             {
                 SPK_HIP(hipEventRecord(e0, s));
-                write_step(in, *configs[c], fused != 0, out, pooled, nullptr, fs.p, s, true);
+                {
+                    SideStream side(s, fused == 0);
+                    write_step(in, *configs[c], fused != 0, out, pooled, nullptr, fs.p, s, &side);
+                    side.join();
+                }
                 SPK_HIP(hipEventRecord(e1, s));
                 SPK_HIP(hipEventSynchronize(e1));
                 SPK_HIP(hipEventElapsedTime(&ms, e0, e1));
@@ -1056,34 +1137,71 @@ void LayerPlan::choose(const uint8_t *in, uint8_t *out, uint8_t *pooled, hipStre
 
 // The region kernel with config rc and the step's outputs: written inside the
 // kernel (the pool only when its windows lie inside one group of region
-// rows; nct is the same for every candidate), or by the separate writers, on
-// `s` after it when `writers` (else the caller launches them on a side stream).
+// rows), or by the separate writers on
+// the side stream (on `s` after the kernel without one).  With the separate
+// writers the kernel runs over the instances in chunks, and each chunk's
+// writers start as soon as its summaries are there, overlapping the next
+// chunk's kernel (which moves little HBM while the writers move little
+// else); the caller joins the side stream.
 bool LayerPlan::pool_in_kernel(const RegionConfig &rc, bool fused, const uint8_t *pooled) const
 {
     return fused && pool && pooled && psh == pkh && psw == pkw && rc.g.nct % pkh == 0 && RW % pkw == 0;
 }
 
 void LayerPlan::write_step(const uint8_t *in, const RegionConfig &rc, bool fused, uint8_t *out, uint8_t *pooled,
-                           float *pot, uint8_t *fsp, hipStream_t s, bool writers)
+                           float *pot, uint8_t *fsp, hipStream_t s, SideStream *side)
 {
-    const RegionOut ro{fused ? out : nullptr, pool_in_kernel(rc, fused, pooled) ? pooled : nullptr, pkh, pkw};
-    dispatch_region(in, frag.p, inst.p, rc, summary.p, pot, fsp, ro, grid, s);
-    if (writers) launch_writers(rc, fused, out, pooled, s);  // else the caller launches them on its side stream
+    const RegionOut ro{fused ? out : nullptr, pool_in_kernel(rc, fused, pooled) ? pooled : nullptr, pkh, pkw, imax.p};
+    const bool writers = (out && !fused) || (pool && pooled && !pool_in_kernel(rc, fused, pooled));
+    const int64_t chunks = int64_t(rc.cut_item.size()) - 1;
+    if (!writers || !side || pot || chunks < 2)
+    {
+        dispatch_region(in, frag.p, inst.p, rc, summary.p, pot, fsp, ro, rc.grid, s);
+        if (writers)
+        {
+            if (side) side->again();
+            launch_writers(fused, rc, out, pooled, side ? side->stream : s, 0, n);
+        }
+        return;
+    }
+    // a chunk's writers on the side stream, beside the next chunks' kernels
+    for (int64_t c = 0; c < chunks; ++c)
+    {
+        RegionConfig part;
+        part.g = rc.g, part.lds = rc.lds, part.items = rc.items + rc.cut_item[size_t(c)];
+        part.g.items = rc.cut_item[size_t(c) + 1] - rc.cut_item[size_t(c)];
+        dispatch_region(in, frag.p, inst.p, part, summary.p, nullptr, fsp, ro,
+                        int(std::min<int64_t>(rc.grid, part.g.items)), s);
+        side->again();
+        launch_writers(fused, rc, out, pooled, side->stream, rc.cut_inst[size_t(c)], rc.cut_inst[size_t(c) + 1]);
+    }
 }
 
-void LayerPlan::launch_writers(const RegionConfig &rc, bool fused, uint8_t *out, uint8_t *pooled, hipStream_t ws)
+// convwta of instances [i0, i1).
+void LayerPlan::launch_select(spk_winner *winners, int64_t *counts, hipStream_t s, int64_t i0, int64_t i1)
+{
+    if (i1 <= i0) return;
+    region_select<<<dim3(unsigned(i1 - i0)), 1024, 0, s>>>(summary.p, inst.p + i0, imax.p + i0, ry, rx, count,
+                                                           winners + i0 * count, counts + i0, wkey.p + i0 * count);
+    SPK_LAUNCHED();
+}
+
+// The separate writers for instances [i0, i1).
+void LayerPlan::launch_writers(bool fused, const RegionConfig &rc, uint8_t *out, uint8_t *pooled, hipStream_t ws,
+                               int64_t i0, int64_t i1)
 {
     const bool pool_in = pool_in_kernel(rc, fused, pooled);
+    if (i1 <= i0) return;
     if (out && !fused)
     {
-        region_spikes<<<dim3(unsigned(cdiv(max_hw, WRITE_THREADS)), unsigned(n)), WRITE_THREADS, 0, ws>>>(
-            summary.p, inst.p, cout, out);
+        region_spikes<<<dim3(unsigned(cdiv(max_hw, WRITE_THREADS)), unsigned(i1 - i0)), WRITE_THREADS, 0, ws>>>(
+            summary.p, inst.p + i0, cout, out);
         SPK_LAUNCHED();
     }
     if (pool && pooled && !pool_in)
     {
-        region_pooled<<<dim3(unsigned(cdiv(max_phw, WRITE_THREADS)), unsigned(n)), WRITE_THREADS, 0, ws>>>(
-            summary.p, inst.p, cout, pooled, pkh, pkw, psh, psw);
+        region_pooled<<<dim3(unsigned(cdiv(max_phw, WRITE_THREADS)), unsigned(i1 - i0)), WRITE_THREADS, 0, ws>>>(
+            summary.p, inst.p + i0, cout, pooled, pkh, pkw, psh, psw);
         SPK_LAUNCHED();
     }
 }
@@ -1111,20 +1229,21 @@ void LayerPlan::run(const uint8_t *in, float *kernel, const spk_stdp_config *cfg
         SPK_HIP(hipEventRecord(e0, s));
     }
     const RegionConfig &rc = *configs[chosen];
-    write_step(in, rc, fuse_out, out, pooled, pot, learn ? fs.p : nullptr, s, false);
+    // outputs the kernel does not write: the separate writers on the side
+    // stream, beside the kernel's later chunks
+    const bool sep = (out && !fuse_out) || (pool && pooled && !pool_in_kernel(rc, fuse_out, pooled));
+    zero_words<<<dim3(unsigned(cdiv(n, 256))), 256, 0, s>>>(imax.p, n);  // a kernel: capturable
+    SPK_LAUNCHED();
+    SideStream side(s, sep);
+    write_step(in, rc, fuse_out, out, pooled, pot, learn ? fs.p : nullptr, s, sep ? &side : nullptr);
     if (tm.on)
     {
         SPK_HIP(hipEventRecord(e1, s));
         tm.events.emplace_back(e0, e1);
     }
-    // outputs the kernel does not write: the separate writers on the side
-    // stream, beside convwta, the records and the STDP chains
-    const bool sep = (out && !fuse_out) || (pool && pooled && !pool_in_kernel(rc, fuse_out, pooled));
-    SideStream side(s, sep);
-    if (sep) launch_writers(rc, fuse_out, out, pooled, side.stream);
-    region_select<<<dim3(unsigned(n)), 1024, 0, s>>>(summary.p, inst.p, ry, rx, count, val.p, chan.p, winners, counts,
-                                                     wkey.p);
-    SPK_LAUNCHED();
+    // convwta on the kernel's stream, beside the last chunk's writers
+    launch_select(winners, counts, s, 0, n);
+    side.join();  // the winners, for the records
     if (learn)
     {
         region_records<<<dim3(unsigned(cdiv(stdp.slots, 4))), 256, 0, s>>>(fs.p, summary.p, inst.p, wkey.p, stdp,
@@ -1134,7 +1253,6 @@ void LayerPlan::run(const uint8_t *in, float *kernel, const spk_stdp_config *cfg
                           bits.p, wkey.p, stdp, *cfg);
         SPK_LAUNCHED();
     }
-    side.join();
 }
 
 // Makes a plan; shapes: host [n][3] of (t, h, w).
@@ -1150,6 +1268,7 @@ LayerPlan *make_plan(int64_t n, const int32_t *shapes, int64_t cin, int64_t cout
     SPK_UNSUP(threshold >= 0.0, "Fused layer: the conv threshold must be >= 0.");
     SPK_UNSUP(threshold_scales(threshold), "Fused layer: the threshold is too small to scale exactly.");
     SPK_ARG(ry >= 0 && rx >= 0 && count >= 1, "Fused layer: invalid convwta parameters.");
+    SPK_UNSUP(count <= SEL_MAX, "Fused layer: at most " << SEL_MAX << " winners per instance on the region path.");
     SPK_UNSUP(kh + RW + kw <= 64, "Fused layer: kernel too large for the region path.");
     auto P = std::make_unique<LayerPlan>();
     SPK_HIP(hipGetDevice(&P->device));
@@ -1199,6 +1318,8 @@ LayerPlan *make_plan(int64_t n, const int32_t *shapes, int64_t cin, int64_t cout
     P->fast = fast;
 
     // candidates: region rows nct * m, m = 1, 2, 4, 8 where the LDS holds them
+    int cus = 0;
+    SPK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, P->device));
     for (int32_t m : {1, 2, 4, 8})
     {
         RegionGeom g;
@@ -1217,6 +1338,18 @@ LayerPlan *make_plan(int64_t n, const int32_t *shapes, int64_t cin, int64_t cout
         SPK_HIP(hipMemcpy(tab->p, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
         auto rc = std::make_unique<RegionConfig>();
         rc->g = g, rc->lds = lds, rc->items = tab->p;
+        rc->grid = int(std::min<int64_t>(g.items, cus));
+        // chunk cuts at instance boundaries, about g.items / PIPE_CHUNKS regions each
+        rc->cut_item.push_back(0), rc->cut_inst.push_back(0);
+        int64_t acc = 0;
+        for (int64_t i = 0; i < n; ++i)
+        {
+            acc += cdiv(P->host[size_t(i)].ho, g.rh) * P->host[size_t(i)].cols;
+            const int64_t c = int64_t(rc->cut_item.size());
+            if (i + 1 < n && c < PIPE_CHUNKS && acc * PIPE_CHUNKS >= c * g.items)
+                rc->cut_item.push_back(acc), rc->cut_inst.push_back(i + 1);
+        }
+        rc->cut_item.push_back(g.items), rc->cut_inst.push_back(n);
         P->item_tables.push_back(std::move(tab));
         P->configs.push_back(std::move(rc));
         if (!tune) break;  // without tuning: the smallest region that fits
@@ -1229,11 +1362,6 @@ LayerPlan *make_plan(int64_t n, const int32_t *shapes, int64_t cin, int64_t cout
     // its 16-byte output segments (about one a cycle for cout * nct segments
     // a block), else by the separate writers, whose stores coalesce
     P->fuse_out = 32 * g.steps >= g.cout * g.nct;
-    int cus = 0;
-    SPK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, P->device));
-    int64_t min_items = g.items;
-    for (auto &c : P->configs) min_items = std::min(min_items, c->g.items);
-    P->grid = int(std::min<int64_t>(min_items, cus));
 
     P->inst.reset(n);
     SPK_HIP(hipMemcpy(P->inst.p, P->host.data(), size_t(n) * sizeof(RegionInst), hipMemcpyHostToDevice));
@@ -1241,8 +1369,7 @@ LayerPlan *make_plan(int64_t n, const int32_t *shapes, int64_t cin, int64_t cout
     P->summary.reset(pix);
     P->fs.reset(fs_b);
     P->wkey.reset(n * count);
-    P->chan.reset(pix);
-    P->val.reset(pix);
+    P->imax.reset(n);
     const int64_t ksz = cin * kh * kw;
     P->stdp = FusedStdp{n * count, int32_t(count), int32_t(cin), int32_t(cin_pad), 0, 0, int32_t(kh), int32_t(kw),
                         0, 0, 0, 0, 0, int32_t(cdiv(ksz, 32))};

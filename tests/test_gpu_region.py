@@ -174,6 +174,54 @@ def test_region_uneven_batch(dev, orc, tune):
     assert np.array_equal(H(conv.kernel), o_k)
 
 
+def test_region_convwta_refills_candidates(dev, orc):
+    """convwta where one channel fires first nearly everywhere: after that
+    channel's winner, the later winners sit in threads whose listed
+    candidates (their 8 best pixels, pixel p on thread p % 1024) all belong to
+    that channel, so those threads find them only by refilling the list."""
+    def sample(seed, t, h, w):
+        rng = np.random.default_rng(seed)
+        x = np.zeros((t, 4, h, w), np.uint8)
+        x[:, 0] = rng.random((t, h, w)) < 0.35
+        for _ in range(6):  # patches without channel 0 that start firing at step 3
+            y0, x0, c = rng.integers(2, h - 14), rng.integers(2, w - 14), rng.integers(1, 4)
+            x[:, 0, y0 - 2:y0 + 14, x0 - 2:x0 + 14] = 0
+            x[3:, c, y0:y0 + 12, x0:x0 + 12] = rng.random((t - 3, 12, 12)) < 0.6
+        return np.maximum.accumulate(x, axis=0)
+
+    cout, pad, radius, count, pool = 32, (2, 2, 2, 2), 2, 6, (2, 2)
+    kern = np.full((cout, 4, 5, 5), 8 / 64, np.float32)
+    kern[0, 0] = 48 / 64
+    for c in range(1, cout):
+        kern[c, 1 + c % 3] = (40 + c) / 64
+    shapes = [(8, 128, 128), (6, 100, 120)]
+    xs = [sample(50 + i, *s) for i, s in enumerate(shapes)]
+    th = float(np.quantile(orc.conv(xs[0], kern, 1, pad), 0.8)) + 0.5 / 64
+    conv = make_conv(dev, kern, pad)
+    plan = spk.LayerPlan(conv, shapes, th, radius, count, *pool, tune=False)
+    out, pooled, wn = plan.run(plan.pack([T(x, dev) for x in xs]))
+    o_k = kern
+    for i, x in enumerate(xs):
+        o_out, o_win, o_cnt, o_pool = oracle_forward(orc, x[None], kern, th, radius, count, pad, pool)
+        # the case: winner 0 in channel 0 at step 0, and every later winner's
+        # thread holds >= 8 channel-0 pixels that fired before it
+        o = o_out[0]
+        fired = o.any(axis=0)
+        chan = np.where(fired.any(axis=0), fired.argmax(axis=0), 0).reshape(-1)
+        t0 = np.where(fired.any(axis=0), o.any(axis=1).argmax(axis=0), o.shape[0]).reshape(-1)
+        hw = chan.size
+        assert o_cnt[0] >= 3 and o_win[0, 0]["z"] == 0, o_cnt
+        for k in range(1, o_cnt[0]):
+            p = int(o_win[0, k]["y"]) * o.shape[3] + int(o_win[0, k]["x"])
+            mine = np.arange(p % 1024, hw, 1024)
+            assert ((chan[mine] == 0) & (t0[mine] < t0[p])).sum() >= 8, (i, k)
+        assert np.array_equal(H(plan.spikes(out, i)), o_out[0]), i
+        assert np.array_equal(H(plan.pooled(pooled, i)), o_pool[0]), i
+        check_winners(wn, o_win, o_cnt, i)
+        o_k = orc.convstdp(x[None], o_k, o_out, [(0.004, -0.003)], o_win, o_cnt, 1, pad)
+    assert np.array_equal(H(conv.kernel), o_k)
+
+
 def test_region_precision_vs_fp64(dev, orc):
     n, t, cin, h, w, cout = 2, 12, 8, 48, 48, 64
     rng = np.random.default_rng(35)
