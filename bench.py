@@ -53,8 +53,8 @@ PEAK_HBM = 8.0e12
 TRAFFIC_FUSED_LAYER = {"fp16": 979.2e6, "int8": 985.7e6}
 # The region kernel per launch on the precision workload, and the whole batching-workload step
 # (region kernel, writers, convwta), from the same counters: profiles/fused_region_traffic.txt.
-TRAFFIC_REGION_C3 = 26.27e9
-TRAFFIC_STEP_C4 = 85.2e9
+TRAFFIC_REGION_C3 = 26.86e9
+TRAFFIC_STEP_C4 = 79.7e9
 WARM_S = 0.3  # seconds of untimed fused steps before the timed ones (at least --warmup steps)
 METRIC = "images\u00b7timesteps/s, conv+fire+wta+STDP layer: FP16-batched vs FP32 plain, 1 GPU"
 
@@ -373,7 +373,17 @@ def c3_inputs(spk, dev, n=C3_N, seed=3):
     return x, k
 
 
-def c3_bench(spk, dev, steps):
+def pick(plan, choice):
+    """Fixes the plan's candidate to (rows a region, write mode), when given."""
+    if choice:
+        cands, _, _ = plan.candidates()
+        rows = [c[0] for c in cands]
+        if choice[0] not in rows:
+            raise SystemExit("no plan candidate with %d rows (candidates: %s)" % (choice[0], rows))
+        plan.select(rows.index(choice[0]), choice[1])
+
+
+def c3_bench(spk, dev, steps, choice=None):
     import torch
     x, k = c3_inputs(spk, dev)
     n = x.shape[0]
@@ -384,6 +394,7 @@ def c3_bench(spk, dev, steps):
     packed = x.reshape(-1)
     bufs = plan.buffers()
     plan.run(packed, buffers=bufs)  # tunes the region height
+    pick(plan, choice)
     spk.call("spk_kernel_timer", 1)
     dt = timed(lambda: plan.run(packed, buffers=bufs), steps, 1)
     total_ms, launches = C.c_double(0), C.c_int64(0)
@@ -471,7 +482,7 @@ def c4_inputs(spk, dev, n=C4_N, seed=4):
     return shapes, coded, k
 
 
-def c4_bench(spk, dev, steps):
+def c4_bench(spk, dev, steps, choice=None):
     import torch
     shapes, coded, k = c4_inputs(spk, dev)
     conv = spk.Conv(4, C4_COUT, 5, pad=2, device=dev)
@@ -482,6 +493,7 @@ def c4_bench(spk, dev, steps):
     del coded
     bufs = plan.buffers()
     plan.run(packed, buffers=bufs)  # tunes the region height, configures the kernels
+    pick(plan, choice)
     dt = timed(lambda: plan.run(packed, buffers=bufs), steps, 1)
     # the same step captured once and replayed (the launches are few, so a
     # replay saves little; the graph executor's queue assignment can order a
@@ -651,6 +663,10 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the precision and batching workloads")
     ap.add_argument("--arith", choices=["int8", "fp16"], default="int8",
                     help="the whole-sample kernel's arithmetic (int8 fixed point, the library default, or fp16)")
+    ap.add_argument("--plan", type=int, nargs=2, metavar=("ROWS", "WRITES"),
+                    help="with --only c3/c4: run the plan candidate of ROWS rows a region and write mode WRITES "
+                         "instead of the tuner's pick (for counter runs, whose serialized dispatches skew the "
+                         "tuner's timing)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (HBM counter calibration under rocprofv3)")
     a = ap.parse_args()
@@ -676,7 +692,7 @@ def main():
     if a.sparse_only:
         a.only = "sparse"
     if a.only:
-        fns = {"c3": lambda: c3_bench(spk, dev, a.steps), "c4": lambda: c4_bench(spk, dev, a.steps),
+        fns = {"c3": lambda: c3_bench(spk, dev, a.steps, a.plan), "c4": lambda: c4_bench(spk, dev, a.steps, a.plan),
                "sparse": lambda: sparse_bench(spk, dev, a.steps, a.warmup, not a.no_cpu),
                "front": lambda: front_end_bench(spk, dev, not a.no_cpu)}
         print(json.dumps({a.only: fns[a.only]()}))
