@@ -49,7 +49,7 @@ using u32x4 = __attribute__((ext_vector_type(4))) uint32_t;
 constexpr int RW = 16;            // region width: one column tile per region row
 constexpr int REGION_WAVES = 8;   // waves per block
 constexpr int LDS_BYTES = 163840;
-constexpr int SU = 8;  // stage rows a thread loads together
+constexpr int SU = 16;  // stage rows a thread loads together
 constexpr int WU = 4;  // window (step, row) pairs a thread converts together
 
 struct RegionInst
