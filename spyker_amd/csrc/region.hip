@@ -396,9 +396,7 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
                     }
                 }
                 __syncthreads();
-                // the chunk's steps without barriers: the loads of step k+2 and
-                // the MFMAs of step k+1 are issued before the update of step k
-                // (loads past the chunk re-read its last step, unused)
+                // the chunk's steps without barriers
                 // B operands: up to 8 K-steps two whole steps of buffers
                 // (b0, b1), loaded two steps ahead of their use; longer chains
                 // (a whole step of buffers spills at 13 K-steps) a ring of R
@@ -442,7 +440,7 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
                     }
                 };
                 auto step_base = [&](int32_t k) { return wb + min(k, tc - 1) * win_step; };
-                if (RING)
+                if constexpr (RING)
                 {
 #pragma unroll
                     for (int s = 0; s < R; ++s) load1(b0[s], s, wb);
@@ -451,8 +449,7 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
                 else
                 {
                     load(b0, wb);
-                    mfma_chain(a, b0, c0, c1);
-                    load(b0, step_base(1));
+                    load(b1, step_base(1));
                 }
                 auto update = [&](int32_t k, const float4v &u0, const float4v &u1) {
                     step_update_first(u0, u1, g.th, mask, mxb, fv);
@@ -487,18 +484,19 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
                             }
                 };
                 if (!RING)
-                    // the loads of step k+2 and the MFMAs of step k+1 are
-                    // issued before step k's update, which reads step k's
-                    // results (loads past the chunk re-read its last step)
+                    // two steps an iteration: both steps' MFMAs, each chain's
+                    // operands reloaded for two steps later as soon as it has
+                    // issued, then both updates (no accumulator lives across
+                    // iterations; loads past the chunk re-read its last step)
                     for (int32_t k = 0; k < tc; k += 2)
                     {
                         if constexpr (!RING)
                         {
-                            load(b1, step_base(k + 2));
-                            if (k + 1 < tc) mfma_chain(a, b0, n0, n1);
+                            mfma_chain(a, b0, c0, c1);
+                            load(b0, step_base(k + 2));
+                            if (k + 1 < tc) mfma_chain(a, b1, n0, n1);
+                            load(b1, step_base(k + 3));
                             update(k, c0, c1);
-                            load(b0, step_base(k + 3));
-                            if (k + 2 < tc) mfma_chain(a, b1, c0, c1);
                             if (k + 1 < tc) update(k + 1, n0, n1);
                         }
                     }
