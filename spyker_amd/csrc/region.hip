@@ -122,7 +122,7 @@ inline int64_t region_fixed(int32_t waves, int32_t nct, int32_t steps)
     return int64_t(waves) * COLS * 12 + int64_t(nct) * COLS * 12 + int64_t(steps) * 4 * 16;
 }
 
-template <int S, bool DIAG, bool PAIR>
+template <int S, bool DIAG, bool PAIR, bool BSTAGE>
 __global__ void __launch_bounds__(64 * REGION_WAVES)
     fused_region(const uint8_t *__restrict__ in, const half8 *__restrict__ frag, const RegionInst *__restrict__ inst,
                  const int2 *__restrict__ items, Summary *__restrict__ summary, float *__restrict__ pot_out,
@@ -241,36 +241,81 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
                 uint32_t *dst = reinterpret_cast<uint32_t *>(stage);
                 const int32_t ix0 = sx0 + 4 * lq;
                 const bool colv = ix0 >= 0 && ix0 < I.w && sc < g.cin;
-                int32_t tt = sslot / g.sh, lr = sslot % g.sh;
-                const int32_t dq = sslots / g.sh, dr = sslots % g.sh;
-                // batches of SU rows a slot, the batch's loads in flight together
-                for (int32_t r0 = sslot; r0 < srows; r0 += SU * sslots)
+                if constexpr (BSTAGE)
                 {
-                    uint32_t v[SU];
-                    int32_t at[SU];
-#pragma unroll
-                    for (int u = 0; u < SU; ++u)
+                    // raw buffer loads over the instance's input (fewer than
+                    // 2^31 bytes, checked when the plan is made) at an offset
+                    // kept by additions (step tt, channel sc, input row
+                    // sy0 + lr, column ix0) as the slot walks the flattened
+                    // stage rows r = tt * sh + lr by sslots = dq * sh + dr;
+                    // the offset of its end reads 0, which stands for rows
+                    // above or below the image, columns or channels outside it
+                    // and the walk's tail
+                    const int32_t plane = I.h * I.w, tstride = g.cin * plane, bytes = I.t * tstride;
+                    const __amdgpu_buffer_rsrc_t rs =
+                        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t *>(src), 0, bytes, 0x00020000);
+                    const int32_t dq = sslots / g.sh, dr = sslots % g.sh;
+                    int32_t lr = sslot % g.sh;
+                    int32_t off = (sslot / g.sh) * tstride + sc * plane + (sy0 + lr) * I.w + ix0;
+                    const int32_t d_off = dq * tstride + dr * I.w, wrap_off = tstride - g.sh * I.w;
+                    const int32_t lane_at = (4 * lq + l4) * D + c4, row_at = g.sw * D;
+                    for (int32_t r0 = sslot; r0 < srows; r0 += SU * sslots)
                     {
-                        const int32_t iy = sy0 + lr;
-                        v[u] = 0;
-                        if (r0 + u * sslots < srows && colv && iy >= 0 && iy < I.h)
-                            v[u] = *reinterpret_cast<const uint32_t *>(src + ((int64_t(tt) * g.cin + sc) * I.h + iy) *
-                                                                                 I.w +
-                                                                       ix0);
-                        at[u] = ((tt * g.sh + lr) * g.sw + 4 * lq + l4) * D + c4;
-                        tt += dq, lr += dr;
-                        if (lr >= g.sh) lr -= g.sh, ++tt;
+                        uint32_t v[SU];
+#pragma unroll
+                        for (int u = 0; u < SU; ++u)
+                        {
+                            const int32_t iy = sy0 + lr;
+                            const bool ok = r0 + u * sslots < srows && colv && iy >= 0 && iy < I.h;
+                            v[u] = uint32_t(__builtin_amdgcn_raw_buffer_load_b32(rs, ok ? off : bytes, 0, 0));
+                            lr += dr, off += d_off;
+                            if (lr >= g.sh) lr -= g.sh, off += wrap_off;
+                        }
+#pragma unroll
+                        for (int u = 0; u < SU; ++u)
+                        {
+                            // the 4x4 byte transpose of the flat walk below
+                            const uint32_t x = v[u];
+                            const uint32_t y = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(x), 0xB1, 0xF, 0xF, true));
+                            const uint32_t r1 = __builtin_amdgcn_perm(y, x, sel1);
+                            const uint32_t y2 = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(r1), 0x4E, 0xF, 0xF, true));
+                            if (r0 + u * sslots < srows)
+                                dst[(r0 + u * sslots) * row_at + lane_at] = __builtin_amdgcn_perm(y2, r1, sel2);
+                        }
                     }
-#pragma unroll
-                    for (int u = 0; u < SU; ++u)
+                }
+                else
+                {
+                    int32_t tt = sslot / g.sh, lr = sslot % g.sh;
+                    const int32_t dq = sslots / g.sh, dr = sslots % g.sh;
+                    // batches of SU rows a slot, the batch's loads in flight together
+                    for (int32_t r0 = sslot; r0 < srows; r0 += SU * sslots)
                     {
-                        // 4x4 byte transpose within the quad: lane l4 then holds
-                        // channels 4*c4 .. 4*c4+3 of pixel 4*lq + l4
-                        const uint32_t x = v[u];
-                        const uint32_t y = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(x), 0xB1, 0xF, 0xF, true));
-                        const uint32_t r1 = __builtin_amdgcn_perm(y, x, sel1);
-                        const uint32_t y2 = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(r1), 0x4E, 0xF, 0xF, true));
-                        if (r0 + u * sslots < srows) dst[at[u]] = __builtin_amdgcn_perm(y2, r1, sel2);
+                        uint32_t v[SU];
+                        int32_t at[SU];
+#pragma unroll
+                        for (int u = 0; u < SU; ++u)
+                        {
+                            const int32_t iy = sy0 + lr;
+                            v[u] = 0;
+                            if (r0 + u * sslots < srows && colv && iy >= 0 && iy < I.h)
+                                v[u] = *reinterpret_cast<const uint32_t *>(
+                                    src + ((int64_t(tt) * g.cin + sc) * I.h + iy) * I.w + ix0);
+                            at[u] = ((tt * g.sh + lr) * g.sw + 4 * lq + l4) * D + c4;
+                            tt += dq, lr += dr;
+                            if (lr >= g.sh) lr -= g.sh, ++tt;
+                        }
+#pragma unroll
+                        for (int u = 0; u < SU; ++u)
+                        {
+                            // 4x4 byte transpose within the quad: lane l4 then holds
+                            // channels 4*c4 .. 4*c4+3 of pixel 4*lq + l4
+                            const uint32_t x = v[u];
+                            const uint32_t y = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(x), 0xB1, 0xF, 0xF, true));
+                            const uint32_t r1 = __builtin_amdgcn_perm(y, x, sel1);
+                            const uint32_t y2 = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(r1), 0x4E, 0xF, 0xF, true));
+                            if (r0 + u * sslots < srows) dst[at[u]] = __builtin_amdgcn_perm(y2, r1, sel2);
+                        }
                     }
                 }
             }
@@ -963,32 +1008,37 @@ struct RegionConfig
 // writers follow it: each chunk's writers overlap the next chunk's kernel.
 constexpr int PIPE_CHUNKS = 8;
 
-template <int S, bool DIAG, bool PAIR>
+template <int S, bool DIAG, bool PAIR, bool BSTAGE>
 void launch_region(const uint8_t *in, const half8 *frag, const RegionInst *inst, const RegionConfig &rc,
                    Summary *summary, float *pot, uint8_t *fs, const RegionOut &ro, int grid, hipStream_t s)
 {
     static bool configured = false;  // per instantiation; the attribute is per function
     if (!configured)
     {
-        SPK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&fused_region<S, DIAG, PAIR>),
+        SPK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&fused_region<S, DIAG, PAIR, BSTAGE>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, LDS_BYTES));
         configured = true;
     }
-    fused_region<S, DIAG, PAIR><<<dim3(grid), dim3(64 * rc.g.waves), rc.lds, s>>>(in, frag, inst, rc.items,
-                                                                                summary, pot, fs, ro, rc.g);
+    fused_region<S, DIAG, PAIR, BSTAGE><<<dim3(grid), dim3(64 * rc.g.waves), rc.lds, s>>>(in, frag, inst, rc.items,
+                                                                                        summary, pot, fs, ro, rc.g);
     SPK_LAUNCHED();
 }
 
+// bstage: the buffer-load staging walk, denser than the flat one (faster
+// alone, but it slows writer kernels running beside the kernel); the
+// potentials variant always stages by flat loads.
 void dispatch_region(const uint8_t *in, const half8 *frag, const RegionInst *inst, const RegionConfig &rc,
-                     Summary *summary, float *pot, uint8_t *fs, const RegionOut &ro, int grid, hipStream_t s)
+                     Summary *summary, float *pot, uint8_t *fs, const RegionOut &ro, int grid, hipStream_t s,
+                     bool bstage)
 {
     if (rc.g.pair)
         switch (rc.g.steps)
         {
 #define CASE(k)                                                                              \
     case k:                                                                                  \
-        return pot ? launch_region<k, true, true>(in, frag, inst, rc, summary, pot, fs, ro, grid, s) \
-                   : launch_region<k, false, true>(in, frag, inst, rc, summary, pot, fs, ro, grid, s);
+        return pot      ? launch_region<k, true, true, false>(in, frag, inst, rc, summary, pot, fs, ro, grid, s) \
+               : bstage ? launch_region<k, false, true, true>(in, frag, inst, rc, summary, pot, fs, ro, grid, s) \
+                        : launch_region<k, false, true, false>(in, frag, inst, rc, summary, pot, fs, ro, grid, s);
             CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE
         default: SPK_ARG(false, "Fused layer: K-steps out of range.");
@@ -997,8 +1047,9 @@ void dispatch_region(const uint8_t *in, const half8 *frag, const RegionInst *ins
     {
 #define CASE(k)                                                                               \
     case k:                                                                                   \
-        return pot ? launch_region<k, true, false>(in, frag, inst, rc, summary, pot, fs, ro, grid, s) \
-                   : launch_region<k, false, false>(in, frag, inst, rc, summary, pot, fs, ro, grid, s);
+        return pot      ? launch_region<k, true, false, false>(in, frag, inst, rc, summary, pot, fs, ro, grid, s) \
+               : bstage ? launch_region<k, false, false, true>(in, frag, inst, rc, summary, pot, fs, ro, grid, s) \
+                        : launch_region<k, false, false, false>(in, frag, inst, rc, summary, pot, fs, ro, grid, s);
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
         CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
 #undef CASE
@@ -1140,7 +1191,8 @@ void LayerPlan::choose(const uint8_t *in, uint8_t *out, uint8_t *pooled, hipStre
 // writers the kernel runs over the instances in chunks, and each chunk's
 // writers start as soon as its summaries are there, overlapping the next
 // chunk's kernel (which moves little HBM while the writers move little
-// else); the caller joins the side stream.
+// else); the caller joins the side stream.  Only a kernel with no writers
+// beside it stages by buffer loads.
 bool LayerPlan::pool_in_kernel(const RegionConfig &rc, bool fused, const uint8_t *pooled) const
 {
     return fused && pool && pooled && psh == pkh && psw == pkw && rc.g.nct % pkh == 0 && RW % pkw == 0;
@@ -1154,7 +1206,7 @@ void LayerPlan::write_step(const uint8_t *in, const RegionConfig &rc, bool fused
     const int64_t chunks = int64_t(rc.cut_item.size()) - 1;
     if (!writers || !side || pot || chunks < 2)
     {
-        dispatch_region(in, frag.p, inst.p, rc, summary.p, pot, fsp, ro, rc.grid, s);
+        dispatch_region(in, frag.p, inst.p, rc, summary.p, pot, fsp, ro, rc.grid, s, true);
         if (writers)
         {
             if (side) side->again();
@@ -1169,7 +1221,7 @@ void LayerPlan::write_step(const uint8_t *in, const RegionConfig &rc, bool fused
         part.g = rc.g, part.lds = rc.lds, part.items = rc.items + rc.cut_item[size_t(c)];
         part.g.items = rc.cut_item[size_t(c) + 1] - rc.cut_item[size_t(c)];
         dispatch_region(in, frag.p, inst.p, part, summary.p, nullptr, fsp, ro,
-                        int(std::min<int64_t>(rc.grid, part.g.items)), s);
+                        int(std::min<int64_t>(rc.grid, part.g.items)), s, false);
         side->again();
         launch_writers(fused, rc, out, pooled, side->stream, rc.cut_inst[size_t(c)], rc.cut_inst[size_t(c) + 1]);
     }
@@ -1292,6 +1344,8 @@ LayerPlan *make_plan(int64_t n, const int32_t *shapes, int64_t cin, int64_t cout
         const int64_t ho = h + pt + pb - kh + 1, wo = w + pl + pr - kw + 1;
         SPK_ARG(ho >= 1 && wo >= 1, "Input and kernel shapes are not compatible.");
         SPK_UNSUP(ho * wo <= 65536, "Fused layer: at most 65536 output pixels per instance.");
+        // the buffer staging's 32-bit offsets run up to one step past the input
+        SPK_UNSUP((t + 1) * cin * h * w < (int64_t(1) << 31), "Fused layer: an instance's input must be under 2 GiB.");
         int64_t pho = 0, pwo = 0;
         if (P->pool)
         {
