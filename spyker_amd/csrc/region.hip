@@ -397,14 +397,19 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
             {
                 const int32_t tc = min(g.tc, I.t - t0);
                 // the FP16 windows of steps [t0, t0 + tc): stage rows
-                // grp0 .. grp0 + wrows, columns xoff .. xoff + wcols; units
-                // outside the input are 0
+                // grp0 .. grp0 + wrows, columns xoff .. xoff + wcols (the
+                // stage holds 0 wherever the input does not reach)
                 if (wslot < wslots)
                 {
-                    const int32_t ix = ox0 - g.pl + wcc;
-                    const bool colv = ix >= 0 && ix < I.w;
-                    const uint8_t *sp0 = stage + ((grp0 * g.sw) + xoff + wcc) * g.cin_pad + wc8 * 8;
-                    int32_t ts = wslot / g.wrows, r = wslot % g.wrows;
+                    // the slot walks the (step ts, row r) pairs by (wdq, wdr):
+                    // its stage byte offset and window unit follow by additions
+                    const int32_t rstride = g.sw * g.cin_pad, wstep = PAIR ? 2 * g.win_units : g.win_units;
+                    int32_t r = wslot % g.wrows;
+                    int32_t sof = ((grp0 + t0 * g.sh) * g.sw + xoff + wcc) * g.cin_pad + wc8 * 8 +
+                                  ((wslot / g.wrows) * g.sh + r) * rstride;
+                    int32_t wat = (wslot / g.wrows) * wstep + r * g.pitch + wcu;
+                    const int32_t d_sof = (wdq * g.sh + wdr) * rstride, wrap_sof = (g.sh - g.wrows) * rstride;
+                    const int32_t d_wat = wdq * wstep + wdr * g.pitch, wrap_wat = wstep - g.wrows * g.pitch;
                     const int32_t prs = tc * g.wrows;
                     // batches of WU (step, row) pairs a slot, their stage reads together
                     for (int32_t pr0 = wslot; pr0 < prs; pr0 += WU * wslots)
@@ -414,19 +419,17 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
 #pragma unroll
                         for (int u = 0; u < WU; ++u)
                         {
-                            const int32_t iy = sy0 + grp0 + r;
-                            const uint8_t *sp = sp0 + ((t0 + ts) * g.sh + r) * g.sw * g.cin_pad;
                             raw[u] = make_uint2(0, 0);
-                            if (pr0 + u * wslots < prs && colv && iy >= 0 && iy < I.h)
+                            if (pr0 + u * wslots < prs)
                             {
                                 if (PAIR)
-                                    raw[u].x = *reinterpret_cast<const uint32_t *>(sp);
+                                    raw[u].x = *reinterpret_cast<const uint32_t *>(stage + sof);
                                 else
-                                    raw[u] = *reinterpret_cast<const uint2 *>(sp);
+                                    raw[u] = *reinterpret_cast<const uint2 *>(stage + sof);
                             }
-                            at[u] = PAIR ? ts * 2 * g.win_units + r * g.pitch + wcu : ts * g.win_units + r * g.pitch + wcu;
-                            ts += wdq, r += wdr;
-                            if (r >= g.wrows) r -= g.wrows, ++ts;
+                            at[u] = wat;
+                            r += wdr, sof += d_sof, wat += d_wat;
+                            if (r >= g.wrows) r -= g.wrows, sof += wrap_sof, wat += wrap_wat;
                         }
 #pragma unroll
                         for (int u = 0; u < WU; ++u)
