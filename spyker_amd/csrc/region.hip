@@ -218,6 +218,18 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
     const int64_t nb = gridDim.x;
     const int64_t first = nb % 8 == 0 ? (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : blockIdx.x;
 
+    // the wave's maximum of its instance's pixels so far (the instance's
+    // maximum for convwta, -0 and below counting as +0), added to the
+    // instance's entry with one atomicMax when the block moves to another
+    // instance and at the end: an atomic every group made each wave wait for
+    // it at the next barrier
+    float imx = 0.0f;
+    int32_t imx_inst = -1;
+    auto flush_imax = [&]() {
+        if (lane == 0 && imx_inst >= 0 && imx > 0.0f)
+            atomicMax(ro.imax + imx_inst, __builtin_bit_cast(uint32_t, imx));
+    };
+
     for (int64_t it = first; it < g.items; it += nb)
     {
         const int2 itm = items[it];
@@ -681,9 +693,9 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
                 sel_c[ct * COLS + j] = cs, sel_m[ct * COLS + j] = pvalid ? fwd : 0u;
             }
             {
-                // the instance's maximum for convwta (-0 and below count as +0)
                 const float mv = wave_max(own && pvalid && pmax > 0.0f ? pmax : 0.0f);
-                if (lane == 0 && mv > 0.0f) atomicMax(ro.imax + itm.x, __builtin_bit_cast(uint32_t, mv));
+                if (itm.x != imx_inst) flush_imax(), imx_inst = itm.x, imx = 0.0f;
+                imx = imx < mv ? mv : imx;
             }
             if (zs + zp > 0)
             {
@@ -704,6 +716,7 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
             }
         }
     }
+    flush_imax();
 }
 
 // convwta over one instance's summaries (fused_select's rule, wta.cpp:49-64,
