@@ -221,8 +221,7 @@ __global__ void __launch_bounds__(64 * REGION_WAVES)
     // the wave's maximum of its instance's pixels so far (the instance's
     // maximum for convwta, -0 and below counting as +0), added to the
     // instance's entry with one atomicMax when the block moves to another
-    // instance and at the end: an atomic every group made each wave wait for
-    // it at the next barrier
+    // instance and at the end, rather than one atomic a group
     float imx = 0.0f;
     int32_t imx_inst = -1;
     auto flush_imax = [&]() {
