@@ -53,7 +53,7 @@ PEAK_HBM = 8.0e12
 TRAFFIC_FUSED_LAYER = {"fp16": 979.2e6, "int8": 985.7e6}
 # The region kernel per launch on the precision workload, and the whole batching-workload step
 # (region kernel, writers, convwta), from the same counters: profiles/fused_region_traffic.txt.
-TRAFFIC_REGION_C3 = 26.86e9
+TRAFFIC_REGION_C3 = 29.16e9
 TRAFFIC_STEP_C4 = 79.7e9
 WARM_S = 0.3  # seconds of untimed fused steps before the timed ones (at least --warmup steps)
 METRIC = "images\u00b7timesteps/s, conv+fire+wta+STDP layer: FP16-batched vs FP32 plain, 1 GPU"
