@@ -14,11 +14,15 @@
 // round-to-nearest turns exact cancellation into +0), so the result equals
 // the tap-skipping form of the CPU oracle.
 //
-// Layout: a block owns a TH x TW tile of output pixels of one image and 32
-// output channels.  The input window of the tile (with halo) is staged in LDS
-// one channel chunk at a time; the weights of the 32 channels are read
-// through the scalar cache from a [cin][kh][kw][cout32] transposed copy, so
-// each tap costs one LDS read and 32 FMAs with SGPR operands.
+// Layout: a block owns a TH x TW tile of output pixels of one image and CT
+// output channels, CT = 8 for layers of at most 8 output channels (filter
+// banks: DoG, LoG, Gabor) and 32 otherwise.  The input window of the tile
+// (with halo) is staged in LDS one channel chunk at a time; the weights of
+// the CT channels are read through the scalar cache from a
+// [cin][kh][kw][cout_pad] transposed copy, so each tap costs one LDS read and
+// CT FMAs with SGPR operands.  When one block holds every filter of a DoG or
+// LoG bank, the on/off combination runs in its epilogue (no intermediate
+// planes).
 #include "common.hpp"
 
 namespace spk
@@ -26,15 +30,16 @@ namespace spk
 namespace
 {
 constexpr int COT = 32;      // output channels per block
+constexpr int COT_NARROW = 8; // ... for layers of at most 8 output channels
 constexpr int THREADS = 256; // one output pixel per thread
-constexpr int LDS_FLOATS = 8192;
+constexpr int LDS_FLOATS = 8192;  // window elements a block stages at most
 
 struct ConvGeom
 {
     int32_t cin, h, w, cout, kh, kw, sy, sx, pt, pl, ho, wo;
     int32_t tw, th, rh, rw, cc;  // tile, window, channels per chunk
     int32_t tiles_x, tiles;
-    int32_t cout_pad;
+    int32_t cout_pad, cot;  // channels padded to a multiple of cot, the channels of a block
 };
 
 ConvGeom geometry(int64_t cin, int64_t h, int64_t w, int64_t cout, int64_t kh, int64_t kw, int64_t sy, int64_t sx,
@@ -54,7 +59,8 @@ ConvGeom geometry(int64_t cin, int64_t h, int64_t w, int64_t cout, int64_t kh, i
     if (g.cc > g.cin) g.cc = g.cin;
     g.tiles_x = int32_t(cdiv(g.wo, g.tw));
     g.tiles = g.tiles_x * int32_t(cdiv(g.ho, g.th));
-    g.cout_pad = int32_t(cdiv(g.cout, COT) * COT);
+    g.cot = g.cout <= COT_NARROW ? COT_NARROW : COT;
+    g.cout_pad = int32_t(cdiv(g.cout, g.cot) * g.cot);
     return g;
 }
 
@@ -83,15 +89,41 @@ __device__ __forceinline__ W load_in(const I *p, int64_t i)
 template <typename K>
 using WinT = typename std::conditional<std::is_same<K, double>::value, double, float>::type;
 
+// The DoG (EPI_DOG) and LoG (EPI_LOG) on/off combination of one output pixel
+// from the accumulators of a block holding all 2 * CNT filters: filter j
+// against filter CNT + j, d = a - b, on = (d < 0 ? 0 : d) and for LoG off =
+// (-d < 0 ? 0 : -d) (cpu/dog.cpp:62-96), out laid out [CNT (on) | CNT (off)]
+// planes per image as the two-pass form writes them.
+constexpr int EPI_NONE = 0, EPI_DOG = 1, EPI_LOG = 2;
+
+template <int CNT, int EPI, int CT>
+__device__ __forceinline__ void combine_pixel(const float (&acc)[CT], float *out, int64_t n, int64_t hw, int64_t pix)
+{
+    static_assert(2 * CNT <= CT, "the block must hold both filters of every pair");
+    float *dst = out + n * (EPI == EPI_LOG ? 2 * CNT : CNT) * hw + pix;
+#pragma unroll
+    for (int j = 0; j < CNT; ++j)
+    {
+        const float d = acc[j] - acc[CNT + j], e = -d;
+        dst[j * hw] = d < 0.0f ? 0.0f : d;
+        if (EPI == EPI_LOG) dst[(CNT + j) * hw] = e < 0.0f ? 0.0f : e;
+    }
+}
+
 // K: kernel type; Acc: float (F16 and F32 kernels) or double (F64 kernels
-// and the FP64 check path of F32 kernels); O: output type.
-template <typename I, typename K, typename Acc, typename O>
+// and the FP64 check path of F32 kernels); O: output type; CT: output
+// channels a block; EPI: plain output or a DoG/LoG combination (CT = 8, one
+// block of channels, cout = 2 * count).
+template <typename I, typename K, typename Acc, typename O, int CT, int EPI>
 __global__ void __launch_bounds__(THREADS) conv_direct(const I *__restrict__ in, const WinT<K> *__restrict__ kt,
                                                        O *__restrict__ out, ConvGeom g)
 {
     using W = WinT<K>;
-    __shared__ W win[LDS_FLOATS];
-    const int32_t tile = blockIdx.x, o0 = blockIdx.y * COT;
+    // the window of cc channels, sized at launch (a one-channel filter bank
+    // stages a few KB, so more blocks fit a CU than a fixed array allows)
+    extern __shared__ __align__(16) unsigned char conv_lds[];
+    W *const win = reinterpret_cast<W *>(conv_lds);
+    const int32_t tile = blockIdx.x, o0 = blockIdx.y * CT;
     const int64_t n = blockIdx.z;
     const int32_t ty0 = (tile / g.tiles_x) * g.th, tx0 = (tile % g.tiles_x) * g.tw;
     const int32_t ly = threadIdx.x / g.tw, lx = threadIdx.x % g.tw;
@@ -100,9 +132,9 @@ __global__ void __launch_bounds__(THREADS) conv_direct(const I *__restrict__ in,
     const int32_t iy0 = ty0 * g.sy - g.pt, ix0 = tx0 * g.sx - g.pl;
     const int32_t taps = g.kh * g.kw;
 
-    Acc acc[COT];
+    Acc acc[CT];
 #pragma unroll
-    for (int j = 0; j < COT; ++j) acc[j] = Acc(0);
+    for (int j = 0; j < CT; ++j) acc[j] = Acc(0);
 
     const I *src = in + n * g.cin * int64_t(g.h) * g.w;
     for (int32_t c0 = 0; c0 < g.cin; c0 += g.cc)
@@ -125,28 +157,41 @@ __global__ void __launch_bounds__(THREADS) conv_direct(const I *__restrict__ in,
             {
                 const W *wrow = win + c * area + (ly * g.sy) * g.rw + lx * g.sx;
                 const W *kc = kt + (int64_t(c0 + c) * taps) * g.cout_pad + o0;
-                for (int32_t a = 0; a < g.kh; ++a)
-                    for (int32_t b = 0; b < g.kw; ++b)
-                    {
-                        const W x = wrow[a * g.rw + b];
-                        const W *kk = kc + (a * g.kw + b) * g.cout_pad;
+                auto tap = [&](W x, const W *kk) {
 #pragma unroll
-                        for (int j = 0; j < COT; ++j)
-                        {
-                            if constexpr (sizeof(Acc) == 4)
-                                acc[j] = __builtin_fmaf(x, kk[j], acc[j]);
-                            else
-                                acc[j] = __builtin_fma(double(x), double(kk[j]), acc[j]);
-                        }
+                    for (int j = 0; j < CT; ++j)
+                    {
+                        if constexpr (sizeof(Acc) == 4)
+                            acc[j] = __builtin_fmaf(x, kk[j], acc[j]);
+                        else
+                            acc[j] = __builtin_fma(double(x), double(kk[j]), acc[j]);
                     }
+                };
+                for (int32_t a = 0; a < g.kh; ++a)
+                    for (int32_t b = 0; b < g.kw; ++b) tap(wrow[a * g.rw + b], kc + (a * g.kw + b) * g.cout_pad);
             }
         }
     }
     if (!active) return;
-    O *dst = out + (n * g.cout + o0) * int64_t(g.ho) * g.wo + int64_t(oy) * g.wo + ox;
+    if constexpr (EPI == EPI_NONE)
+    {
+        O *dst = out + (n * g.cout + o0) * int64_t(g.ho) * g.wo + int64_t(oy) * g.wo + ox;
 #pragma unroll
-    for (int j = 0; j < COT; ++j)
-        if (o0 + j < g.cout) dst[int64_t(j) * g.ho * g.wo] = O(acc[j]);
+        for (int j = 0; j < CT; ++j)
+            if (o0 + j < g.cout) dst[int64_t(j) * g.ho * g.wo] = O(acc[j]);
+    }
+    else
+    {
+        static_assert(std::is_same<Acc, float>::value && std::is_same<O, float>::value, "F32 filter banks only");
+        const int64_t hw = int64_t(g.ho) * g.wo, pix = int64_t(oy) * g.wo + ox;
+        switch (g.cout / 2)
+        {
+        case 1: combine_pixel<1, EPI, CT>(acc, out, n, hw, pix); break;
+        case 2: combine_pixel<2, EPI, CT>(acc, out, n, hw, pix); break;
+        case 3: combine_pixel<3, EPI, CT>(acc, out, n, hw, pix); break;
+        default: combine_pixel<4, EPI, CT>(acc, out, n, hw, pix); break;
+        }
+    }
 }
 
 // DoG combine: mid is [planes][2][count*ho*wo]; out = max(g1 - g2, 0) as
@@ -201,7 +246,7 @@ __global__ void transpose_fc(const float *__restrict__ k, float *__restrict__ kt
     }
 }
 
-template <typename I, typename K, typename Acc, typename O>
+template <typename I, typename K, typename Acc, typename O, int EPI = EPI_NONE>
 void conv_run(const void *in, const void *kernel, void *out, int64_t n, const ConvGeom &g, hipStream_t s)
 {
     using W = WinT<K>;
@@ -215,10 +260,16 @@ void conv_run(const void *in, const void *kernel, void *out, int64_t n, const Co
     for (int64_t n0 = 0; n0 < n; n0 += 65535)
     {
         const int64_t nb = n - n0 < 65535 ? n - n0 : 65535;
-        dim3 grid(unsigned(g.tiles), unsigned(g.cout_pad / COT), unsigned(nb));
+        dim3 grid(unsigned(g.tiles), unsigned(g.cout_pad / g.cot), unsigned(nb));
         const I *src = (const I *)in + n0 * g.cin * int64_t(g.h) * g.w;
-        O *dst = (O *)out + n0 * g.cout * int64_t(g.ho) * g.wo;
-        conv_direct<I, K, Acc, O><<<grid, THREADS, 0, s>>>(src, kt, dst, g);
+        O *dst = (O *)out + n0 * (EPI == EPI_DOG ? g.cout / 2 : g.cout) * int64_t(g.ho) * g.wo;
+        const size_t lds = size_t(g.cc) * g.rh * g.rw * sizeof(W);
+        if constexpr (EPI != EPI_NONE)
+            conv_direct<I, K, Acc, O, COT_NARROW, EPI><<<grid, THREADS, lds, s>>>(src, kt, dst, g);
+        else if (g.cot == COT_NARROW)
+            conv_direct<I, K, Acc, O, COT_NARROW, EPI_NONE><<<grid, THREADS, lds, s>>>(src, kt, dst, g);
+        else
+            conv_direct<I, K, Acc, O, COT, EPI_NONE><<<grid, THREADS, lds, s>>>(src, kt, dst, g);
         SPK_LAUNCHED();
     }
 }
@@ -319,6 +370,8 @@ int32_t spk_dog(const void *in, const void *kernel, void *out, int64_t planes, i
         if (planes == 0) return;
         ConvGeom g = geometry(1, h, w, 2 * count, k, k, 1, 1, pt, pl, pb, pr);
         hipStream_t s = S(stream);
+        // one block holds every filter: the combination in the conv's epilogue
+        if (g.cot == COT_NARROW) return conv_run<float, float, float, float, EPI_DOG>(in, kernel, out, planes, g, s);
         const int64_t len = count * int64_t(g.ho) * g.wo;
         Scratch mid(planes * 2 * len * int64_t(sizeof(float)), s);
         conv_run<float, float, float, float>(in, kernel, mid.ptr, planes, g, s);
@@ -335,6 +388,7 @@ int32_t spk_log(const void *in, const void *kernel, void *out, int64_t planes, i
         if (planes == 0) return;
         ConvGeom g = geometry(1, h, w, 2 * count, k, k, 1, 1, pt, pl, pb, pr);
         hipStream_t s = S(stream);
+        if (g.cot == COT_NARROW) return conv_run<float, float, float, float, EPI_LOG>(in, kernel, out, planes, g, s);
         const int64_t len = count * int64_t(g.ho) * g.wo;
         Scratch mid(planes * 2 * len * int64_t(sizeof(float)), s);
         conv_run<float, float, float, float>(in, kernel, mid.ptr, planes, g, s);

@@ -52,11 +52,12 @@ def spikes(rng, shape, p=0.3):
     return np.cumsum(rng.random(shape) < p, axis=1).clip(0, 1).astype(np.uint8)
 
 
+@pytest.mark.parametrize("cout", [37, 6])
 @pytest.mark.parametrize("dt", [np.float16, np.float64])
 @pytest.mark.parametrize("src", [np.uint8, np.float32, "own"])
-def test_conv_kernel_types(dev, orc, dt, src):
+def test_conv_kernel_types(dev, orc, dt, src, cout):
     rng = np.random.default_rng(71)
-    n, t, cin, h, w, cout = 2, 3, 5, 11, 13, 37
+    n, t, cin, h, w = 2, 3, 5, 11, 13
     if src == np.uint8:
         x = spikes(rng, (n, t, cin, h, w))
     else:

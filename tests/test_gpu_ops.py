@@ -153,6 +153,10 @@ def test_dog_gabor(dev, orc):
     assert np.array_equal(H(g(T(img, dev))), orc.gabor(img, H(g.kernel), (3, 3, 3, 3)))
     u8 = (rng.random((2, 1, 28, 28)) * 255).astype(np.uint8)
     assert np.array_equal(H(d(T(u8, dev))), orc.dog(u8.astype(np.float32), H(d.kernel), (3, 3, 3, 3)))
+    # up to 4 filters a block combines them in its epilogue; 5 take the two-pass form
+    for nf in (1, 4, 5):
+        dn = spk.DoG(2, [spk.DoGFilter(1 + 0.25 * i, 2 + 0.5 * i) for i in range(nf)], pad=(2, 1, 0, 2), device=dev)
+        assert np.array_equal(H(dn(T(img, dev))), orc.dog(img, H(dn.kernel), (2, 1, 0, 2)))
 
 
 def test_log(dev, orc):
@@ -164,6 +168,9 @@ def test_log(dev, orc):
     assert np.array_equal(got, orc.log(img, H(lg.kernel), (3, 2, 1, 3)))
     u8 = (rng.random((1, 28, 28)) * 255).astype(np.uint8)  # 3-D input is one sample (to4)
     assert np.array_equal(H(lg(T(u8, dev))), orc.log(u8.astype(np.float32)[None], H(lg.kernel), (3, 2, 1, 3)))
+    for stds in ([1.5], [1.0, 1.5, 2.0, 2.5], [1.0, 1.25, 1.5, 1.75, 2.0]):
+        ln = spk.LoG(2, stds, pad=2, device=dev)
+        assert np.array_equal(H(ln(T(img, dev))), orc.log(img, H(ln.kernel), (2, 2, 2, 2)))
 
 
 def test_rank_code(dev, golden, orc):
