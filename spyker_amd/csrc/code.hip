@@ -7,7 +7,8 @@
 // so ties are ordered by ascending index.  The reference's std::sort leaves
 // the order inside a tie block unspecified; the two agree whenever no tie
 // block straddles a spike-time boundary.
-//   1. chunks of CHUNK keys are sorted in LDS (bitonic network);
+//   1. chunks of CHUNK keys are sorted by a bitonic network (in registers,
+//      wave shuffles and, for the largest strides, LDS);
 //   2. sorted runs are merged pairwise until one run covers the sample; each
 //      key finds its place by a binary search in the partner run;
 //   3. each sorted position writes its element's time; 4. spikes are
@@ -21,7 +22,8 @@ namespace spk
 namespace
 {
 constexpr int CHUNK = 2048;
-constexpr int SORT_THREADS = 1024;
+constexpr int SORT_THREADS = 256;
+constexpr int SORT_KPT = CHUNK / SORT_THREADS;  // keys a thread holds
 
 __device__ __forceinline__ uint32_t desc_bits(float v)
 {
@@ -39,33 +41,90 @@ __device__ __forceinline__ uint16_t to_u16(float f)
     return uint16_t(uint32_t(int32_t(f)));
 }
 
+// 64-bit lane exchange within a wave (two 32-bit shuffles).
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int mask)
+{
+    const uint32_t lo = uint32_t(__shfl_xor(int(uint32_t(v)), mask, 64));
+    const uint32_t hi = uint32_t(__shfl_xor(int(uint32_t(v >> 32)), mask, 64));
+    return (uint64_t(hi) << 32) | lo;
+}
+
+// One bitonic pass (size, STRIDE) over the thread's own SORT_KPT keys, at
+// positions p0 + r: pairs (r, r + STRIDE) in the same thread.
+template <int STRIDE>
+__device__ __forceinline__ void pass_in_thread(uint64_t (&v)[SORT_KPT], int32_t p0, int32_t size)
+{
+#pragma unroll
+    for (int r = 0; r < SORT_KPT; ++r)
+        if ((r & STRIDE) == 0)
+        {
+            const bool up = ((p0 + r) & size) == 0;
+            const uint64_t a = v[r], b = v[r + STRIDE];
+            if ((a > b) == up) v[r] = b, v[r + STRIDE] = a;
+        }
+}
+
+// The bitonic network of the original LDS form (pairs (i, i ^ stride),
+// ascending where i & size is 0), with each thread holding the SORT_KPT
+// consecutive positions p0 .. p0 + SORT_KPT - 1: strides below SORT_KPT stay in
+// the thread, strides up to 64 * SORT_KPT / 2 exchange with a lane of the same
+// wave, and only the larger ones (512 and 1024) pass through LDS, so a chunk
+// takes 3 barrier pairs instead of one barrier per pass.  Keys are unique, so
+// the sorted chunk does not depend on the network.
 __global__ void __launch_bounds__(SORT_THREADS) chunk_sort(const float *__restrict__ in, uint64_t *__restrict__ keys,
                                                            int64_t len)
 {
     __shared__ uint64_t k[CHUNK];
     const int64_t s = blockIdx.y, c0 = int64_t(blockIdx.x) * CHUNK;
     const float *src = in + s * len;
-    for (int i = threadIdx.x; i < CHUNK; i += SORT_THREADS)
+    const int32_t p0 = int32_t(threadIdx.x) * SORT_KPT;
+    uint64_t v[SORT_KPT];
+#pragma unroll
+    for (int r = 0; r < SORT_KPT; ++r)
     {
-        const int64_t j = c0 + i;
-        k[i] = j < len ? (uint64_t(desc_bits(src[j])) << 32) | uint64_t(j) : ~0ull;
+        const int64_t j = c0 + p0 + r;
+        v[r] = j < len ? (uint64_t(desc_bits(src[j])) << 32) | uint64_t(j) : ~0ull;
     }
-    __syncthreads();
-    for (int size = 2; size <= CHUNK; size <<= 1)
-        for (int stride = size >> 1; stride > 0; stride >>= 1)
+    for (int32_t size = 2; size <= CHUNK; size <<= 1)
+        for (int32_t stride = size >> 1; stride > 0; stride >>= 1)
         {
-            for (int i = threadIdx.x; i < CHUNK / 2; i += SORT_THREADS)
+            if (stride < SORT_KPT)
             {
-                const int lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
-                const bool up = (lo & size) == 0;
-                const uint64_t a = k[lo], b = k[hi];
-                if ((a > b) == up) k[lo] = b, k[hi] = a;
+                if (stride == 4) pass_in_thread<4>(v, p0, size);
+                else if (stride == 2) pass_in_thread<2>(v, p0, size);
+                else pass_in_thread<1>(v, p0, size);
+                continue;
             }
-            __syncthreads();
+            uint64_t w[SORT_KPT];
+            if (stride < 64 * SORT_KPT)
+            {
+#pragma unroll
+                for (int r = 0; r < SORT_KPT; ++r) w[r] = shfl_xor64(v[r], stride / SORT_KPT);
+            }
+            else
+            {
+                __syncthreads();  // every read of the previous LDS pass is done
+#pragma unroll
+                for (int r = 0; r < SORT_KPT; ++r) k[p0 + r] = v[r];
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < SORT_KPT; ++r) w[r] = k[(p0 + r) ^ stride];
+            }
+#pragma unroll
+            for (int r = 0; r < SORT_KPT; ++r)
+            {
+                // the lower position of an ascending pair (or the upper of a
+                // descending one) keeps the smaller key
+                const int32_t p = p0 + r;
+                const bool keep_min = ((p & stride) == 0) == ((p & size) == 0);
+                const uint64_t a = v[r], b = w[r];
+                v[r] = keep_min ? (a < b ? a : b) : (a < b ? b : a);
+            }
         }
     uint64_t *dst = keys + s * len;
-    for (int i = threadIdx.x; i < CHUNK; i += SORT_THREADS)
-        if (c0 + i < len) dst[c0 + i] = k[i];
+#pragma unroll
+    for (int r = 0; r < SORT_KPT; ++r)
+        if (c0 + p0 + r < len) dst[c0 + p0 + r] = v[r];
 }
 
 // Merge runs of length run into runs of length 2*run.
