@@ -180,6 +180,67 @@ __global__ void rank_times(const uint64_t *__restrict__ keys, uint16_t *__restri
         ts[uint32_t(k[i])] = to_u16(float(int32_t(i)) * scale);
 }
 
+// merge_runs and rank_times for a sample of more than CHUNK and at most
+// 2 * CHUNK keys (one merge pass, run = CHUNK): one block per sample holds its
+// keys in LDS, each key finds its merged position by a binary search of the
+// partner run there, the merged run replaces the two in LDS, and the spike
+// times are written from it as rank_times writes them.
+constexpr int MERGE_THREADS = 1024;
+constexpr int MERGE_KPT = 2 * CHUNK / MERGE_THREADS;  // keys a thread places
+
+__global__ void __launch_bounds__(MERGE_THREADS)
+    merge_rank_lds(const uint64_t *__restrict__ src, uint16_t *__restrict__ times, int32_t len, int32_t t)
+{
+    __shared__ uint64_t k[2 * CHUNK];
+    const int64_t s = blockIdx.x;
+    const uint64_t *a = src + s * len;
+    for (int32_t i = threadIdx.x; i < len; i += MERGE_THREADS) k[i] = a[i];
+    __syncthreads();
+    uint64_t key[MERGE_KPT];
+    int32_t at[MERGE_KPT];
+#pragma unroll
+    for (int u = 0; u < MERGE_KPT; ++u)
+    {
+        const int32_t i = int32_t(threadIdx.x) + u * MERGE_THREADS;
+        at[u] = -1;
+        if (i >= len) continue;
+        const int32_t r = i / CHUNK, p = i - r * CHUNK, q = r ^ 1;
+        const int32_t qs = q * CHUNK, qe = min(qs + CHUNK, len);
+        key[u] = k[i];
+        // number of keys in the partner run that are smaller than key
+        int32_t lo = qs, hi = qe;
+        while (lo < hi)
+        {
+            const int32_t mid = (lo + hi) >> 1;
+            if (k[mid] < key[u])
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        at[u] = min(r, q) * CHUNK + p + (lo - qs);
+    }
+    __syncthreads();  // every search is done before the merged run overwrites the two
+#pragma unroll
+    for (int u = 0; u < MERGE_KPT; ++u)
+        if (at[u] >= 0) k[at[u]] = key[u];
+    __syncthreads();
+    // elements strictly above the minimum = first sorted position whose value
+    // equals the last (smallest) one
+    const uint32_t last = uint32_t(k[len - 1] >> 32);
+    int32_t lo = 0, hi = len;
+    while (lo < hi)
+    {
+        const int32_t mid = (lo + hi) >> 1;
+        if (uint32_t(k[mid] >> 32) < last)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    const float scale = (float(t) + 1e-5f) / float(lo);
+    uint16_t *ts = times + s * len;
+    for (int32_t i = threadIdx.x; i < len; i += MERGE_THREADS) ts[uint32_t(k[i])] = to_u16(float(i) * scale);
+}
+
 __global__ void latency_times(const float *__restrict__ in, uint16_t *__restrict__ times, int64_t len, int32_t t)
 {
     __shared__ float mm[32];
@@ -223,6 +284,13 @@ void rank_times_launch(const float *in, int64_t n, int64_t len, int64_t time, bo
         SPK_LAUNCHED();
         uint64_t *src = k0.as<uint64_t>(), *dst = k1.as<uint64_t>();
         const unsigned blocks = unsigned(cdiv(len, 256) < 1024 ? cdiv(len, 256) : 1024);
+        if (len > CHUNK && len <= 2 * CHUNK)
+        {
+            // one merge pass and the times, the whole sample in LDS
+            merge_rank_lds<<<dim3(unsigned(n)), MERGE_THREADS, 0, s>>>(src, times, int32_t(len), int32_t(time));
+            SPK_LAUNCHED();
+            return;
+        }
         for (int64_t run = CHUNK; run < len; run <<= 1)
         {
             merge_runs<<<dim3(blocks, unsigned(n)), 256, 0, s>>>(src, dst, len, run);
