@@ -278,19 +278,22 @@ void rank_times_launch(const float *in, int64_t n, int64_t len, int64_t time, bo
 {
     if (sort)
     {
-        Scratch k0(n * len * int64_t(sizeof(uint64_t)), s), k1(n * len * int64_t(sizeof(uint64_t)), s);
+        Scratch k0(n * len * int64_t(sizeof(uint64_t)), s);
         const int64_t chunks = cdiv(len, CHUNK);
         chunk_sort<<<dim3(unsigned(chunks), unsigned(n)), SORT_THREADS, 0, s>>>(in, k0.as<uint64_t>(), len);
         SPK_LAUNCHED();
-        uint64_t *src = k0.as<uint64_t>(), *dst = k1.as<uint64_t>();
-        const unsigned blocks = unsigned(cdiv(len, 256) < 1024 ? cdiv(len, 256) : 1024);
         if (len > CHUNK && len <= 2 * CHUNK)
         {
             // one merge pass and the times, the whole sample in LDS
-            merge_rank_lds<<<dim3(unsigned(n)), MERGE_THREADS, 0, s>>>(src, times, int32_t(len), int32_t(time));
+            merge_rank_lds<<<dim3(unsigned(n)), MERGE_THREADS, 0, s>>>(k0.as<uint64_t>(), times, int32_t(len),
+                                                                        int32_t(time));
             SPK_LAUNCHED();
             return;
         }
+        // the second key buffer only for the global merge passes
+        Scratch k1(len > CHUNK ? n * len * int64_t(sizeof(uint64_t)) : 0, s);
+        uint64_t *src = k0.as<uint64_t>(), *dst = k1.as<uint64_t>();
+        const unsigned blocks = unsigned(cdiv(len, 256) < 1024 ? cdiv(len, 256) : 1024);
         for (int64_t run = CHUNK; run < len; run <<= 1)
         {
             merge_runs<<<dim3(blocks, unsigned(n)), 256, 0, s>>>(src, dst, len, run);
